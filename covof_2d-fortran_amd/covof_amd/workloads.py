@@ -1,0 +1,255 @@
+"""Seeded synthetic inputs for the hot-path workloads (numpy, host side).
+
+Fields use the reference's halo layout (``src/module_fields.f90:129-169``):
+a numpy array of shape ``(ny+2, nx+2)`` indexed ``F[j, i]`` for Fortran
+``F(i, j)``, ``i, j = 0 .. n+1``.  Face velocities follow the MAC placement of
+``src/module_fields.f90:33-35``: ``u(i, j)`` at ``(x_face(i), y_center(j))``,
+``v(i, j)`` at ``(x_center(i), y_face(j))``.
+
+Workloads
+---------
+* :func:`poisson_case` -- variable-density pressure Poisson problem: seeded
+  random circles (sharp 0/1 indicator, one 3x3 box-smoothing pass), density
+  ``rho = 1000 C + (1 - C)``, zero-mean Gaussian right-hand side, walls.
+* :func:`vof_case` -- seeded random ellipses (area fractions by chord
+  integration over sub-columns, overlaps combined by a clamped sum) in a
+  periodic unit square, advected by an exactly (discretely) divergence-free
+  velocity built from a nodal stream function of seeded Fourier modes.
+* :func:`vortex_velocity` -- the reference's reversible shear-vortex velocity
+  (``src/module_simulation.f90:2254-2288``).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
+
+
+def x_face(i, dx):
+    """``x_face(i) = real(i-1) * dx`` (``src/module_mesh.f90:337-341``)."""
+    return (np.asarray(i, dtype=np.float64) - 1.0) * dx
+
+
+def x_center(i, dx):
+    """``x_center(i) = (real(i-1) + 0.5) * dx`` (``src/module_mesh.f90:357-361``)."""
+    return ((np.asarray(i, dtype=np.float64) - 1.0) + 0.5) * dx
+
+
+# ----------------------------------------------------------------------------- VOF
+
+def _ellipse_fraction(C, nx, ny, dx, dy, xc, yc, a, b, th, sub=32):
+    """Add the area fraction of one rotated ellipse to the interior of ``C``."""
+    ct_, st_ = math.cos(th), math.sin(th)
+    ext_x = math.sqrt((a * ct_) ** 2 + (b * st_) ** 2)
+    ext_y = math.sqrt((a * st_) ** 2 + (b * ct_) ** 2)
+    i0 = max(1, int(math.floor((xc - ext_x) / dx)) + 1 - 1)
+    i1 = min(nx, int(math.floor((xc + ext_x) / dx)) + 1 + 1)
+    j0 = max(1, int(math.floor((yc - ext_y) / dy)) + 1 - 1)
+    j1 = min(ny, int(math.floor((yc + ext_y) / dy)) + 1 + 1)
+    if i0 > i1 or j0 > j1:
+        return
+    ii = np.arange(i0, i1 + 1)
+    jj = np.arange(j0, j1 + 1)
+    # quadratic in y' for every sub-column x': A y'^2 + B y' + Cq <= 0
+    A = (st_ / a) ** 2 + (ct_ / b) ** 2
+    xs = (ii[:, None] - 1.0 + (np.arange(sub)[None, :] + 0.5) / sub) * dx - xc  # (ni, sub)
+    B = 2.0 * xs * (ct_ * st_ / a ** 2 - st_ * ct_ / b ** 2)
+    Cq = xs ** 2 * ((ct_ / a) ** 2 + (st_ / b) ** 2) - 1.0
+    disc = B * B - 4.0 * A * Cq
+    ok = disc > 0.0
+    sq = np.sqrt(np.where(ok, disc, 0.0))
+    ylo = np.where(ok, (-B - sq) / (2.0 * A), 0.0) + yc
+    yhi = np.where(ok, (-B + sq) / (2.0 * A), 0.0) + yc
+    y0 = (jj - 1.0) * dy
+    y1 = jj * dy
+    lo = np.clip(ylo[None, :, :], y0[:, None, None], y1[:, None, None])
+    hi = np.clip(yhi[None, :, :], y0[:, None, None], y1[:, None, None])
+    frac = np.where(ok[None, :, :], hi - lo, 0.0).mean(axis=2) / dy  # (nj, ni)
+    C[j0:j1 + 1, i0:i1 + 1] += frac
+
+
+def ellipses(nx, ny, n, semi_min_cells, semi_max_cells, seed, lx=1.0, ly=1.0):
+    """Clamped union of ``n`` seeded random ellipses; interior only (halo zero)."""
+    rng = np.random.default_rng(seed)
+    dx, dy = lx / nx, ly / ny
+    C = np.zeros((ny + 2, nx + 2))
+    for _ in range(n):
+        a = rng.uniform(semi_min_cells, semi_max_cells) * dx
+        b = rng.uniform(semi_min_cells, semi_max_cells) * dx
+        th = rng.uniform(0.0, math.pi)
+        xc = rng.uniform(0.0, lx)
+        yc = rng.uniform(0.0, ly)
+        _ellipse_fraction(C, nx, ny, dx, dy, xc, yc, a, b, th)
+    np.clip(C, 0.0, 1.0, out=C)
+    return C
+
+
+def fill_halo_periodic(F, nx, ny):
+    """Single-rank halo exchange with both directions periodic
+    (``src/module_mpi_comm.f90:46-56``)."""
+    F[:, 0] = F[:, nx]
+    F[:, nx + 1] = F[:, 1]
+    F[0, :] = F[ny, :]
+    F[ny + 1, :] = F[1, :]
+    return F
+
+
+def fill_halo_zero_gradient(F, nx, ny):
+    """Copy the first/last interior row and column outwards (walls)."""
+    F[1:ny + 1, 0] = F[1:ny + 1, 1]
+    F[1:ny + 1, nx + 1] = F[1:ny + 1, nx]
+    F[0, :] = F[1, :]
+    F[ny + 1, :] = F[ny, :]
+    return F
+
+
+def stream_velocity(nx, ny, nmodes, seed, lx=1.0, ly=1.0, kmax=4, periodic=True, amplitude=1.0):
+    """Exactly divergence-free MAC velocity from a nodal stream function.
+
+    ``psi`` is sampled at nodes ``(x_face(i), y_face(j))``; ``u = dpsi/dy`` and
+    ``v = -dpsi/dx`` by one-sided node differences, so the discrete divergence
+    ``(u(i+1,j)-u(i,j)) dy + (v(i,j+1)-v(i,j)) dx`` telescopes to rounding.
+    Periodic: Fourier modes with integer wave vectors ``0 < |k| <= kmax``, the
+    node row/column ``n+1`` is the wrap copy of ``1``.  Walls: ``psi`` is the
+    same sum multiplied by ``sin^2(pi x) sin^2(pi y)``, so the wall-normal
+    velocity vanishes.
+    """
+    rng = np.random.default_rng(seed)
+    dx, dy = lx / nx, ly / ny
+    ks = []
+    while len(ks) < nmodes:
+        k = rng.integers(-kmax, kmax + 1, size=2)
+        if 0 < k[0] ** 2 + k[1] ** 2 <= kmax ** 2:
+            ks.append(k)
+    amp = rng.normal(size=nmodes)
+    amp = amplitude * amp / np.abs(amp).sum()
+    phs = rng.uniform(0.0, 2.0 * math.pi, size=nmodes)
+    xn = x_face(np.arange(1, nx + 2), dx)  # nodes i = 1..nx+1
+    yn = x_face(np.arange(1, ny + 2), dy)
+    X, Y = np.meshgrid(xn, yn)  # (ny+1, nx+1)
+    psi = np.zeros_like(X)
+    for (kx, ky), a, ph in zip(ks, amp, phs):
+        psi += a * np.sin(2.0 * math.pi * (kx * X / lx + ky * Y / ly) + ph) / (2.0 * math.pi)
+    if periodic:
+        psi[:, nx] = psi[:, 0]
+        psi[ny, :] = psi[0, :]
+    else:
+        psi *= (np.sin(math.pi * X / lx) ** 2) * (np.sin(math.pi * Y / ly) ** 2)
+        psi[:, 0] = psi[:, nx] = 0.0
+        psi[0, :] = psi[ny, :] = 0.0
+    u = np.zeros((ny + 2, nx + 2))
+    v = np.zeros((ny + 2, nx + 2))
+    # u(i,j), i = 1..nx+1, j = 1..ny ; psi index [j-1, i-1]
+    u[1:ny + 1, 1:nx + 2] = (psi[1:ny + 1, 0:nx + 1] - psi[0:ny, 0:nx + 1]) / dy
+    # v(i,j), i = 1..nx, j = 1..ny+1
+    v[1:ny + 2, 1:nx + 1] = -(psi[0:ny + 1, 1:nx + 1] - psi[0:ny + 1, 0:nx]) / dx
+    if periodic:
+        u[1:ny + 1, 0] = u[1:ny + 1, nx]
+        v[0, 1:nx + 1] = v[ny, 1:nx + 1]
+    return u, v
+
+
+def vof_case(n, seed=0, n_ellipses=128, semi=(20.0, 200.0), nmodes=8, cfl=0.25):
+    """Periodic ``n x n`` VOF advection case; returns ``(C, u, v, dt)``."""
+    C = ellipses(n, n, n_ellipses, semi[0], semi[1], seed)
+    fill_halo_periodic(C, n, n)
+    u, v = stream_velocity(n, n, nmodes, seed + 1, periodic=True)
+    vmax = max(np.abs(u).max(), np.abs(v).max())
+    dt = cfl * (1.0 / n) / vmax
+    return C, u, v, dt
+
+
+def vortex_velocity(nx, ny, t, dt, T=6.0, lx=1.0, ly=1.0, walls=True):
+    """Reversible shear vortex of ``src/module_simulation.f90:2254-2288``."""
+    dx, dy = lx / nx, ly / ny
+    cf = math.cos(math.pi * (t + 0.5 * dt) / T)
+    u = np.zeros((ny + 2, nx + 2))
+    v = np.zeros((ny + 2, nx + 2))
+    u_nx = nx + 1 if walls else nx
+    v_ny = ny + 1 if walls else ny
+    iu = np.arange(1, u_nx + 1)
+    jc = np.arange(1, ny + 1)
+    sx = np.sin(math.pi * x_face(iu, dx))
+    sy = np.sin(2.0 * math.pi * x_center(jc, dy))
+    u[1:ny + 1, 1:u_nx + 1] = -(sx * sx)[None, :] * sy[:, None] * cf
+    ic = np.arange(1, nx + 1)
+    jv = np.arange(1, v_ny + 1)
+    sx2 = np.sin(2.0 * math.pi * x_center(ic, dx))
+    sy2 = np.sin(math.pi * x_face(jv, dy))
+    v[1:v_ny + 1, 1:nx + 1] = sx2[None, :] * (sy2 * sy2)[:, None] * cf
+    return u, v
+
+
+def disc(nx, ny, xc, yc, r, lx=1.0, ly=1.0, sub=16):
+    """Area fraction of a disc by ``sub x sub`` point sampling (as the
+    reference's bubble initial condition, ``src/module_interface_vof.f90:128-163``)."""
+    dx, dy = lx / nx, ly / ny
+    C = np.zeros((ny + 2, nx + 2))
+    s = (np.arange(sub) + 0.5) / sub
+    xs = ((np.arange(nx)[:, None] + s[None, :]) * dx).reshape(-1)
+    ys = ((np.arange(ny)[:, None] + s[None, :]) * dy).reshape(-1)
+    inside = ((xs[None, :] - xc) ** 2 + (ys[:, None] - yc) ** 2) <= r * r
+    C[1:ny + 1, 1:nx + 1] = inside.reshape(ny, sub, nx, sub).mean(axis=(1, 3))
+    return C
+
+
+# ------------------------------------------------------------------------- Poisson
+
+def circles_indicator(n, n_circles, rmin_cells, rmax_cells, seed, lx=1.0):
+    """Sharp 0/1 union of seeded circles, then one 3x3 box-average pass
+    (edges replicated); interior ``(n, n)``."""
+    rng = np.random.default_rng(seed)
+    dx = lx / n
+    xc = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc)
+    ind = np.zeros((n, n))
+    for _ in range(n_circles):
+        r = rng.uniform(rmin_cells, rmax_cells) * dx
+        cx, cy = rng.uniform(0.0, lx, size=2)
+        # work on the bounding box only
+        i0 = max(0, int((cx - r) / dx) - 1)
+        i1 = min(n, int((cx + r) / dx) + 2)
+        j0 = max(0, int((cy - r) / dx) - 1)
+        j1 = min(n, int((cy + r) / dx) + 2)
+        sub = (X[j0:j1, i0:i1] - cx) ** 2 + (Y[j0:j1, i0:i1] - cy) ** 2 <= r * r
+        ind[j0:j1, i0:i1] = np.maximum(ind[j0:j1, i0:i1], sub)
+    P = np.pad(ind, 1, mode="edge")
+    sm = sum(P[1 + dj:n + 1 + dj, 1 + di:n + 1 + di] for dj in (-1, 0, 1) for di in (-1, 0, 1)) / 9.0
+    return sm
+
+
+def poisson_case(n, seed=0, n_circles=64, rmin=16.0, rmax=128.0, rho_liquid=1000.0,
+                 rho_gas=1.0):
+    """Variable-density Poisson inputs in the reference's field layout.
+
+    Returns ``dict(C, u, v, rho_u, rho_v, p, rhs, dt, bc)``: ``C`` with a
+    zero-gradient halo; ``rho_u``/``rho_v`` are the arithmetic face means of
+    ``rho = rho_liquid C + rho_gas (1 - C)`` (``src/module_interface_vof.f90:538-591``);
+    ``u`` is zero and ``v`` carries the right-hand side: with ``dx = dy`` the
+    Poisson right-hand side ``b = -[(u_E-u_W) dy + (v_N-v_S) dx]`` of
+    ``src/module_simulation.f90:2483-2486`` equals ``rhs`` (zero-mean
+    N(0,1), seed ``seed``) up to the rounding of one telescoping sum, and the
+    exact ``b`` is returned as ``rhs``.
+    """
+    ind = circles_indicator(n, n_circles, rmin, rmax, seed)
+    C = np.zeros((n + 2, n + 2))
+    C[1:n + 1, 1:n + 1] = ind
+    fill_halo_zero_gradient(C, n, n)
+    rho = C * rho_liquid + (1.0 - C) * rho_gas
+    rho_u = np.ones((n + 2, n + 2))
+    rho_v = np.ones((n + 2, n + 2))
+    rho_u[1:n + 1, 1:n + 2] = 0.5 * (rho[1:n + 1, 0:n + 1] + rho[1:n + 1, 1:n + 2])
+    rho_v[1:n + 2, 1:n + 1] = 0.5 * (rho[0:n + 1, 1:n + 1] + rho[1:n + 2, 1:n + 1])
+    rng = np.random.default_rng(seed)
+    g = rng.normal(size=(n, n))
+    g -= g.mean()
+    dx = 1.0 / n
+    # v(i, j+1) - v(i, j) = -g(i, j) / dx  with v(i, 1) = 0  (column cumsum)
+    v = np.zeros((n + 2, n + 2))
+    v[2:n + 2, 1:n + 1] = np.cumsum(-g / dx, axis=0)
+    u = np.zeros((n + 2, n + 2))
+    rhs = -((v[2:n + 2, 1:n + 1] - v[1:n + 1, 1:n + 1]) * dx)
+    return dict(C=C, u=u, v=v, rho_u=rho_u, rho_v=rho_v, p=np.zeros((n + 2, n + 2)),
+                rhs=rhs, dt=1.0, bc=(BC_FREESLIP,) * 4)

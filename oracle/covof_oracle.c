@@ -1,0 +1,786 @@
+/*
+ * CPU oracle of the COVOF hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference Fortran for the per-timestep hot path
+ * (interface geometry, PLIC/ELVIRA reconstruction, Weymouth-Yue advection,
+ * VOF halo/BC, physical properties, SP Poisson assembly / matvec / SGS / PCG,
+ * BiCGSTAB(2), velocity correction).  Every function cites the reference
+ * file:line it follows.  It is compiled with -ffp-contract=off and evaluates
+ * every expression in the reference's left-to-right order, so that on
+ * identical inputs it reproduces the reference bit-for-bit.
+ *
+ * Who may use it: tests/ (as the checker), __graft_entry__.smoke() (as the
+ * checker) and bench.py's cpu_baseline leg (as the timed CPU baseline).  The
+ * shipped library never links, loads or calls anything here.
+ *
+ * Pins (where each function is checked; see tests/):
+ *   cut_box_area, compute_alpha   golden vectors from the compiled reference,
+ *                                 closed-form areas, bisection round trip
+ *   reconstruct_* / interface     golden vectors (PLIC, ELVIRA, periodic, walls),
+ *                                 straight-line exactness of ELVIRA
+ *   wy_step, apply_vof_bc,        golden multi-step vectors (both Strang orders,
+ *   update_properties             periodic and walls), exact volume telescoping
+ *   poisson_assemble/matvec/sgs   symmetry, SPD, explicit (D-L)D^-1(D-U) solve,
+ *   sp_pcg, bicgstab2             direct sparse solve, golden SP vectors
+ *   sp_correct                    discrete divergence-free after projection
+ */
+#include "covof_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+enum { BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_PERIODIC = 3, BC_FREESLIP = 4 };
+
+#define H(m, i, j) ((long)(j) * ((m)->nx + 2) + (i))      /* halo field index  */
+#define W(m, i, j) ((long)((j) - 1) * (m)->nx + ((i) - 1)) /* (nx,ny) work index */
+
+/* src/module_mesh.f90:67-144 (single rank: i_start_global = 1, dx = lx/nx). */
+void orc_mesh_init(orc_mesh *m, int nx, int ny, double lx, double ly, const int bc[4]) {
+  m->nx = nx;
+  m->ny = ny;
+  m->lx = lx;
+  m->ly = ly;
+  m->dx = lx / (double)nx;
+  m->dy = ly / (double)ny;
+  for (int k = 0; k < 4; ++k) m->bc[k] = bc[k];
+}
+
+/* Periodicity of the Cartesian topology (src/module_simulation.f90:115-117):
+ * on one rank a periodic dimension has no MPI_PROC_NULL neighbour, so the
+ * is_boundary_* flags of src/module_mesh.f90:139-142 are the negation. */
+static int per_x(const orc_mesh *m) { return m->bc[0] == BC_PERIODIC || m->bc[1] == BC_PERIODIC; }
+static int per_y(const orc_mesh *m) { return m->bc[2] == BC_PERIODIC || m->bc[3] == BC_PERIODIC; }
+
+/* src/module_mesh.f90:337-351 */
+static double x_face(const orc_mesh *m, int i) { return (double)(i - 1) * m->dx; }
+static double y_face(const orc_mesh *m, int j) { return (double)(j - 1) * m->dy; }
+
+static double dmin(double a, double b) { return a < b ? a : b; }
+static double dmax(double a, double b) { return a > b ? a : b; }
+
+/* src/module_interface_reconstruction.f90:33-101 */
+double orc_cut_box_area(double xmin, double xmax, double ymin, double ymax,
+                        double nx, double ny, double alpha) {
+  if ((xmax - xmin) <= 1.0e-14 || (ymax - ymin) <= 1.0e-14) return 0.0;
+  const double bx[4] = {xmin, xmax, xmax, xmin};
+  const double by[4] = {ymin, ymin, ymax, ymax};
+  double d[4];
+  for (int k = 0; k < 4; ++k) d[k] = nx * bx[k] + ny * by[k] + alpha;
+  double cx[8], cy[8];
+  int n = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int kn = (k + 1) & 3;
+    const double d1 = d[k], d2 = d[kn];
+    if (d1 <= 0.0) {
+      if (d2 <= 0.0) {
+        cx[n] = bx[kn];
+        cy[n] = by[kn];
+        ++n;
+      } else {
+        const double t = -d1 / (d2 - d1);
+        cx[n] = bx[k] + t * (bx[kn] - bx[k]);
+        cy[n] = by[k] + t * (by[kn] - by[k]);
+        ++n;
+      }
+    } else if (d2 <= 0.0) {
+      const double t = -d1 / (d2 - d1);
+      cx[n] = bx[k] + t * (bx[kn] - bx[k]);
+      cy[n] = by[k] + t * (by[kn] - by[k]);
+      ++n;
+      cx[n] = bx[kn];
+      cy[n] = by[kn];
+      ++n;
+    }
+  }
+  if (n < 3) return 0.0;
+  double s = 0.0;
+  for (int p = 0; p < n; ++p) {
+    const int q = (p + 1) % n;
+    s = s + (cx[p] * cy[q] - cx[q] * cy[p]);
+  }
+  return 0.5 * fabs(s);
+}
+
+/* src/module_interface_reconstruction.f90:105-147 */
+double orc_compute_alpha(double xmin, double xmax, double ymin, double ymax,
+                         double nx, double ny, double c) {
+  const double total = (xmax - xmin) * (ymax - ymin);
+  if (c <= 1.0e-7) return 1.0e9;
+  if (c >= 1.0 - 1.0e-7) return -1.0e9;
+  const double target = dmax(0.0, dmin(1.0, c)) * total;
+  const double v1 = nx * xmin + ny * ymin;
+  const double v2 = nx * xmax + ny * ymin;
+  const double v3 = nx * xmax + ny * ymax;
+  const double v4 = nx * xmin + ny * ymax;
+  const double vmin = dmin(v1, dmin(v2, dmin(v3, v4)));
+  const double vmax = dmax(v1, dmax(v2, dmax(v3, v4)));
+  double lo = -vmax - 1.0e-12;
+  double hi = -vmin + 1.0e-12;
+  for (int it = 0; it < 45; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    const double a = orc_cut_box_area(xmin, xmax, ymin, ymax, nx, ny, mid);
+    if (a < target)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+static int interior(const orc_mesh *m, int i, int j) {
+  return i >= 1 && i <= m->nx && j >= 1 && j <= m->ny;
+}
+
+/* src/module_interface_reconstruction.f90:283-343 (Youngs) */
+static void plic_cell(const orc_mesh *m, const double *C, int i, int j,
+                      double *onx, double *ony, double *oal) {
+  const double c = C[H(m, i, j)];
+  if (c <= 1.0e-6) { *onx = 1.0; *ony = 0.0; *oal = 1.0e9; return; }
+  if (c >= 1.0 - 1.0e-6) { *onx = 1.0; *ony = 0.0; *oal = -1.0e9; return; }
+  double mx, my;
+  if (interior(m, i, j)) {
+    const double c_mm = C[H(m, i - 1, j - 1)], c_0m = C[H(m, i, j - 1)], c_pm = C[H(m, i + 1, j - 1)];
+    const double c_m0 = C[H(m, i - 1, j)], c_p0 = C[H(m, i + 1, j)];
+    const double c_mp = C[H(m, i - 1, j + 1)], c_0p = C[H(m, i, j + 1)], c_pp = C[H(m, i + 1, j + 1)];
+    mx = -(((c_pp + 2.0 * c_p0 + c_pm) - (c_mp + 2.0 * c_m0 + c_mm)) / (8.0 * m->dx));
+    my = -(((c_pp + 2.0 * c_0p + c_mp) - (c_pm + 2.0 * c_0m + c_mm)) / (8.0 * m->dy));
+  } else {
+    mx = 1.0;
+    my = 0.0;
+  }
+  const double nrm = hypot(mx, my);
+  double pnx, pny;
+  if (nrm > 1.0e-12) { pnx = mx / nrm; pny = my / nrm; } else { pnx = 1.0; pny = 0.0; }
+  *onx = pnx;
+  *ony = pny;
+  *oal = orc_compute_alpha(x_face(m, i), x_face(m, i + 1), y_face(m, j), y_face(m, j + 1), pnx, pny, c);
+}
+
+/* src/module_interface_reconstruction.f90:346-457 (ELVIRA, 7 candidates) */
+static void elvira_cell(const orc_mesh *m, const double *C, int i, int j,
+                        double *onx, double *ony, double *oal) {
+  const double c = C[H(m, i, j)];
+  if (c <= 1.0e-6) { *onx = 1.0; *ony = 0.0; *oal = 1.0e9; return; }
+  if (c >= 1.0 - 1.0e-6) { *onx = 1.0; *ony = 0.0; *oal = -1.0e9; return; }
+  const double dx = m->dx, dy = m->dy;
+  const double xmin = x_face(m, i), xmax = x_face(m, i + 1);
+  const double ymin = y_face(m, j), ymax = y_face(m, j + 1);
+  if (!interior(m, i, j)) {
+    *onx = 1.0;
+    *ony = 0.0;
+    *oal = orc_compute_alpha(xmin, xmax, ymin, ymax, 1.0, 0.0, c);
+    return;
+  }
+#define CC(a, b) C[H(m, i + (a), j + (b))]
+  const double H_m1 = (CC(-1, -1) + CC(-1, 0) + CC(-1, 1)) * dy;
+  const double H_0 = (CC(0, -1) + CC(0, 0) + CC(0, 1)) * dy;
+  const double H_p1 = (CC(1, -1) + CC(1, 0) + CC(1, 1)) * dy;
+  const double W_m1 = (CC(-1, -1) + CC(0, -1) + CC(1, -1)) * dx;
+  const double W_0 = (CC(-1, 0) + CC(0, 0) + CC(1, 0)) * dx;
+  const double W_p1 = (CC(-1, 1) + CC(0, 1) + CC(1, 1)) * dx;
+  const double sgn_y = (CC(0, 1) < CC(0, -1)) ? 1.0 : -1.0;
+  const double sgn_x = (CC(1, 0) < CC(-1, 0)) ? 1.0 : -1.0;
+  double cnx[7], cny[7];
+  cnx[0] = -(sgn_y * (H_p1 - H_m1) / (2.0 * dx)); cny[0] = sgn_y * 1.0;
+  cnx[1] = -(sgn_y * (H_p1 - H_0) / dx);          cny[1] = sgn_y * 1.0;
+  cnx[2] = -(sgn_y * (H_0 - H_m1) / dx);          cny[2] = sgn_y * 1.0;
+  cnx[3] = sgn_x * 1.0; cny[3] = -(sgn_x * (W_p1 - W_m1) / (2.0 * dy));
+  cnx[4] = sgn_x * 1.0; cny[4] = -(sgn_x * (W_p1 - W_0) / dy);
+  cnx[5] = sgn_x * 1.0; cny[5] = -(sgn_x * (W_0 - W_m1) / dy);
+  cnx[6] = -(((CC(1, 1) + 2.0 * CC(1, 0) + CC(1, -1)) - (CC(-1, 1) + 2.0 * CC(-1, 0) + CC(-1, -1))) / (8.0 * dx));
+  cny[6] = -(((CC(1, 1) + 2.0 * CC(0, 1) + CC(-1, 1)) - (CC(1, -1) + 2.0 * CC(0, -1) + CC(-1, -1))) / (8.0 * dy));
+  double best_err = 1.0e30, bnx = 1.0, bny = 0.0, bal = 0.0;
+  for (int k = 0; k < 7; ++k) {
+    const double cn = hypot(cnx[k], cny[k]);
+    if (cn < 1.0e-12) continue;
+    const double ux = cnx[k] / cn, uy = cny[k] / cn;
+    const double al = orc_compute_alpha(xmin, xmax, ymin, ymax, ux, uy, c);
+    double err = 0.0;
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int di = -1; di <= 1; ++di) {
+        if (di == 0 && dj == 0) continue;
+        const double vt = CC(di, dj) * dx * dy;
+        const double vr = orc_cut_box_area(x_face(m, i + di), x_face(m, i + di + 1),
+                                           y_face(m, j + dj), y_face(m, j + dj + 1), ux, uy, al);
+        const double df = vr - vt;
+        err = err + df * df;
+      }
+    if (err < best_err) { best_err = err; bnx = ux; bny = uy; bal = al; }
+  }
+#undef CC
+  *onx = bnx;
+  *ony = bny;
+  *oal = bal;
+}
+
+void orc_reconstruct(const orc_mesh *m, int method, const double *C,
+                     double *snx, double *sny, double *sal) {
+  for (int j = 0; j <= m->ny + 1; ++j)
+    for (int i = 0; i <= m->nx + 1; ++i) {
+      const long k = H(m, i, j);
+      if (method == 1)
+        elvira_cell(m, C, i, j, &snx[k], &sny[k], &sal[k]);
+      else
+        plic_cell(m, C, i, j, &snx[k], &sny[k], &sal[k]);
+    }
+}
+
+/* src/module_mpi_comm.f90:46-56 (single-rank branch) */
+void orc_exchange_halos_scalar(const orc_mesh *m, double *f) {
+  const int nx = m->nx, ny = m->ny;
+  if (per_x(m)) {
+    for (int j = 0; j <= ny + 1; ++j) f[H(m, 0, j)] = f[H(m, nx, j)];
+    for (int j = 0; j <= ny + 1; ++j) f[H(m, nx + 1, j)] = f[H(m, 1, j)];
+  }
+  if (per_y(m)) {
+    for (int i = 0; i <= nx + 1; ++i) f[H(m, i, 0)] = f[H(m, i, ny)];
+    for (int i = 0; i <= nx + 1; ++i) f[H(m, i, ny + 1)] = f[H(m, i, 1)];
+  }
+}
+
+/* src/module_interface_reconstruction.f90:580-683 */
+void orc_reconstruct_interface(const orc_mesh *m, int method, const double *C,
+                               double *snx, double *sny, double *sal) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)(nx + 2) * (ny + 2);
+  orc_reconstruct(m, method, C, snx, sny, sal);
+  double *gx = calloc(n, sizeof(double)), *gy = calloc(n, sizeof(double));
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      gx[H(m, i, j)] = snx[H(m, i, j)];
+      gy[H(m, i, j)] = sny[H(m, i, j)];
+    }
+  orc_exchange_halos_scalar(m, gx);
+  orc_exchange_halos_scalar(m, gy);
+  if (!per_x(m)) {
+    for (int j = 1; j <= ny; ++j) { gx[H(m, 0, j)] = gx[H(m, 1, j)]; gy[H(m, 0, j)] = gy[H(m, 1, j)]; }
+    for (int j = 1; j <= ny; ++j) { gx[H(m, nx + 1, j)] = gx[H(m, nx, j)]; gy[H(m, nx + 1, j)] = gy[H(m, nx, j)]; }
+  }
+  if (!per_y(m)) {
+    for (int i = 0; i <= nx + 1; ++i) { gx[H(m, i, 0)] = gx[H(m, i, 1)]; gy[H(m, i, 0)] = gy[H(m, i, 1)]; }
+    for (int i = 0; i <= nx + 1; ++i) { gx[H(m, i, ny + 1)] = gx[H(m, i, ny)]; gy[H(m, i, ny + 1)] = gy[H(m, i, ny)]; }
+  }
+#define FIX(ii, jj)                                                                  \
+  do {                                                                               \
+    const long k_ = H(m, ii, jj);                                                    \
+    const double c_ = C[k_];                                                         \
+    if (c_ > 1.0e-6 && c_ < 1.0 - 1.0e-6) {                                          \
+      double sx_ = gx[k_], sy_ = gy[k_];                                             \
+      const double nr_ = hypot(sx_, sy_);                                            \
+      if (nr_ > 1.0e-12) {                                                           \
+        sx_ = sx_ / nr_;                                                             \
+        sy_ = sy_ / nr_;                                                             \
+        snx[k_] = sx_;                                                               \
+        sny[k_] = sy_;                                                               \
+        sal[k_] = orc_compute_alpha(x_face(m, ii), x_face(m, (ii) + 1), y_face(m, jj), \
+                                    y_face(m, (jj) + 1), sx_, sy_, c_);              \
+      }                                                                              \
+    }                                                                                \
+  } while (0)
+  for (int j = 1; j <= ny; ++j) {
+    FIX(0, j);
+    FIX(nx + 1, j);
+  }
+  for (int i = 0; i <= nx + 1; ++i) {
+    FIX(i, 0);
+    FIX(i, ny + 1);
+  }
+#undef FIX
+  free(gx);
+  free(gy);
+}
+
+/* src/module_mpi_comm.f90:636-683 (single rank, top profile not JET_INLET) */
+void orc_apply_vof_bc(const orc_mesh *m, double *C) {
+  const int nx = m->nx, ny = m->ny;
+  if (m->bc[0] == BC_PERIODIC) {
+    for (int j = 1; j <= ny; ++j) C[H(m, 0, j)] = C[H(m, nx, j)];
+    for (int j = 1; j <= ny; ++j) C[H(m, nx + 1, j)] = C[H(m, 1, j)];
+  } else {
+    if (!per_x(m)) for (int j = 1; j <= ny; ++j) C[H(m, 0, j)] = C[H(m, 1, j)];
+    if (!per_x(m)) for (int j = 1; j <= ny; ++j) C[H(m, nx + 1, j)] = C[H(m, nx, j)];
+  }
+  if (m->bc[2] == BC_PERIODIC) {
+    for (int i = 1; i <= nx; ++i) C[H(m, i, 0)] = C[H(m, i, ny)];
+    for (int i = 1; i <= nx; ++i) C[H(m, i, ny + 1)] = C[H(m, i, 1)];
+  } else {
+    if (!per_y(m)) for (int i = 1; i <= nx; ++i) C[H(m, i, 0)] = C[H(m, i, 1)];
+  }
+  if (!per_y(m))
+    for (int i = 1; i <= nx; ++i) C[H(m, i, ny + 1)] = C[H(m, i, ny)];
+}
+
+/* One directional pass of src/module_interface_vof.f90:1162-1377.
+ * dir 0 = X (Eq. 15), 1 = Y (Eq. 16).  src -> dst (dst's halo starts as src's). */
+static void wy_pass(const orc_mesh *m, int method, int dir, int clamp, const double *src,
+                    double *dst, const double *u, const double *v, const double *cd,
+                    double dt, double *flux, double *snx, double *sny, double *sal) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)(nx + 2) * (ny + 2);
+  const double dx = m->dx, dy = m->dy, area = dx * dy;
+  orc_reconstruct_interface(m, method, src, snx, sny, sal);
+  memset(flux, 0, n * sizeof(double));
+  if (dir == 0) {
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx + 1; ++i) {
+        const double uf = u[H(m, i, j)];
+        const double del = fabs(uf) * dt;
+        if (del <= 1.0e-14) continue;
+        const double xp = x_face(m, i);
+        if (uf > 0.0) {
+          const long k = H(m, i - 1, j);
+          flux[H(m, i, j)] = orc_cut_box_area(xp - del, xp, y_face(m, j), y_face(m, j + 1), snx[k], sny[k], sal[k]);
+        } else {
+          const long k = H(m, i, j);
+          flux[H(m, i, j)] = -orc_cut_box_area(xp, xp + del, y_face(m, j), y_face(m, j + 1), snx[k], sny[k], sal[k]);
+        }
+      }
+    if (m->bc[0] == BC_PERIODIC)
+      for (int j = 1; j <= ny; ++j) {
+        if (u[H(m, 1, j)] >= 0.0)
+          flux[H(m, 1, j)] = flux[H(m, nx + 1, j)];
+        else
+          flux[H(m, nx + 1, j)] = flux[H(m, 1, j)];
+      }
+    memcpy(dst, src, n * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        double c = src[H(m, i, j)] - (flux[H(m, i + 1, j)] - flux[H(m, i, j)]) / area +
+                   cd[H(m, i, j)] * (dt / dx) * (u[H(m, i + 1, j)] - u[H(m, i, j)]);
+        if (clamp) c = dmax(0.0, dmin(1.0, c));
+        dst[H(m, i, j)] = c;
+      }
+  } else {
+    for (int j = 1; j <= ny + 1; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const double vf = v[H(m, i, j)];
+        const double del = fabs(vf) * dt;
+        if (del <= 1.0e-14) continue;
+        const double yp = y_face(m, j);
+        if (vf > 0.0) {
+          const long k = H(m, i, j - 1);
+          flux[H(m, i, j)] = orc_cut_box_area(x_face(m, i), x_face(m, i + 1), yp - del, yp, snx[k], sny[k], sal[k]);
+        } else {
+          const long k = H(m, i, j);
+          flux[H(m, i, j)] = -orc_cut_box_area(x_face(m, i), x_face(m, i + 1), yp, yp + del, snx[k], sny[k], sal[k]);
+        }
+      }
+    if (m->bc[2] == BC_PERIODIC)
+      for (int i = 1; i <= nx; ++i) {
+        if (v[H(m, i, 1)] >= 0.0)
+          flux[H(m, i, 1)] = flux[H(m, i, ny + 1)];
+        else
+          flux[H(m, i, ny + 1)] = flux[H(m, i, 1)];
+      }
+    memcpy(dst, src, n * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        double c = src[H(m, i, j)] - (flux[H(m, i, j + 1)] - flux[H(m, i, j)]) / area +
+                   cd[H(m, i, j)] * (dt / dy) * (v[H(m, i, j + 1)] - v[H(m, i, j)]);
+        if (clamp) c = dmax(0.0, dmin(1.0, c));
+        dst[H(m, i, j)] = c;
+      }
+  }
+}
+
+/* advect_vof with WEYMOUTH_YUE (src/module_interface_vof.f90:792-825, :1162-1377),
+ * without the property update (orc_update_properties is the next call there). */
+void orc_wy_step(const orc_mesh *m, int method, double *C, const double *u, const double *v,
+                 double dt, int *step_count) {
+  const long n = (long)(m->nx + 2) * (m->ny + 2);
+  double *cd = malloc(n * sizeof(double)), *cs = malloc(n * sizeof(double));
+  double *fl = malloc(n * sizeof(double)), *sx = malloc(n * sizeof(double));
+  double *sy = malloc(n * sizeof(double)), *sa = malloc(n * sizeof(double));
+  for (long k = 0; k < n; ++k) cd[k] = (C[k] >= 0.5) ? 1.0 : 0.0;
+  const int x_first = ((*step_count) % 2) == 0;
+  const int d1 = x_first ? 0 : 1, d2 = x_first ? 1 : 0;
+  wy_pass(m, method, d1, 0, C, cs, u, v, cd, dt, fl, sx, sy, sa);
+  orc_exchange_halos_scalar(m, cs);
+  orc_apply_vof_bc(m, cs);
+  wy_pass(m, method, d2, 1, cs, C, u, v, cd, dt, fl, sx, sy, sa);
+  *step_count += 1;
+  orc_exchange_halos_scalar(m, C);
+  orc_apply_vof_bc(m, C);
+  free(cd); free(cs); free(fl); free(sx); free(sy); free(sa);
+}
+
+/* src/module_interface_vof.f90:479-616 (incompressible, no SVOF smoothing). */
+void orc_update_properties(const orc_mesh *m, double *C, double rho1, double rho2,
+                           double mu1, double mu2, double *rho, double *rho_u,
+                           double *rho_v, double *mu, double *mu_node) {
+  const int nx = m->nx, ny = m->ny;
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  if (bl && bb) C[H(m, 0, 0)] = C[H(m, 1, 1)];
+  if (br && bb) C[H(m, nx + 1, 0)] = C[H(m, nx, 1)];
+  if (bl && bt) C[H(m, 0, ny + 1)] = C[H(m, 1, ny)];
+  if (br && bt) C[H(m, nx + 1, ny + 1)] = C[H(m, nx, ny)];
+  for (int j = 0; j <= ny + 1; ++j)
+    for (int i = 0; i <= nx + 1; ++i) {
+      const double c = dmax(0.0, dmin(1.0, C[H(m, i, j)]));
+      rho[H(m, i, j)] = c * rho1 + (1.0 - c) * rho2;
+      mu[H(m, i, j)] = c * mu1 + (1.0 - c) * mu2;
+    }
+  mu[H(m, 0, 0)] = mu[H(m, 1, 1)];
+  mu[H(m, nx + 1, 0)] = mu[H(m, nx, 1)];
+  mu[H(m, 0, ny + 1)] = mu[H(m, 1, ny)];
+  mu[H(m, nx + 1, ny + 1)] = mu[H(m, nx, ny)];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx + 1; ++i) rho_u[H(m, i, j)] = 0.5 * (rho[H(m, i - 1, j)] + rho[H(m, i, j)]);
+  for (int j = 1; j <= ny + 1; ++j)
+    for (int i = 1; i <= nx; ++i) rho_v[H(m, i, j)] = 0.5 * (rho[H(m, i, j - 1)] + rho[H(m, i, j)]);
+  for (int j = 1; j <= ny + 1; ++j)
+    for (int i = 1; i <= nx + 1; ++i) {
+      const double sw = mu[H(m, i - 1, j - 1)], se = mu[H(m, i, j - 1)];
+      const double nw = mu[H(m, i - 1, j)], ne = mu[H(m, i, j)];
+      mu_node[H(m, i, j)] = 4.0 / (1.0 / sw + 1.0 / se + 1.0 / nw + 1.0 / ne);
+    }
+}
+
+/* ---------------------------------------------------------------- Poisson */
+
+static int wall(int btype) { return btype == BC_DIRICHLET || btype == BC_FREESLIP; }
+
+/* src/module_simulation.f90:2477-2533 (no obstacles). */
+void orc_poisson_assemble(const orc_mesh *m, const double *u, const double *v,
+                          const double *rho_u, const double *rho_v, double dt,
+                          double *aW, double *aE, double *aS, double *aN, double *aP,
+                          double *b) {
+  const int nx = m->nx, ny = m->ny;
+  const double dx = m->dx, dy = m->dy, dvol = dx * dy;
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long k = W(m, i, j);
+      const double div = (u[H(m, i + 1, j)] - u[H(m, i, j)]) * dy + (v[H(m, i, j + 1)] - v[H(m, i, j)]) * dx;
+      b[k] = -div;
+      aW[k] = aE[k] = aS[k] = aN[k] = 0.0;
+      if (!(i == 1 && bl && wall(m->bc[0]))) {
+        double r = rho_u[H(m, i, j)];
+        if (r < 1.0e-12) r = 1.0;
+        aW[k] = (dt * dy) / (r * dx);
+      }
+      if (!(i == nx && br && wall(m->bc[1]))) {
+        double r = rho_u[H(m, i + 1, j)];
+        if (r < 1.0e-12) r = 1.0;
+        aE[k] = (dt * dy) / (r * dx);
+      }
+      if (!(j == 1 && bb && wall(m->bc[2]))) {
+        double r = rho_v[H(m, i, j)];
+        if (r < 1.0e-12) r = 1.0;
+        aS[k] = (dt * dx) / (r * dy);
+      }
+      if (!(j == ny && bt && wall(m->bc[3]))) {
+        double r = rho_v[H(m, i, j + 1)];
+        if (r < 1.0e-12) r = 1.0;
+        aN[k] = (dt * dx) / (r * dy);
+      }
+      aP[k] = aW[k] + aE[k] + aS[k] + aN[k] + 1.0e-12 * dvol;
+    }
+}
+
+/* src/module_simulation.f90:2668-2707 */
+void orc_poisson_matvec(const orc_mesh *m, const double *aP, const double *aW,
+                        const double *aE, const double *aS, const double *aN,
+                        const double *x, double *y) {
+  const int nx = m->nx, ny = m->ny;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long k = W(m, i, j);
+      double val = aP[k] * x[k];
+      if (i > 1) val = val - aW[k] * x[W(m, i - 1, j)];
+      else if (m->bc[0] == BC_PERIODIC) val = val - aW[k] * x[W(m, nx, j)];
+      if (i < nx) val = val - aE[k] * x[W(m, i + 1, j)];
+      else if (m->bc[1] == BC_PERIODIC) val = val - aE[k] * x[W(m, 1, j)];
+      if (j > 1) val = val - aS[k] * x[W(m, i, j - 1)];
+      else if (m->bc[2] == BC_PERIODIC) val = val - aS[k] * x[W(m, i, ny)];
+      if (j < ny) val = val - aN[k] * x[W(m, i, j + 1)];
+      else if (m->bc[3] == BC_PERIODIC) val = val - aN[k] * x[W(m, i, 1)];
+      y[k] = val;
+    }
+}
+
+/* src/module_simulation.f90:2709-2758 (symmetric Gauss-Seidel, omega = 1). */
+void orc_poisson_sgs(const orc_mesh *m, const double *aP, const double *aW,
+                     const double *aE, const double *aS, const double *aN,
+                     const double *r, double *z) {
+  const int nx = m->nx, ny = m->ny;
+  memset(z, 0, (size_t)nx * ny * sizeof(double));
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long k = W(m, i, j);
+      double val = r[k];
+      if (i > 1) val = val + aW[k] * z[W(m, i - 1, j)];
+      else if (m->bc[0] == BC_PERIODIC) val = val + aW[k] * z[W(m, nx, j)];
+      if (j > 1) val = val + aS[k] * z[W(m, i, j - 1)];
+      else if (m->bc[2] == BC_PERIODIC) val = val + aS[k] * z[W(m, i, ny)];
+      z[k] = val / aP[k];
+    }
+  for (int j = ny; j >= 1; --j)
+    for (int i = nx; i >= 1; --i) {
+      const long k = W(m, i, j);
+      double val = aP[k] * z[k];
+      if (i < nx) val = val + aE[k] * z[W(m, i + 1, j)];
+      else if (m->bc[1] == BC_PERIODIC) val = val + aE[k] * z[W(m, 1, j)];
+      if (j < ny) val = val + aN[k] * z[W(m, i, j + 1)];
+      else if (m->bc[3] == BC_PERIODIC) val = val + aN[k] * z[W(m, i, 1)];
+      z[k] = val / aP[k];
+    }
+}
+
+/* Jacobi: src/module_linear_solvers.f90:88-135 (z = r/d if |d| > 1e-15). */
+void orc_poisson_precond(const orc_mesh *m, int precond, const double *aP, const double *aW,
+                         const double *aE, const double *aS, const double *aN,
+                         const double *r, double *z) {
+  const long n = (long)m->nx * m->ny;
+  if (precond == 2) {
+    orc_poisson_sgs(m, aP, aW, aE, aS, aN, r, z);
+  } else if (precond == 1) {
+    for (long k = 0; k < n; ++k) z[k] = (fabs(aP[k]) > 1.0e-15) ? r[k] / aP[k] : r[k];
+  } else {
+    memcpy(z, r, n * sizeof(double));
+  }
+}
+
+double orc_dot(const double *x, const double *y, long n) {
+  double s = 0.0;
+  for (long k = 0; k < n; ++k) s = s + x[k] * y[k];
+  return s;
+}
+
+/* src/module_simulation.f90:2535-2575 */
+int orc_sp_pcg(const orc_mesh *m, int precond, const double *aP, const double *aW,
+               const double *aE, const double *aS, const double *aN,
+               const double *b, double *x, double tol, int maxit, double *relres) {
+  const long n = (long)m->nx * m->ny;
+  double *r = malloc(n * sizeof(double)), *Ad = malloc(n * sizeof(double));
+  double *z = malloc(n * sizeof(double)), *d = malloc(n * sizeof(double));
+  orc_poisson_matvec(m, aP, aW, aE, aS, aN, x, Ad);
+  for (long k = 0; k < n; ++k) r[k] = b[k] - Ad[k];
+  double b_norm = sqrt(orc_dot(b, b, n));
+  if (b_norm < 1.0e-30) b_norm = 1.0;
+  double r_norm = sqrt(orc_dot(r, r, n));
+  int iters = 0;
+  if (r_norm / b_norm > tol) {
+    orc_poisson_precond(m, precond, aP, aW, aE, aS, aN, r, z);
+    memcpy(d, z, n * sizeof(double));
+    double rz_old = orc_dot(r, z, n);
+    for (int it = 1; it <= maxit; ++it) {
+      ++iters;
+      orc_poisson_matvec(m, aP, aW, aE, aS, aN, d, Ad);
+      const double dAd = orc_dot(d, Ad, n);
+      if (fabs(dAd) < 1.0e-35) break;
+      const double alpha = rz_old / dAd;
+      for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * d[k];
+      for (long k = 0; k < n; ++k) r[k] = r[k] - alpha * Ad[k];
+      r_norm = sqrt(orc_dot(r, r, n));
+      if (r_norm / b_norm <= tol) break;
+      orc_poisson_precond(m, precond, aP, aW, aE, aS, aN, r, z);
+      const double rz_new = orc_dot(r, z, n);
+      if (fabs(rz_old) < 1.0e-35) break;
+      const double beta = rz_new / rz_old;
+      rz_old = rz_new;
+      for (long k = 0; k < n; ++k) d[k] = z[k] + beta * d[k];
+    }
+  }
+  if (relres) *relres = r_norm / b_norm;
+  free(r); free(Ad); free(z); free(d);
+  return iters;
+}
+
+/* src/module_linear_solvers.f90:361-579, with apply_PA (:323-356) = precond(A x). */
+int orc_bicgstab2(const orc_mesh *m, int precond, const double *aP, const double *aW,
+                  const double *aE, const double *aS, const double *aN,
+                  const double *b, double *x, double tol, int maxit, double *relres) {
+  const long n = (long)m->nx * m->ny;
+  double *bh = malloc(n * sizeof(double)), *rt = malloc(n * sizeof(double));
+  double *r0 = malloc(n * sizeof(double)), *r1 = calloc(n, sizeof(double));
+  double *r2 = calloc(n, sizeof(double)), *u0 = calloc(n, sizeof(double));
+  double *u1 = calloc(n, sizeof(double)), *u2 = calloc(n, sizeof(double));
+  double *tmp = malloc(n * sizeof(double));
+#define PA(src, dst)                                                  \
+  do {                                                                \
+    orc_poisson_matvec(m, aP, aW, aE, aS, aN, src, tmp);              \
+    orc_poisson_precond(m, precond, aP, aW, aE, aS, aN, tmp, dst);    \
+  } while (0)
+  orc_poisson_precond(m, precond, aP, aW, aE, aS, aN, b, bh);
+  PA(x, r0);
+  for (long k = 0; k < n; ++k) r0[k] = bh[k] - r0[k];
+  const double r0_norm = sqrt(orc_dot(r0, r0, n));
+  int iter = 0;
+  double r_norm = r0_norm;
+  if (r0_norm < 1.0e-12) goto done;
+  memcpy(rt, r0, n * sizeof(double));
+  double rho_0 = 1.0, rho_1, alpha = 0.0, omega = 1.0, beta, gamma;
+  while (iter < maxit) {
+    iter += 2;
+    rho_0 = -omega * rho_0;
+    rho_1 = orc_dot(rt, r0, n);
+    if (fabs(rho_0) < 1.0e-300) break;
+    beta = alpha * (rho_1 / rho_0);
+    rho_0 = rho_1;
+    for (long k = 0; k < n; ++k) u0[k] = r0[k] - beta * u0[k];
+    PA(u0, u1);
+    gamma = orc_dot(rt, u1, n);
+    if (fabs(gamma) < 1.0e-300) break;
+    alpha = rho_0 / gamma;
+    for (long k = 0; k < n; ++k) r0[k] = r0[k] - alpha * u1[k];
+    for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * u0[k];
+    PA(r0, r1);
+    rho_1 = orc_dot(rt, r1, n);
+    if (fabs(rho_0) < 1.0e-300) break;
+    beta = alpha * (rho_1 / rho_0);
+    rho_0 = rho_1;
+    for (long k = 0; k < n; ++k) u0[k] = r0[k] - beta * u0[k];
+    for (long k = 0; k < n; ++k) u1[k] = r1[k] - beta * u1[k];
+    PA(u1, u2);
+    gamma = orc_dot(rt, u2, n);
+    if (fabs(gamma) < 1.0e-300) break;
+    alpha = rho_0 / gamma;
+    for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * u0[k];
+    for (long k = 0; k < n; ++k) r0[k] = r0[k] - alpha * u1[k];
+    for (long k = 0; k < n; ++k) r1[k] = r1[k] - alpha * u2[k];
+    PA(r1, r2);
+    const double s11 = orc_dot(r1, r1, n), s12 = orc_dot(r2, r1, n), s22 = orc_dot(r2, r2, n);
+    const double rhs1 = orc_dot(r0, r1, n), rhs2 = orc_dot(r0, r2, n);
+    const double rn1 = sqrt(s11), rn2 = sqrt(s22);
+    double g1, g2;
+    if (rn1 < 1.0e-300) {
+      break;
+    } else if (rn2 < 1.0e-300) {
+      g2 = 0.0;
+      g1 = rhs1 / s11;
+    } else {
+      const double i1 = 1.0 / rn1, i2 = 1.0 / rn2;
+      const double c12 = s12 * i1 * i2, b1 = rhs1 * i1, b2 = rhs2 * i2;
+      const double det = 1.0 - c12 * c12;
+      if (fabs(det) < 1.0e-14) {
+        g2 = 0.0;
+        g1 = b1 * i1;
+      } else {
+        g1 = (b1 - c12 * b2) / (det * rn1);
+        g2 = (b2 - c12 * b1) / (det * rn2);
+      }
+    }
+    for (long k = 0; k < n; ++k) x[k] = x[k] + g1 * r0[k] + g2 * r1[k];
+    for (long k = 0; k < n; ++k) r0[k] = r0[k] - (g1 * r1[k] + g2 * r2[k]);
+    for (long k = 0; k < n; ++k) u0[k] = u0[k] - (g1 * u1[k] + g2 * u2[k]);
+    if (g2 != 0.0) omega = g2;
+    else if (fabs(g1) > 1.0e-300) omega = g1;
+    else omega = 1.0;
+    r_norm = sqrt(orc_dot(r0, r0, n));
+    if (r_norm != r_norm || r_norm > 1.0e30) break;
+    if (r_norm / r0_norm < tol || r_norm < 1.0e-12) break;
+  }
+done:
+#undef PA
+  if (relres) *relres = (r0_norm > 0.0) ? r_norm / r0_norm : 0.0;
+  free(bh); free(rt); free(r0); free(r1); free(r2); free(u0); free(u1); free(u2); free(tmp);
+  return iter;
+}
+
+/* src/module_simulation.f90:2577-2585 plus sync_cell_scalar on one rank
+ * (src/module_mpi_comm.f90:46-56 semantics for the periodic wrap). */
+void orc_sp_finish_pressure(const orc_mesh *m, const double *x, double *p) {
+  const long n = (long)m->nx * m->ny;
+  double s = 0.0;
+  for (long k = 0; k < n; ++k) s = s + x[k];
+  const double mean = s / (double)((long)m->nx * m->ny);
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) p[H(m, i, j)] = x[W(m, i, j)] - mean;
+  orc_exchange_halos_scalar(m, p);
+}
+
+/* src/module_simulation.f90:2590-2654 (no obstacles; halo/BC sync not included). */
+void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
+                    const double *rho_v, double dt, double *u, double *v) {
+  const int nx = m->nx, ny = m->ny;
+  const double dx = m->dx, dy = m->dy;
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  const int u_nx = br ? nx + 1 : nx, v_ny = bt ? ny + 1 : ny;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) {
+      const int is_wall = (i == 1 && bl && wall(m->bc[0])) || (i == u_nx && br && wall(m->bc[1]));
+      if (is_wall) continue;
+      double r = rho_u[H(m, i, j)];
+      if (r < 1.0e-12) r = 1.0;
+      double pw, pe;
+      if (i > 1) pw = p[H(m, i - 1, j)];
+      else if (m->bc[0] == BC_PERIODIC) pw = p[H(m, nx, j)];
+      else pw = p[H(m, 1, j)];
+      if (i <= nx) pe = p[H(m, i, j)];
+      else if (m->bc[1] == BC_PERIODIC) pe = p[H(m, 1, j)];
+      else pe = p[H(m, nx, j)];
+      const double g = (pe - pw) / dx;
+      u[H(m, i, j)] = u[H(m, i, j)] - (dt / r) * g;
+    }
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+      if (is_wall) continue;
+      double r = rho_v[H(m, i, j)];
+      if (r < 1.0e-12) r = 1.0;
+      double ps, pn;
+      if (j > 1) ps = p[H(m, i, j - 1)];
+      else if (m->bc[2] == BC_PERIODIC) ps = p[H(m, i, ny)];
+      else ps = p[H(m, i, 1)];
+      if (j <= ny) pn = p[H(m, i, j)];
+      else if (m->bc[3] == BC_PERIODIC) pn = p[H(m, i, 1)];
+      else pn = p[H(m, i, ny)];
+      const double g = (pn - ps) / dy;
+      v[H(m, i, j)] = v[H(m, i, j)] - (dt / r) * g;
+    }
+}
+
+/* CPU baseline: Jacobi-PCG iterations (same algorithm as the SP PCG loop with
+ * z = r/aP), OpenMP over `threads`.  Returns wall seconds for `iters` iterations. */
+double orc_bench_jacobi_pcg(const orc_mesh *m, const double *aP, const double *aW,
+                            const double *aE, const double *aS, const double *aN,
+                            const double *b, int iters, int threads) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)nx * ny;
+  double *x = calloc(n, sizeof(double)), *r = malloc(n * sizeof(double));
+  double *d = malloc(n * sizeof(double)), *q = malloc(n * sizeof(double));
+  omp_set_num_threads(threads);
+  double rz = 0.0;
+#pragma omp parallel for reduction(+ : rz)
+  for (long k = 0; k < n; ++k) {
+    r[k] = b[k];
+    d[k] = r[k] / aP[k];
+    rz += r[k] * d[k];
+  }
+  const double t0 = omp_get_wtime();
+  for (int it = 0; it < iters; ++it) {
+    double dq = 0.0;
+#pragma omp parallel for reduction(+ : dq)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const long k = W(m, i, j);
+        double val = aP[k] * d[k];
+        if (i > 1) val -= aW[k] * d[k - 1];
+        if (i < nx) val -= aE[k] * d[k + 1];
+        if (j > 1) val -= aS[k] * d[k - nx];
+        if (j < ny) val -= aN[k] * d[k + nx];
+        q[k] = val;
+        dq += d[k] * val;
+      }
+    const double alpha = rz / dq;
+    double rz_new = 0.0, rr = 0.0;
+#pragma omp parallel for reduction(+ : rz_new, rr)
+    for (long k = 0; k < n; ++k) {
+      x[k] += alpha * d[k];
+      r[k] -= alpha * q[k];
+      rz_new += r[k] * (r[k] / aP[k]);
+      rr += r[k] * r[k];
+    }
+    const double beta = rz_new / rz;
+    rz = rz_new;
+#pragma omp parallel for
+    for (long k = 0; k < n; ++k) d[k] = r[k] / aP[k] + beta * d[k];
+    (void)rr;
+  }
+  const double t1 = omp_get_wtime();
+  free(x); free(r); free(d); free(q);
+  return t1 - t0;
+}

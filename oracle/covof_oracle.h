@@ -1,0 +1,93 @@
+/*
+ * CPU oracle of the COVOF hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker (or the timed CPU baseline).
+ * The product path never calls it.
+ *
+ * Layout conventions (the reference's, src/module_fields.f90:129-169):
+ *   halo fields  (0:nx+1, 0:ny+1) column-major, i fastest:  F[j*(nx+2)+i]
+ *   Poisson work arrays (1:nx, 1:ny) column-major:         A[(j-1)*nx+(i-1)]
+ * Boundary types are the reference's integers (src/module_constants.f90:25-31):
+ *   1 DIRICHLET, 2 NEUMANN, 3 PERIODIC, 4 FREESLIP.
+ */
+#ifndef COVOF_ORACLE_H
+#define COVOF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nx, ny;          /* cells (single rank: local == global) */
+  double lx, ly, dx, dy;
+  int bc[4];           /* left, right, bottom, top */
+} orc_mesh;
+
+void orc_mesh_init(orc_mesh *m, int nx, int ny, double lx, double ly, const int bc[4]);
+
+double orc_cut_box_area(double xmin, double xmax, double ymin, double ymax,
+                        double nx, double ny, double alpha);
+double orc_compute_alpha(double xmin, double xmax, double ymin, double ymax,
+                         double nx, double ny, double c_target);
+
+/* method: 0 = PLIC (Youngs), 1 = ELVIRA */
+void orc_reconstruct(const orc_mesh *m, int method, const double *C,
+                     double *snx, double *sny, double *sal);
+void orc_reconstruct_interface(const orc_mesh *m, int method, const double *C,
+                               double *snx, double *sny, double *sal);
+void orc_exchange_halos_scalar(const orc_mesh *m, double *f);
+void orc_apply_vof_bc(const orc_mesh *m, double *C);
+
+/* One advect_vof(WEYMOUTH_YUE) call: both passes, step counter, halo, BC.
+ * If two_phase, also update_physical_properties (rho, rho_u, rho_v, mu, mu_node;
+ * any of those may be NULL except that all are needed when two_phase != 0). */
+void orc_wy_step(const orc_mesh *m, int method, double *C, const double *u, const double *v,
+                 double dt, int *step_count);
+void orc_update_properties(const orc_mesh *m, double *C, double rho1, double rho2,
+                           double mu1, double mu2, double *rho, double *rho_u,
+                           double *rho_v, double *mu, double *mu_node);
+
+/* Poisson (SP projection block). Coefficient arrays are (nx,ny). */
+void orc_poisson_assemble(const orc_mesh *m, const double *u, const double *v,
+                          const double *rho_u, const double *rho_v, double dt,
+                          double *aW, double *aE, double *aS, double *aN, double *aP,
+                          double *b);
+void orc_poisson_matvec(const orc_mesh *m, const double *aP, const double *aW,
+                        const double *aE, const double *aS, const double *aN,
+                        const double *x, double *y);
+void orc_poisson_sgs(const orc_mesh *m, const double *aP, const double *aW,
+                     const double *aE, const double *aS, const double *aN,
+                     const double *r, double *z);
+/* precond: 0 none, 1 Jacobi, 2 SGS ("SSOR" omega=1) */
+void orc_poisson_precond(const orc_mesh *m, int precond, const double *aP, const double *aW,
+                         const double *aE, const double *aS, const double *aN,
+                         const double *r, double *z);
+/* SP PCG driver (src/module_simulation.f90:2535-2585) on x (in/out, (nx,ny)).
+ * Returns iterations; *relres = final ||r||/||b||.  No mean removal here. */
+int orc_sp_pcg(const orc_mesh *m, int precond, const double *aP, const double *aW,
+               const double *aE, const double *aS, const double *aN,
+               const double *b, double *x, double tol, int maxit, double *relres);
+/* BiCGSTAB(2) of src/module_linear_solvers.f90:361-579 on the Poisson operator. */
+int orc_bicgstab2(const orc_mesh *m, int precond, const double *aP, const double *aW,
+                  const double *aE, const double *aS, const double *aN,
+                  const double *b, double *x, double tol, int maxit, double *relres);
+/* Mean removal (src/module_simulation.f90:2578-2585) into p's interior, then
+ * sync_cell_scalar of the single-rank case (periodic wrap). */
+void orc_sp_finish_pressure(const orc_mesh *m, const double *x, double *p);
+void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
+                    const double *rho_v, double dt, double *u, double *v);
+
+/* Serial column-major sum of x*y (the reference's sum(x*y)). */
+double orc_dot(const double *x, const double *y, long n);
+
+/* Timed CPU baseline: `iters` Jacobi-PCG iterations on the face-form
+ * operator with OpenMP over `threads` threads (FP64). Returns seconds. */
+double orc_bench_jacobi_pcg(const orc_mesh *m, const double *aP, const double *aW,
+                            const double *aE, const double *aS, const double *aN,
+                            const double *b, int iters, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+"""ctypes front-end of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module, as the checker (or the timed CPU baseline).  The product package never
+imports it.
+
+Array conventions: a halo field of the reference, Fortran ``F(0:nx+1, 0:ny+1)``
+column-major with ``i`` fastest, is a C-contiguous numpy array of shape
+``(ny+2, nx+2)`` indexed ``F[j, i]``; a Poisson work array ``A(1:nx, 1:ny)`` is
+shape ``(ny, nx)``.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
+PLIC, ELVIRA = 0, 1
+
+
+class Mesh(ct.Structure):
+    _fields_ = [("nx", ct.c_int), ("ny", ct.c_int), ("lx", ct.c_double), ("ly", ct.c_double),
+                ("dx", ct.c_double), ("dy", ct.c_double), ("bc", ct.c_int * 4)]
+
+
+_lib = None
+_D = ct.POINTER(ct.c_double)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ct.CDLL(_LIB)
+        M = ct.POINTER(Mesh)
+        d, i, ip = ct.c_double, ct.c_int, ct.POINTER(ct.c_int)
+        sig = {
+            "orc_mesh_init": (None, [M, i, i, d, d, ct.POINTER(ct.c_int)]),
+            "orc_cut_box_area": (d, [d] * 7),
+            "orc_compute_alpha": (d, [d] * 7),
+            "orc_reconstruct": (None, [M, i, _D, _D, _D, _D]),
+            "orc_reconstruct_interface": (None, [M, i, _D, _D, _D, _D]),
+            "orc_exchange_halos_scalar": (None, [M, _D]),
+            "orc_apply_vof_bc": (None, [M, _D]),
+            "orc_wy_step": (None, [M, i, _D, _D, _D, d, ip]),
+            "orc_update_properties": (None, [M, _D, d, d, d, d, _D, _D, _D, _D, _D]),
+            "orc_poisson_assemble": (None, [M, _D, _D, _D, _D, d] + [_D] * 6),
+            "orc_poisson_matvec": (None, [M] + [_D] * 7),
+            "orc_poisson_sgs": (None, [M] + [_D] * 7),
+            "orc_poisson_precond": (None, [M, i] + [_D] * 7),
+            "orc_sp_pcg": (i, [M, i] + [_D] * 7 + [d, i, _D]),
+            "orc_bicgstab2": (i, [M, i] + [_D] * 7 + [d, i, _D]),
+            "orc_sp_finish_pressure": (None, [M, _D, _D]),
+            "orc_sp_correct": (None, [M, _D, _D, _D, d, _D, _D]),
+            "orc_dot": (d, [_D, _D, ct.c_long]),
+            "orc_bench_jacobi_pcg": (d, [M] + [_D] * 6 + [i, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def mesh(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0, bc=(BC_PERIODIC,) * 4) -> Mesh:
+    m = Mesh()
+    arr = (ct.c_int * 4)(*bc)
+    lib().orc_mesh_init(ct.byref(m), nx, ny, lx, ly, arr)
+    return m
+
+
+def cut_box_area(xmin, xmax, ymin, ymax, nx, ny, alpha) -> float:
+    return lib().orc_cut_box_area(xmin, xmax, ymin, ymax, nx, ny, alpha)
+
+
+def compute_alpha(xmin, xmax, ymin, ymax, nx, ny, c) -> float:
+    return lib().orc_compute_alpha(xmin, xmax, ymin, ymax, nx, ny, c)
+
+
+def reconstruct_interface(m: Mesh, method: int, C: np.ndarray):
+    shp = (m.ny + 2, m.nx + 2)
+    C = np.ascontiguousarray(C, dtype=np.float64)
+    sx, sy, sa = (np.zeros(shp) for _ in range(3))
+    lib().orc_reconstruct_interface(ct.byref(m), method, _p(C), _p(sx), _p(sy), _p(sa))
+    return sx, sy, sa
+
+
+def wy_step(m: Mesh, method: int, C: np.ndarray, u: np.ndarray, v: np.ndarray, dt: float,
+            step_count: int):
+    """advect_vof(WEYMOUTH_YUE) in place on a copy; returns (C_new, step_count_new)."""
+    C = np.array(C, dtype=np.float64, order="C", copy=True)
+    sc = ct.c_int(step_count)
+    lib().orc_wy_step(ct.byref(m), method, _p(C), _p(np.ascontiguousarray(u)),
+                      _p(np.ascontiguousarray(v)), dt, ct.byref(sc))
+    return C, sc.value
+
+
+def update_properties(m: Mesh, C: np.ndarray, rho1, rho2, mu1, mu2, init=1.0):
+    shp = (m.ny + 2, m.nx + 2)
+    out = [np.full(shp, init) for _ in range(5)]
+    lib().orc_update_properties(ct.byref(m), _p(C), rho1, rho2, mu1, mu2, *[_p(a) for a in out])
+    return out  # rho, rho_u, rho_v, mu, mu_node
+
+
+def poisson_assemble(m: Mesh, u, v, rho_u, rho_v, dt):
+    shp = (m.ny, m.nx)
+    aW, aE, aS, aN, aP, b = (np.zeros(shp) for _ in range(6))
+    lib().orc_poisson_assemble(ct.byref(m), _p(u), _p(v), _p(rho_u), _p(rho_v), dt,
+                               _p(aW), _p(aE), _p(aS), _p(aN), _p(aP), _p(b))
+    return dict(aW=aW, aE=aE, aS=aS, aN=aN, aP=aP, b=b)
+
+
+def _co(co):
+    return [_p(co[k]) for k in ("aP", "aW", "aE", "aS", "aN")]
+
+
+def poisson_matvec(m: Mesh, co, x):
+    y = np.zeros_like(co["aP"])
+    lib().orc_poisson_matvec(ct.byref(m), *_co(co), _p(np.ascontiguousarray(x)), _p(y))
+    return y
+
+
+def poisson_sgs(m: Mesh, co, r):
+    z = np.zeros_like(co["aP"])
+    lib().orc_poisson_sgs(ct.byref(m), *_co(co), _p(np.ascontiguousarray(r)), _p(z))
+    return z
+
+
+def poisson_precond(m: Mesh, precond: int, co, r):
+    z = np.zeros_like(co["aP"])
+    lib().orc_poisson_precond(ct.byref(m), precond, *_co(co), _p(np.ascontiguousarray(r)), _p(z))
+    return z
+
+
+def sp_pcg(m: Mesh, precond: int, co, b, x0, tol=1e-7, maxit=1000):
+    x = np.array(x0, dtype=np.float64, copy=True)
+    rr = ct.c_double(0.0)
+    it = lib().orc_sp_pcg(ct.byref(m), precond, *_co(co), _p(np.ascontiguousarray(b)), _p(x),
+                          tol, maxit, ct.byref(rr))
+    return x, it, rr.value
+
+
+def bicgstab2(m: Mesh, precond: int, co, b, x0, tol=1e-10, maxit=1000):
+    x = np.array(x0, dtype=np.float64, copy=True)
+    rr = ct.c_double(0.0)
+    it = lib().orc_bicgstab2(ct.byref(m), precond, *_co(co), _p(np.ascontiguousarray(b)), _p(x),
+                             tol, maxit, ct.byref(rr))
+    return x, it, rr.value
+
+
+def sp_finish_pressure(m: Mesh, x, p):
+    p = np.array(p, dtype=np.float64, copy=True)
+    lib().orc_sp_finish_pressure(ct.byref(m), _p(np.ascontiguousarray(x)), _p(p))
+    return p
+
+
+def sp_correct(m: Mesh, p, rho_u, rho_v, dt, u, v):
+    u = np.array(u, dtype=np.float64, copy=True)
+    v = np.array(v, dtype=np.float64, copy=True)
+    lib().orc_sp_correct(ct.byref(m), _p(p), _p(rho_u), _p(rho_v), dt, _p(u), _p(v))
+    return u, v
+
+
+def bench_jacobi_pcg(m: Mesh, co, b, iters: int, threads: int) -> float:
+    return lib().orc_bench_jacobi_pcg(ct.byref(m), *_co(co), _p(b), iters, threads)

@@ -1,0 +1,154 @@
+! Fixture harness over the compiled reference Fortran (test infrastructure only).
+!
+! This program is original code of this repository.  It is compiled together
+! with the reference's own module sources (read in place under the reference
+! tree by oracle/Makefile, output only into oracle/_ref/) and calls the
+! reference's PUBLIC procedures on inputs read from a binary file, writing the
+! reference's outputs bit-for-bit (unformatted stream, little-endian FP64):
+!
+!   geom    cut_box_area / compute_alpha
+!           (src/module_interface_reconstruction.f90:33-147)
+!   recon   reconstruct_interface, PLIC or ELVIRA, incl. ghost-normal fix-up
+!           (src/module_interface_reconstruction.f90:580-683)
+!   advect  advect_vof with WEYMOUTH_YUE, n steps, state after every step
+!           (src/module_interface_vof.f90:792-825, :1162-1377), followed by
+!           update_physical_properties (:479-616)
+!   sp      sim%solve_standard_projection on a small SP case: captures u* by
+!           running the same predictor calls first, then the full SP call
+!           (src/module_simulation.f90:2424-2666)
+!
+! Usage: covof_ref_harness <mode> <input.bin> <output.bin>
+program covof_ref_harness
+  use module_precision, only: dp
+  use module_constants
+  use module_config, only: config_t
+  use module_mesh, only: mesh_2d_t, init_mesh
+  use module_fields, only: flow_fields_t
+  use module_interface_reconstruction, only: interface_segment_t, cut_box_area, &
+       compute_alpha, reconstruct_interface
+  use module_interface_vof, only: advect_vof, update_physical_properties
+  implicit none
+  include 'mpif.h'
+
+  character(len=256) :: mode, fin, fout
+  integer :: ierr, uin, uout
+
+  call MPI_Init(ierr)
+  call get_command_argument(1, mode)
+  call get_command_argument(2, fin)
+  call get_command_argument(3, fout)
+  open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+  open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
+
+  select case (trim(mode))
+  case ('geom')
+     call run_geom()
+  case ('recon')
+     call run_recon()
+  case ('advect')
+     call run_advect()
+  case default
+     write(*, '(A,A)') 'unknown mode ', trim(mode)
+     call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
+  end select
+
+  close(uin)
+  close(uout)
+  call MPI_Finalize(ierr)
+
+contains
+
+  subroutine run_geom()
+    integer :: n, k
+    real(dp), allocatable :: xmin(:), xmax(:), ymin(:), ymax(:), gx(:), gy(:), ga(:), gc(:)
+    real(dp), allocatable :: area(:), alpha(:)
+    read(uin) n
+    allocate(xmin(n), xmax(n), ymin(n), ymax(n), gx(n), gy(n), ga(n), gc(n), area(n), alpha(n))
+    read(uin) xmin, xmax, ymin, ymax, gx, gy, ga, gc
+    do k = 1, n
+       area(k) = cut_box_area(xmin(k), xmax(k), ymin(k), ymax(k), gx(k), gy(k), ga(k))
+       alpha(k) = compute_alpha(xmin(k), xmax(k), ymin(k), ymax(k), gx(k), gy(k), gc(k))
+    end do
+    write(uout) area, alpha
+  end subroutine run_geom
+
+  subroutine setup(mesh, cfg, nx, ny, lx, ly, bc)
+    type(mesh_2d_t), intent(out) :: mesh
+    type(config_t), intent(inout) :: cfg
+    integer, intent(in) :: nx, ny, bc(4)
+    real(dp), intent(in) :: lx, ly
+    cfg%nx = nx
+    cfg%ny = ny
+    cfg%lx = lx
+    cfg%ly = ly
+    cfg%bc_left%btype = bc(1)
+    cfg%bc_right%btype = bc(2)
+    cfg%bc_bottom%btype = bc(3)
+    cfg%bc_top%btype = bc(4)
+    call init_mesh(mesh, MPI_COMM_WORLD, nx, ny, lx, ly, &
+                   bc(1) == BC_PERIODIC .or. bc(2) == BC_PERIODIC, &
+                   bc(3) == BC_PERIODIC .or. bc(4) == BC_PERIODIC)
+  end subroutine setup
+
+  function method_name(m) result(s)
+    integer, intent(in) :: m
+    character(len=32) :: s
+    if (m == 1) then
+       s = 'ELVIRA'
+    else
+       s = 'PLIC'
+    end if
+  end function method_name
+
+  subroutine run_recon()
+    integer :: nx, ny, bc(4), method
+    real(dp) :: lx, ly
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    real(dp), allocatable :: C(:, :)
+    type(interface_segment_t), allocatable :: seg(:, :)
+    read(uin) nx, ny, bc, method
+    read(uin) lx, ly
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    allocate(C(0:nx+1, 0:ny+1), seg(0:nx+1, 0:ny+1))
+    read(uin) C
+    call reconstruct_interface(method_name(method), mesh, C, seg)
+    write(uout) seg%nx, seg%ny, seg%alpha
+  end subroutine run_recon
+
+  subroutine run_advect()
+    integer :: nx, ny, bc(4), method, two_phase, nsteps, s
+    real(dp) :: lx, ly, dt, rho1, rho2, mu1, mu2
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    read(uin) nx, ny, bc, method, two_phase, nsteps
+    read(uin) lx, ly, dt, rho1, rho2, mu1, mu2
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%advection_method = 'WEYMOUTH_YUE'
+    cfg%reconstruction_method = method_name(method)
+    cfg%enable_two_phase = (two_phase /= 0)
+    cfg%rho1 = rho1
+    cfg%rho2 = rho2
+    cfg%mu1 = mu1
+    cfg%mu2 = mu2
+    cfg%use_markers = .false.
+    cfg%enable_svof_smoothing = .false.
+    cfg%enable_diffusion_smoothing = .false.
+    cfg%enable_surface_tension = .false.
+    cfg%compressible = .false.
+    call fields%allocate(mesh)
+    fields%rho = 1.0_dp
+    fields%rho_u = 1.0_dp
+    fields%rho_v = 1.0_dp
+    fields%mu = 1.0_dp
+    fields%mu_node = 1.0_dp
+    read(uin) fields%C, fields%u, fields%v
+    do s = 1, nsteps
+       call advect_vof(fields, mesh, cfg, dt)
+       write(uout) fields%C
+    end do
+    write(uout) fields%rho, fields%rho_u, fields%rho_v, fields%mu, fields%mu_node
+  end subroutine run_advect
+
+end program covof_ref_harness
