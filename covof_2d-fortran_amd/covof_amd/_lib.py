@@ -1,0 +1,88 @@
+"""ctypes loader of libcovof_hip.so (the HIP kernels + C-ABI of include/covof_hip.h).
+
+The library is built in-tree (``covof_2d-fortran_amd/libcovof_hip.so``) by
+``__graft_entry__.build()`` / ``make -C covof_2d-fortran_amd/csrc``.  There is
+no fallback: if the library or a GPU is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libcovof_hip.so")
+
+# status codes (include/covof_hip.h)
+OK, NOT_CONVERGED, BREAKDOWN, NONFINITE, BAD_ARGUMENT, HIP_ERROR = range(6)
+STATUS_NAMES = {OK: "ok", NOT_CONVERGED: "not converged", BREAKDOWN: "breakdown",
+                NONFINITE: "non-finite", BAD_ARGUMENT: "bad argument", HIP_ERROR: "HIP error"}
+
+# reference selector values (src/module_constants.f90)
+BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
+LINEAR_BICGSTAB2, LINEAR_CG = 1, 3
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+# every symbol the header declares: name -> (restype, argtypes)
+_P = ct.c_void_p
+_I = ct.c_int
+_D = ct.c_double
+_S = ct.c_char_p
+_IP = ct.POINTER(ct.c_int)
+_DP = ct.POINTER(ct.c_double)
+SIGNATURES = {
+    "covof_ctx_create": (_I, [_I, _I, _D, _D, _IP, ct.POINTER(_P)]),
+    "covof_ctx_destroy": (_I, [_P]),
+    "covof_ctx_set_stream": (_I, [_P, _P]),
+    "covof_ctx_synchronize": (_I, [_P]),
+    "covof_last_error": (_S, []),
+    "covof_geom_batch_d": (_I, [_P, _I] + [_P] * 10),
+    "covof_reconstruct_interface_d": (_I, [_P, _S, _P, _P, _P, _P]),
+    "covof_vof_advect_d": (_I, [_P, _P, _P, _P, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_P] * 5),
+    "covof_vof_advect": (_I, [_P, _DP, _DP, _DP, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_DP] * 5),
+    "covof_update_properties_d": (_I, [_P, _P, _D, _D, _D, _D] + [_P] * 5),
+    "covof_poisson_assemble_d": (_I, [_P, _P, _P, _P, _P, _D, _P]),
+    "covof_poisson_matvec_d": (_I, [_P, _P, _P]),
+    "covof_poisson_precond_d": (_I, [_P, _I, _P, _P]),
+    "covof_poisson_solve_d": (_I, [_P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_sp_poisson_d": (_I, [_P, _P, _P, _P, _P, _P, _D, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_sp_poisson": (_I, [_P, _DP, _DP, _DP, _DP, _DP, _D, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_sp_correct_d": (_I, [_P, _P, _P, _P, _D, _P, _P]),
+    "covof_sp_correct": (_I, [_P, _DP, _DP, _DP, _D, _DP, _DP]),
+    "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
+}
+
+_lib = None
+
+
+class CovofError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str = ""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}" + (f" ({msg})" if msg else ""))
+
+
+def load(path: str = LIB_PATH) -> ct.CDLL:
+    """Load the shared library and bind every exported symbol (no GPU needed)."""
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with __graft_entry__.build() "
+                          "or `make -C covof_2d-fortran_amd/csrc`")
+    L = ct.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def lib() -> ct.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def check(status: int, where: str, allow=(OK,)) -> int:
+    if status in allow:
+        return status
+    msg = lib().covof_last_error()
+    raise CovofError(status, where, msg.decode() if msg else "")

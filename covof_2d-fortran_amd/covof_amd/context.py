@@ -1,0 +1,233 @@
+"""Host-side mirror of the reference's hot-path interface over libcovof_hip."""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as L
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+@dataclass
+class SolveResult:
+    """Outcome of a Krylov solve: iterations (inner ones summed in mixed
+    precision), final relative residual, status code (include/covof_hip.h)."""
+    iters: int
+    relres: float
+    status: int
+
+    @property
+    def converged(self) -> bool:
+        return self.status == L.OK
+
+
+class Context:
+    """One mesh + its device buffers (``covof_ctx_create``).
+
+    ``bc`` = (left, right, bottom, top) reference boundary types
+    (src/module_constants.f90:25-31).  Calls run on torch's current CUDA
+    stream of the calling thread.
+    """
+
+    def __init__(self, nx: int, ny: int, lx: float = 1.0, ly: float = 1.0,
+                 bc=(L.BC_PERIODIC,) * 4, device: int | str | torch.device = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("covof_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+        self.device = torch.device("cuda", torch.device(device).index or 0) if not isinstance(device, int) \
+            else torch.device("cuda", device)
+        self.nx, self.ny, self.lx, self.ly = int(nx), int(ny), float(lx), float(ly)
+        self.bc = tuple(int(b) for b in bc)
+        self._lib = L.lib()
+        self._h = ct.c_void_p()
+        arr = (ct.c_int * 4)(*self.bc)
+        with torch.cuda.device(self.device):
+            L.check(self._lib.covof_ctx_create(self.nx, self.ny, self.lx, self.ly, arr, ct.byref(self._h)),
+                    "covof_ctx_create")
+        self.step_count = 0  # the reference's vof_step_count (src/module_interface_vof.f90:27)
+
+    # ------------------------------------------------------------ plumbing
+    def close(self):
+        if self._h:
+            self._lib.covof_ctx_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self._lib.covof_ctx_set_stream(self._h, ct.c_void_p(s)), "covof_ctx_set_stream")
+
+    def _field(self, t: torch.Tensor, name: str, shape=None):
+        shape = shape or (self.ny + 2, self.nx + 2)
+        if not (isinstance(t, torch.Tensor) and t.device.type == "cuda" and t.dtype == torch.float64
+                and tuple(t.shape) == tuple(shape) and t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of shape {shape}")
+        return t
+
+    def _work(self, t, name):
+        return self._field(t, name, (self.ny, self.nx))
+
+    def new_field(self, fill=0.0):
+        return torch.full((self.ny + 2, self.nx + 2), fill, dtype=torch.float64, device=self.device)
+
+    def new_work(self, fill=0.0):
+        return torch.full((self.ny, self.nx), fill, dtype=torch.float64, device=self.device)
+
+    def synchronize(self):
+        L.check(self._lib.covof_ctx_synchronize(self._h), "covof_ctx_synchronize")
+
+    # ------------------------------------------------------------ geometry
+    def geometry_batch(self, xmin, xmax, ymin, ymax, nx, ny, alpha, c):
+        """cut_box_area and compute_alpha over 1-D float64 CUDA tensors
+        (src/module_interface_reconstruction.f90:33-147)."""
+        ins = [torch.as_tensor(a, dtype=torch.float64, device=self.device).contiguous()
+               for a in (xmin, xmax, ymin, ymax, nx, ny, alpha, c)]
+        n = ins[0].numel()
+        area = torch.empty(n, dtype=torch.float64, device=self.device)
+        aout = torch.empty_like(area)
+        self._stream()
+        L.check(self._lib.covof_geom_batch_d(self._h, n, *[_ptr(a) for a in ins], _ptr(area), _ptr(aout)),
+                "covof_geom_batch_d")
+        return area, aout
+
+    def reconstruct_interface(self, method: str, C: torch.Tensor):
+        """reconstruct_interface(method_name, mesh, C, segments)
+        (src/module_interface_reconstruction.f90:580-683) -> (nx, ny, alpha) fields."""
+        self._field(C, "C")
+        out = [self.new_field() for _ in range(3)]
+        self._stream()
+        L.check(self._lib.covof_reconstruct_interface_d(self._h, method.encode(), _ptr(C),
+                                                        *[_ptr(o) for o in out]),
+                "covof_reconstruct_interface_d")
+        return tuple(out)
+
+    # ---------------------------------------------------------------- VOF
+    def advect_vof(self, C, u, v, dt, advection_method="WEYMOUTH_YUE", reconstruction_method="PLIC",
+                   nsteps=1, two_phase=False, rho1=1.0, rho2=1.0, mu1=1.0e-3, mu2=1.0e-3, props=None,
+                   step_count=None):
+        """advect_vof(fields, mesh, cfg, dt) (src/module_interface_vof.f90:792-825), ``nsteps``
+        times, C in place.  ``props`` = dict of rho, rho_u, rho_v, mu, mu_node fields
+        (written when ``two_phase``).  Uses/advances ``self.step_count`` unless given."""
+        for t, n in ((C, "C"), (u, "u"), (v, "v")):
+            self._field(t, n)
+        pp = [None] * 5
+        if two_phase:
+            if props is None:
+                props = {k: self.new_field(1.0) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")}
+            pp = [self._field(props[k], k) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")]
+        sc = ct.c_int(self.step_count if step_count is None else int(step_count))
+        self._stream()
+        L.check(self._lib.covof_vof_advect_d(self._h, _ptr(C), _ptr(u), _ptr(v), float(dt),
+                                             advection_method.encode(), reconstruction_method.encode(),
+                                             ct.byref(sc), int(nsteps), int(bool(two_phase)), float(rho1),
+                                             float(rho2), float(mu1), float(mu2), *[_ptr(p) for p in pp]),
+                "covof_vof_advect_d")
+        if step_count is None:
+            self.step_count = sc.value
+        return props if two_phase else None
+
+    def update_physical_properties(self, C, rho1, rho2, mu1, mu2, props=None):
+        """update_physical_properties, two-phase incompressible branch
+        (src/module_interface_vof.f90:479-616); fixes C's physical corners in place."""
+        self._field(C, "C")
+        if props is None:
+            props = {k: self.new_field(1.0) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")}
+        pp = [self._field(props[k], k) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")]
+        self._stream()
+        L.check(self._lib.covof_update_properties_d(self._h, _ptr(C), float(rho1), float(rho2), float(mu1),
+                                                    float(mu2), *[_ptr(p) for p in pp]),
+                "covof_update_properties_d")
+        return props
+
+    # ------------------------------------------------------------ Poisson
+    def poisson_assemble(self, u, v, rho_u, rho_v, dt, b=None):
+        """SP Poisson assembly (src/module_simulation.f90:2477-2533); returns b = -div(u*)."""
+        for t, n in ((u, "u"), (v, "v"), (rho_u, "rho_u"), (rho_v, "rho_v")):
+            self._field(t, n)
+        b = self.new_work() if b is None else self._work(b, "b")
+        self._stream()
+        L.check(self._lib.covof_poisson_assemble_d(self._h, _ptr(u), _ptr(v), _ptr(rho_u), _ptr(rho_v),
+                                                   float(dt), _ptr(b)), "covof_poisson_assemble_d")
+        return b
+
+    def poisson_matvec(self, x, y=None):
+        """poisson_matvec (src/module_simulation.f90:2668-2707)."""
+        self._work(x, "x")
+        y = self.new_work() if y is None else self._work(y, "y")
+        self._stream()
+        L.check(self._lib.covof_poisson_matvec_d(self._h, _ptr(x), _ptr(y)), "covof_poisson_matvec_d")
+        return y
+
+    def poisson_precond(self, precond, r, z=None):
+        """NONE / JACOBI / SSOR (= poisson_ssor, src/module_simulation.f90:2709-2758)."""
+        self._work(r, "r")
+        z = self.new_work() if z is None else self._work(z, "z")
+        self._stream()
+        L.check(self._lib.covof_poisson_precond_d(self._h, int(precond), _ptr(r), _ptr(z)),
+                "covof_poisson_precond_d")
+        return z
+
+    def poisson_solve(self, b, x, linear_solver=L.LINEAR_CG, precond=L.PRECOND_JACOBI, tol=1e-10,
+                      max_iter=100000, precision=L.PRECISION_FP64) -> SolveResult:
+        """Krylov solve of A x = b on the assembled operator (x in/out)."""
+        self._work(b, "b")
+        self._work(x, "x")
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_poisson_solve_d(self._h, _ptr(b), _ptr(x), int(linear_solver), int(precond),
+                                             float(tol), int(max_iter), int(precision), ct.byref(it),
+                                             ct.byref(rr))
+        L.check(st, "covof_poisson_solve_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return SolveResult(it.value, rr.value, st)
+
+    def solve_standard_projection_poisson(self, u, v, rho_u, rho_v, p, dt, linear_solver=L.LINEAR_CG,
+                                          precond=L.PRECOND_SSOR, tol=1e-7, max_iter=1000,
+                                          precision=L.PRECISION_FP64) -> SolveResult:
+        """The Poisson block of sim_solve_standard_projection
+        (src/module_simulation.f90:2477-2588), p in/out; defaults are the
+        reference's hard-coded CG + SSOR, tol 1e-7, 1000 iterations."""
+        for t, n in ((u, "u"), (v, "v"), (rho_u, "rho_u"), (rho_v, "rho_v"), (p, "p")):
+            self._field(t, n)
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_sp_poisson_d(self._h, _ptr(u), _ptr(v), _ptr(rho_u), _ptr(rho_v), _ptr(p),
+                                          float(dt), int(linear_solver), int(precond), float(tol),
+                                          int(max_iter), int(precision), ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_sp_poisson_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return SolveResult(it.value, rr.value, st)
+
+    def correct_velocity(self, p, rho_u, rho_v, dt, u, v):
+        """Projection u -= (dt/rho_u) dp/dx, v -= (dt/rho_v) dp/dy
+        (src/module_simulation.f90:2590-2654), u and v in place."""
+        for t, n in ((p, "p"), (rho_u, "rho_u"), (rho_v, "rho_v"), (u, "u"), (v, "v")):
+            self._field(t, n)
+        self._stream()
+        L.check(self._lib.covof_sp_correct_d(self._h, _ptr(p), _ptr(rho_u), _ptr(rho_v), float(dt), _ptr(u),
+                                             _ptr(v)), "covof_sp_correct_d")
+
+    def poisson_adjoint(self, rho, dt, x, w, linear_solver=L.LINEAR_CG, precond=L.PRECOND_JACOBI,
+                        tol=1e-12, max_iter=200000, precision=L.PRECISION_FP64):
+        """dJ/db and dJ/d(log rho) for w = dJ/dx at the solution x of the
+        assembled operator (no reference counterpart)."""
+        self._field(rho, "rho")
+        self._work(x, "x")
+        self._work(w, "w")
+        g_b, g_lr = self.new_work(), self.new_work()
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_poisson_adjoint_d(self._h, _ptr(rho), float(dt), _ptr(x), _ptr(w), _ptr(g_b),
+                                               _ptr(g_lr), int(linear_solver), int(precond), float(tol),
+                                               int(max_iter), int(precision), ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_poisson_adjoint_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return g_b, g_lr, SolveResult(it.value, rr.value, st)

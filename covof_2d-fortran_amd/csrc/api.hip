@@ -1,0 +1,331 @@
+// extern "C" entry points of libcovof_hip (declared in include/covof_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "covof_internal.h"
+
+namespace covof {
+int vof_advect(Ctx &c, double *C, const double *u, const double *v, double dt, int method,
+               int *step_count, int nsteps, int two_phase, double rho1, double rho2, double mu1,
+               double mu2, double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node);
+int vof_properties(Ctx &c, const double *C, double rho1, double rho2, double mu1, double mu2,
+                   double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node);
+int vof_reconstruct_interface(Ctx &c, int method, const double *C, double *onx, double *ony,
+                              double *oal);
+int vof_geom_batch(Ctx &c, int n, const double *xmin, const double *xmax, const double *ymin,
+                   const double *ymax, const double *gnx, const double *gny, const double *alpha,
+                   const double *cc, double *area, double *aout);
+int vof_corner_fix(Ctx &c, double *C);
+int poisson_assemble(Ctx &c, const double *u, const double *v, const double *rho_u,
+                     const double *rho_v, double dt, double *b);
+int poisson_matvec(Ctx &c, const double *x, double *y);
+int poisson_precond(Ctx &c, int precond, const double *r, double *z);
+int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol,
+                  int maxit, int precision, int *iters, double *relres);
+int sp_poisson(Ctx &c, const double *u, const double *v, const double *rho_u, const double *rho_v,
+               double *p, double dt, int lin_solver, int precond, double tol, int maxit,
+               int precision, int *iters, double *relres);
+int sp_correct(Ctx &c, const double *p, const double *rho_u, const double *rho_v, double dt,
+               double *u, double *v);
+int poisson_adjoint(Ctx &c, const double *rho, double dt, const double *x, const double *w,
+                    double *dJ_db, double *dJ_dlogrho, int lin_solver, int precond, double tol,
+                    int maxit, int precision, int *iters, double *relres);
+}  // namespace covof
+
+using covof::Ctx;
+
+struct covof_ctx {
+  Ctx c;
+  // device mirrors for the host-array entry points (made on first use)
+  double *mirror[9] = {nullptr};
+};
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
+static int hip_status(int st) {
+  if (st == COVOF_HIP_ERROR) {
+    hipError_t e = hipGetLastError();
+    g_err = std::string("HIP error: ") + hipGetErrorString(e);
+  }
+  return st;
+}
+
+extern "C" const char *covof_last_error(void) { return g_err.c_str(); }
+
+// "WEYMOUTH_YUE" is the only geometric advection scheme of this library; the
+// reconstruction name follows reconstruct_interface's select case
+// (src/module_interface_reconstruction.f90:591-600): "ELVIRA" -> ELVIRA,
+// "SLIC"/"LVIRA" are not provided, anything else -> PLIC.
+static int recon_code(const char *name, int *code) {
+  if (!name) return COVOF_BAD_ARGUMENT;
+  std::string s(name);
+  while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+  if (s == "ELVIRA") *code = 1;
+  else if (s == "SLIC" || s == "LVIRA") return COVOF_BAD_ARGUMENT;
+  else *code = 0;
+  return COVOF_OK;
+}
+
+static bool is_wy(const char *name) {
+  if (!name) return false;
+  std::string s(name);
+  while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+  return s == "WEYMOUTH_YUE";
+}
+
+extern "C" int covof_ctx_create(int nx, int ny, double lx, double ly, const int *bc, covof_ctx **out) {
+  if (!out || !bc || nx < 2 || ny < 2 || !(lx > 0.0) || !(ly > 0.0))
+    return fail(COVOF_BAD_ARGUMENT, "covof_ctx_create: need nx, ny >= 2, lx, ly > 0, bc[4]");
+  if ((long)(nx + 2) * (ny + 2) > 0x7fffffffL)
+    return fail(COVOF_BAD_ARGUMENT, "covof_ctx_create: grid too large for 32-bit cell indices");
+  for (int k = 0; k < 4; ++k)
+    if (bc[k] < 1 || bc[k] > 4) return fail(COVOF_BAD_ARGUMENT, "covof_ctx_create: bc type must be 1..4");
+  covof_ctx *h = new (std::nothrow) covof_ctx();
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "out of host memory");
+  Ctx &c = h->c;
+  c.m.nx = nx;
+  c.m.ny = ny;
+  c.m.lx = lx;
+  c.m.ly = ly;
+  c.m.dx = lx / (double)nx;
+  c.m.dy = ly / (double)ny;
+  for (int k = 0; k < 4; ++k) c.m.bc[k] = bc[k];
+  c.m.perx = bc[0] == covof::BC_PERIODIC || bc[1] == covof::BC_PERIODIC;
+  c.m.pery = bc[2] == covof::BC_PERIODIC || bc[3] == covof::BC_PERIODIC;
+  if (hipStreamCreate(&c.stream) != hipSuccess) {
+    delete h;
+    return hip_status(COVOF_HIP_ERROR);
+  }
+  c.own_stream = 1;
+  int st = covof::vof_alloc(c);
+  if (!st) st = covof::poisson_alloc(c);
+  if (st) {
+    covof_ctx_destroy(h);
+    return hip_status(st);
+  }
+  *out = h;
+  return COVOF_OK;
+}
+
+extern "C" int covof_ctx_destroy(covof_ctx *h) {
+  if (!h) return COVOF_OK;
+  Ctx &c = h->c;
+  if (c.stream) hipStreamSynchronize(c.stream);
+  covof::vof_free(c);
+  covof::poisson_free(c);
+  for (double *p : h->mirror) hipFree(p);
+  if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
+  delete h;
+  return COVOF_OK;
+}
+
+extern "C" int covof_ctx_set_stream(covof_ctx *h, void *stream) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  Ctx &c = h->c;
+  if (stream) {
+    if (c.own_stream) {
+      hipStreamSynchronize(c.stream);
+      hipStreamDestroy(c.stream);
+      c.own_stream = 0;
+    }
+    c.stream = (hipStream_t)stream;
+  } else if (!c.own_stream) {
+    if (hipStreamCreate(&c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+    c.own_stream = 1;
+  }
+  return COVOF_OK;
+}
+
+extern "C" int covof_ctx_synchronize(covof_ctx *h) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+  return COVOF_OK;
+}
+
+extern "C" int covof_geom_batch_d(covof_ctx *h, int n, const double *xmin, const double *xmax,
+                                  const double *ymin, const double *ymax, const double *gnx,
+                                  const double *gny, const double *alpha, const double *c,
+                                  double *area, double *alpha_out) {
+  if (!h || n < 0) return fail(COVOF_BAD_ARGUMENT, "covof_geom_batch_d: bad argument");
+  return hip_status(covof::vof_geom_batch(h->c, n, xmin, xmax, ymin, ymax, gnx, gny, alpha, c, area, alpha_out));
+}
+
+extern "C" int covof_reconstruct_interface_d(covof_ctx *h, const char *method, const double *C,
+                                             double *seg_nx, double *seg_ny, double *seg_alpha) {
+  int code;
+  if (!h || !C || !seg_nx || !seg_ny || !seg_alpha) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (recon_code(method, &code)) return fail(COVOF_BAD_ARGUMENT, "reconstruction method must be PLIC or ELVIRA");
+  return hip_status(covof::vof_reconstruct_interface(h->c, code, C, seg_nx, seg_ny, seg_alpha));
+}
+
+extern "C" int covof_vof_advect_d(covof_ctx *h, double *C, const double *u, const double *v, double dt,
+                                  const char *advection_method, const char *reconstruction_method,
+                                  int *step_count, int nsteps, int two_phase, double rho1, double rho2,
+                                  double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+                                  double *mu, double *mu_node) {
+  int code;
+  if (!h || !C || !u || !v || !step_count || nsteps < 0) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (!is_wy(advection_method)) return fail(COVOF_BAD_ARGUMENT, "advection method must be WEYMOUTH_YUE");
+  if (recon_code(reconstruction_method, &code)) return fail(COVOF_BAD_ARGUMENT, "reconstruction method must be PLIC or ELVIRA");
+  if (two_phase && (!rho || !rho_u || !rho_v || !mu || !mu_node))
+    return fail(COVOF_BAD_ARGUMENT, "two_phase needs rho, rho_u, rho_v, mu, mu_node");
+  if (!(dt >= 0.0)) return fail(COVOF_BAD_ARGUMENT, "dt must be >= 0");
+  return hip_status(covof::vof_advect(h->c, C, u, v, dt, code, step_count, nsteps, two_phase, rho1, rho2,
+                                      mu1, mu2, rho, rho_u, rho_v, mu, mu_node));
+}
+
+// Host mirrors: slot k holds a halo-size device array.
+static double *mirror(covof_ctx *h, int k) {
+  if (!h->mirror[k]) {
+    const long nh = (long)(h->c.m.nx + 2) * (h->c.m.ny + 2);
+    if (hipMalloc(&h->mirror[k], nh * sizeof(double)) != hipSuccess) return nullptr;
+  }
+  return h->mirror[k];
+}
+
+static int h2d(covof_ctx *h, double *d, const double *s) {
+  const long nh = (long)(h->c.m.nx + 2) * (h->c.m.ny + 2);
+  return hipMemcpyAsync(d, s, nh * sizeof(double), hipMemcpyHostToDevice, h->c.stream) == hipSuccess ? 0 : COVOF_HIP_ERROR;
+}
+static int d2h(covof_ctx *h, double *d, const double *s) {
+  const long nh = (long)(h->c.m.nx + 2) * (h->c.m.ny + 2);
+  return hipMemcpyAsync(d, s, nh * sizeof(double), hipMemcpyDeviceToHost, h->c.stream) == hipSuccess ? 0 : COVOF_HIP_ERROR;
+}
+
+extern "C" int covof_vof_advect(covof_ctx *h, double *C, const double *u, const double *v, double dt,
+                                const char *advection_method, const char *reconstruction_method,
+                                int *step_count, int nsteps, int two_phase, double rho1, double rho2,
+                                double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+                                double *mu, double *mu_node) {
+  if (!h || !C || !u || !v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d[8];
+  for (int k = 0; k < 8; ++k)
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+  int st = h2d(h, d[0], C);
+  if (!st) st = h2d(h, d[1], u);
+  if (!st) st = h2d(h, d[2], v);
+  if (!st && two_phase) {
+    if (!rho || !rho_u || !rho_v || !mu || !mu_node) return fail(COVOF_BAD_ARGUMENT, "two_phase needs property arrays");
+    const double *hp[5] = {rho, rho_u, rho_v, mu, mu_node};
+    for (int k = 0; k < 5 && !st; ++k) st = h2d(h, d[3 + k], hp[k]);
+  }
+  if (st) return hip_status(st);
+  st = covof_vof_advect_d(h, d[0], d[1], d[2], dt, advection_method, reconstruction_method, step_count,
+                          nsteps, two_phase, rho1, rho2, mu1, mu2, d[3], d[4], d[5], d[6], d[7]);
+  if (st) return st;
+  st = d2h(h, C, d[0]);
+  if (!st && two_phase) {
+    double *hp[5] = {rho, rho_u, rho_v, mu, mu_node};
+    for (int k = 0; k < 5 && !st; ++k) st = d2h(h, hp[k], d[3 + k]);
+  }
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return hip_status(st);
+}
+
+extern "C" int covof_update_properties_d(covof_ctx *h, double *C, double rho1, double rho2, double mu1,
+                                         double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
+                                         double *mu_node) {
+  if (!h || !C || !rho || !rho_u || !rho_v || !mu || !mu_node) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  int st = covof::vof_corner_fix(h->c, C);
+  if (!st) st = covof::vof_properties(h->c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node);
+  return hip_status(st);
+}
+
+extern "C" int covof_poisson_assemble_d(covof_ctx *h, const double *u, const double *v, const double *rho_u,
+                                        const double *rho_v, double dt, double *b) {
+  if (!h || !u || !v || !rho_u || !rho_v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  return hip_status(covof::poisson_assemble(h->c, u, v, rho_u, rho_v, dt, b));
+}
+
+extern "C" int covof_poisson_matvec_d(covof_ctx *h, const double *x, double *y) {
+  if (!h || !x || !y) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::poisson_matvec(h->c, x, y);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "operator not assembled");
+  return hip_status(st);
+}
+
+extern "C" int covof_poisson_precond_d(covof_ctx *h, int precond, const double *r, double *z) {
+  if (!h || !r || !z) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::poisson_precond(h->c, precond, r, z);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "operator not assembled or unknown preconditioner");
+  return hip_status(st);
+}
+
+extern "C" int covof_poisson_solve_d(covof_ctx *h, const double *b, double *x, int lin_solver, int precond,
+                                     double tol, int maxit, int precision, int *iters, double *relres) {
+  if (!h || !b || !x) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::poisson_solve(h->c, b, x, lin_solver, precond, tol, maxit, precision, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver/preconditioner/precision combination or operator not assembled");
+  return hip_status(st);
+}
+
+extern "C" int covof_sp_poisson_d(covof_ctx *h, const double *u, const double *v, const double *rho_u,
+                                  const double *rho_v, double *p, double dt, int lin_solver, int precond,
+                                  double tol, int maxit, int precision, int *iters, double *relres) {
+  if (!h || !u || !v || !rho_u || !rho_v || !p) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::sp_poisson(h->c, u, v, rho_u, rho_v, p, dt, lin_solver, precond, tol, maxit,
+                                   precision, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver/preconditioner/precision combination");
+  return hip_status(st);
+}
+
+extern "C" int covof_sp_poisson(covof_ctx *h, const double *u, const double *v, const double *rho_u,
+                                const double *rho_v, double *p, double dt, int lin_solver, int precond,
+                                double tol, int maxit, int precision, int *iters, double *relres) {
+  if (!h || !u || !v || !rho_u || !rho_v || !p) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d[5];
+  for (int k = 0; k < 5; ++k)
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+  const double *hs[5] = {u, v, rho_u, rho_v, p};
+  int st = 0;
+  for (int k = 0; k < 5 && !st; ++k) st = h2d(h, d[k], hs[k]);
+  if (st) return hip_status(st);
+  const int sst = covof_sp_poisson_d(h, d[0], d[1], d[2], d[3], d[4], dt, lin_solver, precond, tol, maxit,
+                                     precision, iters, relres);
+  if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
+  st = d2h(h, p, d[4]);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return st ? hip_status(st) : sst;
+}
+
+extern "C" int covof_sp_correct_d(covof_ctx *h, const double *p, const double *rho_u, const double *rho_v,
+                                  double dt, double *u, double *v) {
+  if (!h || !p || !rho_u || !rho_v || !u || !v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  return hip_status(covof::sp_correct(h->c, p, rho_u, rho_v, dt, u, v));
+}
+
+extern "C" int covof_sp_correct(covof_ctx *h, const double *p, const double *rho_u, const double *rho_v,
+                                double dt, double *u, double *v) {
+  if (!h || !p || !rho_u || !rho_v || !u || !v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d[5];
+  for (int k = 0; k < 5; ++k)
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+  const double *hs[5] = {p, rho_u, rho_v, u, v};
+  int st = 0;
+  for (int k = 0; k < 5 && !st; ++k) st = h2d(h, d[k], hs[k]);
+  if (!st) st = covof::sp_correct(h->c, d[0], d[1], d[2], dt, d[3], d[4]);
+  if (!st) st = d2h(h, u, d[3]);
+  if (!st) st = d2h(h, v, d[4]);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return hip_status(st);
+}
+
+extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double dt, const double *x,
+                                       const double *w, double *dJ_db, double *dJ_dlogrho, int lin_solver,
+                                       int precond, double tol, int maxit, int precision, int *iters,
+                                       double *relres) {
+  if (!h || !rho || !x || !w || !dJ_db || !dJ_dlogrho) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::poisson_adjoint(h->c, rho, dt, x, w, dJ_db, dJ_dlogrho, lin_solver, precond, tol,
+                                        maxit, precision, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver combination");
+  return hip_status(st);
+}

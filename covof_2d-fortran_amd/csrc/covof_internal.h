@@ -1,0 +1,74 @@
+// Internal declarations shared by the HIP translation units of libcovof_hip.
+// Not part of the public C-ABI (that is include/covof_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "covof_hip.h"
+
+namespace covof {
+
+enum { BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_PERIODIC = 3, BC_FREESLIP = 4 };
+
+// Geometry of the single-rank mesh (src/module_mesh.f90:67-144).
+struct Mesh {
+  int nx, ny;
+  double lx, ly, dx, dy;
+  int bc[4];     // left, right, bottom, top (reference integer BC types)
+  int perx, pery;  // Cartesian periodicity (src/module_simulation.f90:115-117)
+};
+
+// Device state of one context: every buffer is allocated once at create.
+struct Vof {
+  double *Cb = nullptr;                 // second C buffer (ping-pong), halo layout
+  double *snx = nullptr, *sny = nullptr, *sal = nullptr;  // segments, halo layout, mixed cells only
+  uint8_t *cd = nullptr;                // dilation indicator [C^n >= 0.5], interior, halo layout
+  int *list[2] = {nullptr, nullptr};    // compacted mixed-cell lists (linear halo index)
+  int *count = nullptr;                 // [0],[1]: list sizes
+  double *ring = nullptr;               // ghost-normal scratch of the halo ring (2 x ring length)
+  double *geom = nullptr;               // device scratch for batched geometry tests
+};
+
+struct Poisson {
+  // face coefficients, FP64: kx((nx+1) x ny) kx(i,j) = a_W(i,j), kx(nx+1,j) = a_E(nx,j)
+  //                          ky(nx x (ny+1))  ky(i,j) = a_S(i,j), ky(i,ny+1) = a_N(i,ny)
+  double *kx = nullptr, *ky = nullptr, *diag = nullptr;
+  float *kx32 = nullptr, *ky32 = nullptr, *diag32 = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
+  double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *t4 = nullptr, *t5 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  double *fx = nullptr, *fy = nullptr;  // per-face adjoint sensitivities (kx / ky layout)
+  double *part = nullptr;   // per-block partial sums (nparts x 8)
+  double *scal = nullptr;   // device scalars (see poisson.hip)
+  unsigned *ticket = nullptr;  // last-block-done counter
+  double *hscal = nullptr;  // pinned host mirror of scal
+  int assembled = 0;
+  int nblocks = 0;          // reduction grid size
+};
+
+struct Ctx {
+  Mesh m;
+  hipStream_t stream = nullptr;
+  int own_stream = 0;
+  Vof vof;
+  Poisson ps;
+};
+
+inline __host__ __device__ long hidx(const Mesh &m, int i, int j) { return (long)j * (m.nx + 2) + i; }
+
+// ------------------------------------------------------------------ launch helpers
+#define COVOF_CHECK(expr)                                   \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) return COVOF_HIP_ERROR;           \
+  } while (0)
+
+#define COVOF_CHECK_LAUNCH() COVOF_CHECK(hipGetLastError())
+
+int vof_alloc(Ctx &c);
+void vof_free(Ctx &c);
+int poisson_alloc(Ctx &c);
+void poisson_free(Ctx &c);
+
+}  // namespace covof

@@ -1,0 +1,143 @@
+// Interface geometry shared by the VOF kernels (device + host compilable).
+//
+// Every routine evaluates the reference's IEEE operation sequence in the same
+// order, and the library is compiled with -ffp-contract=off, so that results
+// are bit-identical to the reference Fortran built without FMA contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace covof {
+
+#define CV_HD __host__ __device__ __forceinline__
+
+// x_face / y_face of src/module_mesh.f90:337-351 for one rank.
+CV_HD double face_coord(int i, double d) { return (double)(i - 1) * d; }
+
+// Euclidean norm sqrt(x^2 + y^2) rounded to nearest (the reference calls the
+// Fortran HYPOT intrinsic, which glibc rounds correctly on the inputs tested).
+// Method: exact-residual Newton correction.  With a >= b >= 0 scaled by a
+// power of two, h0 = sqrt(a*a + b*b); the residual (a^2 + b^2) - h0^2 is
+// formed from the exact products p + e of fma-based two-product splits
+// (a^2 - h0^2 is exact by Sterbenz's lemma since h0^2/2 <= a^2 <= h0^2), and
+// h = h0 + residual / (2 h0) rounds the corrected value once.
+CV_HD double hypot_cr(double x, double y) {
+  double a = fabs(x), b = fabs(y);
+  if (a < b) { const double t = a; a = b; b = t; }
+  if (!(a <= 1.7976931348623157e308)) return (a == a && b == b) ? a : (x + y);  // inf / nan
+  if (b == 0.0) return a;
+  // scale so that a is in [1, 2): exact power-of-two scaling
+  int e;
+  (void)frexp(a, &e);
+  const double sa = ldexp(a, 1 - e), sb = ldexp(b, 1 - e);
+  const double pa = sa * sa, ea = fma(sa, sa, -pa);
+  const double pb = sb * sb, eb = fma(sb, sb, -pb);
+  const double h0 = sqrt(pa + pb);
+  const double ph = h0 * h0, eh = fma(h0, h0, -ph);
+  const double res = ((pa - ph) + pb) + ((ea + eb) - eh);
+  const double h = h0 + res / (2.0 * h0);
+  return ldexp(h, e - 1);
+}
+
+// cut_box_area (src/module_interface_reconstruction.f90:33-101): area of the
+// part n.x + alpha <= 0 of the box, Sutherland-Hodgman over the corners in
+// counter-clockwise order from (xmin, ymin), shoelace sum in absolute
+// coordinates.  The shoelace terms are accumulated while the clipped vertices
+// are emitted (term(k-1,k) as vertex k appears, term(last, first) at the end):
+// the same terms added in the same order as the reference's polygon loop,
+// without a vertex array.
+struct Shoelace {
+  double s, fx, fy, px, py;
+  int n;
+  CV_HD void emit(double x, double y) {
+    if (n == 0) {
+      fx = x;
+      fy = y;
+      s = 0.0;
+    } else {
+      s = s + (px * y - x * py);
+    }
+    px = x;
+    py = y;
+    ++n;
+  }
+};
+
+// Edge k -> k+1 of the clip with corner distances d1, d2.
+CV_HD void clip_edge(Shoelace &P, double d1, double d2, double x1, double y1, double x2,
+                     double y2) {
+  if (d1 <= 0.0) {
+    if (d2 <= 0.0) {
+      P.emit(x2, y2);
+    } else {
+      const double t = -d1 / (d2 - d1);
+      P.emit(x1 + t * (x2 - x1), y1 + t * (y2 - y1));
+    }
+  } else if (d2 <= 0.0) {
+    const double t = -d1 / (d2 - d1);
+    P.emit(x1 + t * (x2 - x1), y1 + t * (y2 - y1));
+    P.emit(x2, y2);
+  }
+}
+
+// Area of the box cut by the line whose corner values without alpha are
+// w0..w3 (w_k = nx*x_k + ny*y_k, corners (xmin,ymin),(xmax,ymin),(xmax,ymax),
+// (xmin,ymax)).  d_k = w_k + alpha is the reference's (nx*x + ny*y) + alpha.
+CV_HD double cut_area_w(double xmin, double xmax, double ymin, double ymax, double w0, double w1,
+                        double w2, double w3, double alpha) {
+  const double d0 = w0 + alpha, d1 = w1 + alpha, d2 = w2 + alpha, d3 = w3 + alpha;
+  Shoelace P;
+  P.n = 0;
+  P.s = 0.0;
+  P.fx = P.fy = P.px = P.py = 0.0;
+  clip_edge(P, d0, d1, xmin, ymin, xmax, ymin);
+  clip_edge(P, d1, d2, xmax, ymin, xmax, ymax);
+  clip_edge(P, d2, d3, xmax, ymax, xmin, ymax);
+  clip_edge(P, d3, d0, xmin, ymax, xmin, ymin);
+  if (P.n < 3) return 0.0;
+  const double s = P.s + (P.px * P.fy - P.fx * P.py);
+  return 0.5 * fabs(s);
+}
+
+CV_HD double cut_box_area(double xmin, double xmax, double ymin, double ymax, double nx,
+                          double ny, double alpha) {
+  if ((xmax - xmin) <= 1.0e-14 || (ymax - ymin) <= 1.0e-14) return 0.0;
+  return cut_area_w(xmin, xmax, ymin, ymax, nx * xmin + ny * ymin, nx * xmax + ny * ymin,
+                    nx * xmax + ny * ymax, nx * xmin + ny * ymax, alpha);
+}
+
+// compute_alpha (src/module_interface_reconstruction.f90:105-147): 45-step
+// bisection of the line constant so that the kept area equals c * box area.
+// The corner products n.x_k do not depend on alpha and are formed once.
+CV_HD double compute_alpha(double xmin, double xmax, double ymin, double ymax, double nx,
+                           double ny, double c) {
+  const double total = (xmax - xmin) * (ymax - ymin);
+  if (c <= 1.0e-7) return 1.0e9;
+  if (c >= 1.0 - 1.0e-7) return -1.0e9;
+  const double cc = c < 1.0 ? c : 1.0;
+  const double target = (0.0 > cc ? 0.0 : cc) * total;
+  const double v1 = nx * xmin + ny * ymin;
+  const double v2 = nx * xmax + ny * ymin;
+  const double v3 = nx * xmax + ny * ymax;
+  const double v4 = nx * xmin + ny * ymax;
+  const double m34 = v3 < v4 ? v3 : v4, M34 = v3 > v4 ? v3 : v4;
+  const double m234 = v2 < m34 ? v2 : m34, M234 = v2 > M34 ? v2 : M34;
+  const double vmin = v1 < m234 ? v1 : m234, vmax = v1 > M234 ? v1 : M234;
+  double lo = -vmax - 1.0e-12;
+  double hi = -vmin + 1.0e-12;
+  const bool degenerate = (xmax - xmin) <= 1.0e-14 || (ymax - ymin) <= 1.0e-14;
+  for (int it = 0; it < 45; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    const double a = degenerate ? 0.0 : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
+    if (a < target)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Full / empty short-circuit thresholds of reconstruct_plic / _elvira (:298-310).
+CV_HD bool is_mixed(double c) { return !(c <= 1.0e-6) && !(c >= 1.0 - 1.0e-6); }
+
+}  // namespace covof

@@ -1,0 +1,244 @@
+// BiCGSTAB(2) on the Poisson operator (src/module_linear_solvers.f90:361-579).
+//
+// The reference's left-preconditioned BiCGSTAB(2) re-targeted from its
+// velocity operator to the pressure operator: apply_PA = precond(A x)
+// (:323-356).  Vector updates are elementwise kernels that evaluate the
+// reference's expressions in the same order; inner products are deterministic
+// grid reductions; the scalar recurrences (with every breakdown test of the
+// reference) run on the host, which reads the inner products back once per
+// dependent scalar.
+#include "covof_internal.h"
+
+#include <cmath>
+
+namespace covof {
+
+int poisson_matvec(Ctx &c, const double *x, double *y);
+int poisson_precond(Ctx &c, int precond, const double *r, double *z);
+
+namespace {
+
+constexpr int BLK = 256;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+struct Pairs {
+  const double *a[5];
+  const double *b[5];
+  int n;
+};
+
+// Up to five inner products; per-block partials, then the last block sums the
+// partials in block order and writes out[q].
+__global__ void __launch_bounds__(BLK) k_dots(long n, Pairs P, double *part, unsigned *ticket, double *out) {
+  __shared__ double sh[5][BLK / 64];
+  __shared__ bool last;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < n; t += (long)gridDim.x * BLK)
+    for (int q = 0; q < P.n; ++q) acc[q] += P.a[q][t] * P.b[q][t];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < P.n; ++q) {
+    const double w = wsum(acc[q]);
+    if (lane == 0) sh[q][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < P.n; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
+      part[(long)blockIdx.x * 5 + q] = s;
+    }
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int q = 0; q < P.n; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK) s += __builtin_nontemporal_load(&part[(long)b * 5 + q]);
+    s = wsum(s);
+    if (lane == 0) sh[q][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < P.n; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
+      out[q] = s;
+    }
+    *ticket = 0u;
+  }
+}
+
+// out = x + a*y
+__global__ void k_axpy(long n, double *out, const double *x, double a, const double *y) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    out[t] = x[t] + a * y[t];
+}
+// out = x - y
+__global__ void k_sub(long n, double *out, const double *x, const double *y) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    out[t] = x[t] - y[t];
+}
+// out = x + a*y + b*z   ((x + a y) + b z)
+__global__ void k_axpbz(long n, double *out, const double *x, double a, const double *y, double b,
+                        const double *z) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    out[t] = x[t] + a * y[t] + b * z[t];
+}
+// out = x - (a*y + b*z)
+__global__ void k_xmin(long n, double *out, const double *x, double a, const double *y, double b,
+                       const double *z) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    out[t] = x[t] - (a * y[t] + b * z[t]);
+}
+__global__ void k_zero(long n, double *x) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+    x[t] = 0.0;
+}
+
+}  // namespace
+
+struct Bicg {
+  Ctx &c;
+  long n;
+  int precond;
+  double *tmp;
+  dim3 g{1024}, b{256};
+  int err = 0;
+
+  void PA(const double *src, double *dst) {
+    if (!err) err = poisson_matvec(c, src, tmp);
+    if (!err) err = poisson_precond(c, precond, tmp, dst);
+  }
+  // inner products -> host (synchronises)
+  void dots(int k, const double *const *a, const double *const *bb, double *res) {
+    Pairs P;
+    P.n = k;
+    for (int q = 0; q < k; ++q) {
+      P.a[q] = a[q];
+      P.b[q] = bb[q];
+    }
+    hipLaunchKernelGGL(k_dots, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
+    if (hipGetLastError() != hipSuccess) { err = COVOF_HIP_ERROR; return; }
+    if (hipMemcpyAsync(c.ps.hscal, c.ps.scal + 40, k * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+        hipStreamSynchronize(c.stream) != hipSuccess) { err = COVOF_HIP_ERROR; return; }
+    for (int q = 0; q < k; ++q) res[q] = c.ps.hscal[q];
+  }
+  double dot(const double *x, const double *y) {
+    double r = 0.0;
+    const double *a[1] = {x}, *bb[1] = {y};
+    dots(1, a, bb, &r);
+    return r;
+  }
+  void axpy(double *out, const double *x, double a, const double *y) {
+    hipLaunchKernelGGL(k_axpy, g, b, 0, c.stream, n, out, x, a, y);
+  }
+};
+
+// src/module_linear_solvers.f90:361-579 (iteration counter += 2 per sweep;
+// stop on ||r||/||r0|| < tol or ||r|| < 1e-12 of the preconditioned residual).
+int solve_bicgstab2(Ctx &c, const double *bvec, double *x, int precond, double tol, int maxit,
+                    int *iters, double *relres) {
+  Poisson &P = c.ps;
+  const long n = (long)c.m.nx * c.m.ny;
+  Bicg K{c, n, precond, P.t5};
+  double *bh = P.b == bvec ? P.z : P.b;  // b_hat must not alias b
+  double *rt = P.r, *r0 = P.t4, *r1 = P.q, *r2 = P.t0, *u0 = P.t1, *u1 = P.t2, *u2 = P.t3;
+  const double *work[9] = {rt, r0, r1, r2, u0, u1, u2, bh, P.t5};
+  for (const double *w : work)
+    if (w == x || (w == bvec && w != bh)) return COVOF_BAD_ARGUMENT;
+  const dim3 g(1024), bl(256);
+  for (double *v : {u0, u1, u2, r1, r2}) hipLaunchKernelGGL(k_zero, g, bl, 0, c.stream, n, v);
+  K.err = poisson_precond(c, precond, bvec, bh);
+  K.PA(x, r0);
+  hipLaunchKernelGGL(k_sub, g, bl, 0, c.stream, n, r0, bh, r0);
+  const double r0n = sqrt(K.dot(r0, r0));
+  if (K.err) return K.err;
+  int iter = 0;
+  double rn = r0n;
+  int status = COVOF_NOT_CONVERGED;
+  if (r0n < 1.0e-12) {
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    return COVOF_OK;
+  }
+  hipMemcpyAsync(rt, r0, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+  double rho0 = 1.0, rho1, alpha = 0.0, omega = 1.0, beta, gamma;
+  while (iter < maxit) {
+    iter += 2;
+    rho0 = -omega * rho0;
+    rho1 = K.dot(rt, r0);
+    if (fabs(rho0) < 1.0e-300) { status = COVOF_BREAKDOWN; break; }
+    beta = alpha * (rho1 / rho0);
+    rho0 = rho1;
+    K.axpy(u0, r0, -beta, u0);
+    K.PA(u0, u1);
+    gamma = K.dot(rt, u1);
+    if (fabs(gamma) < 1.0e-300) { status = COVOF_BREAKDOWN; break; }
+    alpha = rho0 / gamma;
+    K.axpy(r0, r0, -alpha, u1);
+    K.axpy(x, x, alpha, u0);
+    K.PA(r0, r1);
+    rho1 = K.dot(rt, r1);
+    if (fabs(rho0) < 1.0e-300) { status = COVOF_BREAKDOWN; break; }
+    beta = alpha * (rho1 / rho0);
+    rho0 = rho1;
+    K.axpy(u0, r0, -beta, u0);
+    K.axpy(u1, r1, -beta, u1);
+    K.PA(u1, u2);
+    gamma = K.dot(rt, u2);
+    if (fabs(gamma) < 1.0e-300) { status = COVOF_BREAKDOWN; break; }
+    alpha = rho0 / gamma;
+    K.axpy(x, x, alpha, u0);
+    K.axpy(r0, r0, -alpha, u1);
+    K.axpy(r1, r1, -alpha, u2);
+    K.PA(r1, r2);
+    double s[5];
+    const double *A5[5] = {r1, r2, r2, r0, r0}, *B5[5] = {r1, r1, r2, r1, r2};
+    K.dots(5, A5, B5, s);
+    const double s11 = s[0], s12 = s[1], s22 = s[2], rhs1 = s[3], rhs2 = s[4];
+    const double rn1 = sqrt(s11), rn2 = sqrt(s22);
+    double g1, g2;
+    if (rn1 < 1.0e-300) {
+      status = COVOF_BREAKDOWN;
+      break;
+    } else if (rn2 < 1.0e-300) {
+      g2 = 0.0;
+      g1 = rhs1 / s11;
+    } else {
+      const double i1 = 1.0 / rn1, i2 = 1.0 / rn2;
+      const double c12 = s12 * i1 * i2, b1 = rhs1 * i1, b2 = rhs2 * i2;
+      const double det = 1.0 - c12 * c12;
+      if (fabs(det) < 1.0e-14) {
+        g2 = 0.0;
+        g1 = b1 * i1;
+      } else {
+        g1 = (b1 - c12 * b2) / (det * rn1);
+        g2 = (b2 - c12 * b1) / (det * rn2);
+      }
+    }
+    hipLaunchKernelGGL(k_axpbz, g, bl, 0, c.stream, n, x, x, g1, r0, g2, r1);
+    hipLaunchKernelGGL(k_xmin, g, bl, 0, c.stream, n, r0, r0, g1, r1, g2, r2);
+    hipLaunchKernelGGL(k_xmin, g, bl, 0, c.stream, n, u0, u0, g1, u1, g2, u2);
+    if (g2 != 0.0) omega = g2;
+    else if (fabs(g1) > 1.0e-300) omega = g1;
+    else omega = 1.0;
+    rn = sqrt(K.dot(r0, r0));
+    if (K.err) return K.err;
+    if (rn != rn || rn > 1.0e30) { status = COVOF_NONFINITE; break; }
+    if (rn / r0n < tol || rn < 1.0e-12) { status = COVOF_OK; break; }
+  }
+  if (K.err) return K.err;
+  if (hipStreamSynchronize(c.stream) != hipSuccess) return COVOF_HIP_ERROR;
+  if (iters) *iters = iter;
+  if (relres) *relres = rn / r0n;
+  return status;
+}
+
+}  // namespace covof

@@ -1,0 +1,174 @@
+/*
+ * covof_hip.h -- C-ABI of the MI355X (gfx950) hot path of COVOF 2D.
+ *
+ * Drop-in replacements for the per-timestep hot path of the reference Fortran
+ * solver: Weymouth-Yue VOF advection with PLIC/ELVIRA reconstruction, the
+ * variable-density pressure-Poisson solve of the standard projection, and the
+ * velocity correction.  Plain C types only (pointers, ints, doubles), so the
+ * Fortran driver binds them with ISO_C_BINDING and Python with ctypes.
+ *
+ * Conventions
+ *  - Field layout is the reference's: Fortran F(0:nx+1, 0:ny+1), column-major,
+ *    i fastest (src/module_fields.f90:129-169): element (i,j) is at offset
+ *    j*(nx+2)+i.  u(i,j) lives on the face x_face(i), v(i,j) on y_face(j)
+ *    (src/module_fields.f90:33-35).  Poisson work arrays are (1:nx,1:ny):
+ *    offset (j-1)*nx+(i-1).
+ *  - Boundary types, linear-solver and preconditioner selectors are the
+ *    reference's integers (src/module_constants.f90:25-31, :53-77).
+ *  - Entry points ending in _d take DEVICE pointers and run on the context's
+ *    HIP stream (set with covof_ctx_set_stream); they return once the work is
+ *    queued unless they must read a result back (iteration counts).  Entry
+ *    points without the suffix take HOST arrays (the Fortran drop-in), copy in,
+ *    run, copy out and synchronise.
+ *  - Every call returns an int status (COVOF_OK on success) and never aborts.
+ *  - One context per thread/stream; calls on one context are serialised.
+ */
+#ifndef COVOF_HIP_H
+#define COVOF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference exits silently; here every outcome is explicit). */
+#define COVOF_OK 0
+#define COVOF_NOT_CONVERGED 1 /* max_iter reached */
+#define COVOF_BREAKDOWN 2     /* |d.Ad| or |r.z| (or BiCGSTAB(2) scalars) underflowed */
+#define COVOF_NONFINITE 3     /* NaN/Inf in the residual */
+#define COVOF_BAD_ARGUMENT 4
+#define COVOF_HIP_ERROR 5
+
+/* Reference selector values (src/module_constants.f90:25-31, :53-77). */
+#define COVOF_BC_DIRICHLET 1
+#define COVOF_BC_NEUMANN 2
+#define COVOF_BC_PERIODIC 3
+#define COVOF_BC_FREESLIP 4
+#define COVOF_LINEAR_BICGSTAB2 1
+#define COVOF_LINEAR_CG 3
+#define COVOF_PRECOND_NONE 0
+#define COVOF_PRECOND_JACOBI 1
+#define COVOF_PRECOND_SSOR 2 /* symmetric Gauss-Seidel, lexicographic (reference SP) */
+/* Arithmetic of the Poisson solve. */
+#define COVOF_PRECISION_FP64 0
+#define COVOF_PRECISION_MIXED 1 /* FP32 inner Krylov, FP64 iterative refinement */
+
+typedef struct covof_ctx covof_ctx;
+
+/* Mesh + device buffers.  Replaces init_mesh (src/module_mesh.f90:67-144) for
+ * one rank and the per-call allocations of advect_vof_weymouth_yue
+ * (src/module_interface_vof.f90:1182-1186) and of the SP Poisson block
+ * (src/module_simulation.f90:2477-2479): all buffers are made here, once.
+ * bc[4] = left, right, bottom, top. */
+int covof_ctx_create(int nx, int ny, double lx, double ly, const int *bc, covof_ctx **out);
+int covof_ctx_destroy(covof_ctx *ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*); NULL = own stream. */
+int covof_ctx_set_stream(covof_ctx *ctx, void *stream);
+int covof_ctx_synchronize(covof_ctx *ctx);
+/* Message of the last error seen on this thread (static storage). */
+const char *covof_last_error(void);
+
+/* --- interface geometry (src/module_interface_reconstruction.f90) ----------- */
+
+/* Batched cut_box_area (:33-101) and compute_alpha (:105-147) on the device:
+ * area[k] = cut_box_area(xmin..ymax, nx, ny, alpha), alpha_out[k] =
+ * compute_alpha(xmin..ymax, nx, ny, c).  Device arrays of length n. */
+int covof_geom_batch_d(covof_ctx *ctx, int n, const double *xmin, const double *xmax,
+                       const double *ymin, const double *ymax, const double *gnx,
+                       const double *gny, const double *alpha, const double *c,
+                       double *area, double *alpha_out);
+
+/* reconstruct_interface(method_name, mesh, C, segments)
+ * (src/module_interface_reconstruction.f90:580-683).  method "PLIC" (default,
+ * any other name) or "ELVIRA".  Outputs the (nx, ny, alpha) of every cell of
+ * the halo grid, as the reference's interface_segment_t array (:23-27), split
+ * into three halo-layout arrays. */
+int covof_reconstruct_interface_d(covof_ctx *ctx, const char *method, const double *C,
+                                  double *seg_nx, double *seg_ny, double *seg_alpha);
+
+/* --- VOF transport (src/module_interface_vof.f90) ---------------------------- */
+
+/* advect_vof(fields, mesh, cfg, dt) (:792-825) with advection_method
+ * "WEYMOUTH_YUE" (:1162-1377), nsteps times.  C (halo layout) is advanced in
+ * place; u, v are the face velocities (halo layout), constant over the call.
+ * *step_count is the reference's Strang counter vof_step_count (:27): the
+ * x-pass goes first when it is even; it is incremented once per step.
+ * If two_phase != 0, update_physical_properties (:479-616) follows every step
+ * (physical-corner fix of C, then rho, rho_u, rho_v, mu, mu_node, all halo
+ * layout; they may be NULL when two_phase == 0).  Other advection methods
+ * return COVOF_BAD_ARGUMENT. */
+int covof_vof_advect_d(covof_ctx *ctx, double *C, const double *u, const double *v, double dt,
+                       const char *advection_method, const char *reconstruction_method,
+                       int *step_count, int nsteps, int two_phase, double rho1, double rho2,
+                       double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+                       double *mu, double *mu_node);
+/* Host-array variant of covof_vof_advect_d (the Fortran drop-in for advect_vof). */
+int covof_vof_advect(covof_ctx *ctx, double *C, const double *u, const double *v, double dt,
+                     const char *advection_method, const char *reconstruction_method,
+                     int *step_count, int nsteps, int two_phase, double rho1, double rho2,
+                     double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+                     double *mu, double *mu_node);
+/* update_physical_properties (src/module_interface_vof.f90:479-616),
+ * two-phase incompressible branch. */
+int covof_update_properties_d(covof_ctx *ctx, double *C, double rho1, double rho2, double mu1,
+                              double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
+                              double *mu_node);
+
+/* --- pressure Poisson of the standard projection (src/module_simulation.f90) */
+
+/* Assembly of the variable-density 5-point operator and of b = -div(u*)
+ * (:2477-2533) into the context; b (nx*ny, device) may be NULL. */
+int covof_poisson_assemble_d(covof_ctx *ctx, const double *u, const double *v,
+                             const double *rho_u, const double *rho_v, double dt, double *b);
+/* y = A x, the reference's poisson_matvec (:2668-2707); (nx,ny) arrays. */
+int covof_poisson_matvec_d(covof_ctx *ctx, const double *x, double *y);
+/* z = M^-1 r: NONE, JACOBI (src/module_linear_solvers.f90:88-135) or SSOR =
+ * the symmetric Gauss-Seidel poisson_ssor (src/module_simulation.f90:2709-2758). */
+int covof_poisson_precond_d(covof_ctx *ctx, int precond, const double *r, double *z);
+/* Krylov solve of A x = b on the assembled operator, x (in: initial guess,
+ * out: solution), (nx,ny) device arrays.
+ *   lin_solver CG: the SP PCG loop (:2535-2575): stop when ||r||/||b|| <= tol,
+ *     ||b|| < 1e-30 -> 1, exits on |d.Ad| < 1e-35 or |r.z| < 1e-35.
+ *   lin_solver BICGSTAB2: src/module_linear_solvers.f90:361-579 (left
+ *     preconditioned, stop on the preconditioned residual ratio or 1e-12).
+ *   precision MIXED: the Krylov loop runs in FP32 inside an FP64
+ *     iterative refinement that recomputes r = b - A x in FP64 and stops on
+ *     the FP64 ||r||/||b|| <= tol (CG + JACOBI/NONE only).
+ * *iters: Krylov iterations (inner iterations summed for MIXED); *relres:
+ * final relative residual as defined by the stop rule. */
+int covof_poisson_solve_d(covof_ctx *ctx, const double *b, double *x, int lin_solver, int precond,
+                          double tol, int maxit, int precision, int *iters, double *relres);
+/* The whole Poisson block of sim_solve_standard_projection (:2477-2588):
+ * assembly from u*, v*, rho_u, rho_v; x0 = p(1:nx,1:ny); solve; p = x - mean(x)
+ * written back to p(1:nx,1:ny), halo wrapped for periodic directions.  The
+ * reference hard-codes CG + SSOR, tol 1e-7, max_iter 1000 (:2437-2438). */
+int covof_sp_poisson_d(covof_ctx *ctx, const double *u, const double *v, const double *rho_u,
+                       const double *rho_v, double *p, double dt, int lin_solver, int precond,
+                       double tol, int maxit, int precision, int *iters, double *relres);
+int covof_sp_poisson(covof_ctx *ctx, const double *u, const double *v, const double *rho_u,
+                     const double *rho_v, double *p, double dt, int lin_solver, int precond,
+                     double tol, int maxit, int precision, int *iters, double *relres);
+/* Velocity correction u -= (dt/rho_u) dp/dx, v -= (dt/rho_v) dp/dy on
+ * non-wall faces (:2590-2654). */
+int covof_sp_correct_d(covof_ctx *ctx, const double *p, const double *rho_u, const double *rho_v,
+                       double dt, double *u, double *v);
+int covof_sp_correct(covof_ctx *ctx, const double *p, const double *rho_u, const double *rho_v,
+                     double dt, double *u, double *v);
+
+/* --- adjoint of the pressure projection (no reference counterpart) --------- */
+
+/* Given the forward solution x of A(rho) x = b and w = dJ/dx (nx*ny device
+ * arrays, J a scalar objective), solve the transpose system A^T lam = w
+ * (A is symmetric) and return dJ/db = lam and dJ/d(log rho) per cell
+ * (nx*ny), where rho enters through the face densities rho_u, rho_v of the
+ * assembly (arithmetic face means of the cell densities).  rho: cell density
+ * (halo layout); dt as in the assembly.  Solver arguments as
+ * covof_poisson_solve_d. */
+int covof_poisson_adjoint_d(covof_ctx *ctx, const double *rho, double dt, const double *x,
+                            const double *w, double *dJ_db, double *dJ_dlogrho, int lin_solver,
+                            int precond, double tol, int maxit, int precision, int *iters,
+                            double *relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COVOF_HIP_H */

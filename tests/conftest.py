@@ -20,3 +20,17 @@ def orc():
     from oracle import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def cv():
+    """The product package on the GPU (tests marked gpu only)."""
+    import torch
+    import covof_amd
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    return covof_amd
+
+
+def to_gpu(a):
+    import torch
+    return torch.tensor(a, dtype=torch.float64, device="cuda")

@@ -1,0 +1,122 @@
+"""Pressure Poisson of the SP projection on the GPU vs the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import to_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n, bc=(4, 4, 4, 4), seed=0, circles=6):
+    from covof_amd import workloads as wl
+    c = wl.poisson_case(n, seed=seed, n_circles=circles, rmin=n / 32, rmax=n / 8)
+    c["bc"] = bc
+    return c
+
+
+@pytest.mark.parametrize("bc", [(4, 4, 4, 4), (3, 3, 3, 3), (1, 2, 3, 3)], ids=["walls", "periodic", "mixed"])
+def test_assembly_matvec_sgs_bitwise(cv, orc, bc):
+    n = 48
+    c = _case(n, bc)
+    ctx = cv.Context(n, n, bc=bc)
+    b = ctx.poisson_assemble(to_gpu(c["u"]), to_gpu(c["v"]), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), c["dt"])
+    m = orc.mesh(n, n, bc=bc)
+    co = orc.poisson_assemble(m, c["u"], c["v"], c["rho_u"], c["rho_v"], c["dt"])
+    assert np.array_equal(b.cpu().numpy(), co["b"])
+    x = np.random.default_rng(1).normal(size=(n, n))
+    y = ctx.poisson_matvec(to_gpu(x))
+    assert np.array_equal(y.cpu().numpy(), orc.poisson_matvec(m, co, x))
+    for pc in (0, 1, 2):
+        z = ctx.poisson_precond(pc, to_gpu(x))
+        assert np.array_equal(z.cpu().numpy(), orc.poisson_precond(m, pc, co, x)), pc
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_pcg_converged_vs_oracle(cv, orc, precond):
+    n = 96
+    c = _case(n)
+    ctx = cv.Context(n, n, bc=c["bc"])
+    b = ctx.poisson_assemble(to_gpu(c["u"]), to_gpu(c["v"]), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), c["dt"])
+    x = ctx.new_work()
+    res = ctx.poisson_solve(b, x, precond=precond, tol=1e-12, max_iter=50000)
+    assert res.converged and res.relres <= 1e-12
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, c["u"], c["v"], c["rho_u"], c["rho_v"], c["dt"])
+    xo, it, rr = orc.sp_pcg(m, precond, co, co["b"], np.zeros((n, n)), tol=1e-12, maxit=50000)
+    xg = x.cpu().numpy()
+    # the operator is singular up to the 1e-12*dvol shift: compare mean-free solutions
+    d = (xg - xg.mean()) - (xo - xo.mean())
+    assert np.abs(d).max() / np.abs(xo - xo.mean()).max() < 1e-9
+    assert abs(res.iters - it) <= max(2, it // 50)
+    # the GPU's own true residual
+    r = co["b"] - orc.poisson_matvec(m, co, xg)
+    assert np.linalg.norm(r) / np.linalg.norm(co["b"]) < 1e-11
+
+
+def test_sp_reference_mode_vs_oracle(cv, orc):
+    """The reference's own SP settings: CG + SGS (SSOR), tol 1e-7, 1000 iterations,
+    x0 = p^n, mean removal; rising-bubble-like density on 64 x 128 walls."""
+    from covof_amd import workloads as wl
+    nx, ny = 64, 128
+    C = wl.disc(nx, ny, 0.5, 0.5, 0.25, lx=1.0, ly=2.0)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    m = orc.mesh(nx, ny, 1.0, 2.0, bc=(1, 1, 1, 1))
+    rho, rho_u, rho_v, mu, mun = orc.update_properties(m, C.copy(), 1000.0, 100.0, 10.0, 1.0)
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=C.shape) * 1e-2
+    v = rng.normal(size=C.shape) * 1e-2
+    p0 = rng.normal(size=C.shape) * 1e-3
+    dt = 1e-3
+    ctx = cv.Context(nx, ny, 1.0, 2.0, bc=(1, 1, 1, 1))
+    pg = to_gpu(p0)
+    res = ctx.solve_standard_projection_poisson(to_gpu(u), to_gpu(v), to_gpu(rho_u), to_gpu(rho_v), pg, dt)
+    co = orc.poisson_assemble(m, u, v, rho_u, rho_v, dt)
+    xo, it, rr = orc.sp_pcg(m, 2, co, co["b"], p0[1:ny + 1, 1:nx + 1].copy(), tol=1e-7, maxit=1000)
+    po = orc.sp_finish_pressure(m, xo, p0)
+    assert abs(res.iters - it) <= 1, (res.iters, it)
+    assert res.relres <= 1e-7
+    pgn = pg.cpu().numpy()
+    scale = np.abs(po[1:ny + 1, 1:nx + 1]).max()
+    assert np.abs(pgn[1:ny + 1, 1:nx + 1] - po[1:ny + 1, 1:nx + 1]).max() / scale < 1e-6
+    # velocity correction, bit-exact given the same p
+    pt = to_gpu(po)
+    ug, vg = to_gpu(u), to_gpu(v)
+    ctx.correct_velocity(pt, to_gpu(rho_u), to_gpu(rho_v), dt, ug, vg)
+    uo, vo = orc.sp_correct(m, po, rho_u, rho_v, dt, u, v)
+    assert np.array_equal(ug.cpu().numpy(), uo)
+    assert np.array_equal(vg.cpu().numpy(), vo)
+
+
+def test_mixed_precision_reaches_fp64_tolerance(cv, orc):
+    n = 256
+    c = _case(n, circles=12)
+    ctx = cv.Context(n, n, bc=c["bc"])
+    b = ctx.poisson_assemble(to_gpu(c["u"]), to_gpu(c["v"]), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), c["dt"])
+    x64 = ctx.new_work()
+    r64 = ctx.poisson_solve(b, x64, precond=1, tol=1e-10, max_iter=200000)
+    xm = ctx.new_work()
+    rm = ctx.poisson_solve(b, xm, precond=1, tol=1e-10, max_iter=200000, precision=cv.PRECISION_MIXED)
+    assert r64.converged and rm.converged, (r64, rm)
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, c["u"], c["v"], c["rho_u"], c["rho_v"], c["dt"])
+    for xg in (x64, xm):
+        xn = xg.cpu().numpy()
+        rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, xn)) / np.linalg.norm(co["b"])
+        assert rr <= 1.01e-10, rr
+
+
+def test_bicgstab2_vs_oracle(cv, orc):
+    n = 64
+    c = _case(n, circles=4)
+    ctx = cv.Context(n, n, bc=c["bc"])
+    b = ctx.poisson_assemble(to_gpu(c["u"]), to_gpu(c["v"]), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), c["dt"])
+    x = ctx.new_work()
+    res = ctx.poisson_solve(b, x, linear_solver=cv.LINEAR_BICGSTAB2, precond=1, tol=1e-12, max_iter=20000)
+    assert res.converged, res
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, c["u"], c["v"], c["rho_u"], c["rho_v"], c["dt"])
+    xo, it, rr = orc.bicgstab2(m, 1, co, co["b"], np.zeros((n, n)), tol=1e-12, maxit=20000)
+    xg = x.cpu().numpy()
+    d = (xg - xg.mean()) - (xo - xo.mean())
+    assert np.abs(d).max() / np.abs(xo - xo.mean()).max() < 1e-8
