@@ -68,7 +68,7 @@ __device__ void plic_mixed(const double *C, const DevMesh &m, int i, int j, doub
                            double &ony, double &oal) {
   double mx, my;
   youngs(C, m, i, j, mx, my);
-  const double nrm = hypot_cr(mx, my);
+  const double nrm = hypot_ref(mx, my);
   double pnx = 1.0, pny = 0.0;
   if (nrm > 1.0e-12) {
     pnx = mx / nrm;
@@ -110,7 +110,7 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
     cnx = -(((CC(1, 1) + 2.0 * CC(1, 0) + CC(1, -1)) - (CC(-1, 1) + 2.0 * CC(-1, 0) + CC(-1, -1))) / (8.0 * dx));
     cny = -(((CC(1, 1) + 2.0 * CC(0, 1) + CC(-1, 1)) - (CC(1, -1) + 2.0 * CC(0, -1) + CC(-1, -1))) / (8.0 * dy));
   }
-  const double cn = hypot_cr(cnx, cny);
+  const double cn = hypot_ref(cnx, cny);
   valid = !(cn < 1.0e-12);
   ux = 1.0;
   uy = 0.0;
@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(1024) k_ghost(const double *__restrict__ C, De
     const double xmin = face_coord(i, m.dx), xmax = face_coord(i + 1, m.dx);
     const double ymin = face_coord(j, m.dy), ymax = face_coord(j + 1, m.dy);
     double sx = gx[r], sy = gy[r];
-    const double nr = hypot_cr(sx, sy);
+    const double nr = hypot_ref(sx, sy);
     if (nr > 1.0e-12) {
       sx = sx / nr;
       sy = sy / nr;

@@ -1,0 +1,60 @@
+"""The device geometry source compiled for the host: its hypot equals the C
+library's hypot bit-for-bit (the reference Fortran's HYPOT calls it, and it is
+not correctly rounded), and cut_box_area / compute_alpha equal the golden
+vectors of the compiled reference."""
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+SRC = os.path.join(ROOT, "tests", "native", "geom_host.cpp")
+INC = os.path.join(ROOT, "covof_2d-fortran_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def host(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("geom") / "libgeomhost.so")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I", INC, SRC,
+                    "-o", out], check=True)
+    L = ct.CDLL(out)
+    for n, k in (("host_hypot_ref", 2), ("host_cut_box_area", 7), ("host_compute_alpha", 7)):
+        f = getattr(L, n)
+        f.restype = ct.c_double
+        f.argtypes = [ct.c_double] * k
+    return L
+
+
+def _libm_hypot():
+    m = ct.CDLL("libm.so.6")
+    m.hypot.restype = ct.c_double
+    m.hypot.argtypes = [ct.c_double, ct.c_double]
+    return m.hypot
+
+
+def test_hypot_matches_libm(host):
+    hyp = _libm_hypot()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.normal(size=200000), rng.uniform(-1, 1, 200000) * 10.0 ** rng.integers(-8, 8, 200000)])
+    ys = np.concatenate([rng.normal(size=200000), rng.uniform(-1, 1, 200000) * 10.0 ** rng.integers(-8, 8, 200000)])
+    # ELVIRA's height-function candidates have one component exactly +-1
+    xs = np.concatenate([xs, rng.uniform(-4, 4, 200000)])
+    ys = np.concatenate([ys, np.where(rng.random(200000) < 0.5, 1.0, -1.0)])
+    bad = [(x, y) for x, y in zip(xs, ys) if host.host_hypot_ref(x, y) != hyp(x, y)]
+    assert not bad, bad[:5]
+    # a case where the C library is not correctly rounded (0.5013 ulp)
+    assert host.host_hypot_ref(1.6005666459918817, 1.0) == hyp(1.6005666459918817, 1.0) == 1.887276765146464
+    for x, y in ((3.0, 4.0), (1.0, 0.0), (0.0, -2.5), (1e-300, 1e-300), (1e300, 1e300), (1e-320, 3e-321)):
+        assert host.host_hypot_ref(x, y) == hyp(x, y), (x, y)
+
+
+def test_geometry_host_vs_oracle(host, orc):
+    g = np.load(os.path.join(ROOT, "tests", "golden", "geom.npz"), allow_pickle=False)
+    n = len(g["xmin"])
+    for k in range(n):
+        a = (g["xmin"][k], g["xmax"][k], g["ymin"][k], g["ymax"][k], g["nx"][k], g["ny"][k])
+        assert host.host_cut_box_area(*a, g["alpha"][k]) == g["area"][k]
+        assert host.host_compute_alpha(*a, g["c"][k]) == g["alpha_out"][k]
