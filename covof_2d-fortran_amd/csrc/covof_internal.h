@@ -24,10 +24,8 @@ struct Vof {
   double *Cb = nullptr;                 // second C buffer (ping-pong), halo layout
   double *snx = nullptr, *sny = nullptr, *sal = nullptr;  // segments, halo layout, mixed cells only
   uint8_t *cd = nullptr;                // dilation indicator [C^n >= 0.5], interior, halo layout
-  int *list[2] = {nullptr, nullptr};    // compacted mixed-cell lists (linear halo index)
-  int *count = nullptr;                 // [0],[1]: list sizes
-  double *ring = nullptr;               // ghost-normal scratch of the halo ring (2 x ring length)
-  double *geom = nullptr;               // device scratch for batched geometry tests
+  int *list[1] = {nullptr};             // per-tile lists of mixed cells (linear halo index)
+  int *count = nullptr;                 // per-tile list sizes
 };
 
 struct Poisson {
@@ -37,9 +35,9 @@ struct Poisson {
   float *kx32 = nullptr, *ky32 = nullptr, *diag32 = nullptr;
   double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
   double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *t4 = nullptr, *t5 = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *z32 = nullptr;
   double *fx = nullptr, *fy = nullptr;  // per-face adjoint sensitivities (kx / ky layout)
-  double *part = nullptr;   // per-block partial sums (nparts x 8)
+  double *part = nullptr;   // per-block partial sums (3 buffers of nblocks x 4)
   double *scal = nullptr;   // device scalars (see poisson.hip)
   unsigned *ticket = nullptr;  // last-block-done counter
   double *hscal = nullptr;  // pinned host mirror of scal

@@ -23,26 +23,28 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace covof {
 
 // ---------------------------------------------------------------- scalars
 enum Slot {
-  S_BNORM = 0, S_RZ = 1, S_ALPHA = 2, S_BETA = 3, S_RR = 4, S_DAD = 5, S_RELRES = 6,
+  S_BNORM = 0, S_RZ = 1, /* and 2 */ S_RR = 4, S_RELRES = 6,
   S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_DOT0 = 10,  // 10..13 generic dot outputs
   // inner (FP32) solve of the mixed mode
-  I_RZ = 16, I_ALPHA = 17, I_BETA = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
-  I_RR = 23, I_DAD = 24,
+  I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
+  I_RR = 23, I_MAXR = 24, I_DELTA = 25,
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
   NSLOT = 64
 };
-enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3 };
+enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_REFRESH = 4 };
 
 struct DevOp {
   int nx, ny;
   int perW, perE, perS, perN;
+  double eps;  // the 1e-12 * dvol diagonal shift of the assembly
 };
 
 template <typename T>
@@ -69,6 +71,27 @@ __device__ __forceinline__ T apply_op(const DevOp &o, const OpT<T> &A, XV xv, in
   return val;
 }
 
+// The same product with the diagonal formed from the four face values,
+// a_P = a_W + a_E + a_S + a_N + 1e-12 dvol in the assembly's order (bit-equal to
+// the stored a_P in FP64), so the diagonal array is not streamed.
+template <typename T, typename XV>
+__device__ __forceinline__ T apply_op_fd(const DevOp &o, const OpT<T> &A, XV xv, int i, int j, long k) {
+  const int nx = o.nx, ny = o.ny;
+  const T aW = A.kx[kxi(nx, i, j)], aE = A.kx[kxi(nx, i + 1, j)];
+  const T aS = A.ky[kyi(nx, i, j)], aN = A.ky[kyi(nx, i, j + 1)];
+  const T aP = aW + aE + aS + aN + (T)o.eps;
+  T val = aP * xv(k);
+  if (i > 1) val = val - aW * xv(k - 1);
+  else if (o.perW) val = val - aW * xv(k + (nx - 1));
+  if (i < nx) val = val - aE * xv(k + 1);
+  else if (o.perE) val = val - aE * xv(k - (nx - 1));
+  if (j > 1) val = val - aS * xv(k - nx);
+  else if (o.perS) val = val - aS * xv(k + (long)(ny - 1) * nx);
+  if (j < ny) val = val - aN * xv(k + nx);
+  else if (o.perN) val = val - aN * xv(k - (long)(ny - 1) * nx);
+  return val;
+}
+
 // Jacobi z = r / d if |d| > 1e-15 else r (src/module_linear_solvers.f90:94-101).
 template <typename T>
 __device__ __forceinline__ T jacobi(T r, T d) {
@@ -76,7 +99,16 @@ __device__ __forceinline__ T jacobi(T r, T d) {
 }
 
 // ---------------------------------------------------- reductions (deterministic)
+//
+// No atomics and no device-scope fences (a release fence on a multi-XCD part
+// writes the whole L2 back): a kernel leaves one partial per block (NP slots)
+// with plain stores, and the NEXT kernel on the stream -- the kernel boundary
+// makes the stores visible -- has every block sum all partials in the same
+// fixed order.  Every block thus derives bit-identical scalars; persistent
+// scalars are written by block 0 only, into slots nobody else reads during
+// that launch (parity-indexed where a value is both read and updated).
 constexpr int BLK = 256;  // threads per block: tile 64 (x) by 4 (y)
+constexpr int NP = 4;     // partial slots per block
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -85,16 +117,15 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Sums NV per-thread values over the block (waves in T, across waves in
-// double) and the grid; returns true in thread 0 of the last block to
-// finish, which then finds the grid totals in tot[].
+__device__ __forceinline__ int block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
+__device__ __forceinline__ bool lead() { return threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0; }
+
+// Block sums of v[0..NV) (wave sums in T, across waves in double) into
+// part[block*NP + first + q].
 template <int NV, typename T>
-__device__ bool grid_reduce(const T (&v)[NV], double *part, unsigned *ticket, double (&tot)[NV]) {
+__device__ void block_partials(const T (&v)[NV], double *part, int first = 0) {
   __shared__ double sh[NV][BLK / 64];
-  __shared__ bool last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int nblk = gridDim.x * gridDim.y;
-  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     const T w = wave_sum<T>(v[q]);
@@ -104,36 +135,33 @@ __device__ bool grid_reduce(const T (&v)[NV], double *part, unsigned *ticket, do
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
-      part[(long)bid * NV + q] = s;
+      double t = 0.0;
+      for (int w = 0; w < BLK / 64; ++w) t += sh[q][w];
+      part[(long)block_id() * NP + first + q] = t;
     }
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    last = (t == (unsigned)(nblk - 1));
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // last block: fixed-order sum of the partials
+}
+
+// Grid totals of slots [0, NV) of the partials left by the previous kernel
+// (nblk blocks); the same value in every thread of every block.
+template <int NV>
+__device__ void grid_totals(const double *part, int nblk, double (&tot)[NV]) {
+  __shared__ double sh[NV][BLK / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += BLK) s += __builtin_nontemporal_load(&part[(long)b * NV + q]);
-    s = wave_sum<double>(s);
-    if (lane == 0) sh[q][wid] = s;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += BLK) t += part[(long)b * NP + q];
+    t = wave_sum<double>(t);
+    if (lane == 0) sh[q][wid] = t;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
-      tot[q] = s;
-    }
-    *ticket = 0u;
+  for (int q = 0; q < NV; ++q) {
+    double t = 0.0;
+    for (int w = 0; w < BLK / 64; ++w) t += sh[q][w];
+    tot[q] = t;
   }
-  return threadIdx.x == 0;
 }
 
 // Cell loop over the block's tile column and its rows (grid-stride in y).
@@ -224,7 +252,8 @@ __global__ void k_set_scalars(double *scal, int first, int n, double v) {
 // lexicographic order sees (still 0 forward, the forward value backward), so
 // the result equals the sequential recurrence bit-for-bit.
 __global__ void __launch_bounds__(1024) k_sgs_wavefront(DevOp o, OpT<double> A, const double *__restrict__ r,
-                                                        double *z) {
+                                                        double *z, const double *flag) {
+  if (flag && *flag != 0.0) return;
   const int nx = o.nx, ny = o.ny;
   const long n = (long)nx * ny;
   for (long t = threadIdx.x; t < n; t += blockDim.x) z[t] = 0.0;
@@ -261,149 +290,164 @@ __global__ void __launch_bounds__(1024) k_sgs_wavefront(DevOp o, OpT<double> A, 
 
 // ------------------------------------------------------------ PCG (fused)
 //
-// Iteration of the SP PCG loop (src/module_simulation.f90:2553-2575):
-//   kernel A: d = z + beta d_prev (d = z on the first iteration, d_prev = 0),
-//             Ad = A d, partial d.Ad;  last block: iters += 1, |dAd| < 1e-35
-//             -> breakdown, alpha = rz / dAd.
-//   kernel B: x += alpha d, r -= alpha Ad, partial r.r and r.z;  last block:
-//             ||r||/||b|| <= tol -> converged, |rz_old| < 1e-35 -> breakdown,
-//             beta = rz_new / rz_old.
-// PC: 0 none (z = r), 1 Jacobi (z = r/diag on the fly), 2 explicit z array.
-// d is ping-ponged (kernel A reads neighbours' d_prev while writing d).
+// Iteration k of the SP PCG loop (src/module_simulation.f90:2553-2575), two
+// kernels, each opening with the grid totals of the previous kernel:
+//   A_k: totals of B_{k-1} (or of the initial residual when k = 1): r.r, r.z
+//        (and b.b at k = 1).  ||r||/||b|| <= tol -> converged (the exit of
+//        iteration k-1); |rz_old| < 1e-35 -> breakdown; beta = rz / rz_old.
+//        Body: d = z + beta d_prev (d = z at k = 1, d_prev = 0), Ad = A d,
+//        partial d.Ad.  The matvec's neighbour values of d are formed on the
+//        fly from z and d_prev, and its diagonal from the face coefficients
+//        (bit-equal to a_P), so A_k reads z, d_prev, kx, ky only.
+//   B_k: totals of A_k: d.Ad; |dAd| < 1e-35 -> breakdown; alpha = rz / dAd.
+//        Body: x += alpha d, r -= alpha Ad, z = r / a_P (Jacobi), partials
+//        r.r and r.z.
+// d is ping-ponged (A_k reads neighbours' d_prev while writing d); rz is kept
+// in two parity slots (A_k reads rz_{k-1} and stores rz_k).
 struct PcgSlots {
-  int rz, alpha, beta, dad, rr, flag, iters, bnorm_or_rr0, tol;
-  int relative_to_rr0;  // inner solve: stop on ||r|| <= tol * ||r0||
+  int rz;        // rz at parity 0, rz at parity 1 in rz + 1
+  int rr, relres, flag, iters, ref, tol;
+  int is_b_norm;  // reference norm: ||b|| (< 1e-30 -> 1) from the k = 1 totals;
+                  // 0: the norm already in slot ref (the outer ||b||)
+  int maxr;       // >= 0: reliable-update mode -- stop with F_REFRESH once
+                  // ||r|| <= delta * scal[maxr] or ||r|| <= tol * ref
 };
 
-template <typename T, int PC>
-__device__ __forceinline__ T zval(const T *r, const T *diag, const T *z, long k) {
-  if (PC == 0) return r[k];
-  if (PC == 1) return jacobi<T>(r[k], diag[k]);
-  return z[k];
-}
-
-template <typename T, int PC>
-__global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__restrict__ r,
-                                               const T *__restrict__ z, const T *__restrict__ dprev,
-                                               T *__restrict__ d, T *__restrict__ Ad, double *scal,
-                                               PcgSlots sl, double *part, unsigned *ticket) {
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__restrict__ zs,
+                                               const T *__restrict__ dprev, T *__restrict__ d,
+                                               T *__restrict__ Ad, double *scal, PcgSlots sl,
+                                               const double *pin, double *pout, int k, int check_only) {
   if (scal[sl.flag] != 0.0) return;
-  const T beta = (T)scal[sl.beta];
-  T acc[1] = {T(0)};
-  CELL_LOOP(o) {
-    const long k = (long)(j - 1) * o.nx + (i - 1);
-    auto dv = [&](long q) { return zval<T, PC>(r, A.diag, z, q) + beta * dprev[q]; };
-    const T dk = dv(k);
-    const T q = apply_op<T>(o, A, dv, i, j, k);
-    d[k] = dk;
-    Ad[k] = q;
-    acc[0] += dk * q;
+  double tot[3];
+  grid_totals<3>(pin, gridDim.x * gridDim.y, tot);
+  double ref;
+  if (k == 1 && sl.is_b_norm) {
+    ref = sqrt(tot[2]);
+    if (sl.is_b_norm == 1 && ref < 1.0e-30) ref = 1.0;
+  } else {
+    ref = scal[sl.ref];
   }
-  double tot[1];
-  if (grid_reduce<1, T>(acc, part, ticket, tot)) {
-    scal[sl.iters] += 1.0;
-    const double dAd = tot[0];
-    scal[sl.dad] = dAd;
-    if (!(fabs(dAd) >= 1.0e-35)) {
-      scal[sl.flag] = (dAd == dAd) ? (double)F_BREAK : (double)F_NONFINITE;
-    } else {
-      scal[sl.alpha] = scal[sl.rz] / dAd;
+  const double rn = sqrt(tot[0]);
+  const double rz_prev = scal[sl.rz + ((k - 1) & 1)];
+  int stop = F_RUN;
+  if (!(rn == rn) || rn > 1.0e300) stop = F_NONFINITE;
+  else if (sl.maxr >= 0 && (rn <= scal[I_DELTA] * scal[sl.maxr] || rn / ref <= scal[sl.tol])) stop = F_REFRESH;
+  else if (!(ref > 0.0) || rn / ref <= scal[sl.tol]) stop = F_CONV;
+  else if (k > 1 && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+  if (lead()) {
+    if (k == 1 && sl.is_b_norm) scal[sl.ref] = ref;
+    scal[sl.rr] = tot[0];
+    scal[sl.relres] = rn / ref;
+    if (stop != F_RUN) {
+      scal[sl.flag] = (double)stop;
+    } else if (!check_only) {
+      scal[sl.iters] = (double)k;
+      scal[sl.rz + (k & 1)] = tot[1];
     }
   }
+  if (stop != F_RUN || check_only) return;
+  const T beta = (k == 1) ? T(0) : (T)(tot[1] / rz_prev);
+  T acc[1] = {T(0)};
+  CELL_LOOP(o) {
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    auto dv = [&](long m) { return zs[m] + beta * dprev[m]; };
+    const T dk = dv(q);
+    const T aq = apply_op_fd<T>(o, A, dv, i, j, q);
+    d[q] = dk;
+    Ad[q] = aq;
+    acc[0] += dk * aq;
+  }
+  block_partials<1, T>(acc, pout);
 }
 
 template <typename T, int PC>
 __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ diag, T *__restrict__ x,
-                                               T *__restrict__ r, const T *__restrict__ d,
+                                               T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
                                                const T *__restrict__ Ad, double *scal, PcgSlots sl,
-                                               double *part, unsigned *ticket) {
+                                               const double *pin, double *pout, int k) {
   if (scal[sl.flag] != 0.0) return;
-  const T alpha = (T)scal[sl.alpha];
-  T acc[2] = {T(0), T(0)};
+  double tot[1];
+  grid_totals<1>(pin, gridDim.x * gridDim.y, tot);
+  const double dAd = tot[0];
+  if (!(fabs(dAd) >= 1.0e-35)) {
+    if (lead()) scal[sl.flag] = (dAd == dAd) ? (double)F_BREAK : (double)F_NONFINITE;
+    return;
+  }
+  const T alpha = (T)(scal[sl.rz + (k & 1)] / dAd);
+  T acc[3] = {T(0), T(0), T(0)};
   CELL_LOOP(o) {
-    const long k = (long)(j - 1) * o.nx + (i - 1);
-    x[k] = x[k] + alpha * d[k];
-    const T rk = r[k] - alpha * Ad[k];
-    r[k] = rk;
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    x[q] = x[q] + alpha * d[q];
+    const T rk = r[q] - alpha * Ad[q];
+    r[q] = rk;
     acc[0] += rk * rk;
     if (PC == 0) acc[1] += rk * rk;
-    if (PC == 1) acc[1] += rk * jacobi<T>(rk, diag[k]);
-  }
-  double tot[2];
-  if (grid_reduce<2, T>(acc, part, ticket, tot)) {
-    const double rn = sqrt(tot[0]);
-    scal[sl.rr] = tot[0];
-    const double ref = sl.relative_to_rr0 ? sqrt(scal[sl.bnorm_or_rr0]) : scal[sl.bnorm_or_rr0];
-    if (!(rn == rn) || rn > 1.0e300) {
-      scal[sl.flag] = (double)F_NONFINITE;
-    } else if (rn / ref <= scal[sl.tol]) {
-      scal[sl.flag] = (double)F_CONV;
-    } else if (PC != 2) {
-      if (fabs(scal[sl.rz]) < 1.0e-35) {
-        scal[sl.flag] = (double)F_BREAK;
-      } else {
-        scal[sl.beta] = tot[1] / scal[sl.rz];
-        scal[sl.rz] = tot[1];
-      }
+    if (PC == 1) {
+      const T zk = jacobi<T>(rk, diag[q]);
+      z[q] = zk;
+      acc[1] += rk * zk;
     }
   }
+  block_partials<3, T>(acc, pout);
 }
 
-// r.z with an explicit z (SGS preconditioner) and the beta update of kernel B.
+// r.z with an explicit z (SGS) into slot 1 of B's partials.
 __global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const double *__restrict__ r, const double *__restrict__ z,
-                                                double *scal, PcgSlots sl, double *part, unsigned *ticket) {
+                                                const double *scal, PcgSlots sl, double *pout) {
   if (scal[sl.flag] != 0.0) return;
   double acc[1] = {0.0};
   CELL_LOOP(o) {
-    const long k = (long)(j - 1) * o.nx + (i - 1);
-    acc[0] += r[k] * z[k];
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    acc[0] += r[q] * z[q];
   }
-  double tot[1];
-  if (grid_reduce<1, double>(acc, part, ticket, tot)) {
-    if (fabs(scal[sl.rz]) < 1.0e-35) {
-      scal[sl.flag] = (double)F_BREAK;
-    } else {
-      scal[sl.beta] = tot[0] / scal[sl.rz];
-      scal[sl.rz] = tot[0];
-    }
-  }
+  block_partials<1, double>(acc, pout, 1);
 }
 
-// Initial residual of the SP loop (:2538-2549): r = b - A x, ||b||, ||r||,
-// converged if ||r||/||b|| <= tol; r.z for PC 0/1.  r32 (optional) receives
-// the FP32 copy of the residual for the inner solve of the mixed mode.
+// Initial residual of the SP loop (:2538-2549): r = b - A x, z = r / a_P
+// (Jacobi), partials r.r, r.z, b.b.  r32 (optional) receives the FP32 copy of
+// r for the inner solve of the mixed mode.
 template <int PC>
 __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const double *__restrict__ b,
                                                   const double *__restrict__ x, double *__restrict__ r,
-                                                  float *__restrict__ r32, double *scal, PcgSlots sl,
-                                                  int want_bnorm, double *part, unsigned *ticket) {
+                                                  double *__restrict__ z, float *__restrict__ r32,
+                                                  double *pout) {
   double acc[3] = {0.0, 0.0, 0.0};
   CELL_LOOP(o) {
-    const long k = (long)(j - 1) * o.nx + (i - 1);
-    const double ax = apply_op<double>(o, A, [&](long q) { return x[q]; }, i, j, k);
-    const double rk = b[k] - ax;
-    r[k] = rk;
-    if (r32) r32[k] = (float)rk;
-    acc[0] += b[k] * b[k];
-    acc[1] += rk * rk;
-    if (PC == 0) acc[2] += rk * rk;
-    if (PC == 1) acc[2] += rk * jacobi<double>(rk, A.diag[k]);
-  }
-  double tot[3];
-  if (grid_reduce<3, double>(acc, part, ticket, tot)) {
-    if (want_bnorm) {
-      double bn = sqrt(tot[0]);
-      if (bn < 1.0e-30) bn = 1.0;
-      scal[S_BNORM] = bn;
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    const double ax = apply_op<double>(o, A, [&](long m) { return x[m]; }, i, j, q);
+    const double rk = b[q] - ax;
+    r[q] = rk;
+    if (r32) r32[q] = (float)rk;
+    acc[0] += rk * rk;
+    if (PC == 0) acc[1] += rk * rk;
+    if (PC == 1) {
+      const double zk = jacobi<double>(rk, A.diag[q]);
+      z[q] = zk;
+      acc[1] += rk * zk;
     }
-    const double rn = sqrt(tot[1]);
-    scal[sl.rr] = tot[1];
-    scal[S_RELRES] = rn / scal[S_BNORM];
-    if (!(rn == rn) || rn > 1.0e300) scal[sl.flag] = (double)F_NONFINITE;
-    else if (rn / scal[S_BNORM] <= scal[sl.tol]) scal[sl.flag] = (double)F_CONV;
-    if (PC != 2) scal[sl.rz] = tot[2];
-    scal[sl.beta] = 0.0;
+    acc[2] += b[q] * b[q];
   }
+  block_partials<3, double>(acc, pout);
+}
+
+// Outer FP64 residual check of the mixed mode (one block): ||b||, ||r||,
+// converged when ||r||/||b|| <= tol.
+__global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int nblk, double *scal, int first) {
+  double tot[3];
+  grid_totals<3>(pin, nblk, tot);
+  if (threadIdx.x != 0) return;
+  if (first) {
+    double bn = sqrt(tot[2]);
+    if (bn < 1.0e-30) bn = 1.0;
+    scal[S_BNORM] = bn;
+  }
+  const double rn = sqrt(tot[0]);
+  scal[S_RR] = tot[0];
+  scal[S_RELRES] = rn / scal[S_BNORM];
+  scal[I_MAXR] = rn;
+  if (!(rn == rn) || rn > 1.0e300) scal[S_FLAG] = (double)F_NONFINITE;
+  else if (rn / scal[S_BNORM] <= scal[S_TOL]) scal[S_FLAG] = (double)F_CONV;
 }
 
 // Refinement update x += (double) dx32.
@@ -412,35 +456,39 @@ __global__ void k_add_correction(long n, double *__restrict__ x, const float *__
     x[t] = x[t] + (double)dx32[t];
 }
 
-// FP32 inner-solve setup: r.z and r.r of r32 (z = Jacobi or identity), beta = 0.
+// FP32 inner-solve setup: z32 = r32 / a_P (Jacobi) or r32, partials r.r,
+// r.z, r.r (the last is the inner reference norm ||r0||^2).
 template <int PC>
-__global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const float *__restrict__ r32, const float *__restrict__ d32,
-                                                    double *scal, PcgSlots sl, double *part, unsigned *ticket) {
-  if (scal[S_FLAG] != 0.0) return;
-  float acc[2] = {0.f, 0.f};
+__global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const float *__restrict__ r32,
+                                                    const float *__restrict__ d32, float *__restrict__ z32,
+                                                    double *pout) {
+  float acc[3] = {0.f, 0.f, 0.f};
   CELL_LOOP(o) {
-    const long k = (long)(j - 1) * o.nx + (i - 1);
-    const float rk = r32[k];
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    const float rk = r32[q];
     acc[0] += rk * rk;
-    acc[1] += rk * zval<float, PC>(r32, d32, (const float *)nullptr, k);
+    if (PC == 0) acc[1] += rk * rk;
+    if (PC == 1) {
+      const float zk = jacobi<float>(rk, d32[q]);
+      z32[q] = zk;
+      acc[1] += rk * zk;
+    }
   }
-  double tot[2];
-  if (grid_reduce<2, float>(acc, part, ticket, tot)) {
-    scal[sl.bnorm_or_rr0] = tot[0];
-    scal[sl.rz] = tot[1];
-    scal[sl.beta] = 0.0;
-    scal[sl.flag] = 0.0;
-    if (!(tot[0] > 0.0)) scal[sl.flag] = (double)F_CONV;
-  }
+  acc[2] = acc[0];
+  block_partials<3, float>(acc, pout);
 }
 
 // p = x - mean(x) (src/module_simulation.f90:2578-2585) into p(1:nx,1:ny).
-__global__ void __launch_bounds__(BLK) k_sum(DevOp o, const double *__restrict__ x, double *scal, int slot,
-                                             double *part, unsigned *ticket) {
+__global__ void __launch_bounds__(BLK) k_sum(DevOp o, const double *__restrict__ x, double *pout) {
   double acc[1] = {0.0};
   CELL_LOOP(o) { acc[0] += x[(long)(j - 1) * o.nx + (i - 1)]; }
+  block_partials<1, double>(acc, pout);
+}
+
+__global__ void __launch_bounds__(BLK) k_sum_finish(const double *pin, int nblk, double *scal, int slot) {
   double tot[1];
-  if (grid_reduce<1, double>(acc, part, ticket, tot)) scal[slot] = tot[0];
+  grid_totals<1>(pin, nblk, tot);
+  if (threadIdx.x == 0) scal[slot] = tot[0];
 }
 
 __global__ void k_write_pressure(DevOp o, const double *__restrict__ x, const double *scal, int slot,
@@ -519,6 +567,7 @@ static DevOp dev_op(const Mesh &m) {
   o.perE = m.bc[1] == BC_PERIODIC;
   o.perS = m.bc[2] == BC_PERIODIC;
   o.perN = m.bc[3] == BC_PERIODIC;
+  o.eps = 1.0e-12 * (m.dx * m.dy);
   return o;
 }
 
@@ -546,7 +595,7 @@ int poisson_alloc(Ctx &c) {
   for (double **pp : d64) COVOF_CHECK(hipMalloc(pp, n * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.fx, nkx * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.fy, nky * sizeof(double)));
-  float **d32[] = {&P.r32, &P.p32, &P.q32, &P.x32};
+  float **d32[] = {&P.r32, &P.p32, &P.q32, &P.x32, &P.z32};
   for (float **pp : d32) COVOF_CHECK(hipMalloc(pp, n * sizeof(float)));
   // z32 doubles as the second ping-pong buffer of p32: allocate 2n for p32
   hipFree(P.p32);
@@ -555,7 +604,7 @@ int poisson_alloc(Ctx &c) {
   P.nblocks = g.x * g.y;
   // partials: up to 4 values per block of the fused kernels, 5 per block of
   // the 512-block inner-product kernel of BiCGSTAB(2)
-  const long npart = std::max<long>((long)P.nblocks * 4, 512L * 5);
+  const long npart = std::max<long>((long)P.nblocks * NP * 3, 512L * 5);
   COVOF_CHECK(hipMalloc(&P.part, npart * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.scal, NSLOT * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.ticket, sizeof(unsigned)));
@@ -568,7 +617,7 @@ int poisson_alloc(Ctx &c) {
 void poisson_free(Ctx &c) {
   Poisson &P = c.ps;
   void *ptrs[] = {P.kx, P.ky, P.diag, P.kx32, P.ky32, P.diag32, P.b, P.x, P.r, P.p, P.q, P.z,
-                  P.t0, P.t1, P.t2, P.t3, P.t4, P.t5, P.r32, P.p32, P.q32, P.x32, P.part, P.scal, P.ticket,
+                  P.t0, P.t1, P.t2, P.t3, P.t4, P.t5, P.r32, P.p32, P.q32, P.x32, P.z32, P.part, P.scal, P.ticket,
                   P.fx, P.fy};
   for (void *q : ptrs) hipFree(q);
   if (P.hscal) hipHostFree(P.hscal);
@@ -608,7 +657,7 @@ int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
   if (precond == COVOF_PRECOND_SSOR) {
     const DevOp o = dev_op(c.m);
     OpT<double> A{c.ps.kx, c.ps.ky, c.ps.diag};
-    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, r, z);
+    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, r, z, (const double *)nullptr);
   } else if (precond == COVOF_PRECOND_JACOBI || precond == COVOF_PRECOND_NONE) {
     hipLaunchKernelGGL(k_precond_simple, dim3((n + 255) / 256), dim3(256), 0, c.stream, precond, n,
                        c.ps.diag, r, z);
@@ -639,24 +688,42 @@ static int fill(Ctx &c, T *x, long n, T v) {
   return COVOF_OK;
 }
 
-static const PcgSlots OUTER = {S_RZ, S_ALPHA, S_BETA, S_DAD, S_RR, S_FLAG, S_ITERS, S_BNORM, S_TOL, 0};
-static const PcgSlots INNER = {I_RZ, I_ALPHA, I_BETA, I_DAD, I_RR, I_FLAG, I_ITERS, I_RR0, I_ETA, 1};
+static const PcgSlots OUTER = {S_RZ, S_RR, S_RELRES, S_FLAG, S_ITERS, S_BNORM, S_TOL, 1, -1};
+static const PcgSlots INNER = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, S_BNORM, S_TOL, 0, I_MAXR};
+// restarted refinement: inner solve from dx = 0 to ||r_in|| <= eta ||r_in,0||
+static const PcgSlots INNER_RESTART = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, I_RR0, I_ETA, 2, -1};
 
-// Launches up to `count` fused PCG iterations (T, PC) on (x, r) with the
-// ping-pong direction buffers d0/d1; *parity tracks which holds d_prev.
+// Partial buffers: [0] written by A kernels, [1] by B / residual / rz kernels,
+// [2] by the sum kernel.
+static double *part_buf(Ctx &c, int w) { return c.ps.part + (long)w * c.ps.nblocks * NP; }
+
+// Chunk of polled iterations: short while the solve may end early, longer later.
+static int chunk_for(int done) { return done < 64 ? 16 : (done < 1024 ? 64 : 256); }
+
+// PCG iterations k = k0 .. k0+count-1 (T, PC) on (x, r, z); zs = z (PC 1/2)
+// or r (PC 0); d0/d1 ping-pong the search direction (d_prev at k = 1 is d0,
+// which must hold zeros).  PC 2 runs the SGS wavefront and r.z after B.
 template <typename T, int PC>
-static int pcg_iterations(Ctx &c, OpT<T> A, T *x, T *r, const T *z, T *d0, T *d1, T *Ad,
-                          const PcgSlots &sl, int count, int *parity) {
+static int pcg_chunk(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
+                     int k0, int count) {
   const DevOp o = dev_op(c.m);
   const dim3 g = red_grid(c.m);
-  for (int it = 0; it < count; ++it) {
-    T *dprev = (*parity == 0) ? d0 : d1;
-    T *dnew = (*parity == 0) ? d1 : d0;
-    hipLaunchKernelGGL((k_pcg_a<T, PC>), g, dim3(BLK), 0, c.stream, o, A, r, z, dprev, dnew, Ad,
-                       c.ps.scal, sl, c.ps.part, c.ps.ticket);
-    hipLaunchKernelGGL((k_pcg_b<T, PC>), g, dim3(BLK), 0, c.stream, o, A.diag, x, r, dnew, Ad,
-                       c.ps.scal, sl, c.ps.part, c.ps.ticket);
-    *parity ^= 1;
+  double *pa = part_buf(c, 0), *pb = part_buf(c, 1);
+  const T *zs = (PC == 0) ? r : z;
+  for (int k = k0; k < k0 + count; ++k) {
+    T *dprev = (k & 1) ? d0 : d1;  // k = 1 reads d0 (zeros), writes d1
+    T *dnew = (k & 1) ? d1 : d0;
+    hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, o, A, zs, dprev, dnew, Ad, c.ps.scal, sl,
+                       (const double *)pb, pa, k, 0);
+    hipLaunchKernelGGL((k_pcg_b<T, PC>), g, dim3(BLK), 0, c.stream, o, A.diag, x, r, z, dnew, Ad,
+                       c.ps.scal, sl, pa, pb, k);
+    if (PC == 2) {
+      OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
+      hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, (const double *)r,
+                         (double *)z, (const double *)(c.ps.scal + sl.flag));
+      hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)r, (const double *)z,
+                         c.ps.scal, sl, pb);
+    }
   }
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
@@ -669,7 +736,29 @@ static int status_of_flag(double f, int iters, int maxit) {
   return iters >= maxit ? COVOF_NOT_CONVERGED : COVOF_OK;
 }
 
-// FP64 PCG of the SP loop with NONE / JACOBI (fused) or SSOR (SGS wavefront).
+// Drives PCG iterations k = 1.. until the flag slot is set or maxit iterations
+// have run; the residual partials for A_1 must be in part_buf(1).
+template <typename T, int PC>
+static int pcg_loop(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
+                    int maxit, int nread, int k_first = 1) {
+  int st, done = k_first - 1;
+  while (done < maxit) {
+    const int chunk = std::min(maxit - done, chunk_for(done - k_first + 1));
+    if ((st = pcg_chunk<T, PC>(c, A, x, r, z, d0, d1, Ad, sl, done + 1, chunk))) return st;
+    done += chunk;
+    if ((st = read_scalars(c, nread))) return st;
+    if (c.ps.hscal[sl.flag] != 0.0) return COVOF_OK;
+  }
+  // the convergence test of the last iteration runs in the prologue of the
+  // next A kernel: launch that prologue alone
+  const dim3 g = red_grid(c.m);
+  hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, dev_op(c.m), A, (const T *)r, (const T *)d0, d1,
+                     Ad, c.ps.scal, sl, (const double *)part_buf(c, 1), part_buf(c, 0), maxit + 1, 1);
+  COVOF_CHECK_LAUNCH();
+  return read_scalars(c, nread);
+}
+
+// FP64 PCG of the SP loop with NONE / JACOBI / SSOR (SGS wavefront).
 static int solve_pcg64(Ctx &c, const double *b, double *x, int precond, double tol, int maxit,
                        int *iters, double *relres) {
   Poisson &P = c.ps;
@@ -680,70 +769,46 @@ static int solve_pcg64(Ctx &c, const double *b, double *x, int precond, double t
   OpT<double> A{P.kx, P.ky, P.diag};
   int st;
   if ((st = fill<double>(c, P.p, n, 0.0))) return st;
-  if ((st = fill<double>(c, P.q, n, 0.0))) return st;
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, 16, 0.0);
   if ((st = set_scalar(c, S_TOL, tol))) return st;
   const int pc = precond == COVOF_PRECOND_SSOR ? 2 : (precond == COVOF_PRECOND_JACOBI ? 1 : 0);
-  if (pc == 2)
-    hipLaunchKernelGGL((k_residual<2>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, (float *)nullptr,
-                       P.scal, OUTER, 1, P.part, P.ticket);
-  else if (pc == 1)
-    hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, (float *)nullptr,
-                       P.scal, OUTER, 1, P.part, P.ticket);
+  double *pb = part_buf(c, 1);
+  if (pc == 1)
+    hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, (float *)nullptr, pb);
   else
-    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, (float *)nullptr,
-                       P.scal, OUTER, 1, P.part, P.ticket);
+    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, (float *)nullptr, pb);
   COVOF_CHECK_LAUNCH();
-  int parity = 0, done = 0;
   if (pc == 2) {
-    // z = SGS(r), rz = r.z before the loop; every iteration: A, B, SGS, rz
-    if ((st = poisson_precond(c, COVOF_PRECOND_SSOR, P.r, P.z))) return st;
-    // rz = r.z via k_pcg_rz (seeded with rz_old = 1 so its breakdown test passes)
-    if ((st = set_scalar(c, S_RZ, 1.0))) return st;
-    hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, P.r, P.z, P.scal, OUTER, P.part, P.ticket);
-    if ((st = set_scalar(c, S_BETA, 0.0))) return st;
-    while (done < maxit) {
-      const int chunk = std::min(maxit - done, done < 64 ? 16 : 64);
-      for (int it = 0; it < chunk; ++it) {
-        double *dprev = parity == 0 ? P.p : P.t0;
-        double *dnew = parity == 0 ? P.t0 : P.p;
-        hipLaunchKernelGGL((k_pcg_a<double, 2>), g, dim3(BLK), 0, c.stream, o, A, P.r, P.z, dprev,
-                           dnew, P.q, P.scal, OUTER, P.part, P.ticket);
-        hipLaunchKernelGGL((k_pcg_b<double, 2>), g, dim3(BLK), 0, c.stream, o, P.diag, x, P.r, dnew,
-                           P.q, P.scal, OUTER, P.part, P.ticket);
-        if ((st = poisson_precond(c, COVOF_PRECOND_SSOR, P.r, P.z))) return st;
-        hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, P.r, P.z, P.scal, OUTER, P.part, P.ticket);
-        parity ^= 1;
-      }
-      COVOF_CHECK_LAUNCH();
-      done += chunk;
-      if ((st = read_scalars(c, 16))) return st;
-      if (P.hscal[S_FLAG] != 0.0) break;
-    }
+    // z = SGS(r) and r.z into slot 1 of the residual partials
+    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, (const double *)P.r, P.z,
+                       (const double *)nullptr);
+    hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
+                       P.scal, OUTER, pb);
+    COVOF_CHECK_LAUNCH();
+    st = pcg_loop<double, 2>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
+  } else if (pc == 1) {
+    st = pcg_loop<double, 1>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
   } else {
-    // d_prev must be 0 on the first iteration (d = z + 0 * d_prev)
-    while (done < maxit) {
-      const int chunk = std::min(maxit - done, done < 64 ? 16 : (done < 1024 ? 64 : 256));
-      if (pc == 1) st = pcg_iterations<double, 1>(c, A, x, P.r, nullptr, P.p, P.t0, P.q, OUTER, chunk, &parity);
-      else st = pcg_iterations<double, 0>(c, A, x, P.r, nullptr, P.p, P.t0, P.q, OUTER, chunk, &parity);
-      if (st) return st;
-      done += chunk;
-      if ((st = read_scalars(c, 16))) return st;
-      if (P.hscal[S_FLAG] != 0.0) break;
-    }
+    st = pcg_loop<double, 0>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
   }
-  if ((st = read_scalars(c, 16))) return st;
+  if (st) return st;
   const int it = (int)P.hscal[S_ITERS];
   if (iters) *iters = it;
-  if (relres) *relres = (P.hscal[S_ITERS] > 0.0) ? sqrt(P.hscal[S_RR]) / P.hscal[S_BNORM] : P.hscal[S_RELRES];
+  if (relres) *relres = P.hscal[S_RELRES];
   return status_of_flag(P.hscal[S_FLAG], it, maxit);
 }
 
-// Mixed precision: FP64 iterative refinement around an FP32 PCG (NONE/JACOBI).
-// Outer: r = b - A x in FP64 (with the previous correction folded into x in
-// the same pass); stop on ||r||/||b|| <= tol.  Inner: PCG on A32 dx = r32 from
-// dx = 0 to ||r_in|| <= eta ||r_in,0|| (FP32 vectors and operator, FP32 wave
-// sums, FP64 across waves and blocks).
+// Mixed precision: FP32 PCG with reliable updates in FP64 (NONE/JACOBI).
+// The Krylov recurrence (x32 correction, r32, z32, d, Ad, scalars) runs on the
+// FP32 operator copy with FP32 wave sums (FP64 across waves and blocks).
+// Whenever the FP32 residual has dropped by delta = 0.1 since the last
+// update, or reaches tol ||b||, the iterate is folded into the FP64 x
+// (x += x32), the true residual r = b - A x is recomputed in FP64, the FP32
+// residual restarts from it and the search direction is kept (beta is formed
+// with the refreshed residual).  The solve stops on the FP64 ||r||/||b|| <= tol.
+// Method: the "reliable update" scheme of Sleijpen & van der Vorst (1996), as
+// used for mixed-precision CG by Clark et al., Comput. Phys. Commun. 181
+// (2010) 1517.
 static int solve_mixed(Ctx &c, const double *b, double *x, int precond, double tol, int maxit,
                        int *iters, double *relres) {
   Poisson &P = c.ps;
@@ -754,57 +819,83 @@ static int solve_mixed(Ctx &c, const double *b, double *x, int precond, double t
   OpT<double> A{P.kx, P.ky, P.diag};
   OpT<float> A32{P.kx32, P.ky32, P.diag32};
   const int pc = precond == COVOF_PRECOND_JACOBI ? 1 : 0;
-  const double eta = 1.0e-4;
-  const int max_outer = 60;
+  const int max_refresh = 4000;
+  const char *mode_env = getenv("COVOF_MIXED_MODE");
+  const bool restart = mode_env && mode_env[0] == 'r' && mode_env[1] == 'e' && mode_env[2] == 's';
+  const char *eta_env = getenv("COVOF_MIXED_ETA");
+  const double eta = eta_env ? atof(eta_env) : 1.0e-4;
+  const char *del_env = getenv("COVOF_MIXED_DELTA");
+  const double delta = del_env ? atof(del_env) : 0.7;
   int st;
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, 32, 0.0);
   if ((st = set_scalar(c, S_TOL, tol))) return st;
+  if ((st = set_scalar(c, I_DELTA, delta))) return st;
   if ((st = set_scalar(c, I_ETA, eta))) return st;
   if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
   float *d0 = P.p32, *d1 = P.p32 + n;
-  int total_inner = 0;
-  for (int outer = 0; outer < max_outer; ++outer) {
-    // x += dx (dx = 0 before the first inner solve), then the FP64 residual
-    if (outer > 0)
+  if ((st = fill<float>(c, d0, n, 0.0f))) return st;
+  double *pb = part_buf(c, 1);
+  int k_next = 1, refreshes = 0, total = 0;
+  for (;;) {
+    // FP64 true residual of x (+ the pending FP32 correction)
+    if (k_next > 1) {
       hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32);
-    if (pc == 1)
-      hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.r32, P.scal,
-                         OUTER, outer == 0, P.part, P.ticket);
-    else
-      hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.r32, P.scal,
-                         OUTER, outer == 0, P.part, P.ticket);
+      if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
+    }
+    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, P.r32, pb);
+    hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks,
+                       P.scal, k_next == 1 ? 1 : 0);
     COVOF_CHECK_LAUNCH();
     if ((st = read_scalars(c, 32))) return st;
-    if (P.hscal[S_FLAG] != 0.0 || total_inner >= maxit) break;
-    // inner FP32 PCG from dx = 0
-    if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
-    if ((st = fill<float>(c, d0, n, 0.0f))) return st;
-    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, I_ITERS, 1, 0.0);
-    if (pc == 1)
-      hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.scal, INNER, P.part, P.ticket);
-    else
-      hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.scal, INNER, P.part, P.ticket);
-    COVOF_CHECK_LAUNCH();
-    int parity = 0, done = 0;
-    const int inner_cap = maxit - total_inner;
-    while (done < inner_cap) {
-      const int chunk = std::min(inner_cap - done, done < 64 ? 32 : (done < 1024 ? 128 : 256));
-      if (pc == 1) st = pcg_iterations<float, 1>(c, A32, P.x32, P.r32, nullptr, d0, d1, P.q32, INNER, chunk, &parity);
-      else st = pcg_iterations<float, 0>(c, A32, P.x32, P.r32, nullptr, d0, d1, P.q32, INNER, chunk, &parity);
-      if (st) return st;
-      done += chunk;
-      if ((st = read_scalars(c, 32))) return st;
-      if (P.hscal[I_FLAG] != 0.0) break;
+    if (P.hscal[S_FLAG] != 0.0 || k_next > maxit || refreshes >= max_refresh) break;
+    if (restart) {
+      if ((st = fill<float>(c, d0, n, 0.0f))) return st;
+      k_next = 1;
     }
-    total_inner += (int)P.hscal[I_ITERS];
-    if (P.hscal[I_FLAG] == F_NONFINITE) break;
+    // restart the FP32 recurrence from the true residual, keeping d
+    if (pc == 1)
+      hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.z32, pb);
+    else
+      hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.z32, pb);
+    if ((st = set_scalar(c, I_FLAG, 0.0))) return st;
+    COVOF_CHECK_LAUNCH();
+    const PcgSlots &isl = restart ? INNER_RESTART : INNER;
+    const int cap = restart ? maxit - total : maxit;
+    if (pc == 1) st = pcg_loop<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, isl, cap, 32, k_next);
+    else st = pcg_loop<float, 0>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, isl, cap, 32, k_next);
+    if (st) return st;
+    double f = P.hscal[I_FLAG];
+    if (restart) {
+      total += (int)P.hscal[I_ITERS];
+      if (f == F_CONV) f = F_REFRESH;
+      k_next = 2;  // any value > 1: fold the correction next time round
+      if (total >= maxit) { k_next = maxit + 1; }
+    } else {
+      k_next = (int)P.hscal[I_ITERS] + 1;
+      total = k_next - 1;
+    }
+    ++refreshes;
+    if (f == F_NONFINITE || f == F_BREAK) {
+      // fold what the FP32 recurrence reached and report from the FP64 residual
+      hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32);
+      hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, P.r32, pb);
+      hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal, 0);
+      COVOF_CHECK_LAUNCH();
+      if ((st = read_scalars(c, 32))) return st;
+      if (P.hscal[S_FLAG] != F_CONV) {
+        if (iters) *iters = total;
+        if (relres) *relres = P.hscal[S_RELRES];
+        return f == F_NONFINITE ? COVOF_NONFINITE : COVOF_BREAKDOWN;
+      }
+      break;
+    }
+    // F_REFRESH, or the iteration cap (loop exits above)
   }
-  if ((st = read_scalars(c, 32))) return st;
-  if (iters) *iters = total_inner;
+  if (iters) *iters = total;
   if (relres) *relres = P.hscal[S_RELRES];
   const double f = P.hscal[S_FLAG];
   if (f == F_CONV) return COVOF_OK;
-  if (f == F_NONFINITE || P.hscal[I_FLAG] == F_NONFINITE) return COVOF_NONFINITE;
+  if (f == F_NONFINITE) return COVOF_NONFINITE;
   return COVOF_NOT_CONVERGED;
 }
 
@@ -843,7 +934,9 @@ int sp_poisson(Ctx &c, const double *u, const double *v, const double *rho_u, co
   const int solve_st = poisson_solve(c, P.b, P.x, lin_solver, precond, tol, maxit, precision, iters, relres);
   if (solve_st == COVOF_BAD_ARGUMENT || solve_st == COVOF_HIP_ERROR) return solve_st;
   const DevOp o = dev_op(m);
-  hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, P.x, P.scal, S_DOT0, P.part, P.ticket);
+  hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, P.x, part_buf(c, 2));
+  hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)part_buf(c, 2), P.nblocks,
+                     P.scal, (int)S_DOT0);
   hipLaunchKernelGGL(k_write_pressure, grd, blk, 0, c.stream, o, P.x, P.scal, S_DOT0, p);
   hipLaunchKernelGGL(k_wrap_halo, dim3(1), dim3(1024), 0, c.stream, m.nx, m.ny, m.perx, m.pery, p);
   COVOF_CHECK_LAUNCH();

@@ -7,25 +7,28 @@
 // (src/module_mpi_comm.f90:46-56, :636-683), update_physical_properties
 // (src/module_interface_vof.f90:479-616).
 //
-// Pipeline of one directional pass (src -> dst, both halo layout):
-//   k_recon_list   reconstruction of the MIXED interior cells only, taken from
-//                  a compacted list (full/empty cells have the fixed segment
-//                  n = (1,0), alpha = +-1e9 and are never stored).  ELVIRA
-//                  spreads the 7 candidates of a cell over 8 lanes.
-//   k_ghost        one workgroup: ghost-normal exchange / outward copy and
-//                  re-inversion of alpha in mixed ghost cells.
+// Pipeline of one directional pass (src -> dst, both halo layout), two
+// launches:
+//   k_recon        one block per 64x16 tile reconstructs the tile's MIXED
+//                  cells from the tile's list (full/empty cells have the fixed
+//                  segment n = (1,0), alpha = +-1e9 and are never stored);
+//                  extra blocks reconstruct the ghost segments of the halo
+//                  ring (ghost-normal exchange / outward copy of
+//                  reconstruct_interface in closed form, alpha re-inverted in
+//                  the ghost box).  ELVIRA spreads the 7 candidates of a cell
+//                  over 8 lanes.
 //   k_flux_update  per interior cell: the two face fluxes (cut_box_area of the
 //                  donor segment over the swept box) and the conservative
-//                  update with the frozen dilation term; writes dst, the
-//                  1-byte dilation indicator (first pass) and appends mixed
-//                  cells of dst to the list of the next pass.
-//   k_halo         one workgroup: dst's halo ring = src's ring, then the
-//                  exchange / VOF-BC sequence, then (two-phase, after the
-//                  second pass) the physical-corner fix.
+//                  update with the frozen dilation term; writes dst, its halo
+//                  (exchange + VOF BC in closed form), the 1-byte dilation
+//                  indicator (first pass) and the tile's list of mixed cells
+//                  of dst, compacted in LDS, for the next reconstruction.
 // Segments of mixed cells live in full-size arrays that are written and read
 // only at mixed cells, so HBM traffic stays with the streaming C/u/v fields.
 #include "covof_internal.h"
 #include "geometry.h"
+
+#include <algorithm>
 
 namespace covof {
 
@@ -136,84 +139,47 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
 #undef CC
 }
 
-// PLIC over the list: one lane per mixed cell.
-__global__ void k_recon_plic(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
-                             const int *__restrict__ count, int slot, int *__restrict__ count_clear,
-                             double *snx, double *sny, double *sal) {
-  const int n = count[slot];
-  if (blockIdx.x == 0 && threadIdx.x == 0) count_clear[0] = 0;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    const int k = list[t];
-    const int j = k / (m.nx + 2), i = k - j * (m.nx + 2);
-    double a, b, c;
-    plic_mixed(C, m, i, j, C[k], a, b, c);
-    snx[k] = a;
-    sny[k] = b;
-    sal[k] = c;
-  }
+// ------------------------------------------------------------------ tiles
+//
+// Cells are processed in tiles of TX x TY = 64 x 16 (1024 cells, 256 threads,
+// 4 rows per thread).  The update kernel of a pass leaves, per tile, the list
+// of its mixed cells (block-local compaction through LDS: no global atomics)
+// for the next reconstruction, which runs one block per tile.
+constexpr int TX = 64, TY = 16, TCELLS = TX * TY, TBLK = 256;
+
+__device__ __forceinline__ int tiles_x(const DevMesh &m) { return (m.nx + TX - 1) / TX; }
+__device__ __forceinline__ int tiles_y(const DevMesh &m) { return (m.ny + TY - 1) / TY; }
+
+// Appends cell k (if pred) to the block's LDS list; returns nothing.
+__device__ __forceinline__ void lds_append(bool pred, int k, int *s_list, int *s_count) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(s_count, __popcll(mask));
+  base = __shfl(base, leader);
+  if (pred) s_list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
 }
 
-// ELVIRA over the list: 8 lanes per mixed cell, lane l < 7 evaluates
-// candidate l; the winner is the first candidate reaching the smallest error
-// below 1e30 (the reference's strict '<' scan from best_err = 1e30).
-__global__ void k_recon_elvira(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
-                               const int *__restrict__ count, int slot, int *__restrict__ count_clear,
-                               double *snx, double *sny, double *sal) {
-  const int n = count[slot];
-  if (blockIdx.x == 0 && threadIdx.x == 0) count_clear[0] = 0;
-  const int lane8 = threadIdx.x & 7;
-  const int groups_per_grid = (gridDim.x * blockDim.x) >> 3;
-  for (int g = (blockIdx.x * blockDim.x + threadIdx.x) >> 3; g - ((threadIdx.x & 63) >> 3) < n;
-       g += groups_per_grid) {
-    // every lane of the wave stays in the loop while any group of the wave has work
-    const bool active = g < n;
-    int k = 0, i = 0, j = 0;
-    double ux = 1.0, uy = 0.0, al = 0.0, err = 0.0;
-    bool valid = false;
-    if (active) {
-      k = list[g];
-      j = k / (m.nx + 2);
-      i = k - j * (m.nx + 2);
-      if (lane8 < 7) elvira_candidate(C, m, i, j, C[k], lane8, ux, uy, al, err, valid);
-    }
-    // key: error (or +inf if not a contender), ties by candidate index
-    double key = (valid && err < 1.0e30) ? err : __builtin_inf();
-    int idx = lane8;
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
-      const double ok = __shfl_xor(key, off, 8);
-      const int oi = __shfl_xor(idx, off, 8);
-      if (ok < key || (ok == key && oi < idx)) {
-        key = ok;
-        idx = oi;
-      }
-    }
-    const double bx = __shfl(ux, idx, 8), by = __shfl(uy, idx, 8), ba = __shfl(al, idx, 8);
-    if (active && lane8 == 0) {
-      if (key == __builtin_inf()) {
-        snx[k] = 1.0;
-        sny[k] = 0.0;
-        sal[k] = 0.0;
-      } else {
-        snx[k] = bx;
-        sny[k] = by;
-        sal[k] = ba;
-      }
-    }
-  }
+// ------------------------------------------------------------ ghost sources
+//
+// Ghost-cell normals of reconstruct_interface (:602-681): the halo exchange
+// of seg_nx / seg_ny followed by the outward copies at physical edges gives
+// every ring cell (i,j) the normal of one interior cell, in closed form:
+// periodic directions wrap (0 -> n, n+1 -> 1), physical ones clamp (0 -> 1,
+// n+1 -> n).  Exception: periodic in y only, the four corners keep the zero
+// the halo was initialised with (no copy reaches them).
+__device__ __forceinline__ int wrap_or_clamp(int i, int n, int per) {
+  if (i < 1) return per ? n : 1;
+  if (i > n) return per ? 1 : n;
+  return i;
+}
+__device__ __forceinline__ bool ghost_zero(const DevMesh &m, int i, int j) {
+  return !m.perx && m.pery && (i == 0 || i == m.nx + 1) && (j == 0 || j == m.ny + 1);
 }
 
-// ----------------------------------------------------------- ghost segments
-
-// Halo ring of the (0:nx+1, 0:ny+1) grid: rows j = 0 and j = ny+1 (nx+2 each),
-// then columns i = 0 and i = nx+1 for j = 1..ny.
 __device__ __forceinline__ int ring_len(const DevMesh &m) { return 2 * (m.nx + 2) + 2 * m.ny; }
-__device__ __forceinline__ int ring_of(const DevMesh &m, int i, int j) {
-  if (j == 0) return i;
-  if (j == m.ny + 1) return (m.nx + 2) + i;
-  if (i == 0) return 2 * (m.nx + 2) + (j - 1);
-  return 2 * (m.nx + 2) + m.ny + (j - 1);
-}
 __device__ __forceinline__ void ring_cell(const DevMesh &m, int r, int &i, int &j) {
   const int W = m.nx + 2;
   if (r < W) { i = r; j = 0; }
@@ -221,96 +187,147 @@ __device__ __forceinline__ void ring_cell(const DevMesh &m, int r, int &i, int &
   else if (r < 2 * W + m.ny) { i = 0; j = r - 2 * W + 1; }
   else { i = m.nx + 1; j = r - 2 * W - m.ny + 1; }
 }
-__device__ __forceinline__ bool on_ring(const DevMesh &m, int i, int j) {
-  return i == 0 || j == 0 || i == m.nx + 1 || j == m.ny + 1;
-}
 
-// Normal of an interior cell as stored in the reference's segments array.
-__device__ __forceinline__ void interior_normal(const double *C, const DevMesh &m, const double *snx,
-                                                const double *sny, int i, int j, double &gx,
-                                                double &gy) {
+// Segment of a mixed ghost cell from the (unnormalised) source normal.
+__device__ void ghost_segment(const DevMesh &m, int i, int j, double c, double gx, double gy,
+                              double *snx, double *sny, double *sal) {
+  const double nr = hypot_ref(gx, gy);
+  double sx = 1.0, sy = 0.0;  // the reconstruction's own ghost value when |n| <= 1e-12
+  if (nr > 1.0e-12) {
+    sx = gx / nr;
+    sy = gy / nr;
+  }
   const long k = H(m, i, j);
-  if (is_mixed(C[k])) {
-    gx = snx[k];
-    gy = sny[k];
+  snx[k] = sx;
+  sny[k] = sy;
+  sal[k] = compute_alpha(face_coord(i, m.dx), face_coord(i + 1, m.dx), face_coord(j, m.dy),
+                         face_coord(j + 1, m.dy), sx, sy, c);
+}
+
+// --------------------------------------------------------- reconstruction
+
+// METHOD 0 = PLIC, 1 = ELVIRA.  Blocks [0, ntiles) reconstruct their tile's
+// mixed cells from the tile list; the blocks after them handle the halo ring:
+// for every mixed ring cell the normal of its source cell is reconstructed
+// again (the same arithmetic), renormalised and alpha re-inverted in the
+// ghost box.  ELVIRA spreads the 7 candidates of a cell over 8 lanes; the
+// winner is the first candidate reaching the smallest error below 1e30 (the
+// reference's strict '<' scan from best_err = 1e30).
+template <int METHOD>
+__global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
+                                                const int *__restrict__ count, double *snx, double *sny,
+                                                double *sal) {
+  const int ntiles = tiles_x(m) * tiles_y(m);
+  const int b = blockIdx.x;
+  const bool ring = b >= ntiles;
+  int n;
+  if (!ring) {
+    n = count[b];
+    if (n == 0) return;
   } else {
-    gx = 1.0;
-    gy = 0.0;
+    n = ring_len(m);
+  }
+  const int per_pass = (METHOD == 1) ? TBLK / 8 : TBLK;
+  const int stride = ring ? per_pass * (gridDim.x - ntiles) : per_pass;
+  const int first = ring ? per_pass * (b - ntiles) : 0;
+  const int lane8 = threadIdx.x & 7;
+  const int slot0 = (METHOD == 1) ? (threadIdx.x >> 3) : threadIdx.x;
+  // wave-uniform trip count: every lane of a wave stays while any has work
+  const int wave_first = (METHOD == 1) ? ((threadIdx.x & ~63) >> 3) : (threadIdx.x & ~63);
+  for (int e = first + slot0, ew = first + wave_first; ew < n; e += stride, ew += stride) {
+    const bool active = e < n;
+    int i = 0, j = 0, si = 0, sj = 0;
+    bool work = false, zero = false;
+    double c = 0.0;
+    if (active) {
+      if (!ring) {
+        const int k = list[(long)b * TCELLS + e];
+        j = k / (m.nx + 2);
+        i = k - j * (m.nx + 2);
+        si = i;
+        sj = j;
+        c = C[k];
+        work = true;
+      } else {
+        ring_cell(m, e, i, j);
+        c = C[H(m, i, j)];
+        if (is_mixed(c)) {
+          work = true;
+          zero = ghost_zero(m, i, j);
+          si = wrap_or_clamp(i, m.nx, m.perx);
+          sj = wrap_or_clamp(j, m.ny, m.pery);
+        }
+      }
+    }
+    // normal (and alpha for interior cells) of the source cell (si, sj)
+    const double cs = (work && !zero) ? C[H(m, si, sj)] : 0.0;
+    const bool src_mixed = work && !zero && is_mixed(cs);
+    double bx = 1.0, by = 0.0, ba = 0.0;
+    if (METHOD == 0) {
+      if (src_mixed) plic_mixed(C, m, si, sj, cs, bx, by, ba);
+    } else {
+      double ux = 1.0, uy = 0.0, al = 0.0, err = 0.0;
+      bool valid = false;
+      if (src_mixed && lane8 < 7) elvira_candidate(C, m, si, sj, cs, lane8, ux, uy, al, err, valid);
+      double key = (valid && err < 1.0e30) ? err : __builtin_inf();
+      int idx = lane8;
+#pragma unroll
+      for (int off = 1; off < 8; off <<= 1) {
+        const double ok = __shfl_xor(key, off, 8);
+        const int oi = __shfl_xor(idx, off, 8);
+        if (ok < key || (ok == key && oi < idx)) {
+          key = ok;
+          idx = oi;
+        }
+      }
+      bx = __shfl(ux, idx, 8);
+      by = __shfl(uy, idx, 8);
+      ba = __shfl(al, idx, 8);
+      if (key == __builtin_inf()) {
+        bx = 1.0;
+        by = 0.0;
+        ba = 0.0;
+      }
+      if (lane8 != 0) work = false;
+    }
+    if (!work) continue;
+    if (!ring) {
+      const long k = H(m, i, j);
+      snx[k] = bx;
+      sny[k] = by;
+      sal[k] = ba;
+    } else {
+      double gx = bx, gy = by;
+      if (zero) {
+        gx = 0.0;
+        gy = 0.0;
+      } else if (!src_mixed) {
+        gx = 1.0;
+        gy = 0.0;
+      }
+      ghost_segment(m, i, j, c, gx, gy, snx, sny, sal);
+    }
   }
 }
 
-// reconstruct_interface ghost handling (:602-681) in one workgroup.  gx/gy
-// hold seg_nx/seg_ny on the ring (interior values are read from the segment
-// arrays); the phases follow the reference's statement order.
-__global__ void __launch_bounds__(1024) k_ghost(const double *__restrict__ C, DevMesh m,
-                                                double *snx, double *sny, double *sal,
-                                                double *gx, double *gy) {
-  const int L = ring_len(m), nx = m.nx, ny = m.ny;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  auto get = [&](int i, int j, double &a, double &b) {
-    if (on_ring(m, i, j)) {
-      const int r = ring_of(m, i, j);
-      a = gx[r];
-      b = gy[r];
-    } else {
-      interior_normal(C, m, snx, sny, i, j, a, b);
-    }
-  };
-  auto put = [&](int i, int j, double a, double b) {
-    const int r = ring_of(m, i, j);
-    gx[r] = a;
-    gy[r] = b;
-  };
-  for (int r = tid; r < L; r += nt) { gx[r] = 0.0; gy[r] = 0.0; }
+// Mixed interior cells of C -> per-tile lists (first pass of a call).
+__global__ void __launch_bounds__(TBLK) k_classify(const double *__restrict__ C, DevMesh m, int *list, int *count) {
+  __shared__ int s_list[TCELLS];
+  __shared__ int s_count;
+  if (threadIdx.x == 0) s_count = 0;
   __syncthreads();
-  if (m.perx) {  // exchange_halos_scalar, x
-    for (int j = tid; j <= ny + 1; j += nt) { double a, b; get(nx, j, a, b); put(0, j, a, b); }
-    __syncthreads();
-    for (int j = tid; j <= ny + 1; j += nt) { double a, b; get(1, j, a, b); put(nx + 1, j, a, b); }
-    __syncthreads();
+  const int tx = blockIdx.x % tiles_x(m), ty = blockIdx.x / tiles_x(m);
+  const int i = tx * TX + (threadIdx.x & 63) + 1;
+  for (int r = 0; r < TY / 4; ++r) {
+    const int j = ty * TY + (threadIdx.x >> 6) + 4 * r + 1;
+    const bool inside = i <= m.nx && j <= m.ny;
+    const long k = inside ? H(m, i, j) : 0;
+    lds_append(inside && is_mixed(C[k]), (int)k, s_list, &s_count);
   }
-  if (m.pery) {  // exchange_halos_scalar, y
-    for (int i = tid; i <= nx + 1; i += nt) { double a, b; get(i, ny, a, b); put(i, 0, a, b); }
-    __syncthreads();
-    for (int i = tid; i <= nx + 1; i += nt) { double a, b; get(i, 1, a, b); put(i, ny + 1, a, b); }
-    __syncthreads();
-  }
-  if (!m.perx) {  // is_boundary_left / right: copy outward
-    for (int j = 1 + tid; j <= ny; j += nt) { double a, b; get(1, j, a, b); put(0, j, a, b); }
-    __syncthreads();
-    for (int j = 1 + tid; j <= ny; j += nt) { double a, b; get(nx, j, a, b); put(nx + 1, j, a, b); }
-    __syncthreads();
-  }
-  if (!m.pery) {  // is_boundary_bottom / top
-    for (int i = tid; i <= nx + 1; i += nt) { double a, b; get(i, 1, a, b); put(i, 0, a, b); }
-    __syncthreads();
-    for (int i = tid; i <= nx + 1; i += nt) { double a, b; get(i, ny, a, b); put(i, ny + 1, a, b); }
-    __syncthreads();
-  }
-  // segments of ghost cells: full / empty keep (1,0,+-1e9) (never stored);
-  // mixed ones take the renormalised ring normal and a re-inverted alpha, or
-  // the reconstruction's own ghost value n = (1,0) when the norm is tiny.
-  for (int r = tid; r < L; r += nt) {
-    int i, j;
-    ring_cell(m, r, i, j);
-    const long k = H(m, i, j);
-    const double c = C[k];
-    if (!is_mixed(c)) continue;
-    const double xmin = face_coord(i, m.dx), xmax = face_coord(i + 1, m.dx);
-    const double ymin = face_coord(j, m.dy), ymax = face_coord(j + 1, m.dy);
-    double sx = gx[r], sy = gy[r];
-    const double nr = hypot_ref(sx, sy);
-    if (nr > 1.0e-12) {
-      sx = sx / nr;
-      sy = sy / nr;
-    } else {
-      sx = 1.0;
-      sy = 0.0;
-    }
-    snx[k] = sx;
-    sny[k] = sy;
-    sal[k] = compute_alpha(xmin, xmax, ymin, ymax, sx, sy, c);
-  }
+  __syncthreads();
+  const int n = s_count;
+  for (int e = threadIdx.x; e < n; e += TBLK) list[(long)blockIdx.x * TCELLS + e] = s_list[e];
+  if (threadIdx.x == 0) count[blockIdx.x] = n;
 }
 
 // ------------------------------------------------------------ flux + update
@@ -352,163 +369,134 @@ __device__ __forceinline__ double face_flux(const double *__restrict__ C, const 
   return uf > 0.0 ? a : -a;
 }
 
-// Wave-aggregated append of cell k to a list.
-__device__ __forceinline__ void list_append(bool pred, int k, int *list, int *count) {
-  const unsigned long long mask = __ballot(pred);
-  if (mask == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(mask));
-  base = __shfl(base, leader);
-  if (pred) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = k;
+// Halo of dst after one exchange_halos_scalar + apply_vof_bc on one rank
+// (src/module_mpi_comm.f90:46-56, :636-683), in closed form: the edge halo
+// cells copy one interior cell (wrap for periodic directions, zero-gradient
+// copy otherwise), which the thread computing that interior value writes.
+// The corners depend on the statement order: fully periodic they take the
+// diagonally opposite interior cell; periodic in one direction only they take
+// a halo value of src (the copy made before the boundary conditions ran);
+// with no periodic direction they keep src's value, or with the two-phase
+// physical-corner fix of update_physical_properties (:532-535) the diagonal
+// interior cell.
+__device__ __forceinline__ void write_halo(double *dst, const double *src, const DevMesh &m, int i, int j,
+                                           double V, bool corner_fix) {
+  const int nx = m.nx, ny = m.ny;
+  if (i == (m.perx ? nx : 1)) dst[H(m, 0, j)] = V;
+  if (i == (m.perx ? 1 : nx)) dst[H(m, nx + 1, j)] = V;
+  if (j == (m.pery ? ny : 1)) dst[H(m, i, 0)] = V;
+  if (j == (m.pery ? 1 : ny)) dst[H(m, i, ny + 1)] = V;
+  if (m.perx && m.pery) {
+    if (i == nx && j == ny) dst[H(m, 0, 0)] = V;
+    if (i == 1 && j == ny) dst[H(m, nx + 1, 0)] = V;
+    if (i == nx && j == 1) dst[H(m, 0, ny + 1)] = V;
+    if (i == 1 && j == 1) dst[H(m, nx + 1, ny + 1)] = V;
+  } else if (!m.perx && !m.pery && corner_fix) {
+    if (i == 1 && j == 1) dst[H(m, 0, 0)] = V;
+    if (i == nx && j == 1) dst[H(m, nx + 1, 0)] = V;
+    if (i == 1 && j == ny) dst[H(m, 0, ny + 1)] = V;
+    if (i == nx && j == ny) dst[H(m, nx + 1, ny + 1)] = V;
+  } else if (i == 1 && j == 1) {
+    double c00, c10, c01, c11;  // (0,0), (nx+1,0), (0,ny+1), (nx+1,ny+1)
+    if (m.perx && !m.pery) {
+      c00 = src[H(m, nx, 0)];
+      c10 = src[H(m, 1, 0)];
+      c01 = src[H(m, nx, ny + 1)];
+      c11 = src[H(m, 1, ny + 1)];
+    } else if (!m.perx && m.pery) {
+      c00 = src[H(m, 0, ny)];
+      c10 = src[H(m, nx + 1, ny)];
+      c01 = src[H(m, 0, 1)];
+      c11 = src[H(m, nx + 1, 1)];
+    } else {
+      c00 = src[H(m, 0, 0)];
+      c10 = src[H(m, nx + 1, 0)];
+      c01 = src[H(m, 0, ny + 1)];
+      c11 = src[H(m, nx + 1, ny + 1)];
+    }
+    dst[H(m, 0, 0)] = c00;
+    dst[H(m, nx + 1, 0)] = c10;
+    dst[H(m, 0, ny + 1)] = c01;
+    dst[H(m, nx + 1, ny + 1)] = c11;
+  }
 }
 
-// DIR 0: x-pass (Eq. 15), DIR 1: y-pass (Eq. 16).  FIRST: first pass of the
-// step (dilation indicator from src = C^n, written to cd); later pass reads cd
-// and clamps to [0,1] (:1234-1239 / :1276-1283).
+// One directional pass on a tile.  DIR 0: x-pass (Eq. 15), DIR 1: y-pass
+// (Eq. 16).  FIRST: first pass of the step (dilation indicator from src = C^n,
+// written to cd); the second pass reads cd and clamps to [0,1]
+// (:1234-1239 / :1276-1283).  Writes dst's interior and halo, and the tile's
+// list of mixed cells of dst for the next reconstruction.
 template <int DIR, bool FIRST>
-__global__ void __launch_bounds__(256) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
-                                                     const double *__restrict__ vel, DevMesh m, double dt,
-                                                     const double *__restrict__ snx, const double *__restrict__ sny,
-                                                     const double *__restrict__ sal, uint8_t *__restrict__ cd,
-                                                     int *__restrict__ list_next, int *__restrict__ count_next) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const bool inside = i <= m.nx && j <= m.ny;
-  long k = 0;
-  double cn = 0.0;
-  if (inside) {
-    k = H(m, i, j);
-    const double c0 = src[k];
-    double flo, fhi, du, h;
-    if (DIR == 0) {
-      if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx)) {
-        // :1223-1231: the boundary face keeps the flux computed with the real donor
-        const double fb = (vel[H(m, 1, j)] >= 0.0) ? face_flux<0>(src, vel, m, snx, sny, sal, m.nx + 1, j, dt)
-                                                    : face_flux<0>(src, vel, m, snx, sny, sal, 1, j, dt);
-        flo = (i == 1) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
-        fhi = (i == m.nx) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
+__global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
+                                                      const double *__restrict__ vel, DevMesh m, double dt,
+                                                      const double *__restrict__ snx, const double *__restrict__ sny,
+                                                      const double *__restrict__ sal, uint8_t *__restrict__ cd,
+                                                      int *__restrict__ list, int *__restrict__ count,
+                                                      int corner_fix) {
+  __shared__ int s_list[TCELLS];
+  __shared__ int s_count;
+  if (threadIdx.x == 0) s_count = 0;
+  __syncthreads();
+  const int tx = blockIdx.x % tiles_x(m), ty = blockIdx.x / tiles_x(m);
+  const int i = tx * TX + (threadIdx.x & 63) + 1;
+  const double area = m.dx * m.dy;
+  for (int r = 0; r < TY / 4; ++r) {
+    const int j = ty * TY + (threadIdx.x >> 6) + 4 * r + 1;
+    const bool inside = i <= m.nx && j <= m.ny;
+    long k = 0;
+    double cn = 0.0;
+    if (inside) {
+      k = H(m, i, j);
+      const double c0 = src[k];
+      double flo, fhi, du, h;
+      if (DIR == 0) {
+        if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx)) {
+          // :1223-1231: the boundary face keeps the flux computed with the real donor
+          const double fb = (vel[H(m, 1, j)] >= 0.0) ? face_flux<0>(src, vel, m, snx, sny, sal, m.nx + 1, j, dt)
+                                                      : face_flux<0>(src, vel, m, snx, sny, sal, 1, j, dt);
+          flo = (i == 1) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
+          fhi = (i == m.nx) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
+        } else {
+          flo = face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
+          fhi = face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
+        }
+        du = vel[H(m, i + 1, j)] - vel[k];
+        h = m.dx;
       } else {
-        flo = face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
-        fhi = face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
+        if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny)) {
+          const double fb = (vel[H(m, i, 1)] >= 0.0) ? face_flux<1>(src, vel, m, snx, sny, sal, i, m.ny + 1, dt)
+                                                      : face_flux<1>(src, vel, m, snx, sny, sal, i, 1, dt);
+          flo = (j == 1) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
+          fhi = (j == m.ny) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
+        } else {
+          flo = face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
+          fhi = face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
+        }
+        du = vel[H(m, i, j + 1)] - vel[k];
+        h = m.dy;
       }
-      du = vel[H(m, i + 1, j)] - vel[k];
-      h = m.dx;
-    } else {
-      if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny)) {
-        const double fb = (vel[H(m, i, 1)] >= 0.0) ? face_flux<1>(src, vel, m, snx, sny, sal, i, m.ny + 1, dt)
-                                                    : face_flux<1>(src, vel, m, snx, sny, sal, i, 1, dt);
-        flo = (j == 1) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
-        fhi = (j == m.ny) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
+      double cdv;
+      if (FIRST) {
+        const bool bb = c0 >= 0.5;
+        cd[k] = bb ? 1 : 0;
+        cdv = bb ? 1.0 : 0.0;
       } else {
-        flo = face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
-        fhi = face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
+        cdv = cd[k] ? 1.0 : 0.0;
       }
-      du = vel[H(m, i, j + 1)] - vel[k];
-      h = m.dy;
+      cn = c0 - (fhi - flo) / area + cdv * (dt / h) * du;
+      if (!FIRST) {
+        const double t = 1.0 < cn ? 1.0 : cn;  // min(1, c)
+        cn = 0.0 > t ? 0.0 : t;                // max(0, .)
+      }
+      dst[k] = cn;
+      if (i == 1 || j == 1 || i == m.nx || j == m.ny) write_halo(dst, src, m, i, j, cn, corner_fix != 0);
     }
-    double cdv;
-    if (FIRST) {
-      const bool b = c0 >= 0.5;
-      cd[k] = b ? 1 : 0;
-      cdv = b ? 1.0 : 0.0;
-    } else {
-      cdv = cd[k] ? 1.0 : 0.0;
-    }
-    const double area = m.dx * m.dy;
-    cn = c0 - (fhi - flo) / area + cdv * (dt / h) * du;
-    if (!FIRST) {
-      const double t = 1.0 < cn ? 1.0 : cn;  // min(1, c)
-      cn = 0.0 > t ? 0.0 : t;                // max(0, .)
-    }
-    dst[k] = cn;
-  }
-  list_append(inside && is_mixed(cn), (int)k, list_next, count_next);
-}
-
-// Mixed interior cells of C -> list (first pass of a call).
-__global__ void k_classify(const double *__restrict__ C, DevMesh m, int *list, int *count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const bool inside = i <= m.nx && j <= m.ny;
-  const long k = inside ? H(m, i, j) : 0;
-  list_append(inside && is_mixed(C[k]), (int)k, list, count);
-}
-
-__global__ void k_zero_int(int *p, int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) p[t] = 0;
-}
-
-// ------------------------------------------------------------------ halo ring
-
-// exchange_halos_scalar (src/module_mpi_comm.f90:46-56) then apply_vof_bc
-// (:636-683) on one rank, as consecutive phases of one workgroup.
-__device__ void halo_sequence(double *F, const DevMesh &m) {
-  const int nx = m.nx, ny = m.ny, tid = threadIdx.x, nt = blockDim.x;
-  if (m.perx) {
-    for (int j = tid; j <= ny + 1; j += nt) F[H(m, 0, j)] = F[H(m, nx, j)];
-    __syncthreads();
-    for (int j = tid; j <= ny + 1; j += nt) F[H(m, nx + 1, j)] = F[H(m, 1, j)];
-    __syncthreads();
-  }
-  if (m.pery) {
-    for (int i = tid; i <= nx + 1; i += nt) F[H(m, i, 0)] = F[H(m, i, ny)];
-    __syncthreads();
-    for (int i = tid; i <= nx + 1; i += nt) F[H(m, i, ny + 1)] = F[H(m, i, 1)];
-    __syncthreads();
-  }
-  if (m.bcl == BC_PERIODIC) {
-    for (int j = 1 + tid; j <= ny; j += nt) F[H(m, 0, j)] = F[H(m, nx, j)];
-    __syncthreads();
-    for (int j = 1 + tid; j <= ny; j += nt) F[H(m, nx + 1, j)] = F[H(m, 1, j)];
-    __syncthreads();
-  } else if (!m.perx) {
-    for (int j = 1 + tid; j <= ny; j += nt) F[H(m, 0, j)] = F[H(m, 1, j)];
-    __syncthreads();
-    for (int j = 1 + tid; j <= ny; j += nt) F[H(m, nx + 1, j)] = F[H(m, nx, j)];
-    __syncthreads();
-  }
-  if (m.bcb == BC_PERIODIC) {
-    for (int i = 1 + tid; i <= nx; i += nt) F[H(m, i, 0)] = F[H(m, i, ny)];
-    __syncthreads();
-    for (int i = 1 + tid; i <= nx; i += nt) F[H(m, i, ny + 1)] = F[H(m, i, 1)];
-    __syncthreads();
-  } else if (!m.pery) {
-    for (int i = 1 + tid; i <= nx; i += nt) F[H(m, i, 0)] = F[H(m, i, 1)];
-    __syncthreads();
-  }
-  if (!m.pery) {
-    for (int i = 1 + tid; i <= nx; i += nt) F[H(m, i, ny + 1)] = F[H(m, i, ny)];
-    __syncthreads();
-  }
-}
-
-// dst's ring starts as src's (c_star = C, :1233 / c_final = c_star, :1275),
-// then one exchange + VOF BC: inside the first pass (:1240-1243) and, after the
-// second, in advect_vof (:809-810).  CORNERS: two-phase physical-corner fix of
-// update_physical_properties (:532-535).
-__global__ void __launch_bounds__(1024) k_halo(const double *__restrict__ src, double *dst, DevMesh m,
-                                               int corners) {
-  const int L = ring_len(m);
-  for (int r = threadIdx.x; r < L; r += blockDim.x) {
-    int i, j;
-    ring_cell(m, r, i, j);
-    dst[H(m, i, j)] = src[H(m, i, j)];
+    lds_append(inside && is_mixed(cn), (int)k, s_list, &s_count);
   }
   __syncthreads();
-  halo_sequence(dst, m);
-  if (corners && threadIdx.x == 0) {
-    const int nx = m.nx, ny = m.ny;
-    const bool bx = !m.perx, by = !m.pery;
-    if (bx && by) {
-      dst[H(m, 0, 0)] = dst[H(m, 1, 1)];
-      dst[H(m, nx + 1, 0)] = dst[H(m, nx, 1)];
-      dst[H(m, 0, ny + 1)] = dst[H(m, 1, ny)];
-      dst[H(m, nx + 1, ny + 1)] = dst[H(m, nx, ny)];
-    }
-  }
+  const int n = s_count;
+  for (int e = threadIdx.x; e < n; e += TBLK) list[(long)blockIdx.x * TCELLS + e] = s_list[e];
+  if (threadIdx.x == 0) count[blockIdx.x] = n;
 }
 
 // ------------------------------------------------------------ properties
@@ -599,19 +587,19 @@ __global__ void k_expand_segments(const double *__restrict__ C, DevMesh m, const
 
 // ================================================================= host side
 
+static int n_tiles(const Mesh &m) { return ((m.nx + TX - 1) / TX) * ((m.ny + TY - 1) / TY); }
+
 int vof_alloc(Ctx &c) {
   const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
-  const long L = 2L * (c.m.nx + 2) + 2L * c.m.ny;
+  const long nt = n_tiles(c.m);
   COVOF_CHECK(hipMalloc(&c.vof.Cb, nh * sizeof(double)));
   COVOF_CHECK(hipMalloc(&c.vof.snx, nh * sizeof(double)));
   COVOF_CHECK(hipMalloc(&c.vof.sny, nh * sizeof(double)));
   COVOF_CHECK(hipMalloc(&c.vof.sal, nh * sizeof(double)));
   COVOF_CHECK(hipMalloc(&c.vof.cd, nh));
-  COVOF_CHECK(hipMalloc(&c.vof.list[0], nh * sizeof(int)));
-  COVOF_CHECK(hipMalloc(&c.vof.list[1], nh * sizeof(int)));
-  COVOF_CHECK(hipMalloc(&c.vof.count, 4 * sizeof(int)));
-  COVOF_CHECK(hipMalloc(&c.vof.ring, 2 * L * sizeof(double)));
-  COVOF_CHECK(hipMemset(c.vof.count, 0, 4 * sizeof(int)));
+  COVOF_CHECK(hipMalloc(&c.vof.list[0], nt * TCELLS * sizeof(int)));
+  COVOF_CHECK(hipMalloc(&c.vof.count, nt * sizeof(int)));
+  COVOF_CHECK(hipMemset(c.vof.count, 0, nt * sizeof(int)));
   COVOF_CHECK(hipMemset(c.vof.cd, 0, nh));
   return COVOF_OK;
 }
@@ -623,74 +611,59 @@ void vof_free(Ctx &c) {
   hipFree(c.vof.sal);
   hipFree(c.vof.cd);
   hipFree(c.vof.list[0]);
-  hipFree(c.vof.list[1]);
   hipFree(c.vof.count);
-  hipFree(c.vof.ring);
 }
 
-static int recon_grid_blocks() { return 2048; }
-
-// Reconstruction of src for one pass: list[slot] holds its mixed interior
-// cells; the kernel clears count[1-slot] for the pass's update to append to.
-static int launch_recon(Ctx &c, int method, const double *src, int slot) {
+// Reconstruction of src: tile lists (left by the previous update or by
+// k_classify) plus the halo ring.
+static int launch_recon(Ctx &c, int method, const double *src) {
   const DevMesh m = dev_mesh(c.m);
-  const long L = 2L * (c.m.nx + 2) + 2L * c.m.ny;
+  const int nt = n_tiles(c.m);
+  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
+  const int per_block = method == 1 ? TBLK / 8 : TBLK;
+  const int ring_blocks = std::min(256, (L + per_block - 1) / per_block);
+  const dim3 grid(nt + ring_blocks);
   if (method == 1)
-    hipLaunchKernelGGL(k_recon_elvira, dim3(recon_grid_blocks()), dim3(256), 0, c.stream, src, m,
-                       c.vof.list[slot], c.vof.count, slot, c.vof.count + (1 - slot), c.vof.snx,
-                       c.vof.sny, c.vof.sal);
+    hipLaunchKernelGGL((k_recon<1>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
+                       c.vof.snx, c.vof.sny, c.vof.sal);
   else
-    hipLaunchKernelGGL(k_recon_plic, dim3(recon_grid_blocks()), dim3(256), 0, c.stream, src, m,
-                       c.vof.list[slot], c.vof.count, slot, c.vof.count + (1 - slot), c.vof.snx,
-                       c.vof.sny, c.vof.sal);
-  COVOF_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ghost, dim3(1), dim3(1024), 0, c.stream, src, m, c.vof.snx, c.vof.sny,
-                     c.vof.sal, c.vof.ring, c.vof.ring + L);
+    hipLaunchKernelGGL((k_recon<0>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
+                       c.vof.snx, c.vof.sny, c.vof.sal);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
 
-static dim3 cell_grid(const Mesh &m, dim3 blk) {
-  return dim3((m.nx + blk.x - 1) / blk.x, (m.ny + blk.y - 1) / blk.y);
-}
-
-static int launch_classify(Ctx &c, const double *C, int slot) {
-  const DevMesh m = dev_mesh(c.m);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c.stream, c.vof.count + slot, 1);
-  COVOF_CHECK_LAUNCH();
-  const dim3 blk(64, 4);
-  hipLaunchKernelGGL(k_classify, cell_grid(c.m, blk), blk, 0, c.stream, C, m, c.vof.list[slot],
-                     c.vof.count + slot);
+static int launch_classify(Ctx &c, const double *C) {
+  hipLaunchKernelGGL(k_classify, dim3(n_tiles(c.m)), dim3(TBLK), 0, c.stream, C, dev_mesh(c.m), c.vof.list[0],
+                     c.vof.count);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
 
 // One directional pass src -> dst.  dir 0 = x, 1 = y.
 static int launch_pass(Ctx &c, int method, int dir, bool first, const double *src, double *dst,
-                       const double *u, const double *v, double dt, int slot, bool corners) {
-  int st = launch_recon(c, method, src, slot);
+                       const double *u, const double *v, double dt, bool corner_fix) {
+  int st = launch_recon(c, method, src);
   if (st) return st;
   const DevMesh m = dev_mesh(c.m);
-  const dim3 blk(64, 4);
-  const dim3 grd = cell_grid(c.m, blk);
-  int *ln = c.vof.list[1 - slot], *cn = c.vof.count + (1 - slot);
+  const dim3 grd(n_tiles(c.m)), blk(TBLK);
+  int *ln = c.vof.list[0], *cn = c.vof.count;
+  const int cf = corner_fix ? 1 : 0;
   if (dir == 0) {
     if (first)
-      hipLaunchKernelGGL((k_flux_update<0, true>), grd, blk, 0, c.stream, src, dst, u, m, dt,
-                         c.vof.snx, c.vof.sny, c.vof.sal, c.vof.cd, ln, cn);
+      hipLaunchKernelGGL((k_flux_update<0, true>), grd, blk, 0, c.stream, src, dst, u, m, dt, c.vof.snx,
+                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
     else
-      hipLaunchKernelGGL((k_flux_update<0, false>), grd, blk, 0, c.stream, src, dst, u, m, dt,
-                         c.vof.snx, c.vof.sny, c.vof.sal, c.vof.cd, ln, cn);
+      hipLaunchKernelGGL((k_flux_update<0, false>), grd, blk, 0, c.stream, src, dst, u, m, dt, c.vof.snx,
+                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
   } else {
     if (first)
-      hipLaunchKernelGGL((k_flux_update<1, true>), grd, blk, 0, c.stream, src, dst, v, m, dt,
-                         c.vof.snx, c.vof.sny, c.vof.sal, c.vof.cd, ln, cn);
+      hipLaunchKernelGGL((k_flux_update<1, true>), grd, blk, 0, c.stream, src, dst, v, m, dt, c.vof.snx,
+                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
     else
-      hipLaunchKernelGGL((k_flux_update<1, false>), grd, blk, 0, c.stream, src, dst, v, m, dt,
-                         c.vof.snx, c.vof.sny, c.vof.sal, c.vof.cd, ln, cn);
+      hipLaunchKernelGGL((k_flux_update<1, false>), grd, blk, 0, c.stream, src, dst, v, m, dt, c.vof.snx,
+                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
   }
-  COVOF_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_halo, dim3(1), dim3(1024), 0, c.stream, src, dst, m, corners ? 1 : 0);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -716,18 +689,16 @@ int vof_properties(Ctx &c, const double *C, double rho1, double rho2, double mu1
 int vof_advect(Ctx &c, double *C, const double *u, const double *v, double dt, int method,
                int *step_count, int nsteps, int two_phase, double rho1, double rho2, double mu1,
                double mu2, double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node) {
-  int st = launch_classify(c, C, 0);
+  if (nsteps <= 0) return COVOF_OK;
+  int st = launch_classify(c, C);
   if (st) return st;
-  int slot = 0;
   for (int s = 0; s < nsteps; ++s) {
     const bool x_first = ((*step_count) % 2) == 0;
     const int d1 = x_first ? 0 : 1, d2 = x_first ? 1 : 0;
-    st = launch_pass(c, method, d1, true, C, c.vof.Cb, u, v, dt, slot, false);
+    st = launch_pass(c, method, d1, true, C, c.vof.Cb, u, v, dt, false);
     if (st) return st;
-    slot = 1 - slot;
-    st = launch_pass(c, method, d2, false, c.vof.Cb, C, u, v, dt, slot, two_phase != 0);
+    st = launch_pass(c, method, d2, false, c.vof.Cb, C, u, v, dt, two_phase != 0);
     if (st) return st;
-    slot = 1 - slot;
     *step_count += 1;
     if (two_phase) {
       st = vof_properties(c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node);
@@ -739,9 +710,9 @@ int vof_advect(Ctx &c, double *C, const double *u, const double *v, double dt, i
 
 int vof_reconstruct_interface(Ctx &c, int method, const double *C, double *onx, double *ony,
                               double *oal) {
-  int st = launch_classify(c, C, 0);
+  int st = launch_classify(c, C);
   if (st) return st;
-  st = launch_recon(c, method, C, 0);
+  st = launch_recon(c, method, C);
   if (st) return st;
   const DevMesh m = dev_mesh(c.m);
   const dim3 blk(64, 4);
