@@ -50,7 +50,13 @@ SIGNATURES = {
     "covof_sp_correct_d": (_I, [_P, _P, _P, _P, _D, _P, _P]),
     "covof_sp_correct": (_I, [_P, _DP, _DP, _DP, _D, _DP, _DP]),
     "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_ctx_profile": (_I, [_P, _I]),
+    "covof_ctx_profile_read": (_I, [_P, _I, _DP, ct.POINTER(ct.c_long)]),
 }
+
+# kernel tags of covof_ctx_profile_read
+KERNEL_PCG_A_FP64, KERNEL_PCG_B_FP64, KERNEL_PCG_A_FP32, KERNEL_PCG_B_FP32, KERNEL_VOF_RECON, \
+    KERNEL_VOF_UPDATE = range(6)
 
 _lib = None
 

@@ -84,6 +84,18 @@ class Context:
     def new_work(self, fill=0.0):
         return torch.full((self.ny, self.nx), fill, dtype=torch.float64, device=self.device)
 
+    def profile(self, period: int):
+        """HIP-event timing of the hot kernels (covof_ctx_profile): every VOF
+        launch, Krylov iterations sampled every ``period``-th; 0 turns it off."""
+        L.check(self._lib.covof_ctx_profile(self._h, int(period)), "covof_ctx_profile")
+
+    def profile_read(self, tag: int):
+        """(total ms, launches) of one kernel tag since profiling was turned on."""
+        ms, n = ct.c_double(0.0), ct.c_long(0)
+        L.check(self._lib.covof_ctx_profile_read(self._h, int(tag), ct.byref(ms), ct.byref(n)),
+                "covof_ctx_profile_read")
+        return ms.value, n.value
+
     def synchronize(self):
         L.check(self._lib.covof_ctx_synchronize(self._h), "covof_ctx_synchronize")
 

@@ -21,7 +21,10 @@ int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precon
 
 namespace {
 
-__device__ __forceinline__ long kxi(int nx, int i, int j) { return (long)(j - 1) * (nx + 1) + (i - 1); }
+// the kx layout of poisson.hip: rows of nx+1 faces padded to a multiple of 64
+__device__ __forceinline__ long kxi(int nx, int i, int j) {
+  return (long)(j - 1) * (((long)(nx + 1) + 63) & ~63L) + (i - 1);
+}
 __device__ __forceinline__ long kyi(int nx, int i, int j) { return (long)(j - 1) * nx + (i - 1); }
 
 // dJ/dk * dk/drho_f * 1/2 for every face (one value per face, kx / ky layout).

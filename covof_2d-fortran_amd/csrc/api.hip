@@ -122,6 +122,8 @@ extern "C" int covof_ctx_destroy(covof_ctx *h) {
   if (c.stream) hipStreamSynchronize(c.stream);
   covof::vof_free(c);
   covof::poisson_free(c);
+  if (c.prof.created)
+    for (int k = 0; k < 2 * 1024; ++k) hipEventDestroy(c.prof.ev[k]);
   for (double *p : h->mirror) hipFree(p);
   if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
   delete h;
@@ -328,4 +330,70 @@ extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double d
                                         maxit, precision, iters, relres);
   if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver combination");
   return hip_status(st);
+}
+
+// ------------------------------------------------------------------ profiling
+namespace covof {
+void prof_begin(Ctx &c, int tag) {
+  Prof &P = c.prof;
+  if (!P.on || tag < 0 || tag >= COVOF_NUM_KERNEL_TAGS) {
+    P.pending = 0;
+    return;
+  }
+  P.pending = 1;
+  if (!P.created) {
+    for (int k = 0; k < 2 * 1024; ++k)
+      if (hipEventCreate(&P.ev[k]) != hipSuccess) { P.on = 0; return; }
+    P.created = 1;
+  }
+  if (P.used >= 1024) {  // pool full: wait for the stream and fold the records
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return;
+    prof_harvest(c);
+  }
+  P.tag[P.used] = tag;
+  hipEventRecord(P.ev[2 * P.used], c.stream);
+}
+void prof_end(Ctx &c) {
+  Prof &P = c.prof;
+  if (!P.on || !P.created || !P.pending) return;
+  P.pending = 0;
+  hipEventRecord(P.ev[2 * P.used + 1], c.stream);
+  ++P.used;
+}
+void prof_harvest(Ctx &c) {
+  Prof &P = c.prof;
+  for (int k = 0; k < P.used; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, P.ev[2 * k], P.ev[2 * k + 1]) == hipSuccess) {
+      P.total_ms[P.tag[k]] += ms;
+      P.count[P.tag[k]] += 1;
+    }
+  }
+  P.used = 0;
+}
+}  // namespace covof
+
+extern "C" int covof_ctx_profile(covof_ctx *h, int on) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  covof::Prof &P = h->c.prof;
+  if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+  covof::prof_harvest(h->c);
+  if (on) {
+    for (int t = 0; t < COVOF_NUM_KERNEL_TAGS; ++t) {
+      P.total_ms[t] = 0.0;
+      P.count[t] = 0;
+    }
+  }
+  P.on = on > 0 ? on : 0;
+  return COVOF_OK;
+}
+
+extern "C" int covof_ctx_profile_read(covof_ctx *h, int tag, double *total_ms, long *count) {
+  if (!h || tag < 0 || tag >= COVOF_NUM_KERNEL_TAGS || !total_ms || !count)
+    return fail(COVOF_BAD_ARGUMENT, "covof_ctx_profile_read: bad argument");
+  if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+  covof::prof_harvest(h->c);
+  *total_ms = h->c.prof.total_ms[tag];
+  *count = h->c.prof.count[tag];
+  return COVOF_OK;
 }

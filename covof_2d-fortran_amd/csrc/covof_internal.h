@@ -28,8 +28,14 @@ struct Vof {
   int *count = nullptr;                 // per-tile list sizes
 };
 
+// One instantiated capture of a solver cycle (see poisson.hip).
+struct PGraph {
+  hipGraphExec_t exec = nullptr;
+  int key = -1;
+};
+
 struct Poisson {
-  // face coefficients, FP64: kx((nx+1) x ny) kx(i,j) = a_W(i,j), kx(nx+1,j) = a_E(nx,j)
+  // face coefficients, FP64: kx((nx+1) x ny, row pitch padded to 64) kx(i,j) = a_W(i,j), kx(nx+1,j) = a_E(nx,j)
   //                          ky(nx x (ny+1))  ky(i,j) = a_S(i,j), ky(i,ny+1) = a_N(i,ny)
   double *kx = nullptr, *ky = nullptr, *diag = nullptr;
   float *kx32 = nullptr, *ky32 = nullptr, *diag32 = nullptr;
@@ -43,6 +49,22 @@ struct Poisson {
   double *hscal = nullptr;  // pinned host mirror of scal
   int assembled = 0;
   int nblocks = 0;          // reduction grid size
+  PGraph graph[2];          // FP64 cycle, mixed cycle
+  hipEvent_t cycle_ev[2] = {nullptr, nullptr};
+};
+
+// Optional per-launch timing of the hot kernels with HIP events on the
+// context's stream (covof_ctx_profile): begin/end events around each launch,
+// read back whenever the solvers synchronise anyway.
+struct Prof {
+  int on = 0;      // sampling period: every on-th iteration of a solve is timed
+  hipEvent_t ev[2 * 1024];
+  int tag[1024];
+  int used = 0;
+  int created = 0;
+  int pending = 0;  // a begin event was recorded for the current launch
+  double total_ms[COVOF_NUM_KERNEL_TAGS] = {0};
+  long count[COVOF_NUM_KERNEL_TAGS] = {0};
 };
 
 struct Ctx {
@@ -51,7 +73,16 @@ struct Ctx {
   int own_stream = 0;
   Vof vof;
   Poisson ps;
+  Prof prof;
 };
+
+// Records the begin event of a launch with the given tag (no-op when off or
+// when tag is not a kernel tag).
+void prof_begin(Ctx &c, int tag);
+inline bool prof_sampled(const Ctx &c, int k) { return c.prof.on > 0 && (k % c.prof.on) == 0; }
+void prof_end(Ctx &c);
+// Folds finished records into the totals; the stream must be idle.
+void prof_harvest(Ctx &c);
 
 inline __host__ __device__ long hidx(const Mesh &m, int i, int j) { return (long)j * (m.nx + 2) + i; }
 

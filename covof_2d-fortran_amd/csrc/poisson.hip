@@ -24,22 +24,24 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <initializer_list>
+#include <utility>
 
 namespace covof {
 
 // ---------------------------------------------------------------- scalars
 enum Slot {
   S_BNORM = 0, S_RZ = 1, /* and 2 */ S_RR = 4, S_RELRES = 6,
-  S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_DOT0 = 10,  // 10..13 generic dot outputs
+  S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_FIRST = 10, S_MAXIT = 11, S_DOT0 = 12,
   // inner (FP32) solve of the mixed mode
   I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
-  I_RR = 23, I_MAXR = 24, I_DELTA = 25,
+  I_RR = 23, I_FIRST = 26, I_MAXIT = 27,
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
   NSLOT = 64
 };
-enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_REFRESH = 4 };
+enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_MAXIT = 5 };
 
 struct DevOp {
   int nx, ny;
@@ -52,7 +54,10 @@ struct OpT {
   const T *kx, *ky, *diag;
 };
 
-__device__ __forceinline__ long kxi(int nx, int i, int j) { return (long)(j - 1) * (nx + 1) + (i - 1); }
+// kx rows are padded to a multiple of 64 entries so every row starts on a
+// 256-byte boundary (the (nx+1)-long rows would otherwise shift by one word).
+__host__ __device__ __forceinline__ long kx_pitch(int nx) { return ((long)(nx + 1) + 63) & ~63L; }
+__device__ __forceinline__ long kxi(int nx, int i, int j) { return (long)(j - 1) * kx_pitch(nx) + (i - 1); }
 __device__ __forceinline__ long kyi(int nx, int i, int j) { return (long)(j - 1) * nx + (i - 1); }
 
 // y(i,j) of poisson_matvec for a vector given by an accessor xv(k).
@@ -164,11 +169,25 @@ __device__ void grid_totals(const double *part, int nblk, double (&tot)[NV]) {
   }
 }
 
-// Cell loop over the block's tile column and its rows (grid-stride in y).
-#define CELL_LOOP(o)                                                                   \
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63) + 1;                              \
-  for (int j = blockIdx.y * 4 + (threadIdx.x >> 6) + 1; j <= (o).ny; j += gridDim.y * 4) \
-    if (i <= (o).nx)
+// Cell loop over a 1-D grid of 8 * gx * chunks blocks (gx = 64-column strips).
+// The dispatcher deals workgroup ids round-robin over the 8 XCDs, so block id
+// goes to XCD id % 8; that XCD owns a horizontal band of ny/8 rows, and its
+// blocks split the band into 64-column strips and row chunks walked 4 rows at
+// a time.  Stencil neighbours left/right and above/below then come from the
+// same XCD's L2 (only the band edges cross), and the rows a block reads as
+// "above" it touched a moment before.
+#define CELL_LOOP(o)                                                                           \
+  const int gx_ = ((o).nx + 63) / 64;                                                          \
+  const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;                                   \
+  const int chunks_ = (gridDim.x >> 3) / gx_;                                                  \
+  const int band_ = ((((o).ny + 7) / 8) + 3) & ~3;                                             \
+  const int rows_ = (((band_ + chunks_ - 1) / chunks_) + 3) & ~3;                              \
+  const int i = (local_ % gx_) * 64 + (threadIdx.x & 63) + 1;                                  \
+  const int jb_ = xcd_ * band_ + (local_ / gx_) * rows_;                                        \
+  const int je_ = min(min(jb_ + rows_, xcd_ * band_ + band_), (o).ny);                          \
+  if (local_ < gx_ * chunks_)                                                                  \
+    for (int j = jb_ + (threadIdx.x >> 6) + 1; j <= je_; j += 4)                               \
+      if (i <= (o).nx)
 
 // ---------------------------------------------------------------- assembly
 
@@ -306,25 +325,26 @@ __global__ void __launch_bounds__(1024) k_sgs_wavefront(DevOp o, OpT<double> A, 
 // in two parity slots (A_k reads rz_{k-1} and stores rz_k).
 struct PcgSlots {
   int rz;        // rz at parity 0, rz at parity 1 in rz + 1
-  int rr, relres, flag, iters, ref, tol;
-  int is_b_norm;  // reference norm: ||b|| (< 1e-30 -> 1) from the k = 1 totals;
+  int rr, relres, flag, iters, ref, tol, first, maxit;
+  int is_b_norm;  // reference norm: ||b|| (< 1e-30 -> 1) from the first totals;
                   // 0: the norm already in slot ref (the outer ||b||)
-  int maxr;       // >= 0: reliable-update mode -- stop with F_REFRESH once
-                  // ||r|| <= delta * scal[maxr] or ||r|| <= tol * ref
+  int inner;      // FP32 recurrence of the mixed mode: no convergence test of
+                  // its own (the FP64 residual of the reliable updates decides)
 };
 
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__restrict__ zs,
                                                const T *__restrict__ dprev, T *__restrict__ d,
                                                T *__restrict__ Ad, double *scal, PcgSlots sl,
-                                               const double *pin, double *pout, int k, int check_only) {
-  if (scal[sl.flag] != 0.0) return;
+                                               const double *pin, double *pout, int k) {
+  if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
   double tot[3];
   grid_totals<3>(pin, gridDim.x * gridDim.y, tot);
+  const bool first = scal[sl.first] != 0.0;
   double ref;
-  if (k == 1 && sl.is_b_norm) {
+  if (first && sl.is_b_norm) {
     ref = sqrt(tot[2]);
-    if (sl.is_b_norm == 1 && ref < 1.0e-30) ref = 1.0;
+    if (ref < 1.0e-30) ref = 1.0;
   } else {
     ref = scal[sl.ref];
   }
@@ -332,31 +352,89 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   const double rz_prev = scal[sl.rz + ((k - 1) & 1)];
   int stop = F_RUN;
   if (!(rn == rn) || rn > 1.0e300) stop = F_NONFINITE;
-  else if (sl.maxr >= 0 && (rn <= scal[I_DELTA] * scal[sl.maxr] || rn / ref <= scal[sl.tol])) stop = F_REFRESH;
-  else if (!(ref > 0.0) || rn / ref <= scal[sl.tol]) stop = F_CONV;
-  else if (k > 1 && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+  else if (!sl.inner && rn / ref <= scal[sl.tol]) stop = F_CONV;
+  else if (!first && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+  else if (scal[sl.iters] >= scal[sl.maxit]) stop = F_MAXIT;
   if (lead()) {
-    if (k == 1 && sl.is_b_norm) scal[sl.ref] = ref;
+    if (first && sl.is_b_norm) scal[sl.ref] = ref;
     scal[sl.rr] = tot[0];
     scal[sl.relres] = rn / ref;
     if (stop != F_RUN) {
       scal[sl.flag] = (double)stop;
-    } else if (!check_only) {
-      scal[sl.iters] = (double)k;
+    } else {
+      scal[sl.iters] += 1.0;
       scal[sl.rz + (k & 1)] = tot[1];
     }
   }
-  if (stop != F_RUN || check_only) return;
-  const T beta = (k == 1) ? T(0) : (T)(tot[1] / rz_prev);
+  if (stop != F_RUN) return;
+  const T beta = first ? T(0) : (T)(tot[1] / rz_prev);
   T acc[1] = {T(0)};
-  CELL_LOOP(o) {
-    const long q = (long)(j - 1) * o.nx + (i - 1);
-    auto dv = [&](long m) { return zs[m] + beta * dprev[m]; };
-    const T dk = dv(q);
-    const T aq = apply_op_fd<T>(o, A, dv, i, j, q);
-    d[q] = dk;
-    Ad[q] = aq;
-    acc[0] += dk * aq;
+  // Each wave walks a contiguous run of rows of its 64-column strip keeping
+  // d of the rows below / at / above in registers: d and the coefficients of a
+  // cell are loaded once, x-neighbours come from the adjacent lanes, and only
+  // the strip edges and the periodic wraps load extra values.  The matvec
+  // evaluates poisson_matvec's expression in its order on the same operands.
+  const int nx = o.nx, ny = o.ny;
+  const int gx_ = (nx + 63) / 64;
+  const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;
+  const int chunks_ = (gridDim.x >> 3) / gx_;
+  const int band_ = (((ny + 7) / 8) + 3) & ~3;
+  const int rows_ = (((band_ + chunks_ - 1) / chunks_) + 3) & ~3;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = (local_ % gx_) * 64 + lane + 1;
+  const int jb = xcd_ * band_ + (local_ / gx_) * rows_;
+  const int je = min(min(jb + rows_, xcd_ * band_ + band_), ny);
+  const int rw = (je - jb + 3) / 4;
+  const int jw0 = jb + 1 + w * rw, jw1 = min(jw0 + rw - 1, je);
+  const bool vi = i <= nx;
+  auto dv = [&](int a, int b) -> T {
+    const long m = (long)(b - 1) * nx + (a - 1);
+    return zs[m] + beta * dprev[m];
+  };
+  if (local_ < gx_ * chunks_ && jw0 <= jw1) {
+    T down = T(0), cur = T(0);
+    if (vi) {
+      cur = dv(i, jw0);
+      if (jw0 > 1) down = dv(i, jw0 - 1);
+      else if (o.perS) down = dv(i, ny);
+    }
+    T aS = vi ? A.ky[kyi(nx, i, jw0)] : T(0);
+    for (int j = jw0; j <= jw1; ++j) {
+      T up = T(0);
+      if (vi) {
+        if (j < ny) up = dv(i, j + 1);
+        else if (o.perN) up = dv(i, 1);
+      }
+      const T aW = vi ? A.kx[kxi(nx, i, j)] : T(0);
+      const T aN = vi ? A.ky[kyi(nx, i, j + 1)] : T(0);
+      T left = __shfl_up(cur, 1, 64);
+      T right = __shfl_down(cur, 1, 64);
+      T aE = __shfl_down(aW, 1, 64);
+      if (vi) {
+        if (i == 1) left = o.perW ? dv(nx, j) : T(0);
+        else if (lane == 0) left = dv(i - 1, j);
+        if (i == nx) {
+          right = o.perE ? dv(1, j) : T(0);
+          aE = A.kx[kxi(nx, nx + 1, j)];
+        } else if (lane == 63) {
+          right = dv(i + 1, j);
+          aE = A.kx[kxi(nx, i + 1, j)];
+        }
+        const T aP = aW + aE + aS + aN + (T)o.eps;
+        T val = aP * cur;
+        if (i > 1 || o.perW) val = val - aW * left;
+        if (i < nx || o.perE) val = val - aE * right;
+        if (j > 1 || o.perS) val = val - aS * down;
+        if (j < ny || o.perN) val = val - aN * up;
+        const long q = (long)(j - 1) * nx + (i - 1);
+        d[q] = cur;
+        Ad[q] = val;
+        acc[0] += cur * val;
+      }
+      down = cur;
+      cur = up;
+      aS = aN;
+    }
   }
   block_partials<1, T>(acc, pout);
 }
@@ -366,7 +444,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
                                                T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
                                                const T *__restrict__ Ad, double *scal, PcgSlots sl,
                                                const double *pin, double *pout, int k) {
-  if (scal[sl.flag] != 0.0) return;
+  if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
   double tot[1];
   grid_totals<1>(pin, gridDim.x * gridDim.y, tot);
   const double dAd = tot[0];
@@ -374,6 +452,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
     if (lead()) scal[sl.flag] = (dAd == dAd) ? (double)F_BREAK : (double)F_NONFINITE;
     return;
   }
+  if (lead()) scal[sl.first] = 0.0;
   const T alpha = (T)(scal[sl.rz + (k & 1)] / dAd);
   T acc[3] = {T(0), T(0), T(0)};
   CELL_LOOP(o) {
@@ -395,7 +474,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
 // r.z with an explicit z (SGS) into slot 1 of B's partials.
 __global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const double *__restrict__ r, const double *__restrict__ z,
                                                 const double *scal, PcgSlots sl, double *pout) {
-  if (scal[sl.flag] != 0.0) return;
+  if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
   double acc[1] = {0.0};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
@@ -411,7 +490,8 @@ template <int PC>
 __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const double *__restrict__ b,
                                                   const double *__restrict__ x, double *__restrict__ r,
                                                   double *__restrict__ z, float *__restrict__ r32,
-                                                  double *pout) {
+                                                  double *pout, const double *scal) {
+  if (scal[S_FLAG] != 0.0) return;
   double acc[3] = {0.0, 0.0, 0.0};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
@@ -431,12 +511,18 @@ __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const 
   block_partials<3, double>(acc, pout);
 }
 
-// Outer FP64 residual check of the mixed mode (one block): ||b||, ||r||,
-// converged when ||r||/||b|| <= tol.
+// FP64 residual check of the mixed mode's reliable updates (one block):
+// ||b|| (first time), ||r||, converged when ||r||/||b|| <= tol; the FP32
+// recurrence's iteration cap ends the solve too.
 __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int nblk, double *scal, int first) {
+  if (scal[S_FLAG] != 0.0) return;
   double tot[3];
   grid_totals<3>(pin, nblk, tot);
   if (threadIdx.x != 0) return;
+  if (scal[I_FLAG] == (double)F_MAXIT || scal[I_ITERS] >= scal[I_MAXIT]) {
+    scal[S_FLAG] = (double)F_MAXIT;
+    return;
+  }
   if (first) {
     double bn = sqrt(tot[2]);
     if (bn < 1.0e-30) bn = 1.0;
@@ -445,15 +531,17 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
   const double rn = sqrt(tot[0]);
   scal[S_RR] = tot[0];
   scal[S_RELRES] = rn / scal[S_BNORM];
-  scal[I_MAXR] = rn;
   if (!(rn == rn) || rn > 1.0e300) scal[S_FLAG] = (double)F_NONFINITE;
   else if (rn / scal[S_BNORM] <= scal[S_TOL]) scal[S_FLAG] = (double)F_CONV;
 }
 
-// Refinement update x += (double) dx32.
-__global__ void k_add_correction(long n, double *__restrict__ x, const float *__restrict__ dx32) {
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+// Reliable update x += (double) dx32, dx32 = 0.
+__global__ void k_add_correction(long n, double *__restrict__ x, float *__restrict__ dx32, const double *scal) {
+  if (scal[S_FLAG] != 0.0) return;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
     x[t] = x[t] + (double)dx32[t];
+    dx32[t] = 0.0f;
+  }
 }
 
 // FP32 inner-solve setup: z32 = r32 / a_P (Jacobi) or r32, partials r.r,
@@ -461,7 +549,9 @@ __global__ void k_add_correction(long n, double *__restrict__ x, const float *__
 template <int PC>
 __global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const float *__restrict__ r32,
                                                     const float *__restrict__ d32, float *__restrict__ z32,
-                                                    double *pout) {
+                                                    double *pout, double *scal) {
+  if (scal[S_FLAG] != 0.0) return;
+  if (lead()) scal[I_FLAG] = 0.0;  // a breakdown of the FP32 recurrence restarts from the true residual
   float acc[3] = {0.f, 0.f, 0.f};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
@@ -571,20 +661,21 @@ static DevOp dev_op(const Mesh &m) {
   return o;
 }
 
+// 1-D grid of 8 XCD bands x gx strips x chunks (about 2048 blocks); see CELL_LOOP.
 static dim3 red_grid(const Mesh &m) {
   const int gx = (m.nx + 63) / 64;
-  int gy = (m.ny + 3) / 4;
-  const int want = 2048;
-  int cap = want / gx;
-  if (cap < 1) cap = 1;
-  if (gy > cap) gy = cap;
-  return dim3(gx, gy);
+  const int band = (((m.ny + 7) / 8) + 3) & ~3;
+  int chunks = 2048 / (8 * gx);
+  if (chunks < 1) chunks = 1;
+  const int max_chunks = (band + 3) / 4;  // at least 4 rows per chunk
+  if (chunks > max_chunks) chunks = max_chunks;
+  return dim3(8 * gx * chunks);
 }
 
 int poisson_alloc(Ctx &c) {
   Poisson &P = c.ps;
   const long n = (long)c.m.nx * c.m.ny;
-  const long nkx = (long)(c.m.nx + 1) * c.m.ny, nky = (long)c.m.nx * (c.m.ny + 1);
+  const long nkx = kx_pitch(c.m.nx) * c.m.ny, nky = (long)c.m.nx * (c.m.ny + 1);
   COVOF_CHECK(hipMalloc(&P.kx, nkx * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.ky, nky * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.diag, n * sizeof(double)));
@@ -611,6 +702,8 @@ int poisson_alloc(Ctx &c) {
   COVOF_CHECK(hipMemset(P.ticket, 0, sizeof(unsigned)));
   COVOF_CHECK(hipMemset(P.scal, 0, NSLOT * sizeof(double)));
   COVOF_CHECK(hipHostMalloc(&P.hscal, NSLOT * sizeof(double), hipHostMallocDefault));
+  COVOF_CHECK(hipEventCreateWithFlags(&P.cycle_ev[0], hipEventDisableTiming));
+  COVOF_CHECK(hipEventCreateWithFlags(&P.cycle_ev[1], hipEventDisableTiming));
   return COVOF_OK;
 }
 
@@ -621,6 +714,10 @@ void poisson_free(Ctx &c) {
                   P.fx, P.fy};
   for (void *q : ptrs) hipFree(q);
   if (P.hscal) hipHostFree(P.hscal);
+  for (PGraph &G : P.graph)
+    if (G.exec) hipGraphExecDestroy(G.exec);
+  if (P.cycle_ev[0]) hipEventDestroy(P.cycle_ev[0]);
+  if (P.cycle_ev[1]) hipEventDestroy(P.cycle_ev[1]);
 }
 
 static int is_wall_type(int t) { return t == BC_DIRICHLET || t == BC_FREESLIP; }
@@ -672,6 +769,7 @@ int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
 static int read_scalars(Ctx &c, int n) {
   COVOF_CHECK(hipMemcpyAsync(c.ps.hscal, c.ps.scal, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
   COVOF_CHECK(hipStreamSynchronize(c.stream));
+  prof_harvest(c);
   return COVOF_OK;
 }
 
@@ -688,79 +786,171 @@ static int fill(Ctx &c, T *x, long n, T v) {
   return COVOF_OK;
 }
 
-static const PcgSlots OUTER = {S_RZ, S_RR, S_RELRES, S_FLAG, S_ITERS, S_BNORM, S_TOL, 1, -1};
-static const PcgSlots INNER = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, S_BNORM, S_TOL, 0, I_MAXR};
-// restarted refinement: inner solve from dx = 0 to ||r_in|| <= eta ||r_in,0||
-static const PcgSlots INNER_RESTART = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, I_RR0, I_ETA, 2, -1};
+static const PcgSlots OUTER = {S_RZ, S_RR, S_RELRES, S_FLAG, S_ITERS, S_BNORM, S_TOL, S_FIRST, S_MAXIT, 1, 0};
+static const PcgSlots INNER = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, S_BNORM, S_TOL, I_FIRST, I_MAXIT, 0, 1};
+constexpr int NREAD = 32;  // scalar slots read back by the host
 
 // Partial buffers: [0] written by A kernels, [1] by B / residual / rz kernels,
 // [2] by the sum kernel.
 static double *part_buf(Ctx &c, int w) { return c.ps.part + (long)w * c.ps.nblocks * NP; }
 
-// Chunk of polled iterations: short while the solve may end early, longer later.
-static int chunk_for(int done) { return done < 64 ? 16 : (done < 1024 ? 64 : 256); }
+static int status_of_flag(double f) {
+  if (f == F_CONV) return COVOF_OK;
+  if (f == F_BREAK) return COVOF_BREAKDOWN;
+  if (f == F_NONFINITE) return COVOF_NONFINITE;
+  return COVOF_NOT_CONVERGED;
+}
 
-// PCG iterations k = k0 .. k0+count-1 (T, PC) on (x, r, z); zs = z (PC 1/2)
-// or r (PC 0); d0/d1 ping-pong the search direction (d_prev at k = 1 is d0,
-// which must hold zeros).  PC 2 runs the SGS wavefront and r.z after B.
+// One PCG iteration k (A_k, B_k) on (x, r, z); zs = z (PC 1/2) or r (PC 0);
+// d_prev / d alternate between d0 and d1 with the parity of k (the first
+// iteration, k odd, reads d0, which must hold zeros).  When `ev` is given the
+// two launches are bracketed by its event pairs (profiling).
 template <typename T, int PC>
-static int pcg_chunk(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
-                     int k0, int count) {
+static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
+                          int k, bool timed) {
   const DevOp o = dev_op(c.m);
   const dim3 g = red_grid(c.m);
   double *pa = part_buf(c, 0), *pb = part_buf(c, 1);
   const T *zs = (PC == 0) ? r : z;
-  for (int k = k0; k < k0 + count; ++k) {
-    T *dprev = (k & 1) ? d0 : d1;  // k = 1 reads d0 (zeros), writes d1
-    T *dnew = (k & 1) ? d1 : d0;
-    hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, o, A, zs, dprev, dnew, Ad, c.ps.scal, sl,
-                       (const double *)pb, pa, k, 0);
-    hipLaunchKernelGGL((k_pcg_b<T, PC>), g, dim3(BLK), 0, c.stream, o, A.diag, x, r, z, dnew, Ad,
-                       c.ps.scal, sl, pa, pb, k);
-    if (PC == 2) {
-      OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
-      hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, (const double *)r,
-                         (double *)z, (const double *)(c.ps.scal + sl.flag));
-      hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)r, (const double *)z,
-                         c.ps.scal, sl, pb);
+  T *dprev = (k & 1) ? d0 : d1;
+  T *dnew = (k & 1) ? d1 : d0;
+  const int tag = timed ? (sizeof(T) == 8 ? COVOF_KERNEL_PCG_A_FP64 : COVOF_KERNEL_PCG_A_FP32) : -1;
+  prof_begin(c, tag);
+  hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, o, A, zs, dprev, dnew, Ad, c.ps.scal, sl,
+                     (const double *)pb, pa, k);
+  prof_end(c);
+  prof_begin(c, timed ? tag + 1 : -1);
+  hipLaunchKernelGGL((k_pcg_b<T, PC>), g, dim3(BLK), 0, c.stream, o, A.diag, x, r, z, dnew, Ad, c.ps.scal, sl,
+                     (const double *)pa, pb, k);
+  prof_end(c);
+  if (PC == 2) {
+    OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
+    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, (const double *)r, (double *)z,
+                       (const double *)(c.ps.scal + sl.flag));
+    hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)r, (const double *)z, c.ps.scal,
+                       sl, pb);
+  }
+}
+
+// The reliable update of the mixed mode: x += x32, x32 = 0, r = b - A x in
+// FP64 (+ its FP32 copy), the convergence test, and the FP32 recurrence's
+// restart values (z32, r.r, r.z) from the true residual.
+static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first) {
+  Poisson &P = c.ps;
+  const long n = (long)c.m.nx * c.m.ny;
+  const DevOp o = dev_op(c.m);
+  const dim3 g = red_grid(c.m);
+  OpT<double> A{P.kx, P.ky, P.diag};
+  double *pb = part_buf(c, 1);
+  if (!first) hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32,
+                                 (const double *)P.scal);
+  hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, (const double *)x, P.r, P.z, P.r32, pb,
+                     (const double *)P.scal);
+  hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
+                     first);
+  if (pc == 1)
+    hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, (const float *)P.r32,
+                       (const float *)P.diag32, P.z32, pb, P.scal);
+  else
+    hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const float *)P.r32,
+                       (const float *)P.diag32, P.z32, pb, P.scal);
+}
+
+// One solver cycle: ITERS PCG iterations (FP64 solve) or ITERS FP32
+// iterations + one reliable update (mixed solve), then a copy of the scalar
+// slots to pinned host memory.  All pointers are context buffers.  With
+// profiling, every prof.on-th iteration's two kernels are bracketed by events.
+static void enqueue_cycle(Ctx &c, int pc, bool mixed, int iters) {
+  Poisson &P = c.ps;
+  const long n = (long)c.m.nx * c.m.ny;
+  OpT<double> A{P.kx, P.ky, P.diag};
+  OpT<float> A32{P.kx32, P.ky32, P.diag32};
+  for (int k = 1; k <= iters; ++k) {
+    const bool timed = prof_sampled(c, k);
+    if (mixed) {
+      if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, P.p32, P.p32 + n, P.q32, INNER, k, timed);
+      else pcg_iteration<float, 0>(c, A32, P.x32, P.r32, P.z32, P.p32, P.p32 + n, P.q32, INNER, k, timed);
+    } else {
+      if (pc == 1) pcg_iteration<double, 1>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
+      else pcg_iteration<double, 0>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
     }
+  }
+  if (mixed) mixed_refresh(c, pc, P.b, P.x, 0);
+  hipMemcpyAsync(P.hscal, P.scal, NREAD * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+}
+
+// The cycle captured once into a HIP graph (one instantiation serves every
+// solve of a context with the same selectors: only context buffers appear).
+static int capture_cycle(Ctx &c, PGraph &G, int key, int pc, bool mixed, int iters) {
+  if (G.exec && G.key == key) return COVOF_OK;
+  if (G.exec) {
+    hipGraphExecDestroy(G.exec);
+    G.exec = nullptr;
+  }
+  COVOF_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+  enqueue_cycle(c, pc, mixed, iters);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c.stream, &graph);
+  if (ec != hipSuccess || !graph) return COVOF_HIP_ERROR;
+  const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) return COVOF_HIP_ERROR;
+  G.key = key;
+  return COVOF_OK;
+}
+
+// Runs cycles until the stop flag is set, keeping one cycle in flight while
+// the previous one's scalars are checked.  Graph replays normally; with
+// profiling on, the same kernels are launched directly so events can bracket
+// the sampled iterations.
+static int run_cycles(Ctx &c, PGraph &G, int key, int pc, bool mixed, int iters, int max_cycles) {
+  Poisson &P = c.ps;
+  const bool direct = c.prof.on > 0;
+  if (!direct) {
+    const int st = capture_cycle(c, G, key, pc, mixed, iters);
+    if (st) return st;
+  }
+  auto launch = [&]() -> int {
+    if (direct) {
+      enqueue_cycle(c, pc, mixed, iters);
+      COVOF_CHECK_LAUNCH();
+    } else {
+      COVOF_CHECK(hipGraphLaunch(G.exec, c.stream));
+    }
+    return COVOF_OK;
+  };
+  int launched = 0, st;
+  if ((st = launch())) return st;
+  COVOF_CHECK(hipEventRecord(P.cycle_ev[0], c.stream));
+  ++launched;
+  for (int cyc = 0;; ++cyc) {
+    if (launched < max_cycles) {
+      if ((st = launch())) return st;
+      COVOF_CHECK(hipEventRecord(P.cycle_ev[launched & 1], c.stream));
+      ++launched;
+    }
+    COVOF_CHECK(hipEventSynchronize(P.cycle_ev[cyc & 1]));
+    if (P.hscal[S_FLAG] != 0.0 || cyc + 1 >= launched) break;
+  }
+  COVOF_CHECK(hipStreamSynchronize(c.stream));
+  prof_harvest(c);
+  return COVOF_OK;
+}
+
+static int set_scalars(Ctx &c, std::initializer_list<std::pair<int, double>> kv) {
+  for (const auto &p : kv) {
+    hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, c.ps.scal, p.first, 1, p.second);
   }
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
 
-static int status_of_flag(double f, int iters, int maxit) {
-  if (f == F_CONV) return COVOF_OK;
-  if (f == F_BREAK) return COVOF_BREAKDOWN;
-  if (f == F_NONFINITE) return COVOF_NONFINITE;
-  return iters >= maxit ? COVOF_NOT_CONVERGED : COVOF_OK;
-}
+constexpr int CYCLE64 = 64;  // PCG iterations per captured FP64 cycle
 
-// Drives PCG iterations k = 1.. until the flag slot is set or maxit iterations
-// have run; the residual partials for A_1 must be in part_buf(1).
-template <typename T, int PC>
-static int pcg_loop(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
-                    int maxit, int nread, int k_first = 1) {
-  int st, done = k_first - 1;
-  while (done < maxit) {
-    const int chunk = std::min(maxit - done, chunk_for(done - k_first + 1));
-    if ((st = pcg_chunk<T, PC>(c, A, x, r, z, d0, d1, Ad, sl, done + 1, chunk))) return st;
-    done += chunk;
-    if ((st = read_scalars(c, nread))) return st;
-    if (c.ps.hscal[sl.flag] != 0.0) return COVOF_OK;
-  }
-  // the convergence test of the last iteration runs in the prologue of the
-  // next A kernel: launch that prologue alone
-  const dim3 g = red_grid(c.m);
-  hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, dev_op(c.m), A, (const T *)r, (const T *)d0, d1,
-                     Ad, c.ps.scal, sl, (const double *)part_buf(c, 1), part_buf(c, 0), maxit + 1, 1);
-  COVOF_CHECK_LAUNCH();
-  return read_scalars(c, nread);
-}
-
-// FP64 PCG of the SP loop with NONE / JACOBI / SSOR (SGS wavefront).
-static int solve_pcg64(Ctx &c, const double *b, double *x, int precond, double tol, int maxit,
-                       int *iters, double *relres) {
+// FP64 PCG of the SP loop on the context vectors P.b -> P.x.  NONE / JACOBI
+// run as captured cycles of fused kernels; SSOR (the sequential SGS
+// wavefront) runs launch by launch with the same kernels.
+static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
   const Mesh &m = c.m;
   const long n = (long)m.nx * m.ny;
@@ -769,129 +959,74 @@ static int solve_pcg64(Ctx &c, const double *b, double *x, int precond, double t
   OpT<double> A{P.kx, P.ky, P.diag};
   int st;
   if ((st = fill<double>(c, P.p, n, 0.0))) return st;
-  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, 16, 0.0);
-  if ((st = set_scalar(c, S_TOL, tol))) return st;
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, NREAD, 0.0);
+  if ((st = set_scalars(c, {{S_TOL, tol}, {S_FIRST, 1.0}, {S_MAXIT, (double)maxit}}))) return st;
   const int pc = precond == COVOF_PRECOND_SSOR ? 2 : (precond == COVOF_PRECOND_JACOBI ? 1 : 0);
   double *pb = part_buf(c, 1);
   if (pc == 1)
-    hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, (float *)nullptr, pb);
+    hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
+                       P.r, P.z, (float *)nullptr, pb, (const double *)P.scal);
   else
-    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, (float *)nullptr, pb);
+    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
+                       P.r, P.z, (float *)nullptr, pb, (const double *)P.scal);
   COVOF_CHECK_LAUNCH();
   if (pc == 2) {
-    // z = SGS(r) and r.z into slot 1 of the residual partials
+    // z = SGS(r) and r.z into slot 1 of the residual partials, then launch by launch
     hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, (const double *)P.r, P.z,
                        (const double *)nullptr);
     hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
-                       P.scal, OUTER, pb);
+                       (const double *)P.scal, OUTER, pb);
     COVOF_CHECK_LAUNCH();
-    st = pcg_loop<double, 2>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
-  } else if (pc == 1) {
-    st = pcg_loop<double, 1>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
+    for (int k = 1;; ++k) {
+      pcg_iteration<double, 2>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, prof_sampled(c, k));
+      COVOF_CHECK_LAUNCH();
+      if (k % 16 == 0 || k > maxit) {
+        if ((st = read_scalars(c, NREAD))) return st;
+        if (P.hscal[S_FLAG] != 0.0) break;
+      }
+    }
   } else {
-    st = pcg_loop<double, 0>(c, A, x, P.r, P.z, P.p, P.t0, P.q, OUTER, maxit, 16);
+    if ((st = run_cycles(c, P.graph[0], 1 + pc, pc, false, CYCLE64, maxit / CYCLE64 + 2))) return st;
   }
-  if (st) return st;
+  if ((st = read_scalars(c, NREAD))) return st;
   const int it = (int)P.hscal[S_ITERS];
   if (iters) *iters = it;
   if (relres) *relres = P.hscal[S_RELRES];
-  return status_of_flag(P.hscal[S_FLAG], it, maxit);
+  return status_of_flag(P.hscal[S_FLAG]);
 }
 
-// Mixed precision: FP32 PCG with reliable updates in FP64 (NONE/JACOBI).
-// The Krylov recurrence (x32 correction, r32, z32, d, Ad, scalars) runs on the
-// FP32 operator copy with FP32 wave sums (FP64 across waves and blocks).
-// Whenever the FP32 residual has dropped by delta = 0.1 since the last
-// update, or reaches tol ||b||, the iterate is folded into the FP64 x
-// (x += x32), the true residual r = b - A x is recomputed in FP64, the FP32
-// residual restarts from it and the search direction is kept (beta is formed
-// with the refreshed residual).  The solve stops on the FP64 ||r||/||b|| <= tol.
-// Method: the "reliable update" scheme of Sleijpen & van der Vorst (1996), as
+// Mixed precision on P.b -> P.x: Jacobi (or unpreconditioned) CG whose Krylov
+// recurrence -- the correction x32, r32, z32, d, Ad and the scalars -- runs on
+// the FP32 operator copy with FP32 wave sums (FP64 across waves and blocks),
+// with a reliable update every `cycle` iterations: the FP32 correction is
+// folded into the FP64 x, the true residual r = b - A x is recomputed in FP64,
+// the FP32 residual restarts from it while the search direction is kept (beta
+// is formed with the refreshed residual), and the FP64 ||r||/||b|| <= tol test
+// ends the solve.  Method: reliable updates (Sleijpen & van der Vorst, 1996) as
 // used for mixed-precision CG by Clark et al., Comput. Phys. Commun. 181
 // (2010) 1517.
-static int solve_mixed(Ctx &c, const double *b, double *x, int precond, double tol, int maxit,
-                       int *iters, double *relres) {
+static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
-  const Mesh &m = c.m;
-  const long n = (long)m.nx * m.ny;
-  const DevOp o = dev_op(m);
-  const dim3 g = red_grid(m);
-  OpT<double> A{P.kx, P.ky, P.diag};
-  OpT<float> A32{P.kx32, P.ky32, P.diag32};
+  const long n = (long)c.m.nx * c.m.ny;
   const int pc = precond == COVOF_PRECOND_JACOBI ? 1 : 0;
-  const int max_refresh = 4000;
-  const char *mode_env = getenv("COVOF_MIXED_MODE");
-  const bool restart = mode_env && mode_env[0] == 'r' && mode_env[1] == 'e' && mode_env[2] == 's';
-  const char *eta_env = getenv("COVOF_MIXED_ETA");
-  const double eta = eta_env ? atof(eta_env) : 1.0e-4;
-  const char *del_env = getenv("COVOF_MIXED_DELTA");
-  const double delta = del_env ? atof(del_env) : 0.7;
+  const char *cyc_env = getenv("COVOF_MIXED_CYCLE");
+  int cycle = cyc_env ? atoi(cyc_env) : 128;
+  if (cycle < 2) cycle = 2;
+  cycle &= ~1;  // even: the parity pattern of k repeats from one replay to the next
   int st;
-  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, 32, 0.0);
-  if ((st = set_scalar(c, S_TOL, tol))) return st;
-  if ((st = set_scalar(c, I_DELTA, delta))) return st;
-  if ((st = set_scalar(c, I_ETA, eta))) return st;
+  hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, NREAD, 0.0);
+  if ((st = set_scalars(c, {{S_TOL, tol}, {I_FIRST, 1.0}, {I_MAXIT, (double)maxit}, {S_MAXIT, (double)maxit}})))
+    return st;
   if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
-  float *d0 = P.p32, *d1 = P.p32 + n;
-  if ((st = fill<float>(c, d0, n, 0.0f))) return st;
-  double *pb = part_buf(c, 1);
-  int k_next = 1, refreshes = 0, total = 0;
-  for (;;) {
-    // FP64 true residual of x (+ the pending FP32 correction)
-    if (k_next > 1) {
-      hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32);
-      if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
-    }
-    hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, P.r32, pb);
-    hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks,
-                       P.scal, k_next == 1 ? 1 : 0);
-    COVOF_CHECK_LAUNCH();
-    if ((st = read_scalars(c, 32))) return st;
-    if (P.hscal[S_FLAG] != 0.0 || k_next > maxit || refreshes >= max_refresh) break;
-    if (restart) {
-      if ((st = fill<float>(c, d0, n, 0.0f))) return st;
-      k_next = 1;
-    }
-    // restart the FP32 recurrence from the true residual, keeping d
-    if (pc == 1)
-      hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.z32, pb);
-    else
-      hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, P.r32, P.diag32, P.z32, pb);
-    if ((st = set_scalar(c, I_FLAG, 0.0))) return st;
-    COVOF_CHECK_LAUNCH();
-    const PcgSlots &isl = restart ? INNER_RESTART : INNER;
-    const int cap = restart ? maxit - total : maxit;
-    if (pc == 1) st = pcg_loop<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, isl, cap, 32, k_next);
-    else st = pcg_loop<float, 0>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, isl, cap, 32, k_next);
-    if (st) return st;
-    double f = P.hscal[I_FLAG];
-    if (restart) {
-      total += (int)P.hscal[I_ITERS];
-      if (f == F_CONV) f = F_REFRESH;
-      k_next = 2;  // any value > 1: fold the correction next time round
-      if (total >= maxit) { k_next = maxit + 1; }
-    } else {
-      k_next = (int)P.hscal[I_ITERS] + 1;
-      total = k_next - 1;
-    }
-    ++refreshes;
-    if (f == F_NONFINITE || f == F_BREAK) {
-      // fold what the FP32 recurrence reached and report from the FP64 residual
-      hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32);
-      hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, x, P.r, P.z, P.r32, pb);
-      hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal, 0);
-      COVOF_CHECK_LAUNCH();
-      if ((st = read_scalars(c, 32))) return st;
-      if (P.hscal[S_FLAG] != F_CONV) {
-        if (iters) *iters = total;
-        if (relres) *relres = P.hscal[S_RELRES];
-        return f == F_NONFINITE ? COVOF_NONFINITE : COVOF_BREAKDOWN;
-      }
-      break;
-    }
-    // F_REFRESH, or the iteration cap (loop exits above)
+  if ((st = fill<float>(c, P.p32, n, 0.0f))) return st;
+  mixed_refresh(c, pc, P.b, P.x, 1);
+  COVOF_CHECK_LAUNCH();
+  if ((st = read_scalars(c, NREAD))) return st;
+  if (P.hscal[S_FLAG] == 0.0) {
+    if ((st = run_cycles(c, P.graph[1], 1000 + cycle * 8 + pc, pc, true, cycle, maxit / cycle + 2))) return st;
   }
-  if (iters) *iters = total;
+  if ((st = read_scalars(c, NREAD))) return st;
+  if (iters) *iters = (int)P.hscal[I_ITERS];
   if (relres) *relres = P.hscal[S_RELRES];
   const double f = P.hscal[S_FLAG];
   if (f == F_CONV) return COVOF_OK;
@@ -906,12 +1041,18 @@ int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precon
     return COVOF_BAD_ARGUMENT;
   if (maxit < 0 || !(tol >= 0.0)) return COVOF_BAD_ARGUMENT;
   if (lin_solver == COVOF_LINEAR_CG) {
-    if (precision == COVOF_PRECISION_MIXED) {
-      if (precond == COVOF_PRECOND_SSOR) return COVOF_BAD_ARGUMENT;
-      return solve_mixed(c, b, x, precond, tol, maxit, iters, relres);
-    }
-    if (precision != COVOF_PRECISION_FP64) return COVOF_BAD_ARGUMENT;
-    return solve_pcg64(c, b, x, precond, tol, maxit, iters, relres);
+    if (precision == COVOF_PRECISION_MIXED && precond == COVOF_PRECOND_SSOR) return COVOF_BAD_ARGUMENT;
+    if (precision != COVOF_PRECISION_MIXED && precision != COVOF_PRECISION_FP64) return COVOF_BAD_ARGUMENT;
+    // the captured cycles work on the context vectors P.b, P.x
+    const long n = (long)c.m.nx * c.m.ny;
+    Poisson &P = c.ps;
+    if (b != P.b) COVOF_CHECK(hipMemcpyAsync(P.b, b, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    if (x != P.x) COVOF_CHECK(hipMemcpyAsync(P.x, x, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    const int st = precision == COVOF_PRECISION_MIXED ? solve_mixed(c, precond, tol, maxit, iters, relres)
+                                                      : solve_pcg64(c, precond, tol, maxit, iters, relres);
+    if (st == COVOF_HIP_ERROR || st == COVOF_BAD_ARGUMENT) return st;
+    if (x != P.x) COVOF_CHECK(hipMemcpyAsync(x, P.x, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    return st;
   }
   if (lin_solver == COVOF_LINEAR_BICGSTAB2) {
     extern int solve_bicgstab2(Ctx &, const double *, double *, int, double, int, int *, double *);

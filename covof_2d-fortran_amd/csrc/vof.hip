@@ -623,12 +623,14 @@ static int launch_recon(Ctx &c, int method, const double *src) {
   const int per_block = method == 1 ? TBLK / 8 : TBLK;
   const int ring_blocks = std::min(256, (L + per_block - 1) / per_block);
   const dim3 grid(nt + ring_blocks);
+  prof_begin(c, COVOF_KERNEL_VOF_RECON);
   if (method == 1)
     hipLaunchKernelGGL((k_recon<1>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
                        c.vof.snx, c.vof.sny, c.vof.sal);
   else
     hipLaunchKernelGGL((k_recon<0>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
                        c.vof.snx, c.vof.sny, c.vof.sal);
+  prof_end(c);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -649,6 +651,7 @@ static int launch_pass(Ctx &c, int method, int dir, bool first, const double *sr
   const dim3 grd(n_tiles(c.m)), blk(TBLK);
   int *ln = c.vof.list[0], *cn = c.vof.count;
   const int cf = corner_fix ? 1 : 0;
+  prof_begin(c, COVOF_KERNEL_VOF_UPDATE);
   if (dir == 0) {
     if (first)
       hipLaunchKernelGGL((k_flux_update<0, true>), grd, blk, 0, c.stream, src, dst, u, m, dt, c.vof.snx,
@@ -664,6 +667,7 @@ static int launch_pass(Ctx &c, int method, int dir, bool first, const double *sr
       hipLaunchKernelGGL((k_flux_update<1, false>), grd, blk, 0, c.stream, src, dst, v, m, dt, c.vof.snx,
                          c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
   }
+  prof_end(c);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }

@@ -154,6 +154,25 @@ int covof_sp_correct_d(covof_ctx *ctx, const double *p, const double *rho_u, con
 int covof_sp_correct(covof_ctx *ctx, const double *p, const double *rho_u, const double *rho_v,
                      double dt, double *u, double *v);
 
+/* --- kernel timing (no reference counterpart: instrumentation) ------------- */
+
+/* Kernel tags of covof_ctx_profile_read. */
+#define COVOF_KERNEL_PCG_A_FP64 0 /* fused PCG direction + matvec, FP64 */
+#define COVOF_KERNEL_PCG_B_FP64 1 /* fused PCG update + Jacobi + dots, FP64 */
+#define COVOF_KERNEL_PCG_A_FP32 2 /* the same kernels of the mixed mode, FP32 */
+#define COVOF_KERNEL_PCG_B_FP32 3
+#define COVOF_KERNEL_VOF_RECON 4  /* PLIC/ELVIRA reconstruction of one pass */
+#define COVOF_KERNEL_VOF_UPDATE 5 /* flux + update + halo of one pass */
+#define COVOF_NUM_KERNEL_TAGS 6
+/* on > 0: bracket launches of the tagged kernels with HIP events on the
+ * context's stream and accumulate their durations -- every launch of the VOF
+ * kernels, the Krylov kernels of every on-th iteration (a sampling period that
+ * keeps the event overhead out of a timed solve); on == 0: stop.  Turning it
+ * on clears the totals. */
+int covof_ctx_profile(covof_ctx *ctx, int on);
+/* Accumulated duration (ms) and number of launches of one kernel tag. */
+int covof_ctx_profile_read(covof_ctx *ctx, int tag, double *total_ms, long *count);
+
 /* --- adjoint of the pressure projection (no reference counterpart) --------- */
 
 /* Given the forward solution x of A(rho) x = b and w = dJ/dx (nx*ny device
