@@ -10,6 +10,13 @@ ellipses, divergence-free Fourier velocity, CFL 0.25), and the oracle's
 Jacobi-PCG on the host cores as the CPU baseline.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --kernel-sample   # bounded run of the same kernels for
+                                      # rocprofv3 counter passes (no JSON)
+
+The kernel-sample run executes 256 iterations of the mixed solve, 64 of the
+FP64 solve and 2 VOF steps on the same workloads, then a plain copy of
+256 MiB (torch copy_ of float64, reads and writes 256 MiB) whose FETCH_SIZE /
+WRITE_SIZE readings calibrate the counters into bytes.
 """
 from __future__ import annotations
 
@@ -25,10 +32,32 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solve s to 1e-10 + VOF cell-updates/s at 2048^2; HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a 256 MiB
+# copy) measured with rocprofv3 --pmc on `bench.py --kernel-sample`; the
+# summary with its commands is profiles/poisson_pcg_wy_2048_hbm_traffic.md.
+TRAFFIC_2048 = {"k_pcg_a<float>": 1.133e8, "k_pcg_b<float,1>": 1.349e8}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_sample(cv, wl, ctx, b, n, dev):
+    import torch
+    for precision, iters in ((cv.PRECISION_MIXED, 256), (cv.PRECISION_FP64, 64)):
+        x = ctx.new_work()
+        r = ctx.poisson_solve(b, x, precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=iters, precision=precision)
+        log(f"sample solve precision={precision}: {r.iters} iterations")
+    Cv, uv, vv, dtv = wl.vof_case(n, seed=0)
+    vctx = cv.Context(n, n, bc=(cv.BC_PERIODIC,) * 4)
+    Cg, ug, vg = (torch.tensor(a, device=dev) for a in (Cv, uv, vv))
+    vctx.advect_vof(Cg, ug, vg, dtv, reconstruction_method="ELVIRA", nsteps=2)
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    log("kernel sample done")
 
 
 def main():
@@ -40,6 +69,7 @@ def main():
     ap.add_argument("--vof-steps", type=int, default=20)
     ap.add_argument("--cpu-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-sample", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hot path runs on a single MI355X here "
@@ -63,6 +93,9 @@ def main():
     T = {k: torch.tensor(case[k], device=dev) for k in ("u", "v", "rho_u", "rho_v")}
     b = ctx.poisson_assemble(T["u"], T["v"], T["rho_u"], T["rho_v"], case["dt"])
     tol = 1e-10
+    if args.kernel_sample:
+        kernel_sample(cv, wl, ctx, b, n, dev)
+        return
 
     def solve(precision):
         x = ctx.new_work()
@@ -184,7 +217,8 @@ def main():
         "mixed_speedup_vs_fp64": t64 / t_mixed,
         "pcg_iteration_GBps_fp32": iter_gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_2048.get(dom) if n == 2048 else None,
                      "bytes_per_launch": dom_bytes, "avg_launch_us": dom_ms * 1e3, "launches": dom_n},
         "cpu_baseline": cpu,
         "vof": vof,

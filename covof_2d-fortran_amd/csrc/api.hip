@@ -334,16 +334,16 @@ extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double d
 
 // ------------------------------------------------------------------ profiling
 namespace covof {
-void prof_begin(Ctx &c, int tag) {
+void prof_events(Ctx &c, int tag, hipEvent_t *start, hipEvent_t *stop) {
   Prof &P = c.prof;
-  if (!P.on || tag < 0 || tag >= COVOF_NUM_KERNEL_TAGS) {
-    P.pending = 0;
-    return;
-  }
-  P.pending = 1;
+  *start = *stop = nullptr;
+  if (!P.on || tag < 0 || tag >= COVOF_NUM_KERNEL_TAGS) return;
   if (!P.created) {
     for (int k = 0; k < 2 * 1024; ++k)
-      if (hipEventCreate(&P.ev[k]) != hipSuccess) { P.on = 0; return; }
+      if (hipEventCreate(&P.ev[k]) != hipSuccess) {
+        P.on = 0;
+        return;
+      }
     P.created = 1;
   }
   if (P.used >= 1024) {  // pool full: wait for the stream and fold the records
@@ -351,13 +351,8 @@ void prof_begin(Ctx &c, int tag) {
     prof_harvest(c);
   }
   P.tag[P.used] = tag;
-  hipEventRecord(P.ev[2 * P.used], c.stream);
-}
-void prof_end(Ctx &c) {
-  Prof &P = c.prof;
-  if (!P.on || !P.created || !P.pending) return;
-  P.pending = 0;
-  hipEventRecord(P.ev[2 * P.used + 1], c.stream);
+  *start = P.ev[2 * P.used];
+  *stop = P.ev[2 * P.used + 1];
   ++P.used;
 }
 void prof_harvest(Ctx &c) {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 
 #include "covof_hip.h"
@@ -62,7 +63,6 @@ struct Prof {
   int tag[1024];
   int used = 0;
   int created = 0;
-  int pending = 0;  // a begin event was recorded for the current launch
   double total_ms[COVOF_NUM_KERNEL_TAGS] = {0};
   long count[COVOF_NUM_KERNEL_TAGS] = {0};
 };
@@ -76,13 +76,26 @@ struct Ctx {
   Prof prof;
 };
 
-// Records the begin event of a launch with the given tag (no-op when off or
-// when tag is not a kernel tag).
-void prof_begin(Ctx &c, int tag);
+// The event pair that times the next launch with the given tag, or nulls
+// when profiling is off or tag is not a kernel tag (-1: not sampled).
+void prof_events(Ctx &c, int tag, hipEvent_t *start, hipEvent_t *stop);
 inline bool prof_sampled(const Ctx &c, int k) { return c.prof.on > 0 && (k % c.prof.on) == 0; }
-void prof_end(Ctx &c);
 // Folds finished records into the totals; the stream must be idle.
 void prof_harvest(Ctx &c);
+
+// Launch on the context's stream; a profiled launch goes through
+// hipExtLaunchKernelGGL, which stamps the dispatch's own start and end times
+// into the event pair (no marker packets around the kernel), so the figures
+// are kernel durations, as a kernel trace reports them.
+template <typename... KArgs, typename... Args>
+inline void launch_tagged(Ctx &c, int tag, void (*kernel)(KArgs...), dim3 grid, dim3 block, Args... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  prof_events(c, tag, &e0, &e1);
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, c.stream, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, c.stream, args...);
+}
 
 inline __host__ __device__ long hidx(const Mesh &m, int i, int j) { return (long)j * (m.nx + 2) + i; }
 

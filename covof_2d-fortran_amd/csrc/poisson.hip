@@ -815,14 +815,10 @@ static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *A
   T *dprev = (k & 1) ? d0 : d1;
   T *dnew = (k & 1) ? d1 : d0;
   const int tag = timed ? (sizeof(T) == 8 ? COVOF_KERNEL_PCG_A_FP64 : COVOF_KERNEL_PCG_A_FP32) : -1;
-  prof_begin(c, tag);
-  hipLaunchKernelGGL((k_pcg_a<T>), g, dim3(BLK), 0, c.stream, o, A, zs, dprev, dnew, Ad, c.ps.scal, sl,
-                     (const double *)pb, pa, k);
-  prof_end(c);
-  prof_begin(c, timed ? tag + 1 : -1);
-  hipLaunchKernelGGL((k_pcg_b<T, PC>), g, dim3(BLK), 0, c.stream, o, A.diag, x, r, z, dnew, Ad, c.ps.scal, sl,
-                     (const double *)pa, pb, k);
-  prof_end(c);
+  launch_tagged(c, tag, k_pcg_a<T>, g, dim3(BLK), o, A, zs, (const T *)dprev, dnew, Ad, c.ps.scal, sl,
+                (const double *)pb, pa, k);
+  launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, x, r, z,
+                (const T *)dnew, (const T *)Ad, c.ps.scal, sl, (const double *)pa, pb, k);
   if (PC == 2) {
     OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
     hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, (const double *)r, (double *)z,

@@ -623,14 +623,8 @@ static int launch_recon(Ctx &c, int method, const double *src) {
   const int per_block = method == 1 ? TBLK / 8 : TBLK;
   const int ring_blocks = std::min(256, (L + per_block - 1) / per_block);
   const dim3 grid(nt + ring_blocks);
-  prof_begin(c, COVOF_KERNEL_VOF_RECON);
-  if (method == 1)
-    hipLaunchKernelGGL((k_recon<1>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
-                       c.vof.snx, c.vof.sny, c.vof.sal);
-  else
-    hipLaunchKernelGGL((k_recon<0>), grid, dim3(TBLK), 0, c.stream, src, m, c.vof.list[0], c.vof.count,
-                       c.vof.snx, c.vof.sny, c.vof.sal);
-  prof_end(c);
+  launch_tagged(c, COVOF_KERNEL_VOF_RECON, method == 1 ? k_recon<1> : k_recon<0>, grid, dim3(TBLK), src, m,
+                (const int *)c.vof.list[0], (const int *)c.vof.count, c.vof.snx, c.vof.sny, c.vof.sal);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -651,23 +645,10 @@ static int launch_pass(Ctx &c, int method, int dir, bool first, const double *sr
   const dim3 grd(n_tiles(c.m)), blk(TBLK);
   int *ln = c.vof.list[0], *cn = c.vof.count;
   const int cf = corner_fix ? 1 : 0;
-  prof_begin(c, COVOF_KERNEL_VOF_UPDATE);
-  if (dir == 0) {
-    if (first)
-      hipLaunchKernelGGL((k_flux_update<0, true>), grd, blk, 0, c.stream, src, dst, u, m, dt, c.vof.snx,
-                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
-    else
-      hipLaunchKernelGGL((k_flux_update<0, false>), grd, blk, 0, c.stream, src, dst, u, m, dt, c.vof.snx,
-                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
-  } else {
-    if (first)
-      hipLaunchKernelGGL((k_flux_update<1, true>), grd, blk, 0, c.stream, src, dst, v, m, dt, c.vof.snx,
-                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
-    else
-      hipLaunchKernelGGL((k_flux_update<1, false>), grd, blk, 0, c.stream, src, dst, v, m, dt, c.vof.snx,
-                         c.vof.sny, c.vof.sal, c.vof.cd, ln, cn, cf);
-  }
-  prof_end(c);
+  auto kern = dir == 0 ? (first ? k_flux_update<0, true> : k_flux_update<0, false>)
+                      : (first ? k_flux_update<1, true> : k_flux_update<1, false>);
+  launch_tagged(c, COVOF_KERNEL_VOF_UPDATE, kern, grd, blk, src, dst, dir == 0 ? u : v, m, dt,
+                (const double *)c.vof.snx, (const double *)c.vof.sny, (const double *)c.vof.sal, c.vof.cd, ln, cn, cf);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
