@@ -106,14 +106,15 @@ __device__ __forceinline__ T jacobi(T r, T d) {
 // ---------------------------------------------------- reductions (deterministic)
 //
 // No atomics and no device-scope fences (a release fence on a multi-XCD part
-// writes the whole L2 back): a kernel leaves one partial per block (NP slots)
-// with plain stores, and the NEXT kernel on the stream -- the kernel boundary
-// makes the stores visible -- has every block sum all partials in the same
-// fixed order.  Every block thus derives bit-identical scalars; persistent
+// writes the whole L2 back): a kernel leaves one partial per block and slot
+// with plain stores, slot-major (slot q of block b at part[q * nblk + b], so a
+// slot is one contiguous run), and the NEXT kernel on the stream -- the kernel
+// boundary makes the stores visible -- has every block sum the slots it needs
+// in the same fixed order.  Every block thus derives bit-identical scalars; persistent
 // scalars are written by block 0 only, into slots nobody else reads during
 // that launch (parity-indexed where a value is both read and updated).
 constexpr int BLK = 256;  // threads per block: tile 64 (x) by 4 (y)
-constexpr int NP = 4;     // partial slots per block
+constexpr int NP = 4;     // partial slots per block (at most)
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -125,8 +126,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 __device__ __forceinline__ int block_id() { return blockIdx.y * gridDim.x + blockIdx.x; }
 __device__ __forceinline__ bool lead() { return threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0; }
 
-// Block sums of v[0..NV) (wave sums in T, across waves in double) into
-// part[block*NP + first + q].
+// Block sums of v[0..NV) (wave sums in T, across waves in double) into slot
+// first + q of this block.
 template <int NV, typename T>
 __device__ void block_partials(const T (&v)[NV], double *part, int first = 0) {
   __shared__ double sh[NV][BLK / 64];
@@ -142,21 +143,29 @@ __device__ void block_partials(const T (&v)[NV], double *part, int first = 0) {
     for (int q = 0; q < NV; ++q) {
       double t = 0.0;
       for (int w = 0; w < BLK / 64; ++w) t += sh[q][w];
-      part[(long)block_id() * NP + first + q] = t;
+      part[(long)(first + q) * (gridDim.x * gridDim.y) + block_id()] = t;
     }
   }
 }
 
-// Grid totals of slots [0, NV) of the partials left by the previous kernel
-// (nblk blocks); the same value in every thread of every block.
+// Grid totals tot[q] of slots q0 <= q < q1 (bounds uniform over the grid;
+// the other entries are 0) of the partials left by the previous kernel (nblk
+// blocks, even): the same value in every thread of every block.  Each thread
+// reads pairs of partials with 16-byte loads.
 template <int NV>
-__device__ void grid_totals(const double *part, int nblk, double (&tot)[NV]) {
+__device__ void grid_totals(const double *part, int nblk, double (&tot)[NV], int q0 = 0, int q1 = NV) {
   __shared__ double sh[NV][BLK / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
+    if (q < q0 || q >= q1) continue;
+    const double2 *p2 = reinterpret_cast<const double2 *>(part + (long)q * nblk);
     double t = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += BLK) t += part[(long)b * NP + q];
+    for (int b = threadIdx.x; 2 * b < nblk; b += BLK) {
+      const double2 v = p2[b];
+      t += v.x;
+      t += v.y;
+    }
     t = wave_sum<double>(t);
     if (lane == 0) sh[q][wid] = t;
   }
@@ -164,7 +173,8 @@ __device__ void grid_totals(const double *part, int nblk, double (&tot)[NV]) {
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     double t = 0.0;
-    for (int w = 0; w < BLK / 64; ++w) t += sh[q][w];
+    if (q >= q0 && q < q1)
+      for (int w = 0; w < BLK / 64; ++w) t += sh[q][w];
     tot[q] = t;
   }
 }
@@ -337,43 +347,14 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
                                                const T *__restrict__ dprev, T *__restrict__ d,
                                                T *__restrict__ Ad, double *scal, PcgSlots sl,
                                                const double *pin, double *pout, int k) {
-  if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
-  double tot[3];
-  grid_totals<3>(pin, gridDim.x * gridDim.y, tot);
-  const bool first = scal[sl.first] != 0.0;
-  double ref;
-  if (first && sl.is_b_norm) {
-    ref = sqrt(tot[2]);
-    if (ref < 1.0e-30) ref = 1.0;
-  } else {
-    ref = scal[sl.ref];
-  }
-  const double rn = sqrt(tot[0]);
-  const double rz_prev = scal[sl.rz + ((k - 1) & 1)];
-  int stop = F_RUN;
-  if (!(rn == rn) || rn > 1.0e300) stop = F_NONFINITE;
-  else if (!sl.inner && rn / ref <= scal[sl.tol]) stop = F_CONV;
-  else if (!first && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
-  else if (scal[sl.iters] >= scal[sl.maxit]) stop = F_MAXIT;
-  if (lead()) {
-    if (first && sl.is_b_norm) scal[sl.ref] = ref;
-    scal[sl.rr] = tot[0];
-    scal[sl.relres] = rn / ref;
-    if (stop != F_RUN) {
-      scal[sl.flag] = (double)stop;
-    } else {
-      scal[sl.iters] += 1.0;
-      scal[sl.rz + (k & 1)] = tot[1];
-    }
-  }
-  if (stop != F_RUN) return;
-  const T beta = first ? T(0) : (T)(tot[1] / rz_prev);
-  T acc[1] = {T(0)};
   // Each wave walks a contiguous run of rows of its 64-column strip keeping
   // d of the rows below / at / above in registers: d and the coefficients of a
   // cell are loaded once, x-neighbours come from the adjacent lanes, and only
-  // the strip edges and the periodic wraps load extra values.  The matvec
-  // evaluates poisson_matvec's expression in its order on the same operands.
+  // the strip edges and the periodic wraps load extra values.  Rows go in
+  // batches of G whose loads are all issued before the batch is evaluated, so
+  // each wave keeps several rows of loads in flight.  The matvec evaluates
+  // poisson_matvec's expression in its order on the same operands.
+  constexpr int G = 4;
   const int nx = o.nx, ny = o.ny;
   const int gx_ = (nx + 63) / 64;
   const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;
@@ -387,53 +368,132 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   const int rw = (je - jb + 3) / 4;
   const int jw0 = jb + 1 + w * rw, jw1 = min(jw0 + rw - 1, je);
   const bool vi = i <= nx;
-  auto dv = [&](int a, int b) -> T {
-    const long m = (long)(b - 1) * nx + (a - 1);
-    return zs[m] + beta * dprev[m];
-  };
-  if (local_ < gx_ * chunks_ && jw0 <= jw1) {
-    T down = T(0), cur = T(0);
-    if (vi) {
-      cur = dv(i, jw0);
-      if (jw0 > 1) down = dv(i, jw0 - 1);
-      else if (o.perS) down = dv(i, ny);
+  const bool act = local_ < gx_ * chunks_ && jw0 <= jw1;  // uniform over the wave
+  auto at = [&](int a, int b) { return (long)(b - 1) * nx + (a - 1); };
+  // The first rows' operands do not depend on beta: they are loaded before the
+  // stop flags and the grid totals are read, so the latencies overlap.
+  T z0 = T(0), p0 = T(0), zd = T(0), pd = T(0), aS = T(0);
+  if (act && vi) {
+    z0 = zs[at(i, jw0)];
+    p0 = dprev[at(i, jw0)];
+    const int jd = jw0 > 1 ? jw0 - 1 : (o.perS ? ny : 0);
+    if (jd > 0) {
+      zd = zs[at(i, jd)];
+      pd = dprev[at(i, jd)];
     }
-    T aS = vi ? A.ky[kyi(nx, i, jw0)] : T(0);
-    for (int j = jw0; j <= jw1; ++j) {
-      T up = T(0);
-      if (vi) {
-        if (j < ny) up = dv(i, j + 1);
-        else if (o.perN) up = dv(i, 1);
-      }
-      const T aW = vi ? A.kx[kxi(nx, i, j)] : T(0);
-      const T aN = vi ? A.ky[kyi(nx, i, j + 1)] : T(0);
-      T left = __shfl_up(cur, 1, 64);
-      T right = __shfl_down(cur, 1, 64);
-      T aE = __shfl_down(aW, 1, 64);
-      if (vi) {
-        if (i == 1) left = o.perW ? dv(nx, j) : T(0);
-        else if (lane == 0) left = dv(i - 1, j);
-        if (i == nx) {
-          right = o.perE ? dv(1, j) : T(0);
-          aE = A.kx[kxi(nx, nx + 1, j)];
-        } else if (lane == 63) {
-          right = dv(i + 1, j);
-          aE = A.kx[kxi(nx, i + 1, j)];
+    aS = A.ky[kyi(nx, i, jw0)];
+  }
+  const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
+  const bool first = scal[sl.first] != 0.0;
+  // slots: 0 r.r, 1 r.z, 2 b.b (first outer iteration only); the inner
+  // recurrence of the mixed mode needs r.z only
+  double tot[3];
+  grid_totals<3>(pin, gridDim.x, tot, sl.inner ? 1 : 0, (first && sl.is_b_norm) ? 3 : 2);
+  if (flag_own != 0.0 || flag_outer != 0.0) return;
+  const double rz_prev = scal[sl.rz + ((k - 1) & 1)];
+  int stop = F_RUN;
+  double ref = 1.0, rn = 0.0;
+  if (sl.inner) {
+    if (!(tot[1] == tot[1]) || fabs(tot[1]) > 1.0e300) stop = F_NONFINITE;
+    else if (!first && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+    else if (scal[sl.iters] >= scal[sl.maxit]) stop = F_MAXIT;
+  } else {
+    if (first && sl.is_b_norm) {
+      ref = sqrt(tot[2]);
+      if (ref < 1.0e-30) ref = 1.0;
+    } else {
+      ref = scal[sl.ref];
+    }
+    rn = sqrt(tot[0]);
+    if (!(rn == rn) || rn > 1.0e300) stop = F_NONFINITE;
+    else if (rn / ref <= scal[sl.tol]) stop = F_CONV;
+    else if (!first && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+    else if (scal[sl.iters] >= scal[sl.maxit]) stop = F_MAXIT;
+  }
+  if (lead()) {
+    if (!sl.inner) {
+      if (first && sl.is_b_norm) scal[sl.ref] = ref;
+      scal[sl.rr] = tot[0];
+      scal[sl.relres] = rn / ref;
+    }
+    if (stop != F_RUN) {
+      scal[sl.flag] = (double)stop;
+    } else {
+      scal[sl.iters] += 1.0;
+      scal[sl.rz + (k & 1)] = tot[1];
+    }
+  }
+  if (stop != F_RUN) return;
+  const T beta = first ? T(0) : (T)(tot[1] / rz_prev);
+  T acc[1] = {T(0)};
+  T cur = z0 + beta * p0, down = zd + beta * pd;
+  if (act) {
+    for (int j0 = jw0; j0 <= jw1; j0 += G) {
+      // loads of the batch: up-row z / d_prev, kx(i), ky(i, j+1), and at the
+      // strip edges the x-neighbours and kx(i+1)
+      T zu[G], pu[G], kw[G], kn[G], ke[G], zl[G], pl[G], zr[G], pr[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = j0 + g;
+        zu[g] = pu[g] = kw[g] = kn[g] = ke[g] = zl[g] = pl[g] = zr[g] = pr[g] = T(0);
+        if (j <= jw1 && vi) {
+          const int ju = j < ny ? j + 1 : (o.perN ? 1 : 0);
+          if (ju > 0) {
+            zu[g] = zs[at(i, ju)];
+            pu[g] = dprev[at(i, ju)];
+          }
+          kw[g] = A.kx[kxi(nx, i, j)];
+          kn[g] = A.ky[kyi(nx, i, j + 1)];
+          if (lane == 0) {
+            const int il = i > 1 ? i - 1 : (o.perW ? nx : 0);
+            if (il > 0) {
+              zl[g] = zs[at(il, j)];
+              pl[g] = dprev[at(il, j)];
+            }
+          }
+          if (i == nx) {
+            ke[g] = A.kx[kxi(nx, nx + 1, j)];
+            if (o.perE) {
+              zr[g] = zs[at(1, j)];
+              pr[g] = dprev[at(1, j)];
+            }
+          } else if (lane == 63) {
+            ke[g] = A.kx[kxi(nx, i + 1, j)];
+            zr[g] = zs[at(i + 1, j)];
+            pr[g] = dprev[at(i + 1, j)];
+          }
         }
-        const T aP = aW + aE + aS + aN + (T)o.eps;
-        T val = aP * cur;
-        if (i > 1 || o.perW) val = val - aW * left;
-        if (i < nx || o.perE) val = val - aE * right;
-        if (j > 1 || o.perS) val = val - aS * down;
-        if (j < ny || o.perN) val = val - aN * up;
-        const long q = (long)(j - 1) * nx + (i - 1);
-        d[q] = cur;
-        Ad[q] = val;
-        acc[0] += cur * val;
       }
-      down = cur;
-      cur = up;
-      aS = aN;
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = j0 + g;
+        if (j > jw1) break;  // uniform over the wave
+        const T up = zu[g] + beta * pu[g];
+        T left = __shfl_up(cur, 1, 64);
+        T right = __shfl_down(cur, 1, 64);
+        T aE = __shfl_down(kw[g], 1, 64);
+        if (lane == 0) left = zl[g] + beta * pl[g];
+        if (i == nx || lane == 63) {
+          right = zr[g] + beta * pr[g];
+          aE = ke[g];
+        }
+        if (vi) {
+          const T aW = kw[g], aN = kn[g];
+          const T aP = aW + aE + aS + aN + (T)o.eps;
+          T val = aP * cur;
+          if (i > 1 || o.perW) val = val - aW * left;
+          if (i < nx || o.perE) val = val - aE * right;
+          if (j > 1 || o.perS) val = val - aS * down;
+          if (j < ny || o.perN) val = val - aN * up;
+          const long q = at(i, j);
+          d[q] = cur;
+          Ad[q] = val;
+          acc[0] += cur * val;
+        }
+        down = cur;
+        cur = up;
+        aS = kn[g];
+      }
     }
   }
   block_partials<1, T>(acc, pout);
@@ -444,9 +504,10 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
                                                T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
                                                const T *__restrict__ Ad, double *scal, PcgSlots sl,
                                                const double *pin, double *pout, int k) {
-  if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
+  const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
   double tot[1];
   grid_totals<1>(pin, gridDim.x * gridDim.y, tot);
+  if (flag_own != 0.0 || flag_outer != 0.0) return;
   const double dAd = tot[0];
   if (!(fabs(dAd) >= 1.0e-35)) {
     if (lead()) scal[sl.flag] = (dAd == dAd) ? (double)F_BREAK : (double)F_NONFINITE;
@@ -661,11 +722,11 @@ static DevOp dev_op(const Mesh &m) {
   return o;
 }
 
-// 1-D grid of 8 XCD bands x gx strips x chunks (about 2048 blocks); see CELL_LOOP.
+// 1-D grid of 8 XCD bands x gx strips x chunks (about 1024 blocks); see CELL_LOOP.
 static dim3 red_grid(const Mesh &m) {
   const int gx = (m.nx + 63) / 64;
   const int band = (((m.ny + 7) / 8) + 3) & ~3;
-  int chunks = 2048 / (8 * gx);
+  int chunks = 1024 / (8 * gx);
   if (chunks < 1) chunks = 1;
   const int max_chunks = (band + 3) / 4;  // at least 4 rows per chunk
   if (chunks > max_chunks) chunks = max_chunks;
@@ -793,6 +854,7 @@ constexpr int NREAD = 32;  // scalar slots read back by the host
 // Partial buffers: [0] written by A kernels, [1] by B / residual / rz kernels,
 // [2] by the sum kernel.
 static double *part_buf(Ctx &c, int w) { return c.ps.part + (long)w * c.ps.nblocks * NP; }
+static_assert(NP % 2 == 0, "partial buffers start on 16-byte boundaries");
 
 static int status_of_flag(double f) {
   if (f == F_CONV) return COVOF_OK;
