@@ -1,0 +1,183 @@
+!> ISO_C_BINDING interface of libcovof_hip (include/covof_hip.h) for the
+!> reference Fortran driver.
+!>
+!> This is the module a COVOF build `use`s to route its per-timestep hot path
+!> to the GPU:
+!>   advect_vof(fields, mesh, cfg, dt)        src/module_interface_vof.f90:792
+!>     -> covof_vof_advect(ctx, fields%C, fields%u, fields%v, dt, ...)
+!>   reconstruct_interface(method, mesh, C, segments)
+!>                                            src/module_interface_reconstruction.f90:580
+!>     -> covof_reconstruct_interface_d (device arrays)
+!>   the Poisson block of sim_solve_standard_projection
+!>                                            src/module_simulation.f90:2477-2588
+!>     -> covof_sp_poisson(ctx, u, v, rho_u, rho_v, p, dt, ...)
+!>   the velocity correction                  src/module_simulation.f90:2590-2654
+!>     -> covof_sp_correct(ctx, p, rho_u, rho_v, dt, u, v)
+!>
+!> Arrays are the reference's own: real(dp) F(0:nx+1, 0:ny+1), column-major,
+!> i fastest (src/module_fields.f90:129-169), passed by reference.  Selector
+!> values are those of src/module_constants.f90.  Every function returns an
+!> integer status (COVOF_OK = 0).
+module covof_hip
+  use iso_c_binding
+  implicit none
+  private :: c_str
+
+  integer(c_int), parameter :: COVOF_OK = 0, COVOF_NOT_CONVERGED = 1, COVOF_BREAKDOWN = 2, &
+       COVOF_NONFINITE = 3, COVOF_BAD_ARGUMENT = 4, COVOF_HIP_ERROR = 5
+  integer(c_int), parameter :: COVOF_BC_DIRICHLET = 1, COVOF_BC_NEUMANN = 2, COVOF_BC_PERIODIC = 3, &
+       COVOF_BC_FREESLIP = 4
+  integer(c_int), parameter :: COVOF_LINEAR_BICGSTAB2 = 1, COVOF_LINEAR_CG = 3
+  integer(c_int), parameter :: COVOF_PRECOND_NONE = 0, COVOF_PRECOND_JACOBI = 1, COVOF_PRECOND_SSOR = 2
+  integer(c_int), parameter :: COVOF_PRECISION_FP64 = 0, COVOF_PRECISION_MIXED = 1
+
+  interface
+     integer(c_int) function covof_ctx_create(nx, ny, lx, ly, bc, ctx) bind(C, name="covof_ctx_create")
+       import :: c_int, c_double, c_ptr
+       integer(c_int), value :: nx, ny
+       real(c_double), value :: lx, ly
+       integer(c_int), intent(in) :: bc(4)
+       type(c_ptr), intent(out) :: ctx
+     end function covof_ctx_create
+
+     integer(c_int) function covof_ctx_destroy(ctx) bind(C, name="covof_ctx_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function covof_ctx_destroy
+
+     integer(c_int) function covof_ctx_synchronize(ctx) bind(C, name="covof_ctx_synchronize")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function covof_ctx_synchronize
+
+     type(c_ptr) function covof_last_error_c() bind(C, name="covof_last_error")
+       import :: c_ptr
+     end function covof_last_error_c
+
+     integer(c_int) function covof_vof_advect_c(ctx, C, u, v, dt, adv, rec, step_count, nsteps, two_phase, &
+          rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node) bind(C, name="covof_vof_advect")
+       import :: c_int, c_double, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: C(*)
+       real(c_double), intent(in) :: u(*), v(*)
+       real(c_double), value :: dt
+       character(kind=c_char), intent(in) :: adv(*), rec(*)
+       integer(c_int), intent(inout) :: step_count
+       integer(c_int), value :: nsteps, two_phase
+       real(c_double), value :: rho1, rho2, mu1, mu2
+       type(c_ptr), value :: rho, rho_u, rho_v, mu, mu_node
+     end function covof_vof_advect_c
+
+     integer(c_int) function covof_sp_poisson(ctx, u, v, rho_u, rho_v, p, dt, lin_solver, precond, tol, maxit, &
+          precision, iters, relres) bind(C, name="covof_sp_poisson")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: u(*), v(*), rho_u(*), rho_v(*)
+       real(c_double), intent(inout) :: p(*)
+       real(c_double), value :: dt, tol
+       integer(c_int), value :: lin_solver, precond, maxit, precision
+       integer(c_int), intent(out) :: iters
+       real(c_double), intent(out) :: relres
+     end function covof_sp_poisson
+
+     integer(c_int) function covof_sp_correct(ctx, p, rho_u, rho_v, dt, u, v) bind(C, name="covof_sp_correct")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: p(*), rho_u(*), rho_v(*)
+       real(c_double), value :: dt
+       real(c_double), intent(inout) :: u(*), v(*)
+     end function covof_sp_correct
+
+     ! device-pointer entry points (arrays already resident in HBM)
+     integer(c_int) function covof_vof_advect_d(ctx, C, u, v, dt, adv, rec, step_count, nsteps, two_phase, &
+          rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node) bind(C, name="covof_vof_advect_d")
+       import :: c_int, c_double, c_ptr, c_char
+       type(c_ptr), value :: ctx, C, u, v
+       real(c_double), value :: dt
+       character(kind=c_char), intent(in) :: adv(*), rec(*)
+       integer(c_int), intent(inout) :: step_count
+       integer(c_int), value :: nsteps, two_phase
+       real(c_double), value :: rho1, rho2, mu1, mu2
+       type(c_ptr), value :: rho, rho_u, rho_v, mu, mu_node
+     end function covof_vof_advect_d
+
+     integer(c_int) function covof_reconstruct_interface_d(ctx, method, C, seg_nx, seg_ny, seg_alpha) &
+          bind(C, name="covof_reconstruct_interface_d")
+       import :: c_int, c_ptr, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: method(*)
+       type(c_ptr), value :: C, seg_nx, seg_ny, seg_alpha
+     end function covof_reconstruct_interface_d
+
+     integer(c_int) function covof_sp_poisson_d(ctx, u, v, rho_u, rho_v, p, dt, lin_solver, precond, tol, maxit, &
+          precision, iters, relres) bind(C, name="covof_sp_poisson_d")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, u, v, rho_u, rho_v, p
+       real(c_double), value :: dt, tol
+       integer(c_int), value :: lin_solver, precond, maxit, precision
+       integer(c_int), intent(out) :: iters
+       real(c_double), intent(out) :: relres
+     end function covof_sp_poisson_d
+  end interface
+
+contains
+
+  !> trim(s) // NUL for the C string arguments.
+  function c_str(s) result(cs)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: cs
+    cs = trim(s) // c_null_char
+  end function c_str
+
+  !> advect_vof(fields, mesh, cfg, dt) on host arrays C, u, v (0:nx+1, 0:ny+1):
+  !> nsteps steps of cfg%advection_method with cfg%reconstruction_method;
+  !> step_count is the reference's vof_step_count.  Without the optional
+  !> property arrays, update_physical_properties is not run.
+  integer(c_int) function covof_advect_vof(ctx, C, u, v, dt, advection_method, reconstruction_method, &
+       step_count, nsteps, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node) result(st)
+    type(c_ptr), intent(in) :: ctx
+    real(c_double), intent(inout) :: C(:, :)
+    real(c_double), intent(in) :: u(:, :), v(:, :)
+    real(c_double), intent(in) :: dt
+    character(len=*), intent(in) :: advection_method, reconstruction_method
+    integer(c_int), intent(inout) :: step_count
+    integer(c_int), intent(in) :: nsteps
+    real(c_double), intent(in), optional :: rho1, rho2, mu1, mu2
+    real(c_double), intent(inout), target, optional :: rho(:, :), rho_u(:, :), rho_v(:, :), mu(:, :), mu_node(:, :)
+    type(c_ptr) :: prho, prho_u, prho_v, pmu, pmu_node
+    integer(c_int) :: two_phase
+    real(c_double) :: r1, r2, m1, m2
+    two_phase = 0
+    prho = c_null_ptr; prho_u = c_null_ptr; prho_v = c_null_ptr; pmu = c_null_ptr; pmu_node = c_null_ptr
+    r1 = 1.0_c_double; r2 = 1.0_c_double; m1 = 1.0_c_double; m2 = 1.0_c_double
+    if (present(rho) .and. present(rho_u) .and. present(rho_v) .and. present(mu) .and. present(mu_node) .and. &
+         present(rho1) .and. present(rho2) .and. present(mu1) .and. present(mu2)) then
+       two_phase = 1
+       r1 = rho1; r2 = rho2; m1 = mu1; m2 = mu2
+       prho = c_loc(rho); prho_u = c_loc(rho_u); prho_v = c_loc(rho_v); pmu = c_loc(mu); pmu_node = c_loc(mu_node)
+    end if
+    st = covof_vof_advect_c(ctx, C, u, v, dt, c_str(advection_method), c_str(reconstruction_method), &
+         step_count, nsteps, two_phase, r1, r2, m1, m2, prho, prho_u, prho_v, pmu, pmu_node)
+  end function covof_advect_vof
+
+  !> Message of the last failed call.
+  function covof_last_error() result(msg)
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n
+    p = covof_last_error_c()
+    if (.not. c_associated(p)) then
+       msg = ""
+       return
+    end if
+    call c_f_pointer(p, chars, [4096])
+    n = 0
+    do while (n < 4096)
+       if (chars(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(chars(1:n), msg)
+  end function covof_last_error
+end module covof_hip

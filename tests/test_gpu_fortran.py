@@ -1,0 +1,78 @@
+"""The Fortran side of the boundary: covof_driver (ISO_C_BINDING module
+covof_hip) calls the host-array entry points the way the reference's time loop
+calls advect_vof and the SP Poisson block; its results must equal the oracle's
+(VOF, bit-exact) and the Python path's over the same C-ABI (bit-exact)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, to_gpu
+
+DRIVER = os.path.join(ROOT, "covof_2d-fortran_amd", "fortran", "covof_driver")
+
+
+def _read(path, narrays):
+    with open(path, "rb") as f:
+        hdr = np.frombuffer(f.read(24), dtype=np.int32)
+        dt, relres = np.frombuffer(f.read(16), dtype=np.float64)
+        n = int(hdr[1])
+        rest = np.frombuffer(f.read(), dtype=np.float64)
+    assert rest.size == narrays * (n + 2) * (n + 2)
+    arrays = [a.reshape(n + 2, n + 2).copy() for a in np.split(rest, narrays)]
+    return hdr, float(dt), float(relres), arrays
+
+
+def test_driver_links_and_prints_usage():
+    """CPU: the driver starts (libcovof_hip.so resolves through its runpath)."""
+    assert os.path.exists(DRIVER), "build the Fortran driver with __graft_entry__.build()"
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_vof_advect_matches_oracle(tmp_path, orc):
+    n, steps = 64, 12
+    out = tmp_path / "vof.bin"
+    r = subprocess.run([DRIVER, "vof", str(n), str(steps), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    hdr, dt, _, (C0, u, v, C) = _read(out, 4)
+    assert hdr[0] == 1 and hdr[4] == steps and hdr[5] == 0
+    m = orc.mesh(n, n, bc=(3, 3, 3, 3))
+    Co, sc = C0.copy(), 0
+    for _ in range(steps):
+        Co, sc = orc.wy_step(m, orc.ELVIRA, Co, u, v, dt, sc)
+    assert sc == steps
+    assert np.array_equal(C, Co)
+    assert abs(C[1:n + 1, 1:n + 1].sum() - C0[1:n + 1, 1:n + 1].sum()) <= 1e-12 * C0[1:n + 1, 1:n + 1].sum()
+
+
+@pytest.mark.gpu
+def test_fortran_sp_poisson_matches_python_and_oracle(tmp_path, cv, orc):
+    n = 64
+    out = tmp_path / "sp.bin"
+    r = subprocess.run([DRIVER, "sp", str(n), "1", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    hdr, dt, relres, (us, vs, rho_u, rho_v, p, u, v) = _read(out, 7)
+    assert hdr[0] == 2 and hdr[5] == 0
+    iters = int(hdr[4])
+    # the same C-ABI from Python on device arrays: identical bits
+    ctx = cv.Context(n, n, bc=(cv.BC_FREESLIP,) * 4)
+    pg = to_gpu(np.zeros_like(p))
+    res = ctx.solve_standard_projection_poisson(to_gpu(us), to_gpu(vs), to_gpu(rho_u), to_gpu(rho_v), pg, dt)
+    ug, vg = to_gpu(us), to_gpu(vs)
+    ctx.correct_velocity(pg, to_gpu(rho_u), to_gpu(rho_v), dt, ug, vg)
+    assert res.iters == iters and res.relres == relres
+    assert np.array_equal(pg.cpu().numpy(), p)
+    assert np.array_equal(ug.cpu().numpy(), u) and np.array_equal(vg.cpu().numpy(), v)
+    # and the oracle's SP loop (CG + SGS, 1e-7, 1000): iterations +-1, p to 1e-6
+    m = orc.mesh(n, n, bc=(4, 4, 4, 4))
+    co = orc.poisson_assemble(m, us, vs, rho_u, rho_v, dt)
+    xo, it, _ = orc.sp_pcg(m, 2, co, co["b"], np.zeros((n, n)), tol=1e-7, maxit=1000)
+    po = orc.sp_finish_pressure(m, xo, np.zeros_like(p))
+    assert abs(iters - it) <= 1, (iters, it)
+    scale = np.abs(po[1:n + 1, 1:n + 1]).max()
+    assert np.abs(p[1:n + 1, 1:n + 1] - po[1:n + 1, 1:n + 1]).max() / scale < 1e-6
