@@ -13,16 +13,26 @@
  * checker) and bench.py's cpu_baseline leg (as the timed CPU baseline).  The
  * shipped library never links, loads or calls anything here.
  *
- * Pins (where each function is checked; see tests/):
- *   cut_box_area, compute_alpha   golden vectors from the compiled reference,
- *                                 closed-form areas, bisection round trip
- *   reconstruct_* / interface     golden vectors (PLIC, ELVIRA, periodic, walls),
- *                                 straight-line exactness of ELVIRA
+ * Pins (where each function is checked):
+ *   cut_box_area, compute_alpha   golden vectors from the compiled reference
+ *                                 (tests/test_oracle_golden.py); closed-form
+ *                                 areas, quadrature of random cuts, bisection
+ *                                 round trip and thresholds (test_oracle_pins.py)
+ *   reconstruct_* / interface     golden vectors (PLIC, ELVIRA; periodic, walls,
+ *                                 mixed); ELVIRA straight-line exactness and
+ *                                 Youngs = gradient of a linear field (pins)
  *   wy_step, apply_vof_bc,        golden multi-step vectors (both Strang orders,
- *   update_properties             periodic and walls), exact volume telescoping
- *   poisson_assemble/matvec/sgs   symmetry, SPD, explicit (D-L)D^-1(D-U) solve,
- *   sp_pcg, bicgstab2             direct sparse solve, golden SP vectors
- *   sp_correct                    discrete divergence-free after projection
+ *   update_properties             periodic and walls, with properties); volume
+ *                                 telescoping and exact translation (pins)
+ *   poisson_assemble / matvec     closed-form spectra (NEUMANN, periodic, walls),
+ *                                 symmetry, SPD, the quadratic form of the face
+ *                                 coefficients (pins)
+ *   poisson_sgs / precond         the explicit (D-L)D^-1(D-U) solve on walls;
+ *                                 Jacobi = r / a_P (pins)
+ *   sp_pcg, bicgstab2             dense direct solves (pins)
+ *   sp_finish_pressure,           discrete divergence of the corrected velocity
+ *   sp_correct                    equals the solve residual (pins)
+ *   bench_jacobi_pcg              timing loop only (the CPU baseline)
  */
 #include "covof_oracle.h"
 
