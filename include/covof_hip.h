@@ -164,11 +164,13 @@ int covof_sp_correct(covof_ctx *ctx, const double *p, const double *rho_u, const
 #define COVOF_KERNEL_VOF_RECON 4  /* PLIC/ELVIRA reconstruction of one pass */
 #define COVOF_KERNEL_VOF_UPDATE 5 /* flux + update + halo of one pass */
 #define COVOF_NUM_KERNEL_TAGS 6
-/* on > 0: bracket launches of the tagged kernels with HIP events on the
- * context's stream and accumulate their durations -- every launch of the VOF
- * kernels, the Krylov kernels of every on-th iteration (a sampling period that
- * keeps the event overhead out of a timed solve); on == 0: stop.  Turning it
- * on clears the totals. */
+/* on > 0: time launches of the tagged kernels from their own dispatch start
+ * and end timestamps (hipExtLaunchKernelGGL event pairs on the context's
+ * stream) and accumulate the durations -- every launch of the VOF kernels,
+ * the Krylov kernels of every on-th iteration (a sampling period that keeps
+ * the bookkeeping out of a timed solve; timed solves run their kernels as
+ * direct launches instead of graph replays); on == 0: stop.  Turning it on
+ * clears the totals. */
 int covof_ctx_profile(covof_ctx *ctx, int on);
 /* Accumulated duration (ms) and number of launches of one kernel tag. */
 int covof_ctx_profile_read(covof_ctx *ctx, int tag, double *total_ms, long *count);

@@ -120,3 +120,37 @@ def test_bicgstab2_vs_oracle(cv, orc):
     xg = x.cpu().numpy()
     d = (xg - xg.mean()) - (xo - xo.mean())
     assert np.abs(d).max() / np.abs(xo - xo.mean()).max() < 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_projection_pressure_velocity_parity(cv, orc, precision):
+    """The SP projection (assembly, Jacobi-PCG solve, mean removal, velocity
+    correction) against the oracle's on the same inputs: pressure and velocity
+    within 1e-10 relative in FP64 mode; the mixed mode's error against the
+    same FP64 oracle is reported and bounded."""
+    n = 128
+    c = _case(n, circles=8)
+    rng = np.random.default_rng(9)
+    u = c["u"] + 1e-2 * rng.normal(size=c["u"].shape)
+    v = c["v"].copy()
+    u[:, 1] = u[:, n + 1] = 0.0
+    dt = 1e-2
+    ctx = cv.Context(n, n, bc=c["bc"])
+    pg = to_gpu(np.zeros((n + 2, n + 2)))
+    res = ctx.solve_standard_projection_poisson(to_gpu(u), to_gpu(v), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), pg,
+                                                dt, precond=1, tol=1e-13, max_iter=200000, precision=precision)
+    assert res.converged, res
+    ug, vg = to_gpu(u), to_gpu(v)
+    ctx.correct_velocity(pg, to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), dt, ug, vg)
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, u, v, c["rho_u"], c["rho_v"], dt)
+    xo, it, _ = orc.sp_pcg(m, 1, co, co["b"], np.zeros((n, n)), tol=1e-13, maxit=200000)
+    po = orc.sp_finish_pressure(m, xo, np.zeros((n + 2, n + 2)))
+    uo, vo = orc.sp_correct(m, po, c["rho_u"], c["rho_v"], dt, u, v)
+    I = (slice(1, n + 1), slice(1, n + 1))
+    ep = np.abs(pg.cpu().numpy()[I] - po[I]).max() / np.abs(po[I]).max()
+    eu = np.abs(ug.cpu().numpy() - uo).max() / np.abs(uo).max()
+    ev = np.abs(vg.cpu().numpy() - vo).max() / np.abs(vo).max()
+    print(f"precision {precision}: {res.iters} iterations (oracle {it}); rel err p {ep:.2e} u {eu:.2e} v {ev:.2e}")
+    bound = 1e-10 if precision == 0 else 1e-9
+    assert ep < bound and eu < bound and ev < bound
