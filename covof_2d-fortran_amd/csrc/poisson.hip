@@ -33,6 +33,7 @@ namespace covof {
 enum Slot {
   S_BNORM = 0, S_RZ = 1, /* and 2 */ S_RR = 4, S_RELRES = 6,
   S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_FIRST = 10, S_MAXIT = 11, S_DOT0 = 12,
+  S_RMEAN = 13,  // constant-mode handling of the mixed mode
   // inner (FP32) solve of the mixed mode
   I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
   I_RR = 23, I_FIRST = 26, I_MAXIT = 27,
@@ -545,21 +546,18 @@ __global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const double *__restric
 }
 
 // Initial residual of the SP loop (:2538-2549): r = b - A x, z = r / a_P
-// (Jacobi), partials r.r, r.z, b.b.  r32 (optional) receives the FP32 copy of
-// r for the inner solve of the mixed mode.
+// (Jacobi), partials r.r, r.z, b.b and sum(r).
 template <int PC>
 __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const double *__restrict__ b,
                                                   const double *__restrict__ x, double *__restrict__ r,
-                                                  double *__restrict__ z, float *__restrict__ r32,
-                                                  double *pout, const double *scal) {
+                                                  double *__restrict__ z, double *pout, const double *scal) {
   if (scal[S_FLAG] != 0.0) return;
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
     const double ax = apply_op<double>(o, A, [&](long m) { return x[m]; }, i, j, q);
     const double rk = b[q] - ax;
     r[q] = rk;
-    if (r32) r32[q] = (float)rk;
     acc[0] += rk * rk;
     if (PC == 0) acc[1] += rk * rk;
     if (PC == 1) {
@@ -568,18 +566,31 @@ __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const 
       acc[1] += rk * zk;
     }
     acc[2] += b[q] * b[q];
+    acc[3] += rk;
   }
-  block_partials<3, double>(acc, pout);
+  block_partials<4, double>(acc, pout);
 }
 
 // FP64 residual check of the mixed mode's reliable updates (one block):
 // ||b|| (first time), ||r||, converged when ||r||/||b|| <= tol; the FP32
 // recurrence's iteration cap ends the solve too.
-__global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int nblk, double *scal, int first) {
+//
+// Constant mode: when no face is a NEUMANN (pressure-Dirichlet) edge, the
+// operator is the singular face Laplacian L plus the 1e-12*dvol shift, which
+// vanishes from the FP32 copy, so the FP32 operator is singular.  Its
+// recurrence must then see a consistent right-hand side and must not carry
+// the null vector into x: the inner solve gets r - mean(r) (S_RMEAN) and the
+// update adds the correction without its mean (k_add_correction).  x keeps
+// the constant of the initial guess; for a right-hand side whose mean is a
+// rounding residue (the projection's b = -div u* telescopes to the boundary
+// flux) the FP64 residual then converges like the FP64 solve's.
+__global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int nblk, double *scal, int first,
+                                                        int singular, double inv_n) {
   if (scal[S_FLAG] != 0.0) return;
-  double tot[3];
-  grid_totals<3>(pin, nblk, tot);
+  double tot[4];
+  grid_totals<4>(pin, nblk, tot);
   if (threadIdx.x != 0) return;
+  scal[S_RMEAN] = singular ? tot[3] * inv_n : 0.0;
   if (scal[I_FLAG] == (double)F_MAXIT || scal[I_ITERS] >= scal[I_MAXIT]) {
     scal[S_FLAG] = (double)F_MAXIT;
     return;
@@ -596,27 +607,48 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
   else if (rn / scal[S_BNORM] <= scal[S_TOL]) scal[S_FLAG] = (double)F_CONV;
 }
 
-// Reliable update x += (double) dx32, dx32 = 0.
-__global__ void k_add_correction(long n, double *__restrict__ x, float *__restrict__ dx32, const double *scal) {
+// sum(dx32) into slot 0 of this kernel's partials (singular operator only).
+__global__ void __launch_bounds__(BLK) k_sum32(DevOp o, const float *__restrict__ x, double *pout,
+                                               const double *scal) {
   if (scal[S_FLAG] != 0.0) return;
+  double acc[1] = {0.0};
+  CELL_LOOP(o) { acc[0] += (double)x[(long)(j - 1) * o.nx + (i - 1)]; }
+  block_partials<1, double>(acc, pout);
+}
+
+// Reliable update x += (double) dx32 (without dx32's mean when the operator
+// is singular), dx32 = 0.  pin: the partials of k_sum32 (nblk blocks), read
+// when singular.
+__global__ void __launch_bounds__(BLK) k_add_correction(long n, double *__restrict__ x, float *__restrict__ dx32,
+                                                        const double *scal, const double *pin, int nblk,
+                                                        int singular, double inv_n) {
+  if (scal[S_FLAG] != 0.0) return;
+  double shift = 0.0;
+  if (singular) {
+    double tot[1];
+    grid_totals<1>(pin, nblk, tot);
+    shift = -(tot[0] * inv_n);
+  }
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-    x[t] = x[t] + (double)dx32[t];
+    x[t] = x[t] + ((double)dx32[t] + shift);
     dx32[t] = 0.0f;
   }
 }
 
-// FP32 inner-solve setup: z32 = r32 / a_P (Jacobi) or r32, partials r.r,
-// r.z, r.r (the last is the inner reference norm ||r0||^2).
+// FP32 inner-solve setup: r32 = r - S_RMEAN rounded to FP32, z32 = r32 / a_P
+// (Jacobi) or r32, partials r.r, r.z, r.r.
 template <int PC>
-__global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const float *__restrict__ r32,
-                                                    const float *__restrict__ d32, float *__restrict__ z32,
-                                                    double *pout, double *scal) {
-  if (scal[S_FLAG] != 0.0) return;
+__global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const double *__restrict__ r,
+                                                    float *__restrict__ r32, const float *__restrict__ d32,
+                                                    float *__restrict__ z32, double *pout, double *scal) {
+  const double flag = scal[S_FLAG], rmean = scal[S_RMEAN];
+  if (flag != 0.0) return;
   if (lead()) scal[I_FLAG] = 0.0;  // a breakdown of the FP32 recurrence restarts from the true residual
   float acc[3] = {0.f, 0.f, 0.f};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
-    const float rk = r32[q];
+    const float rk = (float)(r[q] - rmean);
+    r32[q] = rk;
     acc[0] += rk * rk;
     if (PC == 0) acc[1] += rk * rk;
     if (PC == 1) {
@@ -899,18 +931,26 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
   const DevOp o = dev_op(c.m);
   const dim3 g = red_grid(c.m);
   OpT<double> A{P.kx, P.ky, P.diag};
-  double *pb = part_buf(c, 1);
-  if (!first) hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(256), 0, c.stream, n, x, P.x32,
-                                 (const double *)P.scal);
-  hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, (const double *)x, P.r, P.z, P.r32, pb,
+  double *pb = part_buf(c, 1), *ps = part_buf(c, 2);
+  const Mesh &m = c.m;
+  const int singular = (m.bc[0] != BC_NEUMANN && m.bc[1] != BC_NEUMANN && m.bc[2] != BC_NEUMANN &&
+                        m.bc[3] != BC_NEUMANN) ? 1 : 0;
+  const double inv_n = 1.0 / (double)n;
+  if (!first) {
+    if (singular)
+      hipLaunchKernelGGL(k_sum32, g, dim3(BLK), 0, c.stream, o, (const float *)P.x32, ps, (const double *)P.scal);
+    hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(BLK), 0, c.stream, n, x, P.x32, (const double *)P.scal,
+                       (const double *)ps, P.nblocks, singular, inv_n);
+  }
+  hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, (const double *)x, P.r, P.z, pb,
                      (const double *)P.scal);
   hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
-                     first);
+                     first, singular, inv_n);
   if (pc == 1)
-    hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, (const float *)P.r32,
+    hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, P.r32,
                        (const float *)P.diag32, P.z32, pb, P.scal);
   else
-    hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const float *)P.r32,
+    hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, P.r32,
                        (const float *)P.diag32, P.z32, pb, P.scal);
 }
 
@@ -1023,10 +1063,10 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
   double *pb = part_buf(c, 1);
   if (pc == 1)
     hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
-                       P.r, P.z, (float *)nullptr, pb, (const double *)P.scal);
+                       P.r, P.z, pb, (const double *)P.scal);
   else
     hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
-                       P.r, P.z, (float *)nullptr, pb, (const double *)P.scal);
+                       P.r, P.z, pb, (const double *)P.scal);
   COVOF_CHECK_LAUNCH();
   if (pc == 2) {
     // z = SGS(r) and r.z into slot 1 of the residual partials, then launch by launch
