@@ -145,6 +145,27 @@ def main():
     iter_gbs = (bytes_a + bytes_b) / ((avg_a + avg_b) * 1e-3) / 1e9
     log(f"dominant {dom}: {dom_ms * 1e3:.1f} us avg over {dom_n} launches, {achieved:.0f} GB/s")
 
+    # ------------------------------------------- multigrid-preconditioned CG
+    # the same system with the library's V-cycle preconditioner (no reference
+    # counterpart), FP64 and mixed, timed the same way
+    mg = {}
+    for name, prec in (("fp64", cv.PRECISION_FP64), ("mixed", cv.PRECISION_MIXED)):
+        x = ctx.new_work()
+        ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=tol, max_iter=20000, precision=prec)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            x = ctx.new_work()
+            rmg = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=tol, max_iter=20000, precision=prec)
+        torch.cuda.synchronize()
+        dt_mg = (time.perf_counter() - t) / reps
+        assert rmg.converged and rmg.relres <= tol, rmg
+        mg[name] = {"s": dt_mg, "iters": rmg.iters, "relres": rmg.relres}
+        log(f"MG-PCG {name}: {rmg.iters} it, {dt_mg * 1e3:.1f} ms")
+    mg["speedup_mixed_mg_vs_fp64_jacobi"] = t64 / mg["mixed"]["s"]
+    mg["speedup_fp64_mg_vs_fp64_jacobi"] = t64 / mg["fp64"]["s"]
+
     # -------------------------------------------------------------------- VOF
     vn = n
     t0 = time.time()
@@ -221,6 +242,7 @@ def main():
                      "traffic": TRAFFIC_2048.get(dom) if n == 2048 else None,
                      "bytes_per_launch": dom_bytes, "avg_launch_us": dom_ms * 1e3, "launches": dom_n},
         "cpu_baseline": cpu,
+        "multigrid_pcg": mg,
         "vof": vof,
     }
     print(json.dumps(out), flush=True)

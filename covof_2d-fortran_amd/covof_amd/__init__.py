@@ -12,7 +12,7 @@ correction) over torch CUDA tensors in the reference's field layout: a field
 device memory and streams; every computation runs in libcovof_hip.
 """
 from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BICGSTAB2,  # noqa: F401
-                   LINEAR_CG, PRECISION_FP64, PRECISION_MIXED, PRECOND_JACOBI, PRECOND_NONE,
-                   PRECOND_SSOR, CovofError)
+                   LINEAR_CG, PRECISION_FP64, PRECISION_MIXED, PRECOND_JACOBI, PRECOND_MG, PRECOND_NONE,
+                   PRECOND_SGS_RB, PRECOND_SSOR, CovofError)
 from .context import Context, SolveResult  # noqa: F401
 from .autograd import PressureSolve, pressure_solve  # noqa: F401

@@ -21,6 +21,8 @@ STATUS_NAMES = {OK: "ok", NOT_CONVERGED: "not converged", BREAKDOWN: "breakdown"
 BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
 LINEAR_BICGSTAB2, LINEAR_CG = 1, 3
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR = 0, 1, 2
+# parallel preconditioners of this library (no reference counterpart)
+PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 # every symbol the header declares: name -> (restype, argtypes)

@@ -35,6 +35,21 @@ struct PGraph {
   int key = -1;
 };
 
+// Multigrid hierarchy of one precision (multigrid.hip).  Level 0 is the
+// assembled operator; levels 1.. own their face coefficients, diagonal,
+// correction and right-hand side.
+constexpr int MG_MAX_LEVELS = 16;
+enum { MG_KIND_VCYCLE = 0, MG_KIND_SGS_RB = 1 };
+struct MgLevelBuf {
+  int nx = 0, ny = 0;
+  void *kx = nullptr, *ky = nullptr, *diag = nullptr, *x = nullptr, *b = nullptr;
+};
+struct Mg {
+  int nlev = 0;
+  int built = 0;  // assembly_id the coarse operators were made from
+  MgLevelBuf lev[MG_MAX_LEVELS];
+};
+
 struct Poisson {
   // face coefficients, FP64: kx((nx+1) x ny, row pitch padded to 64) kx(i,j) = a_W(i,j), kx(nx+1,j) = a_E(nx,j)
   //                          ky(nx x (ny+1))  ky(i,j) = a_S(i,j), ky(i,ny+1) = a_N(i,ny)
@@ -49,6 +64,8 @@ struct Poisson {
   unsigned *ticket = nullptr;  // last-block-done counter
   double *hscal = nullptr;  // pinned host mirror of scal
   int assembled = 0;
+  int assembly_id = 0;      // incremented by every assembly
+  Mg mg64, mg32;
   int nblocks = 0;          // reduction grid size
   PGraph graph[2];          // FP64 cycle, mixed cycle
   hipEvent_t cycle_ev[2] = {nullptr, nullptr};
@@ -107,6 +124,13 @@ inline __host__ __device__ long hidx(const Mesh &m, int i, int j) { return (long
   } while (0)
 
 #define COVOF_CHECK_LAUNCH() COVOF_CHECK(hipGetLastError())
+
+int mg_check(const Ctx &c, int kind);
+int mg_prepare(Ctx &c, bool fp32);
+// z = M^-1 r; f0, f1: the scalar slots of the solve's stop flags
+void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1);
+void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1);
+void mg_free(Ctx &c);
 
 int vof_alloc(Ctx &c);
 void vof_free(Ctx &c);

@@ -40,6 +40,7 @@ enum Slot {
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
+  S_ZERO = 63,  // never written: the stop slot of launches outside a solve
   NSLOT = 64
 };
 enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_MAXIT = 5 };
@@ -533,17 +534,19 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
   block_partials<3, T>(acc, pout);
 }
 
-// r.z with an explicit z (SGS) into slot 1 of B's partials.
-__global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const double *__restrict__ r, const double *__restrict__ z,
+// r.z with an explicit z (SGS, RB-SGS, multigrid) into slot 1 of B's partials.
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const T *__restrict__ r, const T *__restrict__ z,
                                                 const double *scal, PcgSlots sl, double *pout) {
   if (scal[sl.flag] != 0.0 || scal[S_FLAG] != 0.0) return;
-  double acc[1] = {0.0};
+  T acc[1] = {T(0)};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
     acc[0] += r[q] * z[q];
   }
-  block_partials<1, double>(acc, pout, 1);
+  block_partials<1, T>(acc, pout, 1);
 }
+
 
 // Initial residual of the SP loop (:2538-2549): r = b - A x, z = r / a_P
 // (Jacobi), partials r.r, r.z, b.b and sum(r).
@@ -809,6 +812,7 @@ void poisson_free(Ctx &c) {
   if (P.hscal) hipHostFree(P.hscal);
   for (PGraph &G : P.graph)
     if (G.exec) hipGraphExecDestroy(G.exec);
+  mg_free(c);
   if (P.cycle_ev[0]) hipEventDestroy(P.cycle_ev[0]);
   if (P.cycle_ev[1]) hipEventDestroy(P.cycle_ev[1]);
 }
@@ -829,6 +833,7 @@ int poisson_assemble(Ctx &c, const double *u, const double *v, const double *rho
                      c.ps.diag32);
   COVOF_CHECK_LAUNCH();
   c.ps.assembled = 1;
+  ++c.ps.assembly_id;
   return COVOF_OK;
 }
 
@@ -841,6 +846,39 @@ int poisson_matvec(Ctx &c, const double *x, double *y) {
   return COVOF_OK;
 }
 
+// Preconditioner codes of the fused PCG: 0 none, 1 Jacobi, 2 exact SGS
+// (lexicographic, sequential), 3 red-black SGS, 4 multigrid V-cycle.
+static int pc_code(int precond) {
+  switch (precond) {
+    case COVOF_PRECOND_NONE: return 0;
+    case COVOF_PRECOND_JACOBI: return 1;
+    case COVOF_PRECOND_SSOR: return 2;
+    case COVOF_PRECOND_SGS_RB: return 3;
+    case COVOF_PRECOND_MG: return 4;
+    default: return -1;
+  }
+}
+static int mg_kind(int pc) { return pc == 3 ? MG_KIND_SGS_RB : MG_KIND_VCYCLE; }
+
+// z = M^-1 r for the explicit preconditioners of precision T.
+template <typename T>
+static void explicit_precond(Ctx &c, int pc, const T *r, T *z, int fslot);
+template <>
+void explicit_precond<double>(Ctx &c, int pc, const double *r, double *z, int fslot) {
+  if (pc == 2) {
+    const DevOp o = dev_op(c.m);
+    OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
+    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, r, z,
+                       (const double *)(c.ps.scal + fslot));
+  } else {
+    mg_apply64(c, mg_kind(pc), r, z, S_FLAG, fslot);
+  }
+}
+template <>
+void explicit_precond<float>(Ctx &c, int pc, const float *r, float *z, int fslot) {
+  mg_apply32(c, mg_kind(pc), r, z, S_FLAG, fslot);
+}
+
 int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
   if (!c.ps.assembled) return COVOF_BAD_ARGUMENT;
   const long n = (long)c.m.nx * c.m.ny;
@@ -851,6 +889,12 @@ int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
   } else if (precond == COVOF_PRECOND_JACOBI || precond == COVOF_PRECOND_NONE) {
     hipLaunchKernelGGL(k_precond_simple, dim3((n + 255) / 256), dim3(256), 0, c.stream, precond, n,
                        c.ps.diag, r, z);
+  } else if (precond == COVOF_PRECOND_SGS_RB || precond == COVOF_PRECOND_MG) {
+    const int pc = pc_code(precond);
+    if (mg_check(c, mg_kind(pc))) return COVOF_BAD_ARGUMENT;
+    int st;
+    if ((st = mg_prepare(c, false))) return st;
+    mg_apply64(c, mg_kind(pc), r, z, S_ZERO, S_ZERO);
   } else {
     return COVOF_BAD_ARGUMENT;
   }
@@ -913,12 +957,9 @@ static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *A
                 (const double *)pb, pa, k);
   launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, x, r, z,
                 (const T *)dnew, (const T *)Ad, c.ps.scal, sl, (const double *)pa, pb, k);
-  if (PC == 2) {
-    OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
-    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, (const double *)r, (double *)z,
-                       (const double *)(c.ps.scal + sl.flag));
-    hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)r, (const double *)z, c.ps.scal,
-                       sl, pb);
+  if (PC >= 2) {
+    explicit_precond<T>(c, PC, (const T *)r, z, sl.flag);
+    hipLaunchKernelGGL(k_pcg_rz<T>, g, dim3(BLK), 0, c.stream, o, (const T *)r, (const T *)z, c.ps.scal, sl, pb);
   }
 }
 
@@ -952,6 +993,11 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
   else
     hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, P.r32,
                        (const float *)P.diag32, P.z32, pb, P.scal);
+  if (pc >= 3) {  // explicit z32 = M^-1 r32, r.z into slot 1
+    explicit_precond<float>(c, pc, (const float *)P.r32, P.z32, INNER.flag);
+    hipLaunchKernelGGL(k_pcg_rz<float>, g, dim3(BLK), 0, c.stream, o, (const float *)P.r32, (const float *)P.z32,
+                       P.scal, INNER, pb);
+  }
 }
 
 // One solver cycle: ITERS PCG iterations (FP64 solve) or ITERS FP32
@@ -966,10 +1012,15 @@ static void enqueue_cycle(Ctx &c, int pc, bool mixed, int iters) {
   for (int k = 1; k <= iters; ++k) {
     const bool timed = prof_sampled(c, k);
     if (mixed) {
-      if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, P.p32, P.p32 + n, P.q32, INNER, k, timed);
-      else pcg_iteration<float, 0>(c, A32, P.x32, P.r32, P.z32, P.p32, P.p32 + n, P.q32, INNER, k, timed);
+      float *d0 = P.p32, *d1 = P.p32 + n;
+      if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
+      else if (pc == 3) pcg_iteration<float, 3>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
+      else if (pc == 4) pcg_iteration<float, 4>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
+      else pcg_iteration<float, 0>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
     } else {
       if (pc == 1) pcg_iteration<double, 1>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
+      else if (pc == 3) pcg_iteration<double, 3>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
+      else if (pc == 4) pcg_iteration<double, 4>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
       else pcg_iteration<double, 0>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, timed);
     }
   }
@@ -1043,7 +1094,9 @@ static int set_scalars(Ctx &c, std::initializer_list<std::pair<int, double>> kv)
   return COVOF_OK;
 }
 
-constexpr int CYCLE64 = 64;  // PCG iterations per captured FP64 cycle
+// PCG iterations per captured FP64 cycle (a multigrid iteration costs many
+// launches and the solve takes few of them: shorter cycles)
+static int cycle64(int pc) { return pc == 4 ? 8 : (pc == 3 ? 32 : 64); }
 
 // FP64 PCG of the SP loop on the context vectors P.b -> P.x.  NONE / JACOBI
 // run as captured cycles of fused kernels; SSOR (the sequential SGS
@@ -1059,7 +1112,8 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
   if ((st = fill<double>(c, P.p, n, 0.0))) return st;
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, NREAD, 0.0);
   if ((st = set_scalars(c, {{S_TOL, tol}, {S_FIRST, 1.0}, {S_MAXIT, (double)maxit}}))) return st;
-  const int pc = precond == COVOF_PRECOND_SSOR ? 2 : (precond == COVOF_PRECOND_JACOBI ? 1 : 0);
+  const int pc = pc_code(precond);
+  if (pc >= 3 && (st = mg_prepare(c, false))) return st;
   double *pb = part_buf(c, 1);
   if (pc == 1)
     hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
@@ -1084,7 +1138,14 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
       }
     }
   } else {
-    if ((st = run_cycles(c, P.graph[0], 1 + pc, pc, false, CYCLE64, maxit / CYCLE64 + 2))) return st;
+    if (pc >= 3) {  // z = M^-1 r and r.z into slot 1 of the residual partials
+      explicit_precond<double>(c, pc, (const double *)P.r, P.z, OUTER.flag);
+      hipLaunchKernelGGL(k_pcg_rz<double>, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
+                         (const double *)P.scal, OUTER, pb);
+      COVOF_CHECK_LAUNCH();
+    }
+    const int cyc = cycle64(pc);
+    if ((st = run_cycles(c, P.graph[0], 1 + pc, pc, false, cyc, maxit / cyc + 2))) return st;
   }
   if ((st = read_scalars(c, NREAD))) return st;
   const int it = (int)P.hscal[S_ITERS];
@@ -1106,12 +1167,13 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
 static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
   const long n = (long)c.m.nx * c.m.ny;
-  const int pc = precond == COVOF_PRECOND_JACOBI ? 1 : 0;
+  const int pc = pc_code(precond);
   const char *cyc_env = getenv("COVOF_MIXED_CYCLE");
-  int cycle = cyc_env ? atoi(cyc_env) : 128;
+  int cycle = cyc_env ? atoi(cyc_env) : (pc == 4 ? 8 : (pc == 3 ? 32 : 128));
   if (cycle < 2) cycle = 2;
   cycle &= ~1;  // even: the parity pattern of k repeats from one replay to the next
   int st;
+  if (pc >= 3 && (st = mg_prepare(c, true))) return st;
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, NREAD, 0.0);
   if ((st = set_scalars(c, {{S_TOL, tol}, {I_FIRST, 1.0}, {I_MAXIT, (double)maxit}, {S_MAXIT, (double)maxit}})))
     return st;
@@ -1135,8 +1197,9 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
 int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol,
                   int maxit, int precision, int *iters, double *relres) {
   if (!c.ps.assembled) return COVOF_BAD_ARGUMENT;
-  if (precond != COVOF_PRECOND_NONE && precond != COVOF_PRECOND_JACOBI && precond != COVOF_PRECOND_SSOR)
-    return COVOF_BAD_ARGUMENT;
+  const int pc = pc_code(precond);
+  if (pc < 0) return COVOF_BAD_ARGUMENT;
+  if (pc >= 3 && mg_check(c, mg_kind(pc))) return COVOF_BAD_ARGUMENT;
   if (maxit < 0 || !(tol >= 0.0)) return COVOF_BAD_ARGUMENT;
   if (lin_solver == COVOF_LINEAR_CG) {
     if (precision == COVOF_PRECISION_MIXED && precond == COVOF_PRECOND_SSOR) return COVOF_BAD_ARGUMENT;

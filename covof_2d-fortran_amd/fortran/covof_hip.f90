@@ -29,6 +29,8 @@ module covof_hip
        COVOF_BC_FREESLIP = 4
   integer(c_int), parameter :: COVOF_LINEAR_BICGSTAB2 = 1, COVOF_LINEAR_CG = 3
   integer(c_int), parameter :: COVOF_PRECOND_NONE = 0, COVOF_PRECOND_JACOBI = 1, COVOF_PRECOND_SSOR = 2
+  !> parallel preconditioners of the GPU library (no reference counterpart)
+  integer(c_int), parameter :: COVOF_PRECOND_SGS_RB = 16, COVOF_PRECOND_MG = 17
   integer(c_int), parameter :: COVOF_PRECISION_FP64 = 0, COVOF_PRECISION_MIXED = 1
 
   interface

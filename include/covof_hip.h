@@ -48,6 +48,11 @@ extern "C" {
 #define COVOF_PRECOND_NONE 0
 #define COVOF_PRECOND_JACOBI 1
 #define COVOF_PRECOND_SSOR 2 /* symmetric Gauss-Seidel, lexicographic (reference SP) */
+/* Parallel preconditioners with no reference counterpart (values clear of
+ * the reference's 0-5; they take the place PRECOND_CUSTOM = 5 gives a user
+ * hook, src/module_linear_solvers.f90:22-41): */
+#define COVOF_PRECOND_SGS_RB 16 /* one symmetric red-black Gauss-Seidel sweep */
+#define COVOF_PRECOND_MG 17     /* one geometric multigrid V-cycle (RB-GS smoothing) */
 /* Arithmetic of the Poisson solve. */
 #define COVOF_PRECISION_FP64 0
 #define COVOF_PRECISION_MIXED 1 /* FP32 inner Krylov, FP64 iterative refinement */
@@ -122,7 +127,8 @@ int covof_poisson_assemble_d(covof_ctx *ctx, const double *u, const double *v,
 /* y = A x, the reference's poisson_matvec (:2668-2707); (nx,ny) arrays. */
 int covof_poisson_matvec_d(covof_ctx *ctx, const double *x, double *y);
 /* z = M^-1 r: NONE, JACOBI (src/module_linear_solvers.f90:88-135) or SSOR =
- * the symmetric Gauss-Seidel poisson_ssor (src/module_simulation.f90:2709-2758). */
+ * the symmetric Gauss-Seidel poisson_ssor (src/module_simulation.f90:2709-2758);
+ * SGS_RB and MG as above (FP64; they need even sizes in periodic directions). */
 int covof_poisson_precond_d(covof_ctx *ctx, int precond, const double *r, double *z);
 /* Krylov solve of A x = b on the assembled operator, x (in: initial guess,
  * out: solution), (nx,ny) device arrays.
@@ -132,7 +138,7 @@ int covof_poisson_precond_d(covof_ctx *ctx, int precond, const double *r, double
  *     preconditioned, stop on the preconditioned residual ratio or 1e-12).
  *   precision MIXED: the Krylov loop runs in FP32 inside an FP64
  *     iterative refinement that recomputes r = b - A x in FP64 and stops on
- *     the FP64 ||r||/||b|| <= tol (CG + JACOBI/NONE only).
+ *     the FP64 ||r||/||b|| <= tol (CG with NONE, JACOBI, SGS_RB or MG).
  * *iters: Krylov iterations (inner iterations summed for MIXED); *relres:
  * final relative residual as defined by the stop rule. */
 int covof_poisson_solve_d(covof_ctx *ctx, const double *b, double *x, int lin_solver, int precond,

@@ -1,0 +1,105 @@
+"""The parallel preconditioners of the Poisson CG (no reference counterpart):
+red-black symmetric Gauss-Seidel and the multigrid V-cycle.  CG needs them to
+be fixed symmetric positive definite operators, and a solve with them must
+land on the same solution as the reference's Jacobi-PCG."""
+import numpy as np
+import pytest
+
+from conftest import to_gpu
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((96, 80), (4, 4, 4, 4)), ((128, 64), (3, 3, 3, 3)), ((100, 60), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4))]
+IDS = ["walls", "periodic", "wall-neumann-periodic", "odd-coarsening"]
+
+
+def _ctx(cv, shape, bc, seed=0):
+    from covof_amd import workloads as wl
+    nx, ny = shape
+    n = max(nx, ny)
+    c = wl.poisson_case(n, seed=seed, n_circles=6, rmin=n / 24, rmax=n / 6)
+    C = c["C"][:ny + 2, :nx + 2].copy()
+    rho = 1000.0 * C + (1.0 - C)
+    rho_u = np.ones_like(rho)
+    rho_v = np.ones_like(rho)
+    rho_u[:, 1:] = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    rho_v[1:, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
+    if bc[0] == 3:
+        rho_u[:, nx + 1] = rho_u[:, 1]
+    if bc[2] == 3:
+        rho_v[ny + 1, :] = rho_v[1, :]
+    ctx = cv.Context(nx, ny, bc=bc)
+    z = ctx.new_field()
+    ctx.poisson_assemble(z, z, to_gpu(rho_u), to_gpu(rho_v), 1.0)
+    return ctx
+
+
+@pytest.mark.parametrize("shape,bc", CASES, ids=IDS)
+@pytest.mark.parametrize("pc", ["SGS_RB", "MG"])
+def test_preconditioner_symmetric_positive(cv, shape, bc, pc):
+    precond = getattr(cv, "PRECOND_" + pc)
+    ctx = _ctx(cv, shape, bc)
+    nx, ny = shape
+    rng = np.random.default_rng(1)
+    r1, r2 = (rng.normal(size=(ny, nx)) for _ in range(2))
+    z1 = ctx.poisson_precond(precond, to_gpu(r1)).cpu().numpy()
+    z2 = ctx.poisson_precond(precond, to_gpu(r2)).cpu().numpy()
+    a, b = (z1 * r2).sum(), (r1 * z2).sum()
+    assert abs(a - b) <= 1e-11 * np.sqrt((z1 * r1).sum() * (z2 * r2).sum()), (a, b)
+    assert (z1 * r1).sum() > 0 and (z2 * r2).sum() > 0
+    # deterministic: the same input gives the same bits
+    assert np.array_equal(ctx.poisson_precond(precond, to_gpu(r1)).cpu().numpy(), z1)
+
+
+@pytest.mark.parametrize("shape,bc", CASES, ids=IDS)
+def test_mg_pcg_matches_jacobi_pcg(cv, orc, shape, bc):
+    ctx = _ctx(cv, shape, bc)
+    nx, ny = shape
+    rng = np.random.default_rng(2)
+    b = rng.normal(size=(ny, nx))
+    if bc[0] != 2 and bc[1] != 2:  # singular operator: a consistent right-hand side
+        b -= b.mean()
+    bg = to_gpu(b)
+    res = {}
+    for name in ("JACOBI", "SGS_RB", "MG"):
+        for prec in (cv.PRECISION_FP64, cv.PRECISION_MIXED):
+            x = ctx.new_work()
+            r = ctx.poisson_solve(bg, x, precond=getattr(cv, "PRECOND_" + name), tol=1e-11, max_iter=100000,
+                                  precision=prec)
+            assert r.converged, (name, prec, r)
+            res[(name, prec)] = (x.cpu().numpy(), r.iters)
+    print({k: v[1] for k, v in res.items()})
+    xj = res[("JACOBI", cv.PRECISION_FP64)][0]
+    xj = xj - xj.mean()
+    for k, (x, _) in res.items():
+        assert np.abs((x - x.mean()) - xj).max() <= 1e-8 * np.abs(xj).max(), k
+    assert res[("MG", cv.PRECISION_FP64)][1] * 5 < res[("JACOBI", cv.PRECISION_FP64)][1]
+    # BiCGSTAB(2) takes the same preconditioners
+    x = ctx.new_work()
+    r = ctx.poisson_solve(bg, x, linear_solver=cv.LINEAR_BICGSTAB2, precond=cv.PRECOND_MG, tol=1e-11,
+                          max_iter=10000)
+    assert r.converged, r
+    xn = x.cpu().numpy()
+    assert np.abs((xn - xn.mean()) - xj).max() <= 1e-7 * np.abs(xj).max()
+
+
+def test_mg_mixed_2048_reaches_1e10(cv, orc):
+    """The 2048^2 Poisson workload: mixed-precision MG-PCG to 1e-10 (FP64 true
+    residual checked on the host with the oracle's matvec)."""
+    from covof_amd import workloads as wl
+    n = 2048
+    case = wl.poisson_case(n, seed=0)
+    ctx = cv.Context(n, n, bc=case["bc"])
+    b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
+    out = {}
+    for prec in (cv.PRECISION_FP64, cv.PRECISION_MIXED):
+        x = ctx.new_work()
+        r = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=1e-10, max_iter=10000, precision=prec)
+        assert r.converged, (prec, r)
+        out[prec] = (x.cpu().numpy(), r.iters)
+    m = orc.mesh(n, n, bc=case["bc"])
+    co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
+    for prec, (x, it) in out.items():
+        rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x)) / np.linalg.norm(co["b"])
+        print(f"precision {prec}: {it} MG-PCG iterations, true relres {rr:.3e}")
+        assert rr <= 1.01e-10
