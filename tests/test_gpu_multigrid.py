@@ -9,8 +9,9 @@ from conftest import to_gpu
 
 pytestmark = pytest.mark.gpu
 
-CASES = [((96, 80), (4, 4, 4, 4)), ((128, 64), (3, 3, 3, 3)), ((100, 60), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4))]
-IDS = ["walls", "periodic", "wall-neumann-periodic", "odd-coarsening"]
+CASES = [((96, 80), (4, 4, 4, 4)), ((128, 64), (3, 3, 3, 3)), ((100, 60), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4)),
+         ((40, 30), (4, 4, 3, 3))]
+IDS = ["walls", "periodic", "wall-neumann-periodic", "odd-coarsening", "one-workgroup"]
 
 
 def _ctx(cv, shape, bc, seed=0):
