@@ -27,6 +27,7 @@ struct Vof {
   uint8_t *cd = nullptr;                // dilation indicator [C^n >= 0.5], interior, halo layout
   int *list[1] = {nullptr};             // per-tile lists of mixed cells (linear halo index)
   int *count = nullptr;                 // per-tile list sizes
+  int *offs = nullptr;                  // exclusive prefix of count (ntiles + 1 entries)
 };
 
 // One instantiated capture of a solver cycle (see poisson.hip).

@@ -206,30 +206,50 @@ __device__ void ghost_segment(const DevMesh &m, int i, int j, double c, double g
 
 // --------------------------------------------------------- reconstruction
 
-// METHOD 0 = PLIC, 1 = ELVIRA.  Blocks [0, ntiles) reconstruct their tile's
-// mixed cells from the tile list; the blocks after them handle the halo ring:
-// for every mixed ring cell the normal of its source cell is reconstructed
-// again (the same arithmetic), renormalised and alpha re-inverted in the
-// ghost box.  ELVIRA spreads the 7 candidates of a cell over 8 lanes; the
-// winner is the first candidate reaching the smallest error below 1e30 (the
-// reference's strict '<' scan from best_err = 1e30).
+// Exclusive prefix sum of the per-tile counts (one workgroup of 1024
+// threads, each summing a contiguous run of tiles): offs[t] = first global
+// index of tile t's mixed cells, offs[ntiles] = their total.
+__global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ count, int ntiles, int *offs) {
+  __shared__ int part[1024];
+  const int per = (ntiles + 1023) / 1024;
+  const int t0 = threadIdx.x * per, t1 = min(t0 + per, ntiles);
+  int s = 0;
+  for (int t = t0; t < t1; ++t) s += count[t];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;  // exclusive
+  for (int t = t0; t < t1; ++t) {
+    offs[t] = run;
+    run += count[t];
+  }
+  if (threadIdx.x == 1023) offs[ntiles] = part[1023];
+}
+
+// METHOD 0 = PLIC, 1 = ELVIRA.  The mixed interior cells of all tile lists
+// followed by the halo ring form one index space, dealt in equal slices to a
+// fixed grid, so every wave is full whatever the interface's layout: an
+// interior index maps to its tile by binary search in offs.  For every mixed
+// ring cell the normal of its source cell is reconstructed again (the same
+// arithmetic), renormalised and alpha re-inverted in the ghost box.  ELVIRA
+// spreads the 7 candidates of a cell over 8 lanes; the winner is the first
+// candidate reaching the smallest error below 1e30 (the reference's strict '<'
+// scan from best_err = 1e30).
 template <int METHOD>
 __global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
-                                                const int *__restrict__ count, double *snx, double *sny,
+                                                const int *__restrict__ offs, double *snx, double *sny,
                                                 double *sal) {
   const int ntiles = tiles_x(m) * tiles_y(m);
-  const int b = blockIdx.x;
-  const bool ring = b >= ntiles;
-  int n;
-  if (!ring) {
-    n = count[b];
-    if (n == 0) return;
-  } else {
-    n = ring_len(m);
-  }
+  const int nmix = offs[ntiles];
+  const int n = nmix + ring_len(m);
   const int per_pass = (METHOD == 1) ? TBLK / 8 : TBLK;
-  const int stride = ring ? per_pass * (gridDim.x - ntiles) : per_pass;
-  const int first = ring ? per_pass * (b - ntiles) : 0;
+  const int stride = per_pass * gridDim.x;
+  const int first = per_pass * blockIdx.x;
   const int lane8 = threadIdx.x & 7;
   const int slot0 = (METHOD == 1) ? (threadIdx.x >> 3) : threadIdx.x;
   // wave-uniform trip count: every lane of a wave stays while any has work
@@ -239,9 +259,16 @@ __global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, De
     int i = 0, j = 0, si = 0, sj = 0;
     bool work = false, zero = false;
     double c = 0.0;
+    const bool ring = e >= nmix;
     if (active) {
       if (!ring) {
-        const int k = list[(long)b * TCELLS + e];
+        int lo = 0, hi = ntiles;  // offs[lo] <= e < offs[hi]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (offs[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        const int k = list[(long)lo * TCELLS + (e - offs[lo])];
         j = k / (m.nx + 2);
         i = k - j * (m.nx + 2);
         si = i;
@@ -249,7 +276,7 @@ __global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, De
         c = C[k];
         work = true;
       } else {
-        ring_cell(m, e, i, j);
+        ring_cell(m, e - nmix, i, j);
         c = C[H(m, i, j)];
         if (is_mixed(c)) {
           work = true;
@@ -599,6 +626,7 @@ int vof_alloc(Ctx &c) {
   COVOF_CHECK(hipMalloc(&c.vof.cd, nh));
   COVOF_CHECK(hipMalloc(&c.vof.list[0], nt * TCELLS * sizeof(int)));
   COVOF_CHECK(hipMalloc(&c.vof.count, nt * sizeof(int)));
+  COVOF_CHECK(hipMalloc(&c.vof.offs, (nt + 1) * sizeof(int)));
   COVOF_CHECK(hipMemset(c.vof.count, 0, nt * sizeof(int)));
   COVOF_CHECK(hipMemset(c.vof.cd, 0, nh));
   return COVOF_OK;
@@ -612,6 +640,7 @@ void vof_free(Ctx &c) {
   hipFree(c.vof.cd);
   hipFree(c.vof.list[0]);
   hipFree(c.vof.count);
+  hipFree(c.vof.offs);
 }
 
 // Reconstruction of src: tile lists (left by the previous update or by
@@ -619,12 +648,14 @@ void vof_free(Ctx &c) {
 static int launch_recon(Ctx &c, int method, const double *src) {
   const DevMesh m = dev_mesh(c.m);
   const int nt = n_tiles(c.m);
-  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, c.stream, (const int *)c.vof.count, nt, c.vof.offs);
+  // one workgroup per tile plus the ring's share: a typical interface fills
+  // the grid's first slice; the blocks past the index space return at once
   const int per_block = method == 1 ? TBLK / 8 : TBLK;
-  const int ring_blocks = std::min(256, (L + per_block - 1) / per_block);
-  const dim3 grid(nt + ring_blocks);
+  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
+  const dim3 grid(nt + (L + per_block - 1) / per_block);
   launch_tagged(c, COVOF_KERNEL_VOF_RECON, method == 1 ? k_recon<1> : k_recon<0>, grid, dim3(TBLK), src, m,
-                (const int *)c.vof.list[0], (const int *)c.vof.count, c.vof.snx, c.vof.sny, c.vof.sal);
+                (const int *)c.vof.list[0], (const int *)c.vof.offs, c.vof.snx, c.vof.sny, c.vof.sal);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
