@@ -137,12 +137,16 @@ def main():
     ms_a, n_a = ka
     ms_b, n_b = kb
     avg_a, avg_b = ms_a / max(n_a, 1), ms_b / max(n_b, 1)
-    if ms_b >= ms_a:
+    if n_b == 0:  # the single-reduction loop: one kernel per iteration, 12 words per cell
+        bytes_a = 12 * 4 * cells
+        dom, dom_bytes, dom_ms, dom_n = "k_cg2", bytes_a, avg_a, n_a
+    elif ms_b >= ms_a:
         dom, dom_bytes, dom_ms, dom_n = "k_pcg_b<float,1>", bytes_b, avg_b, n_b
     else:
         dom, dom_bytes, dom_ms, dom_n = "k_pcg_a<float>", bytes_a, avg_a, n_a
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    iter_gbs = (bytes_a + bytes_b) / ((avg_a + avg_b) * 1e-3) / 1e9
+    iter_gbs = ((bytes_a + bytes_b) / ((avg_a + avg_b) * 1e-3) / 1e9 if n_b else
+                bytes_a / (avg_a * 1e-3) / 1e9)
     log(f"dominant {dom}: {dom_ms * 1e3:.1f} us avg over {dom_n} launches, {achieved:.0f} GB/s")
 
     # ------------------------------------------- multigrid-preconditioned CG

@@ -59,6 +59,7 @@ struct Poisson {
   double *b = nullptr, *x = nullptr, *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr;
   double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *t4 = nullptr, *t5 = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *z32 = nullptr;
+  float *cg2 = nullptr;     // single-reduction loop: r, w, p, s, two buffers each (8 n)
   double *fx = nullptr, *fy = nullptr;  // per-face adjoint sensitivities (kx / ky layout)
   double *part = nullptr;   // per-block partial sums (3 buffers of nblocks x 4)
   double *scal = nullptr;   // device scalars (see poisson.hip)

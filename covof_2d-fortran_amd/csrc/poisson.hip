@@ -34,9 +34,10 @@ enum Slot {
   S_BNORM = 0, S_RZ = 1, /* and 2 */ S_RR = 4, S_RELRES = 6,
   S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_FIRST = 10, S_MAXIT = 11, S_DOT0 = 12,
   S_RMEAN = 13,  // constant-mode handling of the mixed mode
+  S_BEST = 14, S_STALL = 15,  // stall guard of the mixed mode
   // inner (FP32) solve of the mixed mode
   I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
-  I_RR = 23, I_FIRST = 26, I_MAXIT = 27,
+  I_RR = 23, I_ALPHA = 24, /* and 25 */ I_FIRST = 26, I_MAXIT = 27,
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
@@ -388,16 +389,19 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
   const bool first = scal[sl.first] != 0.0;
   // slots: 0 r.r, 1 r.z, 2 b.b (first outer iteration only); the inner
-  // recurrence of the mixed mode needs r.z only
-  double tot[3];
-  grid_totals<3>(pin, gridDim.x, tot, sl.inner ? 1 : 0, (first && sl.is_b_norm) ? 3 : 2);
+  // recurrence of the mixed mode needs r.z only -- and, at the first
+  // iteration after a reliable update, slots 2 and 3: z.(A d_prev) and
+  // d_prev.(A d_prev) (see k_refresh_conj)
+  const bool post_refresh = sl.inner && k == 1 && !first;
+  double tot[4];
+  grid_totals<4>(pin, gridDim.x, tot, sl.inner ? 1 : 0, post_refresh ? 4 : ((first && sl.is_b_norm) ? 3 : 2));
   if (flag_own != 0.0 || flag_outer != 0.0) return;
   const double rz_prev = scal[sl.rz + ((k - 1) & 1)];
   int stop = F_RUN;
   double ref = 1.0, rn = 0.0;
   if (sl.inner) {
     if (!(tot[1] == tot[1]) || fabs(tot[1]) > 1.0e300) stop = F_NONFINITE;
-    else if (!first && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
+    else if (!first && !post_refresh && fabs(rz_prev) < 1.0e-35) stop = F_BREAK;
     else if (scal[sl.iters] >= scal[sl.maxit]) stop = F_MAXIT;
   } else {
     if (first && sl.is_b_norm) {
@@ -426,7 +430,15 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
     }
   }
   if (stop != F_RUN) return;
-  const T beta = first ? T(0) : (T)(tot[1] / rz_prev);
+  // After a reliable update the residual was replaced, and beta = rz / rz_prev
+  // no longer makes d conjugate to d_prev (once the FP32 recurrence's
+  // residual has fallen below the true one, the ratio is large and the old
+  // direction swamps the new one); the conjugate choice
+  // beta = -z.(A d_prev) / d_prev.(A d_prev) keeps the step sound.
+  double bd = 0.0;
+  if (post_refresh) bd = (tot[3] > 0.0) ? -tot[2] / tot[3] : 0.0;
+  else if (!first) bd = tot[1] / rz_prev;
+  const T beta = (T)bd;
   T acc[1] = {T(0)};
   T cur = z0 + beta * p0, down = zd + beta * pd;
   if (act) {
@@ -534,6 +546,250 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
   block_partials<3, T>(acc, pout);
 }
 
+// ---------------------------------------------- single-reduction CG (FP32)
+//
+// The FP32 recurrence of the mixed mode with Jacobi as one kernel and one
+// grid reduction per iteration: the Chronopoulos-Gear form of PCG
+// (A. T. Chronopoulos, C. W. Gear, J. Comput. Appl. Math. 25 (1989) 153-168),
+// mathematically the same iterates as the two-kernel loop:
+//   u = M r, w = A u, gamma = (r, u), delta = (w, u)
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev)
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, then the next
+//   u = M r, w = A u and the partials of (r, u), (w, u).
+// A reads r, w, s, p, x, kx, ky and writes p, s, x, r, w: 12 words per cell
+// against the 14 of the two passes.  The next w needs u = M r at the four
+// neighbours, formed on the fly from their r, w, s (the same register window
+// over a 64-column strip as k_pcg_a).  After a reliable update the residual
+// is replaced and the recurrences' orthogonality assumptions no longer hold,
+// so the first iteration of a cycle makes the new direction A-conjugate to
+// the kept one explicitly: beta = -(u, A p_prev) / (p_prev, A p_prev) and
+// (p, A p) = delta + 2 beta (u, A p_prev) + beta^2 (p_prev, A p_prev), from
+// the partials of k_cg2_init (which recomputes s = A p_prev); with
+// beta = gamma / gamma_prev there instead, the FP32 loop stalls near 1e-6 on
+// pressure-Dirichlet (NEUMANN-velocity) boundaries.  r, w, p, s ping-pong
+// between two buffers with the parity of k.
+struct Cg2Row {
+  float s, rn, uo, un, ue;  // s, new r, old / new M r, new M r of the strip-edge neighbour
+  float kW, kS, kN, kE, d;
+};
+
+__global__ void __launch_bounds__(BLK) k_cg2(DevOp o, OpT<float> A, const float *__restrict__ r,
+                                             const float *__restrict__ w, const float *__restrict__ so,
+                                             const float *__restrict__ po, float *__restrict__ rn,
+                                             float *__restrict__ wn, float *__restrict__ pn, float *__restrict__ sn,
+                                             float *__restrict__ x, double *scal, const double *pin, double *pout,
+                                             int k) {
+  constexpr int G = 4;
+  const int nx = o.nx, ny = o.ny;
+  const int gx_ = (nx + 63) / 64;
+  const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;
+  const int chunks_ = (gridDim.x >> 3) / gx_;
+  const int band_ = (((ny + 7) / 8) + 3) & ~3;
+  const int rows_ = (((band_ + chunks_ - 1) / chunks_) + 3) & ~3;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = (local_ % gx_) * 64 + lane + 1;
+  const int jb = xcd_ * band_ + (local_ / gx_) * rows_;
+  const int je = min(min(jb + rows_, xcd_ * band_ + band_), ny);
+  const int rw = (je - jb + 3) / 4;
+  const int jw0 = jb + 1 + wv * rw, jw1 = min(jw0 + rw - 1, je);
+  const bool vi = i <= nx;
+  const bool act = local_ < gx_ * chunks_ && jw0 <= jw1;  // uniform over the wave
+  const double flag_own = scal[I_FLAG], flag_outer = scal[S_FLAG];
+  const bool first = scal[I_FIRST] != 0.0;
+  double tot[4];
+  grid_totals<4>(pin, gridDim.x, tot, 0, k == 1 ? 4 : 2);
+  if (flag_own != 0.0 || flag_outer != 0.0) return;
+  const double gam = tot[0], del = tot[1];
+  const double gam_prev = scal[I_RZ + ((k - 1) & 1)], alpha_prev = scal[I_ALPHA + ((k - 1) & 1)];
+  double beta = 0.0, den = del;
+  int stop = F_RUN;
+  if (!(gam == gam) || fabs(gam) > 1.0e300) stop = F_NONFINITE;
+  else if (!first && fabs(gam_prev) < 1.0e-35) stop = F_BREAK;
+  else if (scal[I_ITERS] >= scal[I_MAXIT]) stop = F_MAXIT;
+  if (stop == F_RUN && !first) {
+    if (k == 1) {
+      beta = (tot[3] > 0.0) ? -tot[2] / tot[3] : 0.0;
+      den = del + 2.0 * beta * tot[2] + beta * beta * tot[3];
+    } else {
+      beta = gam / gam_prev;
+      den = del - beta * gam / alpha_prev;
+    }
+  }
+  if (stop == F_RUN && !(fabs(den) >= 1.0e-35)) stop = (den == den) ? F_BREAK : F_NONFINITE;
+  const double alpha = (stop == F_RUN) ? gam / den : 0.0;
+  if (lead()) {
+    if (stop != F_RUN) {
+      scal[I_FLAG] = (double)stop;
+    } else {
+      scal[I_ITERS] += 1.0;
+      scal[I_RZ + (k & 1)] = gam;
+      scal[I_ALPHA + (k & 1)] = alpha;
+      if (k == 2) scal[I_FIRST] = 0.0;
+    }
+  }
+  if (stop != F_RUN) return;
+  const float fb = (float)beta, fa = (float)alpha, eps = (float)o.eps;
+  auto at = [&](int a, int b) { return (long)(b - 1) * nx + (a - 1); };
+  // M r_new of cell (a, b) off the strip: its own faces and r, w, s
+  auto u_off = [&](int a, int b) {
+    const long q = at(a, b);
+    const float d = A.kx[kxi(nx, a, b)] + A.kx[kxi(nx, a + 1, b)] + A.ky[kyi(nx, a, b)] + A.ky[kyi(nx, a, b + 1)] + eps;
+    const float sv = w[q] + fb * so[q];
+    return jacobi<float>(r[q] - fa * sv, d);
+  };
+  // row R entering the window: everything but the strip-edge neighbour
+  auto enter = [&](int R, Cg2Row &E) {
+    E.s = E.rn = E.uo = E.un = E.ue = 0.f;
+    E.kW = E.kS = E.kN = E.kE = 0.f;
+    E.d = 1.f;
+    if (!vi) return;
+    const long q = at(i, R);
+    E.kW = A.kx[kxi(nx, i, R)];
+    E.kS = A.ky[kyi(nx, i, R)];
+    E.kN = A.ky[kyi(nx, i, R + 1)];
+    const float rv = r[q], wv2 = w[q], sv = so[q];
+    E.s = wv2 + fb * sv;
+    E.rn = rv - fa * E.s;
+    if (i == nx || lane == 63) E.kE = A.kx[kxi(nx, i + 1, R)];
+    E.uo = rv;  // M applied once d is known (needs kE from the neighbour lane)
+    E.un = E.rn;
+    if (lane == 0) {
+      const int il = i > 1 ? i - 1 : (o.perW ? nx : 0);
+      if (il > 0) E.ue = u_off(il, R);
+    }
+    if (i == nx) {
+      if (o.perE) E.ue = u_off(1, R);
+    } else if (lane == 63) {
+      E.ue = u_off(i + 1, R);
+    }
+  };
+  // finish a row: kE from the right lane, d, M r (all lanes take part in the shuffle)
+  auto finish = [&](Cg2Row &E) {
+    const float kEs = __shfl_down(E.kW, 1, 64);
+    if (!(i == nx || lane == 63)) E.kE = kEs;
+    E.d = E.kW + E.kE + E.kS + E.kN + eps;
+    E.uo = jacobi<float>(E.uo, E.d);
+    E.un = jacobi<float>(E.un, E.d);
+  };
+  double acc[2] = {0.0, 0.0};
+  float accf0 = 0.f, accf1 = 0.f;
+  if (act) {
+    Cg2Row dn, cu;
+    const int jd = jw0 > 1 ? jw0 - 1 : (o.perS ? ny : 0);
+    if (jd > 0) enter(jd, dn);
+    else {
+      dn.s = dn.rn = dn.uo = dn.un = dn.ue = 0.f;
+      dn.kW = dn.kS = dn.kN = dn.kE = 0.f;
+      dn.d = 1.f;
+    }
+    enter(jw0, cu);
+    finish(dn);
+    finish(cu);
+    for (int j0 = jw0; j0 <= jw1; j0 += G) {
+      Cg2Row up[G];
+      float pov[G], xv[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = j0 + g;
+        pov[g] = xv[g] = 0.f;
+        up[g].s = up[g].rn = up[g].uo = up[g].un = up[g].ue = 0.f;
+        up[g].kW = up[g].kS = up[g].kN = up[g].kE = 0.f;
+        up[g].d = 1.f;
+        if (j <= jw1) {
+          const int ju = j < ny ? j + 1 : (o.perN ? 1 : 0);
+          if (ju > 0) enter(ju, up[g]);
+          if (vi) {
+            pov[g] = po[at(i, j)];
+            xv[g] = x[at(i, j)];
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = j0 + g;
+        if (j > jw1) break;  // uniform over the wave
+        finish(up[g]);
+        float left = __shfl_up(cu.un, 1, 64);
+        float right = __shfl_down(cu.un, 1, 64);
+        if (lane == 0) left = cu.ue;
+        if (i == nx || lane == 63) right = cu.ue;
+        if (vi) {
+          const long q = at(i, j);
+          const float pv = cu.uo + fb * pov[g];
+          const float xnew = xv[g] + fa * pv;
+          float val = cu.d * cu.un;
+          if (i > 1 || o.perW) val = val - cu.kW * left;
+          if (i < nx || o.perE) val = val - cu.kE * right;
+          if (j > 1 || o.perS) val = val - cu.kS * dn.un;
+          if (j < ny || o.perN) val = val - cu.kN * up[g].un;
+          pn[q] = pv;
+          sn[q] = cu.s;
+          x[q] = xnew;
+          rn[q] = cu.rn;
+          wn[q] = val;
+          accf0 += cu.rn * cu.un;
+          accf1 += val * cu.un;
+        }
+        dn = cu;
+        cu = up[g];
+      }
+    }
+  }
+  acc[0] = accf0;
+  acc[1] = accf1;
+  block_partials<2, double>(acc, pout);
+}
+
+// Start of a cycle: u = M r32 (r32 = the refreshed residual), w = A u, and
+// s = A p recomputed from the kept direction p (the recurrence s = w + beta s
+// drifts from A p in FP32 over many iterations; the refresh resets it with
+// the residual), then the partials (r, u), (w, u), (u, s), (p, s).
+__global__ void __launch_bounds__(BLK) k_cg2_init(DevOp o, OpT<float> A, const float *__restrict__ r,
+                                                  float *__restrict__ w, float *__restrict__ sp,
+                                                  const float *__restrict__ pp, double *pout, const double *scal) {
+  if (scal[S_FLAG] != 0.0) return;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  const float eps = (float)o.eps;
+  auto u_at = [&](long q, int a, int b) {
+    const float d = A.kx[kxi(o.nx, a, b)] + A.kx[kxi(o.nx, a + 1, b)] + A.ky[kyi(o.nx, a, b)] +
+                    A.ky[kyi(o.nx, a, b + 1)] + eps;
+    return jacobi<float>(r[q], d);
+  };
+  CELL_LOOP(o) {
+    const int nx = o.nx, ny = o.ny;
+    const long q = (long)(j - 1) * nx + (i - 1);
+    const float u = u_at(q, i, j);
+    const float aW = A.kx[kxi(nx, i, j)], aE = A.kx[kxi(nx, i + 1, j)];
+    const float aS = A.ky[kyi(nx, i, j)], aN = A.ky[kyi(nx, i, j + 1)];
+    float val = (aW + aE + aS + aN + eps) * u;
+    if (i > 1) val = val - aW * u_at(q - 1, i - 1, j);
+    else if (o.perW) val = val - aW * u_at(q + (nx - 1), nx, j);
+    if (i < nx) val = val - aE * u_at(q + 1, i + 1, j);
+    else if (o.perE) val = val - aE * u_at(q - (nx - 1), 1, j);
+    if (j > 1) val = val - aS * u_at(q - nx, i, j - 1);
+    else if (o.perS) val = val - aS * u_at(q + (long)(ny - 1) * nx, i, ny);
+    if (j < ny) val = val - aN * u_at(q + nx, i, j + 1);
+    else if (o.perN) val = val - aN * u_at(q - (long)(ny - 1) * nx, i, 1);
+    w[q] = val;
+    const float pc = pp[q];
+    float ap = (aW + aE + aS + aN + eps) * pc;
+    if (i > 1) ap = ap - aW * pp[q - 1];
+    else if (o.perW) ap = ap - aW * pp[q + (nx - 1)];
+    if (i < nx) ap = ap - aE * pp[q + 1];
+    else if (o.perE) ap = ap - aE * pp[q - (nx - 1)];
+    if (j > 1) ap = ap - aS * pp[q - nx];
+    else if (o.perS) ap = ap - aS * pp[q + (long)(ny - 1) * nx];
+    if (j < ny) ap = ap - aN * pp[q + nx];
+    else if (o.perN) ap = ap - aN * pp[q - (long)(ny - 1) * nx];
+    sp[q] = ap;
+    acc[0] += r[q] * u;
+    acc[1] += val * u;
+    acc[2] += u * ap;
+    acc[3] += pc * ap;
+  }
+  block_partials<4, float>(acc, pout);
+}
+
 // r.z with an explicit z (SGS, RB-SGS, multigrid) into slot 1 of B's partials.
 template <typename T>
 __global__ void __launch_bounds__(BLK) k_pcg_rz(DevOp o, const T *__restrict__ r, const T *__restrict__ z,
@@ -604,10 +860,24 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
     scal[S_BNORM] = bn;
   }
   const double rn = sqrt(tot[0]);
+  const double rel = rn / scal[S_BNORM];
   scal[S_RR] = tot[0];
-  scal[S_RELRES] = rn / scal[S_BNORM];
-  if (!(rn == rn) || rn > 1.0e300) scal[S_FLAG] = (double)F_NONFINITE;
-  else if (rn / scal[S_BNORM] <= scal[S_TOL]) scal[S_FLAG] = (double)F_CONV;
+  scal[S_RELRES] = rel;
+  if (!(rn == rn) || rn > 1.0e300) {
+    scal[S_FLAG] = (double)F_NONFINITE;
+  } else if (rel <= scal[S_TOL]) {
+    scal[S_FLAG] = (double)F_CONV;
+  } else if (first || rel < 0.9 * scal[S_BEST]) {
+    scal[S_BEST] = rel;
+    scal[S_STALL] = 0.0;
+  } else if ((scal[S_STALL] += 1.0) >= 48.0) {
+    // 48 reliable updates without a 10 % gain over the best residual (CG's
+    // residual norm plateaus for about a thousand Jacobi iterations at a time
+    // on the density-jump systems; this is far past that): the FP32
+    // recurrence has hit the attainable accuracy of this system, stop as not
+    // converged
+    scal[S_FLAG] = (double)F_MAXIT;
+  }
 }
 
 // sum(dx32) into slot 0 of this kernel's partials (singular operator only).
@@ -646,7 +916,10 @@ __global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const double *__res
                                                     float *__restrict__ z32, double *pout, double *scal) {
   const double flag = scal[S_FLAG], rmean = scal[S_RMEAN];
   if (flag != 0.0) return;
-  if (lead()) scal[I_FLAG] = 0.0;  // a breakdown of the FP32 recurrence restarts from the true residual
+  if (lead()) {  // a cycle the FP32 recurrence left early (breakdown, non-finite) restarts it
+    if (scal[I_FLAG] != 0.0) scal[I_FIRST] = 1.0;
+    scal[I_FLAG] = 0.0;
+  }
   float acc[3] = {0.f, 0.f, 0.f};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
@@ -662,6 +935,23 @@ __global__ void __launch_bounds__(BLK) k_inner_init(DevOp o, const double *__res
   }
   acc[2] = acc[0];
   block_partials<3, float>(acc, pout);
+}
+
+// Slots 2, 3 of a reliable update's partials: z.(A d_prev), d_prev.(A d_prev)
+// for the conjugate restart of the FP32 recurrence (d_prev, A d_prev: the
+// last iteration's direction and product).
+__global__ void __launch_bounds__(BLK) k_refresh_conj(DevOp o, const float *__restrict__ z, const float *__restrict__ dp,
+                                                      const float *__restrict__ adp, double *pout,
+                                                      const double *scal) {
+  if (scal[S_FLAG] != 0.0) return;
+  float acc[2] = {0.f, 0.f};
+  CELL_LOOP(o) {
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    const float a = adp[q];
+    acc[0] += z[q] * a;
+    acc[1] += dp[q] * a;
+  }
+  block_partials<2, float>(acc, pout, 2);
 }
 
 // p = x - mean(x) (src/module_simulation.f90:2578-2585) into p(1:nx,1:ny).
@@ -807,7 +1097,7 @@ void poisson_free(Ctx &c) {
   Poisson &P = c.ps;
   void *ptrs[] = {P.kx, P.ky, P.diag, P.kx32, P.ky32, P.diag32, P.b, P.x, P.r, P.p, P.q, P.z,
                   P.t0, P.t1, P.t2, P.t3, P.t4, P.t5, P.r32, P.p32, P.q32, P.x32, P.z32, P.part, P.scal, P.ticket,
-                  P.fx, P.fy};
+                  P.fx, P.fy, P.cg2};
   for (void *q : ptrs) hipFree(q);
   if (P.hscal) hipHostFree(P.hscal);
   for (PGraph &G : P.graph)
@@ -963,6 +1253,37 @@ static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *A
   }
 }
 
+// The single-reduction loop's buffers: r, w, p, s at parity 0 and 1.
+struct Cg2Bufs {
+  float *r[2], *w[2], *p[2], *s[2];
+};
+static Cg2Bufs cg2_bufs(Ctx &c) {
+  const long n = (long)c.m.nx * c.m.ny;
+  float *b = c.ps.cg2;
+  return Cg2Bufs{{b, b + n}, {b + 2 * n, b + 3 * n}, {b + 4 * n, b + 5 * n}, {b + 6 * n, b + 7 * n}};
+}
+
+// Iteration k of the single-reduction loop: reads parity (k+1)&1, writes k&1.
+static void cg2_iteration(Ctx &c, int k, bool timed) {
+  Poisson &P = c.ps;
+  const Cg2Bufs B = cg2_bufs(c);
+  const int ri = (k + 1) & 1, wo = k & 1;
+  OpT<float> A32{P.kx32, P.ky32, P.diag32};
+  launch_tagged(c, timed ? COVOF_KERNEL_PCG_A_FP32 : -1, k_cg2, red_grid(c.m), dim3(BLK), dev_op(c.m), A32,
+                (const float *)B.r[ri], (const float *)B.w[ri], (const float *)B.s[ri], (const float *)B.p[ri],
+                B.r[wo], B.w[wo], B.p[wo], B.s[wo], P.x32, P.scal, (const double *)part_buf(c, ri),
+                part_buf(c, wo), k);
+}
+
+// Whether the mixed Jacobi loop runs as the single-reduction kernel
+// (COVOF_MIXED_LOOP=cg2) or as the two-kernel PCG (the default: at 2048^2
+// it is as fast per iteration and needs 4 % fewer iterations).
+static bool mixed_cg2(int pc) {
+  if (pc != 1) return false;
+  const char *e = getenv("COVOF_MIXED_LOOP");
+  return e && e[0] == 'c' && e[1] == 'g' && e[2] == '2' && e[3] == 0;
+}
+
 // The reliable update of the mixed mode: x += x32, x32 = 0, r = b - A x in
 // FP64 (+ its FP32 copy), the convergence test, and the FP32 recurrence's
 // restart values (z32, r.r, r.z) from the true residual.
@@ -987,16 +1308,30 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
                      (const double *)P.scal);
   hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
                      first, singular, inv_n);
-  if (pc == 1)
-    hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, P.r32,
+  const bool cg2 = mixed_cg2(pc);
+  // the single-reduction loop restarts from r32 in its parity-0 r buffer
+  float *r32 = cg2 ? cg2_bufs(c).r[0] : P.r32;
+  if (pc == 1 && !cg2)
+    hipLaunchKernelGGL((k_inner_init<1>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, r32,
                        (const float *)P.diag32, P.z32, pb, P.scal);
   else
-    hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, P.r32,
+    hipLaunchKernelGGL((k_inner_init<0>), g, dim3(BLK), 0, c.stream, o, (const double *)P.r, r32,
                        (const float *)P.diag32, P.z32, pb, P.scal);
   if (pc >= 3) {  // explicit z32 = M^-1 r32, r.z into slot 1
     explicit_precond<float>(c, pc, (const float *)P.r32, P.z32, INNER.flag);
     hipLaunchKernelGGL(k_pcg_rz<float>, g, dim3(BLK), 0, c.stream, o, (const float *)P.r32, (const float *)P.z32,
                        P.scal, INNER, pb);
+  }
+  if (!cg2) {  // conjugate restart of the two-kernel loop (d_prev = d0, A d_prev = q32)
+    const float *z32 = (pc == 0) ? (const float *)P.r32 : (const float *)P.z32;
+    hipLaunchKernelGGL(k_refresh_conj, g, dim3(BLK), 0, c.stream, o, z32, (const float *)P.p32,
+                       (const float *)P.q32, pb, (const double *)P.scal);
+  }
+  if (cg2) {
+    const Cg2Bufs B = cg2_bufs(c);
+    OpT<float> A32{P.kx32, P.ky32, P.diag32};
+    hipLaunchKernelGGL(k_cg2_init, g, dim3(BLK), 0, c.stream, o, A32, (const float *)B.r[0], B.w[0], B.s[0],
+                       (const float *)B.p[0], part_buf(c, 0), (const double *)P.scal);
   }
 }
 
@@ -1013,7 +1348,8 @@ static void enqueue_cycle(Ctx &c, int pc, bool mixed, int iters) {
     const bool timed = prof_sampled(c, k);
     if (mixed) {
       float *d0 = P.p32, *d1 = P.p32 + n;
-      if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
+      if (mixed_cg2(pc)) cg2_iteration(c, k, timed);
+      else if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
       else if (pc == 3) pcg_iteration<float, 3>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
       else if (pc == 4) pcg_iteration<float, 4>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
       else pcg_iteration<float, 0>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
@@ -1179,11 +1515,19 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     return st;
   if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
   if ((st = fill<float>(c, P.p32, n, 0.0f))) return st;
+  if ((st = fill<float>(c, P.q32, n, 0.0f))) return st;
+  if (mixed_cg2(pc)) {
+    if (!P.cg2) COVOF_CHECK(hipMalloc(&P.cg2, 8 * n * sizeof(float)));
+    const Cg2Bufs B = cg2_bufs(c);
+    if ((st = fill<float>(c, B.p[0], n, 0.0f))) return st;
+    if ((st = fill<float>(c, B.s[0], n, 0.0f))) return st;
+  }
   mixed_refresh(c, pc, P.b, P.x, 1);
   COVOF_CHECK_LAUNCH();
   if ((st = read_scalars(c, NREAD))) return st;
   if (P.hscal[S_FLAG] == 0.0) {
-    if ((st = run_cycles(c, P.graph[1], 1000 + cycle * 8 + pc, pc, true, cycle, maxit / cycle + 2))) return st;
+    const int key = 1000 + cycle * 16 + pc * 2 + (mixed_cg2(pc) ? 1 : 0);
+    if ((st = run_cycles(c, P.graph[1], key, pc, true, cycle, maxit / cycle + 2))) return st;
   }
   if ((st = read_scalars(c, NREAD))) return st;
   if (iters) *iters = (int)P.hscal[I_ITERS];
