@@ -57,38 +57,36 @@ CV_HD double hypot_ref(double x, double y) {
 // are emitted (term(k-1,k) as vertex k appears, term(last, first) at the end):
 // the same terms added in the same order as the reference's polygon loop,
 // without a vertex array.
+// emit() is written with selects rather than branches (the term of the first
+// vertex is formed and dropped), so the clip compiles to short straight-line
+// code.
 struct Shoelace {
   double s, fx, fy, px, py;
   int n;
   CV_HD void emit(double x, double y) {
-    if (n == 0) {
-      fx = x;
-      fy = y;
-      s = 0.0;
-    } else {
-      s = s + (px * y - x * py);
-    }
+    const double term = px * y - x * py;
+    const bool first = n == 0;
+    s = first ? 0.0 : s + term;
+    fx = first ? x : fx;
+    fy = first ? y : fy;
     px = x;
     py = y;
     ++n;
   }
 };
 
-// Edge k -> k+1 of the clip with corner distances d1, d2.
+// Edge k -> k+1 of the clip with corner distances d1, d2: the intersection is
+// kept when the edge crosses the line (inside -> outside or outside ->
+// inside), then the end corner when it is inside -- the reference's four
+// cases in their emission order.
 CV_HD void clip_edge(Shoelace &P, double d1, double d2, double x1, double y1, double x2,
                      double y2) {
-  if (d1 <= 0.0) {
-    if (d2 <= 0.0) {
-      P.emit(x2, y2);
-    } else {
-      const double t = -d1 / (d2 - d1);
-      P.emit(x1 + t * (x2 - x1), y1 + t * (y2 - y1));
-    }
-  } else if (d2 <= 0.0) {
+  const bool in1 = d1 <= 0.0, in2 = d2 <= 0.0;
+  if (in1 != in2) {
     const double t = -d1 / (d2 - d1);
     P.emit(x1 + t * (x2 - x1), y1 + t * (y2 - y1));
-    P.emit(x2, y2);
   }
+  if (in2) P.emit(x2, y2);
 }
 
 // Area of the box cut by the line whose corner values without alpha are

@@ -448,11 +448,39 @@ __device__ __forceinline__ void write_halo(double *dst, const double *src, const
   }
 }
 
+// Flux of face (i, j) of the DIR pass as the update uses it: on a periodic
+// axis the faces 1 and n+1 both carry the flux computed with the real donor
+// (face n+1 when the velocity at face 1 is >= 0, face 1 otherwise;
+// :1223-1231 for x, :1268-1275 for y).
+template <int DIR>
+__device__ __forceinline__ double pass_face_flux(const double *__restrict__ C, const double *__restrict__ vel,
+                                              const DevMesh &m, const double *snx, const double *sny,
+                                              const double *sal, int i, int j, double dt) {
+  if (DIR == 0) {
+    if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx + 1)) i = (vel[H(m, 1, j)] >= 0.0) ? m.nx + 1 : 1;
+  } else {
+    if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny + 1)) j = (vel[H(m, i, 1)] >= 0.0) ? m.ny + 1 : 1;
+  }
+  return face_flux<DIR>(C, vel, m, snx, sny, sal, i, j, dt);
+}
+
+// Tile of block b: consecutive workgroups go to the eight XCDs in turn, so
+// XCD x is given the contiguous run of tiles [start_x, start_x + count_x)
+// (row-major over tiles): the tiles above and below a tile, whose edge rows
+// the y-pass reads, mostly sit in the same XCD's L2.
+__device__ __forceinline__ int xcd_tile(int b, int nt) {
+  const int q = nt >> 3, rem = nt & 7, x = b & 7;
+  return x * q + min(x, rem) + (b >> 3);
+}
+
 // One directional pass on a tile.  DIR 0: x-pass (Eq. 15), DIR 1: y-pass
 // (Eq. 16).  FIRST: first pass of the step (dilation indicator from src = C^n,
 // written to cd); the second pass reads cd and clamps to [0,1]
 // (:1234-1239 / :1276-1283).  Writes dst's interior and halo, and the tile's
 // list of mixed cells of dst for the next reconstruction.
+// Phase 1 evaluates every face of the tile once (65 x 16 x-faces or 64 x 17
+// y-faces, one call site) into LDS with the face velocities; phase 2 updates
+// the cells from the two face values each.
 template <int DIR, bool FIRST>
 __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
                                                       const double *__restrict__ vel, DevMesh m, double dt,
@@ -460,48 +488,42 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
                                                       const double *__restrict__ sal, uint8_t *__restrict__ cd,
                                                       int *__restrict__ list, int *__restrict__ count,
                                                       int corner_fix) {
+  constexpr int FW = DIR == 0 ? TX + 1 : TX;  // faces per LDS row
+  constexpr int NF = DIR == 0 ? (TX + 1) * TY : TX * (TY + 1);
+  __shared__ double s_flux[NF];
+  __shared__ double s_vel[NF];
   __shared__ int s_list[TCELLS];
   __shared__ int s_count;
   if (threadIdx.x == 0) s_count = 0;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % tiles_x(m), ty = t / tiles_x(m);
+  const int i0 = tx * TX + 1, j0 = ty * TY + 1;
+  const int imax = DIR == 0 ? m.nx + 1 : m.nx, jmax = DIR == 0 ? m.ny : m.ny + 1;
+  for (int f = threadIdx.x; f < NF; f += TBLK) {
+    const int fr = f / FW;
+    const int fi = i0 + (f - fr * FW), fj = j0 + fr;
+    if (fi <= imax && fj <= jmax) {
+      s_vel[f] = vel[H(m, fi, fj)];
+      s_flux[f] = pass_face_flux<DIR>(src, vel, m, snx, sny, sal, fi, fj, dt);
+    }
+  }
   __syncthreads();
-  const int tx = blockIdx.x % tiles_x(m), ty = blockIdx.x / tiles_x(m);
-  const int i = tx * TX + (threadIdx.x & 63) + 1;
+  const int lane = threadIdx.x & 63;
+  const int i = i0 + lane;
   const double area = m.dx * m.dy;
+  const double h = DIR == 0 ? m.dx : m.dy;
   for (int r = 0; r < TY / 4; ++r) {
-    const int j = ty * TY + (threadIdx.x >> 6) + 4 * r + 1;
+    const int row = (threadIdx.x >> 6) + 4 * r;
+    const int j = j0 + row;
     const bool inside = i <= m.nx && j <= m.ny;
     long k = 0;
     double cn = 0.0;
     if (inside) {
       k = H(m, i, j);
       const double c0 = src[k];
-      double flo, fhi, du, h;
-      if (DIR == 0) {
-        if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx)) {
-          // :1223-1231: the boundary face keeps the flux computed with the real donor
-          const double fb = (vel[H(m, 1, j)] >= 0.0) ? face_flux<0>(src, vel, m, snx, sny, sal, m.nx + 1, j, dt)
-                                                      : face_flux<0>(src, vel, m, snx, sny, sal, 1, j, dt);
-          flo = (i == 1) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
-          fhi = (i == m.nx) ? fb : face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
-        } else {
-          flo = face_flux<0>(src, vel, m, snx, sny, sal, i, j, dt);
-          fhi = face_flux<0>(src, vel, m, snx, sny, sal, i + 1, j, dt);
-        }
-        du = vel[H(m, i + 1, j)] - vel[k];
-        h = m.dx;
-      } else {
-        if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny)) {
-          const double fb = (vel[H(m, i, 1)] >= 0.0) ? face_flux<1>(src, vel, m, snx, sny, sal, i, m.ny + 1, dt)
-                                                      : face_flux<1>(src, vel, m, snx, sny, sal, i, 1, dt);
-          flo = (j == 1) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
-          fhi = (j == m.ny) ? fb : face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
-        } else {
-          flo = face_flux<1>(src, vel, m, snx, sny, sal, i, j, dt);
-          fhi = face_flux<1>(src, vel, m, snx, sny, sal, i, j + 1, dt);
-        }
-        du = vel[H(m, i, j + 1)] - vel[k];
-        h = m.dy;
-      }
+      const int lo = row * FW + lane, hi = DIR == 0 ? lo + 1 : lo + FW;
+      const double flo = s_flux[lo], fhi = s_flux[hi];
+      const double du = s_vel[hi] - s_vel[lo];
       double cdv;
       if (FIRST) {
         const bool bb = c0 >= 0.5;
@@ -512,8 +534,8 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
       }
       cn = c0 - (fhi - flo) / area + cdv * (dt / h) * du;
       if (!FIRST) {
-        const double t = 1.0 < cn ? 1.0 : cn;  // min(1, c)
-        cn = 0.0 > t ? 0.0 : t;                // max(0, .)
+        const double tc = 1.0 < cn ? 1.0 : cn;  // min(1, c)
+        cn = 0.0 > tc ? 0.0 : tc;               // max(0, .)
       }
       dst[k] = cn;
       if (i == 1 || j == 1 || i == m.nx || j == m.ny) write_halo(dst, src, m, i, j, cn, corner_fix != 0);
@@ -522,8 +544,8 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   }
   __syncthreads();
   const int n = s_count;
-  for (int e = threadIdx.x; e < n; e += TBLK) list[(long)blockIdx.x * TCELLS + e] = s_list[e];
-  if (threadIdx.x == 0) count[blockIdx.x] = n;
+  for (int e = threadIdx.x; e < n; e += TBLK) list[(long)t * TCELLS + e] = s_list[e];
+  if (threadIdx.x == 0) count[t] = n;
 }
 
 // ------------------------------------------------------------ properties
