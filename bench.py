@@ -131,9 +131,9 @@ def main():
     err_vs_fp64 = (d.abs().max() / (x64 - x64.mean()).abs().max()).item()
 
     cells = n * n
-    # algorithmic bytes per launch: A reads z, d_prev, kx, ky and writes d, Ad;
-    # B reads x, d, r, Ad, a_P and writes x, r, z (FP32 words)
-    bytes_a, bytes_b = 6 * 4 * cells, 8 * 4 * cells
+    # algorithmic bytes per launch: A reads r, d_prev, x, kx, ky and writes d,
+    # Ad, x; B reads r, Ad, a_P and writes r (FP32 words)
+    bytes_a, bytes_b = 8 * 4 * cells, 4 * 4 * cells
     ms_a, n_a = ka
     ms_b, n_b = kb
     avg_a, avg_b = ms_a / max(n_a, 1), ms_b / max(n_b, 1)
@@ -143,7 +143,7 @@ def main():
     elif ms_b >= ms_a:
         dom, dom_bytes, dom_ms, dom_n = "k_pcg_b<float,1>", bytes_b, avg_b, n_b
     else:
-        dom, dom_bytes, dom_ms, dom_n = "k_pcg_a<float>", bytes_a, avg_a, n_a
+        dom, dom_bytes, dom_ms, dom_n = "k_pcg_a<float,1>", bytes_a, avg_a, n_a
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     iter_gbs = ((bytes_a + bytes_b) / ((avg_a + avg_b) * 1e-3) / 1e9 if n_b else
                 bytes_a / (avg_a * 1e-3) / 1e9)

@@ -31,13 +31,13 @@ namespace covof {
 
 // ---------------------------------------------------------------- scalars
 enum Slot {
-  S_BNORM = 0, S_RZ = 1, /* and 2 */ S_RR = 4, S_RELRES = 6,
+  S_BNORM = 0, S_RZ = 1, /* and 2 */ S_STEP = 3, S_RR = 4, S_PEND = 5, S_RELRES = 6,
   S_FLAG = 7, S_ITERS = 8, S_TOL = 9, S_FIRST = 10, S_MAXIT = 11, S_DOT0 = 12,
   S_RMEAN = 13,  // constant-mode handling of the mixed mode
   S_BEST = 14, S_STALL = 15,  // stall guard of the mixed mode
   // inner (FP32) solve of the mixed mode
-  I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_ETA = 22,
-  I_RR = 23, I_ALPHA = 24, /* and 25 */ I_FIRST = 26, I_MAXIT = 27,
+  I_RZ = 16, /* and 17 */ I_RELRES = 18, I_RR0 = 19, I_FLAG = 20, I_ITERS = 21, I_STEP = 22,
+  I_RR = 23, I_ALPHA = 24, /* and 25 */ I_FIRST = 26, I_MAXIT = 27, I_PEND = 28,
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
@@ -245,7 +245,9 @@ __global__ void k_assemble(const double *__restrict__ u, const double *__restric
   }
   const double aP = aW + aE + aS + aN + 1.0e-12 * (dx * dy);
   diag[k] = aP;
-  diag32[k] = (float)aP;
+  // the FP32 operator's diagonal is summed from its own FP32 faces (what the
+  // FP32 PCG kernels form on the fly), not rounded from the FP64 one
+  diag32[k] = (float)aW + (float)aE + (float)aS + (float)aN + (float)(1.0e-12 * (dx * dy));
   const double div = (u[H(i + 1, j)] - u[H(i, j)]) * dy + (v[H(i, j + 1)] - v[H(i, j)]) * dx;
   if (b) b[k] = -div;
 }
@@ -327,13 +329,22 @@ __global__ void __launch_bounds__(1024) k_sgs_wavefront(DevOp o, OpT<double> A, 
 //   A_k: totals of B_{k-1} (or of the initial residual when k = 1): r.r, r.z
 //        (and b.b at k = 1).  ||r||/||b|| <= tol -> converged (the exit of
 //        iteration k-1); |rz_old| < 1e-35 -> breakdown; beta = rz / rz_old.
-//        Body: d = z + beta d_prev (d = z at k = 1, d_prev = 0), Ad = A d,
-//        partial d.Ad.  The matvec's neighbour values of d are formed on the
-//        fly from z and d_prev, and its diagonal from the face coefficients
-//        (bit-equal to a_P), so A_k reads z, d_prev, kx, ky only.
+//        Body: x += alpha_{k-1} d_prev (the solution update of iteration
+//        k-1, one kernel late), d = z + beta d_prev (d = z at k = 1,
+//        d_prev = 0), Ad = A d, partial d.Ad.  z is formed on the fly from r
+//        (Jacobi: z = r / a_P with a_P summed from the four face
+//        coefficients, bit-equal to the assembled diagonal; none: z = r) or
+//        read (explicit preconditioners), so A_k reads r or z, d_prev, x, kx,
+//        ky and writes d, Ad, x.
 //   B_k: totals of A_k: d.Ad; |dAd| < 1e-35 -> breakdown; alpha = rz / dAd.
-//        Body: x += alpha d, r -= alpha Ad, z = r / a_P (Jacobi), partials
-//        r.r and r.z.
+//        Body: r -= alpha Ad, partials r.r and r.z (z = r / a_P from the
+//        stored diagonal).  B_k reads r, Ad, diag and writes r.
+// Each x / r value is formed by the reference's expression on the same
+// operands; only the x update moves into the next A.  The update of the last
+// iteration of a solve (A stopped, or the FP32 recurrence reached a reliable
+// update) is left pending: B records alpha and which d buffer it used, A
+// clears the record when it applies the update, and k_flush_x (or the
+// reliable update) applies a record left standing.
 // d is ping-ponged (A_k reads neighbours' d_prev while writing d); rz is kept
 // in two parity slots (A_k reads rz_{k-1} and stores rz_k).
 struct PcgSlots {
@@ -343,21 +354,37 @@ struct PcgSlots {
                   // 0: the norm already in slot ref (the outer ||b||)
   int inner;      // FP32 recurrence of the mixed mode: no convergence test of
                   // its own (the FP64 residual of the reliable updates decides)
+  int step;       // alpha of the last B (as the value of type T)
+  int pend;       // 0: no pending x update; 1 / 2: alpha * d0 / d1 is pending
 };
 
-template <typename T>
-__global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__restrict__ zs,
+// Rows per load batch of k_pcg_a (FP64 / FP32) and the target block count
+// of the reduction grid (build-time tunables).
+#ifndef PCG_ROWS_F64
+#define PCG_ROWS_F64 2
+#endif
+#ifndef PCG_ROWS_F32
+#define PCG_ROWS_F32 4
+#endif
+#ifndef RED_BLOCKS
+#define RED_BLOCKS 1024
+#endif
+
+template <typename T, int PC>
+__global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__restrict__ zr,
                                                const T *__restrict__ dprev, T *__restrict__ d,
-                                               T *__restrict__ Ad, double *scal, PcgSlots sl,
-                                               const double *pin, double *pout, int k) {
+                                               T *__restrict__ Ad, T *__restrict__ x, double *scal,
+                                               PcgSlots sl, const double *pin, double *pout, int k) {
   // Each wave walks a contiguous run of rows of its 64-column strip keeping
-  // d of the rows below / at / above in registers: d and the coefficients of a
-  // cell are loaded once, x-neighbours come from the adjacent lanes, and only
-  // the strip edges and the periodic wraps load extra values.  Rows go in
-  // batches of G whose loads are all issued before the batch is evaluated, so
-  // each wave keeps several rows of loads in flight.  The matvec evaluates
-  // poisson_matvec's expression in its order on the same operands.
-  constexpr int G = 4;
+  // d, a_P and the face coefficients of the rows below / at / above in
+  // registers: every value is loaded once, x-neighbours come from the
+  // adjacent lanes, and only the strip edges and the periodic wraps load
+  // extra values.  The row above is loaded one step ahead (its z needs its
+  // own a_P).  Rows go in batches of G whose loads are all issued before the
+  // batch is evaluated, so each wave keeps several rows of loads in flight.
+  // The matvec evaluates poisson_matvec's expression in its order on the
+  // same operands.
+  constexpr int G = sizeof(T) == 8 ? PCG_ROWS_F64 : PCG_ROWS_F32;
   const int nx = o.nx, ny = o.ny;
   const int gx_ = (nx + 63) / 64;
   const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;
@@ -372,22 +399,31 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   const int jw0 = jb + 1 + w * rw, jw1 = min(jw0 + rw - 1, je);
   const bool vi = i <= nx;
   const bool act = local_ < gx_ * chunks_ && jw0 <= jw1;  // uniform over the wave
+  const bool edge_r = i == nx || lane == 63;              // right neighbour not in the wave
   auto at = [&](int a, int b) { return (long)(b - 1) * nx + (a - 1); };
-  // The first rows' operands do not depend on beta: they are loaded before the
-  // stop flags and the grid totals are read, so the latencies overlap.
-  T z0 = T(0), p0 = T(0), zd = T(0), pd = T(0), aS = T(0);
+  auto zval = [&](T rv, T ap) -> T { return PC == 1 ? jacobi<T>(rv, ap) : rv; };
+  // The first row's operands and the row below do not depend on beta: they
+  // are loaded before the stop flags and the grid totals are read, so the
+  // latencies overlap.
+  T r0 = T(0), p0 = T(0), rd = T(0), pd = T(0), dgd = T(0), aS = T(0), kw = T(0), kn = T(0), ke = T(0);
   if (act && vi) {
-    z0 = zs[at(i, jw0)];
+    r0 = zr[at(i, jw0)];
     p0 = dprev[at(i, jw0)];
     const int jd = jw0 > 1 ? jw0 - 1 : (o.perS ? ny : 0);
     if (jd > 0) {
-      zd = zs[at(i, jd)];
+      rd = zr[at(i, jd)];
       pd = dprev[at(i, jd)];
+      if (PC == 1) dgd = A.diag[at(i, jd)];
     }
     aS = A.ky[kyi(nx, i, jw0)];
+    kw = A.kx[kxi(nx, i, jw0)];
+    kn = A.ky[kyi(nx, i, jw0 + 1)];
+    if (i == nx) ke = A.kx[kxi(nx, nx + 1, jw0)];
+    else if (lane == 63) ke = A.kx[kxi(nx, i + 1, jw0)];
   }
   const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
   const bool first = scal[sl.first] != 0.0;
+  const T alpha_prev = (T)scal[sl.step];
   // slots: 0 r.r, 1 r.z, 2 b.b (first outer iteration only); the inner
   // recurrence of the mixed mode needs r.z only -- and, at the first
   // iteration after a reliable update, slots 2 and 3: z.(A d_prev) and
@@ -427,9 +463,13 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
     } else {
       scal[sl.iters] += 1.0;
       scal[sl.rz + (k & 1)] = tot[1];
+      scal[sl.pend] = 0.0;  // this kernel applies the pending update
     }
   }
-  if (stop != F_RUN) return;
+  if (stop != F_RUN) return;  // the pending update stays recorded
+  // the update of iteration k-1 is pending unless this is the first
+  // iteration (after a reliable update it was folded into the correction)
+  const bool pending = !first && !(sl.inner && k == 1);
   // After a reliable update the residual was replaced, and beta = rz / rz_prev
   // no longer makes d conjugate to d_prev (once the FP32 recurrence's
   // residual has fallen below the true one, the ratio is large and the old
@@ -440,41 +480,53 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   else if (!first) bd = tot[1] / rz_prev;
   const T beta = (T)bd;
   T acc[1] = {T(0)};
-  T cur = z0 + beta * p0, down = zd + beta * pd;
+  T aE = __shfl_down(kw, 1, 64);
+  if (edge_r) aE = ke;
+  T aP = kw + aE + aS + kn + (T)o.eps;
+  T pcur = p0;
+  T cur = zval(r0, aP) + beta * p0;
+  T down = zval(rd, dgd) + beta * pd;
   if (act) {
     for (int j0 = jw0; j0 <= jw1; j0 += G) {
-      // loads of the batch: up-row z / d_prev, kx(i), ky(i, j+1), and at the
-      // strip edges the x-neighbours and kx(i+1)
-      T zu[G], pu[G], kw[G], kn[G], ke[G], zl[G], pl[G], zr[G], pr[G];
+      // loads of the batch: the row above (r or z, d_prev, kx(i), ky(i, +1),
+      // and kx(i+1) at the strip edge), x, and at the strip edges the
+      // x-neighbours (r or z, d_prev, diagonal)
+      T ru[G], pu[G], kwu[G], knu[G], keu[G], ksu[G], xo[G], rl[G], pl[G], dl[G], rr[G], pr[G], dr[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int j = j0 + g;
-        zu[g] = pu[g] = kw[g] = kn[g] = ke[g] = zl[g] = pl[g] = zr[g] = pr[g] = T(0);
+        ru[g] = pu[g] = kwu[g] = knu[g] = keu[g] = ksu[g] = xo[g] = T(0);
+        rl[g] = pl[g] = dl[g] = rr[g] = pr[g] = dr[g] = T(0);
         if (j <= jw1 && vi) {
           const int ju = j < ny ? j + 1 : (o.perN ? 1 : 0);
           if (ju > 0) {
-            zu[g] = zs[at(i, ju)];
+            ru[g] = zr[at(i, ju)];
             pu[g] = dprev[at(i, ju)];
+            kwu[g] = A.kx[kxi(nx, i, ju)];
+            knu[g] = A.ky[kyi(nx, i, ju + 1)];
+            if (ju != j + 1) ksu[g] = A.ky[kyi(nx, i, ju)];  // periodic wrap: the row's own south face
+            if (i == nx) keu[g] = A.kx[kxi(nx, nx + 1, ju)];
+            else if (lane == 63) keu[g] = A.kx[kxi(nx, i + 1, ju)];
           }
-          kw[g] = A.kx[kxi(nx, i, j)];
-          kn[g] = A.ky[kyi(nx, i, j + 1)];
+          if (pending) xo[g] = x[at(i, j)];
           if (lane == 0) {
             const int il = i > 1 ? i - 1 : (o.perW ? nx : 0);
             if (il > 0) {
-              zl[g] = zs[at(il, j)];
+              rl[g] = zr[at(il, j)];
               pl[g] = dprev[at(il, j)];
+              if (PC == 1) dl[g] = A.diag[at(il, j)];
             }
           }
           if (i == nx) {
-            ke[g] = A.kx[kxi(nx, nx + 1, j)];
             if (o.perE) {
-              zr[g] = zs[at(1, j)];
+              rr[g] = zr[at(1, j)];
               pr[g] = dprev[at(1, j)];
+              if (PC == 1) dr[g] = A.diag[at(1, j)];
             }
           } else if (lane == 63) {
-            ke[g] = A.kx[kxi(nx, i + 1, j)];
-            zr[g] = zs[at(i + 1, j)];
+            rr[g] = zr[at(i + 1, j)];
             pr[g] = dprev[at(i + 1, j)];
+            if (PC == 1) dr[g] = A.diag[at(i + 1, j)];
           }
         }
       }
@@ -482,31 +534,37 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
       for (int g = 0; g < G; ++g) {
         const int j = j0 + g;
         if (j > jw1) break;  // uniform over the wave
-        const T up = zu[g] + beta * pu[g];
+        // the row above: a_P from its four faces (its south face is this
+        // row's north face unless the row wraps), z, d
+        T aEu = __shfl_down(kwu[g], 1, 64);
+        if (edge_r) aEu = keu[g];
+        const T aSu = (j < ny) ? kn : ksu[g];
+        const T aPu = kwu[g] + aEu + aSu + knu[g] + (T)o.eps;
+        const T up = zval(ru[g], aPu) + beta * pu[g];
         T left = __shfl_up(cur, 1, 64);
         T right = __shfl_down(cur, 1, 64);
-        T aE = __shfl_down(kw[g], 1, 64);
-        if (lane == 0) left = zl[g] + beta * pl[g];
-        if (i == nx || lane == 63) {
-          right = zr[g] + beta * pr[g];
-          aE = ke[g];
-        }
+        if (lane == 0) left = zval(rl[g], dl[g]) + beta * pl[g];
+        if (edge_r) right = zval(rr[g], dr[g]) + beta * pr[g];
         if (vi) {
-          const T aW = kw[g], aN = kn[g];
-          const T aP = aW + aE + aS + aN + (T)o.eps;
           T val = aP * cur;
-          if (i > 1 || o.perW) val = val - aW * left;
+          if (i > 1 || o.perW) val = val - kw * left;
           if (i < nx || o.perE) val = val - aE * right;
           if (j > 1 || o.perS) val = val - aS * down;
-          if (j < ny || o.perN) val = val - aN * up;
+          if (j < ny || o.perN) val = val - kn * up;
           const long q = at(i, j);
           d[q] = cur;
           Ad[q] = val;
+          if (pending) x[q] = xo[g] + alpha_prev * pcur;
           acc[0] += cur * val;
         }
         down = cur;
         cur = up;
-        aS = kn[g];
+        pcur = pu[g];
+        aS = kn;
+        kw = kwu[g];
+        kn = knu[g];
+        aE = aEu;
+        aP = aPu;
       }
     }
   }
@@ -514,11 +572,11 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
 }
 
 template <typename T, int PC>
-__global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ diag, T *__restrict__ x,
-                                               T *__restrict__ r, T *__restrict__ z, const T *__restrict__ d,
+__global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ diag, T *__restrict__ r,
                                                const T *__restrict__ Ad, double *scal, PcgSlots sl,
                                                const double *pin, double *pout, int k) {
   const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
+  const double rz = scal[sl.rz + (k & 1)];
   double tot[1];
   grid_totals<1>(pin, gridDim.x * gridDim.y, tot);
   if (flag_own != 0.0 || flag_outer != 0.0) return;
@@ -527,23 +585,38 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
     if (lead()) scal[sl.flag] = (dAd == dAd) ? (double)F_BREAK : (double)F_NONFINITE;
     return;
   }
-  if (lead()) scal[sl.first] = 0.0;
-  const T alpha = (T)(scal[sl.rz + (k & 1)] / dAd);
+  const T alpha = (T)(rz / dAd);
+  if (lead()) {
+    scal[sl.first] = 0.0;
+    scal[sl.step] = (double)alpha;
+    scal[sl.pend] = (k & 1) ? 2.0 : 1.0;  // d of iteration k: d1 when k is odd
+  }
   T acc[3] = {T(0), T(0), T(0)};
   CELL_LOOP(o) {
     const long q = (long)(j - 1) * o.nx + (i - 1);
-    x[q] = x[q] + alpha * d[q];
     const T rk = r[q] - alpha * Ad[q];
     r[q] = rk;
     acc[0] += rk * rk;
     if (PC == 0) acc[1] += rk * rk;
-    if (PC == 1) {
-      const T zk = jacobi<T>(rk, diag[q]);
-      z[q] = zk;
-      acc[1] += rk * zk;
-    }
+    if (PC == 1) acc[1] += rk * jacobi<T>(rk, diag[q]);
   }
   block_partials<3, T>(acc, pout);
+}
+
+// The solution update left pending by the last B of a loop: x += alpha d
+// with the d buffer B recorded (d0 or d1), unless the guard slot is set.
+template <typename T>
+__global__ void __launch_bounds__(BLK) k_flush_x(DevOp o, T *__restrict__ x, const T *__restrict__ d0,
+                                                 const T *__restrict__ d1, const double *scal, int pend, int step,
+                                                 int guard) {
+  const double pv = scal[pend];
+  if (pv == 0.0 || scal[guard] != 0.0) return;
+  const T *dl = (pv == 1.0) ? d0 : d1;
+  const T alpha = (T)scal[step];
+  CELL_LOOP(o) {
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    x[q] = x[q] + alpha * dl[q];
+  }
 }
 
 // ---------------------------------------------- single-reduction CG (FP32)
@@ -850,6 +923,7 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
   grid_totals<4>(pin, nblk, tot);
   if (threadIdx.x != 0) return;
   scal[S_RMEAN] = singular ? tot[3] * inv_n : 0.0;
+  scal[I_PEND] = 0.0;  // folded into x by the flush before the correction
   if (scal[I_FLAG] == (double)F_MAXIT || scal[I_ITERS] >= scal[I_MAXIT]) {
     scal[S_FLAG] = (double)F_MAXIT;
     return;
@@ -1051,7 +1125,7 @@ static DevOp dev_op(const Mesh &m) {
 static dim3 red_grid(const Mesh &m) {
   const int gx = (m.nx + 63) / 64;
   const int band = (((m.ny + 7) / 8) + 3) & ~3;
-  int chunks = 1024 / (8 * gx);
+  int chunks = RED_BLOCKS / (8 * gx);
   if (chunks < 1) chunks = 1;
   const int max_chunks = (band + 3) / 4;  // at least 4 rows per chunk
   if (chunks > max_chunks) chunks = max_chunks;
@@ -1213,8 +1287,10 @@ static int fill(Ctx &c, T *x, long n, T v) {
   return COVOF_OK;
 }
 
-static const PcgSlots OUTER = {S_RZ, S_RR, S_RELRES, S_FLAG, S_ITERS, S_BNORM, S_TOL, S_FIRST, S_MAXIT, 1, 0};
-static const PcgSlots INNER = {I_RZ, I_RR, I_RELRES, I_FLAG, I_ITERS, S_BNORM, S_TOL, I_FIRST, I_MAXIT, 0, 1};
+static const PcgSlots OUTER = {S_RZ,  S_RR,    S_RELRES, S_FLAG, S_ITERS, S_BNORM, S_TOL,
+                               S_FIRST, S_MAXIT, 1, 0, S_STEP, S_PEND};
+static const PcgSlots INNER = {I_RZ,    I_RR,    I_RELRES, I_FLAG, I_ITERS, S_BNORM, S_TOL,
+                               I_FIRST, I_MAXIT, 0, 1, I_STEP, I_PEND};
 constexpr int NREAD = 32;  // scalar slots read back by the host
 
 // Partial buffers: [0] written by A kernels, [1] by B / residual / rz kernels,
@@ -1229,28 +1305,35 @@ static int status_of_flag(double f) {
   return COVOF_NOT_CONVERGED;
 }
 
-// One PCG iteration k (A_k, B_k) on (x, r, z); zs = z (PC 1/2) or r (PC 0);
-// d_prev / d alternate between d0 and d1 with the parity of k (the first
-// iteration, k odd, reads d0, which must hold zeros).  When `ev` is given the
-// two launches are bracketed by its event pairs (profiling).
+// One PCG iteration k (A_k, B_k) on (x, r, z); A reads r (PC 0/1: z is
+// formed from r) or z (explicit preconditioners); d_prev / d alternate between
+// d0 and d1 with the parity of k (the first iteration, k odd, reads d0, which
+// must hold zeros).  When `timed`, the two launches are bracketed by events.
 template <typename T, int PC>
 static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *Ad, const PcgSlots &sl,
                           int k, bool timed) {
   const DevOp o = dev_op(c.m);
   const dim3 g = red_grid(c.m);
   double *pa = part_buf(c, 0), *pb = part_buf(c, 1);
-  const T *zs = (PC == 0) ? r : z;
+  const T *zr = (PC <= 1) ? r : z;
   T *dprev = (k & 1) ? d0 : d1;
   T *dnew = (k & 1) ? d1 : d0;
   const int tag = timed ? (sizeof(T) == 8 ? COVOF_KERNEL_PCG_A_FP64 : COVOF_KERNEL_PCG_A_FP32) : -1;
-  launch_tagged(c, tag, k_pcg_a<T>, g, dim3(BLK), o, A, zs, (const T *)dprev, dnew, Ad, c.ps.scal, sl,
+  launch_tagged(c, tag, k_pcg_a<T, PC>, g, dim3(BLK), o, A, zr, (const T *)dprev, dnew, Ad, x, c.ps.scal, sl,
                 (const double *)pb, pa, k);
-  launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, x, r, z,
-                (const T *)dnew, (const T *)Ad, c.ps.scal, sl, (const double *)pa, pb, k);
+  launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, r, (const T *)Ad,
+                c.ps.scal, sl, (const double *)pa, pb, k);
   if (PC >= 2) {
     explicit_precond<T>(c, PC, (const T *)r, z, sl.flag);
     hipLaunchKernelGGL(k_pcg_rz<T>, g, dim3(BLK), 0, c.stream, o, (const T *)r, (const T *)z, c.ps.scal, sl, pb);
   }
+}
+
+// Applies the solution update a loop left pending (see k_pcg_a).
+template <typename T>
+static void flush_x(Ctx &c, T *x, const T *d0, const T *d1, const PcgSlots &sl, int guard) {
+  hipLaunchKernelGGL(k_flush_x<T>, red_grid(c.m), dim3(BLK), 0, c.stream, dev_op(c.m), x, d0, d1,
+                     (const double *)c.ps.scal, sl.pend, sl.step, guard);
 }
 
 // The single-reduction loop's buffers: r, w, p, s at parity 0 and 1.
@@ -1299,6 +1382,8 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
                         m.bc[3] != BC_NEUMANN) ? 1 : 0;
   const double inv_n = 1.0 / (double)n;
   if (!first) {
+    // the FP32 loop's last solution update, then the correction
+    flush_x<float>(c, P.x32, P.p32, P.p32 + n, INNER, S_FLAG);
     if (singular)
       hipLaunchKernelGGL(k_sum32, g, dim3(BLK), 0, c.stream, o, (const float *)P.x32, ps, (const double *)P.scal);
     hipLaunchKernelGGL(k_add_correction, dim3(1024), dim3(BLK), 0, c.stream, n, x, P.x32, (const double *)P.scal,
@@ -1483,6 +1568,8 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
     const int cyc = cycle64(pc);
     if ((st = run_cycles(c, P.graph[0], 1 + pc, pc, false, cyc, maxit / cyc + 2))) return st;
   }
+  flush_x<double>(c, P.x, P.p, P.t0, OUTER, S_ZERO);
+  COVOF_CHECK_LAUNCH();
   if ((st = read_scalars(c, NREAD))) return st;
   const int it = (int)P.hscal[S_ITERS];
   if (iters) *iters = it;

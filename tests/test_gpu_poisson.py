@@ -54,6 +54,28 @@ def test_pcg_converged_vs_oracle(cv, orc, precond):
     assert np.linalg.norm(r) / np.linalg.norm(co["b"]) < 1e-11
 
 
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_pcg_iteration_cap_vs_oracle(cv, orc, precond):
+    """A solve stopped by its iteration cap in the middle of a graph cycle
+    returns the iterate of that iteration: the GPU applies each solution
+    update one kernel late, and the update of the last iteration must not be
+    lost (nor applied twice)."""
+    n = 64
+    c = _case(n)
+    ctx = cv.Context(n, n, bc=c["bc"])
+    b = ctx.poisson_assemble(to_gpu(c["u"]), to_gpu(c["v"]), to_gpu(c["rho_u"]), to_gpu(c["rho_v"]), c["dt"])
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, c["u"], c["v"], c["rho_u"], c["rho_v"], c["dt"])
+    for cap in (1, 37, 64, 65):
+        x = ctx.new_work()
+        res = ctx.poisson_solve(b, x, precond=precond, tol=1e-30, max_iter=cap)
+        assert not res.converged and res.iters == cap, (cap, res)
+        xo, it, _ = orc.sp_pcg(m, precond, co, co["b"], np.zeros((n, n)), tol=1e-30, maxit=cap)
+        assert it == cap
+        xg = x.cpu().numpy()
+        assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max(), cap
+
+
 def test_sp_reference_mode_vs_oracle(cv, orc):
     """The reference's own SP settings: CG + SGS (SSOR), tol 1e-7, 1000 iterations,
     x0 = p^n, mean removal; rising-bubble-like density on 64 x 128 walls."""
