@@ -115,6 +115,20 @@ CV_HD double cut_box_area(double xmin, double xmax, double ymin, double ymax, do
                     nx * xmax + ny * ymax, nx * xmin + ny * ymax, alpha);
 }
 
+// cut_box_area of a full cell's segment (n = (1,0), alpha = -1e9: every
+// corner is inside whenever |x| < 1e9), without the clip: the four corners
+// are emitted in the clip's order -- (xmax,ymin), (xmax,ymax), (xmin,ymax),
+// (xmin,ymin) -- so the shoelace terms, and their order, are the clip's.
+CV_HD double full_box_area(double xmin, double xmax, double ymin, double ymax) {
+  if ((xmax - xmin) <= 1.0e-14 || (ymax - ymin) <= 1.0e-14) return 0.0;
+  double s = 0.0;
+  s = s + (xmax * ymax - xmax * ymin);
+  s = s + (xmax * ymax - xmin * ymax);
+  s = s + (xmin * ymin - xmin * ymax);
+  s = s + (xmin * ymin - xmax * ymin);
+  return 0.5 * fabs(s);
+}
+
 // compute_alpha (src/module_interface_reconstruction.f90:105-147): 45-step
 // bisection of the line constant so that the kept area equals c * box area.
 // The corner products n.x_k do not depend on alpha and are formed once.

@@ -359,41 +359,49 @@ __global__ void __launch_bounds__(TBLK) k_classify(const double *__restrict__ C,
 
 // ------------------------------------------------------------ flux + update
 
-// Volume through face (i,j) of the DIR pass (:1204-1222 for x, :1249-1267 for y).
-template <int DIR>
-__device__ __forceinline__ double face_flux(const double *__restrict__ C, const double *__restrict__ vel,
-                                            const DevMesh &m, const double *snx, const double *sny,
-                                            const double *sal, int i, int j, double dt) {
-  const double uf = vel[H(m, i, j)];
-  const double del = fabs(uf) * dt;
-  if (del <= 1.0e-14) return 0.0;
-  int di, dj;
+// Swept box and donor cell of face (i,j) of the DIR pass (:1204-1222 for x,
+// :1249-1267 for y); zero: |u| dt <= 1e-14 (no flux through the face).
+struct FaceBox {
   double xmin, xmax, ymin, ymax;
+  long kd;
+  bool pos, zero;
+};
+
+template <int DIR>
+__device__ __forceinline__ FaceBox face_box(double uf, const DevMesh &m, int i, int j, double dt) {
+  FaceBox b;
+  const double del = fabs(uf) * dt;
+  b.zero = del <= 1.0e-14;
+  b.pos = uf > 0.0;
+  int di, dj;
   if (DIR == 0) {
     const double xp = face_coord(i, m.dx);
-    ymin = face_coord(j, m.dy);
-    ymax = face_coord(j + 1, m.dy);
-    if (uf > 0.0) { di = i - 1; dj = j; xmin = xp - del; xmax = xp; }
-    else { di = i; dj = j; xmin = xp; xmax = xp + del; }
+    b.ymin = face_coord(j, m.dy);
+    b.ymax = face_coord(j + 1, m.dy);
+    if (b.pos) { di = i - 1; dj = j; b.xmin = xp - del; b.xmax = xp; }
+    else { di = i; dj = j; b.xmin = xp; b.xmax = xp + del; }
   } else {
     const double yp = face_coord(j, m.dy);
-    xmin = face_coord(i, m.dx);
-    xmax = face_coord(i + 1, m.dx);
-    if (uf > 0.0) { di = i; dj = j - 1; ymin = yp - del; ymax = yp; }
-    else { di = i; dj = j; ymin = yp; ymax = yp + del; }
+    b.xmin = face_coord(i, m.dx);
+    b.xmax = face_coord(i + 1, m.dx);
+    if (b.pos) { di = i; dj = j - 1; b.ymin = yp - del; b.ymax = yp; }
+    else { di = i; dj = j; b.ymin = yp; b.ymax = yp + del; }
   }
-  const long kd = H(m, di, dj);
-  const double cdn = C[kd];
-  double gnx = 1.0, gny = 0.0, gal;
-  if (is_mixed(cdn)) {
-    gnx = snx[kd];
-    gny = sny[kd];
-    gal = sal[kd];
+  b.kd = H(m, di, dj);
+  return b;
+}
+
+// Face (i, j) as the update uses it: on a periodic axis the faces 1 and n+1
+// both carry the flux computed with the real donor (face n+1 when the
+// velocity at face 1 is >= 0, face 1 otherwise; :1223-1231 for x,
+// :1268-1275 for y).
+template <int DIR>
+__device__ __forceinline__ void pass_face(const double *__restrict__ vel, const DevMesh &m, int &i, int &j) {
+  if (DIR == 0) {
+    if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx + 1)) i = (vel[H(m, 1, j)] >= 0.0) ? m.nx + 1 : 1;
   } else {
-    gal = (cdn <= 1.0e-6) ? 1.0e9 : -1.0e9;
+    if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny + 1)) j = (vel[H(m, i, 1)] >= 0.0) ? m.ny + 1 : 1;
   }
-  const double a = cut_box_area(xmin, xmax, ymin, ymax, gnx, gny, gal);
-  return uf > 0.0 ? a : -a;
 }
 
 // Halo of dst after one exchange_halos_scalar + apply_vof_bc on one rank
@@ -448,22 +456,6 @@ __device__ __forceinline__ void write_halo(double *dst, const double *src, const
   }
 }
 
-// Flux of face (i, j) of the DIR pass as the update uses it: on a periodic
-// axis the faces 1 and n+1 both carry the flux computed with the real donor
-// (face n+1 when the velocity at face 1 is >= 0, face 1 otherwise;
-// :1223-1231 for x, :1268-1275 for y).
-template <int DIR>
-__device__ __forceinline__ double pass_face_flux(const double *__restrict__ C, const double *__restrict__ vel,
-                                              const DevMesh &m, const double *snx, const double *sny,
-                                              const double *sal, int i, int j, double dt) {
-  if (DIR == 0) {
-    if (m.bcl == BC_PERIODIC && (i == 1 || i == m.nx + 1)) i = (vel[H(m, 1, j)] >= 0.0) ? m.nx + 1 : 1;
-  } else {
-    if (m.bcb == BC_PERIODIC && (j == 1 || j == m.ny + 1)) j = (vel[H(m, i, 1)] >= 0.0) ? m.ny + 1 : 1;
-  }
-  return face_flux<DIR>(C, vel, m, snx, sny, sal, i, j, dt);
-}
-
 // Tile of block b: consecutive workgroups go to the eight XCDs in turn, so
 // XCD x is given the contiguous run of tiles [start_x, start_x + count_x)
 // (row-major over tiles): the tiles above and below a tile, whose edge rows
@@ -479,8 +471,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // (:1234-1239 / :1276-1283).  Writes dst's interior and halo, and the tile's
 // list of mixed cells of dst for the next reconstruction.
 // Phase 1 evaluates every face of the tile once (65 x 16 x-faces or 64 x 17
-// y-faces, one call site) into LDS with the face velocities; phase 2 updates
-// the cells from the two face values each.
+// y-faces) into LDS: a face whose donor is empty or
+// full has the closed-form value of the clip (0, or the swept box's
+// shoelace area), a face whose donor is mixed joins the tile's list, and
+// phase 2 clips those with every lane busy; phase 3 updates the cells from
+// the two face values each.
 template <int DIR, bool FIRST>
 __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
                                                       const double *__restrict__ vel, DevMesh m, double dt,
@@ -490,29 +485,102 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
                                                       int corner_fix) {
   constexpr int FW = DIR == 0 ? TX + 1 : TX;  // faces per LDS row
   constexpr int NF = DIR == 0 ? (TX + 1) * TY : TX * (TY + 1);
+  // the list of mixed-donor faces is dead once phase 2 has run: the list of
+  // mixed cells of dst reuses its memory (12.8 KiB of LDS per block, so
+  // VGPRs rather than LDS bound the blocks per CU)
+  static_assert(NF >= TCELLS, "the face list's memory holds the cell list");
   __shared__ double s_flux[NF];
-  __shared__ double s_vel[NF];
-  __shared__ int s_list[TCELLS];
-  __shared__ int s_count;
-  if (threadIdx.x == 0) s_count = 0;
+  __shared__ int s_mix[NF];
+  int *s_list = s_mix;
+  __shared__ int s_count, s_nmix;
+  if (threadIdx.x == 0) {
+    s_count = 0;
+    s_nmix = 0;
+  }
+  __syncthreads();
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % tiles_x(m), ty = t / tiles_x(m);
   const int i0 = tx * TX + 1, j0 = ty * TY + 1;
   const int imax = DIR == 0 ? m.nx + 1 : m.nx, jmax = DIR == 0 ? m.ny : m.ny + 1;
-  for (int f = threadIdx.x; f < NF; f += TBLK) {
+  // all loads of a thread's faces first -- the face velocity and both
+  // candidate donors (the velocity's sign picks one) -- so they are in flight
+  // together; then the closed forms and the list of mixed-donor faces
+  constexpr int NR = (NF + TBLK - 1) / TBLK;
+  double uf[NR], cpos[NR], cneg[NR];
+  int fis[NR], fjs[NR];
+  bool ok[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int f = r * TBLK + threadIdx.x;
     const int fr = f / FW;
-    const int fi = i0 + (f - fr * FW), fj = j0 + fr;
-    if (fi <= imax && fj <= jmax) {
-      s_vel[f] = vel[H(m, fi, fj)];
-      s_flux[f] = pass_face_flux<DIR>(src, vel, m, snx, sny, sal, fi, fj, dt);
+    int fi = i0 + (f - fr * FW), fj = j0 + fr;
+    ok[r] = f < NF && fi <= imax && fj <= jmax;
+    uf[r] = cpos[r] = cneg[r] = 0.0;
+    if (ok[r]) {
+      pass_face<DIR>(vel, m, fi, fj);
+      uf[r] = vel[H(m, fi, fj)];
+      cpos[r] = DIR == 0 ? src[H(m, fi - 1, fj)] : src[H(m, fi, fj - 1)];
+      cneg[r] = src[H(m, fi, fj)];
     }
+    fis[r] = fi;
+    fjs[r] = fj;
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int f = r * TBLK + threadIdx.x;
+    bool mixed = false;
+    if (ok[r]) {
+      const FaceBox b = face_box<DIR>(uf[r], m, fis[r], fjs[r], dt);
+      double fl = 0.0;
+      if (!b.zero) {
+        const double cdn = b.pos ? cpos[r] : cneg[r];
+        if (is_mixed(cdn)) {
+          mixed = true;
+        } else if (cdn <= 1.0e-6) {
+          fl = b.pos ? 0.0 : -0.0;  // empty donor: no vertex inside
+        } else {
+          const double a = full_box_area(b.xmin, b.xmax, b.ymin, b.ymax);
+          fl = b.pos ? a : -a;
+        }
+      }
+      s_flux[f] = fl;
+    }
+    lds_append(mixed, f, s_mix, &s_nmix);  // every lane of the wave takes part
+  }
+  __syncthreads();
+  const int nmix = s_nmix;
+  for (int e = threadIdx.x; e < nmix; e += TBLK) {
+    const int f = s_mix[e];
+    const int fr = f / FW;
+    int fi = i0 + (f - fr * FW), fj = j0 + fr;
+    pass_face<DIR>(vel, m, fi, fj);
+    const FaceBox b = face_box<DIR>(vel[H(m, fi, fj)], m, fi, fj, dt);
+    const double a = cut_box_area(b.xmin, b.xmax, b.ymin, b.ymax, snx[b.kd], sny[b.kd], sal[b.kd]);
+    s_flux[f] = b.pos ? a : -a;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int i = i0 + lane;
   const double area = m.dx * m.dy;
-  const double h = DIR == 0 ? m.dx : m.dy;
-  for (int r = 0; r < TY / 4; ++r) {
+  const double dth = dt / (DIR == 0 ? m.dx : m.dy);
+  constexpr int NRC = TY / 4;  // rows per thread
+  double c0[NRC], vlo[NRC], vhi[NRC];
+  bool cdo[NRC];
+#pragma unroll
+  for (int r = 0; r < NRC; ++r) {  // loads of all rows first
+    const int j = j0 + (threadIdx.x >> 6) + 4 * r;
+    c0[r] = vlo[r] = vhi[r] = 0.0;
+    cdo[r] = false;
+    if (i <= m.nx && j <= m.ny) {
+      const long k = H(m, i, j);
+      c0[r] = src[k];
+      vlo[r] = vel[k];
+      vhi[r] = vel[DIR == 0 ? H(m, i + 1, j) : H(m, i, j + 1)];
+      if (!FIRST) cdo[r] = cd[k] != 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NRC; ++r) {
     const int row = (threadIdx.x >> 6) + 4 * r;
     const int j = j0 + row;
     const bool inside = i <= m.nx && j <= m.ny;
@@ -520,19 +588,18 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
     double cn = 0.0;
     if (inside) {
       k = H(m, i, j);
-      const double c0 = src[k];
       const int lo = row * FW + lane, hi = DIR == 0 ? lo + 1 : lo + FW;
       const double flo = s_flux[lo], fhi = s_flux[hi];
-      const double du = s_vel[hi] - s_vel[lo];
+      const double du = vhi[r] - vlo[r];
       double cdv;
       if (FIRST) {
-        const bool bb = c0 >= 0.5;
+        const bool bb = c0[r] >= 0.5;
         cd[k] = bb ? 1 : 0;
         cdv = bb ? 1.0 : 0.0;
       } else {
-        cdv = cd[k] ? 1.0 : 0.0;
+        cdv = cdo[r] ? 1.0 : 0.0;
       }
-      cn = c0 - (fhi - flo) / area + cdv * (dt / h) * du;
+      cn = c0[r] - (fhi - flo) / area + cdv * dth * du;
       if (!FIRST) {
         const double tc = 1.0 < cn ? 1.0 : cn;  // min(1, c)
         cn = 0.0 > tc ? 0.0 : tc;               // max(0, .)

@@ -9,3 +9,6 @@ extern "C" double host_cut_box_area(double a, double b, double c, double d, doub
 extern "C" double host_compute_alpha(double a, double b, double c, double d, double nx, double ny, double cc) {
   return covof::compute_alpha(a, b, c, d, nx, ny, cc);
 }
+extern "C" double host_full_box_area(double a, double b, double c, double d) {
+  return covof::full_box_area(a, b, c, d);
+}

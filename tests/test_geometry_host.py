@@ -21,7 +21,8 @@ def host(tmp_path_factory):
     subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I", INC, SRC,
                     "-o", out], check=True)
     L = ct.CDLL(out)
-    for n, k in (("host_hypot_ref", 2), ("host_cut_box_area", 7), ("host_compute_alpha", 7)):
+    for n, k in (("host_hypot_ref", 2), ("host_cut_box_area", 7), ("host_compute_alpha", 7),
+                 ("host_full_box_area", 4)):
         f = getattr(L, n)
         f.restype = ct.c_double
         f.argtypes = [ct.c_double] * k
@@ -58,3 +59,19 @@ def test_geometry_host_vs_oracle(host, orc):
         a = (g["xmin"][k], g["xmax"][k], g["ymin"][k], g["ymax"][k], g["nx"][k], g["ny"][k])
         assert host.host_cut_box_area(*a, g["alpha"][k]) == g["area"][k]
         assert host.host_compute_alpha(*a, g["c"][k]) == g["alpha_out"][k]
+
+
+def test_full_box_area_is_the_clip_of_a_full_cell(host):
+    """The flux kernel's shortcut for full donor cells equals cut_box_area with
+    the full cell's fixed segment (n = (1, 0), alpha = -1e9) bit-for-bit, on
+    swept boxes of unit-square and stretched domains and on degenerate ones."""
+    rng = np.random.default_rng(3)
+    for lx in (1.0, 3.7, 1e-3, 250.0):
+        x0 = rng.uniform(0, lx, 20000)
+        y0 = rng.uniform(0, lx, 20000)
+        w = lx * 10.0 ** rng.uniform(-9, -2, 20000)
+        h = lx * 10.0 ** rng.uniform(-9, -2, 20000)
+        w[::97] = 0.0
+        h[::89] = 5e-15
+        for a, b, c, d in zip(x0, x0 + w, y0, y0 + h):
+            assert host.host_full_box_area(a, b, c, d) == host.host_cut_box_area(a, b, c, d, 1.0, 0.0, -1.0e9)
