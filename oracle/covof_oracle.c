@@ -448,6 +448,157 @@ void orc_update_properties(const orc_mesh *m, double *C, double rho1, double rho
     }
 }
 
+/* ------------------------------------------------- smoothing, curvature, CSF */
+
+static double clamp01(double c) {
+  const double t = 1.0 < c ? 1.0 : c;
+  return 0.0 > t ? 0.0 : t;
+}
+
+/* Homogeneous Neumann copies on the physical sides (interior rows / columns). */
+static void neumann_sides(const orc_mesh *m, double *f) {
+  const int nx = m->nx, ny = m->ny;
+  for (int j = 1; j <= ny; ++j) {
+    if (!per_x(m)) f[H(m, 0, j)] = f[H(m, 1, j)];
+    if (!per_x(m)) f[H(m, nx + 1, j)] = f[H(m, nx, j)];
+  }
+  for (int i = 1; i <= nx; ++i) {
+    if (!per_y(m)) f[H(m, i, 0)] = f[H(m, i, 1)];
+    if (!per_y(m)) f[H(m, i, ny + 1)] = f[H(m, i, ny)];
+  }
+}
+
+/* One lexicographic Gauss-Seidel sweep of (I - D Lap) Cs = C (forward or
+ * backward), zero weights across physical sides (:413-449). */
+static void svof_sweep(const orc_mesh *m, const double *C, double *Cs, double wx, double wy, int backward) {
+  const int nx = m->nx, ny = m->ny;
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  for (int jj = 1; jj <= ny; ++jj)
+    for (int ii = 1; ii <= nx; ++ii) {
+      const int i = backward ? nx + 1 - ii : ii, j = backward ? ny + 1 - jj : jj;
+      const double aE = (i == nx && br) ? 0.0 : wx;
+      const double aW = (i == 1 && bl) ? 0.0 : wx;
+      const double aN = (j == ny && bt) ? 0.0 : wy;
+      const double aS = (j == 1 && bb) ? 0.0 : wy;
+      const double aP = 1.0 + aE + aW + aN + aS;
+      const double sum = aE * Cs[H(m, i + 1, j)] + aW * Cs[H(m, i - 1, j)] + aN * Cs[H(m, i, j + 1)] +
+                         aS * Cs[H(m, i, j - 1)];
+      Cs[H(m, i, j)] = (C[H(m, i, j)] + sum) / aP;
+    }
+}
+
+/* src/module_interface_vof.f90:387-476 */
+void orc_svof_smooth(const orc_mesh *m, const double *C, double width, double *Cs) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)(nx + 2) * (ny + 2);
+  double k = width;
+  if (k <= 0.0) k = 1.0;
+  const double D = k * m->dx * m->dy;
+  const double wx = D / (m->dx * m->dx), wy = D / (m->dy * m->dy);
+  memcpy(Cs, C, n * sizeof(double));
+  for (int it = 0; it < 25; ++it) {
+    neumann_sides(m, Cs);
+    orc_exchange_halos_scalar(m, Cs);
+    svof_sweep(m, C, Cs, wx, wy, 0);
+    svof_sweep(m, C, Cs, wx, wy, 1);
+  }
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) Cs[H(m, i, j)] = clamp01(Cs[H(m, i, j)]);
+  neumann_sides(m, Cs);
+  Cs[H(m, 0, 0)] = Cs[H(m, 1, 1)];
+  Cs[H(m, nx + 1, 0)] = Cs[H(m, nx, 1)];
+  Cs[H(m, 0, ny + 1)] = Cs[H(m, 1, ny)];
+  Cs[H(m, nx + 1, ny + 1)] = Cs[H(m, nx, ny)];
+  orc_exchange_halos_scalar(m, Cs);
+}
+
+/* Halo of vof_smooth_diffusion: exchange, then zero-gradient copies on the
+ * physical sides (rows 0 / ny+1 over the full width, :1436-1440). */
+static void diffusion_halo(const orc_mesh *m, double *f) {
+  const int nx = m->nx, ny = m->ny;
+  orc_exchange_halos_scalar(m, f);
+  if (!per_x(m)) {
+    for (int j = 1; j <= ny; ++j) f[H(m, 0, j)] = f[H(m, 1, j)];
+    for (int j = 1; j <= ny; ++j) f[H(m, nx + 1, j)] = f[H(m, nx, j)];
+  }
+  if (!per_y(m)) {
+    for (int i = 0; i <= nx + 1; ++i) f[H(m, i, 0)] = f[H(m, i, 1)];
+    for (int i = 0; i <= nx + 1; ++i) f[H(m, i, ny + 1)] = f[H(m, i, ny)];
+  }
+}
+
+/* src/module_interface_vof.f90:1416-1461 */
+void orc_smooth_diffusion(const orc_mesh *m, const double *C, int steps, double *Ct) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)(nx + 2) * (ny + 2);
+  const double dx = m->dx, dy = m->dy;
+  memcpy(Ct, C, n * sizeof(double));
+  const double dtau = 0.2 * dmin(dx * dx, dy * dy);
+  if (steps < 1) steps = 3;
+  double *cn = malloc(n * sizeof(double));
+  for (int s = 0; s < steps; ++s) {
+    diffusion_halo(m, Ct);
+    memcpy(cn, Ct, n * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const double c = Ct[H(m, i, j)];
+        const double lx = (Ct[H(m, i + 1, j)] - 2.0 * c + Ct[H(m, i - 1, j)]) / (dx * dx);
+        const double ly = (Ct[H(m, i, j + 1)] - 2.0 * c + Ct[H(m, i, j - 1)]) / (dy * dy);
+        cn[H(m, i, j)] = c + dtau * (lx + ly);
+      }
+    memcpy(Ct, cn, n * sizeof(double));
+  }
+  diffusion_halo(m, Ct);
+  free(cn);
+}
+
+/* src/module_interface_vof.f90:627-789 (no markers) */
+void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
+                       double *curv, double *fx, double *fy) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)(nx + 2) * (ny + 2);
+  const double dx = m->dx, dy = m->dy;
+  if (!enable || sigma <= 0.0) {
+    for (long k = 0; k < n; ++k) curv[k] = fx[k] = fy[k] = 0.0;
+    return;
+  }
+  double *nxs = calloc(n, sizeof(double)), *nys = calloc(n, sizeof(double)), *kap = calloc(n, sizeof(double));
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double gx = (G[H(m, i + 1, j)] - G[H(m, i - 1, j)]) / (2.0 * dx);
+      const double gy = (G[H(m, i, j + 1)] - G[H(m, i, j - 1)]) / (2.0 * dy);
+      const double gm = sqrt(gx * gx + gy * gy);
+      if (gm > 1.0e-3 / dx) {
+        nxs[H(m, i, j)] = gx / gm;
+        nys[H(m, i, j)] = gy / gm;
+      }
+    }
+  orc_exchange_halos_scalar(m, nxs);
+  orc_exchange_halos_scalar(m, nys);
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx; ++i)
+      if (nxs[H(m, i, j)] != 0.0 || nys[H(m, i, j)] != 0.0)
+        kap[H(m, i, j)] = -((nxs[H(m, i + 1, j)] - nxs[H(m, i - 1, j)]) / (2.0 * dx) +
+                            (nys[H(m, i, j + 1)] - nys[H(m, i, j - 1)]) / (2.0 * dy));
+  orc_exchange_halos_scalar(m, kap);
+  memcpy(curv, kap, n * sizeof(double));
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx + 1; ++i) {
+      const double ka = 0.5 * (kap[H(m, i - 1, j)] + kap[H(m, i, j)]);
+      const double gc = (C[H(m, i, j)] - C[H(m, i - 1, j)]) / dx;
+      fx[H(m, i, j)] = sigma * ka * gc;
+    }
+  for (int j = 1; j <= ny + 1; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const double ka = 0.5 * (kap[H(m, i, j - 1)] + kap[H(m, i, j)]);
+      const double gc = (C[H(m, i, j)] - C[H(m, i, j - 1)]) / dy;
+      fy[H(m, i, j)] = sigma * ka * gc;
+    }
+  free(nxs);
+  free(nys);
+  free(kap);
+}
+
 /* ---------------------------------------------------------------- Poisson */
 
 static int wall(int btype) { return btype == BC_DIRICHLET || btype == BC_FREESLIP; }

@@ -48,6 +48,17 @@ void orc_update_properties(const orc_mesh *m, double *C, double rho1, double rho
                            double mu1, double mu2, double *rho, double *rho_u,
                            double *rho_v, double *mu, double *mu_node);
 
+/* The steps advect_vof runs after the transport (single rank, no markers):
+ * SVOF Helmholtz smoothing C -> Cs (src/module_interface_vof.f90:387-476),
+ * diffusion smoothing C -> Ct (:1416-1461; steps < 1 means 3), and the
+ * curvature + CSF forces (:627-789) with the normals taken from G (C, C_s or
+ * C_tilde) and the force gradient from C; enable == 0 or sigma <= 0 zeroes
+ * curv, fx, fy.  All arrays are halo fields. */
+void orc_svof_smooth(const orc_mesh *m, const double *C, double width, double *Cs);
+void orc_smooth_diffusion(const orc_mesh *m, const double *C, int steps, double *Ct);
+void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
+                       double *curv, double *fx, double *fy);
+
 /* Poisson (SP projection block). Coefficient arrays are (nx,ny). */
 void orc_poisson_assemble(const orc_mesh *m, const double *u, const double *v,
                           const double *rho_u, const double *rho_v, double dt,

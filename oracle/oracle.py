@@ -55,6 +55,9 @@ def lib():
             "orc_apply_vof_bc": (None, [M, _D]),
             "orc_wy_step": (None, [M, i, _D, _D, _D, d, ip]),
             "orc_update_properties": (None, [M, _D, d, d, d, d, _D, _D, _D, _D, _D]),
+            "orc_svof_smooth": (None, [M, _D, d, _D]),
+            "orc_smooth_diffusion": (None, [M, _D, i, _D]),
+            "orc_curvature_csf": (None, [M, _D, _D, i, d, _D, _D, _D]),
             "orc_poisson_assemble": (None, [M, _D, _D, _D, _D, d] + [_D] * 6),
             "orc_poisson_matvec": (None, [M] + [_D] * 7),
             "orc_poisson_sgs": (None, [M] + [_D] * 7),
@@ -110,6 +113,33 @@ def wy_step(m: Mesh, method: int, C: np.ndarray, u: np.ndarray, v: np.ndarray, d
     lib().orc_wy_step(ct.byref(m), method, _p(C), _p(np.ascontiguousarray(u)),
                       _p(np.ascontiguousarray(v)), dt, ct.byref(sc))
     return C, sc.value
+
+
+def svof_smooth(m: Mesh, C: np.ndarray, width: float) -> np.ndarray:
+    """smooth_svof_helmholtz: C_s (halo field) from C."""
+    Cs = np.empty((m.ny + 2, m.nx + 2))
+    lib().orc_svof_smooth(ct.byref(m), _p(np.ascontiguousarray(C, dtype=np.float64)), width, _p(Cs))
+    return Cs
+
+
+def smooth_diffusion(m: Mesh, C: np.ndarray, steps: int) -> np.ndarray:
+    """vof_smooth_diffusion: C_tilde (halo field) from C."""
+    Ct = np.empty((m.ny + 2, m.nx + 2))
+    lib().orc_smooth_diffusion(ct.byref(m), _p(np.ascontiguousarray(C, dtype=np.float64)), steps, _p(Ct))
+    return Ct
+
+
+def curvature_csf(m: Mesh, G: np.ndarray, C: np.ndarray, sigma: float, enable: int = 1, fx0=None, fy0=None):
+    """compute_curvature_and_csf: (curvature, fx_sigma, fy_sigma); normals from
+    G, force gradient from C.  fx0 / fy0: the force arrays' previous contents
+    (the entries outside the face ranges keep them; zeros by default)."""
+    shp = (m.ny + 2, m.nx + 2)
+    curv = np.zeros(shp)
+    fx = np.zeros(shp) if fx0 is None else np.array(fx0, dtype=np.float64, copy=True)
+    fy = np.zeros(shp) if fy0 is None else np.array(fy0, dtype=np.float64, copy=True)
+    lib().orc_curvature_csf(ct.byref(m), _p(np.ascontiguousarray(G, dtype=np.float64)),
+                            _p(np.ascontiguousarray(C, dtype=np.float64)), enable, sigma, _p(curv), _p(fx), _p(fy))
+    return curv, fx, fy
 
 
 def update_properties(m: Mesh, C: np.ndarray, rho1, rho2, mu1, mu2, init=1.0):

@@ -16,6 +16,11 @@
 !   sp      sim%solve_standard_projection on a small SP case: captures u* by
 !           running the same predictor calls first, then the full SP call
 !           (src/module_simulation.f90:2424-2666)
+!   surface advect_vof with WEYMOUTH_YUE and the optional steps that follow the
+!           transport in advect_vof: SVOF smoothing, diffusion smoothing,
+!           curvature and CSF forces (src/module_interface_vof.f90:387-476,
+!           :627-789, :792-825, :1416-1461); C, C_s, C_tilde, curvature,
+!           fx_sigma, fy_sigma after every step
 !
 ! Usage: covof_ref_harness <mode> <input.bin> <output.bin>
 program covof_ref_harness
@@ -47,6 +52,8 @@ program covof_ref_harness
      call run_recon()
   case ('advect')
      call run_advect()
+  case ('surface')
+     call run_surface()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
@@ -150,5 +157,38 @@ contains
     end do
     write(uout) fields%rho, fields%rho_u, fields%rho_v, fields%mu, fields%mu_node
   end subroutine run_advect
+
+  subroutine run_surface()
+    integer :: nx, ny, bc(4), method, svof, diff, dsteps, tension, nsteps, s
+    real(dp) :: lx, ly, dt, sigma, width
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    read(uin) nx, ny, bc, method, svof, diff, dsteps, tension, nsteps
+    read(uin) lx, ly, dt, sigma, width
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%advection_method = 'WEYMOUTH_YUE'
+    cfg%reconstruction_method = method_name(method)
+    cfg%enable_two_phase = .false.
+    cfg%use_markers = .false.
+    cfg%enable_svof_smoothing = (svof /= 0)
+    cfg%svof_filter_width = width
+    cfg%enable_diffusion_smoothing = (diff /= 0)
+    cfg%diffusion_smoothing_steps = dsteps
+    cfg%enable_surface_tension = (tension /= 0)
+    cfg%sigma = sigma
+    cfg%compressible = .false.
+    call fields%allocate(mesh)
+    fields%rho = 1.0_dp
+    fields%rho_u = 1.0_dp
+    fields%rho_v = 1.0_dp
+    fields%mu = 1.0_dp
+    fields%mu_node = 1.0_dp
+    read(uin) fields%C, fields%u, fields%v
+    do s = 1, nsteps
+       call advect_vof(fields, mesh, cfg, dt)
+       write(uout) fields%C, fields%C_s, fields%C_tilde, fields%curvature, fields%fx_sigma, fields%fy_sigma
+    end do
+  end subroutine run_surface
 
 end program covof_ref_harness

@@ -6,7 +6,8 @@ the fixtures) on seeded inputs and stores inputs + reference outputs as
 compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface)
 """
 from __future__ import annotations
 
@@ -98,9 +99,62 @@ def advect(name, C, u, v, nx, ny, bc, method, dt, nsteps, two_phase=0,
                         rho_v=out[nsteps + 2], mu=out[nsteps + 3], mu_node=out[nsteps + 4], flags=FLAGS)
 
 
+def surface(name, C, u, v, nx, ny, bc, method, dt, nsteps, svof=0, diff=0, dsteps=3, tension=1,
+            sigma=0.07, width=1.0, lx=1.0, ly=1.0):
+    """advect_vof with the steps that follow the transport: SVOF / diffusion
+    smoothing, curvature and CSF forces (state after every step)."""
+    payload = (i32(nx, ny, *bc, method, svof, diff, dsteps, tension, nsteps) + f64(lx, ly, dt, sigma, width)
+               + C.astype("<f8").tobytes() + u.astype("<f8").tobytes() + v.astype("<f8").tobytes())
+    out = np.frombuffer(run("surface", payload), dtype="<f8").reshape(nsteps, 6, ny + 2, nx + 2)
+    np.savez_compressed(os.path.join(HERE, f"surface_{name}.npz"), C0=C, u=u, v=v, nx=nx, ny=ny,
+                        bc=np.array(bc), method=method, dt=dt, nsteps=nsteps, svof=svof, diff=diff,
+                        dsteps=dsteps, tension=tension, sigma=sigma, width=width, lx=lx, ly=ly,
+                        C=out[:, 0], C_s=out[:, 1], C_tilde=out[:, 2], curvature=out[:, 3],
+                        fx_sigma=out[:, 4], fy_sigma=out[:, 5], flags=FLAGS)
+
+
+def surface_cases():
+    # walls, pure CSF on a resting interface (u = v = 0: the transport leaves C as it is)
+    nx, ny = 32, 40
+    C = wl.ellipses(nx, ny, 5, 3.0, 9.0, seed=41)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    z = np.zeros_like(C)
+    surface("csf_walls", C, z, z, nx, ny, (FS, FS, FS, FS), 1, 0.0, 1, sigma=0.0728, ly=1.25)
+    # periodic, SVOF (Helmholtz) smoothing, moving interface
+    n = 32
+    Cp = wl.ellipses(n, n, 4, 3.0, 8.0, seed=42)
+    wl.fill_halo_periodic(Cp, n, n)
+    up, vp = wl.stream_velocity(n, n, 5, seed=43, periodic=True)
+    dtp = 0.3 * (1.0 / n) / max(np.abs(up).max(), np.abs(vp).max())
+    surface("svof_periodic", Cp, up, vp, n, n, (P, P, P, P), 1, dtp, 2, svof=1, width=1.5)
+    # periodic x / walls y, diffusion smoothing, PLIC
+    nx, ny = 36, 28
+    Cm = wl.ellipses(nx, ny, 4, 2.5, 7.0, seed=44)
+    Cm[:, 0] = Cm[:, nx]
+    Cm[:, nx + 1] = Cm[:, 1]
+    um, vm = wl.stream_velocity(nx, ny, 5, seed=45, periodic=False)
+    um[1:ny + 1, 1:nx + 2] += 0.3
+    dtm = 0.3 * (1.0 / nx) / max(np.abs(um).max(), np.abs(vm).max())
+    surface("diffusion_mixed", Cm, um, vm, nx, ny, (P, P, FS, FS), 0, dtm, 2, diff=1, dsteps=4, sigma=1.5)
+    # walls with pressure-Dirichlet / NEUMANN sides, both smoothings (C_tilde wins for the normals)
+    nx, ny = 30, 34
+    Cb = wl.disc(nx, ny, 0.45, 0.55, 0.3)
+    wl.fill_halo_zero_gradient(Cb, nx, ny)
+    z = np.zeros_like(Cb)
+    surface("both_walls", Cb, z, z, nx, ny, (DI, DI, NE, NE), 1, 0.0, 1, svof=1, diff=1, dsteps=2, sigma=24.5)
+    # periodic y only, surface tension off (zero forces and curvature)
+    surface("off_periodic_y", Cb, z, z, nx, ny, (FS, FS, P, P), 1, 0.0, 1, tension=0)
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface"}
+    if "surface" in groups:
+        surface_cases()
+    if not groups & {"geom", "recon", "advect"}:
+        print("golden fixtures written to", HERE)
+        return
     geom()
 
     # reconstruction: periodic disc (PLIC and ELVIRA), walls ellipses, mixed BCs

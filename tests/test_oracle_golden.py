@@ -57,3 +57,30 @@ def test_advect_vof_bitwise(orc, path):
     if int(g["two_phase"]):
         for k, name in enumerate(("rho", "rho_u", "rho_v", "mu", "mu_node")):
             assert np.array_equal(props[k], g[name]), name
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "surface_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_surface_steps_bitwise(orc, path):
+    """advect_vof with SVOF / diffusion smoothing, curvature and CSF forces:
+    C, C_s, C_tilde, curvature, fx_sigma, fy_sigma after every step."""
+    g = np.load(path, allow_pickle=False)
+    nx, ny = int(g["nx"]), int(g["ny"])
+    m = orc.mesh(nx, ny, float(g["lx"]), float(g["ly"]), tuple(g["bc"]))
+    C = g["C0"].copy()
+    shp = C.shape
+    Cs, Ct = np.ones(shp), np.ones(shp)  # the reference's initial values
+    fx, fy = np.zeros(shp), np.zeros(shp)
+    sc = 0
+    for s in range(int(g["nsteps"])):
+        C, sc = orc.wy_step(m, int(g["method"]), C, g["u"], g["v"], float(g["dt"]), sc)
+        if int(g["svof"]):
+            Cs = orc.svof_smooth(m, C, float(g["width"]))
+        if int(g["diff"]):
+            Ct = orc.smooth_diffusion(m, C, int(g["dsteps"]))
+        G = Ct if int(g["diff"]) else (Cs if int(g["svof"]) else C)
+        curv, fx, fy = orc.curvature_csf(m, G, C, float(g["sigma"]), int(g["tension"]), fx, fy)
+        for name, val in (("C", C), ("C_s", Cs), ("C_tilde", Ct), ("curvature", curv), ("fx_sigma", fx),
+                          ("fy_sigma", fy)):
+            ref = g[name][s]
+            assert np.array_equal(val, ref), f"step {s + 1} {name}: max diff {np.abs(val - ref).max()}"

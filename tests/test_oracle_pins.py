@@ -366,3 +366,114 @@ def test_projection_makes_velocity_discretely_divergence_free(orc):
 
     d0, d1 = np.abs(div(u, v)).max(), np.abs(div(un, vn)).max()
     assert d1 <= 1e-9 * d0, (d0, d1)  # a sign or index slip leaves O(d0)
+
+
+# ------------------------------------------------- smoothing, curvature, CSF
+
+def _circle_profile(n, R, w, lx=1.0):
+    """Smooth indicator tanh((R - r) / w) / 2 + 1/2 of a centred disc (halo field)."""
+    h = lx / n
+    xc = (np.arange(n + 2) - 0.5) * h
+    X, Y = np.meshgrid(xc, xc)
+    r = np.hypot(X - 0.5 * lx, Y - 0.5 * lx)
+    return 0.5 + 0.5 * np.tanh((R - r) / w)
+
+
+def test_curvature_of_a_disc_and_a_plane(orc):
+    """kappa = -div(n) with n = grad G / |grad G|: +1/r on a smooth disc
+    (positive for the liquid side; second-order: the error falls 3x as the
+    profile widens from 6 to 10 cells); 0 on a planar (linear) field away from
+    the walls."""
+    n, R = 128, 0.3
+    m = orc.mesh(n, n, 1.0, 1.0, (WALL,) * 4)
+    h = 1.0 / n
+    xc = (np.arange(n + 2) - 0.5) * h
+    X, Y = np.meshgrid(xc, xc)
+    r = np.hypot(X - 0.5, Y - 0.5)
+    ring = np.abs(r - R) < 0.5 * h
+    assert ring.sum() > 100
+    errs = []
+    for w in (0.05, 0.08):
+        curv, _, _ = orc.curvature_csf(m, _circle_profile(n, R, w), _circle_profile(n, R, w), 1.0)
+        errs.append(np.abs(curv[ring] * r[ring] - 1.0).max())
+    assert errs[0] < 0.01 and errs[1] < errs[0] / 2.5, errs
+    G = _circle_profile(n, R, 0.02)
+    curv, fx, fy = orc.curvature_csf(m, G, G, 1.0)
+    # fx = sigma * kappa_face * dC/dx: with sigma = 1 and kappa ~ 1/R it points to the centre
+    assert (fx[ring[:, :]] * np.sign(X[ring] - 0.5 + 1e-9) <= 1e-9).all()
+    # planar field: constant normal -> zero divergence
+    Gp = 0.37 * X + 0.11 * Y
+    curv, fx, fy = orc.curvature_csf(m, Gp, Gp, 2.0)
+    inner = (slice(2, n), slice(2, n))
+    assert np.abs(curv[inner]).max() < 1e-9
+    # the force is the face curvature times the face gradient (sigma scales it)
+    curv2, fx2, fy2 = orc.curvature_csf(m, G, G, 3.0)
+    curv1, fx1, fy1 = orc.curvature_csf(m, G, G, 1.0)
+    assert np.array_equal(curv1, curv2)
+    assert np.allclose(fx2, 3.0 * fx1, rtol=1e-15, atol=0) and np.allclose(fy2, 3.0 * fy1, rtol=1e-15, atol=0)
+
+
+def test_curvature_mirror_symmetry_and_switch_off(orc):
+    """Mirroring the field in x mirrors the curvature and flips fx (a wrong
+    index or sign in one direction breaks it); sigma <= 0 zeroes all three."""
+    n = 48
+    m = orc.mesh(n, n, 1.0, 1.0, (PER,) * 4)
+    rng = np.random.default_rng(4)
+    G = _circle_profile(n, 0.2, 0.03) + 0.05 * rng.random((n + 2, n + 2))
+    G[:, 0], G[:, n + 1] = G[:, n], G[:, 1]
+    G[0, :], G[n + 1, :] = G[n, :], G[1, :]
+    curv, fx, fy = orc.curvature_csf(m, G, G, 1.0)
+    Gm = G[:, ::-1].copy()  # i -> n + 1 - i
+    cm, fxm, fym = orc.curvature_csf(m, Gm, Gm, 1.0)
+    assert np.allclose(cm[1:n + 1, 1:n + 1], curv[1:n + 1, 1:n + 1][:, ::-1], rtol=0, atol=1e-9)
+    # x-faces: face i of the mirror is face n + 2 - i of the original
+    assert np.allclose(fxm[1:n + 1, 1:n + 2], -fx[1:n + 1, 1:n + 2][:, ::-1], rtol=0, atol=1e-7)
+    assert np.allclose(fym[1:n + 2, 1:n + 1], fy[1:n + 2, 1:n + 1][:, ::-1], rtol=0, atol=1e-7)
+    z = orc.curvature_csf(m, G, G, 0.0, fx0=np.ones_like(G), fy0=np.ones_like(G))
+    assert all(np.array_equal(a, np.zeros_like(G)) for a in z)
+
+
+def test_diffusion_smoothing_constant_and_fourier_mode(orc):
+    """Explicit diffusion keeps a constant bitwise and damps a periodic
+    Fourier mode by (1 - dtau * 4/h^2 * sin^2(pi k / n)) per step."""
+    n, steps = 64, 5
+    m = orc.mesh(n, n, 1.0, 1.0, (PER,) * 4)
+    Cc = np.full((n + 2, n + 2), 0.625)
+    assert np.array_equal(orc.smooth_diffusion(m, Cc, steps), Cc)
+    h = 1.0 / n
+    k = 3
+    xc = (np.arange(n + 2) - 0.5) * h
+    mode = np.cos(2 * np.pi * k * xc)[None, :] * np.ones((n + 2, 1))
+    out = orc.smooth_diffusion(m, mode, steps)
+    dtau = 0.2 * h * h
+    g = 1.0 - dtau * 4.0 / h ** 2 * math.sin(math.pi * k / n) ** 2
+    assert np.abs(out - g ** steps * mode).max() < 1e-13
+    # walls: zero-gradient halos, total mass conserved (the flux through a wall is zero)
+    mw = orc.mesh(n, n, 1.0, 1.0, (WALL,) * 4)
+    C = _circle_profile(n, 0.2, 0.01)
+    out = orc.smooth_diffusion(mw, C, steps)
+    assert abs(out[1:n + 1, 1:n + 1].sum() - C[1:n + 1, 1:n + 1].sum()) < 1e-10
+    assert np.array_equal(out[0, 1:n + 1], out[1, 1:n + 1]) and np.array_equal(out[1:n + 1, 0], out[1:n + 1, 1])
+
+
+def test_svof_smoothing_solves_the_helmholtz_filter(orc):
+    """25 symmetric Gauss-Seidel sweeps of (I - D Lap) C_s = C: on a periodic
+    Fourier mode C_s approaches C / (1 + D * 4/h^2 * sin^2(pi k / n)); a
+    constant is kept; the result is clamped to [0, 1]."""
+    n, width = 64, 1.0
+    m = orc.mesh(n, n, 1.0, 1.0, (PER,) * 4)
+    h = 1.0 / n
+    k = 2
+    xc = (np.arange(n + 2) - 0.5) * h
+    mode = 0.5 + 0.25 * np.cos(2 * np.pi * k * xc)[None, :] * np.ones((n + 2, 1))
+    out = orc.svof_smooth(m, mode, width)
+    lam = 1.0 + width * 4.0 * math.sin(math.pi * k / n) ** 2  # D / h^2 = width
+    ref = 0.5 + 0.25 * np.cos(2 * np.pi * k * xc)[None, :] / lam * np.ones((n + 2, 1))
+    assert np.abs(out - ref).max() < 1e-6
+    Cc = np.full((n + 2, n + 2), 0.4)
+    assert np.abs(orc.svof_smooth(m, Cc, width) - 0.4).max() < 1e-15
+    mw = orc.mesh(n, n, 1.0, 1.0, (WALL,) * 4)
+    C = (_circle_profile(n, 0.25, 0.001) > 0.5).astype(float)
+    out = orc.svof_smooth(mw, C, 2.0)
+    assert out.min() >= 0.0 and out.max() <= 1.0
+    assert 0.0 < out[n // 2, n // 2 + int(0.25 * n)] < 1.0  # the jump is smoothed
