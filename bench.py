@@ -200,6 +200,27 @@ def main():
         "volume_drift_rel": abs(v1 - v0) / v0,
     }
     log(f"VOF: {t_vof * 1e3:.3f} ms/step, {vof['cell_updates_per_s']:.3e} cell-updates/s")
+    # the tail of advect_vof when surface tension is on: 3 diffusion-smoothing
+    # steps (16 B/cell each + the halo pass) and curvature + CSF (3 passes:
+    # normals 24 B/cell, curvature 24 B/cell, forces 32 B/cell)
+    Ct = vctx.smooth_diffusion(Cg, 3)
+    out_csf = vctx.compute_curvature_and_csf(Cg, 0.07, G=Ct)
+    reps = 20
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        vctx.smooth_diffusion(Cg, 3, C_tilde=Ct)
+    torch.cuda.synchronize()
+    t_dif = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        vctx.compute_curvature_and_csf(Cg, 0.07, G=Ct, out=out_csf)
+    torch.cuda.synchronize()
+    t_csf = (time.perf_counter() - t) / reps
+    vof["diffusion_smoothing_3_steps_us"] = t_dif * 1e6
+    vof["curvature_csf_us"] = t_csf * 1e6
+    vof["curvature_csf_GBps"] = 80.0 * vn * vn / t_csf / 1e9
+    log(f"surface steps: diffusion x3 {t_dif * 1e6:.1f} us, curvature + CSF {t_csf * 1e6:.1f} us")
 
     # ----------------------------------------------------------- CPU baseline
     cpu = None

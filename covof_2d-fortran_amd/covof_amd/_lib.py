@@ -43,6 +43,12 @@ SIGNATURES = {
     "covof_vof_advect_d": (_I, [_P, _P, _P, _P, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_P] * 5),
     "covof_vof_advect": (_I, [_P, _DP, _DP, _DP, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_DP] * 5),
     "covof_update_properties_d": (_I, [_P, _P, _D, _D, _D, _D] + [_P] * 5),
+    "covof_curvature_csf_d": (_I, [_P, _P, _P, _I, _D, _P, _P, _P]),
+    "covof_curvature_csf": (_I, [_P, _P, _P, _I, _D, _P, _P, _P]),
+    "covof_smooth_diffusion_d": (_I, [_P, _P, _I, _P]),
+    "covof_smooth_diffusion": (_I, [_P, _P, _I, _P]),
+    "covof_svof_smooth_d": (_I, [_P, _P, _D, _P]),
+    "covof_svof_smooth": (_I, [_P, _P, _D, _P]),
     "covof_poisson_assemble_d": (_I, [_P, _P, _P, _P, _P, _D, _P]),
     "covof_poisson_matvec_d": (_I, [_P, _P, _P]),
     "covof_poisson_precond_d": (_I, [_P, _I, _P, _P]),

@@ -162,6 +162,42 @@ class Context:
                 "covof_update_properties_d")
         return props
 
+    # ------------------------------------------------ after the transport
+    def compute_curvature_and_csf(self, C, sigma, G=None, enable=True, out=None):
+        """compute_curvature_and_csf (src/module_interface_vof.f90:627-789):
+        returns {"curvature", "fx_sigma", "fy_sigma"} (halo fields); normals
+        from G (C_tilde / C_s when a smoothing is on; default C).  ``out``
+        supplies the three arrays (the force arrays keep their entries outside
+        the face ranges); by default they start at zero."""
+        self._field(C, "C")
+        if G is not None:
+            self._field(G, "G")
+        if out is None:
+            out = {k: self.new_field(0.0) for k in ("curvature", "fx_sigma", "fy_sigma")}
+        arrs = [self._field(out[k], k) for k in ("curvature", "fx_sigma", "fy_sigma")]
+        self._stream()
+        L.check(self._lib.covof_curvature_csf_d(self._h, _ptr(G), _ptr(C), 1 if enable else 0, float(sigma),
+                                                *[_ptr(a) for a in arrs]), "covof_curvature_csf_d")
+        return out
+
+    def smooth_diffusion(self, C, steps=3, C_tilde=None):
+        """vof_smooth_diffusion (src/module_interface_vof.f90:1416-1461): C_tilde."""
+        self._field(C, "C")
+        C_tilde = self.new_field(0.0) if C_tilde is None else self._field(C_tilde, "C_tilde")
+        self._stream()
+        L.check(self._lib.covof_smooth_diffusion_d(self._h, _ptr(C), int(steps), _ptr(C_tilde)),
+                "covof_smooth_diffusion_d")
+        return C_tilde
+
+    def smooth_svof_helmholtz(self, C, filter_width=1.0, C_s=None):
+        """smooth_svof_helmholtz (src/module_interface_vof.f90:387-476): C_s."""
+        self._field(C, "C")
+        C_s = self.new_field(0.0) if C_s is None else self._field(C_s, "C_s")
+        self._stream()
+        L.check(self._lib.covof_svof_smooth_d(self._h, _ptr(C), float(filter_width), _ptr(C_s)),
+                "covof_svof_smooth_d")
+        return C_s
+
     # ------------------------------------------------------------ Poisson
     def poisson_assemble(self, u, v, rho_u, rho_v, dt, b=None):
         """SP Poisson assembly (src/module_simulation.f90:2477-2533); returns b = -div(u*)."""

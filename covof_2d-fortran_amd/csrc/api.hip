@@ -121,6 +121,7 @@ extern "C" int covof_ctx_destroy(covof_ctx *h) {
   Ctx &c = h->c;
   if (c.stream) hipStreamSynchronize(c.stream);
   covof::vof_free(c);
+  covof::surface_free(c);
   covof::poisson_free(c);
   if (c.prof.created)
     for (int k = 0; k < 2 * 1024; ++k) hipEventDestroy(c.prof.ev[k]);
@@ -239,6 +240,82 @@ extern "C" int covof_update_properties_d(covof_ctx *h, double *C, double rho1, d
   if (!h || !C || !rho || !rho_u || !rho_v || !mu || !mu_node) return fail(COVOF_BAD_ARGUMENT, "null argument");
   int st = covof::vof_corner_fix(h->c, C);
   if (!st) st = covof::vof_properties(h->c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node);
+  return hip_status(st);
+}
+
+static bool same_halo_array(const double *a, const double *b) { return a && b && a == b; }
+
+extern "C" int covof_curvature_csf_d(covof_ctx *h, const double *G, const double *C, int enable, double sigma,
+                                     double *curvature, double *fx_sigma, double *fy_sigma) {
+  if (!h || !C || !curvature || !fx_sigma || !fy_sigma) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (!G) G = C;
+  const double *in[2] = {G, C};
+  const double *out[3] = {curvature, fx_sigma, fy_sigma};
+  for (const double *o : out)
+    for (const double *q : in)
+      if (same_halo_array(o, q)) return fail(COVOF_BAD_ARGUMENT, "covof_curvature_csf: an output aliases an input");
+  if (curvature == fx_sigma || curvature == fy_sigma || fx_sigma == fy_sigma)
+    return fail(COVOF_BAD_ARGUMENT, "covof_curvature_csf: outputs must be distinct");
+  return hip_status(covof::curvature_csf(h->c, G, C, enable, sigma, curvature, fx_sigma, fy_sigma));
+}
+
+extern "C" int covof_smooth_diffusion_d(covof_ctx *h, const double *C, int steps, double *C_tilde) {
+  if (!h || !C || !C_tilde) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (C == C_tilde) return fail(COVOF_BAD_ARGUMENT, "covof_smooth_diffusion: C_tilde must not alias C");
+  return hip_status(covof::smooth_diffusion(h->c, C, steps, C_tilde));
+}
+
+extern "C" int covof_svof_smooth_d(covof_ctx *h, const double *C, double filter_width, double *C_s) {
+  if (!h || !C || !C_s) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (C == C_s) return fail(COVOF_BAD_ARGUMENT, "covof_svof_smooth: C_s must not alias C");
+  return hip_status(covof::svof_smooth(h->c, C, filter_width, C_s));
+}
+
+// Host-array forms: copy in, run, copy out (the outputs' previous contents
+// travel too: the force arrays keep their entries outside the face ranges).
+extern "C" int covof_curvature_csf(covof_ctx *h, const double *G, const double *C, int enable, double sigma,
+                                   double *curvature, double *fx_sigma, double *fy_sigma) {
+  if (!h || !C || !curvature || !fx_sigma || !fy_sigma) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d[5];
+  for (int k = 0; k < 5; ++k)
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+  int st = h2d(h, d[0], C);
+  if (!st && G && G != C) st = h2d(h, d[1], G);
+  if (!st) st = h2d(h, d[3], fx_sigma);
+  if (!st) st = h2d(h, d[4], fy_sigma);
+  if (st) return hip_status(st);
+  st = covof_curvature_csf_d(h, (G && G != C) ? d[1] : d[0], d[0], enable, sigma, d[2], d[3], d[4]);
+  if (st) return st;
+  st = d2h(h, curvature, d[2]);
+  if (!st) st = d2h(h, fx_sigma, d[3]);
+  if (!st) st = d2h(h, fy_sigma, d[4]);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return hip_status(st);
+}
+
+extern "C" int covof_smooth_diffusion(covof_ctx *h, const double *C, int steps, double *C_tilde) {
+  if (!h || !C || !C_tilde) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d0 = mirror(h, 0), *d1 = mirror(h, 1);
+  if (!d0 || !d1) return hip_status(COVOF_HIP_ERROR);
+  int st = h2d(h, d0, C);
+  if (st) return hip_status(st);
+  st = covof_smooth_diffusion_d(h, d0, steps, d1);
+  if (st) return st;
+  st = d2h(h, C_tilde, d1);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return hip_status(st);
+}
+
+extern "C" int covof_svof_smooth(covof_ctx *h, const double *C, double filter_width, double *C_s) {
+  if (!h || !C || !C_s) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  double *d0 = mirror(h, 0), *d1 = mirror(h, 1);
+  if (!d0 || !d1) return hip_status(COVOF_HIP_ERROR);
+  int st = h2d(h, d0, C);
+  if (st) return hip_status(st);
+  st = covof_svof_smooth_d(h, d0, filter_width, d1);
+  if (st) return st;
+  st = d2h(h, C_s, d1);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
   return hip_status(st);
 }
 

@@ -30,6 +30,12 @@ struct Vof {
   int *offs = nullptr;                  // exclusive prefix of count (ntiles + 1 entries)
 };
 
+// Work arrays of the surface steps (surface.hip), made on first use: two
+// halo-size FP64 arrays.
+struct Surface {
+  double *work = nullptr;
+};
+
 // One instantiated capture of a solver cycle (see poisson.hip).
 struct PGraph {
   hipGraphExec_t exec = nullptr;
@@ -91,6 +97,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int own_stream = 0;
   Vof vof;
+  Surface surf;
   Poisson ps;
   Prof prof;
 };
@@ -136,6 +143,14 @@ void mg_free(Ctx &c);
 
 int vof_alloc(Ctx &c);
 void vof_free(Ctx &c);
+// surface.hip: curvature + CSF (G: the field the normals come from), the two
+// smoothings of advect_vof; device pointers, halo layout, no aliasing of an
+// output with an input
+int curvature_csf(Ctx &c, const double *G, const double *C, int enable, double sigma, double *curv,
+                  double *fx, double *fy);
+int smooth_diffusion(Ctx &c, const double *C, int steps, double *Ct);
+int svof_smooth(Ctx &c, const double *C, double width, double *Cs);
+void surface_free(Ctx &c);
 int poisson_alloc(Ctx &c);
 void poisson_free(Ctx &c);
 

@@ -5,9 +5,13 @@
 !>   sp   : the Poisson block and the velocity correction of
 !>          sim_solve_standard_projection (src/module_simulation.f90:2477-2654)
 !>          with the reference's CG + SSOR, tol 1e-7, 1000 iterations, on a
-!>          walled box holding a heavy disc (rho 1000 in 1).
+!>          walled box holding a heavy disc (rho 1000 in 1);
+!>   csf  : the tail of advect_vof (src/module_interface_vof.f90:810-824) on a
+!>          walled box holding a disc: smooth_svof_helmholtz (filter width
+!>          1.5), vof_smooth_diffusion (NSTEPS steps) and
+!>          compute_curvature_and_csf with the normals from C_tilde, sigma 0.07.
 !>
-!> usage: covof_driver vof|sp N NSTEPS OUTFILE
+!> usage: covof_driver vof|sp|csf N NSTEPS OUTFILE
 !>
 !> OUTFILE (unformatted stream, native endianness):
 !>   int32 mode (1 vof, 2 sp), nx, ny, nsteps, counter (vof: final
@@ -15,6 +19,7 @@
 !>   float64 dt, relres (sp; 0 for vof);
 !>   vof: C0, u, v, C            (each (0:nx+1, 0:ny+1))
 !>   sp : u*, v*, rho_u, rho_v, p, u, v (each (0:nx+1, 0:ny+1))
+!>   csf: C, C_s, C_tilde, curvature, fx_sigma, fy_sigma
 program covof_driver
   use iso_c_binding
   use covof_hip
@@ -28,10 +33,11 @@ program covof_driver
   real(dp) :: dx, dy, dt, relres, umax
   real(dp), allocatable :: C(:, :), C0(:, :), u(:, :), v(:, :), psi(:, :)
   real(dp), allocatable :: rho(:, :), rho_u(:, :), rho_v(:, :), p(:, :), us(:, :), vs(:, :)
+  real(dp), allocatable :: Cs(:, :), Ct(:, :), curv(:, :), fx(:, :), fy(:, :)
   real(dp), parameter :: pi = 3.14159265358979323846_dp
 
   if (command_argument_count() < 4) then
-     print '(a)', "usage: covof_driver vof|sp N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf N NSTEPS OUTFILE"
      stop 2
   end if
   call get_command_argument(1, mode)
@@ -124,8 +130,31 @@ program covof_driver
      write (unit) us, vs, rho_u, rho_v, p, u, v
      close (unit)
      print '(a,i0,a,es10.3)', "sp poisson iterations ", counter, " relres ", relres
+  case ("csf")
+     bc = COVOF_BC_FREESLIP
+     st = covof_ctx_create(int(n, c_int), int(n, c_int), 1.0_dp, 1.0_dp, bc, ctx)
+     call check(st, "covof_ctx_create")
+     allocate(Cs(0:n + 1, 0:n + 1), Ct(0:n + 1, 0:n + 1), curv(0:n + 1, 0:n + 1))
+     allocate(fx(0:n + 1, 0:n + 1), fy(0:n + 1, 0:n + 1))
+     call disc(C, n, 0.45_dp, 0.55_dp, 0.27_dp)
+     C(0, :) = C(1, :); C(n + 1, :) = C(n, :); C(:, 0) = C(:, 1); C(:, n + 1) = C(:, n)
+     fx = 0.0_dp
+     fy = 0.0_dp
+     st = covof_svof_smooth(ctx, C, 1.5_dp, Cs)
+     call check(st, "covof_svof_smooth")
+     st = covof_smooth_diffusion(ctx, C, int(nsteps, c_int), Ct)
+     call check(st, "covof_smooth_diffusion")
+     st = covof_curvature_csf(ctx, Ct, C, 1_c_int, 0.07_dp, curv, fx, fy)
+     call check(st, "covof_curvature_csf")
+     dt = 0.0_dp
+     open (newunit=unit, file=trim(outfile), access="stream", form="unformatted", status="replace")
+     write (unit) 3_c_int, int(n, c_int), int(n, c_int), int(nsteps, c_int), int(counter, c_int), int(st, c_int)
+     write (unit) dt, relres
+     write (unit) C, Cs, Ct, curv, fx, fy
+     close (unit)
+     print '(a,es23.16)', "csf max |curvature| ", maxval(abs(curv))
   case default
-     print '(a)', "usage: covof_driver vof|sp N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf N NSTEPS OUTFILE"
      stop 2
   end select
   st = covof_ctx_destroy(ctx)

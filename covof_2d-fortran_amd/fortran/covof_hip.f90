@@ -13,6 +13,11 @@
 !>     -> covof_sp_poisson(ctx, u, v, rho_u, rho_v, p, dt, ...)
 !>   the velocity correction                  src/module_simulation.f90:2590-2654
 !>     -> covof_sp_correct(ctx, p, rho_u, rho_v, dt, u, v)
+!>   the tail of advect_vof                   src/module_interface_vof.f90:810-824
+!>     smooth_svof_helmholtz   -> covof_svof_smooth(ctx, C, width, C_s)
+!>     vof_smooth_diffusion    -> covof_smooth_diffusion(ctx, C, steps, C_tilde)
+!>     compute_curvature_and_csf -> covof_curvature_csf(ctx, G, C, enable, sigma,
+!>                                     curvature, fx_sigma, fy_sigma)
 !>
 !> Arrays are the reference's own: real(dp) F(0:nx+1, 0:ny+1), column-major,
 !> i fastest (src/module_fields.f90:129-169), passed by reference.  Selector
@@ -90,6 +95,32 @@ module covof_hip
        real(c_double), intent(inout) :: u(*), v(*)
      end function covof_sp_correct
 
+     integer(c_int) function covof_curvature_csf(ctx, G, C, enable, sigma, curvature, fx_sigma, fy_sigma) &
+          bind(C, name="covof_curvature_csf")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: G(*), C(*)
+       integer(c_int), value :: enable
+       real(c_double), value :: sigma
+       real(c_double), intent(inout) :: curvature(*), fx_sigma(*), fy_sigma(*)
+     end function covof_curvature_csf
+
+     integer(c_int) function covof_smooth_diffusion(ctx, C, steps, C_tilde) bind(C, name="covof_smooth_diffusion")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: C(*)
+       integer(c_int), value :: steps
+       real(c_double), intent(inout) :: C_tilde(*)
+     end function covof_smooth_diffusion
+
+     integer(c_int) function covof_svof_smooth(ctx, C, filter_width, C_s) bind(C, name="covof_svof_smooth")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: C(*)
+       real(c_double), value :: filter_width
+       real(c_double), intent(inout) :: C_s(*)
+     end function covof_svof_smooth
+
      ! device-pointer entry points (arrays already resident in HBM)
      integer(c_int) function covof_vof_advect_d(ctx, C, u, v, dt, adv, rec, step_count, nsteps, two_phase, &
           rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node) bind(C, name="covof_vof_advect_d")
@@ -110,6 +141,29 @@ module covof_hip
        character(kind=c_char), intent(in) :: method(*)
        type(c_ptr), value :: C, seg_nx, seg_ny, seg_alpha
      end function covof_reconstruct_interface_d
+
+     integer(c_int) function covof_curvature_csf_d(ctx, G, C, enable, sigma, curvature, fx_sigma, fy_sigma) &
+          bind(C, name="covof_curvature_csf_d")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, G, C
+       integer(c_int), value :: enable
+       real(c_double), value :: sigma
+       type(c_ptr), value :: curvature, fx_sigma, fy_sigma
+     end function covof_curvature_csf_d
+
+     integer(c_int) function covof_smooth_diffusion_d(ctx, C, steps, C_tilde) bind(C, name="covof_smooth_diffusion_d")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, C
+       integer(c_int), value :: steps
+       type(c_ptr), value :: C_tilde
+     end function covof_smooth_diffusion_d
+
+     integer(c_int) function covof_svof_smooth_d(ctx, C, filter_width, C_s) bind(C, name="covof_svof_smooth_d")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, C
+       real(c_double), value :: filter_width
+       type(c_ptr), value :: C_s
+     end function covof_svof_smooth_d
 
      integer(c_int) function covof_sp_poisson_d(ctx, u, v, rho_u, rho_v, p, dt, lin_solver, precond, tol, maxit, &
           precision, iters, relres) bind(C, name="covof_sp_poisson_d")

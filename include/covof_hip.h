@@ -118,6 +118,32 @@ int covof_update_properties_d(covof_ctx *ctx, double *C, double rho1, double rho
                               double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
                               double *mu_node);
 
+/* --- the steps advect_vof runs after the transport (single rank, no markers);
+ * halo fields (0:nx+1, 0:ny+1); an output never aliases an input. */
+
+/* compute_curvature_and_csf (src/module_interface_vof.f90:627-789): unit
+ * normals of grad G (G = C_tilde with diffusion smoothing, C_s with SVOF
+ * smoothing, else C; NULL means C), curvature = -div(n) with its exchanged
+ * halo, and the CSF forces fx_sigma on the x-faces (1..nx+1, 1..ny) and
+ * fy_sigma on the y-faces (1..nx, 1..ny+1) from the face curvature and the
+ * face gradient of C; their other entries are left as they are.  enable == 0
+ * (cfg%enable_surface_tension false) or sigma <= 0 zeroes all three arrays. */
+int covof_curvature_csf_d(covof_ctx *ctx, const double *G, const double *C, int enable, double sigma,
+                          double *curvature, double *fx_sigma, double *fy_sigma);
+int covof_curvature_csf(covof_ctx *ctx, const double *G, const double *C, int enable, double sigma,
+                        double *curvature, double *fx_sigma, double *fy_sigma);
+/* vof_smooth_diffusion (:1416-1461): steps explicit diffusion steps of C
+ * (steps < 1 means 3, cfg%diffusion_smoothing_steps) into C_tilde, with the
+ * reference's halo fill (exchange, zero-gradient copies on physical sides). */
+int covof_smooth_diffusion_d(covof_ctx *ctx, const double *C, int steps, double *C_tilde);
+int covof_smooth_diffusion(covof_ctx *ctx, const double *C, int steps, double *C_tilde);
+/* smooth_svof_helmholtz (:387-476): 25 symmetric Gauss-Seidel iterations of
+ * (I - D Lap) C_s = C, D = filter_width * dx * dy (filter_width <= 0 means
+ * 1, cfg%svof_filter_width), then the clamp to [0, 1] and the halo.  The
+ * recurrence is sequential: it runs in one workgroup (the parity form). */
+int covof_svof_smooth_d(covof_ctx *ctx, const double *C, double filter_width, double *C_s);
+int covof_svof_smooth(covof_ctx *ctx, const double *C, double filter_width, double *C_s);
+
 /* --- pressure Poisson of the standard projection (src/module_simulation.f90) */
 
 /* Assembly of the variable-density 5-point operator and of b = -div(u*)

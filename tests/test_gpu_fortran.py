@@ -76,3 +76,24 @@ def test_fortran_sp_poisson_matches_python_and_oracle(tmp_path, cv, orc):
     assert abs(iters - it) <= 1, (iters, it)
     scale = np.abs(po[1:n + 1, 1:n + 1]).max()
     assert np.abs(p[1:n + 1, 1:n + 1] - po[1:n + 1, 1:n + 1]).max() / scale < 1e-6
+
+
+@pytest.mark.gpu
+def test_fortran_curvature_csf_matches_oracle(tmp_path, orc):
+    """The tail of advect_vof from Fortran host arrays: SVOF and diffusion
+    smoothing, curvature and CSF forces, bit-exact against the oracle."""
+    n, steps = 64, 4
+    out = tmp_path / "csf.bin"
+    r = subprocess.run([DRIVER, "csf", str(n), str(steps), str(out)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    hdr, _, _, (C, Cs, Ct, curv, fx, fy) = _read(out, 6)
+    assert hdr[0] == 3 and hdr[5] == 0
+    m = orc.mesh(n, n, bc=(4, 4, 4, 4))
+    assert np.array_equal(Cs, orc.svof_smooth(m, C, 1.5))
+    Cto = orc.smooth_diffusion(m, C, steps)
+    assert np.array_equal(Ct, Cto)
+    co, fxo, fyo = orc.curvature_csf(m, Cto, C, 0.07)
+    assert np.array_equal(curv, co) and np.array_equal(fx, fxo) and np.array_equal(fy, fyo)
+    assert np.abs(curv).max() > 0.0
