@@ -160,6 +160,110 @@ CV_HD double compute_alpha(double xmin, double xmax, double ymin, double ymax, d
   return 0.5 * (lo + hi);
 }
 
+// Area of {a X + b Y <= g} inside [0, W] x [0, H] for a, b >= 0 in closed
+// form: with the roles of x and y arranged so that a W <= b H, a triangle
+// (g <= a W), a trapezoid across the width (g <= b H) or the box less a
+// triangle.  The constants depend on the box and the normal only; evaluate()
+// is branch-free (all three forms, then selects), so the lanes of a wave that
+// sit in different cases do not serialise.  Not bit-exact: used with an
+// error margin only.
+struct HalfPlaneArea {
+  double W, H, aW, bH, k2, ib, WH;
+  CV_HD void init(double W_, double H_, double a, double b) {
+    if (a * W_ > b * H_) {
+      const double t = a;
+      a = b;
+      b = t;
+      const double u = W_;
+      W_ = H_;
+      H_ = u;
+    }
+    W = W_;
+    H = H_;
+    aW = a * W;
+    bH = b * H;
+    k2 = 0.5 / (a * b);  // inf when a == 0: its form is then never selected
+    ib = 1.0 / b;
+    WH = W * H;
+  }
+  CV_HD double evaluate(double g) const {
+    const double tri = g * g * k2;
+    const double trap = W * (g - 0.5 * aW) * ib;
+    const double r = aW + bH - g;
+    const double top = WH - r * r * k2;
+    double v = g <= bH ? trap : top;
+    v = g <= aW ? tri : v;
+    v = g >= aW + bH ? WH : v;
+    return g <= 0.0 ? 0.0 : v;
+  }
+};
+
+// compute_alpha with the same result bit for bit and fewer clips.  The
+// bisection's step k only needs the sign of a(mid) - target, where a is
+// cut_area_w's floating-point area.  That area differs from the exact area A
+// of the line's cut by at most Ef: the shoelace terms are cross products of
+// vertices in absolute coordinates (|x| <= X, |y| <= Y, |p| <= Rp), each
+// product rounded (2u XY per term, u = 2^-53), the terms and partial sums
+// bounded by Rp * diag; the corner values n.p + mid carry at most ~4u S
+// (S = |nx| X + |ny| Y) and every computed cut vertex lies within ~3 of
+// those of the exact line, so the cut polygon differs from the exact one
+// within a band of that width along a chord of at most diag.  A is evaluated
+// in closed form from the box corner of least n.p (HalfPlaneArea; its
+// offset vmin carries ~2u S, moving the area by at most 2u S diag / |n|).
+// Steps whose closed-form area is farther than E (four times the sum of
+// those bounds) from the target have the same outcome as the clip and skip
+// it; the steps near the root -- about the last 20 of 45 at cell sizes of
+// 1e-3 -- evaluate cut_area_w exactly as compute_alpha does.
+CV_HD double compute_alpha_fast(double xmin, double xmax, double ymin, double ymax, double nx,
+                                double ny, double c) {
+  const double total = (xmax - xmin) * (ymax - ymin);
+  if (c <= 1.0e-7) return 1.0e9;
+  if (c >= 1.0 - 1.0e-7) return -1.0e9;
+  const double cc = c < 1.0 ? c : 1.0;
+  const double target = (0.0 > cc ? 0.0 : cc) * total;
+  const double v1 = nx * xmin + ny * ymin;
+  const double v2 = nx * xmax + ny * ymin;
+  const double v3 = nx * xmax + ny * ymax;
+  const double v4 = nx * xmin + ny * ymax;
+  const double m34 = v3 < v4 ? v3 : v4, M34 = v3 > v4 ? v3 : v4;
+  const double m234 = v2 < m34 ? v2 : m34, M234 = v2 > M34 ? v2 : M34;
+  const double vmin = v1 < m234 ? v1 : m234, vmax = v1 > M234 ? v1 : M234;
+  double lo = -vmax - 1.0e-12;
+  double hi = -vmin + 1.0e-12;
+  const bool degenerate = (xmax - xmin) <= 1.0e-14 || (ymax - ymin) <= 1.0e-14;
+  // certification bounds
+  const double W = xmax - xmin, Hh = ymax - ymin;
+  const double a = fabs(nx), b = fabs(ny);
+  const double nn = sqrt(a * a + b * b);
+  const double X = fabs(xmin) > fabs(xmax) ? fabs(xmin) : fabs(xmax);
+  const double Y = fabs(ymin) > fabs(ymax) ? fabs(ymin) : fabs(ymax);
+  const double Rp = sqrt(X * X + Y * Y), diag = sqrt(W * W + Hh * Hh), S = a * X + b * Y;
+  const double amin = a < b ? a : b;
+  const bool cert = !degenerate && nn > 0.5 && nn < 2.0 && X < 1.0e100 && Y < 1.0e100 &&
+                    (amin == 0.0 || amin > 1.0e-150);
+  const double E = 4.0 * 0x1p-53 * (8.0 * X * Y + 20.0 * Rp * diag + 30.0 * S * diag / nn + 8.0 * W * Hh);
+  HalfPlaneArea hp;
+  hp.init(W, Hh, a, b);
+  for (int it = 0; it < 45; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    int below = -1;  // 1: a(mid) < target, 0: a(mid) >= target
+    if (cert) {
+      const double ah = hp.evaluate(-(mid + vmin));
+      if (ah < target - E) below = 1;
+      else if (ah > target + E) below = 0;
+    }
+    if (below < 0) {
+      const double ar = degenerate ? 0.0 : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
+      below = ar < target ? 1 : 0;
+    }
+    if (below)
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
 // Full / empty short-circuit thresholds of reconstruct_plic / _elvira (:298-310).
 CV_HD bool is_mixed(double c) { return !(c <= 1.0e-6) && !(c >= 1.0 - 1.0e-6); }
 

@@ -79,7 +79,7 @@ __device__ void plic_mixed(const double *C, const DevMesh &m, int i, int j, doub
   }
   onx = pnx;
   ony = pny;
-  oal = compute_alpha(face_coord(i, m.dx), face_coord(i + 1, m.dx), face_coord(j, m.dy),
+  oal = compute_alpha_fast(face_coord(i, m.dx), face_coord(i + 1, m.dx), face_coord(j, m.dy),
                       face_coord(j + 1, m.dy), pnx, pny, c);
 }
 
@@ -124,7 +124,7 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
   uy = cny / cn;
   const double xmin = face_coord(i, dx), xmax = face_coord(i + 1, dx);
   const double ymin = face_coord(j, dy), ymax = face_coord(j + 1, dy);
-  al = compute_alpha(xmin, xmax, ymin, ymax, ux, uy, c);
+  al = compute_alpha_fast(xmin, xmax, ymin, ymax, ux, uy, c);
   double e = 0.0;
   for (int dj = -1; dj <= 1; ++dj)
     for (int di = -1; di <= 1; ++di) {
@@ -200,7 +200,7 @@ __device__ void ghost_segment(const DevMesh &m, int i, int j, double c, double g
   const long k = H(m, i, j);
   snx[k] = sx;
   sny[k] = sy;
-  sal[k] = compute_alpha(face_coord(i, m.dx), face_coord(i + 1, m.dx), face_coord(j, m.dy),
+  sal[k] = compute_alpha_fast(face_coord(i, m.dx), face_coord(i + 1, m.dx), face_coord(j, m.dy),
                          face_coord(j + 1, m.dy), sx, sy, c);
 }
 
@@ -676,7 +676,7 @@ __global__ void k_geom_batch(int n, const double *xmin, const double *xmax, cons
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   area[t] = cut_box_area(xmin[t], xmax[t], ymin[t], ymax[t], gnx[t], gny[t], alpha[t]);
-  aout[t] = compute_alpha(xmin[t], xmax[t], ymin[t], ymax[t], gnx[t], gny[t], c[t]);
+  aout[t] = compute_alpha_fast(xmin[t], xmax[t], ymin[t], ymax[t], gnx[t], gny[t], c[t]);
 }
 
 // Ghost-cell segments of the halo grid for the full reconstruct_interface
