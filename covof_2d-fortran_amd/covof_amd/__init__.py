@@ -1,18 +1,21 @@
 """MI355X (gfx950) hot path of COVOF 2D: Weymouth-Yue VOF advection with
-PLIC/ELVIRA reconstruction and the variable-density pressure-Poisson solve of
-the standard projection, as HIP kernels behind the C-ABI of
-``include/covof_hip.h``.
+PLIC/ELVIRA reconstruction, the SP velocity predictor (momentum block) and the
+variable-density pressure-Poisson solve of the standard projection, as HIP
+kernels behind the C-ABI of ``include/covof_hip.h``.
 
 This Python layer mirrors the reference's operator interface for the path
 (``advect_vof``, ``reconstruct_interface``, ``update_physical_properties``,
-``cut_box_area``/``compute_alpha``, the SP Poisson block, the velocity
-correction) over torch CUDA tensors in the reference's field layout: a field
+``cut_box_area``/``compute_alpha``, the momentum block's
+``assemble_preconditioner_blocks`` / ``assemble_rhs`` / ``apply_system_matrix``
+/ ``solve_linear_system``, the SP Poisson block, the velocity correction) over torch CUDA tensors in the reference's field layout: a field
 ``F(0:nx+1, 0:ny+1)`` is a float64 tensor of shape ``(ny+2, nx+2)`` with
 ``F[j, i]``; Poisson work vectors are ``(ny, nx)``.  PyTorch only supplies
 device memory and streams; every computation runs in libcovof_hip.
 """
 from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BICGSTAB2,  # noqa: F401
                    LINEAR_CG, PRECISION_FP64, PRECISION_MIXED, PRECOND_JACOBI, PRECOND_MG, PRECOND_NONE,
-                   PRECOND_SGS_RB, PRECOND_SSOR, CovofError)
+                   PRECOND_SGS_RB, PRECOND_SSOR, TIME_SCHEME_EULER, TIME_SCHEME_GEAR, CONV_UPWIND,
+                   CONV_CENTERED, CONV_TVD_SUPERBEE, CONV_LIN_PICARD, CONV_LIN_ADAMS_BASHFORTH, CONV_LIN_NEWTON,
+                   CovofError)
 from .context import Context, SolveResult  # noqa: F401
 from .autograd import PressureSolve, pressure_solve  # noqa: F401

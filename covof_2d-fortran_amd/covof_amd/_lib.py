@@ -24,6 +24,16 @@ PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR = 0, 1, 2
 # parallel preconditioners of this library (no reference counterpart)
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+TIME_SCHEME_EULER, TIME_SCHEME_GEAR = 1, 2
+CONV_UPWIND, CONV_CENTERED, CONV_TVD_SUPERBEE = 1, 2, 3
+CONV_LIN_PICARD, CONV_LIN_ADAMS_BASHFORTH, CONV_LIN_NEWTON = 1, 2, 3
+
+
+class MomentumCfg(ct.Structure):
+    """covof_momentum_cfg (include/covof_hip.h)."""
+    _fields_ = [("time_scheme", ct.c_int), ("has_nm1", ct.c_int), ("enable_convection", ct.c_int),
+                ("convective_scheme", ct.c_int), ("convective_linearization", ct.c_int),
+                ("gx", ct.c_double), ("gy", ct.c_double), ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4)]
 
 # every symbol the header declares: name -> (restype, argtypes)
 _P = ct.c_void_p
@@ -58,6 +68,14 @@ SIGNATURES = {
     "covof_sp_correct_d": (_I, [_P, _P, _P, _P, _D, _P, _P]),
     "covof_sp_correct": (_I, [_P, _DP, _DP, _DP, _D, _DP, _DP]),
     "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_momentum_size": (_I, [_P, _IP, _IP, _IP]),
+    "covof_momentum_set_fields_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D]),
+    "covof_momentum_diag_d": (_I, [_P, _P]),
+    "covof_momentum_rhs_d": (_I, [_P, _P]),
+    "covof_momentum_matvec_d": (_I, [_P, _P, _P]),
+    "covof_momentum_solve_d": (_I, [_P, _P, _P, _I, _I, _D, _I, _IP, _DP]),
+    "covof_sp_predictor_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
+    "covof_sp_predictor": (_I, [_P, ct.c_void_p] + [_DP] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
     "covof_ctx_profile": (_I, [_P, _I]),
     "covof_ctx_profile_read": (_I, [_P, _I, _DP, ct.POINTER(ct.c_long)]),
 }

@@ -279,3 +279,103 @@ class Context:
                                                int(max_iter), int(precision), ct.byref(it), ct.byref(rr))
         L.check(st, "covof_poisson_adjoint_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return g_b, g_lr, SolveResult(it.value, rr.value, st)
+
+    # ------------------------------------------------- SP velocity predictor
+    MOMENTUM_FIELDS = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",
+                       "fx_sigma", "fy_sigma")
+
+    def momentum_size(self):
+        """(n, u_nx, v_ny) of the SP momentum unknowns (ns_init,
+        src/module_navier_stokes.f90:136-226)."""
+        n, a, b = ct.c_int(), ct.c_int(), ct.c_int()
+        L.check(self._lib.covof_momentum_size(self._h, ct.byref(n), ct.byref(a), ct.byref(b)),
+                "covof_momentum_size")
+        return n.value, a.value, b.value
+
+    @staticmethod
+    def momentum_cfg(time_scheme=L.TIME_SCHEME_EULER, has_nm1=False, enable_convection=True,
+                     convective_scheme=L.CONV_UPWIND, convective_linearization=L.CONV_LIN_PICARD, gx=0.0, gy=0.0,
+                     bc_u=(0.0,) * 4, bc_v=(0.0,) * 4) -> L.MomentumCfg:
+        """The config_t fields the momentum block reads (reference defaults:
+        EULER, convection on, UPWIND, PICARD)."""
+        return L.MomentumCfg(int(time_scheme), int(bool(has_nm1)), int(bool(enable_convection)),
+                             int(convective_scheme), int(convective_linearization), float(gx), float(gy),
+                             (ct.c_double * 4)(*bc_u), (ct.c_double * 4)(*bc_v))
+
+    def _mom_ptrs(self, fields: dict):
+        out = []
+        for k in self.MOMENTUM_FIELDS:
+            t = fields.get(k)
+            if t is None and k in ("u_nm1", "v_nm1"):
+                out.append(None)
+                continue
+            out.append(_ptr(self._field(t, k)))
+        return out
+
+    def momentum_set_fields(self, cfg: L.MomentumCfg, fields: dict, dt: float):
+        """Binds the fields_old of the momentum block (tensors named as
+        MOMENTUM_FIELDS; u_nm1 / v_nm1 may be absent without an n-1 state).
+        The tensors must stay alive while the block is used."""
+        self._mom_keep = (cfg, dict(fields))
+        self._stream()
+        L.check(self._lib.covof_momentum_set_fields_d(self._h, ct.byref(cfg), *self._mom_ptrs(fields), float(dt)),
+                "covof_momentum_set_fields_d")
+
+    def _mom_vec(self, t, name):
+        n = self.momentum_size()[0]
+        if not (isinstance(t, torch.Tensor) and t.device.type == "cuda" and t.dtype == torch.float64
+                and t.numel() == n and t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous float64 CUDA tensor of {n} entries")
+        return t
+
+    def new_momentum_vector(self, fill=0.0):
+        return torch.full((self.momentum_size()[0],), float(fill), dtype=torch.float64, device=self.device)
+
+    def assemble_preconditioner_blocks(self, diag=None):
+        """diag_u, diag_v (src/module_navier_stokes.f90:1268-1595) of the bound fields."""
+        diag = self.new_momentum_vector() if diag is None else self._mom_vec(diag, "diag")
+        self._stream()
+        L.check(self._lib.covof_momentum_diag_d(self._h, _ptr(diag)), "covof_momentum_diag_d")
+        return diag
+
+    def assemble_rhs(self, rhs=None):
+        """ns_assemble_rhs (src/module_navier_stokes.f90:785-1101) of the bound fields."""
+        rhs = self.new_momentum_vector() if rhs is None else self._mom_vec(rhs, "rhs")
+        self._stream()
+        L.check(self._lib.covof_momentum_rhs_d(self._h, _ptr(rhs)), "covof_momentum_rhs_d")
+        return rhs
+
+    def apply_system_matrix(self, x, y=None):
+        """y = A x, ns_apply_system_matrix (src/module_navier_stokes.f90:380-780)."""
+        self._mom_vec(x, "x")
+        y = self.new_momentum_vector() if y is None else self._mom_vec(y, "y")
+        self._stream()
+        L.check(self._lib.covof_momentum_matvec_d(self._h, _ptr(x), _ptr(y)), "covof_momentum_matvec_d")
+        return y
+
+    def solve_linear_system(self, b, x, linear_solver=L.LINEAR_BICGSTAB2, precond=L.PRECOND_JACOBI, tol=1e-5,
+                            max_iter=300) -> SolveResult:
+        """solve_linear_system (src/module_linear_solvers.f90:1167-1208) on the
+        momentum operator, x in/out; defaults are config_t's (tol 1e-5, 300)."""
+        self._mom_vec(b, "b")
+        self._mom_vec(x, "x")
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_momentum_solve_d(self._h, _ptr(b), _ptr(x), int(linear_solver), int(precond),
+                                              float(tol), int(max_iter), ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_momentum_solve_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return SolveResult(it.value, rr.value, st)
+
+    def solve_standard_projection_predictor(self, cfg: L.MomentumCfg, fields: dict, dt: float,
+                                            linear_solver=L.LINEAR_BICGSTAB2, precond=L.PRECOND_JACOBI,
+                                            tol=1e-5, max_iter=300) -> SolveResult:
+        """The momentum block of sim_solve_standard_projection
+        (src/module_simulation.f90:2454-2469): fields["u"], fields["v"] are
+        replaced in place by the predicted velocity."""
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_sp_predictor_d(self._h, ct.byref(cfg), *self._mom_ptrs(fields), float(dt),
+                                            int(linear_solver), int(precond), float(tol), int(max_iter),
+                                            ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_sp_predictor_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return SolveResult(it.value, rr.value, st)

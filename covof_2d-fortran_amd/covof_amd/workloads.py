@@ -253,3 +253,56 @@ def poisson_case(n, seed=0, n_circles=64, rmin=16.0, rmax=128.0, rho_liquid=1000
     rhs = -((v[2:n + 2, 1:n + 1] - v[1:n + 1, 1:n + 1]) * dx)
     return dict(C=C, u=u, v=v, rho_u=rho_u, rho_v=rho_v, p=np.zeros((n + 2, n + 2)),
                 rhs=rhs, dt=1.0, bc=(BC_FREESLIP,) * 4)
+
+
+def momentum_case(nx, ny, seed=0, n_circles=16, rmin=4.0, rmax=24.0, rho_liquid=1000.0, rho_gas=100.0,
+                  mu_liquid=10.0, mu_gas=1.0, amplitude=0.1, lx=1.0, ly=None, bc=(BC_DIRICHLET,) * 4):
+    """Inputs of the SP momentum block in the reference's field layout.
+
+    Two-phase properties from seeded circles (arithmetic face densities,
+    cell viscosities and arithmetic-mean node viscosities as the shapes of
+    ``update_physical_properties``, src/module_interface_vof.f90:479-616),
+    a smooth velocity field (a sum of Fourier modes, amplitude ``amplitude``)
+    as the current state, ``u_old``/``v_old`` and ``u_nm1``/``v_nm1`` as
+    nearby states, and small smooth forces standing in for the CSF terms.
+    Returns ``dict`` of halo arrays named as the C-ABI's momentum fields, and
+    ``dt`` (CFL 0.25 on the velocity).
+    """
+    ly = lx * ny / nx if ly is None else ly
+    n = max(nx, ny)
+    ind = circles_indicator(n, n_circles, rmin, rmax, seed)[:ny, :nx]
+    C = np.zeros((ny + 2, nx + 2))
+    C[1:ny + 1, 1:nx + 1] = ind
+    fill_halo_zero_gradient(C, nx, ny)
+    rho = C * rho_liquid + (1.0 - C) * rho_gas
+    mu = C * mu_liquid + (1.0 - C) * mu_gas
+    f = {}
+    f["rho_u"] = rho.copy()
+    f["rho_u"][:, 1:] = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    f["rho_v"] = rho.copy()
+    f["rho_v"][1:, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
+    f["mu"] = mu
+    f["mu_node"] = mu.copy()
+    f["mu_node"][1:, 1:] = 0.25 * (mu[:-1, :-1] + mu[1:, :-1] + mu[:-1, 1:] + mu[1:, 1:])
+    rng = np.random.default_rng(seed + 7)
+    x = (np.arange(nx + 2) - 0.5) / nx * lx
+    y = (np.arange(ny + 2) - 0.5) / ny * ly
+    X, Y = np.meshgrid(x, y)
+
+    def smooth(phase):
+        out = np.zeros((ny + 2, nx + 2))
+        for _ in range(4):
+            kx, ky = rng.integers(1, 4, size=2)
+            out += np.sin(2 * np.pi * (kx * X / lx + ky * Y / ly) + phase + rng.uniform(0, 2 * np.pi))
+        return out / 4.0
+
+    for k in ("u", "v"):
+        base = amplitude * smooth(0.0)
+        f[k] = base
+        f[k + "_old"] = base + 0.01 * amplitude * smooth(1.0)
+        f[k + "_nm1"] = base + 0.02 * amplitude * smooth(2.0)
+    f["fx_sigma"] = 10.0 * smooth(3.0)
+    f["fy_sigma"] = 10.0 * smooth(4.0)
+    umax = max(np.abs(f["u"]).max(), np.abs(f["v"]).max(), 1e-30)
+    dt = 0.25 * min(lx / nx, ly / ny) / umax
+    return f, dt, bc, lx, ly

@@ -41,7 +41,7 @@ using covof::Ctx;
 struct covof_ctx {
   Ctx c;
   // device mirrors for the host-array entry points (made on first use)
-  double *mirror[9] = {nullptr};
+  double *mirror[12] = {nullptr};
 };
 
 static thread_local std::string g_err;
@@ -123,6 +123,7 @@ extern "C" int covof_ctx_destroy(covof_ctx *h) {
   covof::vof_free(c);
   covof::surface_free(c);
   covof::poisson_free(c);
+  covof::mom_free(c);
   if (c.prof.created)
     for (int k = 0; k < 2 * 1024; ++k) hipEventDestroy(c.prof.ev[k]);
   for (double *p : h->mirror) hipFree(p);
@@ -407,6 +408,123 @@ extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double d
                                         maxit, precision, iters, relres);
   if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver combination");
   return hip_status(st);
+}
+
+// ------------------------------------------------------------- momentum block
+static int mom_cfg(const covof_momentum_cfg *in, covof::MomCfg *out) {
+  if (!in) return COVOF_BAD_ARGUMENT;
+  if (in->time_scheme < 1 || in->time_scheme > 2 || in->convective_scheme < 1 || in->convective_scheme > 3 ||
+      in->convective_linearization < 1 || in->convective_linearization > 3)
+    return COVOF_BAD_ARGUMENT;
+  out->gear = in->time_scheme == 2 && in->has_nm1;
+  out->conv = in->enable_convection != 0;
+  out->scheme = in->convective_scheme;
+  out->lin = in->convective_linearization;
+  out->ab_nm1 = in->convective_linearization == 2 && in->has_nm1;
+  out->gx = in->gx;
+  out->gy = in->gy;
+  for (int k = 0; k < 4; ++k) {
+    out->bcu[k] = in->bc_u[k];
+    out->bcv[k] = in->bc_v[k];
+  }
+  return COVOF_OK;
+}
+
+extern "C" int covof_momentum_size(covof_ctx *h, int *n, int *u_nx, int *v_ny) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  const covof::Mesh &m = h->c.m;
+  const int a = m.nx + (m.perx ? 0 : 1), b = m.ny + (m.pery ? 0 : 1);
+  if (u_nx) *u_nx = a;
+  if (v_ny) *v_ny = b;
+  if (n) *n = a * m.ny + m.nx * b;
+  return COVOF_OK;
+}
+
+extern "C" int covof_momentum_set_fields_d(covof_ctx *h, const covof_momentum_cfg *cfg, const double *u,
+                                           const double *v, const double *u_old, const double *v_old,
+                                           const double *u_nm1, const double *v_nm1, const double *rho_u,
+                                           const double *rho_v, const double *mu, const double *mu_node,
+                                           const double *fx_sigma, const double *fy_sigma, double dt) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  covof::MomCfg c;
+  if (mom_cfg(cfg, &c)) return fail(COVOF_BAD_ARGUMENT, "covof_momentum_cfg: selector out of range");
+  const covof::MomFields f{u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma};
+  const int st = covof::mom_bind(h->c, c, f, dt);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "momentum block: null field, dt <= 0, missing n-1 fields, or unsupported boundary sides "
+                    "(periodic sides in pairs, no NEUMANN top)");
+  return hip_status(st);
+}
+
+extern "C" int covof_momentum_diag_d(covof_ctx *h, double *diag) {
+  if (!h || !diag) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::mom_diag(h->c, diag);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "momentum fields not set");
+  return hip_status(st);
+}
+
+extern "C" int covof_momentum_rhs_d(covof_ctx *h, double *rhs) {
+  if (!h || !rhs) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::mom_rhs(h->c, rhs);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "momentum fields not set");
+  return hip_status(st);
+}
+
+extern "C" int covof_momentum_matvec_d(covof_ctx *h, const double *x, double *y) {
+  if (!h || !x || !y) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::mom_matvec(h->c, x, y);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "momentum fields not set");
+  return hip_status(st);
+}
+
+extern "C" int covof_momentum_solve_d(covof_ctx *h, const double *b, double *x, int lin_solver, int precond,
+                                      double tol, int maxit, int *iters, double *relres) {
+  if (!h || !b || !x) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::mom_solve(h->c, b, x, lin_solver, precond, tol, maxit, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "momentum fields not set, or solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI");
+  return hip_status(st);
+}
+
+extern "C" int covof_sp_predictor_d(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,
+                                    const double *u_old, const double *v_old, const double *u_nm1,
+                                    const double *v_nm1, const double *rho_u, const double *rho_v,
+                                    const double *mu, const double *mu_node, const double *fx_sigma,
+                                    const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                                    int maxit, int *iters, double *relres) {
+  int st = covof_momentum_set_fields_d(h, cfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node,
+                                       fx_sigma, fy_sigma, dt);
+  if (st) return st;
+  st = covof::mom_predictor(h->c, u, v, lin_solver, precond, tol, maxit, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI");
+  return hip_status(st);
+}
+
+extern "C" int covof_sp_predictor(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,
+                                  const double *u_old, const double *v_old, const double *u_nm1,
+                                  const double *v_nm1, const double *rho_u, const double *rho_v,
+                                  const double *mu, const double *mu_node, const double *fx_sigma,
+                                  const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                                  int maxit, int *iters, double *relres) {
+  if (!h || !u || !v || !u_old || !v_old || !rho_u || !rho_v || !mu || !mu_node || !fx_sigma || !fy_sigma)
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const double *hs[12] = {u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma};
+  double *d[12] = {nullptr};
+  int st = 0;
+  for (int k = 0; k < 12 && !st; ++k) {
+    if (!hs[k]) continue;
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+    st = h2d(h, d[k], hs[k]);
+  }
+  if (st) return hip_status(st);
+  const int sst = covof_sp_predictor_d(h, cfg, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10],
+                                       d[11], dt, lin_solver, precond, tol, maxit, iters, relres);
+  if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
+  st = d2h(h, u, d[0]);
+  if (!st) st = d2h(h, v, d[1]);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return st ? hip_status(st) : sst;
 }
 
 // ------------------------------------------------------------------ profiling

@@ -36,6 +36,44 @@ struct Surface {
   double *work = nullptr;
 };
 
+// Options of the SP momentum block (momentum.hip), the covof_momentum_cfg of
+// the C-ABI with the derived switches.
+struct MomCfg {
+  int gear;      // time_scheme == GEAR with a valid n-1 state: alpha_time 1.5 and BDF2 inertia
+  int conv;      // enable_convection
+  int scheme;    // 1 UPWIND, 2 CENTERED, other: TVD superbee
+  int lin;       // 1 PICARD, 2 ADAMS_BASHFORTH, 3 NEWTON
+  int ab_nm1;    // ADAMS_BASHFORTH with a valid n-1 state: extrapolated advecting velocity
+  double gx, gy;
+  double bcu[4], bcv[4];
+};
+// The fields the momentum operator, its diagonal and right-hand side read
+// (the reference's fields_old), halo layout.
+struct MomFields {
+  const double *u, *v, *uo, *vo, *un, *vn, *ru, *rv, *mu, *mn, *fx, *fy;
+};
+// Device state of the momentum block, made on first use.
+struct Mom {
+  int bound = 0;              // fields and options set
+  int u_nx = 0, v_ny = 0;     // unknowns per row of u, rows of v
+  long nu = 0, n = 0;         // u unknowns, all unknowns
+  MomCfg cfg;
+  MomFields f;
+  double dt = 0.0;
+  double *tu = nullptr, *tv = nullptr;  // the operator's persistent work fields (halo layout, zero at first use)
+  double *diag = nullptr, *rhs = nullptr, *x = nullptr;
+  double *w[9] = {nullptr};   // Krylov work vectors
+};
+
+// A linear operator with its preconditioner for the generic Krylov loops
+// (krylov.hip); vectors of n entries, device pointers.
+struct LinOp {
+  long n = 0;
+  virtual int apply(const double *x, double *y) = 0;
+  virtual int precond(const double *r, double *z) = 0;
+  virtual ~LinOp() {}
+};
+
 // One instantiated capture of a solver cycle (see poisson.hip).
 struct PGraph {
   hipGraphExec_t exec = nullptr;
@@ -99,6 +137,7 @@ struct Ctx {
   Vof vof;
   Surface surf;
   Poisson ps;
+  Mom mom;
   Prof prof;
 };
 
@@ -153,5 +192,23 @@ int svof_smooth(Ctx &c, const double *C, double width, double *Cs);
 void surface_free(Ctx &c);
 int poisson_alloc(Ctx &c);
 void poisson_free(Ctx &c);
+int poisson_matvec(Ctx &c, const double *x, double *y);
+int poisson_precond(Ctx &c, int precond, const double *r, double *z);
+// krylov.hip: BiCGSTAB(2) and CG of module_linear_solvers on any operator
+int krylov_bicgstab2(Ctx &c, LinOp &op, const double *b, double *x, double *const w[9], double tol,
+                     int maxit, int *iters, double *relres);
+int krylov_cg(Ctx &c, LinOp &op, const double *b, double *x, double *const w[4], double tol, int maxit,
+              int *iters, double *relres);
+// momentum.hip: the SP momentum block
+int mom_bind(Ctx &c, const MomCfg &cfg, const MomFields &f, double dt);
+int mom_diag(Ctx &c, double *diag);
+int mom_rhs(Ctx &c, double *rhs);
+int mom_pack(Ctx &c, const double *u, const double *v, double *x);
+int mom_matvec(Ctx &c, const double *x, double *y);
+int mom_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol, int maxit,
+              int *iters, double *relres);
+int mom_predictor(Ctx &c, double *u, double *v, int lin_solver, int precond, double tol, int maxit,
+                  int *iters, double *relres);
+void mom_free(Ctx &c);
 
 }  // namespace covof

@@ -1,7 +1,8 @@
-// BiCGSTAB(2) on the Poisson operator (src/module_linear_solvers.f90:361-579).
+// BiCGSTAB(2) (src/module_linear_solvers.f90:361-579) and CG (:735-838) on
+// any operator: the pressure operator (BiCGSTAB(2) re-targeted from the
+// reference's velocity system) and the SP momentum operator (momentum.hip).
 //
-// The reference's left-preconditioned BiCGSTAB(2) re-targeted from its
-// velocity operator to the pressure operator: apply_PA = precond(A x)
+// The reference's left-preconditioned BiCGSTAB(2): apply_PA = precond(A x)
 // (:323-356).  Vector updates are elementwise kernels that evaluate the
 // reference's expressions in the same order; inner products are deterministic
 // grid reductions; the scalar recurrences (with every breakdown test of the
@@ -12,9 +13,6 @@
 #include <cmath>
 
 namespace covof {
-
-int poisson_matvec(Ctx &c, const double *x, double *y);
-int poisson_precond(Ctx &c, int precond, const double *r, double *z);
 
 namespace {
 
@@ -106,18 +104,20 @@ __global__ void k_zero(long n, double *x) {
 
 struct Bicg {
   Ctx &c;
+  LinOp &op;
   long n;
-  int precond;
   double *tmp;
   dim3 g{1024}, b{256};
   int err = 0;
 
+  // apply_PA (src/module_linear_solvers.f90:323-356): dst = M^-1 (A src)
   void PA(const double *src, double *dst) {
-    if (!err) err = poisson_matvec(c, src, tmp);
-    if (!err) err = poisson_precond(c, precond, tmp, dst);
+    if (!err) err = op.apply(src, tmp);
+    if (!err) err = op.precond(tmp, dst);
   }
   // inner products -> host (synchronises)
   void dots(int k, const double *const *a, const double *const *bb, double *res) {
+    if (err) return;
     Pairs P;
     P.n = k;
     for (int q = 0; q < k; ++q) {
@@ -143,19 +143,17 @@ struct Bicg {
 
 // src/module_linear_solvers.f90:361-579 (iteration counter += 2 per sweep;
 // stop on ||r||/||r0|| < tol or ||r|| < 1e-12 of the preconditioned residual).
-int solve_bicgstab2(Ctx &c, const double *bvec, double *x, int precond, double tol, int maxit,
-                    int *iters, double *relres) {
-  Poisson &P = c.ps;
-  const long n = (long)c.m.nx * c.m.ny;
-  Bicg K{c, n, precond, P.t5};
-  double *bh = P.b == bvec ? P.z : P.b;  // b_hat must not alias b
-  double *rt = P.r, *r0 = P.t4, *r1 = P.q, *r2 = P.t0, *u0 = P.t1, *u1 = P.t2, *u2 = P.t3;
-  const double *work[9] = {rt, r0, r1, r2, u0, u1, u2, bh, P.t5};
-  for (const double *w : work)
-    if (w == x || (w == bvec && w != bh)) return COVOF_BAD_ARGUMENT;
+// w: nine work vectors of op.n entries, none aliasing b or x.
+int krylov_bicgstab2(Ctx &c, LinOp &op, const double *bvec, double *x, double *const w[9], double tol,
+                     int maxit, int *iters, double *relres) {
+  const long n = op.n;
+  double *bh = w[0], *rt = w[1], *r0 = w[2], *r1 = w[3], *r2 = w[4], *u0 = w[5], *u1 = w[6], *u2 = w[7];
+  Bicg K{c, op, n, w[8]};
+  for (int k = 0; k < 9; ++k)
+    if (w[k] == x || w[k] == bvec) return COVOF_BAD_ARGUMENT;
   const dim3 g(1024), bl(256);
   for (double *v : {u0, u1, u2, r1, r2}) hipLaunchKernelGGL(k_zero, g, bl, 0, c.stream, n, v);
-  K.err = poisson_precond(c, precond, bvec, bh);
+  K.err = op.precond(bvec, bh);
   K.PA(x, r0);
   hipLaunchKernelGGL(k_sub, g, bl, 0, c.stream, n, r0, bh, r0);
   const double r0n = sqrt(K.dot(r0, r0));
@@ -239,6 +237,85 @@ int solve_bicgstab2(Ctx &c, const double *bvec, double *x, int precond, double t
   if (iters) *iters = iter;
   if (relres) *relres = rn / r0n;
   return status;
+}
+
+// solve_cg (src/module_linear_solvers.f90:735-838): split PCG from x0, stop
+// on ||r||/||r0|| < tol or ||r|| < 1e-13; |p.Ap| < 1e-30 ends the loop
+// (COVOF_BREAKDOWN).  iters follows the reference: the stopping iteration,
+// else maxit.  z is formed before the stop test (its dot rides with r.r), so
+// each iteration synchronises twice.  w: four work vectors (r, z, p, Ap).
+int krylov_cg(Ctx &c, LinOp &op, const double *bvec, double *x, double *const w[4], double tol, int maxit,
+              int *iters, double *relres) {
+  const long n = op.n;
+  double *r = w[0], *z = w[1], *p = w[2], *Ap = w[3];
+  for (int k = 0; k < 4; ++k)
+    if (w[k] == x || w[k] == bvec) return COVOF_BAD_ARGUMENT;
+  Bicg K{c, op, n, nullptr};
+  const dim3 g(1024), bl(256);
+  K.err = op.apply(x, Ap);
+  hipLaunchKernelGGL(k_sub, g, bl, 0, c.stream, n, r, bvec, Ap);
+  const double r0n = sqrt(K.dot(r, r));
+  if (K.err) return K.err;
+  if (iters) *iters = maxit;
+  if (relres) *relres = 1.0;
+  if (r0n < 1.0e-12) {
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    return COVOF_OK;
+  }
+  if (!K.err) K.err = op.precond(r, z);
+  hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+  double rz_old = K.dot(r, z);
+  int status = COVOF_NOT_CONVERGED;
+  for (int it = 1; it <= maxit && !K.err; ++it) {
+    K.err = op.apply(p, Ap);
+    const double pAp = K.dot(p, Ap);
+    if (K.err) break;
+    if (fabs(pAp) < 1.0e-30) { status = COVOF_BREAKDOWN; break; }
+    const double alpha = rz_old / pAp;
+    K.axpy(x, x, alpha, p);
+    K.axpy(r, r, -alpha, Ap);
+    if (!K.err) K.err = op.precond(r, z);
+    double s[2];
+    const double *A2[2] = {r, r}, *B2[2] = {r, z};
+    K.dots(2, A2, B2, s);
+    if (K.err) break;
+    const double rn = sqrt(s[0]);
+    if (relres) *relres = rn / r0n;
+    if (rn != rn) { status = COVOF_NONFINITE; break; }
+    if (rn / r0n < tol || rn < 1.0e-13) {
+      if (iters) *iters = it;
+      status = COVOF_OK;
+      break;
+    }
+    const double beta = s[1] / rz_old;
+    rz_old = s[1];
+    K.axpy(p, z, beta, p);
+  }
+  if (K.err) return K.err;
+  if (hipStreamSynchronize(c.stream) != hipSuccess) return COVOF_HIP_ERROR;
+  return status;
+}
+
+namespace {
+// The Poisson operator and its preconditioners as a LinOp.
+struct PoissonOp : LinOp {
+  Ctx &c;
+  int pc;
+  PoissonOp(Ctx &cc, int precond) : c(cc), pc(precond) { n = (long)cc.m.nx * cc.m.ny; }
+  int apply(const double *x, double *y) override { return poisson_matvec(c, x, y); }
+  int precond(const double *r, double *z) override { return poisson_precond(c, pc, r, z); }
+};
+}  // namespace
+
+// BiCGSTAB(2) re-targeted to the pressure operator.
+int solve_bicgstab2(Ctx &c, const double *bvec, double *x, int precond, double tol, int maxit,
+                    int *iters, double *relres) {
+  Poisson &P = c.ps;
+  PoissonOp op(c, precond);
+  double *bh = P.b == bvec ? P.z : P.b;  // b_hat must not alias b
+  double *const w[9] = {bh, P.r, P.t4, P.q, P.t0, P.t1, P.t2, P.t3, P.t5};
+  return krylov_bicgstab2(c, op, bvec, x, w, tol, maxit, iters, relres);
 }
 
 }  // namespace covof

@@ -2,9 +2,10 @@
  * covof_hip.h -- C-ABI of the MI355X (gfx950) hot path of COVOF 2D.
  *
  * Drop-in replacements for the per-timestep hot path of the reference Fortran
- * solver: Weymouth-Yue VOF advection with PLIC/ELVIRA reconstruction, the
- * variable-density pressure-Poisson solve of the standard projection, and the
- * velocity correction.  Plain C types only (pointers, ints, doubles), so the
+ * solver: Weymouth-Yue VOF advection with PLIC/ELVIRA reconstruction and the
+ * steps that follow it in advect_vof, the SP velocity predictor (momentum
+ * block), the variable-density pressure-Poisson solve of the standard
+ * projection, and the velocity correction.  Plain C types only (pointers, ints, doubles), so the
  * Fortran driver binds them with ISO_C_BINDING and Python with ctypes.
  *
  * Conventions
@@ -185,6 +186,74 @@ int covof_sp_correct_d(covof_ctx *ctx, const double *p, const double *rho_u, con
                        double dt, double *u, double *v);
 int covof_sp_correct(covof_ctx *ctx, const double *p, const double *rho_u, const double *rho_v,
                      double dt, double *u, double *v);
+
+/* --- SP velocity predictor (the momentum block of the standard projection) - */
+
+/* Options of the momentum block (the config_t fields it reads; reference
+ * integers of src/module_constants.f90:81-85, :132-140).  bc_u/bc_v: the
+ * Dirichlet values cfg%bc_<side>%u / %v of left, right, bottom, top. */
+typedef struct covof_momentum_cfg {
+  int time_scheme;              /* 1 EULER, 2 GEAR (BDF2) */
+  int has_nm1;                  /* fields%has_nm1: u_nm1, v_nm1 hold the n-1 state */
+  int enable_convection;        /* cfg%enable_convection */
+  int convective_scheme;        /* 1 UPWIND, 2 CENTERED, 3 TVD_SUPERBEE */
+  int convective_linearization; /* 1 PICARD, 2 ADAMS_BASHFORTH, 3 NEWTON */
+  double gx, gy;
+  double bc_u[4], bc_v[4];
+} covof_momentum_cfg;
+
+/* Unknowns of the SP momentum system, ns_init (src/module_navier_stokes.f90:
+ * 136-226): u(1:u_nx, 1:ny) then v(1:nx, 1:v_ny), i fastest, with u_nx = nx+1
+ * (nx if periodic in x) and v_ny = ny+1 (ny if periodic in y); *n = total. */
+int covof_momentum_size(covof_ctx *ctx, int *n, int *u_nx, int *v_ny);
+/* The fields_old of the momentum block (device, halo layout): the current
+ * velocity u, v (linearisation state), u_old, v_old (the time level n),
+ * u_nm1, v_nm1 (level n-1; may be NULL unless has_nm1 with GEAR or
+ * ADAMS_BASHFORTH), rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma, and dt.
+ * The pointers are kept (not copied) until the next call.  Supported: the
+ * standard projection with UNIFORM boundary profiles, no obstacles, no energy
+ * equation, no SSP-RK3 momentum, periodic sides in pairs, and no NEUMANN top
+ * (the reference's v-row at j = ny+1 reads past its array); anything else
+ * returns COVOF_BAD_ARGUMENT. */
+int covof_momentum_set_fields_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, const double *u,
+                                const double *v, const double *u_old, const double *v_old,
+                                const double *u_nm1, const double *v_nm1, const double *rho_u,
+                                const double *rho_v, const double *mu, const double *mu_node,
+                                const double *fx_sigma, const double *fy_sigma, double dt);
+/* diag_u, diag_v of assemble_preconditioner_blocks (src/module_navier_stokes.f90:
+ * 1268-1595), the Jacobi preconditioner of the momentum solve; n entries. */
+int covof_momentum_diag_d(covof_ctx *ctx, double *diag);
+/* assemble_rhs (src/module_navier_stokes.f90:785-1101); n entries. */
+int covof_momentum_rhs_d(covof_ctx *ctx, double *rhs);
+/* y = A x, apply_system_matrix (src/module_navier_stokes.f90:380-780), with
+ * the reference's unpack_vector(homogeneous) into the operator's persistent
+ * work fields (its SAVEd tf, zero at the context's first product). */
+int covof_momentum_matvec_d(covof_ctx *ctx, const double *x, double *y);
+/* solve_linear_system (src/module_linear_solvers.f90:1167-1208) on the
+ * momentum operator: lin_solver BICGSTAB2 (:361-579) or CG (:735-838),
+ * precond NONE or JACOBI (:88-135).  x: initial guess in, solution out.
+ * *iters as the reference reports it. */
+int covof_momentum_solve_d(covof_ctx *ctx, const double *b, double *x, int lin_solver, int precond,
+                           double tol, int maxit, int *iters, double *relres);
+/* The momentum block of sim_solve_standard_projection (src/module_simulation.f90:
+ * 2454-2469): diagonal, right-hand side, pack (the current u, v are the
+ * initial guess), solve, unpack_vector(.false.), sync_u / sync_v and
+ * apply_boundary_conditions(.false.).  u, v (halo layout) are updated in place
+ * with the predicted velocity u*, v*; the other arguments as
+ * covof_momentum_set_fields_d.  Returns the solve's status. */
+int covof_sp_predictor_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v,
+                         const double *u_old, const double *v_old, const double *u_nm1,
+                         const double *v_nm1, const double *rho_u, const double *rho_v,
+                         const double *mu, const double *mu_node, const double *fx_sigma,
+                         const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                         int maxit, int *iters, double *relres);
+/* Host-array variant of covof_sp_predictor_d (the Fortran drop-in). */
+int covof_sp_predictor(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v,
+                       const double *u_old, const double *v_old, const double *u_nm1,
+                       const double *v_nm1, const double *rho_u, const double *rho_v,
+                       const double *mu, const double *mu_node, const double *fx_sigma,
+                       const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                       int maxit, int *iters, double *relres);
 
 /* --- kernel timing (no reference counterpart: instrumentation) ------------- */
 

@@ -945,3 +945,677 @@ double orc_bench_jacobi_pcg(const orc_mesh *m, const double *aP, const double *a
   free(x); free(r); free(d); free(q);
   return t1 - t0;
 }
+
+/* ------------------------------------------------ SP velocity predictor
+ *
+ * The momentum step of the standard projection (src/module_simulation.f90:
+ * 2454-2469): assemble_preconditioner_blocks (diagonal), assemble_rhs,
+ * pack_vector, solve_linear_system, unpack_vector, sync + velocity BC.
+ * Supported subset: SOLVER_STANDARD_PROJECTION, no obstacles, UNIFORM
+ * boundary profiles, no energy equation, no SSP-RK3 momentum. */
+
+/* The reference's has_nm1 / u_old / u_nm1 inputs of the momentum step. */
+static int gear(const orc_mom_cfg *c) { return c->time_scheme == 2 && c->has_nm1; }
+static int ab_nm1(const orc_mom_cfg *c) { return c->conv_lin == 2 && c->has_nm1; }
+
+/* src/module_mpi_comm.f90:290-537 (apply_velocity_bc) for UNIFORM profiles. */
+void orc_velocity_bc(const orc_mesh *m, const orc_mom_cfg *c, double *u, double *v, int homogeneous) {
+  const int nx = m->nx, ny = m->ny;
+  const int *bc = m->bc;
+  const double u_l = homogeneous ? 0.0 : c->bc_u[0], v_l = homogeneous ? 0.0 : c->bc_v[0];
+  const double v_r = homogeneous ? 0.0 : c->bc_v[1];
+  const double u_r = homogeneous ? 0.0 : c->bc_u[1];
+  const double u_b = homogeneous ? 0.0 : c->bc_u[2], v_b = homogeneous ? 0.0 : c->bc_v[2];
+  const double u_t = homogeneous ? 0.0 : c->bc_u[3], v_t = homogeneous ? 0.0 : c->bc_v[3];
+  if (bc[0] == BC_PERIODIC) {
+    for (int j = 1; j <= ny; ++j) {
+      u[H(m, 0, j)] = u[H(m, nx, j)];
+      u[H(m, nx + 1, j)] = u[H(m, 1, j)];
+      v[H(m, 0, j)] = v[H(m, nx, j)];
+      v[H(m, nx + 1, j)] = v[H(m, 1, j)];
+    }
+  }
+  if (bc[2] == BC_PERIODIC) {
+    for (int i = 1; i <= nx; ++i) {
+      v[H(m, i, 0)] = v[H(m, i, ny)];
+      v[H(m, i, ny + 1)] = v[H(m, i, 1)];
+    }
+    for (int i = 0; i <= nx + 1; ++i) {
+      u[H(m, i, 0)] = u[H(m, i, ny)];
+      u[H(m, i, ny + 1)] = u[H(m, i, 1)];
+    }
+  }
+  if (!per_x(m) && bc[0] != BC_PERIODIC) {
+    for (int j = 1; j <= ny; ++j) {
+      if (bc[0] == BC_DIRICHLET) {
+        const double u_inlet = u_l;
+        if (!homogeneous) u[H(m, 1, j)] = u_inlet;
+        u[H(m, 0, j)] = u_inlet;
+        v[H(m, 0, j)] = 2.0 * v_l - v[H(m, 1, j)];
+      } else if (bc[0] == BC_FREESLIP) {
+        if (!homogeneous) u[H(m, 1, j)] = 0.0;
+        u[H(m, 0, j)] = 0.0;
+        v[H(m, 0, j)] = v[H(m, 1, j)];
+      } else if (bc[0] == BC_NEUMANN) {
+        u[H(m, 0, j)] = u[H(m, 1, j)];
+        v[H(m, 0, j)] = v[H(m, 1, j)];
+      }
+    }
+  }
+  if (!per_x(m) && bc[1] != BC_PERIODIC) {
+    for (int j = 1; j <= ny; ++j) {
+      if (bc[1] == BC_DIRICHLET) {
+        if (!homogeneous) u[H(m, nx + 1, j)] = u_r;
+        v[H(m, nx + 1, j)] = 2.0 * v_r - v[H(m, nx, j)];
+      } else if (bc[1] == BC_FREESLIP) {
+        if (!homogeneous) u[H(m, nx + 1, j)] = 0.0;
+        v[H(m, nx + 1, j)] = v[H(m, nx, j)];
+      } else if (bc[1] == BC_NEUMANN) {
+        if (!homogeneous) u[H(m, nx + 1, j)] = u[H(m, nx, j)];
+        v[H(m, nx + 1, j)] = v[H(m, nx, j)];
+      }
+    }
+  }
+  if (!per_y(m) && bc[2] != BC_PERIODIC) {
+    for (int i = 1; i <= nx; ++i) {
+      if (bc[2] == BC_DIRICHLET) {
+        if (!homogeneous) v[H(m, i, 1)] = v_b;
+        v[H(m, i, 0)] = v_b;
+        if (ny >= 2) u[H(m, i, 0)] = (8.0 * u_b - 6.0 * u[H(m, i, 1)] + u[H(m, i, 2)]) / 3.0;
+        else u[H(m, i, 0)] = 2.0 * u_b - u[H(m, i, 1)];
+      } else if (bc[2] == BC_FREESLIP) {
+        if (!homogeneous) v[H(m, i, 1)] = 0.0;
+        v[H(m, i, 0)] = 0.0;
+        u[H(m, i, 0)] = u[H(m, i, 1)];
+      } else if (bc[2] == BC_NEUMANN) {
+        u[H(m, i, 0)] = u[H(m, i, 1)];
+        v[H(m, i, 0)] = v[H(m, i, 1)];
+      }
+    }
+  }
+  if (!per_y(m) && bc[3] != BC_PERIODIC) {
+    for (int i = 1; i <= nx; ++i) {
+      if (bc[3] == BC_DIRICHLET) {
+        if (!homogeneous) v[H(m, i, ny + 1)] = v_t;
+        if (ny >= 2) u[H(m, i, ny + 1)] = (8.0 * u_t - 6.0 * u[H(m, i, ny)] + u[H(m, i, ny - 1)]) / 3.0;
+        else u[H(m, i, ny + 1)] = 2.0 * u_t - u[H(m, i, ny)];
+      } else if (bc[3] == BC_FREESLIP) {
+        if (!homogeneous) v[H(m, i, ny + 1)] = 0.0;
+        u[H(m, i, ny + 1)] = u[H(m, i, ny)];
+      } else if (bc[3] == BC_NEUMANN) {
+        u[H(m, i, ny + 1)] = u[H(m, i, ny)];
+        if (!homogeneous) v[H(m, i, ny + 1)] = v[H(m, i, ny)];
+      }
+    }
+  }
+  if (!homogeneous) {
+    if (!per_x(m) && bc[1] != BC_PERIODIC) {
+      if (!per_y(m) && bc[2] != BC_PERIODIC) {
+        u[H(m, nx + 1, 0)] = u[H(m, nx + 1, 1)];
+        v[H(m, nx + 1, 0)] = v[H(m, nx, 0)];
+      }
+      if (!per_y(m) && bc[3] != BC_PERIODIC) {
+        u[H(m, nx + 1, ny + 1)] = u[H(m, nx + 1, ny)];
+        v[H(m, nx + 1, ny + 1)] = v[H(m, nx, ny + 1)];
+      }
+    }
+    if (!per_x(m) && bc[0] != BC_PERIODIC && !per_y(m) && bc[3] != BC_PERIODIC)
+      v[H(m, 0, ny + 1)] = v[H(m, 1, ny + 1)];
+  }
+}
+
+/* Unknown counts of ns_init (src/module_navier_stokes.f90:136-226), SP. */
+void orc_mom_sizes(const orc_mesh *m, int *u_nx, int *v_ny) {
+  *u_nx = m->nx + (per_x(m) ? 0 : 1);
+  *v_ny = m->ny + (per_y(m) ? 0 : 1);
+}
+
+/* ns_pack_vector (src/module_navier_stokes.f90:231-295), SP unknowns. */
+void orc_mom_pack(const orc_mesh *m, const double *u, const double *v, double *vec) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long nu = (long)u_nx * m->ny;
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) vec[(long)(j - 1) * u_nx + (i - 1)] = u[H(m, i, j)];
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) vec[nu + (long)(j - 1) * m->nx + (i - 1)] = v[H(m, i, j)];
+}
+
+/* ns_unpack_vector (src/module_navier_stokes.f90:300-375): interior unknowns,
+ * sync_u / sync_v (the single-rank exchange), apply_boundary_conditions. */
+void orc_mom_unpack(const orc_mesh *m, const orc_mom_cfg *c, const double *vec, double *u, double *v,
+                    int homogeneous) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long nu = (long)u_nx * m->ny;
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) u[H(m, i, j)] = vec[(long)(j - 1) * u_nx + (i - 1)];
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) v[H(m, i, j)] = vec[nu + (long)(j - 1) * m->nx + (i - 1)];
+  orc_exchange_halos_scalar(m, u);
+  orc_exchange_halos_scalar(m, v);
+  orc_velocity_bc(m, c, u, v, homogeneous);
+}
+
+/* conv_face_flux (src/module_navier_stokes.f90:2388-2416). */
+static double conv_face_flux(int scheme, double flux, double phi_L, double phi_R, double q_LL,
+                             double q_L, double q_R, double q_RR, int has_LL, int has_RR) {
+  if (scheme == 2) return 0.5 * flux * (phi_L + phi_R);
+  if (scheme == 1) return flux >= 0.0 ? flux * phi_L : flux * phi_R;
+  const double dq = q_R - q_L;
+  double psi = 0.0, r;
+  if (flux >= 0.0) {
+    if (has_LL && fabs(dq) > 1.0e-14) {
+      r = (q_L - q_LL) / dq;
+      psi = dmax(dmax(0.0, dmin(2.0 * r, 1.0)), dmin(r, 2.0));
+    }
+    return flux * (phi_L + 0.5 * psi * (phi_R - phi_L));
+  }
+  if (has_RR && fabs(dq) > 1.0e-14) {
+    r = (q_RR - q_R) / dq;
+    psi = dmax(dmax(0.0, dmin(2.0 * r, 1.0)), dmin(r, 2.0));
+  }
+  return flux * (phi_R + 0.5 * psi * (phi_L - phi_R));
+}
+
+/* compute_u_conv_fluxes (src/module_navier_stokes.f90:2418-2457). */
+static void u_conv_fluxes(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, int i,
+                          int j, double *FE, double *FW, double *FN, double *FS) {
+  const double *uu = c->conv_lin == 2 ? F->u_old : F->u, *vv = c->conv_lin == 2 ? F->v_old : F->v;
+  double u_n = uu[H(m, i, j)], u_n_E = uu[H(m, i + 1, j)], u_n_W = uu[H(m, i - 1, j)];
+  double v_n_N = 0.5 * (vv[H(m, i - 1, j + 1)] + vv[H(m, i, j + 1)]);
+  double v_n_S = 0.5 * (vv[H(m, i - 1, j)] + vv[H(m, i, j)]);
+  if (ab_nm1(c)) {
+    u_n = 2.0 * u_n - F->u_nm1[H(m, i, j)];
+    u_n_E = 2.0 * u_n_E - F->u_nm1[H(m, i + 1, j)];
+    u_n_W = 2.0 * u_n_W - F->u_nm1[H(m, i - 1, j)];
+    const double v_nm1_N = 0.5 * (F->v_nm1[H(m, i - 1, j + 1)] + F->v_nm1[H(m, i, j + 1)]);
+    const double v_nm1_S = 0.5 * (F->v_nm1[H(m, i - 1, j)] + F->v_nm1[H(m, i, j)]);
+    v_n_N = 2.0 * v_n_N - v_nm1_N;
+    v_n_S = 2.0 * v_n_S - v_nm1_S;
+  }
+  *FE = 0.5 * (u_n + u_n_E) * m->dy;
+  *FW = 0.5 * (u_n_W + u_n) * m->dy;
+  *FN = v_n_N * m->dx;
+  *FS = v_n_S * m->dx;
+}
+
+/* compute_v_conv_fluxes (src/module_navier_stokes.f90:2459-2498). */
+static void v_conv_fluxes(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, int i,
+                          int j, double *FE, double *FW, double *FN, double *FS) {
+  const double *uu = c->conv_lin == 2 ? F->u_old : F->u, *vv = c->conv_lin == 2 ? F->v_old : F->v;
+  double v_n = vv[H(m, i, j)], v_n_N = vv[H(m, i, j + 1)], v_n_S = vv[H(m, i, j - 1)];
+  double u_n_E = 0.5 * (uu[H(m, i + 1, j - 1)] + uu[H(m, i + 1, j)]);
+  double u_n_W = 0.5 * (uu[H(m, i, j - 1)] + uu[H(m, i, j)]);
+  if (ab_nm1(c)) {
+    v_n = 2.0 * v_n - F->v_nm1[H(m, i, j)];
+    v_n_N = 2.0 * v_n_N - F->v_nm1[H(m, i, j + 1)];
+    v_n_S = 2.0 * v_n_S - F->v_nm1[H(m, i, j - 1)];
+    const double u_nm1_E = 0.5 * (F->u_nm1[H(m, i + 1, j - 1)] + F->u_nm1[H(m, i + 1, j)]);
+    const double u_nm1_W = 0.5 * (F->u_nm1[H(m, i, j - 1)] + F->u_nm1[H(m, i, j)]);
+    u_n_E = 2.0 * u_n_E - u_nm1_E;
+    u_n_W = 2.0 * u_n_W - u_nm1_W;
+  }
+  *FE = u_n_E * m->dy;
+  *FW = u_n_W * m->dy;
+  *FN = 0.5 * (v_n + v_n_N) * m->dx;
+  *FS = 0.5 * (v_n_S + v_n) * m->dx;
+}
+
+/* ns_apply_system_matrix (src/module_navier_stokes.f90:380-780), SP branch:
+ * tu, tv are the persistent work fields of the reference's `tf` (SAVE),
+ * which unpack_vector(homogeneous) fills before every product; the caller
+ * keeps them (zero-initialised, like the reference's first allocation). */
+void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                    double *tu, double *tv, const double *x, double *y) {
+  const int nx = m->nx, ny = m->ny;
+  const double dx = m->dx, dy = m->dy, dvol = dx * dy;
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  const long nu = (long)u_nx * ny;
+  const double at = gear(c) ? 1.5 : 1.0;
+  const int sch = c->conv_scheme;
+  orc_mom_unpack(m, c, x, tu, tv, 1);
+  const double *u = F->u, *v = F->v;
+#define TU(a, b) tu[H(m, a, b)]
+#define TV(a, b) tv[H(m, a, b)]
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) {
+      const long dof = (long)(j - 1) * u_nx + (i - 1);
+      const double rho_u = F->rho_u[H(m, i, j)];
+      const double diag_bc = (at * rho_u / dt) * dvol;
+      if (i == 1 && bl) {
+        y[dof] = m->bc[0] == BC_NEUMANN ? diag_bc * (TU(1, j) - TU(2, j)) : diag_bc * TU(1, j);
+        continue;
+      }
+      if (i == u_nx && br) {
+        y[dof] = m->bc[1] == BC_NEUMANN ? diag_bc * (TU(u_nx, j) - TU(u_nx - 1, j)) : diag_bc * TU(u_nx, j);
+        continue;
+      }
+      double val = (at * rho_u / dt) * TU(i, j) * dvol;
+      if (c->enable_convection) {
+        double cE, cW, cN, cS;
+        u_conv_fluxes(m, c, F, i, j, &cE, &cW, &cN, &cS);
+        const int has_ee = i + 2 <= nx + 1, has_ww = i - 2 >= 0, has_nn = j + 2 <= ny + 1, has_ss = j - 2 >= 0;
+        const double q_ee = has_ee ? u[H(m, i + 2, j)] : 0.0, q_ww = has_ww ? u[H(m, i - 2, j)] : 0.0;
+        const double q_nn = has_nn ? u[H(m, i, j + 2)] : 0.0, q_ss = has_ss ? u[H(m, i, j - 2)] : 0.0;
+        const double pE = conv_face_flux(sch, cE, TU(i, j), TU(i + 1, j), u[H(m, i - 1, j)], u[H(m, i, j)],
+                                         u[H(m, i + 1, j)], q_ee, 1, has_ee);
+        const double pW = conv_face_flux(sch, cW, TU(i - 1, j), TU(i, j), q_ww, u[H(m, i - 1, j)], u[H(m, i, j)],
+                                         u[H(m, i + 1, j)], has_ww, 1);
+        const double pN = conv_face_flux(sch, cN, TU(i, j), TU(i, j + 1), u[H(m, i, j - 1)], u[H(m, i, j)],
+                                         u[H(m, i, j + 1)], q_nn, 1, has_nn);
+        const double pS = conv_face_flux(sch, cS, TU(i, j - 1), TU(i, j), q_ss, u[H(m, i, j - 1)], u[H(m, i, j)],
+                                         u[H(m, i, j + 1)], has_ss, 1);
+        val = val + rho_u * (pE - pW + pN - pS);
+        if (c->conv_lin == 3) {
+          const double du_dx = (u[H(m, i + 1, j)] - u[H(m, i - 1, j)]) / (2.0 * dx);
+          const double du_dy = (u[H(m, i, j + 1)] - u[H(m, i, j - 1)]) / (2.0 * dy);
+          const double tvf = 0.25 * (TV(i - 1, j) + TV(i, j) + TV(i - 1, j + 1) + TV(i, j + 1));
+          val = val + rho_u * (TU(i, j) * du_dx + tvf * du_dy) * dvol;
+        }
+      }
+      const double mu_E = F->mu[H(m, i, j)], mu_W = F->mu[H(m, i - 1, j)];
+      const double deE = 2.0 * mu_E * (TU(i + 1, j) - TU(i, j)) / dx * dy;
+      const double deW = 2.0 * mu_W * (TU(i, j) - TU(i - 1, j)) / dx * dy;
+      val = val - (deE - deW);
+      const double mnN = F->mu_node[H(m, i, j + 1)], mnS = F->mu_node[H(m, i, j)];
+      const double shN = mnN * ((TU(i, j + 1) - TU(i, j)) / dy + (TV(i, j + 1) - TV(i - 1, j + 1)) / dx) * dx;
+      const double shS = mnS * ((TU(i, j) - TU(i, j - 1)) / dy + (TV(i, j) - TV(i - 1, j)) / dx) * dx;
+      val = val - (shN - shS);
+      y[dof] = val;
+    }
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long dof = nu + (long)(j - 1) * nx + (i - 1);
+      const double rho_v = F->rho_v[H(m, i, j)];
+      const double diag_bc = (at * rho_v / dt) * dvol;
+      const int wb = j == 1 && bb && wall(m->bc[2]), wt = j == v_ny && bt && wall(m->bc[3]);
+      if (wb || wt) {
+        y[dof] = diag_bc * TV(i, j);
+        continue;
+      }
+      double val = (at * rho_v / dt) * TV(i, j) * dvol;
+      if (c->enable_convection) {
+        double cE, cW, cN, cS;
+        v_conv_fluxes(m, c, F, i, j, &cE, &cW, &cN, &cS);
+        const int has_ee = i + 2 <= nx + 1, has_ww = i - 2 >= 0, has_nn = j + 2 <= ny + 1, has_ss = j - 2 >= 0;
+        const double q_ee = has_ee ? v[H(m, i + 2, j)] : 0.0, q_ww = has_ww ? v[H(m, i - 2, j)] : 0.0;
+        const double q_nn = has_nn ? v[H(m, i, j + 2)] : 0.0, q_ss = has_ss ? v[H(m, i, j - 2)] : 0.0;
+        const double pE = conv_face_flux(sch, cE, TV(i, j), TV(i + 1, j), v[H(m, i - 1, j)], v[H(m, i, j)],
+                                         v[H(m, i + 1, j)], q_ee, 1, has_ee);
+        const double pW = conv_face_flux(sch, cW, TV(i - 1, j), TV(i, j), q_ww, v[H(m, i - 1, j)], v[H(m, i, j)],
+                                         v[H(m, i + 1, j)], has_ww, 1);
+        const double pN = conv_face_flux(sch, cN, TV(i, j), TV(i, j + 1), v[H(m, i, j - 1)], v[H(m, i, j)],
+                                         v[H(m, i, j + 1)], q_nn, 1, has_nn);
+        const double pS = conv_face_flux(sch, cS, TV(i, j - 1), TV(i, j), q_ss, v[H(m, i, j - 1)], v[H(m, i, j)],
+                                         v[H(m, i, j + 1)], has_ss, 1);
+        val = val + rho_v * (pE - pW + pN - pS);
+        if (c->conv_lin == 3) {
+          const double dv_dx = (v[H(m, i + 1, j)] - v[H(m, i - 1, j)]) / (2.0 * dx);
+          const double dv_dy = (v[H(m, i, j + 1)] - v[H(m, i, j - 1)]) / (2.0 * dy);
+          const double tuf = 0.25 * (TU(i, j - 1) + TU(i + 1, j - 1) + TU(i, j) + TU(i + 1, j));
+          val = val + rho_v * (tuf * dv_dx + TV(i, j) * dv_dy) * dvol;
+        }
+      }
+      const double mu_N = F->mu[H(m, i, j)], mu_S = F->mu[H(m, i, j - 1)];
+      const double deN = 2.0 * mu_N * (TV(i, j + 1) - TV(i, j)) / dy * dx;
+      const double deS = 2.0 * mu_S * (TV(i, j) - TV(i, j - 1)) / dy * dx;
+      val = val - (deN - deS);
+      const double mnE = F->mu_node[H(m, i + 1, j)], mnW = F->mu_node[H(m, i, j)];
+      const double shE = mnE * ((TV(i + 1, j) - TV(i, j)) / dx + (TU(i + 1, j) - TU(i + 1, j - 1)) / dy) * dy;
+      const double shW = mnW * ((TV(i, j) - TV(i - 1, j)) / dx + (TU(i, j) - TU(i, j - 1)) / dy) * dy;
+      val = val - (shE - shW);
+      y[dof] = val;
+    }
+#undef TU
+#undef TV
+}
+
+/* ns_assemble_preconditioner_blocks (src/module_navier_stokes.f90:1268-1595):
+ * the diagonals diag_u, diag_v (Jacobi) of the SP momentum block. */
+void orc_mom_diag(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  double *diag) {
+  const int nx = m->nx, ny = m->ny;
+  const double dx = m->dx, dy = m->dy, dvol = dx * dy;
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  const long nu = (long)u_nx * ny;
+  const double dt_safe = dt > 1.0e-30 ? dt : 1.0;
+  const double at = gear(c) ? 1.5 : 1.0;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) {
+      const long dof = (long)(j - 1) * u_nx + (i - 1);
+      const double rho_u = F->rho_u[H(m, i, j)];
+      double Du = (at * rho_u / dt_safe) * dvol;
+      if (Du <= 1.0e-30) Du = 1.0;
+      if ((i == 1 && bl) || (i == u_nx && br)) {
+        diag[dof] = Du;
+        continue;
+      }
+      const double mu_e = F->mu[H(m, i, j)], mu_w = F->mu[H(m, i - 1, j)];
+      const double cE = 2.0 * mu_e * (dy / dx), cW = 2.0 * mu_w * (dy / dx);
+      const double cN_raw = F->mu_node[H(m, i, j + 1)] * (dx / dy), cS_raw = F->mu_node[H(m, i, j)] * (dx / dy);
+      double cN_visc = cN_raw, cS_visc = cS_raw, dwy = 0.0;
+      if (j == 1 && bb) {
+        if (m->bc[2] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cS_raw : dwy + 2.0 * cS_raw;
+        cS_visc = 0.0;
+      }
+      if (j == ny && bt) {
+        if (m->bc[3] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cN_raw : dwy + 2.0 * cN_raw;
+        cN_visc = 0.0;
+      }
+      double conv_diag = 0.0, newt = 0.0;
+      if (c->enable_convection) {
+        double FE, FW, FN, FS;
+        u_conv_fluxes(m, c, F, i, j, &FE, &FW, &FN, &FS);
+        conv_diag = rho_u * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+        if (c->conv_lin == 3) {
+          const double du_dx = (F->u[H(m, i + 1, j)] - F->u[H(m, i - 1, j)]) / (2.0 * dx);
+          newt = rho_u * dmax(0.0, du_dx) * dvol;
+        }
+      }
+      double aP = Du + dwy + conv_diag + 0.0 + newt + cW + cE + cS_visc + cN_visc + 0.0 + 0.0;
+      if (aP <= 1.0e-15) aP = 1.0;
+      diag[dof] = aP;
+    }
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long dof = nu + (long)(j - 1) * nx + (i - 1);
+      const double rho_v = F->rho_v[H(m, i, j)];
+      double Dv = (at * rho_v / dt_safe) * dvol;
+      if (Dv <= 1.0e-30) Dv = 1.0;
+      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+      if (is_wall || (j == 1 && bb && m->bc[2] == BC_NEUMANN && v_ny >= 2) ||
+          (j == v_ny && bt && m->bc[3] == BC_NEUMANN && v_ny >= 2)) {
+        diag[dof] = Dv;
+        continue;
+      }
+      const double mu_n = F->mu[H(m, i, j)], mu_s = F->mu[H(m, i, j - 1)];
+      const double cN = 2.0 * mu_n * (dx / dy), cS = 2.0 * mu_s * (dx / dy);
+      const double cE_raw = F->mu_node[H(m, i + 1, j)] * (dy / dx), cW_raw = F->mu_node[H(m, i, j)] * (dy / dx);
+      double cE_visc = cE_raw, cW_visc = cW_raw, dwx = 0.0;
+      if (i == 1 && bl) {
+        if (m->bc[0] == BC_DIRICHLET) dwx = dwx + 2.0 * cW_raw;
+        cW_visc = 0.0;
+      }
+      if (i == nx && br) {
+        if (m->bc[1] == BC_DIRICHLET) dwx = dwx + 2.0 * cE_raw;
+        cE_visc = 0.0;
+      }
+      double conv_diag = 0.0, newt = 0.0;
+      if (c->enable_convection) {
+        double FE, FW, FN, FS;
+        v_conv_fluxes(m, c, F, i, j, &FE, &FW, &FN, &FS);
+        conv_diag = rho_v * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+        if (c->conv_lin == 3) {
+          const double dv_dy = (F->v[H(m, i, j + 1)] - F->v[H(m, i, j - 1)]) / (2.0 * dy);
+          newt = rho_v * dmax(0.0, dv_dy) * dvol;
+        }
+      }
+      double aP = Dv + dwx + conv_diag + 0.0 + newt + cW_visc + cE_visc + cS + cN + 0.0 + 0.0;
+      if (aP <= 1.0e-15) aP = 1.0;
+      diag[dof] = aP;
+    }
+}
+
+/* ns_assemble_rhs (src/module_navier_stokes.f90:785-1101), SP branch. */
+void orc_mom_rhs(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                 double *rhs) {
+  const int nx = m->nx, ny = m->ny;
+  const double dx = m->dx, dy = m->dy, dvol = dx * dy;
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  const long nu = (long)u_nx * ny;
+  const double at = gear(c) ? 1.5 : 1.0;
+  const double *u = F->u, *v = F->v;
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) {
+      const long dof = (long)(j - 1) * u_nx + (i - 1);
+      const double rho_u = F->rho_u[H(m, i, j)];
+      const double diag_bc = (at * rho_u / dt) * dvol;
+      if (i == 1 && bl) {
+        rhs[dof] = diag_bc * (m->bc[0] == BC_DIRICHLET ? c->bc_u[0] : 0.0);
+        continue;
+      }
+      if (i == u_nx && br) {
+        rhs[dof] = diag_bc * (m->bc[1] == BC_DIRICHLET ? c->bc_u[1] : 0.0);
+        continue;
+      }
+      double val;
+      if (gear(c)) {
+        const double ui = rho_u * (2.0 * F->u_old[H(m, i, j)] - 0.5 * F->u_nm1[H(m, i, j)]);
+        val = (ui / dt) * dvol;
+      } else {
+        val = (rho_u / dt) * F->u_old[H(m, i, j)] * dvol;
+      }
+      val = val + rho_u * c->gx * dvol;
+      val = val + F->fx[H(m, i, j)] * dvol;
+      if (c->conv_lin == 3) {
+        const double du_dx = (u[H(m, i + 1, j)] - u[H(m, i - 1, j)]) / (2.0 * dx);
+        const double du_dy = (u[H(m, i, j + 1)] - u[H(m, i, j - 1)]) / (2.0 * dy);
+        const double vf = 0.25 * (v[H(m, i - 1, j)] + v[H(m, i, j)] + v[H(m, i - 1, j + 1)] + v[H(m, i, j + 1)]);
+        val = val + rho_u * (u[H(m, i, j)] * du_dx + vf * du_dy) * dvol;
+      }
+      if (j == ny && bt && m->bc[3] == BC_DIRICHLET) {
+        const double mu_N = F->mu_node[H(m, i, j + 1)];
+        val = ny >= 2 ? val + mu_N * ((8.0 / 3.0) * c->bc_u[3] / dy) * dx
+                      : val + mu_N * (2.0 * c->bc_u[3] / dy) * dx;
+      }
+      if (j == 1 && bb && m->bc[2] == BC_DIRICHLET) {
+        const double mu_S = F->mu_node[H(m, i, j)];
+        val = ny >= 2 ? val + mu_S * ((8.0 / 3.0) * c->bc_u[2] / dy) * dx
+                      : val + mu_S * (2.0 * c->bc_u[2] / dy) * dx;
+      }
+      rhs[dof] = val;
+    }
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= nx; ++i) {
+      const long dof = nu + (long)(j - 1) * nx + (i - 1);
+      const double rho_v = F->rho_v[H(m, i, j)];
+      const double diag_bc = (at * rho_v / dt) * dvol;
+      if (j == 1 && bb && wall(m->bc[2])) {
+        rhs[dof] = diag_bc * (m->bc[2] == BC_DIRICHLET ? c->bc_v[2] : 0.0);
+        continue;
+      }
+      if (j == v_ny && bt && wall(m->bc[3])) {
+        rhs[dof] = diag_bc * (m->bc[3] == BC_DIRICHLET ? c->bc_v[3] : 0.0);
+        continue;
+      }
+      double val;
+      if (gear(c)) {
+        const double vi = rho_v * (2.0 * F->v_old[H(m, i, j)] - 0.5 * F->v_nm1[H(m, i, j)]);
+        val = (vi / dt) * dvol;
+      } else {
+        val = (rho_v / dt) * F->v_old[H(m, i, j)] * dvol;
+      }
+      val = val + rho_v * c->gy * dvol;
+      val = val + F->fy[H(m, i, j)] * dvol;
+      if (c->conv_lin == 3) {
+        const double dv_dx = (v[H(m, i + 1, j)] - v[H(m, i - 1, j)]) / (2.0 * dx);
+        const double dv_dy = (v[H(m, i, j + 1)] - v[H(m, i, j - 1)]) / (2.0 * dy);
+        const double uf = 0.25 * (u[H(m, i, j - 1)] + u[H(m, i + 1, j - 1)] + u[H(m, i, j)] + u[H(m, i + 1, j)]);
+        val = val + rho_v * (uf * dv_dx + v[H(m, i, j)] * dv_dy) * dvol;
+      }
+      if (i == 1 && bl && m->bc[0] == BC_DIRICHLET) {
+        const double mu_W = F->mu_node[H(m, i, j)];
+        val = val + mu_W * (2.0 * c->bc_v[0] / dx) * dy;
+      }
+      if (i == nx && br && m->bc[1] == BC_DIRICHLET) {
+        const double mu_E = F->mu_node[H(m, i + 1, j)];
+        val = val + mu_E * (2.0 * c->bc_v[1] / dx) * dy;
+      }
+      rhs[dof] = val;
+    }
+}
+
+/* apply_preconditioner NONE / JACOBI (src/module_linear_solvers.f90:88-135). */
+static void mom_precond(int precond, const double *diag, const double *r, double *z, long n) {
+  if (precond == 1) {
+    for (long k = 0; k < n; ++k) z[k] = fabs(diag[k]) > 1.0e-15 ? r[k] / diag[k] : r[k];
+  } else {
+    memcpy(z, r, n * sizeof(double));
+  }
+}
+
+/* solve_cg (src/module_linear_solvers.f90:735-838) and solve_bicgstab2
+ * (:361-579) on the momentum operator; returns the reference's iters. */
+int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  const double *diag, double *tu, double *tv, int lin_solver, int precond, double tol,
+                  int maxit, const double *b, double *x, double *relres) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long n = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  double *w[9];
+  for (int k = 0; k < 9; ++k) w[k] = calloc(n, sizeof(double));
+  int iters;
+  double rr = 0.0;
+  if (lin_solver == 3) {
+    double *Ax = w[0], *r = w[1], *z = w[2], *p = w[3], *Ap = w[4];
+    iters = maxit;
+    orc_mom_matvec(m, c, F, dt, tu, tv, x, Ax);
+    for (long k = 0; k < n; ++k) r[k] = b[k] - Ax[k];
+    const double r0 = sqrt(orc_dot(r, r, n));
+    if (r0 < 1.0e-12) {
+      iters = 0;
+    } else {
+      mom_precond(precond, diag, r, z, n);
+      memcpy(p, z, n * sizeof(double));
+      double rz_old = orc_dot(r, z, n);
+      for (int it = 1; it <= maxit; ++it) {
+        orc_mom_matvec(m, c, F, dt, tu, tv, p, Ap);
+        const double pAp = orc_dot(p, Ap, n);
+        if (fabs(pAp) < 1.0e-30) break;
+        const double alpha = rz_old / pAp;
+        for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * p[k];
+        for (long k = 0; k < n; ++k) r[k] = r[k] - alpha * Ap[k];
+        const double rn = sqrt(orc_dot(r, r, n));
+        rr = rn / r0;
+        if (rn / r0 < tol || rn < 1.0e-13) {
+          iters = it;
+          break;
+        }
+        mom_precond(precond, diag, r, z, n);
+        const double rz_new = orc_dot(r, z, n);
+        const double beta = rz_new / rz_old;
+        rz_old = rz_new;
+        for (long k = 0; k < n; ++k) p[k] = z[k] + beta * p[k];
+      }
+    }
+  } else {
+    double *bh = w[0], *rt = w[1], *r0 = w[2], *r1 = w[3], *r2 = w[4], *u0 = w[5], *u1 = w[6], *u2 = w[7];
+    double *tmp = w[8];
+#define PA(src, dst)                                      \
+  do {                                                    \
+    orc_mom_matvec(m, c, F, dt, tu, tv, src, tmp);        \
+    mom_precond(precond, diag, tmp, dst, n);              \
+  } while (0)
+    iters = 0;
+    mom_precond(precond, diag, b, bh, n);
+    PA(x, r0);
+    for (long k = 0; k < n; ++k) r0[k] = bh[k] - r0[k];
+    const double r0n = sqrt(orc_dot(r0, r0, n));
+    int iter = 0;
+    if (r0n >= 1.0e-12) {
+      memcpy(rt, r0, n * sizeof(double));
+      double rho_0 = 1.0, rho_1, alpha = 0.0, omega = 1.0, beta, gamma;
+      while (iter < maxit) {
+        iter += 2;
+        rho_0 = -omega * rho_0;
+        rho_1 = orc_dot(rt, r0, n);
+        if (fabs(rho_0) < 1.0e-300) break;
+        beta = alpha * (rho_1 / rho_0);
+        rho_0 = rho_1;
+        for (long k = 0; k < n; ++k) u0[k] = r0[k] - beta * u0[k];
+        PA(u0, u1);
+        gamma = orc_dot(rt, u1, n);
+        if (fabs(gamma) < 1.0e-300) break;
+        alpha = rho_0 / gamma;
+        for (long k = 0; k < n; ++k) r0[k] = r0[k] - alpha * u1[k];
+        for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * u0[k];
+        PA(r0, r1);
+        rho_1 = orc_dot(rt, r1, n);
+        if (fabs(rho_0) < 1.0e-300) break;
+        beta = alpha * (rho_1 / rho_0);
+        rho_0 = rho_1;
+        for (long k = 0; k < n; ++k) u0[k] = r0[k] - beta * u0[k];
+        for (long k = 0; k < n; ++k) u1[k] = r1[k] - beta * u1[k];
+        PA(u1, u2);
+        gamma = orc_dot(rt, u2, n);
+        if (fabs(gamma) < 1.0e-300) break;
+        alpha = rho_0 / gamma;
+        for (long k = 0; k < n; ++k) x[k] = x[k] + alpha * u0[k];
+        for (long k = 0; k < n; ++k) r0[k] = r0[k] - alpha * u1[k];
+        for (long k = 0; k < n; ++k) r1[k] = r1[k] - alpha * u2[k];
+        PA(r1, r2);
+        const double s11 = orc_dot(r1, r1, n), s12 = orc_dot(r2, r1, n), s22 = orc_dot(r2, r2, n);
+        const double rhs1 = orc_dot(r0, r1, n), rhs2 = orc_dot(r0, r2, n);
+        const double rn1 = sqrt(s11), rn2 = sqrt(s22);
+        double g1, g2;
+        if (rn1 < 1.0e-300) {
+          break;
+        } else if (rn2 < 1.0e-300) {
+          g2 = 0.0;
+          g1 = rhs1 / s11;
+        } else {
+          const double i1 = 1.0 / rn1, i2 = 1.0 / rn2;
+          const double c12 = s12 * i1 * i2, b1 = rhs1 * i1, b2 = rhs2 * i2;
+          const double det = 1.0 - c12 * c12;
+          if (fabs(det) < 1.0e-14) {
+            g2 = 0.0;
+            g1 = b1 * i1;
+          } else {
+            g1 = (b1 - c12 * b2) / (det * rn1);
+            g2 = (b2 - c12 * b1) / (det * rn2);
+          }
+        }
+        for (long k = 0; k < n; ++k) x[k] = x[k] + g1 * r0[k] + g2 * r1[k];
+        for (long k = 0; k < n; ++k) r0[k] = r0[k] - (g1 * r1[k] + g2 * r2[k]);
+        for (long k = 0; k < n; ++k) u0[k] = u0[k] - (g1 * u1[k] + g2 * u2[k]);
+        if (g2 != 0.0) omega = g2;
+        else if (fabs(g1) > 1.0e-300) omega = g1;
+        else omega = 1.0;
+        const double rn = sqrt(orc_dot(r0, r0, n));
+        rr = rn / r0n;
+        if (rn != rn || rn > 1.0e30) break;
+        if (rn / r0n < tol || rn < 1.0e-12) break;
+      }
+    }
+    iters = iter;
+#undef PA
+  }
+  if (relres) *relres = rr;
+  for (int k = 0; k < 9; ++k) free(w[k]);
+  return iters;
+}
+
+/* The momentum block of sim_solve_standard_projection
+ * (src/module_simulation.f90:2454-2469): diagonal, right-hand side, pack
+ * (initial guess = the current u, v), solve, unpack_vector(.false.), then
+ * sync_u, sync_v and apply_boundary_conditions(.false.) once more.  u, v are
+ * updated in place (the reference's fields%u / fields%v, which are also the
+ * fields_old of the operator during the solve: F->u, F->v must be copies). */
+int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                     double *tu, double *tv, int lin_solver, int precond, double tol, int maxit,
+                     double *u, double *v, double *relres) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long n = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  double *diag = malloc(n * sizeof(double)), *rhs = malloc(n * sizeof(double));
+  double *x = malloc(n * sizeof(double));
+  orc_mom_diag(m, c, F, dt, diag);
+  orc_mom_rhs(m, c, F, dt, rhs);
+  orc_mom_pack(m, u, v, x);
+  const int it = orc_mom_solve(m, c, F, dt, diag, tu, tv, lin_solver, precond, tol, maxit, rhs, x, relres);
+  orc_mom_unpack(m, c, x, u, v, 0);
+  orc_exchange_halos_scalar(m, u);
+  orc_exchange_halos_scalar(m, v);
+  orc_velocity_bc(m, c, u, v, 0);
+  free(diag);
+  free(rhs);
+  free(x);
+  return it;
+}

@@ -89,6 +89,44 @@ void orc_sp_finish_pressure(const orc_mesh *m, const double *x, double *p);
 void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
                     const double *rho_v, double dt, double *u, double *v);
 
+/* SP velocity predictor (src/module_navier_stokes.f90, src/module_linear_solvers.f90,
+ * src/module_simulation.f90:2454-2469).  Supported subset: standard projection,
+ * no obstacles, UNIFORM boundary profiles, no energy equation, no SSP-RK3.
+ * Unknown vector: u(1:u_nx,1:ny) then v(1:nx,1:v_ny), i fastest, with
+ * u_nx = nx + 1 (nx if periodic in x) and v_ny = ny + 1 (ny if periodic in y). */
+typedef struct {
+  int time_scheme;        /* 1 EULER, 2 GEAR (BDF2) */
+  int has_nm1;            /* fields%has_nm1 */
+  int enable_convection;
+  int conv_scheme;        /* 1 UPWIND, 2 CENTERED, 3 TVD_SUPERBEE */
+  int conv_lin;           /* 1 PICARD, 2 ADAMS_BASHFORTH, 3 NEWTON */
+  double gx, gy;
+  double bc_u[4], bc_v[4];  /* Dirichlet values: left, right, bottom, top */
+} orc_mom_cfg;
+typedef struct {
+  const double *u, *v, *u_old, *v_old, *u_nm1, *v_nm1;
+  const double *rho_u, *rho_v, *mu, *mu_node, *fx, *fy;  /* fx/fy: fx_sigma, fy_sigma */
+} orc_mom_fields;
+void orc_velocity_bc(const orc_mesh *m, const orc_mom_cfg *c, double *u, double *v, int homogeneous);
+void orc_mom_sizes(const orc_mesh *m, int *u_nx, int *v_ny);
+void orc_mom_pack(const orc_mesh *m, const double *u, const double *v, double *vec);
+void orc_mom_unpack(const orc_mesh *m, const orc_mom_cfg *c, const double *vec, double *u, double *v,
+                    int homogeneous);
+/* tu, tv: persistent halo work fields (the reference's SAVEd `tf`), zero at first use. */
+void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                    double *tu, double *tv, const double *x, double *y);
+void orc_mom_diag(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  double *diag);
+void orc_mom_rhs(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                 double *rhs);
+/* lin_solver 1 BICGSTAB2, 3 CG; precond 0 NONE, 1 JACOBI.  Returns iters. */
+int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  const double *diag, double *tu, double *tv, int lin_solver, int precond, double tol,
+                  int maxit, const double *b, double *x, double *relres);
+int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                     double *tu, double *tv, int lin_solver, int precond, double tol, int maxit,
+                     double *u, double *v, double *relres);
+
 /* Serial column-major sum of x*y (the reference's sum(x*y)). */
 double orc_dot(const double *x, const double *y, long n);
 

@@ -24,6 +24,17 @@ BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
 PLIC, ELVIRA = 0, 1
 
 
+class MomCfg(ct.Structure):
+    _fields_ = [("time_scheme", ct.c_int), ("has_nm1", ct.c_int), ("enable_convection", ct.c_int),
+                ("conv_scheme", ct.c_int), ("conv_lin", ct.c_int), ("gx", ct.c_double), ("gy", ct.c_double),
+                ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4)]
+
+
+class MomFields(ct.Structure):
+    _fields_ = [(k, ct.POINTER(ct.c_double)) for k in
+                ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node", "fx", "fy")]
+
+
 class Mesh(ct.Structure):
     _fields_ = [("nx", ct.c_int), ("ny", ct.c_int), ("lx", ct.c_double), ("ly", ct.c_double),
                 ("dx", ct.c_double), ("dy", ct.c_double), ("bc", ct.c_int * 4)]
@@ -68,6 +79,17 @@ def lib():
             "orc_sp_correct": (None, [M, _D, _D, _D, d, _D, _D]),
             "orc_dot": (d, [_D, _D, ct.c_long]),
             "orc_bench_jacobi_pcg": (d, [M] + [_D] * 6 + [i, i]),
+            "orc_velocity_bc": (None, [M, ct.POINTER(MomCfg), _D, _D, i]),
+            "orc_mom_sizes": (None, [M, ip, ip]),
+            "orc_mom_pack": (None, [M, _D, _D, _D]),
+            "orc_mom_unpack": (None, [M, ct.POINTER(MomCfg), _D, _D, _D, i]),
+            "orc_mom_matvec": (None, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, _D, _D]),
+            "orc_mom_diag": (None, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D]),
+            "orc_mom_rhs": (None, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D]),
+            "orc_mom_solve": (i, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, _D, i, i, d, i,
+                                  _D, _D, _D]),
+            "orc_sp_predictor": (i, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, i, i, d, i,
+                                     _D, _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -210,3 +232,71 @@ def sp_correct(m: Mesh, p, rho_u, rho_v, dt, u, v):
 
 def bench_jacobi_pcg(m: Mesh, co, b, iters: int, threads: int) -> float:
     return lib().orc_bench_jacobi_pcg(ct.byref(m), *_co(co), _p(b), iters, threads)
+
+
+# ------------------------------------------------------- SP velocity predictor
+
+MOM_FIELDS = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node", "fx", "fy")
+
+
+class Momentum:
+    """The SP momentum block on one mesh: fields (dict of halo arrays named as
+    MOM_FIELDS), options (dict: time_scheme, has_nm1, enable_convection,
+    conv_scheme, conv_lin, gx, gy, bc_u, bc_v) and the persistent work fields
+    of the reference's SAVEd operator state (zero at construction)."""
+
+    def __init__(self, m: Mesh, fields: dict, opts: dict, dt: float):
+        self.m, self.dt = m, float(dt)
+        self.f = {k: np.ascontiguousarray(fields[k], dtype=np.float64) for k in MOM_FIELDS}
+        self.cfg = MomCfg(int(opts.get("time_scheme", 1)), int(opts.get("has_nm1", 0)),
+                          int(opts.get("enable_convection", 1)), int(opts.get("conv_scheme", 1)),
+                          int(opts.get("conv_lin", 1)), float(opts.get("gx", 0.0)), float(opts.get("gy", 0.0)),
+                          (ct.c_double * 4)(*opts.get("bc_u", (0.0,) * 4)),
+                          (ct.c_double * 4)(*opts.get("bc_v", (0.0,) * 4)))
+        self.F = MomFields(*[_p(self.f[k]) for k in MOM_FIELDS])
+        shp = (m.ny + 2, m.nx + 2)
+        self.tu, self.tv = np.zeros(shp), np.zeros(shp)
+        a, b = ct.c_int(), ct.c_int()
+        lib().orc_mom_sizes(ct.byref(m), ct.byref(a), ct.byref(b))
+        self.u_nx, self.v_ny = a.value, b.value
+        self.n = self.u_nx * m.ny + m.nx * self.v_ny
+
+    def diag(self):
+        d = np.empty(self.n)
+        lib().orc_mom_diag(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(d))
+        return d
+
+    def rhs(self):
+        r = np.empty(self.n)
+        lib().orc_mom_rhs(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(r))
+        return r
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n)
+        lib().orc_mom_matvec(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
+                             _p(self.tu), _p(self.tv), _p(x), _p(y))
+        return y
+
+    def pack(self, u, v):
+        x = np.empty(self.n)
+        lib().orc_mom_pack(ct.byref(self.m), _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(v)), _p(x))
+        return x
+
+    def solve(self, b, x0, lin_solver=1, precond=1, tol=1e-10, maxit=1000):
+        x = np.array(x0, dtype=np.float64)
+        d = self.diag()
+        rr = ct.c_double()
+        it = lib().orc_mom_solve(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(d),
+                                 _p(self.tu), _p(self.tv), lin_solver, precond, tol, maxit,
+                                 _p(np.ascontiguousarray(b, dtype=np.float64)), _p(x), ct.byref(rr))
+        return x, it, rr.value
+
+    def predictor(self, lin_solver=1, precond=1, tol=1e-10, maxit=1000):
+        """Returns (u, v, iters, relres): the predicted velocity (halo arrays)."""
+        u, v = self.f["u"].copy(), self.f["v"].copy()
+        rr = ct.c_double()
+        it = lib().orc_sp_predictor(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
+                                    _p(self.tu), _p(self.tv), lin_solver, precond, tol, maxit,
+                                    _p(u), _p(v), ct.byref(rr))
+        return u, v, it, rr.value

@@ -21,6 +21,13 @@
 !           curvature and CSF forces (src/module_interface_vof.f90:387-476,
 !           :627-789, :792-825, :1416-1461); C, C_s, C_tilde, curvature,
 !           fx_sigma, fy_sigma after every step
+!   momentum the momentum block of the standard projection
+!           (src/module_simulation.f90:2454-2469): diag_u/diag_v of
+!           assemble_preconditioner_blocks, assemble_rhs, two products of
+!           apply_system_matrix, then pack / solve_linear_system / unpack and
+!           the trailing sync + apply_boundary_conditions; u, v and iters
+!           (src/module_navier_stokes.f90:231-1101, :1268-1595;
+!           src/module_linear_solvers.f90:361-579, :735-838, :1167-1208)
 !
 ! Usage: covof_ref_harness <mode> <input.bin> <output.bin>
 program covof_ref_harness
@@ -32,6 +39,9 @@ program covof_ref_harness
   use module_interface_reconstruction, only: interface_segment_t, cut_box_area, &
        compute_alpha, reconstruct_interface
   use module_interface_vof, only: advect_vof, update_physical_properties
+  use module_navier_stokes, only: navier_stokes_t
+  use module_linear_solvers, only: solve_linear_system
+  use module_mpi_comm, only: sync_u, sync_v, apply_boundary_conditions
   implicit none
   include 'mpif.h'
 
@@ -54,6 +64,8 @@ program covof_ref_harness
      call run_advect()
   case ('surface')
      call run_surface()
+  case ('momentum')
+     call run_momentum()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
@@ -190,5 +202,65 @@ contains
        write(uout) fields%C, fields%C_s, fields%C_tilde, fields%curvature, fields%fx_sigma, fields%fy_sigma
     end do
   end subroutine run_surface
+
+  subroutine run_momentum()
+    integer :: nx, ny, bc(4), tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit, iters
+    real(dp) :: lx, ly, dt, gx, gy, tol, bcu(4), bcv(4)
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    type(navier_stokes_t) :: ns
+    real(dp), allocatable :: rhs(:), x1(:), x2(:), y1(:), y2(:), state(:)
+    read(uin) nx, ny, bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit
+    read(uin) lx, ly, dt, gx, gy, tol, bcu, bcv
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%solver_type = SOLVER_STANDARD_PROJECTION
+    cfg%time_scheme = tscheme
+    cfg%enable_convection = (conv /= 0)
+    cfg%convective_scheme = scheme
+    cfg%convective_linearization = lin
+    cfg%gx = gx
+    cfg%gy = gy
+    cfg%linear_solver = solver
+    cfg%precond_type = precond
+    cfg%use_precond = .true.
+    cfg%tol = tol
+    cfg%max_iter = maxit
+    cfg%obs_enabled = .false.
+    cfg%enable_energy = .false.
+    cfg%enable_ssp_rk3_momentum = .false.
+    cfg%compressible = .false.
+    cfg%bc_left%u = bcu(1)
+    cfg%bc_right%u = bcu(2)
+    cfg%bc_bottom%u = bcu(3)
+    cfg%bc_top%u = bcu(4)
+    cfg%bc_left%v = bcv(1)
+    cfg%bc_right%v = bcv(2)
+    cfg%bc_bottom%v = bcv(3)
+    cfg%bc_top%v = bcv(4)
+    call fields%allocate(mesh)
+    call ns%init(mesh, cfg)
+    read(uin) fields%u, fields%v, fields%u_old, fields%v_old, fields%u_nm1, fields%v_nm1
+    read(uin) fields%rho_u, fields%rho_v, fields%mu, fields%mu_node, fields%fx_sigma, fields%fy_sigma
+    fields%has_nm1 = (has_nm1 /= 0)
+    allocate(rhs(ns%n_owned_total), x1(ns%n_owned_total), x2(ns%n_owned_total))
+    allocate(y1(ns%n_owned_total), y2(ns%n_owned_total), state(ns%n_owned_total))
+    read(uin) x1, x2
+    call ns%assemble_preconditioner_blocks(mesh, cfg, fields, dt)
+    write(uout) ns%diag_u, ns%diag_v
+    call ns%assemble_rhs(mesh, cfg, fields, dt, rhs)
+    write(uout) rhs
+    call ns%apply_system_matrix(mesh, cfg, fields, dt, x1, y1)
+    call ns%apply_system_matrix(mesh, cfg, fields, dt, x2, y2)
+    write(uout) y1, y2
+    call ns%pack_vector(mesh, cfg, fields, state)
+    call solve_linear_system(ns, mesh, cfg, fields, dt, rhs, state, iters)
+    call ns%unpack_vector(mesh, cfg, state, fields, .false.)
+    call sync_u(mesh, fields%u)
+    call sync_v(mesh, fields%v)
+    call apply_boundary_conditions(mesh, cfg, fields, .false.)
+    write(uout) fields%u, fields%v
+    write(uout) iters
+  end subroutine run_momentum
 
 end program covof_ref_harness

@@ -7,7 +7,7 @@ compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface)
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum)
 """
 from __future__ import annotations
 
@@ -146,12 +146,95 @@ def surface_cases():
     surface("off_periodic_y", Cb, z, z, nx, ny, (FS, FS, P, P), 1, 0.0, 1, tension=0)
 
 
+MOM_NAMES = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node", "fx", "fy")
+
+
+def momentum_fields(nx, ny, seed, rho_ratio=10.0, mu_ratio=10.0):
+    """Seeded inputs of the momentum block: two-phase-like densities and
+    viscosities from a disc, smooth velocities, small CSF-like forces."""
+    rng = np.random.default_rng(seed)
+    C = wl.disc(nx, ny, 0.5, 0.45, 0.25)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    rho = C * rho_ratio + (1.0 - C) * 1.0
+    mu = C * (0.01 * mu_ratio) + (1.0 - C) * 0.01
+    f = {}
+    f["rho_u"] = rho.copy()
+    f["rho_u"][:, 1:] = 0.5 * (rho[:, :-1] + rho[:, 1:])
+    f["rho_v"] = rho.copy()
+    f["rho_v"][1:, :] = 0.5 * (rho[:-1, :] + rho[1:, :])
+    f["mu"] = mu
+    f["mu_node"] = np.full_like(mu, 0.01)
+    f["mu_node"][1:, 1:] = 0.25 * (mu[:-1, :-1] + mu[1:, :-1] + mu[:-1, 1:] + mu[1:, 1:])
+    for k in ("u", "v"):
+        f[k] = rng.standard_normal((ny + 2, nx + 2))
+        f[k + "_old"] = f[k] + 0.05 * rng.standard_normal((ny + 2, nx + 2))
+        f[k + "_nm1"] = f[k] + 0.1 * rng.standard_normal((ny + 2, nx + 2))
+    f["fx"] = 0.3 * rng.standard_normal((ny + 2, nx + 2))
+    f["fy"] = 0.3 * rng.standard_normal((ny + 2, nx + 2))
+    return f
+
+
+def momentum(name, nx, ny, bc, f, dt, tscheme=1, has_nm1=0, conv=1, scheme=1, lin=1, solver=1, precond=1,
+             maxit=400, tol=1e-12, gx=0.0, gy=-9.81, bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=1.0, seed=0):
+    """The SP momentum block: diagonal, right-hand side, two operator
+    products, and the predicted velocity after the solve."""
+    u_nx = nx + (0 if bc[0] == P else 1)
+    v_ny = ny + (0 if bc[2] == P else 1)
+    n = u_nx * ny + nx * v_ny
+    rng = np.random.default_rng(seed + 1000)
+    x1, x2 = rng.standard_normal(n), rng.standard_normal(n)
+    payload = (i32(nx, ny, *bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit)
+               + f64(lx, ly, dt, gx, gy, tol, *bcu, *bcv)
+               + b"".join(f[k].astype("<f8").tobytes() for k in MOM_NAMES) + x1.astype("<f8").tobytes()
+               + x2.astype("<f8").tobytes())
+    raw = run("momentum", payload)
+    h = (ny + 2) * (nx + 2)
+    out = np.frombuffer(raw[:-4], dtype="<f8")
+    iters = int(np.frombuffer(raw[-4:], dtype="<i4")[0])
+    o = 0
+    diag = out[o:o + n]; o += n
+    rhs = out[o:o + n]; o += n
+    y1 = out[o:o + n]; o += n
+    y2 = out[o:o + n]; o += n
+    u_out = out[o:o + h].reshape(ny + 2, nx + 2); o += h
+    v_out = out[o:o + h].reshape(ny + 2, nx + 2); o += h
+    assert o == out.size
+    np.savez_compressed(os.path.join(HERE, f"momentum_{name}.npz"), nx=nx, ny=ny, bc=np.array(bc), dt=dt,
+                        tscheme=tscheme, has_nm1=has_nm1, conv=conv, scheme=scheme, lin=lin, solver=solver,
+                        precond=precond, maxit=maxit, tol=tol, gx=gx, gy=gy, bcu=np.array(bcu),
+                        bcv=np.array(bcv), lx=lx, ly=ly, x1=x1, x2=x2, diag=diag, rhs=rhs, y1=y1, y2=y2,
+                        u_out=u_out, v_out=v_out, iters=iters, flags=FLAGS, **{k: f[k] for k in MOM_NAMES})
+
+
+def momentum_cases():
+    # lid-driven box: all walls, moving top lid, Euler, upwind Picard, BiCGSTAB(2) + Jacobi
+    nx, ny = 24, 20
+    momentum("lid_walls", nx, ny, (DI, DI, DI, DI), momentum_fields(nx, ny, 51), 2e-3, bcu=(0.0, 0.0, 0.0, 1.0))
+    # channel: periodic x, no-slip y, BDF2 with history, TVD superbee, Adams-Bashforth, CG + Jacobi
+    nx, ny = 28, 18
+    momentum("channel_bdf2_tvd_ab", nx, ny, (P, P, DI, DI), momentum_fields(nx, ny, 52), 1e-3, tscheme=2,
+             has_nm1=1, scheme=3, lin=2, solver=3, gx=1.0, gy=0.0)
+    # fully periodic, centered Newton, BiCGSTAB(2) without preconditioner
+    n = 20
+    momentum("periodic_newton", n, n, (P, P, P, P), momentum_fields(n, n, 53), 1e-3, scheme=2, lin=3, precond=0)
+    # inflow / outflow: Dirichlet inlet with v, NEUMANN outlet, free-slip bottom, no-slip top
+    nx, ny = 30, 16
+    momentum("inflow_outflow", nx, ny, (DI, NE, FS, DI), momentum_fields(nx, ny, 54), 1e-3, tscheme=2,
+             has_nm1=0, scheme=3, lin=1, bcu=(1.0, 0.0, 0.0, 0.0), bcv=(0.2, 0.0, 0.0, 0.0), lx=2.0)
+    # periodic x over a NEUMANN bottom (reads ghost values the exchange carries over), Stokes (no convection)
+    nx, ny = 22, 26
+    momentum("neumann_bottom_stokes", nx, ny, (P, P, NE, DI), momentum_fields(nx, ny, 55), 5e-3, conv=0,
+             lin=3, solver=3, bcu=(0.0, 0.0, 0.0, 0.5))
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface"}
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum"}
     if "surface" in groups:
         surface_cases()
+    if "momentum" in groups:
+        momentum_cases()
     if not groups & {"geom", "recon", "advect"}:
         print("golden fixtures written to", HERE)
         return

@@ -477,3 +477,82 @@ def test_svof_smoothing_solves_the_helmholtz_filter(orc):
     out = orc.svof_smooth(mw, C, 2.0)
     assert out.min() >= 0.0 and out.max() <= 1.0
     assert 0.0 < out[n // 2, n // 2 + int(0.25 * n)] < 1.0  # the jump is smoothed
+
+
+# ------------------------------------------------------------ momentum block
+
+def _mom_case(orc, nx, ny, bc, opts, seed=0, mu_scale=1.0, lx=1.0, ly=1.0):
+    rng = np.random.default_rng(seed)
+    shp = (ny + 2, nx + 2)
+    f = {k: rng.standard_normal(shp) for k in ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "fx", "fy")}
+    f["rho_u"] = 1.0 + rng.random(shp)
+    f["rho_v"] = 1.0 + rng.random(shp)
+    f["mu"] = mu_scale * (0.01 + 0.02 * rng.random(shp))
+    f["mu_node"] = mu_scale * (0.01 + 0.02 * rng.random(shp))
+    return orc.Momentum(orc.mesh(nx, ny, lx, ly, bc=bc), f, opts, 1e-2), f
+
+
+@pytest.mark.parametrize("bc", [(1, 1, 1, 1), (3, 3, 1, 1), (3, 3, 3, 3), (1, 2, 4, 1)],
+                         ids=["walls", "channel", "periodic", "inflow-outflow"])
+def test_momentum_diag_is_the_operator_diagonal(orc, bc):
+    """Away from the boundary rows, diag_u / diag_v of the preconditioner
+    blocks equal the diagonal entries A e_k of the operator for Stokes and
+    for upwind convection with Picard linearisation (the upwind flux puts
+    rho max(F,0) on the diagonal); the two functions are written apart."""
+    nx, ny = 12, 10
+    for opts in (dict(enable_convection=0), dict(enable_convection=1, conv_scheme=1, conv_lin=1)):
+        M, _ = _mom_case(orc, nx, ny, bc, opts, seed=1)
+        d = M.diag()
+        for k in range(M.n):
+            if k < M.u_nx * ny:
+                i, j = k % M.u_nx + 1, k // M.u_nx + 1
+                inner = 3 <= i <= nx - 2 and 3 <= j <= ny - 2
+            else:
+                kv = k - M.u_nx * ny
+                i, j = kv % nx + 1, kv // nx + 1
+                inner = 3 <= i <= nx - 2 and 3 <= j <= ny - 2
+            if not inner:
+                continue
+            e = np.zeros(M.n)
+            e[k] = 1.0
+            akk = M.matvec(e)[k]
+            assert abs(akk - d[k]) <= 1e-12 * abs(d[k]), (opts, k, akk, d[k])
+
+
+def test_momentum_operator_closed_forms(orc):
+    """Mass term: with zero viscosity and no convection, A x = (rho/dt) dvol x
+    on every interior row; a uniform velocity in a periodic box feels no
+    viscous stress (A x = mass x exactly); the operator is linear in x."""
+    nx, ny = 10, 8
+    M, f = _mom_case(orc, nx, ny, (3, 3, 3, 3), dict(enable_convection=0), seed=2, mu_scale=0.0)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(M.n)
+    y = M.matvec(x)
+    dvol = (1.0 / nx) * (1.0 / ny)
+    mass_u = (f["rho_u"][1:ny + 1, 1:nx + 1] / 1e-2 * dvol).reshape(-1)
+    mass_v = (f["rho_v"][1:ny + 1, 1:nx + 1] / 1e-2 * dvol).reshape(-1)
+    mass = np.concatenate([mass_u, mass_v])
+    assert np.allclose(y, mass * x, rtol=1e-14, atol=0)
+    M2, f2 = _mom_case(orc, nx, ny, (3, 3, 3, 3), dict(enable_convection=0), seed=4)
+    c = np.full(M2.n, 0.7)
+    mass2 = np.concatenate([(f2["rho_u"][1:ny + 1, 1:nx + 1] / 1e-2 * dvol).reshape(-1),
+                            (f2["rho_v"][1:ny + 1, 1:nx + 1] / 1e-2 * dvol).reshape(-1)])
+    assert np.allclose(M2.matvec(c), mass2 * 0.7, rtol=1e-14, atol=0)
+    M3, _ = _mom_case(orc, nx, ny, (1, 2, 4, 1), dict(conv_scheme=3, conv_lin=3), seed=5)
+    a, b = rng.standard_normal(M3.n), rng.standard_normal(M3.n)
+    lhs = M3.matvec(2.0 * a - 3.0 * b)
+    rhs = 2.0 * M3.matvec(a) - 3.0 * M3.matvec(b)
+    assert np.abs(lhs - rhs).max() <= 1e-12 * np.abs(rhs).max()
+
+
+@pytest.mark.parametrize("solver", [1, 3], ids=["bicgstab2", "cg"])
+def test_momentum_solve_vs_dense(orc, solver):
+    """solve_linear_system on the momentum operator against a dense solve of
+    the matrix assembled column by column from the operator."""
+    nx, ny = 9, 7
+    M, _ = _mom_case(orc, nx, ny, (1, 1, 1, 1), dict(enable_convection=0), seed=6)
+    A = np.column_stack([M.matvec(np.eye(M.n)[k]) for k in range(M.n)])
+    b = M.rhs()
+    x, it, _ = M.solve(b, np.zeros(M.n), solver, 1, 1e-13, 4000)
+    xd = np.linalg.solve(A, b)
+    assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max(), it
