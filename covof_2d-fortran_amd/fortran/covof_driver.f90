@@ -9,9 +9,13 @@
 !>   csf  : the tail of advect_vof (src/module_interface_vof.f90:810-824) on a
 !>          walled box holding a disc: smooth_svof_helmholtz (filter width
 !>          1.5), vof_smooth_diffusion (NSTEPS steps) and
-!>          compute_curvature_and_csf with the normals from C_tilde, sigma 0.07.
+!>          compute_curvature_and_csf with the normals from C_tilde, sigma 0.07;
+!>   pred : the momentum block of sim_solve_standard_projection
+!>          (src/module_simulation.f90:2454-2469) on a lid-driven box holding a
+!>          heavy viscous disc (rho 1000 / 100, mu 10 / 1), BDF2 with an n-1
+!>          state, TVD superbee convection, gravity, BiCGSTAB(2) + Jacobi to 1e-10.
 !>
-!> usage: covof_driver vof|sp|csf N NSTEPS OUTFILE
+!> usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE
 !>
 !> OUTFILE (unformatted stream, native endianness):
 !>   int32 mode (1 vof, 2 sp), nx, ny, nsteps, counter (vof: final
@@ -20,6 +24,7 @@
 !>   vof: C0, u, v, C            (each (0:nx+1, 0:ny+1))
 !>   sp : u*, v*, rho_u, rho_v, p, u, v (each (0:nx+1, 0:ny+1))
 !>   csf: C, C_s, C_tilde, curvature, fx_sigma, fy_sigma
+!>   pred: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, u*, v*
 program covof_driver
   use iso_c_binding
   use covof_hip
@@ -34,10 +39,12 @@ program covof_driver
   real(dp), allocatable :: C(:, :), C0(:, :), u(:, :), v(:, :), psi(:, :)
   real(dp), allocatable :: rho(:, :), rho_u(:, :), rho_v(:, :), p(:, :), us(:, :), vs(:, :)
   real(dp), allocatable :: Cs(:, :), Ct(:, :), curv(:, :), fx(:, :), fy(:, :)
+  real(dp), allocatable :: mu(:, :), mun(:, :), u0(:, :), v0(:, :), um(:, :), vm(:, :)
+  type(covof_momentum_cfg) :: mcfg
   real(dp), parameter :: pi = 3.14159265358979323846_dp
 
   if (command_argument_count() < 4) then
-     print '(a)', "usage: covof_driver vof|sp|csf N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE"
      stop 2
   end if
   call get_command_argument(1, mode)
@@ -153,8 +160,65 @@ program covof_driver
      write (unit) C, Cs, Ct, curv, fx, fy
      close (unit)
      print '(a,es23.16)', "csf max |curvature| ", maxval(abs(curv))
+  case ("pred")
+     bc = COVOF_BC_DIRICHLET
+     st = covof_ctx_create(int(n, c_int), int(n, c_int), 1.0_dp, 1.0_dp, bc, ctx)
+     call check(st, "covof_ctx_create")
+     allocate(rho(0:n + 1, 0:n + 1), rho_u(0:n + 1, 0:n + 1), rho_v(0:n + 1, 0:n + 1))
+     allocate(mu(0:n + 1, 0:n + 1), mun(0:n + 1, 0:n + 1), u0(0:n + 1, 0:n + 1), v0(0:n + 1, 0:n + 1))
+     allocate(um(0:n + 1, 0:n + 1), vm(0:n + 1, 0:n + 1), fx(0:n + 1, 0:n + 1), fy(0:n + 1, 0:n + 1))
+     call disc(C, n, 0.5_dp, 0.4_dp, 0.2_dp)
+     C(0, :) = C(1, :); C(n + 1, :) = C(n, :); C(:, 0) = C(:, 1); C(:, n + 1) = C(:, n)
+     rho = C * 1000.0_dp + (1.0_dp - C) * 100.0_dp
+     mu = C * 10.0_dp + (1.0_dp - C) * 1.0_dp
+     rho_u = rho
+     rho_v = rho
+     mun = mu
+     do j = 0, n + 1
+        do i = 1, n + 1
+           rho_u(i, j) = 0.5_dp * (rho(i - 1, j) + rho(i, j))
+        end do
+     end do
+     do j = 1, n + 1
+        do i = 0, n + 1
+           rho_v(i, j) = 0.5_dp * (rho(i, j - 1) + rho(i, j))
+        end do
+     end do
+     do j = 1, n + 1
+        do i = 1, n + 1
+           mun(i, j) = 0.25_dp * (mu(i - 1, j - 1) + mu(i, j - 1) + mu(i - 1, j) + mu(i, j))
+        end do
+     end do
+     do j = 0, n + 1
+        do i = 0, n + 1
+           u0(i, j) = 0.2_dp * sin(2.0_dp * pi * (i - 1) * dx) * cos(pi * (j - 0.5_dp) * dy)
+           v0(i, j) = -0.1_dp * cos(pi * (i - 0.5_dp) * dx) * sin(2.0_dp * pi * (j - 1) * dy)
+        end do
+     end do
+     um = 0.9_dp * u0
+     vm = 0.9_dp * v0
+     u = u0
+     v = v0
+     fx = 0.0_dp
+     fy = 0.0_dp
+     dt = 1.0e-3_dp
+     mcfg%time_scheme = COVOF_TIME_SCHEME_GEAR
+     mcfg%has_nm1 = 1
+     mcfg%convective_scheme = COVOF_CONV_TVD_SUPERBEE
+     mcfg%gy = -0.98_dp
+     mcfg%bc_u(4) = 1.0_dp
+     st = covof_sp_predictor(ctx, mcfg, u, v, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt, &
+          COVOF_LINEAR_BICGSTAB2, COVOF_PRECOND_JACOBI, 1.0e-10_dp, 300_c_int, iters, relres)
+     call check(st, "covof_sp_predictor")
+     counter = iters
+     open (newunit=unit, file=trim(outfile), access="stream", form="unformatted", status="replace")
+     write (unit) 4_c_int, int(n, c_int), int(n, c_int), int(nsteps, c_int), int(counter, c_int), int(st, c_int)
+     write (unit) dt, relres
+     write (unit) u0, v0, um, vm, rho_u, rho_v, mu, mun, u, v
+     close (unit)
+     print '(a,i0,a,es10.3)', "predictor iterations ", counter, " relres ", relres
   case default
-     print '(a)', "usage: covof_driver vof|sp|csf N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE"
      stop 2
   end select
   st = covof_ctx_destroy(ctx)

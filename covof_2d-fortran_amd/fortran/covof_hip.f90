@@ -13,6 +13,11 @@
 !>     -> covof_sp_poisson(ctx, u, v, rho_u, rho_v, p, dt, ...)
 !>   the velocity correction                  src/module_simulation.f90:2590-2654
 !>     -> covof_sp_correct(ctx, p, rho_u, rho_v, dt, u, v)
+!>   the momentum block (velocity predictor) of sim_solve_standard_projection
+!>                                            src/module_simulation.f90:2454-2469
+!>     -> covof_sp_predictor(ctx, mcfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u,
+!>            rho_v, mu, mu_node, fx_sigma, fy_sigma, dt, linear_solver,
+!>            precond_type, tol, max_iter, iters, relres)
 !>   the tail of advect_vof                   src/module_interface_vof.f90:810-824
 !>     smooth_svof_helmholtz   -> covof_svof_smooth(ctx, C, width, C_s)
 !>     vof_smooth_diffusion    -> covof_smooth_diffusion(ctx, C, steps, C_tilde)
@@ -37,6 +42,20 @@ module covof_hip
   !> parallel preconditioners of the GPU library (no reference counterpart)
   integer(c_int), parameter :: COVOF_PRECOND_SGS_RB = 16, COVOF_PRECOND_MG = 17
   integer(c_int), parameter :: COVOF_PRECISION_FP64 = 0, COVOF_PRECISION_MIXED = 1
+  integer(c_int), parameter :: COVOF_TIME_SCHEME_EULER = 1, COVOF_TIME_SCHEME_GEAR = 2
+  integer(c_int), parameter :: COVOF_CONV_UPWIND = 1, COVOF_CONV_CENTERED = 2, COVOF_CONV_TVD_SUPERBEE = 3
+  integer(c_int), parameter :: COVOF_CONV_LIN_PICARD = 1, COVOF_CONV_LIN_ADAMS_BASHFORTH = 2, &
+       COVOF_CONV_LIN_NEWTON = 3
+
+  !> covof_momentum_cfg: the config_t fields the momentum block reads
+  !> (time_scheme, fields%has_nm1, enable_convection, convective_scheme,
+  !> convective_linearization, gx, gy, bc_<side>%u / %v of left, right, bottom, top)
+  type, bind(C) :: covof_momentum_cfg
+     integer(c_int) :: time_scheme = 1, has_nm1 = 0, enable_convection = 1, convective_scheme = 1, &
+          convective_linearization = 1
+     real(c_double) :: gx = 0.0_c_double, gy = 0.0_c_double
+     real(c_double) :: bc_u(4) = 0.0_c_double, bc_v(4) = 0.0_c_double
+  end type covof_momentum_cfg
 
   interface
      integer(c_int) function covof_ctx_create(nx, ny, lx, ly, bc, ctx) bind(C, name="covof_ctx_create")
@@ -94,6 +113,21 @@ module covof_hip
        real(c_double), value :: dt
        real(c_double), intent(inout) :: u(*), v(*)
      end function covof_sp_correct
+
+     integer(c_int) function covof_sp_predictor(ctx, mcfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, &
+          mu, mu_node, fx_sigma, fy_sigma, dt, lin_solver, precond, tol, maxit, iters, relres) &
+          bind(C, name="covof_sp_predictor")
+       import :: c_int, c_double, c_ptr, covof_momentum_cfg
+       type(c_ptr), value :: ctx
+       type(covof_momentum_cfg), intent(in) :: mcfg
+       real(c_double), intent(inout) :: u(*), v(*)
+       real(c_double), intent(in) :: u_old(*), v_old(*), u_nm1(*), v_nm1(*), rho_u(*), rho_v(*), mu(*), &
+            mu_node(*), fx_sigma(*), fy_sigma(*)
+       real(c_double), value :: dt, tol
+       integer(c_int), value :: lin_solver, precond, maxit
+       integer(c_int), intent(out) :: iters
+       real(c_double), intent(out) :: relres
+     end function covof_sp_predictor
 
      integer(c_int) function covof_curvature_csf(ctx, G, C, enable, sigma, curvature, fx_sigma, fy_sigma) &
           bind(C, name="covof_curvature_csf")

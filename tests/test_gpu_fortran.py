@@ -97,3 +97,33 @@ def test_fortran_curvature_csf_matches_oracle(tmp_path, orc):
     co, fxo, fyo = orc.curvature_csf(m, Cto, C, 0.07)
     assert np.array_equal(curv, co) and np.array_equal(fx, fxo) and np.array_equal(fy, fyo)
     assert np.abs(curv).max() > 0.0
+
+
+@pytest.mark.gpu
+def test_fortran_sp_predictor_matches_python_and_oracle(tmp_path, cv, orc):
+    """covof_sp_predictor from Fortran (a derived type bind(C) for the
+    options) gives the Python device path's u*, v* bit for bit, and the
+    oracle's within the solve's tolerance."""
+    n = 48
+    out = tmp_path / "pred.bin"
+    r = subprocess.run([DRIVER, "pred", str(n), "1", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    hdr, dt, relres, (u0, v0, um, vm, rho_u, rho_v, mu, mun, us, vs) = _read(out, 10)
+    assert hdr[0] == 4 and hdr[5] == 0
+    z = np.zeros_like(u0)
+    cfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98, bc_u=(0, 0, 0, 1.0))
+    ctx = cv.Context(n, n, bc=(1, 1, 1, 1))
+    F = {"u": to_gpu(u0), "v": to_gpu(v0), "u_old": to_gpu(u0), "v_old": to_gpu(v0), "u_nm1": to_gpu(um),
+         "v_nm1": to_gpu(vm), "rho_u": to_gpu(rho_u), "rho_v": to_gpu(rho_v), "mu": to_gpu(mu),
+         "mu_node": to_gpu(mun), "fx_sigma": to_gpu(z), "fy_sigma": to_gpu(z)}
+    res = ctx.solve_standard_projection_predictor(cfg, F, dt, 1, 1, 1e-10, 300)
+    assert res.converged and res.iters == hdr[4]
+    assert np.array_equal(F["u"].cpu().numpy(), us) and np.array_equal(F["v"].cpu().numpy(), vs)
+    o = orc.Momentum(orc.mesh(n, n, bc=(1, 1, 1, 1)),
+                     dict(u=u0, v=v0, u_old=u0, v_old=v0, u_nm1=um, v_nm1=vm, rho_u=rho_u, rho_v=rho_v, mu=mu,
+                          mu_node=mun, fx=z, fy=z),
+                     dict(time_scheme=2, has_nm1=1, conv_scheme=3, gy=-0.98, bc_u=(0, 0, 0, 1.0)), dt)
+    uo, vo, _, _ = o.predictor(1, 1, 1e-10, 300)
+    scale = max(np.abs(uo).max(), np.abs(vo).max())
+    assert np.abs(us - uo).max() < 1e-8 * scale and np.abs(vs - vo).max() < 1e-8 * scale
