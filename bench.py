@@ -200,6 +200,31 @@ def main():
         "volume_drift_rel": abs(v1 - v0) / v0,
     }
     log(f"VOF: {t_vof * 1e3:.3f} ms/step, {vof['cell_updates_per_s']:.3e} cell-updates/s")
+    # the same step at 4096^2 (the VOF workload's own size): 100 steps
+    if vn == 2048 and not args.kernel_sample:
+        del Cg, ug, vg
+        vbig = 4096
+        t0 = time.time()
+        Cb, ub, vb, dtb = wl.vof_case(vbig, seed=0)
+        log(f"vof case {vbig}^2 built in {time.time() - t0:.1f}s")
+        bctx = cv.Context(vbig, vbig, bc=(cv.BC_PERIODIC,) * 4)
+        Cg, ug, vg = (torch.tensor(a, device=dev) for a in (Cb, ub, vb))
+        del Cb, ub, vb
+        w0 = Cg[1:vbig + 1, 1:vbig + 1].sum().item()
+        bctx.advect_vof(Cg, ug, vg, dtb, reconstruction_method="ELVIRA", nsteps=2)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        bctx.advect_vof(Cg, ug, vg, dtb, reconstruction_method="ELVIRA", nsteps=100)
+        torch.cuda.synchronize()
+        t_big = (time.perf_counter() - t) / 100
+        w1 = Cg[1:vbig + 1, 1:vbig + 1].sum().item()
+        vof["n4096"] = {"workload": "ELVIRA + Weymouth-Yue, 4096x4096 periodic, 128 seeded ellipses, CFL 0.25, "
+                                    "100 steps",
+                        "ms_per_step": t_big * 1e3, "cell_updates_per_s": vbig * vbig / t_big,
+                        "algorithmic_GBps": 50.0 * vbig * vbig / t_big / 1e9, "volume_drift_rel": abs(w1 - w0) / w0}
+        log(f"VOF 4096^2: {t_big * 1e3:.3f} ms/step, {vbig * vbig / t_big:.3e} cell-updates/s")
+        del bctx, Cg, ug, vg
+        Cg = torch.tensor(Cv, device=dev)
     # the tail of advect_vof when surface tension is on: 3 diffusion-smoothing
     # steps (16 B/cell each + the halo pass) and curvature + CSF (3 passes:
     # normals 24 B/cell, curvature 24 B/cell, forces 32 B/cell)
@@ -221,6 +246,48 @@ def main():
     vof["curvature_csf_us"] = t_csf * 1e6
     vof["curvature_csf_GBps"] = 80.0 * vn * vn / t_csf / 1e9
     log(f"surface steps: diffusion x3 {t_dif * 1e6:.1f} us, curvature + CSF {t_csf * 1e6:.1f} us")
+
+    # ------------------------------------------------- SP velocity predictor
+    # the momentum block of the standard projection at the same size:
+    # two-phase properties (rho 1000 / 100, mu 10 / 1 from seeded circles),
+    # BDF2 with an n-1 state, TVD superbee convection, gravity, no-slip box,
+    # CFL 0.25; BiCGSTAB(2) + Jacobi to 1e-10
+    t0 = time.time()
+    fm, dtm, bcm, lxm, lym = wl.momentum_case(n, n, seed=0)
+    log(f"momentum case {n}^2 built in {time.time() - t0:.1f}s")
+    mctx = cv.Context(n, n, lxm, lym, bc=bcm)
+    mcfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98)
+    base = {k: torch.tensor(fm[k], device=dev) for k in cv.Context.MOMENTUM_FIELDS}
+    del fm
+    n_unk = mctx.momentum_size()[0]
+    reps = 3
+    copies = [{**base, "u": base["u"].clone(), "v": base["v"].clone()} for _ in range(reps + 1)]
+    mctx.solve_standard_projection_predictor(mcfg, copies[0], dtm, cv.LINEAR_BICGSTAB2, cv.PRECOND_JACOBI, tol,
+                                             1000)  # warm-up
+    mctx.profile(1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(reps):
+        rp = mctx.solve_standard_projection_predictor(mcfg, copies[k + 1], dtm, cv.LINEAR_BICGSTAB2,
+                                                      cv.PRECOND_JACOBI, tol, 1000)
+    torch.cuda.synchronize()
+    t_pred = (time.perf_counter() - t) / reps
+    mv = mctx.profile_read(L.KERNEL_MOM_MATVEC)
+    mctx.profile(0)
+    assert rp.converged, rp
+    # algorithmic bytes of one operator product per unknown: x, the two
+    # velocity components of fields_old, rho, mu, mu_node, y (7 FP64 words)
+    mv_us = mv[0] * 1e3 / max(mv[1], 1)
+    predictor = {
+        "workload": f"SP momentum block {n}x{n}: rho 1000/100, mu 10/1 (16 seeded circles), BDF2, TVD superbee, "
+                    "gravity, no-slip walls, CFL 0.25; BiCGSTAB(2) + Jacobi to 1e-10",
+        "unknowns": n_unk, "iters": rp.iters, "relres": rp.relres, "s": t_pred,
+        "matvec_us": mv_us, "matvec_launches": mv[1],
+        "matvec_GBps": 56.0 * n_unk / (mv_us * 1e-6) / 1e9,
+    }
+    log(f"predictor: {rp.iters} it, {t_pred * 1e3:.2f} ms, operator {mv_us:.1f} us "
+        f"({predictor['matvec_GBps']:.0f} GB/s)")
+    del copies, base, mctx
 
     # ----------------------------------------------------------- CPU baseline
     cpu = None
@@ -269,6 +336,7 @@ def main():
         "cpu_baseline": cpu,
         "multigrid_pcg": mg,
         "vof": vof,
+        "sp_predictor": predictor,
     }
     print(json.dumps(out), flush=True)
 

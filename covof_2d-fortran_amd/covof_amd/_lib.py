@@ -82,7 +82,7 @@ SIGNATURES = {
 
 # kernel tags of covof_ctx_profile_read
 KERNEL_PCG_A_FP64, KERNEL_PCG_B_FP64, KERNEL_PCG_A_FP32, KERNEL_PCG_B_FP32, KERNEL_VOF_RECON, \
-    KERNEL_VOF_UPDATE = range(6)
+    KERNEL_VOF_UPDATE, KERNEL_MOM_MATVEC = range(7)
 
 _lib = None
 

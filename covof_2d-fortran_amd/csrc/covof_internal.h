@@ -71,6 +71,11 @@ struct LinOp {
   long n = 0;
   virtual int apply(const double *x, double *y) = 0;
   virtual int precond(const double *r, double *z) = 0;
+  // y = M^-1 (A x); tmp: a work vector the default form may use
+  virtual int apply_pa(const double *x, double *y, double *tmp) {
+    int st = apply(x, tmp);
+    return st ? st : precond(tmp, y);
+  }
   virtual ~LinOp() {}
 };
 

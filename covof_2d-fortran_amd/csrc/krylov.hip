@@ -24,31 +24,49 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
+// Up to five inner products over at most three distinct vectors: each
+// element of every distinct vector is loaded once and all their pairwise
+// products are accumulated (BiCGSTAB(2)'s five-product step reads three
+// vectors instead of ten).
 struct Pairs {
-  const double *a[5];
-  const double *b[5];
-  int n;
+  const double *v[3];  // the distinct vectors
+  int nq;              // products wanted
+  int ia[5], ib[5];    // operands of product q (indices into v, ia <= ib)
 };
 
-// Up to five inner products; per-block partials, then the last block sums the
-// partials in block order and writes out[q].
+// per-block partials of the NV(NV+1)/2 pairwise products, then the last block
+// sums the partials in block order and writes out[q]
+template <int NV>
 __global__ void __launch_bounds__(BLK) k_dots(long n, Pairs P, double *part, unsigned *ticket, double *out) {
-  __shared__ double sh[5][BLK / 64];
+  constexpr int NP = NV * (NV + 1) / 2;
+  __shared__ double sh[NP][BLK / 64];
+  __shared__ double tot[NP];
   __shared__ bool last;
-  double acc[5] = {0, 0, 0, 0, 0};
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < n; t += (long)gridDim.x * BLK)
-    for (int q = 0; q < P.n; ++q) acc[q] += P.a[q][t] * P.b[q][t];
+  double acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) acc[p] = 0.0;
+  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < n; t += (long)gridDim.x * BLK) {
+    double x[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] = P.v[k][t];
+    int p = 0;
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int b = a; b < NV; ++b) acc[p++] += x[a] * x[b];
+  }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int q = 0; q < P.n; ++q) {
-    const double w = wsum(acc[q]);
-    if (lane == 0) sh[q][wid] = w;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const double w = wsum(acc[p]);
+    if (lane == 0) sh[p][wid] = w;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 0; q < P.n; ++q) {
+    for (int p = 0; p < NP; ++p) {
       double s = 0.0;
-      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
-      part[(long)blockIdx.x * 5 + q] = s;
+      for (int w = 0; w < BLK / 64; ++w) s += sh[p][w];
+      part[(long)blockIdx.x * 6 + p] = s;
     }
     __threadfence();
     last = atomicAdd(ticket, 1u) == gridDim.x - 1;
@@ -56,18 +74,22 @@ __global__ void __launch_bounds__(BLK) k_dots(long n, Pairs P, double *part, uns
   __syncthreads();
   if (!last) return;
   __threadfence();
-  for (int q = 0; q < P.n; ++q) {
+  for (int p = 0; p < NP; ++p) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK) s += __builtin_nontemporal_load(&part[(long)b * 5 + q]);
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK) s += __builtin_nontemporal_load(&part[(long)b * 6 + p]);
     s = wsum(s);
-    if (lane == 0) sh[q][wid] = s;
+    if (lane == 0) sh[p][wid] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 0; q < P.n; ++q) {
+    for (int p = 0; p < NP; ++p) {
       double s = 0.0;
-      for (int w = 0; w < BLK / 64; ++w) s += sh[q][w];
-      out[q] = s;
+      for (int w = 0; w < BLK / 64; ++w) s += sh[p][w];
+      tot[p] = s;
+    }
+    for (int q = 0; q < P.nq; ++q) {
+      const int a = P.ia[q], b = P.ib[q];
+      out[q] = tot[a * NV - a * (a - 1) / 2 + (b - a)];
     }
     *ticket = 0u;
   }
@@ -112,19 +134,34 @@ struct Bicg {
 
   // apply_PA (src/module_linear_solvers.f90:323-356): dst = M^-1 (A src)
   void PA(const double *src, double *dst) {
-    if (!err) err = op.apply(src, tmp);
-    if (!err) err = op.precond(tmp, dst);
+    if (!err) err = op.apply_pa(src, dst, tmp);
   }
-  // inner products -> host (synchronises)
+  // inner products -> host (synchronises); k <= 5 products of at most three
+  // distinct vectors
   void dots(int k, const double *const *a, const double *const *bb, double *res) {
     if (err) return;
     Pairs P;
-    P.n = k;
+    P.nq = k;
+    int nv = 0;
+    auto slot = [&](const double *p) {
+      for (int i = 0; i < nv; ++i)
+        if (P.v[i] == p) return i;
+      if (nv == 3) return -1;
+      P.v[nv] = p;
+      return nv++;
+    };
     for (int q = 0; q < k; ++q) {
-      P.a[q] = a[q];
-      P.b[q] = bb[q];
+      int ia = slot(a[q]), ib = slot(bb[q]);
+      if (ia < 0 || ib < 0) { err = COVOF_BAD_ARGUMENT; return; }
+      P.ia[q] = ia < ib ? ia : ib;
+      P.ib[q] = ia < ib ? ib : ia;
     }
-    hipLaunchKernelGGL(k_dots, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
+    if (nv == 1)
+      hipLaunchKernelGGL(k_dots<1>, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
+    else if (nv == 2)
+      hipLaunchKernelGGL(k_dots<2>, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
+    else
+      hipLaunchKernelGGL(k_dots<3>, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
     if (hipGetLastError() != hipSuccess) { err = COVOF_HIP_ERROR; return; }
     if (hipMemcpyAsync(c.ps.hscal, c.ps.scal + 40, k * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
         hipStreamSynchronize(c.stream) != hipSuccess) { err = COVOF_HIP_ERROR; return; }

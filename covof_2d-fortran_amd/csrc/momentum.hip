@@ -287,12 +287,10 @@ __global__ void __launch_bounds__(1024) k_vel_halo(Mesh m, MomCfg c, double *__r
 }
 
 // ns_apply_system_matrix, SP branch (src/module_navier_stokes.f90:380-780):
-// y = A x from the unpacked work fields tu, tv; one unknown per thread.
-__global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields F, double dt, long nu, long n,
-                                                    int u_nx, int v_ny, const double *__restrict__ tu,
-                                                    const double *__restrict__ tv, double *__restrict__ y) {
-  const long k = (long)blockIdx.x * BLK + threadIdx.x;
-  if (k >= n) return;
+// row k of A x from the unpacked work fields tu, tv.
+__device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const MomFields &F, double dt, long k,
+                                          long nu, int u_nx, int v_ny, const double *__restrict__ tu,
+                                          const double *__restrict__ tv) {
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   const int i = d.i, j = d.j, nx = m.nx, ny = m.ny;
   const double dx = m.dx, dy = m.dy, dvol = dx * dy;
@@ -307,12 +305,10 @@ __global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields 
     const double rho_u = F.ru[HX(i, j)];
     const double diag_bc = (at * rho_u / dt) * dvol;
     if (i == 1 && bl) {
-      y[k] = m.bc[0] == BC_NEUMANN ? diag_bc * (TU(1, j) - TU(2, j)) : diag_bc * TU(1, j);
-      return;
+      return m.bc[0] == BC_NEUMANN ? diag_bc * (TU(1, j) - TU(2, j)) : diag_bc * TU(1, j);
     }
     if (i == u_nx && bl) {
-      y[k] = m.bc[1] == BC_NEUMANN ? diag_bc * (TU(u_nx, j) - TU(u_nx - 1, j)) : diag_bc * TU(u_nx, j);
-      return;
+      return m.bc[1] == BC_NEUMANN ? diag_bc * (TU(u_nx, j) - TU(u_nx - 1, j)) : diag_bc * TU(u_nx, j);
     }
     double val = (at * rho_u / dt) * TU(i, j) * dvol;
     if (c.conv) {
@@ -342,14 +338,13 @@ __global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields 
     const double shN = mnN * ((TU(i, j + 1) - TU(i, j)) / dy + (TV(i, j + 1) - TV(i - 1, j + 1)) / dx) * dx;
     const double shS = mnS * ((TU(i, j) - TU(i, j - 1)) / dy + (TV(i, j) - TV(i - 1, j)) / dx) * dx;
     val = val - (shN - shS);
-    y[k] = val;
+    return val;
   } else {
     const double *v = F.v;
     const double rho_v = F.rv[HX(i, j)];
     const double diag_bc = (at * rho_v / dt) * dvol;
     if ((j == 1 && bb && wallbc(m.bc[2])) || (j == v_ny && bb && wallbc(m.bc[3]))) {
-      y[k] = diag_bc * TV(i, j);
-      return;
+      return diag_bc * TV(i, j);
     }
     double val = (at * rho_v / dt) * TV(i, j) * dvol;
     if (c.conv) {
@@ -379,10 +374,29 @@ __global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields 
     const double shE = mnE * ((TV(i + 1, j) - TV(i, j)) / dx + (TU(i + 1, j) - TU(i + 1, j - 1)) / dy) * dy;
     const double shW = mnW * ((TV(i, j) - TV(i - 1, j)) / dx + (TU(i, j) - TU(i, j - 1)) / dy) * dy;
     val = val - (shE - shW);
-    y[k] = val;
+    return val;
   }
 #undef TU
 #undef TV
+}
+
+// y = A x, one unknown per thread; with JAC, y = M^-1 A x for the Jacobi
+// preconditioner (apply_PA, src/module_linear_solvers.f90:323-356): the same
+// row value divided as apply_preconditioner divides it.
+template <bool JAC>
+__global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields F, double dt, long nu, long n,
+                                                    int u_nx, int v_ny, const double *__restrict__ tu,
+                                                    const double *__restrict__ tv, const double *__restrict__ diag,
+                                                    double *__restrict__ y) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= n) return;
+  const double val = mom_row(m, c, F, dt, k, nu, u_nx, v_ny, tu, tv);
+  if (JAC) {
+    const double dk = diag[k];
+    y[k] = fabs(dk) > 1.0e-15 ? val / dk : val;
+  } else {
+    y[k] = val;
+  }
 }
 
 // ns_assemble_preconditioner_blocks: diag_u, diag_v
@@ -581,12 +595,19 @@ int mom_alloc(Ctx &c) {
   return COVOF_OK;
 }
 
-// The momentum operator with its NONE / JACOBI preconditioner as a LinOp.
+int mom_product(Ctx &c, const double *x, double *y, bool jacobi);
+
+// The momentum operator with its NONE / JACOBI preconditioner as a LinOp;
+// the preconditioned product is one fused kernel.
 struct MomOp : LinOp {
   Ctx &c;
   int pc;
   MomOp(Ctx &cc, int precond) : c(cc), pc(precond) { n = cc.mom.n; }
   int apply(const double *x, double *y) override { return mom_matvec(c, x, y); }
+  int apply_pa(const double *x, double *y, double *tmp) override {
+    if (pc == COVOF_PRECOND_JACOBI) return mom_product(c, x, y, true);
+    return LinOp::apply_pa(x, y, tmp);
+  }
   int precond(const double *r, double *z) override {
     if (pc == COVOF_PRECOND_JACOBI) {
       hipLaunchKernelGGL(k_mom_jacobi, grid_of(n), dim3(BLK), 0, c.stream, n, c.mom.diag, r, z);
@@ -657,16 +678,24 @@ int mom_pack(Ctx &c, const double *u, const double *v, double *x) {
   return COVOF_OK;
 }
 
-// unpack_vector(homogeneous) into the work fields, then the operator.
-int mom_matvec(Ctx &c, const double *x, double *y) {
+namespace {
+// unpack_vector(homogeneous) into the work fields, then the operator (with
+// the Jacobi division when jacobi: M.diag must hold the diagonal).
+int mom_product(Ctx &c, const double *x, double *y, bool jacobi) {
   Mom &M = c.mom;
-  if (!M.bound) return COVOF_BAD_ARGUMENT;
   hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.n, M.u_nx, x, M.tu, M.tv);
   hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, M.tu, M.tv, 1, 1);
-  hipLaunchKernelGGL(k_mom_matvec, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.cfg, M.f, M.dt, M.nu, M.n, M.u_nx,
-                     M.v_ny, (const double *)M.tu, (const double *)M.tv, y);
+  launch_tagged(c, COVOF_KERNEL_MOM_MATVEC, jacobi ? k_mom_matvec<true> : k_mom_matvec<false>, grid_of(M.n),
+                dim3(BLK), c.m, M.cfg, M.f, M.dt, M.nu, M.n, M.u_nx, M.v_ny, (const double *)M.tu,
+                (const double *)M.tv, (const double *)M.diag, y);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
+}
+}  // namespace
+
+int mom_matvec(Ctx &c, const double *x, double *y) {
+  if (!c.mom.bound) return COVOF_BAD_ARGUMENT;
+  return mom_product(c, x, y, false);
 }
 
 // solve_linear_system on the momentum operator, the diagonal of the bound

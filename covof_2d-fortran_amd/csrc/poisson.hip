@@ -1155,7 +1155,7 @@ int poisson_alloc(Ctx &c) {
   P.nblocks = g.x * g.y;
   // partials: up to 4 values per block of the fused kernels, 5 per block of
   // the 512-block inner-product kernel of BiCGSTAB(2)
-  const long npart = std::max<long>((long)P.nblocks * NP * 3, 512L * 5);
+  const long npart = std::max<long>((long)P.nblocks * NP * 3, 512L * 6);
   COVOF_CHECK(hipMalloc(&P.part, npart * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.scal, NSLOT * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.ticket, sizeof(unsigned)));

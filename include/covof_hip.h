@@ -264,10 +264,12 @@ int covof_sp_predictor(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u,
 #define COVOF_KERNEL_PCG_B_FP32 3
 #define COVOF_KERNEL_VOF_RECON 4  /* PLIC/ELVIRA reconstruction of one pass */
 #define COVOF_KERNEL_VOF_UPDATE 5 /* flux + update + halo of one pass */
-#define COVOF_NUM_KERNEL_TAGS 6
+#define COVOF_KERNEL_MOM_MATVEC 6 /* the SP momentum operator (one product) */
+#define COVOF_NUM_KERNEL_TAGS 7
 /* on > 0: time launches of the tagged kernels from their own dispatch start
  * and end timestamps (hipExtLaunchKernelGGL event pairs on the context's
- * stream) and accumulate the durations -- every launch of the VOF kernels,
+ * stream) and accumulate the durations -- every launch of the VOF kernels
+ * and of the momentum operator,
  * the Krylov kernels of every on-th iteration (a sampling period that keeps
  * the bookkeeping out of a timed solve; timed solves run their kernels as
  * direct launches instead of graph replays); on == 0: stop.  Turning it on
