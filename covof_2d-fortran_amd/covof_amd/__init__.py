@@ -19,3 +19,4 @@ from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BI
                    CovofError)
 from .context import Context, SolveResult  # noqa: F401
 from .autograd import PressureSolve, pressure_solve  # noqa: F401
+from .sp_run import StandardProjectionRun  # noqa: F401

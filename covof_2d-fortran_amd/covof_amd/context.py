@@ -366,6 +366,15 @@ class Context:
         L.check(st, "covof_momentum_solve_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return SolveResult(it.value, rr.value, st)
 
+    def apply_velocity_bc(self, cfg: L.MomentumCfg, u, v, homogeneous=False):
+        """sync_u, sync_v, apply_boundary_conditions (src/module_mpi_comm.f90:
+        733-775, :290-537) on the halo fields u, v in place."""
+        self._field(u, "u")
+        self._field(v, "v")
+        self._stream()
+        L.check(self._lib.covof_velocity_bc_d(self._h, ct.byref(cfg), _ptr(u), _ptr(v), int(bool(homogeneous))),
+                "covof_velocity_bc_d")
+
     def solve_standard_projection_predictor(self, cfg: L.MomentumCfg, fields: dict, dt: float,
                                             linear_solver=L.LINEAR_BICGSTAB2, precond=L.PRECOND_JACOBI,
                                             tol=1e-5, max_iter=300) -> SolveResult:

@@ -486,6 +486,16 @@ extern "C" int covof_momentum_solve_d(covof_ctx *h, const double *b, double *x, 
   return hip_status(st);
 }
 
+extern "C" int covof_velocity_bc_d(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,
+                                   int homogeneous) {
+  if (!h || !u || !v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  covof::MomCfg c;
+  if (mom_cfg(cfg, &c)) return fail(COVOF_BAD_ARGUMENT, "covof_momentum_cfg: selector out of range");
+  const int st = covof::velocity_bc(h->c, c, u, v, homogeneous);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "periodic sides must come in pairs");
+  return hip_status(st);
+}
+
 extern "C" int covof_sp_predictor_d(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,
                                     const double *u_old, const double *v_old, const double *u_nm1,
                                     const double *v_nm1, const double *rho_u, const double *rho_v,

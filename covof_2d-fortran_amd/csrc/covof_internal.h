@@ -215,5 +215,6 @@ int mom_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, d
 int mom_predictor(Ctx &c, double *u, double *v, int lin_solver, int precond, double tol, int maxit,
                   int *iters, double *relres);
 void mom_free(Ctx &c);
+int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous);
 
 }  // namespace covof

@@ -714,6 +714,17 @@ int mom_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, d
   return krylov_bicgstab2(c, op, b, x, M.w, tol, maxit, iters, relres);
 }
 
+// sync_u, sync_v and apply_boundary_conditions of one rank
+// (src/module_mpi_comm.f90:733-775, :290-537) on halo fields u, v.
+int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous) {
+  const int *bc = c.m.bc;
+  if ((bc[0] == BC_PERIODIC) != (bc[1] == BC_PERIODIC) || (bc[2] == BC_PERIODIC) != (bc[3] == BC_PERIODIC))
+    return COVOF_BAD_ARGUMENT;
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, cfg, u, v, homogeneous ? 1 : 0, 1);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
 // The momentum block of sim_solve_standard_projection (:2454-2469): the
 // current u, v are the initial guess and the operator's fields_old; the
 // solution is unpacked into them with the inhomogeneous BC, then synced and

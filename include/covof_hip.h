@@ -247,6 +247,14 @@ int covof_sp_predictor_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *
                          const double *mu, const double *mu_node, const double *fx_sigma,
                          const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
                          int maxit, int *iters, double *relres);
+/* sync_u, sync_v and apply_boundary_conditions(homogeneous) of one rank
+ * (src/module_mpi_comm.f90:733-775, :290-537, UNIFORM profiles): the periodic
+ * halo exchange of the halo fields u, v (device), then the velocity BC with
+ * the Dirichlet values of cfg (zero when homogeneous != 0).  The standard
+ * projection calls it after the VOF transport and after the velocity
+ * correction (src/module_simulation.f90:541, :2657-2660). */
+int covof_velocity_bc_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v,
+                        int homogeneous);
 /* Host-array variant of covof_sp_predictor_d (the Fortran drop-in). */
 int covof_sp_predictor(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v,
                        const double *u_old, const double *v_old, const double *u_nm1,

@@ -248,11 +248,7 @@ class Momentum:
     def __init__(self, m: Mesh, fields: dict, opts: dict, dt: float):
         self.m, self.dt = m, float(dt)
         self.f = {k: np.ascontiguousarray(fields[k], dtype=np.float64) for k in MOM_FIELDS}
-        self.cfg = MomCfg(int(opts.get("time_scheme", 1)), int(opts.get("has_nm1", 0)),
-                          int(opts.get("enable_convection", 1)), int(opts.get("conv_scheme", 1)),
-                          int(opts.get("conv_lin", 1)), float(opts.get("gx", 0.0)), float(opts.get("gy", 0.0)),
-                          (ct.c_double * 4)(*opts.get("bc_u", (0.0,) * 4)),
-                          (ct.c_double * 4)(*opts.get("bc_v", (0.0,) * 4)))
+        self.cfg = self._cfg(opts)
         self.F = MomFields(*[_p(self.f[k]) for k in MOM_FIELDS])
         shp = (m.ny + 2, m.nx + 2)
         self.tu, self.tv = np.zeros(shp), np.zeros(shp)
@@ -260,6 +256,13 @@ class Momentum:
         lib().orc_mom_sizes(ct.byref(m), ct.byref(a), ct.byref(b))
         self.u_nx, self.v_ny = a.value, b.value
         self.n = self.u_nx * m.ny + m.nx * self.v_ny
+
+    @staticmethod
+    def _cfg(opts: dict) -> MomCfg:
+        return MomCfg(int(opts.get("time_scheme", 1)), int(opts.get("has_nm1", 0)),
+                      int(opts.get("enable_convection", 1)), int(opts.get("conv_scheme", 1)),
+                      int(opts.get("conv_lin", 1)), float(opts.get("gx", 0.0)), float(opts.get("gy", 0.0)),
+                      (ct.c_double * 4)(*opts.get("bc_u", (0.0,) * 4)), (ct.c_double * 4)(*opts.get("bc_v", (0.0,) * 4)))
 
     def diag(self):
         d = np.empty(self.n)
@@ -300,3 +303,77 @@ class Momentum:
                                     _p(self.tu), _p(self.tv), lin_solver, precond, tol, maxit,
                                     _p(u), _p(v), ct.byref(rr))
         return u, v, it, rr.value
+
+
+def velocity_bc(m: Mesh, opts: dict, u, v, homogeneous=False):
+    """sync_u, sync_v, apply_boundary_conditions in place (one rank)."""
+    cfg = Momentum._cfg(opts)
+    lib().orc_exchange_halos_scalar(ct.byref(m), _p(u))
+    lib().orc_exchange_halos_scalar(ct.byref(m), _p(v))
+    lib().orc_velocity_bc(ct.byref(m), ct.byref(cfg), _p(u), _p(v), int(bool(homogeneous)))
+
+
+class SPRun:
+    """The body of sim_run's time loop for a two-phase standard-projection run
+    (src/module_simulation.f90:495-559, :2424-2666), restated over the oracle:
+    n-1 / old swaps, advect_vof (WY + update_physical_properties + CSF from C),
+    apply_velocity_bc, the momentum block, the Poisson block (CG + SGS, 1e-7,
+    1000 iterations, mean removal), the velocity correction, sync + BC.  The
+    initial properties are the reference's reset values, then one
+    update_physical_properties.  opts: the momentum options (see Momentum)
+    with conv_lin etc.; the momentum operator's persistent work fields carry
+    over from step to step, as the reference's do."""
+
+    def __init__(self, m: Mesh, C, u, v, rho1, rho2, mu1, mu2, sigma, opts: dict, method=PLIC,
+                 lin_solver=3, precond=1, tol=1e-12, maxit=500):
+        self.m, self.opts, self.method = m, dict(opts), method
+        self.rho1, self.rho2, self.mu1, self.mu2, self.sigma = rho1, rho2, mu1, mu2, sigma
+        self.solver = (lin_solver, precond, tol, maxit)
+        shp = (m.ny + 2, m.nx + 2)
+        self.C = np.array(C, dtype=np.float64, copy=True)
+        self.u = np.array(u, dtype=np.float64, copy=True)
+        self.v = np.array(v, dtype=np.float64, copy=True)
+        self.p = np.zeros(shp)
+        self.rho, self.rho_u, self.rho_v = np.ones(shp), np.ones(shp), np.ones(shp)
+        self.mu, self.mu_node = np.full(shp, 1.0e-3), np.full(shp, 1.0e-3)
+        self.fx, self.fy, self.curv = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+        self.u_old, self.v_old = np.zeros(shp), np.zeros(shp)
+        self.u_nm1, self.v_nm1 = np.zeros(shp), np.zeros(shp)
+        self.tu, self.tv = np.zeros(shp), np.zeros(shp)
+        self.step_count = 0
+        self.nstep = 0
+        self._props()
+
+    def _props(self):
+        lib().orc_update_properties(ct.byref(self.m), _p(self.C), self.rho1, self.rho2, self.mu1, self.mu2,
+                                    _p(self.rho), _p(self.rho_u), _p(self.rho_v), _p(self.mu), _p(self.mu_node))
+
+    def step(self, dt):
+        m = self.m
+        self.nstep += 1
+        if self.nstep > 1:
+            self.u_nm1, self.v_nm1 = self.u_old.copy(), self.v_old.copy()
+        else:
+            self.u_nm1, self.v_nm1 = self.u.copy(), self.v.copy()
+        has_nm1 = self.nstep > 1
+        self.u_old, self.v_old = self.u.copy(), self.v.copy()
+        # advect_vof: transport, properties, CSF (normals and gradient from C)
+        self.C, self.step_count = wy_step(m, self.method, self.C, self.u, self.v, dt, self.step_count)
+        self._props()
+        lib().orc_curvature_csf(ct.byref(m), _p(self.C), _p(self.C), int(self.sigma > 0.0), self.sigma,
+                                _p(self.curv), _p(self.fx), _p(self.fy))
+        velocity_bc(m, self.opts, self.u, self.v, False)
+        # the momentum block
+        opts = dict(self.opts, has_nm1=int(has_nm1))
+        fields = dict(u=self.u, v=self.v, u_old=self.u_old, v_old=self.v_old, u_nm1=self.u_nm1, v_nm1=self.v_nm1,
+                      rho_u=self.rho_u, rho_v=self.rho_v, mu=self.mu, mu_node=self.mu_node, fx=self.fx, fy=self.fy)
+        M = Momentum(m, fields, opts, dt)
+        M.tu, M.tv = self.tu, self.tv
+        self.u, self.v, it_u, _ = M.predictor(*self.solver)
+        # the Poisson block and the correction
+        co = poisson_assemble(m, self.u, self.v, self.rho_u, self.rho_v, dt)
+        x, it_p, _ = sp_pcg(m, 2, co, co["b"], self.p[1:m.ny + 1, 1:m.nx + 1].copy(), tol=1e-7, maxit=1000)
+        self.p = sp_finish_pressure(m, x, self.p)
+        self.u, self.v = sp_correct(m, self.p, self.rho_u, self.rho_v, dt, self.u, self.v)
+        velocity_bc(m, self.opts, self.u, self.v, False)
+        return it_u + it_p

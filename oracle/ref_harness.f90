@@ -28,6 +28,11 @@
 !           the trailing sync + apply_boundary_conditions; u, v and iters
 !           (src/module_navier_stokes.f90:231-1101, :1268-1595;
 !           src/module_linear_solvers.f90:361-579, :735-838, :1167-1208)
+!   spstep  the body of sim_run's time loop for a two-phase standard-projection
+!           run (src/module_simulation.f90:495-559): n-1 / old swaps,
+!           advect_vof (WY + properties + CSF), apply_velocity_bc,
+!           apply_vof_bc, solve_standard_projection; C, u, v, p and the
+!           predictor + Poisson iteration total after every step
 !
 ! Usage: covof_ref_harness <mode> <input.bin> <output.bin>
 program covof_ref_harness
@@ -41,7 +46,8 @@ program covof_ref_harness
   use module_interface_vof, only: advect_vof, update_physical_properties
   use module_navier_stokes, only: navier_stokes_t
   use module_linear_solvers, only: solve_linear_system
-  use module_mpi_comm, only: sync_u, sync_v, apply_boundary_conditions
+  use module_mpi_comm, only: sync_u, sync_v, apply_boundary_conditions, apply_velocity_bc, apply_vof_bc
+  use module_simulation, only: simulation_t
   implicit none
   include 'mpif.h'
 
@@ -66,6 +72,8 @@ program covof_ref_harness
      call run_surface()
   case ('momentum')
      call run_momentum()
+  case ('spstep')
+     call run_spstep()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
@@ -262,5 +270,75 @@ contains
     write(uout) fields%u, fields%v
     write(uout) iters
   end subroutine run_momentum
+
+  subroutine run_spstep()
+    integer :: nx, ny, bc(4), method, tscheme, scheme, lin, solver, precond, maxit, nsteps, s, iters
+    real(dp) :: lx, ly, dt, rho1, rho2, mu1, mu2, sigma, gx, gy, tol, bcu(4), bcv(4)
+    type(simulation_t) :: sim
+    read(uin) nx, ny, bc, method, tscheme, scheme, lin, solver, precond, maxit, nsteps
+    read(uin) lx, ly, dt, rho1, rho2, mu1, mu2, sigma, gx, gy, tol, bcu, bcv
+    call setup(sim%mesh, sim%config, nx, ny, lx, ly, bc)
+    sim%config%solver_type = SOLVER_STANDARD_PROJECTION
+    sim%config%advection_method = 'WEYMOUTH_YUE'
+    sim%config%reconstruction_method = method_name(method)
+    sim%config%enable_two_phase = .true.
+    sim%config%rho1 = rho1
+    sim%config%rho2 = rho2
+    sim%config%mu1 = mu1
+    sim%config%mu2 = mu2
+    sim%config%sigma = sigma
+    sim%config%enable_surface_tension = (sigma > 0.0_dp)
+    sim%config%use_markers = .false.
+    sim%config%enable_svof_smoothing = .false.
+    sim%config%enable_diffusion_smoothing = .false.
+    sim%config%compressible = .false.
+    sim%config%time_scheme = tscheme
+    sim%config%enable_convection = .true.
+    sim%config%convective_scheme = scheme
+    sim%config%convective_linearization = lin
+    sim%config%gx = gx
+    sim%config%gy = gy
+    sim%config%linear_solver = solver
+    sim%config%precond_type = precond
+    sim%config%use_precond = .true.
+    sim%config%tol = tol
+    sim%config%max_iter = maxit
+    sim%config%obs_enabled = .false.
+    sim%config%enable_energy = .false.
+    sim%config%enable_ssp_rk3_momentum = .false.
+    sim%config%bc_left%u = bcu(1)
+    sim%config%bc_right%u = bcu(2)
+    sim%config%bc_bottom%u = bcu(3)
+    sim%config%bc_top%u = bcu(4)
+    sim%config%bc_left%v = bcv(1)
+    sim%config%bc_right%v = bcv(2)
+    sim%config%bc_bottom%v = bcv(3)
+    sim%config%bc_top%v = bcv(4)
+    call sim%fields%allocate(sim%mesh)
+    call sim%ns_solver%init(sim%mesh, sim%config)
+    read(uin) sim%fields%C, sim%fields%u, sim%fields%v
+    sim%fields%p = 0.0_dp
+    call update_physical_properties(sim%fields, sim%mesh, sim%config)
+    do s = 1, nsteps
+       if (s > 1) then
+          sim%fields%u_nm1 = sim%fields%u_old
+          sim%fields%v_nm1 = sim%fields%v_old
+          sim%fields%has_nm1 = .true.
+       else
+          sim%fields%u_nm1 = sim%fields%u
+          sim%fields%v_nm1 = sim%fields%v
+          sim%fields%has_nm1 = .false.
+       end if
+       sim%fields%u_old = sim%fields%u
+       sim%fields%v_old = sim%fields%v
+       sim%fields%p_old = sim%fields%p
+       call advect_vof(sim%fields, sim%mesh, sim%config, dt)
+       call apply_velocity_bc(sim%fields, sim%mesh, sim%config, .false.)
+       call apply_vof_bc(sim%fields, sim%mesh, sim%config)
+       call sim%solve_standard_projection(dt, iters)
+       write(uout) sim%fields%C, sim%fields%u, sim%fields%v, sim%fields%p
+       write(uout) iters
+    end do
+  end subroutine run_spstep
 
 end program covof_ref_harness

@@ -7,7 +7,7 @@ compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum)
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep)
 """
 from __future__ import annotations
 
@@ -227,14 +227,56 @@ def momentum_cases():
              lin=3, solver=3, bcu=(0.0, 0.0, 0.0, 0.5))
 
 
+def spstep(name, nx, ny, bc, C, u, v, dt, nsteps, method=0, tscheme=2, scheme=3, lin=1, solver=3, precond=1,
+           maxit=500, tol=1e-12, rho1=1000.0, rho2=100.0, mu1=10.0, mu2=1.0, sigma=24.5, gx=0.0, gy=-0.98,
+           bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=2.0):
+    """The body of sim_run's time loop for a two-phase standard-projection run
+    (n-1/old swaps, advect_vof, velocity + VOF BC, solve_standard_projection):
+    the state after the first and the last step and every step's iterations."""
+    payload = (i32(nx, ny, *bc, method, tscheme, scheme, lin, solver, precond, maxit, nsteps)
+               + f64(lx, ly, dt, rho1, rho2, mu1, mu2, sigma, gx, gy, tol, *bcu, *bcv)
+               + C.astype("<f8").tobytes() + u.astype("<f8").tobytes() + v.astype("<f8").tobytes())
+    raw = run("spstep", payload)
+    h = (ny + 2) * (nx + 2)
+    rec = 4 * h * 8 + 4
+    assert len(raw) == nsteps * rec
+    states, iters = [], []
+    for s_ in range(nsteps):
+        chunk = raw[s_ * rec:(s_ + 1) * rec]
+        states.append(np.frombuffer(chunk[:-4], dtype="<f8").reshape(4, ny + 2, nx + 2))
+        iters.append(int(np.frombuffer(chunk[-4:], dtype="<i4")[0]))
+    first, last = states[0], states[-1]
+    np.savez_compressed(os.path.join(HERE, f"spstep_{name}.npz"), nx=nx, ny=ny, bc=np.array(bc), C0=C, u0=u, v0=v,
+                        dt=dt, nsteps=nsteps, method=method, tscheme=tscheme, scheme=scheme, lin=lin, solver=solver,
+                        precond=precond, maxit=maxit, tol=tol, rho1=rho1, rho2=rho2, mu1=mu1, mu2=mu2, sigma=sigma,
+                        gx=gx, gy=gy, bcu=np.array(bcu), bcv=np.array(bcv), lx=lx, ly=ly,
+                        C1=first[0], u1=first[1], v1=first[2], p1=first[3],
+                        C=last[0], u=last[1], v=last[2], p=last[3], iters=np.array(iters), flags=FLAGS)
+
+
+def spstep_cases():
+    # the rising bubble of the small plain-method case at a quarter of its
+    # resolution: 1 x 2 box, free-slip sides, no-slip top and bottom, light
+    # bubble (C = 0, rho 100, mu 1) of radius 0.25 at (0.5, 0.5) in liquid
+    # (rho 1000, mu 10), sigma 24.5, g 0.98 down, BDF2, dt 1e-3, PLIC + WY,
+    # predictor CG + Jacobi
+    nx, ny = 32, 64
+    C = 1.0 - wl.disc(nx, ny, 0.5, 0.5, 0.25, lx=1.0, ly=2.0)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    z = np.zeros_like(C)
+    spstep("rising_bubble", nx, ny, (FS, FS, DI, DI), C, z, z, 1e-3, 6)
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum"}
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep"}
     if "surface" in groups:
         surface_cases()
     if "momentum" in groups:
         momentum_cases()
+    if "spstep" in groups:
+        spstep_cases()
     if not groups & {"geom", "recon", "advect"}:
         print("golden fixtures written to", HERE)
         return
