@@ -306,3 +306,34 @@ def momentum_case(nx, ny, seed=0, n_circles=16, rmin=4.0, rmax=24.0, rho_liquid=
     umax = max(np.abs(f["u"]).max(), np.abs(f["v"]).max(), 1e-30)
     dt = 0.25 * min(lx / nx, ly / ny) / umax
     return f, dt, bc, lx, ly
+
+
+def bubbles_case(nx, ny, n_bubbles=256, rmin=8.0, rmax=32.0, seed=0, lx=2.0, ly=1.0, sub=4):
+    """Liquid (C = 1) holding ``n_bubbles`` seeded non-overlapping circular
+    bubbles (C = 0), radii uniform in [rmin, rmax] cells, at least two cells
+    apart and from the walls; volume fractions by sub x sub sampling per cell;
+    zero-gradient halo.  Returns the halo array ``C`` of shape (ny+2, nx+2)."""
+    rng = np.random.default_rng(seed)
+    dx, dy = lx / nx, ly / ny
+    h = min(dx, dy)
+    placed = []
+    tries = 0
+    while len(placed) < n_bubbles and tries < 200000:
+        tries += 1
+        r = rng.uniform(rmin, rmax) * h
+        cx = rng.uniform(r + 2 * h, lx - r - 2 * h)
+        cy = rng.uniform(r + 2 * h, ly - r - 2 * h)
+        if all((cx - x) ** 2 + (cy - y) ** 2 > (r + q + 2 * h) ** 2 for x, y, q in placed):
+            placed.append((cx, cy, r))
+    C = np.ones((ny + 2, nx + 2))
+    off = (np.arange(sub) + 0.5) / sub
+    for cx, cy, r in placed:
+        i0, i1 = max(1, int((cx - r) / dx)), min(nx, int((cx + r) / dx) + 2)
+        j0, j1 = max(1, int((cy - r) / dy)), min(ny, int((cy + r) / dy) + 2)
+        xs = (np.arange(i0, i1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dx
+        ys = (np.arange(j0, j1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dy
+        inside = ((xs[None, :] - cx) ** 2 + (ys[:, None] - cy) ** 2 <= r * r).astype(np.float64)
+        gas = inside.reshape(j1 - j0 + 1, sub, i1 - i0 + 1, sub).mean(axis=(1, 3))
+        C[j0:j1 + 1, i0:i1 + 1] -= gas
+    fill_halo_zero_gradient(C, nx, ny)
+    return C, len(placed)
