@@ -14,7 +14,8 @@ Jacobi-PCG on the host cores as the CPU baseline.  Prints one JSON line.
                                       # rocprofv3 counter passes (no JSON)
 
 The kernel-sample run executes 256 iterations of the mixed solve, 64 of the
-FP64 solve and 2 VOF steps on the same workloads, then a plain copy of
+FP64 solve and 2 VOF steps on the same workloads, 4 momentum-operator
+products and one BiCGSTAB(2) sweep of the 2048^2 predictor, then a plain copy of
 256 MiB (torch copy_ of float64, reads and writes 256 MiB) whose FETCH_SIZE /
 WRITE_SIZE readings calibrate the counters into bytes.
 """
@@ -34,8 +35,8 @@ METRIC = "Poisson solve s to 1e-10 + VOF cell-updates/s at 2048^2; HBM GB/s vs p
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a 256 MiB
 # copy) measured with rocprofv3 --pmc on `bench.py --kernel-sample`; the
-# summary with its commands is profiles/poisson_pcg_wy_2048_hbm_traffic.md.
-TRAFFIC_2048 = {"k_pcg_a<float,1>": 1.615e8, "k_pcg_b<float,1>": 6.733e7}
+# summary with its commands is profiles/poisson_wy_momentum_2048_hbm_traffic.md.
+TRAFFIC_2048 = {"k_pcg_a<float,1>": 1.620e8, "k_pcg_b<float,1>": 6.733e7}
 
 
 def log(*a):
@@ -52,6 +53,18 @@ def kernel_sample(cv, wl, ctx, b, n, dev):
     vctx = cv.Context(n, n, bc=(cv.BC_PERIODIC,) * 4)
     Cg, ug, vg = (torch.tensor(a, device=dev) for a in (Cv, uv, vv))
     vctx.advect_vof(Cg, ug, vg, dtv, reconstruction_method="ELVIRA", nsteps=2)
+    # the momentum operator at 2048^2: 4 plain and 4 Jacobi-divided products
+    fm, dtm, bcm, lxm, lym = wl.momentum_case(n, n, seed=0)
+    mctx = cv.Context(n, n, lxm, lym, bc=bcm)
+    mcfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98)
+    F = {k: torch.tensor(fm[k], device=dev) for k in cv.Context.MOMENTUM_FIELDS}
+    mctx.momentum_set_fields(mcfg, F, dtm)
+    xm = mctx.assemble_rhs()
+    for _ in range(4):
+        mctx.apply_system_matrix(xm)
+    mctx.solve_linear_system(xm, mctx.new_momentum_vector(), cv.LINEAR_BICGSTAB2, cv.PRECOND_JACOBI, 1e-30, 2)
+    torch.cuda.synchronize()
+    log("momentum sample done")
     a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
     c = torch.empty_like(a)
     for _ in range(3):
