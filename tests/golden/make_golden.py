@@ -229,10 +229,11 @@ def momentum_cases():
 
 def spstep(name, nx, ny, bc, C, u, v, dt, nsteps, method=0, tscheme=2, scheme=3, lin=1, solver=3, precond=1,
            maxit=500, tol=1e-12, rho1=1000.0, rho2=100.0, mu1=10.0, mu2=1.0, sigma=24.5, gx=0.0, gy=-0.98,
-           bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=2.0):
+           bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=2.0, first=True):
     """The body of sim_run's time loop for a two-phase standard-projection run
     (n-1/old swaps, advect_vof, velocity + VOF BC, solve_standard_projection):
-    the state after the first and the last step and every step's iterations."""
+    the state after the last step (and the first, if first) and every step's
+    iterations."""
     payload = (i32(nx, ny, *bc, method, tscheme, scheme, lin, solver, precond, maxit, nsteps)
                + f64(lx, ly, dt, rho1, rho2, mu1, mu2, sigma, gx, gy, tol, *bcu, *bcv)
                + C.astype("<f8").tobytes() + u.astype("<f8").tobytes() + v.astype("<f8").tobytes())
@@ -245,13 +246,13 @@ def spstep(name, nx, ny, bc, C, u, v, dt, nsteps, method=0, tscheme=2, scheme=3,
         chunk = raw[s_ * rec:(s_ + 1) * rec]
         states.append(np.frombuffer(chunk[:-4], dtype="<f8").reshape(4, ny + 2, nx + 2))
         iters.append(int(np.frombuffer(chunk[-4:], dtype="<i4")[0]))
-    first, last = states[0], states[-1]
+    s1, last = states[0], states[-1]
+    extra = dict(C1=s1[0], u1=s1[1], v1=s1[2], p1=s1[3]) if first else {}
     np.savez_compressed(os.path.join(HERE, f"spstep_{name}.npz"), nx=nx, ny=ny, bc=np.array(bc), C0=C, u0=u, v0=v,
                         dt=dt, nsteps=nsteps, method=method, tscheme=tscheme, scheme=scheme, lin=lin, solver=solver,
                         precond=precond, maxit=maxit, tol=tol, rho1=rho1, rho2=rho2, mu1=mu1, mu2=mu2, sigma=sigma,
                         gx=gx, gy=gy, bcu=np.array(bcu), bcv=np.array(bcv), lx=lx, ly=ly,
-                        C1=first[0], u1=first[1], v1=first[2], p1=first[3],
-                        C=last[0], u=last[1], v=last[2], p=last[3], iters=np.array(iters), flags=FLAGS)
+                        C=last[0], u=last[1], v=last[2], p=last[3], iters=np.array(iters), flags=FLAGS, **extra)
 
 
 def spstep_cases():
@@ -265,6 +266,12 @@ def spstep_cases():
     wl.fill_halo_zero_gradient(C, nx, ny)
     z = np.zeros_like(C)
     spstep("rising_bubble", nx, ny, (FS, FS, DI, DI), C, z, z, 1e-3, 6)
+    # the plain-method case itself: 128 x 256, 200 steps of dt 1e-3
+    nx, ny = 128, 256
+    C = 1.0 - wl.disc(nx, ny, 0.5, 0.5, 0.25, lx=1.0, ly=2.0)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    z = np.zeros_like(C)
+    spstep("rising_bubble_128x256_200", nx, ny, (FS, FS, DI, DI), C, z, z, 1e-3, 200, first=False)
 
 
 def main():
