@@ -77,6 +77,8 @@ SIGNATURES = {
     "covof_velocity_bc_d": (_I, [_P, ct.c_void_p, _P, _P, _I]),
     "covof_sp_predictor_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
     "covof_sp_predictor": (_I, [_P, ct.c_void_p] + [_DP] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
+    "covof_history_d": (_I, [_P] + [_P] * 6 + [_DP]),
+    "covof_cell_output_d": (_I, [_P] + [_P] * 5),
     "covof_ctx_profile": (_I, [_P, _I]),
     "covof_ctx_profile_read": (_I, [_P, _I, _DP, ct.POINTER(ct.c_long)]),
 }

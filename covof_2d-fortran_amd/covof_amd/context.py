@@ -388,3 +388,29 @@ class Context:
                                             ct.byref(it), ct.byref(rr))
         L.check(st, "covof_sp_predictor_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return SolveResult(it.value, rr.value, st)
+
+    # ------------------------------------------------------------- output
+    def history_row(self, C, u, v, p, rho, T=None):
+        """The quantities of one _history.csv row (vtk_write_diagnostics,
+        src/module_vtk_writer.f90:399-531): [center_of_mass_y, rise_velocity,
+        bubble_volume, kinetic_energy, max_u, max_div, avg_rho_gas,
+        avg_p_gas].  T None: 300 everywhere (the reference's reset value)."""
+        for t, n in ((C, "C"), (u, "u"), (v, "v"), (p, "p"), (rho, "rho")):
+            self._field(t, n)
+        if T is not None:
+            self._field(T, "T")
+        out = (ct.c_double * 8)()
+        self._stream()
+        L.check(self._lib.covof_history_d(self._h, _ptr(C), _ptr(u), _ptr(v), _ptr(p), _ptr(rho), _ptr(T), out),
+                "covof_history_d")
+        return list(out)
+
+    def cell_output(self, u, v):
+        """Divergence and cell-centred velocity of vtk_write_step, (ny, nx)."""
+        self._field(u, "u")
+        self._field(v, "v")
+        div, uc, vc = (self.new_work() for _ in range(3))
+        self._stream()
+        L.check(self._lib.covof_cell_output_d(self._h, _ptr(u), _ptr(v), _ptr(div), _ptr(uc), _ptr(vc)),
+                "covof_cell_output_d")
+        return div, uc, vc

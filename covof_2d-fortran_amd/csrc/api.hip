@@ -537,6 +537,19 @@ extern "C" int covof_sp_predictor(covof_ctx *h, const covof_momentum_cfg *cfg, d
   return st ? hip_status(st) : sst;
 }
 
+// ------------------------------------------------------------------ output
+extern "C" int covof_history_d(covof_ctx *h, const double *C, const double *u, const double *v, const double *p,
+                               const double *rho, const double *T, double *out) {
+  if (!h || !C || !u || !v || !p || !rho || !out) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  return hip_status(covof::history_row(h->c, C, u, v, p, rho, T, out));
+}
+
+extern "C" int covof_cell_output_d(covof_ctx *h, const double *u, const double *v, double *div, double *uc,
+                                   double *vc) {
+  if (!h || !u || !v || !div || !uc || !vc) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  return hip_status(covof::cell_output(h->c, u, v, div, uc, vc));
+}
+
 // ------------------------------------------------------------------ profiling
 namespace covof {
 void prof_events(Ctx &c, int tag, hipEvent_t *start, hipEvent_t *stop) {

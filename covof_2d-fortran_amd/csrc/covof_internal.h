@@ -216,5 +216,9 @@ int mom_predictor(Ctx &c, double *u, double *v, int lin_solver, int precond, dou
                   int *iters, double *relres);
 void mom_free(Ctx &c);
 int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous);
+// diagnostics.hip: output-side reductions
+int history_row(Ctx &c, const double *C, const double *u, const double *v, const double *p, const double *rho,
+                const double *T, double *out);
+int cell_output(Ctx &c, const double *u, const double *v, double *div, double *uc, double *vc);
 
 }  // namespace covof

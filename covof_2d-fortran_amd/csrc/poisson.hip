@@ -41,6 +41,7 @@ enum Slot {
   // BiCGSTAB(2)
   B_RHO0 = 32, B_RHO1, B_ALPHA, B_OMEGA, B_BETA, B_GAMMA, B_R0N, B_G1, B_G2, B_RN,
   B_S11, B_S12, B_S22, B_RHS1, B_RHS2,
+  // 48-55: the history row of diagnostics.hip
   S_ZERO = 63,  // never written: the stop slot of launches outside a solve
   NSLOT = 64
 };
@@ -1155,7 +1156,7 @@ int poisson_alloc(Ctx &c) {
   P.nblocks = g.x * g.y;
   // partials: up to 4 values per block of the fused kernels, 5 per block of
   // the 512-block inner-product kernel of BiCGSTAB(2)
-  const long npart = std::max<long>((long)P.nblocks * NP * 3, 512L * 6);
+  const long npart = std::max<long>((long)P.nblocks * NP * 3, 512L * 9);
   COVOF_CHECK(hipMalloc(&P.part, npart * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.scal, NSLOT * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.ticket, sizeof(unsigned)));

@@ -263,6 +263,20 @@ int covof_sp_predictor(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u,
                        const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
                        int maxit, int *iters, double *relres);
 
+/* --- output (src/module_vtk_writer.f90) ------------------------------------ */
+
+/* The quantities of one _history.csv row of vtk_write_diagnostics (:399-531)
+ * from the device halo fields C, u, v, p, rho and T (T may be NULL: the
+ * reference's reset temperature, 300 everywhere): out[8] (host memory) =
+ * center_of_mass_y, rise_velocity, bubble_volume, kinetic_energy, max_u,
+ * max_div, avg_rho_gas, avg_p_gas.  Synchronises the context's stream. */
+int covof_history_d(covof_ctx *ctx, const double *C, const double *u, const double *v, const double *p,
+                    const double *rho, const double *T, double *out);
+/* The derived cell fields of vtk_write_step (:90-394): divergence and the
+ * cell-centred velocity (uc, vc), (nx, ny) device arrays. */
+int covof_cell_output_d(covof_ctx *ctx, const double *u, const double *v, double *div, double *uc,
+                        double *vc);
+
 /* --- kernel timing (no reference counterpart: instrumentation) ------------- */
 
 /* Kernel tags of covof_ctx_profile_read. */

@@ -1619,3 +1619,49 @@ int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fiel
   free(x);
   return it;
 }
+
+/* ------------------------------------------------------------ output */
+
+/* vtk_write_diagnostics (src/module_vtk_writer.f90:399-531), one rank, serial
+ * sums: out[8] = center_of_mass_y, rise_velocity, bubble_volume,
+ * kinetic_energy, max_u, max_div, avg_rho_gas, avg_p_gas.  T may be NULL
+ * (300 everywhere). */
+void orc_history(const orc_mesh *m, const double *C, const double *u, const double *v, const double *p,
+                 const double *rho, const double *T, double *out) {
+  double vol = 0.0, gvol = 0.0, yint = 0.0, vint = 0.0, mgas = 0.0, pgas = 0.0, ke = 0.0, mu = 0.0, md = 0.0;
+  const double dvol = m->dx * m->dy;
+  for (int j = 1; j <= m->ny; ++j) {
+    const double y_pos = ((double)(j - 1) + 0.5) * m->dy;
+    for (int i = 1; i <= m->nx; ++i) {
+      const double c_val = C[H(m, i, j)];
+      const double ind_bubble = 1.0 - c_val;
+      vol = vol + ind_bubble * dvol;
+      yint = yint + y_pos * ind_bubble * dvol;
+      const double ind_gas = 1.0 - c_val;
+      const double Tc = T ? T[H(m, i, j)] : 300.0;
+      const double rho_gas_c = p[H(m, i, j)] / (287.05 * dmax(1.0, Tc));
+      mgas = mgas + ind_gas * rho_gas_c * dvol;
+      pgas = pgas + ind_gas * p[H(m, i, j)] * dvol;
+      gvol = gvol + ind_gas * dvol;
+      const double uc = 0.5 * (u[H(m, i, j)] + u[H(m, i + 1, j)]);
+      const double vc = 0.5 * (v[H(m, i, j)] + v[H(m, i, j + 1)]);
+      vint = vint + vc * ind_bubble * dvol;
+      const double speed2 = uc * uc + vc * vc;
+      const double u2p = 0.5 * (u[H(m, i, j)] * u[H(m, i, j)] + u[H(m, i + 1, j)] * u[H(m, i + 1, j)]);
+      const double v2p = 0.5 * (v[H(m, i, j)] * v[H(m, i, j)] + v[H(m, i, j + 1)] * v[H(m, i, j + 1)]);
+      ke = ke + 0.5 * rho[H(m, i, j)] * (u2p + v2p) * dvol;
+      const double speed = sqrt(speed2);
+      if (speed > mu) mu = speed;
+      const double dv = fabs((u[H(m, i + 1, j)] - u[H(m, i, j)]) / m->dx + (v[H(m, i, j + 1)] - v[H(m, i, j)]) / m->dy);
+      if (dv > md) md = dv;
+    }
+  }
+  out[0] = vol > 1.0e-12 ? yint / vol : 0.0;
+  out[1] = vol > 1.0e-12 ? vint / vol : 0.0;
+  out[2] = vol;
+  out[3] = ke;
+  out[4] = mu;
+  out[5] = md;
+  out[6] = gvol > 1.0e-12 ? mgas / gvol : 0.0;
+  out[7] = gvol > 1.0e-12 ? pgas / gvol : 0.0;
+}

@@ -127,6 +127,12 @@ int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fiel
                      double *tu, double *tv, int lin_solver, int precond, double tol, int maxit,
                      double *u, double *v, double *relres);
 
+/* vtk_write_diagnostics (src/module_vtk_writer.f90:399-531), serial sums:
+ * out[8] = center_of_mass_y, rise_velocity, bubble_volume, kinetic_energy,
+ * max_u, max_div, avg_rho_gas, avg_p_gas; T may be NULL (300 everywhere). */
+void orc_history(const orc_mesh *m, const double *C, const double *u, const double *v, const double *p,
+                 const double *rho, const double *T, double *out);
+
 /* Serial column-major sum of x*y (the reference's sum(x*y)). */
 double orc_dot(const double *x, const double *y, long n);
 

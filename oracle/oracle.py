@@ -79,6 +79,7 @@ def lib():
             "orc_sp_correct": (None, [M, _D, _D, _D, d, _D, _D]),
             "orc_dot": (d, [_D, _D, ct.c_long]),
             "orc_bench_jacobi_pcg": (d, [M] + [_D] * 6 + [i, i]),
+            "orc_history": (None, [M] + [_D] * 7),
             "orc_velocity_bc": (None, [M, ct.POINTER(MomCfg), _D, _D, i]),
             "orc_mom_sizes": (None, [M, ip, ip]),
             "orc_mom_pack": (None, [M, _D, _D, _D]),
@@ -377,3 +378,12 @@ class SPRun:
         self.u, self.v = sp_correct(m, self.p, self.rho_u, self.rho_v, dt, self.u, self.v)
         velocity_bc(m, self.opts, self.u, self.v, False)
         return it_u + it_p
+
+
+def history(m: Mesh, C, u, v, p, rho, T=None):
+    """vtk_write_diagnostics' eight quantities (serial sums)."""
+    out = np.zeros(8)
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (C, u, v, p, rho)]
+    Tp = _p(np.ascontiguousarray(T, dtype=np.float64)) if T is not None else None
+    lib().orc_history(ct.byref(m), *[_p(a) for a in arr], Tp, _p(out))
+    return out

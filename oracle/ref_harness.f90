@@ -28,6 +28,9 @@
 !           the trailing sync + apply_boundary_conditions; u, v and iters
 !           (src/module_navier_stokes.f90:231-1101, :1268-1595;
 !           src/module_linear_solvers.f90:361-579, :735-838, :1167-1208)
+!   output  vtk_writer_t on given fields: init, then write_step and
+!           write_diagnostics per snapshot, finalize (src/module_vtk_writer.f90:
+!           46-545); the files go to <output.bin>_dir/
 !   spstep  the body of sim_run's time loop for a two-phase standard-projection
 !           run (src/module_simulation.f90:495-559): n-1 / old swaps,
 !           advect_vof (WY + properties + CSF), apply_velocity_bc,
@@ -48,6 +51,7 @@ program covof_ref_harness
   use module_linear_solvers, only: solve_linear_system
   use module_mpi_comm, only: sync_u, sync_v, apply_boundary_conditions, apply_velocity_bc, apply_vof_bc
   use module_simulation, only: simulation_t
+  use module_vtk_writer, only: vtk_writer_t
   implicit none
   include 'mpif.h'
 
@@ -74,6 +78,8 @@ program covof_ref_harness
      call run_momentum()
   case ('spstep')
      call run_spstep()
+  case ('output')
+     call run_output()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
@@ -340,5 +346,29 @@ contains
        write(uout) iters
     end do
   end subroutine run_spstep
+
+  subroutine run_output()
+    integer :: nx, ny, bc(4), nsnap, s, step
+    real(dp) :: lx, ly, t, dt
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    type(vtk_writer_t) :: vtk
+    read(uin) nx, ny, bc, nsnap
+    read(uin) lx, ly
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    call fields%allocate(mesh)
+    call vtk%init(mesh, trim(fout) // '_dir', 'case')
+    do s = 1, nsnap
+       read(uin) step
+       read(uin) t, dt
+       read(uin) fields%C, fields%u, fields%v, fields%p, fields%rho, fields%mu, fields%T, fields%curvature, &
+            fields%C_s
+       call vtk%write_step(mesh, fields, step, t)
+       call vtk%write_diagnostics(mesh, fields, step, t, dt)
+    end do
+    call vtk%finalize()
+    write(uout) nsnap
+  end subroutine run_output
 
 end program covof_ref_harness

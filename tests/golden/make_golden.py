@@ -7,7 +7,7 @@ compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep)
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep, output)
 """
 from __future__ import annotations
 
@@ -274,16 +274,57 @@ def spstep_cases():
     spstep("rising_bubble_128x256_200", nx, ny, (FS, FS, DI, DI), C, z, z, 1e-3, 200, first=False)
 
 
+def output_cases():
+    """vtk_writer_t of the reference on seeded fields: the _history.csv rows
+    and the .vtr / .pvtr files of two snapshots, stored as text."""
+    nx, ny, lx, ly = 20, 14, 1.0, 0.7
+    rng = np.random.default_rng(61)
+    C = 1.0 - wl.disc(nx, ny, 0.45, 0.35, 0.2, lx=lx, ly=ly)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    snaps = []
+    for step, t, dt in ((1, 1.0e-3, 1.0e-3), (250, 0.25, 1.0e-3)):
+        f = {"C": C if step == 1 else np.clip(C + 0.05 * rng.standard_normal(C.shape), 0.0, 1.0)}
+        for k in ("u", "v", "p"):
+            f[k] = rng.standard_normal(C.shape) * (1e3 if k == "p" else 0.1)
+        f["rho"] = f["C"] * 1000.0 + (1.0 - f["C"]) * 100.0
+        f["mu"] = f["C"] * 10.0 + (1.0 - f["C"]) * 1.0
+        f["T"] = np.full(C.shape, 300.0) if step == 1 else 280.0 + 40.0 * rng.random(C.shape)
+        f["curvature"] = rng.standard_normal(C.shape) * 5.0
+        f["C_s"] = np.clip(f["C"] + 0.01 * rng.standard_normal(C.shape), 0.0, 1.0)
+        snaps.append((step, t, dt, f))
+    keys = ("C", "u", "v", "p", "rho", "mu", "T", "curvature", "C_s")
+    payload = i32(nx, ny, FS, FS, DI, DI, len(snaps)) + f64(lx, ly)
+    for step, t, dt, f in snaps:
+        payload += i32(step) + f64(t, dt) + b"".join(f[k].astype("<f8").tobytes() for k in keys)
+    with tempfile.TemporaryDirectory() as td:
+        fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fi, "wb") as fh:
+            fh.write(payload)
+        subprocess.run([HARNESS, "output", fi, fo], check=True, cwd=td)
+        d = fo + "_dir"
+        texts = {"history": open(os.path.join(d, "case_history.csv")).read()}
+        for step, _, _, _ in snaps:
+            texts[f"vtr_{step}"] = open(os.path.join(d, f"case_p0_{step:06d}.vtr")).read()
+            texts[f"pvtr_{step}"] = open(os.path.join(d, f"case_{step:06d}.pvtr")).read()
+    arrays = {f"{k}_{i}": snaps[i][3][k] for i in range(len(snaps)) for k in keys}
+    np.savez_compressed(os.path.join(HERE, "output_writer.npz"), nx=nx, ny=ny, lx=lx, ly=ly,
+                        bc=np.array((FS, FS, DI, DI)), steps=np.array([s_[0] for s_ in snaps]),
+                        times=np.array([s_[1] for s_ in snaps]), dts=np.array([s_[2] for s_ in snaps]),
+                        flags=FLAGS, **arrays, **{k: np.array(v) for k, v in texts.items()})
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep"}
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output"}
     if "surface" in groups:
         surface_cases()
     if "momentum" in groups:
         momentum_cases()
     if "spstep" in groups:
         spstep_cases()
+    if "output" in groups:
+        output_cases()
     if not groups & {"geom", "recon", "advect"}:
         print("golden fixtures written to", HERE)
         return

@@ -134,3 +134,32 @@ def test_sp_run_bitwise(orc, path):
                 assert np.array_equal(a, g[k]), k
     for a, k in ((R.C, "C"), (R.u, "u"), (R.v, "v"), (R.p, "p")):
         assert np.array_equal(a, g[k]), k
+
+
+def test_output_files_text(orc):
+    """vtk_writer_t's files: the oracle's history quantities through the
+    Fortran ES14.6 edit give the reference's _history.csv character for
+    character, and the .vtr / .pvtr builders give the reference's files from
+    the same cell data (divergence and cell-centred velocity formed as
+    vtk_write_step forms them)."""
+    from covof_amd.output import HISTORY_HEADER, fortran_es, pvtr_text, vtr_text
+    g = np.load(os.path.join(GOLDEN, "output_writer.npz"), allow_pickle=False)
+    nx, ny, lx, ly = int(g["nx"]), int(g["ny"]), float(g["lx"]), float(g["ly"])
+    m = orc.mesh(nx, ny, lx, ly, tuple(int(b) for b in g["bc"]))
+    lines = [HISTORY_HEADER]
+    for k, step in enumerate(g["steps"]):
+        f = {n: g[f"{n}_{k}"] for n in ("C", "u", "v", "p", "rho", "mu", "T", "curvature", "C_s")}
+        h = orc.history(m, f["C"], f["u"], f["v"], f["p"], f["rho"], f["T"])
+        lines.append(f"{int(step)}," + ",".join(fortran_es(x, 14, 6)
+                                                for x in (float(g["times"][k]), float(g["dts"][k]), *h)))
+        u, v = f["u"], f["v"]
+        dx, dy = lx / nx, ly / ny
+        arrays = {n: f[n][1:ny + 1, 1:nx + 1] for n in ("C", "C_s", "p", "rho", "mu", "T", "curvature")}
+        arrays["div"] = (u[1:ny + 1, 2:nx + 2] - u[1:ny + 1, 1:nx + 1]) / dx + \
+            (v[2:ny + 2, 1:nx + 1] - v[1:ny + 1, 1:nx + 1]) / dy
+        uc = 0.5 * (u[1:ny + 1, 1:nx + 1] + u[1:ny + 1, 2:nx + 2])
+        vc = 0.5 * (v[1:ny + 1, 1:nx + 1] + v[2:ny + 2, 1:nx + 1])
+        assert vtr_text(nx, ny, dx, dy, arrays, uc, vc) == str(g[f"vtr_{step}"])
+        assert pvtr_text(nx, ny, "case", int(step)) == str(g[f"pvtr_{step}"])
+    assert "\n".join(lines) + "\n" == str(g["history"])
+    assert fortran_es(1.5e-123, 14, 6) == "  1.500000-123" and fortran_es(-2.0e150, 16, 8) == " -2.00000000+150"
