@@ -45,10 +45,10 @@ class StandardProjectionRun:
                  convective_scheme=L.CONV_UPWIND, convective_linearization=L.CONV_LIN_PICARD, gx=0.0, gy=0.0,
                  bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, reconstruction_method="PLIC",
                  predictor=(L.LINEAR_CG, L.PRECOND_JACOBI, 1e-10, 500),
-                 poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64)):
+                 poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True):
         self.ctx = ctx
         self.rho1, self.rho2, self.mu1, self.mu2, self.sigma = (float(a) for a in (rho1, rho2, mu1, mu2, sigma))
-        self.opts = dict(time_scheme=time_scheme, convective_scheme=convective_scheme,
+        self.opts = dict(time_scheme=time_scheme, enable_convection=bool(convection), convective_scheme=convective_scheme,
                          convective_linearization=convective_linearization, gx=gx, gy=gy, bc_u=tuple(bc_u),
                          bc_v=tuple(bc_v))
         self.recon = reconstruction_method

@@ -31,6 +31,9 @@
 !   output  vtk_writer_t on given fields: init, then write_step and
 !           write_diagnostics per snapshot, finalize (src/module_vtk_writer.f90:
 !           46-545); the files go to <output.bin>_dir/
+!   sim     the reference's own program on a JSON case: sim%init(comm, file)
+!           then sim%run() (src/module_simulation.f90:96-351, :416-755); the
+!           JSON path is the input file, the outputs go where it says
 !   spstep  the body of sim_run's time loop for a two-phase standard-projection
 !           run (src/module_simulation.f90:495-559): n-1 / old swaps,
 !           advect_vof (WY + properties + CSF), apply_velocity_bc,
@@ -62,7 +65,9 @@ program covof_ref_harness
   call get_command_argument(1, mode)
   call get_command_argument(2, fin)
   call get_command_argument(3, fout)
-  open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old', action='read')
+  ! the sim mode reads its JSON case through the reference's own loader
+  if (trim(mode) /= 'sim') open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old', &
+       action='read')
   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
 
   select case (trim(mode))
@@ -80,12 +85,14 @@ program covof_ref_harness
      call run_spstep()
   case ('output')
      call run_output()
+  case ('sim')
+     call run_sim()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
   end select
 
-  close(uin)
+  if (trim(mode) /= 'sim') close(uin)
   close(uout)
   call MPI_Finalize(ierr)
 
@@ -346,6 +353,16 @@ contains
        write(uout) iters
     end do
   end subroutine run_spstep
+
+  subroutine run_sim()
+    type(simulation_t) :: sim
+    if (.not. sim%init(MPI_COMM_WORLD, trim(fin))) then
+       write(*, '(A)') 'sim%init failed'
+       call MPI_Abort(MPI_COMM_WORLD, 3, ierr)
+    end if
+    call sim%run()
+    write(uout) 1
+  end subroutine run_sim
 
   subroutine run_output()
     integer :: nx, ny, bc(4), nsnap, s, step

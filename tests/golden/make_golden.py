@@ -7,7 +7,7 @@ compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep, output)
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep, output, case)
 """
 from __future__ import annotations
 
@@ -313,10 +313,49 @@ def output_cases():
                         flags=FLAGS, **arrays, **{k: np.array(v) for k, v in texts.items()})
 
 
+CASE_JSON = {
+    "case_name": "bubble",
+    "mode": "NAVIER_STOKES",
+    "mesh": {"nx": 32, "ny": 64, "lx": 1.0, "ly": 2.0},
+    "physics": {"enable_two_phase": True, "enable_surface_tension": True, "enable_convection": True,
+                "sigma": 24.5, "gx": 0.0, "gy": -0.98, "convective_scheme": "TVD_SUPERBEE",
+                "fluid1": {"rho": 1000.0, "mu": 10.0}, "fluid2": {"rho": 100.0, "mu": 1.0}},
+    "time": {"time_scheme": "BDF2", "dt": 0.001, "t_final": 0.0125},
+    "solver": {"type": "STANDARD_PROJECTION", "linear_solver": "CG", "preconditioner": "JACOBI", "tol": 1e-12,
+               "max_iter": 500},
+    "vof": {"advection_method": "WEYMOUTH_YUE", "reconstruction_method": "PLIC"},
+    "boundary_conditions": {"left": {"type": "FREE_SLIP"}, "right": {"type": "FREE_SLIP"},
+                            "bottom": {"type": "DIRICHLET"}, "top": {"type": "DIRICHLET"}},
+    "initial_condition": {"type": "BUBBLE", "center_x": 0.5, "center_y": 0.5, "radius": 0.25},
+    "output": {"output_dir": "OUTDIR", "vtk_interval": 5, "log_interval": 5, "write_csv_history": True},
+}
+
+
+def case_run():
+    """The reference program on a JSON case (sim%init + sim%run): a 32 x 64
+    rising bubble to t = 0.0125 (12 steps of 1e-3 and a clamped last step of
+    5e-4); the history file and the .vtr files of steps 0, 5, 10, 13."""
+    import json
+    with tempfile.TemporaryDirectory() as td:
+        cfg = json.loads(json.dumps(CASE_JSON))
+        cfg["output"]["output_dir"] = os.path.join(td, "out")
+        path = os.path.join(td, "case.json")
+        with open(path, "w") as f:
+            json.dump(cfg, f, indent=1)
+        subprocess.run([HARNESS, "sim", path, os.path.join(td, "o.bin")], check=True, cwd=td,
+                       stdout=subprocess.DEVNULL)
+        d = cfg["output"]["output_dir"]
+        texts = {"history": open(os.path.join(d, "bubble_history.csv")).read()}
+        for step in (0, 5, 10, 13):
+            texts[f"vtr_{step}"] = open(os.path.join(d, f"bubble_p0_{step:06d}.vtr")).read()
+    np.savez_compressed(os.path.join(HERE, "case_bubble_json.npz"), json=np.array(json.dumps(CASE_JSON)),
+                        flags=FLAGS, **{k: np.array(v) for k, v in texts.items()})
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output"}
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output", "case"}
     if "surface" in groups:
         surface_cases()
     if "momentum" in groups:
@@ -325,6 +364,8 @@ def main():
         spstep_cases()
     if "output" in groups:
         output_cases()
+    if "case" in groups:
+        case_run()
     if not groups & {"geom", "recon", "advect"}:
         print("golden fixtures written to", HERE)
         return

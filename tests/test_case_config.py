@@ -1,0 +1,67 @@
+"""The JSON case front end (host logic, no GPU): key mapping and defaults of
+the reference's load_config_from_json, the supported-subset check, and the
+initial volume fraction against the compiled reference's step-0 file."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _case(tmp_path, edit=None):
+    from covof_amd import case
+    g = np.load(os.path.join(GOLDEN, "case_bubble_json.npz"), allow_pickle=False)
+    cfg = json.loads(str(g["json"]))
+    cfg["output"]["output_dir"] = str(tmp_path / "out")
+    if edit:
+        edit(cfg)
+    path = tmp_path / "case.json"
+    path.write_text(json.dumps(cfg))
+    return case.load_config(str(path)), g
+
+
+def test_load_config_keys_and_defaults(tmp_path):
+    from covof_amd import _lib as L
+    c, _ = _case(tmp_path)
+    assert (c["nx"], c["ny"], c["lx"], c["ly"]) == (32, 64, 1.0, 2.0)
+    assert c["time_scheme"] == L.TIME_SCHEME_GEAR and c["convective_scheme"] == L.CONV_TVD_SUPERBEE
+    assert c["convective_linearization"] == L.CONV_LIN_PICARD and c["linear_solver"] == L.LINEAR_CG
+    assert c["precond"] == L.PRECOND_JACOBI and c["bc_left"] == L.BC_FREESLIP and c["bc_top"] == L.BC_DIRICHLET
+    assert c["case_name"] == "bubble" and c["vtk_interval"] == 5 and c["write_csv"]
+    # reference defaults where the file is silent
+    c2, _ = _case(tmp_path, lambda j: (j.pop("solver"), j["physics"].pop("fluid2")))
+    assert c2["linear_solver"] == L.LINEAR_BICGSTAB2 and c2["tol"] == 1e-5 and c2["max_iter"] == 300
+    assert c2["solver_type"] == "AUGMENTED_LAGRANGIAN" and c2["rho2"] == c2["rho1"] and c2["mu2"] == c2["mu1"]
+
+
+@pytest.mark.parametrize("edit,key", [
+    (lambda j: j["solver"].update(type="MONOLITHIC"), "solver.type"),
+    (lambda j: j["solver"].update(preconditioner="SSOR"), "solver.preconditioner"),
+    (lambda j: j["time"].update(adaptive_dt=True), "time.adaptive_dt"),
+    (lambda j: j["vof"].update(advection_method="CIAM"), "vof.advection_method"),
+    (lambda j: j["boundary_conditions"]["top"].update(type="NEUMANN"), "boundary_conditions.top"),
+    (lambda j: j["boundary_conditions"]["left"].update(profile="PARABOLIC"), "boundary_conditions.left.profile"),
+], ids=["monolithic", "ssor", "adaptive-dt", "ciam", "neumann-top", "parabolic"])
+def test_unsupported_settings_are_named(tmp_path, edit, key):
+    from covof_amd import case
+    c, _ = _case(tmp_path, edit)
+    with pytest.raises(ValueError, match=key.split(" ")[0]):
+        case.check_supported(c)
+
+
+def test_initial_volume_fraction_matches_reference_step0(tmp_path):
+    """set_initial_conditions' BUBBLE by 16 x 16 sub-cell counts (PLIC), and
+    the VOF halo: the interior, printed with the reference's ES16.8 edit,
+    is the PhaseIndicator array of the reference's step-0 file."""
+    from covof_amd import case
+    from covof_amd.output import _rows
+    c, g = _case(tmp_path)
+    C = case.initial_volume_fraction(c)
+    case._vof_halo(C, c)
+    v = str(g["vtr_0"])
+    head = 'Name="PhaseIndicator" format="ascii">\n'
+    start = v.index(head) + len(head)
+    end = v.index("        </DataArray>", start)
+    assert v[start:end] == _rows(C[1:c["ny"] + 1, 1:c["nx"] + 1])
