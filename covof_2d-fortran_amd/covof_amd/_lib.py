@@ -20,7 +20,8 @@ STATUS_NAMES = {OK: "ok", NOT_CONVERGED: "not converged", BREAKDOWN: "breakdown"
 # reference selector values (src/module_constants.f90)
 BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
 LINEAR_BICGSTAB2, LINEAR_CG = 1, 3
-PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR, PRECOND_BLOCK_TRIANGULAR = 0, 1, 2, 4
+SOLVER_MONOLITHIC, SOLVER_STANDARD_PROJECTION = 2, 3
 # parallel preconditioners of this library (no reference counterpart)
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -33,7 +34,8 @@ class MomentumCfg(ct.Structure):
     """covof_momentum_cfg (include/covof_hip.h)."""
     _fields_ = [("time_scheme", ct.c_int), ("has_nm1", ct.c_int), ("enable_convection", ct.c_int),
                 ("convective_scheme", ct.c_int), ("convective_linearization", ct.c_int),
-                ("gx", ct.c_double), ("gy", ct.c_double), ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4)]
+                ("gx", ct.c_double), ("gy", ct.c_double), ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4),
+                ("solver_type", ct.c_int)]
 
 # every symbol the header declares: name -> (restype, argtypes)
 _P = ct.c_void_p
@@ -68,15 +70,18 @@ SIGNATURES = {
     "covof_sp_correct_d": (_I, [_P, _P, _P, _P, _D, _P, _P]),
     "covof_sp_correct": (_I, [_P, _DP, _DP, _DP, _D, _DP, _DP]),
     "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
-    "covof_momentum_size": (_I, [_P, _IP, _IP, _IP]),
+    "covof_momentum_size": (_I, [_P, _I, _IP, _IP, _IP]),
     "covof_momentum_set_fields_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D]),
     "covof_momentum_diag_d": (_I, [_P, _P]),
     "covof_momentum_rhs_d": (_I, [_P, _P]),
     "covof_momentum_matvec_d": (_I, [_P, _P, _P]),
     "covof_momentum_solve_d": (_I, [_P, _P, _P, _I, _I, _D, _I, _IP, _DP]),
+    "covof_momentum_precond_d": (_I, [_P, _I, _P, _P]),
     "covof_velocity_bc_d": (_I, [_P, ct.c_void_p, _P, _P, _I]),
     "covof_sp_predictor_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
     "covof_sp_predictor": (_I, [_P, ct.c_void_p] + [_DP] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
+    "covof_monolithic_step_d": (_I, [_P, ct.c_void_p] + [_P] * 13 + [_D, _I, _I, _D, _I, _IP, _DP]),
+    "covof_monolithic_step": (_I, [_P, ct.c_void_p] + [_DP] * 13 + [_D, _I, _I, _D, _I, _IP, _DP]),
     "covof_history_d": (_I, [_P] + [_P] * 6 + [_DP]),
     "covof_cell_output_d": (_I, [_P] + [_P] * 5),
     "covof_ctx_profile": (_I, [_P, _I]),

@@ -284,23 +284,27 @@ class Context:
     MOMENTUM_FIELDS = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",
                        "fx_sigma", "fy_sigma")
 
-    def momentum_size(self):
-        """(n, u_nx, v_ny) of the SP momentum unknowns (ns_init,
-        src/module_navier_stokes.f90:136-226)."""
+    def momentum_size(self, solver_type=None):
+        """(n, u_nx, v_ny) of the momentum unknowns (ns_init,
+        src/module_navier_stokes.f90:136-226): u, v, and p when monolithic
+        (solver_type; default: that of the bound fields)."""
+        if solver_type is None:
+            solver_type = getattr(self, "_mom_solver", L.SOLVER_STANDARD_PROJECTION)
         n, a, b = ct.c_int(), ct.c_int(), ct.c_int()
-        L.check(self._lib.covof_momentum_size(self._h, ct.byref(n), ct.byref(a), ct.byref(b)),
+        L.check(self._lib.covof_momentum_size(self._h, int(solver_type), ct.byref(n), ct.byref(a), ct.byref(b)),
                 "covof_momentum_size")
         return n.value, a.value, b.value
 
     @staticmethod
     def momentum_cfg(time_scheme=L.TIME_SCHEME_EULER, has_nm1=False, enable_convection=True,
                      convective_scheme=L.CONV_UPWIND, convective_linearization=L.CONV_LIN_PICARD, gx=0.0, gy=0.0,
-                     bc_u=(0.0,) * 4, bc_v=(0.0,) * 4) -> L.MomentumCfg:
+                     bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, solver_type=L.SOLVER_STANDARD_PROJECTION) -> L.MomentumCfg:
         """The config_t fields the momentum block reads (reference defaults:
-        EULER, convection on, UPWIND, PICARD)."""
+        EULER, convection on, UPWIND, PICARD; solver_type STANDARD_PROJECTION
+        or MONOLITHIC)."""
         return L.MomentumCfg(int(time_scheme), int(bool(has_nm1)), int(bool(enable_convection)),
                              int(convective_scheme), int(convective_linearization), float(gx), float(gy),
-                             (ct.c_double * 4)(*bc_u), (ct.c_double * 4)(*bc_v))
+                             (ct.c_double * 4)(*bc_u), (ct.c_double * 4)(*bc_v), int(solver_type))
 
     def _mom_ptrs(self, fields: dict):
         out = []
@@ -320,6 +324,7 @@ class Context:
         self._stream()
         L.check(self._lib.covof_momentum_set_fields_d(self._h, ct.byref(cfg), *self._mom_ptrs(fields), float(dt)),
                 "covof_momentum_set_fields_d")
+        self._mom_solver = cfg.solver_type or L.SOLVER_STANDARD_PROJECTION
 
     def _mom_vec(self, t, name):
         n = self.momentum_size()[0]
@@ -332,7 +337,9 @@ class Context:
         return torch.full((self.momentum_size()[0],), float(fill), dtype=torch.float64, device=self.device)
 
     def assemble_preconditioner_blocks(self, diag=None):
-        """diag_u, diag_v (src/module_navier_stokes.f90:1268-1595) of the bound fields."""
+        """diag_u, diag_v (and diag_p when monolithic) of
+        assemble_preconditioner_blocks (src/module_navier_stokes.f90:1268-1595)
+        of the bound fields."""
         diag = self.new_momentum_vector() if diag is None else self._mom_vec(diag, "diag")
         self._stream()
         L.check(self._lib.covof_momentum_diag_d(self._h, _ptr(diag)), "covof_momentum_diag_d")
@@ -366,6 +373,17 @@ class Context:
         L.check(st, "covof_momentum_solve_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return SolveResult(it.value, rr.value, st)
 
+    def apply_preconditioner(self, r, z=None, precond=L.PRECOND_JACOBI):
+        """z = M^-1 r, apply_preconditioner (src/module_linear_solvers.f90:
+        48-316) with the blocks of the bound fields: NONE, JACOBI, SSOR or
+        BLOCK_TRIANGULAR."""
+        self._mom_vec(r, "r")
+        z = self.new_momentum_vector() if z is None else self._mom_vec(z, "z")
+        self._stream()
+        L.check(self._lib.covof_momentum_precond_d(self._h, int(precond), _ptr(r), _ptr(z)),
+                "covof_momentum_precond_d")
+        return z
+
     def apply_velocity_bc(self, cfg: L.MomentumCfg, u, v, homogeneous=False):
         """sync_u, sync_v, apply_boundary_conditions (src/module_mpi_comm.f90:
         733-775, :290-537) on the halo fields u, v in place."""
@@ -387,6 +405,25 @@ class Context:
                                             int(linear_solver), int(precond), float(tol), int(max_iter),
                                             ct.byref(it), ct.byref(rr))
         L.check(st, "covof_sp_predictor_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        self._mom_solver = L.SOLVER_STANDARD_PROJECTION
+        return SolveResult(it.value, rr.value, st)
+
+    def solve_monolithic_step(self, cfg: L.MomentumCfg, fields: dict, p, dt: float,
+                              linear_solver=L.LINEAR_BICGSTAB2, precond=L.PRECOND_BLOCK_TRIANGULAR, tol=1e-5,
+                              max_iter=300) -> SolveResult:
+        """The monolithic branch of the time step (src/module_simulation.f90:
+        553-583, one Uzawa pass): fields["u"], fields["v"] and p (halo
+        tensors) are replaced in place by the coupled solution.  cfg must
+        have solver_type MONOLITHIC."""
+        self._field(p, "p")
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        ptrs = self._mom_ptrs(fields)
+        st = self._lib.covof_monolithic_step_d(self._h, ct.byref(cfg), ptrs[0], ptrs[1], _ptr(p), *ptrs[2:],
+                                               float(dt), int(linear_solver), int(precond), float(tol),
+                                               int(max_iter), ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_monolithic_step_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        self._mom_solver = L.SOLVER_MONOLITHIC
         return SolveResult(it.value, rr.value, st)
 
     # ------------------------------------------------------------- output

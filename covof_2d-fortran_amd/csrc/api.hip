@@ -41,7 +41,7 @@ using covof::Ctx;
 struct covof_ctx {
   Ctx c;
   // device mirrors for the host-array entry points (made on first use)
-  double *mirror[12] = {nullptr};
+  double *mirror[13] = {nullptr};
 };
 
 static thread_local std::string g_err;
@@ -414,7 +414,9 @@ extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double d
 static int mom_cfg(const covof_momentum_cfg *in, covof::MomCfg *out) {
   if (!in) return COVOF_BAD_ARGUMENT;
   if (in->time_scheme < 1 || in->time_scheme > 2 || in->convective_scheme < 1 || in->convective_scheme > 3 ||
-      in->convective_linearization < 1 || in->convective_linearization > 3)
+      in->convective_linearization < 1 || in->convective_linearization > 3 ||
+      (in->solver_type != 0 && in->solver_type != COVOF_SOLVER_MONOLITHIC &&
+       in->solver_type != COVOF_SOLVER_STANDARD_PROJECTION))
     return COVOF_BAD_ARGUMENT;
   out->gear = in->time_scheme == 2 && in->has_nm1;
   out->conv = in->enable_convection != 0;
@@ -427,16 +429,17 @@ static int mom_cfg(const covof_momentum_cfg *in, covof::MomCfg *out) {
     out->bcu[k] = in->bc_u[k];
     out->bcv[k] = in->bc_v[k];
   }
+  out->mono = in->solver_type == COVOF_SOLVER_MONOLITHIC;
   return COVOF_OK;
 }
 
-extern "C" int covof_momentum_size(covof_ctx *h, int *n, int *u_nx, int *v_ny) {
+extern "C" int covof_momentum_size(covof_ctx *h, int solver_type, int *n, int *u_nx, int *v_ny) {
   if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
   const covof::Mesh &m = h->c.m;
   const int a = m.nx + (m.perx ? 0 : 1), b = m.ny + (m.pery ? 0 : 1);
   if (u_nx) *u_nx = a;
   if (v_ny) *v_ny = b;
-  if (n) *n = a * m.ny + m.nx * b;
+  if (n) *n = a * m.ny + m.nx * b + (solver_type == COVOF_SOLVER_MONOLITHIC ? m.nx * m.ny : 0);
   return COVOF_OK;
 }
 
@@ -482,7 +485,16 @@ extern "C" int covof_momentum_solve_d(covof_ctx *h, const double *b, double *x, 
   if (!h || !b || !x) return fail(COVOF_BAD_ARGUMENT, "null argument");
   const int st = covof::mom_solve(h->c, b, x, lin_solver, precond, tol, maxit, iters, relres);
   if (st == COVOF_BAD_ARGUMENT)
-    return fail(st, "momentum fields not set, or solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI");
+    return fail(st, "momentum fields not set, or solver not BICGSTAB2/CG, or preconditioner not "
+                    "NONE/JACOBI/SSOR/BLOCK_TRIANGULAR");
+  return hip_status(st);
+}
+
+extern "C" int covof_momentum_precond_d(covof_ctx *h, int precond, const double *r, double *z) {
+  if (!h || !r || !z) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::mom_apply_precond(h->c, precond, r, z);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "momentum fields not set, or preconditioner not NONE/JACOBI/SSOR/BLOCK_TRIANGULAR");
   return hip_status(st);
 }
 
@@ -507,7 +519,7 @@ extern "C" int covof_sp_predictor_d(covof_ctx *h, const covof_momentum_cfg *cfg,
   if (st) return st;
   st = covof::mom_predictor(h->c, u, v, lin_solver, precond, tol, maxit, iters, relres);
   if (st == COVOF_BAD_ARGUMENT)
-    return fail(st, "solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI");
+    return fail(st, "monolithic cfg, solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI/SSOR");
   return hip_status(st);
 }
 
@@ -533,6 +545,51 @@ extern "C" int covof_sp_predictor(covof_ctx *h, const covof_momentum_cfg *cfg, d
   if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
   st = d2h(h, u, d[0]);
   if (!st) st = d2h(h, v, d[1]);
+  if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return st ? hip_status(st) : sst;
+}
+
+extern "C" int covof_monolithic_step_d(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,
+                                      double *p, const double *u_old, const double *v_old, const double *u_nm1,
+                                      const double *v_nm1, const double *rho_u, const double *rho_v,
+                                      const double *mu, const double *mu_node, const double *fx_sigma,
+                                      const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                                      int maxit, int *iters, double *relres) {
+  if (!cfg || cfg->solver_type != COVOF_SOLVER_MONOLITHIC)
+    return fail(COVOF_BAD_ARGUMENT, "covof_monolithic_step_d: cfg->solver_type must be SOLVER_MONOLITHIC");
+  if (!p) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  int st = covof_momentum_set_fields_d(h, cfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node,
+                                       fx_sigma, fy_sigma, dt);
+  if (st) return st;
+  st = covof::mono_step(h->c, u, v, p, lin_solver, precond, tol, maxit, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "solver not BICGSTAB2/CG, or preconditioner not NONE/JACOBI/SSOR/BLOCK_TRIANGULAR");
+  return hip_status(st);
+}
+
+extern "C" int covof_monolithic_step(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v, double *p,
+                                     const double *u_old, const double *v_old, const double *u_nm1,
+                                     const double *v_nm1, const double *rho_u, const double *rho_v,
+                                     const double *mu, const double *mu_node, const double *fx_sigma,
+                                     const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                                     int maxit, int *iters, double *relres) {
+  if (!h || !u || !v || !p || !u_old || !v_old || !rho_u || !rho_v || !mu || !mu_node || !fx_sigma || !fy_sigma)
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const double *hs[13] = {u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma, p};
+  double *d[13] = {nullptr};
+  int st = 0;
+  for (int k = 0; k < 13 && !st; ++k) {
+    if (!hs[k]) continue;
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+    st = h2d(h, d[k], hs[k]);
+  }
+  if (st) return hip_status(st);
+  const int sst = covof_monolithic_step_d(h, cfg, d[0], d[1], d[12], d[2], d[3], d[4], d[5], d[6], d[7], d[8],
+                                          d[9], d[10], d[11], dt, lin_solver, precond, tol, maxit, iters, relres);
+  if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
+  st = d2h(h, u, d[0]);
+  if (!st) st = d2h(h, v, d[1]);
+  if (!st) st = d2h(h, p, d[12]);
   if (!st && hipStreamSynchronize(h->c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
   return st ? hip_status(st) : sst;
 }

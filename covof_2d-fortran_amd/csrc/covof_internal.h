@@ -46,6 +46,7 @@ struct MomCfg {
   int ab_nm1;    // ADAMS_BASHFORTH with a valid n-1 state: extrapolated advecting velocity
   double gx, gy;
   double bcu[4], bcv[4];
+  int mono;      // SOLVER_MONOLITHIC: unknowns u, v, p; pressure gradient and continuity rows
 };
 // The fields the momentum operator, its diagonal and right-hand side read
 // (the reference's fields_old), halo layout.
@@ -56,13 +57,18 @@ struct MomFields {
 struct Mom {
   int bound = 0;              // fields and options set
   int u_nx = 0, v_ny = 0;     // unknowns per row of u, rows of v
-  long nu = 0, n = 0;         // u unknowns, all unknowns
+  long nu = 0, nuv = 0, n = 0;  // u unknowns, u + v unknowns, all unknowns (+ p when monolithic)
   MomCfg cfg;
   MomFields f;
   double dt = 0.0;
-  double *tu = nullptr, *tv = nullptr;  // the operator's persistent work fields (halo layout, zero at first use)
+  double *tu = nullptr, *tv = nullptr, *tp = nullptr;  // the operator's persistent work fields (halo layout, zero at first use)
   double *diag = nullptr, *rhs = nullptr, *x = nullptr;
   double *w[9] = {nullptr};   // Krylov work vectors
+  // assemble_preconditioner_blocks' SSOR and Schur coefficients (made on first use):
+  // u_a{W,E,S,N,P}, v_a{W,E,S,N,P}, schur_a{W,E,S,N,P}
+  double *bk[15] = {nullptr};
+  double *aP2[2] = {nullptr};  // u_aP, v_aP on the halo grid (the Schur coefficients' source)
+  double *sx[3] = {nullptr};   // halo work fields of the three SSOR block solves
 };
 
 // A linear operator with its preconditioner for the generic Krylov loops
@@ -208,12 +214,16 @@ int krylov_cg(Ctx &c, LinOp &op, const double *b, double *x, double *const w[4],
 int mom_bind(Ctx &c, const MomCfg &cfg, const MomFields &f, double dt);
 int mom_diag(Ctx &c, double *diag);
 int mom_rhs(Ctx &c, double *rhs);
-int mom_pack(Ctx &c, const double *u, const double *v, double *x);
+int mom_pack(Ctx &c, const double *u, const double *v, const double *p, double *x);
 int mom_matvec(Ctx &c, const double *x, double *y);
 int mom_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol, int maxit,
               int *iters, double *relres);
 int mom_predictor(Ctx &c, double *u, double *v, int lin_solver, int precond, double tol, int maxit,
                   int *iters, double *relres);
+int mono_step(Ctx &c, double *u, double *v, double *p, int lin_solver, int precond, double tol, int maxit,
+              int *iters, double *relres);
+int mom_precond(Ctx &c, int precond, const double *r, double *z);
+int mom_apply_precond(Ctx &c, int precond, const double *r, double *z);
 void mom_free(Ctx &c);
 int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous);
 // diagnostics.hip: output-side reductions

@@ -128,20 +128,35 @@ __device__ void v_fluxes(const Mesh &m, const MomCfg &c, const MomFields &F, int
   FS = 0.5 * (v_n_S + v_n) * m.dx;
 }
 
-// x -> interior unknowns of the halo fields u, v (ns_unpack_vector's copy loops)
-__global__ void __launch_bounds__(BLK) k_unpack(Mesh m, long nu, long n, int u_nx, const double *__restrict__ x,
-                                                double *__restrict__ u, double *__restrict__ v) {
+// x -> interior unknowns of the halo fields u, v (and p beyond nuv: the
+// monolithic system) -- ns_unpack_vector's copy loops
+__global__ void __launch_bounds__(BLK) k_unpack(Mesh m, long nu, long nuv, long n, int u_nx,
+                                                const double *__restrict__ x, double *__restrict__ u,
+                                                double *__restrict__ v, double *__restrict__ p) {
   const long k = (long)blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
+  if (k >= nuv) {
+    const long kp = k - nuv;
+    const int j = (int)(kp / m.nx) + 1;
+    p[HX((int)(kp - (long)(j - 1) * m.nx) + 1, j)] = x[k];
+    return;
+  }
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   (d.isv ? v : u)[HX(d.i, d.j)] = x[k];
 }
 
 // ns_pack_vector
-__global__ void __launch_bounds__(BLK) k_pack(Mesh m, long nu, long n, int u_nx, const double *__restrict__ u,
-                                              const double *__restrict__ v, double *__restrict__ x) {
+__global__ void __launch_bounds__(BLK) k_pack(Mesh m, long nu, long nuv, long n, int u_nx,
+                                              const double *__restrict__ u, const double *__restrict__ v,
+                                              const double *__restrict__ p, double *__restrict__ x) {
   const long k = (long)blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
+  if (k >= nuv) {
+    const long kp = k - nuv;
+    const int j = (int)(kp / m.nx) + 1;
+    x[k] = p[HX((int)(kp - (long)(j - 1) * m.nx) + 1, j)];
+    return;
+  }
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   x[k] = (d.isv ? v : u)[HX(d.i, d.j)];
 }
@@ -150,34 +165,34 @@ __global__ void __launch_bounds__(BLK) k_pack(Mesh m, long nu, long n, int u_nx,
 // all rows, then periodic rows over all columns) and apply_velocity_bc, in
 // the reference's statement order: one workgroup, a barrier between every
 // step whose reads depend on an earlier step's writes.
+// The single-rank exchange_halos_scalar of one halo field inside a
+// workgroup (all threads of the block call it).
+__device__ void wg_exchange(const Mesh &m, double *f) {
+  const int nx = m.nx, ny = m.ny, t = threadIdx.x, T = blockDim.x;
+  if (m.perx) {
+    for (int j = t; j <= ny + 1; j += T) f[HX(0, j)] = f[HX(nx, j)];
+    __syncthreads();
+    for (int j = t; j <= ny + 1; j += T) f[HX(nx + 1, j)] = f[HX(1, j)];
+    __syncthreads();
+  }
+  if (m.pery) {
+    for (int i = t; i <= nx + 1; i += T) f[HX(i, 0)] = f[HX(i, ny)];
+    __syncthreads();
+    for (int i = t; i <= nx + 1; i += T) f[HX(i, ny + 1)] = f[HX(i, 1)];
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_vel_halo(Mesh m, MomCfg c, double *__restrict__ u,
-                                                   double *__restrict__ v, int homogeneous, int exchange) {
+                                                   double *__restrict__ v, int homogeneous, int exchange,
+                                                   double *__restrict__ p) {
   const int nx = m.nx, ny = m.ny, t = threadIdx.x, T = blockDim.x;
   const int *bc = m.bc;
   const bool bl = !m.perx, bb = !m.pery;  // physical sides (single rank)
-  if (exchange && m.perx) {
-    for (int j = t; j <= ny + 1; j += T) {
-      u[HX(0, j)] = u[HX(nx, j)];
-      v[HX(0, j)] = v[HX(nx, j)];
-    }
-    __syncthreads();
-    for (int j = t; j <= ny + 1; j += T) {
-      u[HX(nx + 1, j)] = u[HX(1, j)];
-      v[HX(nx + 1, j)] = v[HX(1, j)];
-    }
-    __syncthreads();
-  }
-  if (exchange && m.pery) {
-    for (int i = t; i <= nx + 1; i += T) {
-      u[HX(i, 0)] = u[HX(i, ny)];
-      v[HX(i, 0)] = v[HX(i, ny)];
-    }
-    __syncthreads();
-    for (int i = t; i <= nx + 1; i += T) {
-      u[HX(i, ny + 1)] = u[HX(i, 1)];
-      v[HX(i, ny + 1)] = v[HX(i, 1)];
-    }
-    __syncthreads();
+  if (exchange) {
+    if (p) wg_exchange(m, p);  // sync_cell_scalar of the monolithic pressure
+    wg_exchange(m, u);
+    wg_exchange(m, v);
   }
   // apply_velocity_bc (src/module_mpi_comm.f90:290-537), UNIFORM profiles
   const double u_l = homogeneous ? 0.0 : c.bcu[0], v_l = homogeneous ? 0.0 : c.bcv[0];
@@ -290,7 +305,7 @@ __global__ void __launch_bounds__(1024) k_vel_halo(Mesh m, MomCfg c, double *__r
 // row k of A x from the unpacked work fields tu, tv.
 __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const MomFields &F, double dt, long k,
                                           long nu, int u_nx, int v_ny, const double *__restrict__ tu,
-                                          const double *__restrict__ tv) {
+                                          const double *__restrict__ tv, const double *__restrict__ tp) {
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   const int i = d.i, j = d.j, nx = m.nx, ny = m.ny;
   const double dx = m.dx, dy = m.dy, dvol = dx * dy;
@@ -338,6 +353,7 @@ __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const 
     const double shN = mnN * ((TU(i, j + 1) - TU(i, j)) / dy + (TV(i, j + 1) - TV(i - 1, j + 1)) / dx) * dx;
     const double shS = mnS * ((TU(i, j) - TU(i, j - 1)) / dy + (TV(i, j) - TV(i - 1, j)) / dx) * dx;
     val = val - (shN - shS);
+    if (c.mono) val = val + (tp[HX(i, j)] - tp[HX(i - 1, j)]) * dy;  // B_u^T p
     return val;
   } else {
     const double *v = F.v;
@@ -374,6 +390,7 @@ __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const 
     const double shE = mnE * ((TV(i + 1, j) - TV(i, j)) / dx + (TU(i + 1, j) - TU(i + 1, j - 1)) / dy) * dy;
     const double shW = mnW * ((TV(i, j) - TV(i - 1, j)) / dx + (TU(i, j) - TU(i, j - 1)) / dy) * dy;
     val = val - (shE - shW);
+    if (c.mono) val = val + (tp[HX(i, j)] - tp[HX(i, j - 1)]) * dx;  // B_v^T p
     return val;
   }
 #undef TU
@@ -384,13 +401,24 @@ __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const 
 // preconditioner (apply_PA, src/module_linear_solvers.f90:323-356): the same
 // row value divided as apply_preconditioner divides it.
 template <bool JAC>
-__global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields F, double dt, long nu, long n,
-                                                    int u_nx, int v_ny, const double *__restrict__ tu,
-                                                    const double *__restrict__ tv, const double *__restrict__ diag,
-                                                    double *__restrict__ y) {
+__global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields F, double dt, long nu, long nuv,
+                                                    long n, int u_nx, int v_ny, const double *__restrict__ tu,
+                                                    const double *__restrict__ tv, const double *__restrict__ tp,
+                                                    const double *__restrict__ diag, double *__restrict__ y) {
   const long k = (long)blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
-  const double val = mom_row(m, c, F, dt, k, nu, u_nx, v_ny, tu, tv);
+  double val;
+  if (k >= nuv) {
+    // continuity row of the monolithic system: div(u) dvol, and the
+    // regularisation 1e-10 p(1,1) dvol at cell (1,1)
+    const long kp = k - nuv;
+    const int j = (int)(kp / m.nx) + 1, i = (int)(kp - (long)(j - 1) * m.nx) + 1;
+    const double dvol = m.dx * m.dy;
+    const double div = (tu[HX(i + 1, j)] - tu[HX(i, j)]) / m.dx + (tv[HX(i, j + 1)] - tv[HX(i, j)]) / m.dy;
+    val = (i == 1 && j == 1) ? div * dvol + 1.0e-10 * tp[HX(1, 1)] * dvol : div * dvol;
+  } else {
+    val = mom_row(m, c, F, dt, k, nu, u_nx, v_ny, tu, tv, tp);
+  }
   if (JAC) {
     const double dk = diag[k];
     y[k] = fabs(dk) > 1.0e-15 ? val / dk : val;
@@ -399,12 +427,27 @@ __global__ void __launch_bounds__(BLK) k_mom_matvec(Mesh m, MomCfg c, MomFields 
   }
 }
 
-// ns_assemble_preconditioner_blocks: diag_u, diag_v
-// (src/module_navier_stokes.f90:1268-1595; no AL, Brinkman or obstacle terms)
-__global__ void __launch_bounds__(BLK) k_mom_diag(Mesh m, MomCfg c, MomFields F, double dt, long nu, long n,
-                                                  int u_nx, int v_ny, double *__restrict__ diag) {
+// Outputs of the blocks kernel; any pointer may be null.  u*: (u_nx, ny),
+// v*: (nx, v_ny) block coefficients; uaP2, vaP2: u_aP, v_aP on the halo
+// grid, the Schur coefficients' source.
+struct BlkOut {
+  double *diag;
+  double *uW, *uE, *uS, *uN, *uP;
+  double *vW, *vE, *vS, *vN, *vP;
+  double *uaP2, *vaP2;
+};
+#define PUT(arr, val)                    \
+  do {                                   \
+    if (o.arr) o.arr[dof] = (val);       \
+  } while (0)
+
+// ns_assemble_preconditioner_blocks (src/module_navier_stokes.f90:1268-1595;
+// no AL, Brinkman or obstacle terms) over the u and v unknowns: diag_u,
+// diag_v, the SSOR coefficients u_a*, v_a*, and u_aP / v_aP on the halo grid.
+__global__ void __launch_bounds__(BLK) k_mom_blocks(Mesh m, MomCfg c, MomFields F, double dt, long nu, long nuv,
+                                                    int u_nx, int v_ny, BlkOut o) {
   const long k = (long)blockIdx.x * BLK + threadIdx.x;
-  if (k >= n) return;
+  if (k >= nuv) return;
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   const int i = d.i, j = d.j, nx = m.nx, ny = m.ny;
   const double dx = m.dx, dy = m.dy, dvol = dx * dy;
@@ -412,46 +455,87 @@ __global__ void __launch_bounds__(BLK) k_mom_diag(Mesh m, MomCfg c, MomFields F,
   const double at = c.gear ? 1.5 : 1.0;
   const bool bl = !m.perx, bb = !m.pery;
   if (!d.isv) {
+    const long dof = k;
     const double rho_u = F.ru[HX(i, j)];
     double Du = (at * rho_u / dt_safe) * dvol;
     if (Du <= 1.0e-30) Du = 1.0;
     if ((i == 1 && bl) || (i == u_nx && bl)) {
-      diag[k] = Du;
+      const bool west = i == 1;
+      if (o.diag) o.diag[k] = Du;
+      if (o.uaP2) o.uaP2[HX(i, j)] = Du;
+      const double nb = (m.bc[west ? 0 : 1] == BC_NEUMANN && u_nx >= 2) ? Du : 0.0;
+      PUT(uW, west ? 0.0 : nb);
+      PUT(uE, west ? nb : 0.0);
+      PUT(uS, 0.0);
+      PUT(uN, 0.0);
+      PUT(uP, Du);
       return;
     }
     const double mu_e = F.mu[HX(i, j)], mu_w = F.mu[HX(i - 1, j)];
     const double cE = 2.0 * mu_e * (dy / dx), cW = 2.0 * mu_w * (dy / dx);
     const double cN_raw = F.mn[HX(i, j + 1)] * (dx / dy), cS_raw = F.mn[HX(i, j)] * (dx / dy);
-    double cN_visc = cN_raw, cS_visc = cS_raw, dwy = 0.0;
+    double cN_visc = cN_raw, cS_visc = cS_raw, dwy = 0.0, cN_extra = 0.0, cS_extra = 0.0;
     if (j == 1 && bb) {
-      if (m.bc[2] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cS_raw : dwy + 2.0 * cS_raw;
+      if (m.bc[2] == BC_DIRICHLET) {
+        if (ny >= 2) {
+          dwy = dwy + 3.0 * cS_raw;
+          cN_extra = cN_extra + cS_raw / 3.0;
+        } else {
+          dwy = dwy + 2.0 * cS_raw;
+        }
+      }
       cS_visc = 0.0;
     }
     if (j == ny && bb) {
-      if (m.bc[3] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cN_raw : dwy + 2.0 * cN_raw;
+      if (m.bc[3] == BC_DIRICHLET) {
+        if (ny >= 2) {
+          dwy = dwy + 3.0 * cN_raw;
+          cS_extra = cS_extra + cN_raw / 3.0;
+        } else {
+          dwy = dwy + 2.0 * cN_raw;
+        }
+      }
       cN_visc = 0.0;
     }
-    double conv_diag = 0.0, newt = 0.0;
+    double conv_diag = 0.0, conv_W = 0.0, conv_E = 0.0, conv_S = 0.0, conv_N = 0.0, newt = 0.0;
     if (c.conv) {
       double FE, FW, FN, FS;
       u_fluxes(m, c, F, i, j, FE, FW, FN, FS);
       conv_diag = rho_u * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+      conv_E = rho_u * dmax(-FE, 0.0);
+      conv_W = rho_u * dmax(FW, 0.0);
+      conv_N = (j == ny && bb) ? 0.0 : rho_u * dmax(-FN, 0.0);
+      conv_S = (j == 1 && bb) ? 0.0 : rho_u * dmax(FS, 0.0);
       if (c.lin == 3) {
         const double du_dx = (F.u[HX(i + 1, j)] - F.u[HX(i - 1, j)]) / (2.0 * dx);
         newt = rho_u * dmax(0.0, du_dx) * dvol;
       }
     }
+    PUT(uW, cW + conv_W + 0.0);
+    PUT(uE, cE + conv_E + 0.0);
+    PUT(uS, cS_visc + conv_S + cS_extra);
+    PUT(uN, cN_visc + conv_N + cN_extra);
     double aP = Du + dwy + conv_diag + 0.0 + newt + cW + cE + cS_visc + cN_visc + 0.0 + 0.0;
     if (aP <= 1.0e-15) aP = 1.0;
-    diag[k] = aP;
+    if (o.diag) o.diag[k] = aP;
+    PUT(uP, aP);
+    if (o.uaP2) o.uaP2[HX(i, j)] = aP;
   } else {
+    const long dof = k - nu;
     const double rho_v = F.rv[HX(i, j)];
     double Dv = (at * rho_v / dt_safe) * dvol;
     if (Dv <= 1.0e-30) Dv = 1.0;
     const bool is_wall = (j == 1 && bb && wallbc(m.bc[2])) || (j == v_ny && bb && wallbc(m.bc[3]));
-    if (is_wall || (j == 1 && bb && m.bc[2] == BC_NEUMANN && v_ny >= 2) ||
-        (j == v_ny && bb && m.bc[3] == BC_NEUMANN && v_ny >= 2)) {
-      diag[k] = Dv;
+    const bool nm_s = j == 1 && bb && m.bc[2] == BC_NEUMANN && v_ny >= 2;
+    const bool nm_n = j == v_ny && bb && m.bc[3] == BC_NEUMANN && v_ny >= 2;
+    if (is_wall || nm_s || nm_n) {
+      if (o.diag) o.diag[k] = Dv;
+      if (o.vaP2) o.vaP2[HX(i, j)] = Dv;
+      PUT(vE, 0.0);
+      PUT(vW, 0.0);
+      PUT(vN, (!is_wall && nm_s) ? Dv : 0.0);
+      PUT(vS, (!is_wall && !nm_s && nm_n) ? Dv : 0.0);
+      PUT(vP, Dv);
       return;
     }
     const double mu_n = F.mu[HX(i, j)], mu_s = F.mu[HX(i, j - 1)];
@@ -466,27 +550,203 @@ __global__ void __launch_bounds__(BLK) k_mom_diag(Mesh m, MomCfg c, MomFields F,
       if (m.bc[1] == BC_DIRICHLET) dwx = dwx + 2.0 * cE_raw;
       cE_visc = 0.0;
     }
-    double conv_diag = 0.0, newt = 0.0;
+    double conv_diag = 0.0, conv_W = 0.0, conv_E = 0.0, conv_S = 0.0, conv_N = 0.0, newt = 0.0;
     if (c.conv) {
       double FE, FW, FN, FS;
       v_fluxes(m, c, F, i, j, FE, FW, FN, FS);
       conv_diag = rho_v * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+      conv_E = (i == nx && bl) ? 0.0 : rho_v * dmax(-FE, 0.0);
+      conv_W = (i == 1 && bl) ? 0.0 : rho_v * dmax(FW, 0.0);
+      conv_N = rho_v * dmax(-FN, 0.0);
+      conv_S = rho_v * dmax(FS, 0.0);
       if (c.lin == 3) {
         const double dv_dy = (F.v[HX(i, j + 1)] - F.v[HX(i, j - 1)]) / (2.0 * dy);
         newt = rho_v * dmax(0.0, dv_dy) * dvol;
       }
     }
+    PUT(vW, cW_visc + conv_W);
+    PUT(vE, cE_visc + conv_E);
+    PUT(vS, cS + conv_S + 0.0);
+    PUT(vN, cN + conv_N + 0.0);
     double aP = Dv + dwx + conv_diag + 0.0 + newt + cW_visc + cE_visc + cS + cN + 0.0 + 0.0;
     if (aP <= 1.0e-15) aP = 1.0;
-    diag[k] = aP;
+    if (o.diag) o.diag[k] = aP;
+    PUT(vP, aP);
+    if (o.vaP2) o.vaP2[HX(i, j)] = aP;
+  }
+}
+#undef PUT
+
+// The Schur block of the monolithic system (same routine, :1540-1590): the
+// 5-point coefficients of B diag(A)^-1 B^T from u_aP, v_aP (halo, exchanged),
+// no coupling through wall sides, and the regularisation (1e-10 dvol at cell
+// (1,1), 1e-12 dvol elsewhere).  diag: the pressure rows of diag_p.
+__global__ void __launch_bounds__(BLK) k_schur(Mesh m, const double *__restrict__ uaP2,
+                                               const double *__restrict__ vaP2, double *__restrict__ sW,
+                                               double *__restrict__ sE, double *__restrict__ sS,
+                                               double *__restrict__ sN, double *__restrict__ sP,
+                                               double *__restrict__ diag) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)m.nx * m.ny) return;
+  const int j = (int)(k / m.nx) + 1, i = (int)(k - (long)(j - 1) * m.nx) + 1;
+  const bool bl = !m.perx, bb = !m.pery;
+  const double dx = m.dx, dy = m.dy, dvol = dx * dy;
+  double aW = 0.0, aE = 0.0, aS = 0.0, aN = 0.0;
+  if (!(i == 1 && bl && wallbc(m.bc[0]))) aW = (dy * dy) / dmax(1.0e-30, uaP2[HX(i, j)]);
+  if (!(i == m.nx && bl && wallbc(m.bc[1]))) aE = (dy * dy) / dmax(1.0e-30, uaP2[HX(i + 1, j)]);
+  if (!(j == 1 && bb && wallbc(m.bc[2]))) aS = (dx * dx) / dmax(1.0e-30, vaP2[HX(i, j)]);
+  if (!(j == m.ny && bb && wallbc(m.bc[3]))) aN = (dx * dx) / dmax(1.0e-30, vaP2[HX(i, j + 1)]);
+  const double reg = (i == 1 && j == 1) ? 1.0e-10 * dvol : 1.0e-12 * dvol;
+  const double aP = aW + aE + aS + aN + 0.0 + reg;
+  if (sW) {
+    sW[k] = aW;
+    sE[k] = aE;
+    sS[k] = aS;
+    sN[k] = aN;
+    sP[k] = aP;
+  }
+  if (diag) diag[k] = aP;
+}
+
+__global__ void __launch_bounds__(BLK) k_fill(long n, double val, double *__restrict__ a) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k < n) a[k] = val;
+}
+
+// exchange_halos_scalar of up to two halo fields, one workgroup
+__global__ void __launch_bounds__(1024) k_exchange2(Mesh m, double *a, double *b) {
+  if (a) wg_exchange(m, a);
+  if (b) wg_exchange(m, b);
+}
+
+// ----- the sequential preconditioners of apply_preconditioner
+// (src/module_linear_solvers.f90:137-316) with ssor_solve_u / _v / _p
+// (:1278-1403).  Lexicographic Gauss-Seidel is order dependent; it runs here
+// in one workgroup per block along anti-diagonals i + j = const: a cell
+// depends only on its W/S neighbours of the previous diagonal (already
+// updated) and its E/N neighbours of the next one (not yet updated), so the
+// wavefront reproduces the lexicographic sweep exactly, value for value.
+struct Blk5 {
+  const double *aW, *aE, *aS, *aN, *aP;
+};
+
+// The right-hand side of a block: the residual, or (kind 1: u, 2: v) the
+// residual less the pressure gradient of the block-triangular step.
+__device__ __forceinline__ double blk_rhs(const Mesh &m, const double *src, const double *xp, int kind, int i,
+                                          int j, long dof, int ni, int nj) {
+  const bool bl = !m.perx, bb = !m.pery;
+  if (kind == 1) {
+    const bool is_wall = (i == 1 && bl) || (i == ni && bl);
+    return is_wall ? src[dof] : src[dof] - (xp[HX(i, j)] - xp[HX(i - 1, j)]) * m.dy;
+  }
+  if (kind == 2) {
+    const bool is_wall = (j == 1 && bb && wallbc(m.bc[2])) || (j == nj && bb && wallbc(m.bc[3]));
+    return is_wall ? src[dof] : src[dof] - (xp[HX(i, j)] - xp[HX(i, j - 1)]) * m.dx;
+  }
+  return src[dof];
+}
+
+// n_sweeps symmetric sweeps with relaxation omega, from zero, on the (ni, nj)
+// block held in the halo field x; sync: the single-rank exchange of x before
+// every sweep but the first and between the halves of a sweep.
+__device__ void wg_ssor(const Mesh &m, int ni, int nj, Blk5 A, const double *src, const double *xp, int kind,
+                        double *x, int n_sweeps, double omega, bool sync) {
+  const int t = threadIdx.x, T = blockDim.x;
+  const long nh = (long)(m.nx + 2) * (m.ny + 2);
+  for (long k = t; k < nh; k += T) x[k] = 0.0;
+  __syncthreads();
+  auto cell = [&](int i, int j) {
+    const long dof = (long)(j - 1) * ni + (i - 1);
+    double d = A.aP[dof];
+    if (fabs(d) < 1.0e-30) d = 1.0;
+    double val = blk_rhs(m, src, xp, kind, i, j, dof, ni, nj);
+    const double aw = A.aW[dof], ae = A.aE[dof], as = A.aS[dof], an = A.aN[dof];
+    if (aw != 0.0) val = val + aw * x[HX(i - 1, j)];
+    if (ae != 0.0) val = val + ae * x[HX(i + 1, j)];
+    if (as != 0.0) val = val + as * x[HX(i, j - 1)];
+    if (an != 0.0) val = val + an * x[HX(i, j + 1)];
+    x[HX(i, j)] = (1.0 - omega) * x[HX(i, j)] + omega * (val / d);
+  };
+  for (int sweep = 1; sweep <= n_sweeps; ++sweep) {
+    if (sync && sweep > 1) wg_exchange(m, x);
+    for (int dg = 2; dg <= ni + nj; ++dg) {
+      const int lo = dg - nj > 1 ? dg - nj : 1, hi = dg - 1 < ni ? dg - 1 : ni;
+      for (int i = lo + t; i <= hi; i += T) cell(i, dg - i);
+      __syncthreads();
+    }
+    if (sync) wg_exchange(m, x);
+    for (int dg = ni + nj; dg >= 2; --dg) {
+      const int lo = dg - nj > 1 ? dg - nj : 1, hi = dg - 1 < ni ? dg - 1 : ni;
+      for (int i = hi - t; i >= lo; i -= T) cell(i, dg - i);
+      __syncthreads();
+    }
   }
 }
 
+// interior of the halo field x -> the block's part of dst
+__device__ void wg_store(const Mesh &m, int ni, int nj, const double *x, double *dst) {
+  const long nb = (long)ni * nj;
+  for (long k = threadIdx.x; k < nb; k += blockDim.x) {
+    const int j = (int)(k / ni) + 1, i = (int)(k - (long)(j - 1) * ni) + 1;
+    dst[k] = x[HX(i, j)];
+  }
+}
+
+struct SeqArgs {
+  Blk5 blk[3];     // u, v, Schur
+  int ni[3], nj[3];
+  long off[3];     // offsets of the blocks in the vectors
+  double *x[3];    // halo work fields
+};
+
+// Blocks 0, 1 (and 2, the pressure) of the SSOR preconditioner, one
+// workgroup each (kind 0, 2 sweeps, omega 1), or the u and v solves of the
+// block-triangular one (kind 1 / 2 with the pressure correction xp, 4 sweeps).
+__global__ void __launch_bounds__(1024) k_pc_blocks(Mesh m, SeqArgs a, const double *__restrict__ src,
+                                                    double *__restrict__ dst, int bt) {
+  const int b = blockIdx.x;
+  const bool sync = m.perx || m.pery;
+  const int kind = bt ? b + 1 : 0;
+  wg_ssor(m, a.ni[b], a.nj[b], a.blk[b], src + a.off[b], a.x[2], kind, a.x[b], bt ? 4 : 2, 1.0, sync);
+  wg_store(m, a.ni[b], a.nj[b], a.x[b], dst + a.off[b]);
+}
+
+// The pressure step of the block-triangular preconditioner: 12 SSOR sweeps
+// with omega 1.25 on the Schur block, the mean removed (serial sum in the
+// reference's order), the halo exchanged.
+__global__ void __launch_bounds__(1024) k_pc_bt_pressure(Mesh m, SeqArgs a, const double *__restrict__ src,
+                                                         double *__restrict__ dst) {
+  __shared__ double mean;
+  const bool sync = m.perx || m.pery;
+  double *xp = a.x[2];
+  wg_ssor(m, m.nx, m.ny, a.blk[2], src + a.off[2], nullptr, 0, xp, 12, 1.25, sync);
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int j = 1; j <= m.ny; ++j)
+      for (int i = 1; i <= m.nx; ++i) sum = sum + xp[HX(i, j)];
+    mean = sum / (double)((long)m.nx * m.ny);
+  }
+  __syncthreads();
+  const long np = (long)m.nx * m.ny;
+  for (long k = threadIdx.x; k < np; k += blockDim.x) {
+    const int j = (int)(k / m.nx) + 1, i = (int)(k - (long)(j - 1) * m.nx) + 1;
+    xp[HX(i, j)] = xp[HX(i, j)] - mean;
+  }
+  __syncthreads();
+  if (sync) wg_exchange(m, xp);
+  wg_store(m, m.nx, m.ny, xp, dst + a.off[2]);
+}
+
 // ns_assemble_rhs, SP branch (src/module_navier_stokes.f90:785-1101)
-__global__ void __launch_bounds__(BLK) k_mom_rhs(Mesh m, MomCfg c, MomFields F, double dt, long nu, long n,
-                                                 int u_nx, int v_ny, double *__restrict__ rhs) {
+// (the continuity rows of the monolithic system, k >= nuv, are zero)
+__global__ void __launch_bounds__(BLK) k_mom_rhs(Mesh m, MomCfg c, MomFields F, double dt, long nu, long nuv,
+                                                 long n, int u_nx, int v_ny, double *__restrict__ rhs) {
   const long k = (long)blockIdx.x * BLK + threadIdx.x;
   if (k >= n) return;
+  if (k >= nuv) {
+    rhs[k] = 0.0;
+    return;
+  }
   const Dof d = dof_ij(k, nu, u_nx, m.nx);
   const int i = d.i, j = d.j, nx = m.nx, ny = m.ny;
   const double dx = m.dx, dy = m.dy, dvol = dx * dy;
@@ -583,11 +843,13 @@ int mom_alloc(Ctx &c) {
   Mom &M = c.mom;
   if (M.tu) return COVOF_OK;
   const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
-  const long n = (long)(c.m.nx + 1) * c.m.ny + (long)c.m.nx * (c.m.ny + 1);  // at most
-  COVOF_CHECK(hipMalloc(&M.tu, nh * sizeof(double)));
-  COVOF_CHECK(hipMalloc(&M.tv, nh * sizeof(double)));
-  COVOF_CHECK(hipMemsetAsync(M.tu, 0, nh * sizeof(double), c.stream));
-  COVOF_CHECK(hipMemsetAsync(M.tv, 0, nh * sizeof(double), c.stream));
+  // at most: u, v and p unknowns
+  const long n = (long)(c.m.nx + 1) * c.m.ny + (long)c.m.nx * (c.m.ny + 1) + (long)c.m.nx * c.m.ny;
+  for (double **f : {&M.tu, &M.tv, &M.tp}) {
+    COVOF_CHECK(hipMalloc(f, nh * sizeof(double)));
+    hipLaunchKernelGGL(k_fill, grid_of(nh), dim3(BLK), 0, c.stream, nh, 0.0, *f);
+  }
+  COVOF_CHECK_LAUNCH();
   COVOF_CHECK(hipMalloc(&M.diag, n * sizeof(double)));
   COVOF_CHECK(hipMalloc(&M.rhs, n * sizeof(double)));
   COVOF_CHECK(hipMalloc(&M.x, n * sizeof(double)));
@@ -595,10 +857,22 @@ int mom_alloc(Ctx &c) {
   return COVOF_OK;
 }
 
+// the SSOR / Schur coefficients and their work fields, made on first use
+int mom_alloc_blocks(Ctx &c) {
+  Mom &M = c.mom;
+  if (M.bk[0]) return COVOF_OK;
+  const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+  const long nb = (long)(c.m.nx + 1) * (c.m.ny + 1);  // covers every block
+  for (double *&b : M.bk) COVOF_CHECK(hipMalloc(&b, nb * sizeof(double)));
+  for (double *&b : M.aP2) COVOF_CHECK(hipMalloc(&b, nh * sizeof(double)));
+  for (double *&b : M.sx) COVOF_CHECK(hipMalloc(&b, nh * sizeof(double)));
+  return COVOF_OK;
+}
+
 int mom_product(Ctx &c, const double *x, double *y, bool jacobi);
 
-// The momentum operator with its NONE / JACOBI preconditioner as a LinOp;
-// the preconditioned product is one fused kernel.
+// The momentum (or monolithic) operator with its preconditioner as a LinOp;
+// the Jacobi-preconditioned product is one fused kernel.
 struct MomOp : LinOp {
   Ctx &c;
   int pc;
@@ -608,22 +882,53 @@ struct MomOp : LinOp {
     if (pc == COVOF_PRECOND_JACOBI) return mom_product(c, x, y, true);
     return LinOp::apply_pa(x, y, tmp);
   }
-  int precond(const double *r, double *z) override {
-    if (pc == COVOF_PRECOND_JACOBI) {
-      hipLaunchKernelGGL(k_mom_jacobi, grid_of(n), dim3(BLK), 0, c.stream, n, c.mom.diag, r, z);
-      return hipGetLastError() == hipSuccess ? COVOF_OK : COVOF_HIP_ERROR;
-    }
-    if (z != r) COVOF_CHECK(hipMemcpyAsync(z, r, n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-    return COVOF_OK;
-  }
+  int precond(const double *r, double *z) override { return mom_precond(c, pc, r, z); }
 };
+
+// assemble_preconditioner_blocks into M.diag and (full) the SSOR / Schur
+// coefficients of M.bk
+int mom_blocks(Ctx &c, bool full) {
+  Mom &M = c.mom;
+  const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+  if (full || M.cfg.mono) {
+    const int st = mom_alloc_blocks(c);
+    if (st) return st;
+  }
+  BlkOut o{};
+  o.diag = M.diag;
+  if (full) {
+    o.uW = M.bk[0], o.uE = M.bk[1], o.uS = M.bk[2], o.uN = M.bk[3], o.uP = M.bk[4];
+    o.vW = M.bk[5], o.vE = M.bk[6], o.vS = M.bk[7], o.vN = M.bk[8], o.vP = M.bk[9];
+  }
+  if (M.cfg.mono) {
+    o.uaP2 = M.aP2[0], o.vaP2 = M.aP2[1];
+    hipLaunchKernelGGL(k_fill, grid_of(nh), dim3(BLK), 0, c.stream, nh, 1.0, M.aP2[0]);
+    hipLaunchKernelGGL(k_fill, grid_of(nh), dim3(BLK), 0, c.stream, nh, 1.0, M.aP2[1]);
+  }
+  hipLaunchKernelGGL(k_mom_blocks, grid_of(M.nuv), dim3(BLK), 0, c.stream, c.m, M.cfg, M.f, M.dt, M.nu, M.nuv,
+                     M.u_nx, M.v_ny, o);
+  if (M.cfg.mono) {
+    hipLaunchKernelGGL(k_exchange2, dim3(1), dim3(1024), 0, c.stream, c.m, M.aP2[0], M.aP2[1]);
+    double *s[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (full)
+      for (int q = 0; q < 5; ++q) s[q] = M.bk[10 + q];
+    hipLaunchKernelGGL(k_schur, grid_of((long)c.m.nx * c.m.ny), dim3(BLK), 0, c.stream, c.m,
+                       (const double *)M.aP2[0], (const double *)M.aP2[1], s[0], s[1], s[2], s[3], s[4],
+                       M.diag + M.nuv);
+  }
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
 
 }  // namespace
 
 void mom_free(Ctx &c) {
   Mom &M = c.mom;
-  for (double *p : {M.tu, M.tv, M.diag, M.rhs, M.x}) hipFree(p);
+  for (double *p : {M.tu, M.tv, M.tp, M.diag, M.rhs, M.x}) hipFree(p);
   for (double *p : M.w) hipFree(p);
+  for (double *p : M.bk) hipFree(p);
+  for (double *p : M.aP2) hipFree(p);
+  for (double *p : M.sx) hipFree(p);
   M = Mom();
 }
 
@@ -647,7 +952,8 @@ int mom_bind(Ctx &c, const MomCfg &cfg, const MomFields &f, double dt) {
   M.u_nx = c.m.nx + (c.m.perx ? 0 : 1);
   M.v_ny = c.m.ny + (c.m.pery ? 0 : 1);
   M.nu = (long)M.u_nx * c.m.ny;
-  M.n = M.nu + (long)c.m.nx * M.v_ny;
+  M.nuv = M.nu + (long)c.m.nx * M.v_ny;
+  M.n = M.nuv + (cfg.mono ? (long)c.m.nx * c.m.ny : 0);
   M.bound = 1;
   return COVOF_OK;
 }
@@ -655,25 +961,26 @@ int mom_bind(Ctx &c, const MomCfg &cfg, const MomFields &f, double dt) {
 int mom_diag(Ctx &c, double *diag) {
   Mom &M = c.mom;
   if (!M.bound) return COVOF_BAD_ARGUMENT;
-  hipLaunchKernelGGL(k_mom_diag, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.cfg, M.f, M.dt, M.nu, M.n, M.u_nx,
-                     M.v_ny, diag);
-  COVOF_CHECK_LAUNCH();
+  int st = mom_blocks(c, false);
+  if (st) return st;
+  if (diag != M.diag)
+    COVOF_CHECK(hipMemcpyAsync(diag, M.diag, M.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
   return COVOF_OK;
 }
 
 int mom_rhs(Ctx &c, double *rhs) {
   Mom &M = c.mom;
   if (!M.bound) return COVOF_BAD_ARGUMENT;
-  hipLaunchKernelGGL(k_mom_rhs, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.cfg, M.f, M.dt, M.nu, M.n, M.u_nx,
-                     M.v_ny, rhs);
+  hipLaunchKernelGGL(k_mom_rhs, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.cfg, M.f, M.dt, M.nu, M.nuv, M.n,
+                     M.u_nx, M.v_ny, rhs);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
 
-int mom_pack(Ctx &c, const double *u, const double *v, double *x) {
+int mom_pack(Ctx &c, const double *u, const double *v, const double *p, double *x) {
   Mom &M = c.mom;
-  if (!M.bound) return COVOF_BAD_ARGUMENT;
-  hipLaunchKernelGGL(k_pack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.n, M.u_nx, u, v, x);
+  if (!M.bound || (M.cfg.mono && !p)) return COVOF_BAD_ARGUMENT;
+  hipLaunchKernelGGL(k_pack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.nuv, M.n, M.u_nx, u, v, p, x);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -683,13 +990,28 @@ namespace {
 // the Jacobi division when jacobi: M.diag must hold the diagonal).
 int mom_product(Ctx &c, const double *x, double *y, bool jacobi) {
   Mom &M = c.mom;
-  hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.n, M.u_nx, x, M.tu, M.tv);
-  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, M.tu, M.tv, 1, 1);
+  double *tp = M.cfg.mono ? M.tp : nullptr;
+  hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.nuv, M.n, M.u_nx, x, M.tu,
+                     M.tv, tp);
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, M.tu, M.tv, 1, 1, tp);
   launch_tagged(c, COVOF_KERNEL_MOM_MATVEC, jacobi ? k_mom_matvec<true> : k_mom_matvec<false>, grid_of(M.n),
-                dim3(BLK), c.m, M.cfg, M.f, M.dt, M.nu, M.n, M.u_nx, M.v_ny, (const double *)M.tu,
-                (const double *)M.tv, (const double *)M.diag, y);
+                dim3(BLK), c.m, M.cfg, M.f, M.dt, M.nu, M.nuv, M.n, M.u_nx, M.v_ny, (const double *)M.tu,
+                (const double *)M.tv, (const double *)M.tp, (const double *)M.diag, y);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
+}
+
+// the blocks of the sequential preconditioners, as the kernels take them
+SeqArgs seq_args(Ctx &c) {
+  Mom &M = c.mom;
+  SeqArgs a{};
+  for (int b = 0; b < 3; ++b)
+    a.blk[b] = Blk5{M.bk[5 * b], M.bk[5 * b + 1], M.bk[5 * b + 2], M.bk[5 * b + 3], M.bk[5 * b + 4]};
+  a.ni[0] = M.u_nx, a.nj[0] = c.m.ny, a.off[0] = 0;
+  a.ni[1] = c.m.nx, a.nj[1] = M.v_ny, a.off[1] = M.nu;
+  a.ni[2] = c.m.nx, a.nj[2] = c.m.ny, a.off[2] = M.nuv;
+  for (int b = 0; b < 3; ++b) a.x[b] = M.sx[b];
+  return a;
 }
 }  // namespace
 
@@ -698,16 +1020,53 @@ int mom_matvec(Ctx &c, const double *x, double *y) {
   return mom_product(c, x, y, false);
 }
 
-// solve_linear_system on the momentum operator, the diagonal of the bound
-// fields as the Jacobi preconditioner.  b, x: n unknowns, not a work vector.
+// apply_preconditioner (src/module_linear_solvers.f90:48-316) with the blocks
+// last assembled: NONE copies, JACOBI divides by the diagonal, SSOR runs two
+// symmetric sweeps per block, BLOCK_TRIANGULAR (monolithic) the pressure
+// Schur solve then the velocity solves against its gradient (SSOR otherwise).
+int mom_precond(Ctx &c, int precond, const double *r, double *z) {
+  Mom &M = c.mom;
+  if (precond == COVOF_PRECOND_JACOBI) {
+    hipLaunchKernelGGL(k_mom_jacobi, grid_of(M.n), dim3(BLK), 0, c.stream, M.n, (const double *)M.diag, r, z);
+  } else if (precond == COVOF_PRECOND_SSOR || precond == COVOF_PRECOND_BLOCK_TRIANGULAR) {
+    const SeqArgs a = seq_args(c);
+    if (precond == COVOF_PRECOND_BLOCK_TRIANGULAR && M.cfg.mono) {
+      hipLaunchKernelGGL(k_pc_bt_pressure, dim3(1), dim3(1024), 0, c.stream, c.m, a, r, z);
+      hipLaunchKernelGGL(k_pc_blocks, dim3(2), dim3(1024), 0, c.stream, c.m, a, r, z, 1);
+    } else {
+      hipLaunchKernelGGL(k_pc_blocks, dim3(M.cfg.mono ? 3 : 2), dim3(1024), 0, c.stream, c.m, a, r, z, 0);
+    }
+  } else {
+    if (z != r) COVOF_CHECK(hipMemcpyAsync(z, r, M.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    return COVOF_OK;
+  }
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+static bool precond_ok(int precond) {
+  return precond == COVOF_PRECOND_NONE || precond == COVOF_PRECOND_JACOBI || precond == COVOF_PRECOND_SSOR ||
+         precond == COVOF_PRECOND_BLOCK_TRIANGULAR;
+}
+
+// assemble_preconditioner_blocks of the bound fields, then apply_preconditioner
+int mom_apply_precond(Ctx &c, int precond, const double *r, double *z) {
+  if (!c.mom.bound || !precond_ok(precond)) return COVOF_BAD_ARGUMENT;
+  const int st = mom_blocks(c, precond == COVOF_PRECOND_SSOR || precond == COVOF_PRECOND_BLOCK_TRIANGULAR);
+  return st ? st : mom_precond(c, precond, r, z);
+}
+
+// solve_linear_system on the momentum (or monolithic) operator, the
+// preconditioner from the blocks of the bound fields.  b, x: n unknowns, not
+// a work vector.
 int mom_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol, int maxit, int *iters,
               double *relres) {
   Mom &M = c.mom;
   if (!M.bound) return COVOF_BAD_ARGUMENT;
-  if ((lin_solver != COVOF_LINEAR_BICGSTAB2 && lin_solver != COVOF_LINEAR_CG) ||
-      (precond != COVOF_PRECOND_NONE && precond != COVOF_PRECOND_JACOBI) || maxit < 0)
+  if ((lin_solver != COVOF_LINEAR_BICGSTAB2 && lin_solver != COVOF_LINEAR_CG) || !precond_ok(precond) ||
+      maxit < 0)
     return COVOF_BAD_ARGUMENT;
-  int st = mom_diag(c, M.diag);
+  int st = mom_blocks(c, precond == COVOF_PRECOND_SSOR || precond == COVOF_PRECOND_BLOCK_TRIANGULAR);
   if (st) return st;
   MomOp op(c, precond);
   if (lin_solver == COVOF_LINEAR_CG) return krylov_cg(c, op, b, x, M.w, tol, maxit, iters, relres);
@@ -720,7 +1079,8 @@ int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous
   const int *bc = c.m.bc;
   if ((bc[0] == BC_PERIODIC) != (bc[1] == BC_PERIODIC) || (bc[2] == BC_PERIODIC) != (bc[3] == BC_PERIODIC))
     return COVOF_BAD_ARGUMENT;
-  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, cfg, u, v, homogeneous ? 1 : 0, 1);
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, cfg, u, v, homogeneous ? 1 : 0, 1,
+                     (double *)nullptr);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -732,16 +1092,36 @@ int velocity_bc(Ctx &c, const MomCfg &cfg, double *u, double *v, int homogeneous
 int mom_predictor(Ctx &c, double *u, double *v, int lin_solver, int precond, double tol, int maxit, int *iters,
                   double *relres) {
   Mom &M = c.mom;
-  if (!M.bound) return COVOF_BAD_ARGUMENT;
+  if (!M.bound || M.cfg.mono) return COVOF_BAD_ARGUMENT;
   int st = mom_rhs(c, M.rhs);
-  if (!st) st = mom_pack(c, u, v, M.x);
+  if (!st) st = mom_pack(c, u, v, nullptr, M.x);
   if (st) return st;
   const int sst = mom_solve(c, M.rhs, M.x, lin_solver, precond, tol, maxit, iters, relres);
   if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
-  hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.n, M.u_nx, (const double *)M.x,
-                     u, v);
-  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, u, v, 0, 1);
-  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, u, v, 0, 1);
+  hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.nuv, M.n, M.u_nx,
+                     (const double *)M.x, u, v, (double *)nullptr);
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, u, v, 0, 1, (double *)nullptr);
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, u, v, 0, 1, (double *)nullptr);
+  COVOF_CHECK_LAUNCH();
+  return sst;
+}
+
+// The monolithic branch of the time step (src/module_simulation.f90:553-583,
+// one Uzawa pass): blocks, right-hand side with zero continuity rows, pack
+// (u, v, p: the initial guess), solve, unpack_vector(.false.) into u, v, p
+// (pressure exchanged, velocity exchanged and its BC applied).
+int mono_step(Ctx &c, double *u, double *v, double *p, int lin_solver, int precond, double tol, int maxit,
+              int *iters, double *relres) {
+  Mom &M = c.mom;
+  if (!M.bound || !M.cfg.mono || !p) return COVOF_BAD_ARGUMENT;
+  int st = mom_rhs(c, M.rhs);
+  if (!st) st = mom_pack(c, u, v, p, M.x);
+  if (st) return st;
+  const int sst = mom_solve(c, M.rhs, M.x, lin_solver, precond, tol, maxit, iters, relres);
+  if (sst == COVOF_BAD_ARGUMENT || sst == COVOF_HIP_ERROR) return sst;
+  hipLaunchKernelGGL(k_unpack, grid_of(M.n), dim3(BLK), 0, c.stream, c.m, M.nu, M.nuv, M.n, M.u_nx,
+                     (const double *)M.x, u, v, p);
+  hipLaunchKernelGGL(k_vel_halo, dim3(1), dim3(1024), 0, c.stream, c.m, M.cfg, u, v, 0, 1, p);
   COVOF_CHECK_LAUNCH();
   return sst;
 }

@@ -49,12 +49,14 @@ module covof_hip
 
   !> covof_momentum_cfg: the config_t fields the momentum block reads
   !> (time_scheme, fields%has_nm1, enable_convection, convective_scheme,
-  !> convective_linearization, gx, gy, bc_<side>%u / %v of left, right, bottom, top)
+  !> convective_linearization, gx, gy, bc_<side>%u / %v of left, right, bottom, top,
+  !> solver_type: 3 STANDARD_PROJECTION, 2 MONOLITHIC)
   type, bind(C) :: covof_momentum_cfg
      integer(c_int) :: time_scheme = 1, has_nm1 = 0, enable_convection = 1, convective_scheme = 1, &
           convective_linearization = 1
      real(c_double) :: gx = 0.0_c_double, gy = 0.0_c_double
      real(c_double) :: bc_u(4) = 0.0_c_double, bc_v(4) = 0.0_c_double
+     integer(c_int) :: solver_type = 3
   end type covof_momentum_cfg
 
   interface

@@ -49,6 +49,9 @@ extern "C" {
 #define COVOF_PRECOND_NONE 0
 #define COVOF_PRECOND_JACOBI 1
 #define COVOF_PRECOND_SSOR 2 /* symmetric Gauss-Seidel, lexicographic (reference SP) */
+#define COVOF_PRECOND_BLOCK_TRIANGULAR 4 /* Schur-SSOR then velocity SSOR (monolithic) */
+#define COVOF_SOLVER_MONOLITHIC 2
+#define COVOF_SOLVER_STANDARD_PROJECTION 3
 /* Parallel preconditioners with no reference counterpart (values clear of
  * the reference's 0-5; they take the place PRECOND_CUSTOM = 5 gives a user
  * hook, src/module_linear_solvers.f90:22-41): */
@@ -200,12 +203,14 @@ typedef struct covof_momentum_cfg {
   int convective_linearization; /* 1 PICARD, 2 ADAMS_BASHFORTH, 3 NEWTON */
   double gx, gy;
   double bc_u[4], bc_v[4];
+  int solver_type;              /* cfg%solver_type: 0 (unset) or 3 STANDARD_PROJECTION, 2 MONOLITHIC */
 } covof_momentum_cfg;
 
-/* Unknowns of the SP momentum system, ns_init (src/module_navier_stokes.f90:
+/* Unknowns of the momentum system, ns_init (src/module_navier_stokes.f90:
  * 136-226): u(1:u_nx, 1:ny) then v(1:nx, 1:v_ny), i fastest, with u_nx = nx+1
- * (nx if periodic in x) and v_ny = ny+1 (ny if periodic in y); *n = total. */
-int covof_momentum_size(covof_ctx *ctx, int *n, int *u_nx, int *v_ny);
+ * (nx if periodic in x) and v_ny = ny+1 (ny if periodic in y), then for
+ * solver_type MONOLITHIC p(1:nx, 1:ny); *n = total. */
+int covof_momentum_size(covof_ctx *ctx, int solver_type, int *n, int *u_nx, int *v_ny);
 /* The fields_old of the momentum block (device, halo layout): the current
  * velocity u, v (linearisation state), u_old, v_old (the time level n),
  * u_nm1, v_nm1 (level n-1; may be NULL unless has_nm1 with GEAR or
@@ -220,21 +225,32 @@ int covof_momentum_set_fields_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, c
                                 const double *u_nm1, const double *v_nm1, const double *rho_u,
                                 const double *rho_v, const double *mu, const double *mu_node,
                                 const double *fx_sigma, const double *fy_sigma, double dt);
-/* diag_u, diag_v of assemble_preconditioner_blocks (src/module_navier_stokes.f90:
- * 1268-1595), the Jacobi preconditioner of the momentum solve; n entries. */
+/* diag_u, diag_v (and diag_p, the Schur diagonal, when monolithic) of
+ * assemble_preconditioner_blocks (src/module_navier_stokes.f90:1268-1595), the
+ * Jacobi preconditioner of the momentum solve; n entries. */
 int covof_momentum_diag_d(covof_ctx *ctx, double *diag);
 /* assemble_rhs (src/module_navier_stokes.f90:785-1101); n entries. */
 int covof_momentum_rhs_d(covof_ctx *ctx, double *rhs);
 /* y = A x, apply_system_matrix (src/module_navier_stokes.f90:380-780), with
  * the reference's unpack_vector(homogeneous) into the operator's persistent
- * work fields (its SAVEd tf, zero at the context's first product). */
+ * work fields (its SAVEd tf, zero at the context's first product); when
+ * monolithic, the momentum rows carry B^T p and the pressure rows are the
+ * continuity rows div(u) dvol. */
 int covof_momentum_matvec_d(covof_ctx *ctx, const double *x, double *y);
 /* solve_linear_system (src/module_linear_solvers.f90:1167-1208) on the
- * momentum operator: lin_solver BICGSTAB2 (:361-579) or CG (:735-838),
- * precond NONE or JACOBI (:88-135).  x: initial guess in, solution out.
- * *iters as the reference reports it. */
+ * momentum (or monolithic) operator: lin_solver BICGSTAB2 (:361-579) or CG
+ * (:735-838), precond NONE, JACOBI (:88-135), SSOR (:137-166) or
+ * BLOCK_TRIANGULAR (:168-316), its blocks assembled from the bound fields.
+ * x: initial guess in, solution out.  *iters as the reference reports it. */
 int covof_momentum_solve_d(covof_ctx *ctx, const double *b, double *x, int lin_solver, int precond,
                            double tol, int maxit, int *iters, double *relres);
+/* z = M^-1 r, apply_preconditioner (src/module_linear_solvers.f90:48-316) with
+ * the blocks of assemble_preconditioner_blocks of the bound fields: NONE,
+ * JACOBI, SSOR (two symmetric lexicographic sweeps per block, :1278-1403) or
+ * BLOCK_TRIANGULAR (monolithic: 12 SSOR sweeps at omega 1.25 on the Schur
+ * block, mean removed, then 4 sweeps on u and v against its gradient; SSOR
+ * otherwise); n entries. */
+int covof_momentum_precond_d(covof_ctx *ctx, int precond, const double *r, double *z);
 /* The momentum block of sim_solve_standard_projection (src/module_simulation.f90:
  * 2454-2469): diagonal, right-hand side, pack (the current u, v are the
  * initial guess), solve, unpack_vector(.false.), sync_u / sync_v and
@@ -262,6 +278,25 @@ int covof_sp_predictor(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u,
                        const double *mu, const double *mu_node, const double *fx_sigma,
                        const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
                        int maxit, int *iters, double *relres);
+/* The monolithic branch of the time step (src/module_simulation.f90:553-583,
+ * one Uzawa pass; SOLVER_MONOLITHIC, cfg->solver_type must say so):
+ * assemble_preconditioner_blocks, assemble_rhs (zero continuity rows), pack
+ * (u, v, p: the initial guess), solve_linear_system, unpack_vector(.false.)
+ * into u, v, p (pressure exchanged, velocity exchanged with its BC).  The
+ * fields as for covof_momentum_set_fields_d; returns as covof_sp_predictor_d. */
+int covof_monolithic_step_d(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v, double *p,
+                            const double *u_old, const double *v_old, const double *u_nm1,
+                            const double *v_nm1, const double *rho_u, const double *rho_v,
+                            const double *mu, const double *mu_node, const double *fx_sigma,
+                            const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                            int maxit, int *iters, double *relres);
+/* The same on host arrays (copied in and out through device mirrors). */
+int covof_monolithic_step(covof_ctx *ctx, const covof_momentum_cfg *cfg, double *u, double *v, double *p,
+                          const double *u_old, const double *v_old, const double *u_nm1,
+                          const double *v_nm1, const double *rho_u, const double *rho_v,
+                          const double *mu, const double *mu_node, const double *fx_sigma,
+                          const double *fy_sigma, double dt, int lin_solver, int precond, double tol,
+                          int maxit, int *iters, double *relres);
 
 /* --- output (src/module_vtk_writer.f90) ------------------------------------ */
 

@@ -1166,8 +1166,10 @@ static void v_conv_fluxes(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom
  * tu, tv are the persistent work fields of the reference's `tf` (SAVE),
  * which unpack_vector(homogeneous) fills before every product; the caller
  * keeps them (zero-initialised, like the reference's first allocation). */
-void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
-                    double *tu, double *tv, const double *x, double *y) {
+/* The u and v rows of the operator from unpacked work fields; with tp (the
+ * monolithic system) the pressure-gradient terms B_u^T p, B_v^T p too. */
+static void mom_rows(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                     const double *tu, const double *tv, const double *tp, double *y) {
   const int nx = m->nx, ny = m->ny;
   const double dx = m->dx, dy = m->dy, dvol = dx * dy;
   int u_nx, v_ny;
@@ -1176,7 +1178,6 @@ void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_field
   const long nu = (long)u_nx * ny;
   const double at = gear(c) ? 1.5 : 1.0;
   const int sch = c->conv_scheme;
-  orc_mom_unpack(m, c, x, tu, tv, 1);
   const double *u = F->u, *v = F->v;
 #define TU(a, b) tu[H(m, a, b)]
 #define TV(a, b) tv[H(m, a, b)]
@@ -1224,6 +1225,7 @@ void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_field
       const double shN = mnN * ((TU(i, j + 1) - TU(i, j)) / dy + (TV(i, j + 1) - TV(i - 1, j + 1)) / dx) * dx;
       const double shS = mnS * ((TU(i, j) - TU(i, j - 1)) / dy + (TV(i, j) - TV(i - 1, j)) / dx) * dx;
       val = val - (shN - shS);
+      if (tp) val = val + (tp[H(m, i, j)] - tp[H(m, i - 1, j)]) * dy;
       y[dof] = val;
     }
   for (int j = 1; j <= v_ny; ++j)
@@ -1267,70 +1269,174 @@ void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_field
       const double shE = mnE * ((TV(i + 1, j) - TV(i, j)) / dx + (TU(i + 1, j) - TU(i + 1, j - 1)) / dy) * dy;
       const double shW = mnW * ((TV(i, j) - TV(i - 1, j)) / dx + (TU(i, j) - TU(i, j - 1)) / dy) * dy;
       val = val - (shE - shW);
+      if (tp) val = val + (tp[H(m, i, j)] - tp[H(m, i, j - 1)]) * dx;
       y[dof] = val;
     }
 #undef TU
 #undef TV
 }
 
+void orc_mom_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                    double *tu, double *tv, const double *x, double *y) {
+  orc_mom_unpack(m, c, x, tu, tv, 1);
+  mom_rows(m, c, F, dt, tu, tv, NULL, y);
+}
+
+/* ns_unpack_vector of the monolithic system: u, v, then p with its
+ * sync_cell_scalar, then the u, v exchange and velocity BC. */
+void orc_mono_unpack(const orc_mesh *m, const orc_mom_cfg *c, const double *vec, double *u, double *v,
+                     double *p, int homogeneous) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long off = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) p[H(m, i, j)] = vec[off + W(m, i, j)];
+  orc_exchange_halos_scalar(m, p);
+  orc_mom_unpack(m, c, vec, u, v, homogeneous);
+}
+
+/* ns_pack_vector of the monolithic system. */
+void orc_mono_pack(const orc_mesh *m, const double *u, const double *v, const double *p, double *vec) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long off = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  orc_mom_pack(m, u, v, vec);
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) vec[off + W(m, i, j)] = p[H(m, i, j)];
+}
+
+/* ns_apply_system_matrix, SOLVER_MONOLITHIC (src/module_navier_stokes.f90:
+ * 380-780): the momentum rows with B^T p, and the continuity rows
+ * div(u) dvol (+ 1e-10 p(1,1) dvol at cell (1,1), incompressible). */
+void orc_mono_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                     double *tu, double *tv, double *tp, const double *x, double *y) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long off = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  const double dx = m->dx, dy = m->dy, dvol = dx * dy;
+  orc_mono_unpack(m, c, x, tu, tv, tp, 1);
+  mom_rows(m, c, F, dt, tu, tv, tp, y);
+  for (int j = 1; j <= m->ny; ++j)
+    for (int i = 1; i <= m->nx; ++i) {
+      const double div = (tu[H(m, i + 1, j)] - tu[H(m, i, j)]) / dx + (tv[H(m, i, j + 1)] - tv[H(m, i, j)]) / dy;
+      y[off + W(m, i, j)] = (i == 1 && j == 1) ? div * dvol + 1.0e-10 * tp[H(m, 1, 1)] * dvol : div * dvol;
+    }
+}
+
 /* ns_assemble_preconditioner_blocks (src/module_navier_stokes.f90:1268-1595):
- * the diagonals diag_u, diag_v (Jacobi) of the SP momentum block. */
-void orc_mom_diag(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
-                  double *diag) {
+ * the diagonals diag_u, diag_v (and diag_p = schur_aP, monolithic), the
+ * momentum SSOR coefficients u_a*, v_a* and the Schur coefficients schur_a*
+ * (the latter (nx, ny), zero except in the monolithic system).  Any output
+ * pointer of B may be NULL. */
+void orc_mom_blocks(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                    const orc_mom_blocks_t *B) {
   const int nx = m->nx, ny = m->ny;
   const double dx = m->dx, dy = m->dy, dvol = dx * dy;
   int u_nx, v_ny;
   orc_mom_sizes(m, &u_nx, &v_ny);
   const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
-  const long nu = (long)u_nx * ny;
+  const long nu = (long)u_nx * ny, nv = (long)nx * v_ny;
   const double dt_safe = dt > 1.0e-30 ? dt : 1.0;
   const double at = gear(c) ? 1.5 : 1.0;
+  const int mono = c->solver_type == 2;
+  const long nh = (long)(nx + 2) * (ny + 2);
+  double *uaP2 = malloc(nh * sizeof(double)), *vaP2 = malloc(nh * sizeof(double));
+  for (long k = 0; k < nh; ++k) uaP2[k] = vaP2[k] = 1.0;
+#define SETU(arr, val) do { if (B->arr) B->arr[dof] = (val); } while (0)
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= u_nx; ++i) {
       const long dof = (long)(j - 1) * u_nx + (i - 1);
       const double rho_u = F->rho_u[H(m, i, j)];
       double Du = (at * rho_u / dt_safe) * dvol;
       if (Du <= 1.0e-30) Du = 1.0;
-      if ((i == 1 && bl) || (i == u_nx && br)) {
-        diag[dof] = Du;
+      if (i == 1 && bl) {
+        SETU(diag, Du); uaP2[H(m, i, j)] = Du;
+        SETU(uW, 0.0); SETU(uN, 0.0); SETU(uS, 0.0);
+        SETU(uE, (m->bc[0] == BC_NEUMANN && u_nx >= 2) ? Du : 0.0);
+        SETU(uP, Du);
+        continue;
+      }
+      if (i == u_nx && br) {
+        SETU(diag, Du); uaP2[H(m, i, j)] = Du;
+        SETU(uE, 0.0); SETU(uN, 0.0); SETU(uS, 0.0);
+        SETU(uW, (m->bc[1] == BC_NEUMANN && u_nx >= 2) ? Du : 0.0);
+        SETU(uP, Du);
         continue;
       }
       const double mu_e = F->mu[H(m, i, j)], mu_w = F->mu[H(m, i - 1, j)];
       const double cE = 2.0 * mu_e * (dy / dx), cW = 2.0 * mu_w * (dy / dx);
       const double cN_raw = F->mu_node[H(m, i, j + 1)] * (dx / dy), cS_raw = F->mu_node[H(m, i, j)] * (dx / dy);
-      double cN_visc = cN_raw, cS_visc = cS_raw, dwy = 0.0;
+      double cN_visc = cN_raw, cS_visc = cS_raw, dwy = 0.0, cN_extra = 0.0, cS_extra = 0.0;
       if (j == 1 && bb) {
-        if (m->bc[2] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cS_raw : dwy + 2.0 * cS_raw;
+        if (m->bc[2] == BC_DIRICHLET) {
+          if (ny >= 2) {
+            dwy = dwy + 3.0 * cS_raw;
+            cN_extra = cN_extra + cS_raw / 3.0;
+          } else {
+            dwy = dwy + 2.0 * cS_raw;
+          }
+        }
         cS_visc = 0.0;
       }
       if (j == ny && bt) {
-        if (m->bc[3] == BC_DIRICHLET) dwy = ny >= 2 ? dwy + 3.0 * cN_raw : dwy + 2.0 * cN_raw;
+        if (m->bc[3] == BC_DIRICHLET) {
+          if (ny >= 2) {
+            dwy = dwy + 3.0 * cN_raw;
+            cS_extra = cS_extra + cN_raw / 3.0;
+          } else {
+            dwy = dwy + 2.0 * cN_raw;
+          }
+        }
         cN_visc = 0.0;
       }
-      double conv_diag = 0.0, newt = 0.0;
+      double conv_diag = 0.0, conv_W = 0.0, conv_E = 0.0, conv_S = 0.0, conv_N = 0.0, newt = 0.0;
       if (c->enable_convection) {
         double FE, FW, FN, FS;
         u_conv_fluxes(m, c, F, i, j, &FE, &FW, &FN, &FS);
         conv_diag = rho_u * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+        conv_E = rho_u * dmax(-FE, 0.0);
+        conv_W = rho_u * dmax(FW, 0.0);
+        conv_N = (j == ny && bt) ? 0.0 : rho_u * dmax(-FN, 0.0);
+        conv_S = (j == 1 && bb) ? 0.0 : rho_u * dmax(FS, 0.0);
         if (c->conv_lin == 3) {
           const double du_dx = (F->u[H(m, i + 1, j)] - F->u[H(m, i - 1, j)]) / (2.0 * dx);
           newt = rho_u * dmax(0.0, du_dx) * dvol;
         }
       }
+      SETU(uW, cW + conv_W + 0.0);
+      SETU(uE, cE + conv_E + 0.0);
+      SETU(uS, cS_visc + conv_S + cS_extra);
+      SETU(uN, cN_visc + conv_N + cN_extra);
       double aP = Du + dwy + conv_diag + 0.0 + newt + cW + cE + cS_visc + cN_visc + 0.0 + 0.0;
       if (aP <= 1.0e-15) aP = 1.0;
-      diag[dof] = aP;
+      SETU(diag, aP); SETU(uP, aP); uaP2[H(m, i, j)] = aP;
     }
+  orc_exchange_halos_scalar(m, uaP2);
+#undef SETU
+#define SETV(arr, val) do { if (B->arr) B->arr[dof] = (val); } while (0)
   for (int j = 1; j <= v_ny; ++j)
     for (int i = 1; i <= nx; ++i) {
-      const long dof = nu + (long)(j - 1) * nx + (i - 1);
+      const long dof = (long)(j - 1) * nx + (i - 1);
       const double rho_v = F->rho_v[H(m, i, j)];
       double Dv = (at * rho_v / dt_safe) * dvol;
       if (Dv <= 1.0e-30) Dv = 1.0;
       const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
-      if (is_wall || (j == 1 && bb && m->bc[2] == BC_NEUMANN && v_ny >= 2) ||
-          (j == v_ny && bt && m->bc[3] == BC_NEUMANN && v_ny >= 2)) {
-        diag[dof] = Dv;
+      if (is_wall) {
+        if (B->diag) B->diag[nu + dof] = Dv;
+        vaP2[H(m, i, j)] = Dv;
+        SETV(vE, 0.0); SETV(vW, 0.0); SETV(vN, 0.0); SETV(vS, 0.0); SETV(vP, Dv);
+        continue;
+      }
+      if (j == 1 && bb && m->bc[2] == BC_NEUMANN && v_ny >= 2) {
+        if (B->diag) B->diag[nu + dof] = Dv;
+        vaP2[H(m, i, j)] = Dv;
+        SETV(vE, 0.0); SETV(vW, 0.0); SETV(vS, 0.0); SETV(vN, Dv); SETV(vP, Dv);
+        continue;
+      }
+      if (j == v_ny && bt && m->bc[3] == BC_NEUMANN && v_ny >= 2) {
+        if (B->diag) B->diag[nu + dof] = Dv;
+        vaP2[H(m, i, j)] = Dv;
+        SETV(vE, 0.0); SETV(vW, 0.0); SETV(vN, 0.0); SETV(vS, Dv); SETV(vP, Dv);
         continue;
       }
       const double mu_n = F->mu[H(m, i, j)], mu_s = F->mu[H(m, i, j - 1)];
@@ -1345,20 +1451,158 @@ void orc_mom_diag(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
         if (m->bc[1] == BC_DIRICHLET) dwx = dwx + 2.0 * cE_raw;
         cE_visc = 0.0;
       }
-      double conv_diag = 0.0, newt = 0.0;
+      double conv_diag = 0.0, conv_W = 0.0, conv_E = 0.0, conv_S = 0.0, conv_N = 0.0, newt = 0.0;
       if (c->enable_convection) {
         double FE, FW, FN, FS;
         v_conv_fluxes(m, c, F, i, j, &FE, &FW, &FN, &FS);
         conv_diag = rho_v * (dmax(FE, 0.0) + dmax(-FW, 0.0) + dmax(FN, 0.0) + dmax(-FS, 0.0));
+        conv_E = (i == nx && br) ? 0.0 : rho_v * dmax(-FE, 0.0);
+        conv_W = (i == 1 && bl) ? 0.0 : rho_v * dmax(FW, 0.0);
+        conv_N = rho_v * dmax(-FN, 0.0);
+        conv_S = rho_v * dmax(FS, 0.0);
         if (c->conv_lin == 3) {
           const double dv_dy = (F->v[H(m, i, j + 1)] - F->v[H(m, i, j - 1)]) / (2.0 * dy);
           newt = rho_v * dmax(0.0, dv_dy) * dvol;
         }
       }
+      SETV(vW, cW_visc + conv_W);
+      SETV(vE, cE_visc + conv_E);
+      SETV(vS, cS + conv_S + 0.0);
+      SETV(vN, cN + conv_N + 0.0);
       double aP = Dv + dwx + conv_diag + 0.0 + newt + cW_visc + cE_visc + cS + cN + 0.0 + 0.0;
       if (aP <= 1.0e-15) aP = 1.0;
-      diag[dof] = aP;
+      if (B->diag) B->diag[nu + dof] = aP;
+      SETV(vP, aP);
+      vaP2[H(m, i, j)] = aP;
     }
+  orc_exchange_halos_scalar(m, vaP2);
+#undef SETV
+  if (mono) {
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const long k = W(m, i, j);
+        double aW = 0.0, aE = 0.0, aS = 0.0, aN = 0.0;
+        if (!(i == 1 && bl && wall(m->bc[0]))) aW = (dy * dy) / dmax(1.0e-30, uaP2[H(m, i, j)]);
+        if (!(i == nx && br && wall(m->bc[1]))) aE = (dy * dy) / dmax(1.0e-30, uaP2[H(m, i + 1, j)]);
+        if (!(j == 1 && bb && wall(m->bc[2]))) aS = (dx * dx) / dmax(1.0e-30, vaP2[H(m, i, j)]);
+        if (!(j == ny && bt && wall(m->bc[3]))) aN = (dx * dx) / dmax(1.0e-30, vaP2[H(m, i, j + 1)]);
+        const double reg = (i == 1 && j == 1) ? 1.0e-10 * dvol : 1.0e-12 * dvol;
+        const double aP = aW + aE + aS + aN + 0.0 + reg;
+        if (B->sW) { B->sW[k] = aW; B->sE[k] = aE; B->sS[k] = aS; B->sN[k] = aN; B->sP[k] = aP; }
+        if (B->diag) B->diag[nu + nv + k] = aP;
+      }
+  } else if (B->sW) {
+    for (long k = 0; k < (long)nx * ny; ++k) {
+      B->sW[k] = B->sE[k] = B->sS[k] = B->sN[k] = 0.0;
+      B->sP[k] = 1.0;
+    }
+  }
+  free(uaP2);
+  free(vaP2);
+}
+
+void orc_mom_diag(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  double *diag) {
+  orc_mom_blocks_t B;
+  memset(&B, 0, sizeof(B));
+  B.diag = diag;
+  orc_mom_blocks(m, c, F, dt, &B);
+}
+
+/* ssor_solve_u / _v / _p (src/module_linear_solvers.f90:1278-1403): n_sweeps
+ * symmetric lexicographic Gauss-Seidel sweeps with relaxation omega from
+ * zero on a (ni, nj) block of unknowns (dof (j-1) ni + i-1) held in the halo
+ * field x (0:nx+1, 0:ny+1), aP / a_nb the block's coefficients; with sync
+ * (a periodic side on one rank) the single-rank exchange of x before every
+ * sweep but the first and between the two halves of a sweep. */
+void orc_ssor_block(const orc_mesh *m, int ni, int nj, const double *aW, const double *aE, const double *aS,
+                    const double *aN, const double *aP, const double *rhs, double *x, int n_sweeps,
+                    double omega, int sync) {
+  const long nh = (long)(m->nx + 2) * (m->ny + 2);
+  for (long k = 0; k < nh; ++k) x[k] = 0.0;
+#define GS_CELL                                                              \
+  do {                                                                       \
+    const long dof = (long)(j - 1) * ni + (i - 1);                           \
+    double d = aP[dof];                                                      \
+    if (fabs(d) < 1.0e-30) d = 1.0;                                          \
+    double val = rhs[dof];                                                   \
+    if (aW[dof] != 0.0) val = val + aW[dof] * x[H(m, i - 1, j)];             \
+    if (aE[dof] != 0.0) val = val + aE[dof] * x[H(m, i + 1, j)];             \
+    if (aS[dof] != 0.0) val = val + aS[dof] * x[H(m, i, j - 1)];             \
+    if (aN[dof] != 0.0) val = val + aN[dof] * x[H(m, i, j + 1)];             \
+    x[H(m, i, j)] = (1.0 - omega) * x[H(m, i, j)] + omega * (val / d);       \
+  } while (0)
+  for (int sweep = 1; sweep <= n_sweeps; ++sweep) {
+    if (sync && sweep > 1) orc_exchange_halos_scalar(m, x);
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni; ++i) GS_CELL;
+    if (sync) orc_exchange_halos_scalar(m, x);
+    for (int j = nj; j >= 1; --j)
+      for (int i = ni; i >= 1; --i) GS_CELL;
+  }
+#undef GS_CELL
+}
+
+/* apply_preconditioner (src/module_linear_solvers.f90:48-316) for NONE,
+ * JACOBI, SSOR and BLOCK_TRIANGULAR on the SP or monolithic unknowns; B holds
+ * the blocks of orc_mom_blocks. */
+void orc_mom_precond(const orc_mesh *m, const orc_mom_cfg *c, int precond, const orc_mom_blocks_t *B,
+                     const double *src, double *dst) {
+  const int nx = m->nx, ny = m->ny;
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long nu = (long)u_nx * ny, nv = (long)nx * v_ny, np = (long)nx * ny;
+  const int mono = c->solver_type == 2;
+  const long n = nu + nv + (mono ? np : 0);
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  const int sync = per_x(m) || per_y(m);
+  if (precond == 1) {
+    for (long k = 0; k < n; ++k) dst[k] = fabs(B->diag[k]) > 1.0e-15 ? src[k] / B->diag[k] : src[k];
+    return;
+  }
+  if (precond != 2 && precond != 4) {
+    memcpy(dst, src, n * sizeof(double));
+    return;
+  }
+  const long nh = (long)(nx + 2) * (ny + 2);
+  double *xu = malloc(nh * sizeof(double)), *xv = malloc(nh * sizeof(double)), *xp = malloc(nh * sizeof(double));
+  double *ru = malloc(nu * sizeof(double)), *rv = malloc(nv * sizeof(double));
+  if (precond == 2 || !mono) {
+    orc_ssor_block(m, u_nx, ny, B->uW, B->uE, B->uS, B->uN, B->uP, src, xu, 2, 1.0, sync);
+    orc_ssor_block(m, nx, v_ny, B->vW, B->vE, B->vS, B->vN, B->vP, src + nu, xv, 2, 1.0, sync);
+    if (mono) orc_ssor_block(m, nx, ny, B->sW, B->sE, B->sS, B->sN, B->sP, src + nu + nv, xp, 2, 1.0, sync);
+  } else {
+    orc_ssor_block(m, nx, ny, B->sW, B->sE, B->sS, B->sN, B->sP, src + nu + nv, xp, 12, 1.25, sync);
+    double sum = 0.0;
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) sum = sum + xp[H(m, i, j)];
+    const double mean = sum / (double)((long)nx * ny);
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) xp[H(m, i, j)] = xp[H(m, i, j)] - mean;
+    if (sync) orc_exchange_halos_scalar(m, xp);
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= u_nx; ++i) {
+        const long dof = (long)(j - 1) * u_nx + (i - 1);
+        const int is_wall = (i == 1 && bl) || (i == u_nx && br);
+        ru[dof] = is_wall ? src[dof] : src[dof] - (xp[H(m, i, j)] - xp[H(m, i - 1, j)]) * m->dy;
+      }
+    orc_ssor_block(m, u_nx, ny, B->uW, B->uE, B->uS, B->uN, B->uP, ru, xu, 4, 1.0, sync);
+    for (int j = 1; j <= v_ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const long dof = (long)(j - 1) * nx + (i - 1);
+        const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+        rv[dof] = is_wall ? src[nu + dof] : src[nu + dof] - (xp[H(m, i, j)] - xp[H(m, i, j - 1)]) * m->dx;
+      }
+    orc_ssor_block(m, nx, v_ny, B->vW, B->vE, B->vS, B->vN, B->vP, rv, xv, 4, 1.0, sync);
+  }
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= u_nx; ++i) dst[(long)(j - 1) * u_nx + (i - 1)] = xu[H(m, i, j)];
+  for (int j = 1; j <= v_ny; ++j)
+    for (int i = 1; i <= nx; ++i) dst[nu + (long)(j - 1) * nx + (i - 1)] = xv[H(m, i, j)];
+  if (mono)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) dst[nu + nv + W(m, i, j)] = xp[H(m, i, j)];
+  free(xu); free(xv); free(xp); free(ru); free(rv);
 }
 
 /* ns_assemble_rhs (src/module_navier_stokes.f90:785-1101), SP branch. */
@@ -1452,23 +1696,29 @@ void orc_mom_rhs(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *
     }
 }
 
-/* apply_preconditioner NONE / JACOBI (src/module_linear_solvers.f90:88-135). */
-static void mom_precond(int precond, const double *diag, const double *r, double *z, long n) {
-  if (precond == 1) {
-    for (long k = 0; k < n; ++k) z[k] = fabs(diag[k]) > 1.0e-15 ? r[k] / diag[k] : r[k];
-  } else {
-    memcpy(z, r, n * sizeof(double));
-  }
-}
-
 /* solve_cg (src/module_linear_solvers.f90:735-838) and solve_bicgstab2
- * (:361-579) on the momentum operator; returns the reference's iters. */
-int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
-                  const double *diag, double *tu, double *tv, int lin_solver, int precond, double tol,
-                  int maxit, const double *b, double *x, double *relres) {
+ * (:361-579) on the momentum operator (SP) or the monolithic operator (tp:
+ * its pressure work field); the preconditioner from the blocks of the
+ * fields; returns the reference's iters. */
+int orc_mom_solve2(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt, double *tu,
+                   double *tv, double *tp, int lin_solver, int precond, double tol, int maxit, const double *b,
+                   double *x, double *relres) {
   int u_nx, v_ny;
   orc_mom_sizes(m, &u_nx, &v_ny);
-  const long n = (long)u_nx * m->ny + (long)m->nx * v_ny;
+  const long nu = (long)u_nx * m->ny, nv = (long)m->nx * v_ny, np = (long)m->nx * m->ny;
+  const int mono = c->solver_type == 2;
+  const long n = nu + nv + (mono ? np : 0);
+  double *bk[16];
+  for (int k = 0; k < 16; ++k) bk[k] = malloc((k == 0 ? n : (k <= 5 ? nu : (k <= 10 ? nv : np))) * sizeof(double));
+  orc_mom_blocks_t B = {bk[0], bk[1], bk[2], bk[3], bk[4], bk[5], bk[6], bk[7], bk[8], bk[9], bk[10],
+                        bk[11], bk[12], bk[13], bk[14], bk[15]};
+  orc_mom_blocks(m, c, F, dt, &B);
+#define MATVEC(src, dst)                                                      \
+  do {                                                                        \
+    if (mono) orc_mono_matvec(m, c, F, dt, tu, tv, tp, src, dst);             \
+    else orc_mom_matvec(m, c, F, dt, tu, tv, src, dst);                       \
+  } while (0)
+#define PRECOND(src, dst) orc_mom_precond(m, c, precond, &B, src, dst)
   double *w[9];
   for (int k = 0; k < 9; ++k) w[k] = calloc(n, sizeof(double));
   int iters;
@@ -1476,17 +1726,17 @@ int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
   if (lin_solver == 3) {
     double *Ax = w[0], *r = w[1], *z = w[2], *p = w[3], *Ap = w[4];
     iters = maxit;
-    orc_mom_matvec(m, c, F, dt, tu, tv, x, Ax);
+    MATVEC(x, Ax);
     for (long k = 0; k < n; ++k) r[k] = b[k] - Ax[k];
     const double r0 = sqrt(orc_dot(r, r, n));
     if (r0 < 1.0e-12) {
       iters = 0;
     } else {
-      mom_precond(precond, diag, r, z, n);
+      PRECOND(r, z);
       memcpy(p, z, n * sizeof(double));
       double rz_old = orc_dot(r, z, n);
       for (int it = 1; it <= maxit; ++it) {
-        orc_mom_matvec(m, c, F, dt, tu, tv, p, Ap);
+        MATVEC(p, Ap);
         const double pAp = orc_dot(p, Ap, n);
         if (fabs(pAp) < 1.0e-30) break;
         const double alpha = rz_old / pAp;
@@ -1498,7 +1748,7 @@ int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
           iters = it;
           break;
         }
-        mom_precond(precond, diag, r, z, n);
+        PRECOND(r, z);
         const double rz_new = orc_dot(r, z, n);
         const double beta = rz_new / rz_old;
         rz_old = rz_new;
@@ -1510,11 +1760,11 @@ int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
     double *tmp = w[8];
 #define PA(src, dst)                                      \
   do {                                                    \
-    orc_mom_matvec(m, c, F, dt, tu, tv, src, tmp);        \
-    mom_precond(precond, diag, tmp, dst, n);              \
+    MATVEC(src, tmp);                                     \
+    PRECOND(tmp, dst);                                    \
   } while (0)
     iters = 0;
-    mom_precond(precond, diag, b, bh, n);
+    PRECOND(b, bh);
     PA(x, r0);
     for (long k = 0; k < n; ++k) r0[k] = bh[k] - r0[k];
     const double r0n = sqrt(orc_dot(r0, r0, n));
@@ -1587,9 +1837,19 @@ int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
     iters = iter;
 #undef PA
   }
+#undef MATVEC
+#undef PRECOND
   if (relres) *relres = rr;
   for (int k = 0; k < 9; ++k) free(w[k]);
+  for (int k = 0; k < 16; ++k) free(bk[k]);
   return iters;
+}
+
+int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                  const double *diag, double *tu, double *tv, int lin_solver, int precond, double tol,
+                  int maxit, const double *b, double *x, double *relres) {
+  (void)diag;
+  return orc_mom_solve2(m, c, F, dt, tu, tv, NULL, lin_solver, precond, tol, maxit, b, x, relres);
 }
 
 /* The momentum block of sim_solve_standard_projection
@@ -1604,17 +1864,35 @@ int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fiel
   int u_nx, v_ny;
   orc_mom_sizes(m, &u_nx, &v_ny);
   const long n = (long)u_nx * m->ny + (long)m->nx * v_ny;
-  double *diag = malloc(n * sizeof(double)), *rhs = malloc(n * sizeof(double));
+  double *rhs = malloc(n * sizeof(double));
   double *x = malloc(n * sizeof(double));
-  orc_mom_diag(m, c, F, dt, diag);
   orc_mom_rhs(m, c, F, dt, rhs);
   orc_mom_pack(m, u, v, x);
-  const int it = orc_mom_solve(m, c, F, dt, diag, tu, tv, lin_solver, precond, tol, maxit, rhs, x, relres);
+  const int it = orc_mom_solve2(m, c, F, dt, tu, tv, NULL, lin_solver, precond, tol, maxit, rhs, x, relres);
   orc_mom_unpack(m, c, x, u, v, 0);
   orc_exchange_halos_scalar(m, u);
   orc_exchange_halos_scalar(m, v);
   orc_velocity_bc(m, c, u, v, 0);
-  free(diag);
+  free(rhs);
+  free(x);
+  return it;
+}
+
+/* The monolithic branch of sim_run's solve (src/module_simulation.f90:
+ * 553-583 with uzawa_iter 1): assemble_preconditioner_blocks, assemble_rhs
+ * (zero continuity rows), pack (u, v, p), solve_linear_system,
+ * unpack_vector(.false.) into u, v, p.  F->u, F->v are the fields_old. */
+int orc_mono_step(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt, double *tu,
+                  double *tv, double *tp, int lin_solver, int precond, double tol, int maxit, double *u, double *v,
+                  double *p, double *relres) {
+  int u_nx, v_ny;
+  orc_mom_sizes(m, &u_nx, &v_ny);
+  const long nuv = (long)u_nx * m->ny + (long)m->nx * v_ny, n = nuv + (long)m->nx * m->ny;
+  double *rhs = calloc(n, sizeof(double)), *x = malloc(n * sizeof(double));
+  orc_mom_rhs(m, c, F, dt, rhs);
+  orc_mono_pack(m, u, v, p, x);
+  const int it = orc_mom_solve2(m, c, F, dt, tu, tv, tp, lin_solver, precond, tol, maxit, rhs, x, relres);
+  orc_mono_unpack(m, c, x, u, v, p, 0);
   free(rhs);
   free(x);
   return it;

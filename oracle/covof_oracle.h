@@ -102,7 +102,13 @@ typedef struct {
   int conv_lin;           /* 1 PICARD, 2 ADAMS_BASHFORTH, 3 NEWTON */
   double gx, gy;
   double bc_u[4], bc_v[4];  /* Dirichlet values: left, right, bottom, top */
+  int solver_type;        /* 3 STANDARD_PROJECTION (u, v unknowns), 2 MONOLITHIC (u, v, p) */
 } orc_mom_cfg;
+/* Outputs of assemble_preconditioner_blocks: diag (n), u_a* (nu), v_a* (nv),
+ * schur_a* (nx*ny); any may be NULL. */
+typedef struct {
+  double *diag, *uW, *uE, *uS, *uN, *uP, *vW, *vE, *vS, *vN, *vP, *sW, *sE, *sS, *sN, *sP;
+} orc_mom_blocks_t;
 typedef struct {
   const double *u, *v, *u_old, *v_old, *u_nm1, *v_nm1;
   const double *rho_u, *rho_v, *mu, *mu_node, *fx, *fy;  /* fx/fy: fx_sigma, fy_sigma */
@@ -126,6 +132,26 @@ int orc_mom_solve(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields 
 int orc_sp_predictor(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
                      double *tu, double *tv, int lin_solver, int precond, double tol, int maxit,
                      double *u, double *v, double *relres);
+/* The monolithic system (solver_type 2): unknowns u, v, then p(1:nx,1:ny). */
+void orc_mono_pack(const orc_mesh *m, const double *u, const double *v, const double *p, double *vec);
+void orc_mono_unpack(const orc_mesh *m, const orc_mom_cfg *c, const double *vec, double *u, double *v,
+                     double *p, int homogeneous);
+void orc_mono_matvec(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                     double *tu, double *tv, double *tp, const double *x, double *y);
+void orc_mom_blocks(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt,
+                    const orc_mom_blocks_t *B);
+void orc_ssor_block(const orc_mesh *m, int ni, int nj, const double *aW, const double *aE, const double *aS,
+                    const double *aN, const double *aP, const double *rhs, double *x, int n_sweeps,
+                    double omega, int sync);
+/* precond 0 NONE, 1 JACOBI, 2 SSOR, 4 BLOCK_TRIANGULAR */
+void orc_mom_precond(const orc_mesh *m, const orc_mom_cfg *c, int precond, const orc_mom_blocks_t *B,
+                     const double *src, double *dst);
+int orc_mom_solve2(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt, double *tu,
+                   double *tv, double *tp, int lin_solver, int precond, double tol, int maxit, const double *b,
+                   double *x, double *relres);
+int orc_mono_step(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_fields *F, double dt, double *tu,
+                  double *tv, double *tp, int lin_solver, int precond, double tol, int maxit, double *u, double *v,
+                  double *p, double *relres);
 
 /* vtk_write_diagnostics (src/module_vtk_writer.f90:399-531), serial sums:
  * out[8] = center_of_mass_y, rise_velocity, bubble_volume, kinetic_energy,

@@ -27,7 +27,12 @@ PLIC, ELVIRA = 0, 1
 class MomCfg(ct.Structure):
     _fields_ = [("time_scheme", ct.c_int), ("has_nm1", ct.c_int), ("enable_convection", ct.c_int),
                 ("conv_scheme", ct.c_int), ("conv_lin", ct.c_int), ("gx", ct.c_double), ("gy", ct.c_double),
-                ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4)]
+                ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4), ("solver_type", ct.c_int)]
+
+
+class MomBlocks(ct.Structure):
+    _fields_ = [(k, ct.POINTER(ct.c_double)) for k in
+                ("diag", "uW", "uE", "uS", "uN", "uP", "vW", "vE", "vS", "vN", "vP", "sW", "sE", "sS", "sN", "sP")]
 
 
 class MomFields(ct.Structure):
@@ -91,6 +96,14 @@ def lib():
                                   _D, _D, _D]),
             "orc_sp_predictor": (i, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, i, i, d, i,
                                      _D, _D, _D]),
+            "orc_mono_pack": (None, [M, _D, _D, _D, _D]),
+            "orc_mono_matvec": (None, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, _D, _D, _D]),
+            "orc_mom_blocks": (None, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, ct.POINTER(MomBlocks)]),
+            "orc_mom_precond": (None, [M, ct.POINTER(MomCfg), i, ct.POINTER(MomBlocks), _D, _D]),
+            "orc_mom_solve2": (i, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, _D, i, i, d, i, _D, _D,
+                                   _D]),
+            "orc_mono_step": (i, [M, ct.POINTER(MomCfg), ct.POINTER(MomFields), d, _D, _D, _D, i, i, d, i, _D, _D,
+                                  _D, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -252,18 +265,21 @@ class Momentum:
         self.cfg = self._cfg(opts)
         self.F = MomFields(*[_p(self.f[k]) for k in MOM_FIELDS])
         shp = (m.ny + 2, m.nx + 2)
-        self.tu, self.tv = np.zeros(shp), np.zeros(shp)
+        self.tu, self.tv, self.tp = np.zeros(shp), np.zeros(shp), np.zeros(shp)
         a, b = ct.c_int(), ct.c_int()
         lib().orc_mom_sizes(ct.byref(m), ct.byref(a), ct.byref(b))
         self.u_nx, self.v_ny = a.value, b.value
-        self.n = self.u_nx * m.ny + m.nx * self.v_ny
+        self.mono = int(opts.get("solver_type", 3)) == 2
+        self.nu, self.nv = self.u_nx * m.ny, m.nx * self.v_ny
+        self.n = self.nu + self.nv + (m.nx * m.ny if self.mono else 0)
 
     @staticmethod
     def _cfg(opts: dict) -> MomCfg:
         return MomCfg(int(opts.get("time_scheme", 1)), int(opts.get("has_nm1", 0)),
                       int(opts.get("enable_convection", 1)), int(opts.get("conv_scheme", 1)),
                       int(opts.get("conv_lin", 1)), float(opts.get("gx", 0.0)), float(opts.get("gy", 0.0)),
-                      (ct.c_double * 4)(*opts.get("bc_u", (0.0,) * 4)), (ct.c_double * 4)(*opts.get("bc_v", (0.0,) * 4)))
+                      (ct.c_double * 4)(*opts.get("bc_u", (0.0,) * 4)), (ct.c_double * 4)(*opts.get("bc_v", (0.0,) * 4)),
+                      int(opts.get("solver_type", 3)))
 
     def diag(self):
         d = np.empty(self.n)
@@ -271,16 +287,41 @@ class Momentum:
         return d
 
     def rhs(self):
-        r = np.empty(self.n)
+        r = np.zeros(self.n)
         lib().orc_mom_rhs(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(r))
         return r
 
     def matvec(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty(self.n)
-        lib().orc_mom_matvec(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
-                             _p(self.tu), _p(self.tv), _p(x), _p(y))
+        if self.mono:
+            lib().orc_mono_matvec(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
+                                  _p(self.tu), _p(self.tv), _p(self.tp), _p(x), _p(y))
+        else:
+            lib().orc_mom_matvec(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
+                                 _p(self.tu), _p(self.tv), _p(x), _p(y))
         return y
+
+    def blocks(self):
+        """assemble_preconditioner_blocks: dict of diag, u_a*, v_a*, schur_a*."""
+        npp = self.m.nx * self.m.ny
+        sizes = dict(diag=self.n, uW=self.nu, uE=self.nu, uS=self.nu, uN=self.nu, uP=self.nu, vW=self.nv,
+                     vE=self.nv, vS=self.nv, vN=self.nv, vP=self.nv, sW=npp, sE=npp, sS=npp, sN=npp, sP=npp)
+        arrs = {k: np.zeros(v) for k, v in sizes.items()}
+        B = MomBlocks(*[_p(arrs[k]) for k, _ in MomBlocks._fields_])
+        lib().orc_mom_blocks(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, ct.byref(B))
+        self._blocks = (arrs, B)
+        return arrs
+
+    def precond(self, precond, src):
+        """apply_preconditioner (NONE, JACOBI, SSOR, BLOCK_TRIANGULAR) with
+        the blocks of the fields."""
+        arrs = self.blocks()
+        B = self._blocks[1]
+        dst = np.empty(self.n)
+        lib().orc_mom_precond(ct.byref(self.m), ct.byref(self.cfg), int(precond), ct.byref(B),
+                              _p(np.ascontiguousarray(src, dtype=np.float64)), _p(dst))
+        return dst
 
     def pack(self, u, v):
         x = np.empty(self.n)
@@ -289,12 +330,21 @@ class Momentum:
 
     def solve(self, b, x0, lin_solver=1, precond=1, tol=1e-10, maxit=1000):
         x = np.array(x0, dtype=np.float64)
-        d = self.diag()
         rr = ct.c_double()
-        it = lib().orc_mom_solve(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(d),
-                                 _p(self.tu), _p(self.tv), lin_solver, precond, tol, maxit,
-                                 _p(np.ascontiguousarray(b, dtype=np.float64)), _p(x), ct.byref(rr))
+        it = lib().orc_mom_solve2(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt,
+                                  _p(self.tu), _p(self.tv), _p(self.tp) if self.mono else None, lin_solver, precond,
+                                  tol, maxit, _p(np.ascontiguousarray(b, dtype=np.float64)), _p(x), ct.byref(rr))
         return x, it, rr.value
+
+    def mono_step(self, p, lin_solver=1, precond=4, tol=1e-10, maxit=1000):
+        """The monolithic solve of sim_run: (u, v, p, iters, relres)."""
+        u, v = self.f["u"].copy(), self.f["v"].copy()
+        p = np.array(p, dtype=np.float64, copy=True)
+        rr = ct.c_double()
+        it = lib().orc_mono_step(ct.byref(self.m), ct.byref(self.cfg), ct.byref(self.F), self.dt, _p(self.tu),
+                                 _p(self.tv), _p(self.tp), lin_solver, precond, tol, maxit, _p(u), _p(v), _p(p),
+                                 ct.byref(rr))
+        return u, v, p, it, rr.value
 
     def predictor(self, lin_solver=1, precond=1, tol=1e-10, maxit=1000):
         """Returns (u, v, iters, relres): the predicted velocity (halo arrays)."""

@@ -22,10 +22,12 @@
 !           :627-789, :792-825, :1416-1461); C, C_s, C_tilde, curvature,
 !           fx_sigma, fy_sigma after every step
 !   momentum the momentum block of the standard projection
-!           (src/module_simulation.f90:2454-2469): diag_u/diag_v of
+!           (src/module_simulation.f90:2454-2469), or the monolithic system
+!           (solver type 2): diag_u/diag_v(/diag_p) of
 !           assemble_preconditioner_blocks, assemble_rhs, two products of
-!           apply_system_matrix, then pack / solve_linear_system / unpack and
-!           the trailing sync + apply_boundary_conditions; u, v and iters
+!           apply_system_matrix, one apply_preconditioner, then pack /
+!           solve_linear_system / unpack (and, standard projection, the
+!           trailing sync + apply_boundary_conditions); u, v, p and iters
 !           (src/module_navier_stokes.f90:231-1101, :1268-1595;
 !           src/module_linear_solvers.f90:361-579, :735-838, :1167-1208)
 !   output  vtk_writer_t on given fields: init, then write_step and
@@ -51,7 +53,7 @@ program covof_ref_harness
        compute_alpha, reconstruct_interface
   use module_interface_vof, only: advect_vof, update_physical_properties
   use module_navier_stokes, only: navier_stokes_t
-  use module_linear_solvers, only: solve_linear_system
+  use module_linear_solvers, only: solve_linear_system, apply_preconditioner
   use module_mpi_comm, only: sync_u, sync_v, apply_boundary_conditions, apply_velocity_bc, apply_vof_bc
   use module_simulation, only: simulation_t
   use module_vtk_writer, only: vtk_writer_t
@@ -225,17 +227,17 @@ contains
   end subroutine run_surface
 
   subroutine run_momentum()
-    integer :: nx, ny, bc(4), tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit, iters
+    integer :: nx, ny, bc(4), tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit, iters, stype
     real(dp) :: lx, ly, dt, gx, gy, tol, bcu(4), bcv(4)
     type(mesh_2d_t) :: mesh
     type(config_t) :: cfg
     type(flow_fields_t) :: fields
     type(navier_stokes_t) :: ns
-    real(dp), allocatable :: rhs(:), x1(:), x2(:), y1(:), y2(:), state(:)
-    read(uin) nx, ny, bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit
+    real(dp), allocatable :: rhs(:), x1(:), x2(:), y1(:), y2(:), z1(:), state(:)
+    read(uin) nx, ny, bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit, stype
     read(uin) lx, ly, dt, gx, gy, tol, bcu, bcv
     call setup(mesh, cfg, nx, ny, lx, ly, bc)
-    cfg%solver_type = SOLVER_STANDARD_PROJECTION
+    cfg%solver_type = stype
     cfg%time_scheme = tscheme
     cfg%enable_convection = (conv /= 0)
     cfg%convective_scheme = scheme
@@ -262,25 +264,33 @@ contains
     call fields%allocate(mesh)
     call ns%init(mesh, cfg)
     read(uin) fields%u, fields%v, fields%u_old, fields%v_old, fields%u_nm1, fields%v_nm1
-    read(uin) fields%rho_u, fields%rho_v, fields%mu, fields%mu_node, fields%fx_sigma, fields%fy_sigma
+    read(uin) fields%rho_u, fields%rho_v, fields%mu, fields%mu_node, fields%fx_sigma, fields%fy_sigma, fields%p
     fields%has_nm1 = (has_nm1 /= 0)
-    allocate(rhs(ns%n_owned_total), x1(ns%n_owned_total), x2(ns%n_owned_total))
+    allocate(rhs(ns%n_owned_total), x1(ns%n_owned_total), x2(ns%n_owned_total), z1(ns%n_owned_total))
     allocate(y1(ns%n_owned_total), y2(ns%n_owned_total), state(ns%n_owned_total))
     read(uin) x1, x2
     call ns%assemble_preconditioner_blocks(mesh, cfg, fields, dt)
-    write(uout) ns%diag_u, ns%diag_v
+    if (stype == SOLVER_MONOLITHIC) then
+       write(uout) ns%diag_u, ns%diag_v, ns%diag_p
+    else
+       write(uout) ns%diag_u, ns%diag_v
+    end if
     call ns%assemble_rhs(mesh, cfg, fields, dt, rhs)
     write(uout) rhs
     call ns%apply_system_matrix(mesh, cfg, fields, dt, x1, y1)
     call ns%apply_system_matrix(mesh, cfg, fields, dt, x2, y2)
     write(uout) y1, y2
+    call apply_preconditioner(ns, mesh, cfg, x1, z1)
+    write(uout) z1
     call ns%pack_vector(mesh, cfg, fields, state)
     call solve_linear_system(ns, mesh, cfg, fields, dt, rhs, state, iters)
     call ns%unpack_vector(mesh, cfg, state, fields, .false.)
-    call sync_u(mesh, fields%u)
-    call sync_v(mesh, fields%v)
-    call apply_boundary_conditions(mesh, cfg, fields, .false.)
-    write(uout) fields%u, fields%v
+    if (stype /= SOLVER_MONOLITHIC) then
+       call sync_u(mesh, fields%u)
+       call sync_v(mesh, fields%v)
+       call apply_boundary_conditions(mesh, cfg, fields, .false.)
+    end if
+    write(uout) fields%u, fields%v, fields%p
     write(uout) iters
   end subroutine run_momentum
 

@@ -175,18 +175,21 @@ def momentum_fields(nx, ny, seed, rho_ratio=10.0, mu_ratio=10.0):
 
 
 def momentum(name, nx, ny, bc, f, dt, tscheme=1, has_nm1=0, conv=1, scheme=1, lin=1, solver=1, precond=1,
-             maxit=400, tol=1e-12, gx=0.0, gy=-9.81, bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=1.0, seed=0):
-    """The SP momentum block: diagonal, right-hand side, two operator
-    products, and the predicted velocity after the solve."""
+             maxit=400, tol=1e-12, gx=0.0, gy=-9.81, bcu=(0.0,) * 4, bcv=(0.0,) * 4, lx=1.0, ly=1.0, seed=0,
+             stype=3):
+    """The SP momentum block (stype 3) or the monolithic system (stype 2):
+    diagonal, right-hand side, two operator products, one preconditioner
+    application, and the velocity (and pressure) after the solve."""
     u_nx = nx + (0 if bc[0] == P else 1)
     v_ny = ny + (0 if bc[2] == P else 1)
-    n = u_nx * ny + nx * v_ny
+    n = u_nx * ny + nx * v_ny + (nx * ny if stype == 2 else 0)
     rng = np.random.default_rng(seed + 1000)
     x1, x2 = rng.standard_normal(n), rng.standard_normal(n)
-    payload = (i32(nx, ny, *bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit)
+    pfield = f.get("p", np.zeros_like(f["u"]))
+    payload = (i32(nx, ny, *bc, tscheme, has_nm1, conv, scheme, lin, solver, precond, maxit, stype)
                + f64(lx, ly, dt, gx, gy, tol, *bcu, *bcv)
-               + b"".join(f[k].astype("<f8").tobytes() for k in MOM_NAMES) + x1.astype("<f8").tobytes()
-               + x2.astype("<f8").tobytes())
+               + b"".join(f[k].astype("<f8").tobytes() for k in MOM_NAMES) + pfield.astype("<f8").tobytes()
+               + x1.astype("<f8").tobytes() + x2.astype("<f8").tobytes())
     raw = run("momentum", payload)
     h = (ny + 2) * (nx + 2)
     out = np.frombuffer(raw[:-4], dtype="<f8")
@@ -196,14 +199,17 @@ def momentum(name, nx, ny, bc, f, dt, tscheme=1, has_nm1=0, conv=1, scheme=1, li
     rhs = out[o:o + n]; o += n
     y1 = out[o:o + n]; o += n
     y2 = out[o:o + n]; o += n
+    z1 = out[o:o + n]; o += n
     u_out = out[o:o + h].reshape(ny + 2, nx + 2); o += h
     v_out = out[o:o + h].reshape(ny + 2, nx + 2); o += h
+    p_out = out[o:o + h].reshape(ny + 2, nx + 2); o += h
     assert o == out.size
     np.savez_compressed(os.path.join(HERE, f"momentum_{name}.npz"), nx=nx, ny=ny, bc=np.array(bc), dt=dt,
                         tscheme=tscheme, has_nm1=has_nm1, conv=conv, scheme=scheme, lin=lin, solver=solver,
                         precond=precond, maxit=maxit, tol=tol, gx=gx, gy=gy, bcu=np.array(bcu),
-                        bcv=np.array(bcv), lx=lx, ly=ly, x1=x1, x2=x2, diag=diag, rhs=rhs, y1=y1, y2=y2,
-                        u_out=u_out, v_out=v_out, iters=iters, flags=FLAGS, **{k: f[k] for k in MOM_NAMES})
+                        bcv=np.array(bcv), lx=lx, ly=ly, x1=x1, x2=x2, diag=diag, rhs=rhs, y1=y1, y2=y2, z1=z1,
+                        u_out=u_out, v_out=v_out, p_out=p_out, p=pfield, stype=stype, iters=iters, flags=FLAGS,
+                        **{k: f[k] for k in MOM_NAMES})
 
 
 def momentum_cases():
@@ -225,6 +231,21 @@ def momentum_cases():
     nx, ny = 22, 26
     momentum("neumann_bottom_stokes", nx, ny, (P, P, NE, DI), momentum_fields(nx, ny, 55), 5e-3, conv=0,
              lin=3, solver=3, bcu=(0.0, 0.0, 0.0, 0.5))
+    # the SP predictor with the reference's velocity SSOR (2 sweeps each of u and v)
+    nx, ny = 20, 18
+    momentum("ssor_walls", nx, ny, (DI, DI, DI, DI), momentum_fields(nx, ny, 56), 2e-3, precond=2,
+             bcu=(0.0, 0.0, 0.0, 1.0))
+    momentum("ssor_periodic_x", nx, ny, (P, P, FS, DI), momentum_fields(nx, ny, 57), 2e-3, precond=2, scheme=3)
+    # the monolithic system (fully coupled u, v, p): BiCGSTAB(2) with the
+    # block-triangular Schur preconditioner, with SSOR, with Jacobi
+    nx, ny = 16, 14
+    fm = momentum_fields(nx, ny, 58)
+    fm["p"] = np.random.default_rng(59).standard_normal(fm["u"].shape)
+    momentum("mono_bt_walls", nx, ny, (DI, DI, DI, DI), fm, 2e-3, precond=4, stype=2, bcu=(0.0, 0.0, 0.0, 1.0),
+             maxit=600, tol=1e-10)
+    momentum("mono_ssor_channel", nx, ny, (P, P, DI, FS), fm, 2e-3, precond=2, stype=2, scheme=3, maxit=600,
+             tol=1e-10)
+    momentum("mono_jacobi_walls", nx, ny, (FS, FS, DI, DI), fm, 2e-3, precond=1, stype=2, maxit=600, tol=1e-10)
 
 
 def spstep(name, nx, ny, bc, C, u, v, dt, nsteps, method=0, tscheme=2, scheme=3, lin=1, solver=3, precond=1,

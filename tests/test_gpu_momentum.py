@@ -22,7 +22,8 @@ ORC = dict(zip(NAMES, ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "r
 
 def _cfg(cv, g):
     return cv.Context.momentum_cfg(int(g["tscheme"]), int(g["has_nm1"]), int(g["conv"]), int(g["scheme"]),
-                                   int(g["lin"]), float(g["gx"]), float(g["gy"]), tuple(g["bcu"]), tuple(g["bcv"]))
+                                   int(g["lin"]), float(g["gx"]), float(g["gy"]), tuple(g["bcu"]), tuple(g["bcv"]),
+                                   int(g["stype"]))
 
 
 def _rel(a, b):
@@ -32,11 +33,16 @@ def _rel(a, b):
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "momentum_*.npz"))),
                          ids=lambda p: os.path.basename(p))
 def test_momentum_golden(cv, path):
-    """diag_u/diag_v, assemble_rhs and two products of apply_system_matrix
-    bitwise; then the whole predictor (pack, solve_linear_system, unpack,
-    sync, BC) on the same operator state as the reference's run."""
+    """diag_u/diag_v (and the Schur diagonal diag_p of the monolithic
+    system), assemble_rhs, two products of apply_system_matrix and
+    apply_preconditioner (JACOBI, the sequential SSOR or the block-triangular
+    Schur-SSOR) bitwise; then the whole block -- the SP predictor (pack,
+    solve_linear_system, unpack, sync, BC) or the monolithic solve of u, v,
+    p -- on the same operator state as the reference's run."""
     g = np.load(path, allow_pickle=False)
     nx, ny = int(g["nx"]), int(g["ny"])
+    print(f"\n[momentum golden] {os.path.basename(path)} {nx}x{ny} stype {int(g['stype'])} "
+          f"precond {int(g['precond'])}", flush=True)
     ctx = cv.Context(nx, ny, float(g["lx"]), float(g["ly"]), bc=tuple(int(b) for b in g["bc"]))
     F = {k: to_gpu(g[ORC[k]]) for k in NAMES}
     cfg = _cfg(cv, g)
@@ -48,13 +54,33 @@ def test_momentum_golden(cv, path):
     for key in ("1", "2"):
         y = ctx.apply_system_matrix(to_gpu(g["x" + key])).cpu().numpy()
         assert np.array_equal(y, g["y" + key]), f"A x{key}: max diff {np.abs(y - g['y' + key]).max()}"
-    res = ctx.solve_standard_projection_predictor(cfg, F, float(g["dt"]), int(g["solver"]), int(g["precond"]),
-                                                  float(g["tol"]), int(g["maxit"]))
+    z = ctx.apply_preconditioner(to_gpu(g["x1"]), precond=int(g["precond"])).cpu().numpy()
+    assert np.array_equal(z, g["z1"]), f"M^-1 x1: max diff {np.abs(z - g['z1']).max()}"
+    args = (float(g["dt"]), int(g["solver"]), int(g["precond"]), float(g["tol"]), int(g["maxit"]))
+    if int(g["stype"]) == cv.SOLVER_MONOLITHIC:
+        p = to_gpu(g["p"])
+        res = ctx.solve_monolithic_step(cfg, F, p, *args)
+    else:
+        res = ctx.solve_standard_projection_predictor(cfg, F, *args)
     assert res.converged, res
-    assert abs(res.iters - int(g["iters"])) <= 2, (res.iters, int(g["iters"]))
+    # the inner products sum in another order than the reference's serial
+    # ones: a long, slowly converging run (the Jacobi-preconditioned coupled
+    # system takes 130 BiCGSTAB(2) iterations) may stop a few steps apart
+    slack = max(2, int(0.05 * int(g["iters"])))
+    assert abs(res.iters - int(g["iters"])) <= slack, (res.iters, int(g["iters"]))
+    print(f"  iters {res.iters} (reference {int(g['iters'])})", flush=True)
+    # a solve stopped at relative residual tol leaves an error of up to
+    # cond(A) tol in each program; the fast runs agree to 1e-10, the slow
+    # Jacobi-preconditioned coupled solve (cond ~ 1e2 on its 1e-10 stop) to 1e-8
+    bound = 1e-10 if int(g["iters"]) < 50 else 1e-8
     for k in ("u", "v"):
         got, ref = F[k].cpu().numpy(), g[k + "_out"]
-        assert _rel(got, ref) < 1e-10, f"{k}: {_rel(got, ref):.3e}"
+        print(f"  {k}: rel diff {_rel(got, ref):.3e}", flush=True)
+        assert _rel(got, ref) < bound, f"{k}: {_rel(got, ref):.3e}"
+    if int(g["stype"]) == cv.SOLVER_MONOLITHIC:
+        got, ref = p.cpu().numpy(), g["p_out"]
+        print(f"  p: rel diff {_rel(got, ref):.3e}", flush=True)
+        assert _rel(got, ref) < bound, f"p: {_rel(got, ref):.3e}"
 
 
 # (name, bc, options, solver, preconditioner, dt factor).  CG runs with a
@@ -113,6 +139,100 @@ def test_momentum_vs_oracle(cv, orc, name, bc, opts, solver, precond, dt_factor)
         xs = o.pack(F["u"].cpu().numpy(), F["v"].cpu().numpy())
         r = o.matvec(xs) - b
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+
+
+# (name, bc, options, solver, preconditioner): the coupled (monolithic)
+# system and the sequential preconditioners on a non-square grid
+MONO_CASES = [
+    ("mono-walls-bt", (1, 1, 1, 1), dict(solver_type=2), 1, 4),
+    ("mono-channel-bdf2-tvd-ssor", (3, 3, 1, 4), dict(solver_type=2, time_scheme=2, has_nm1=True,
+                                                      convective_scheme=3, gx=0.5), 1, 2),
+    ("mono-periodic-bt", (3, 3, 3, 3), dict(solver_type=2, convective_scheme=2), 1, 4),
+    ("mono-inflow-outflow-bt", (1, 2, 4, 1), dict(solver_type=2, bc_u=(1.0, 0, 0, 0)), 1, 4),
+    ("sp-walls-ssor", (1, 1, 1, 1), dict(), 1, 2),
+    ("sp-periodic-x-ssor", (3, 3, 4, 1), dict(time_scheme=2, has_nm1=True), 1, 2),
+]
+
+
+@pytest.mark.parametrize("name,bc,opts,solver,precond", MONO_CASES, ids=[c[0] for c in MONO_CASES])
+def test_coupled_and_ssor_vs_oracle(cv, orc, name, bc, opts, solver, precond):
+    """A 112 x 80 grid: the blocks (diag_u, diag_v, diag_p), the operator
+    (with B^T p and the continuity rows when monolithic) and
+    apply_preconditioner -- the one-workgroup wavefront SSOR and the
+    block-triangular Schur-SSOR reproduce the lexicographic sweeps -- bitwise
+    against the oracle; the monolithic step (u, v, p) or the SP predictor
+    within 1e-9 of the oracle's."""
+    from covof_amd import workloads as wl
+    nx, ny = 112, 80
+    print(f"\n[coupled/ssor] {name} {nx}x{ny} bc {bc} precond {precond}", flush=True)
+    f, dt, _, lx, ly = wl.momentum_case(nx, ny, seed=11, n_circles=5, rmin=5, rmax=16, amplitude=1.0)
+    mono = opts.get("solver_type", 3) == 2
+    ctx = cv.Context(nx, ny, lx, ly, bc=bc)
+    cfg = cv.Context.momentum_cfg(**opts)
+    F = {k: to_gpu(f[k]) for k in NAMES}
+    ctx.momentum_set_fields(cfg, F, dt)
+    o = orc.Momentum(orc.mesh(nx, ny, lx, ly, bc=bc), {ORC[k]: f[k] for k in NAMES},
+                     dict(time_scheme=cfg.time_scheme, has_nm1=cfg.has_nm1, enable_convection=cfg.enable_convection,
+                          conv_scheme=cfg.convective_scheme, conv_lin=cfg.convective_linearization, gx=cfg.gx,
+                          gy=cfg.gy, bc_u=tuple(cfg.bc_u), bc_v=tuple(cfg.bc_v), solver_type=cfg.solver_type), dt)
+    assert ctx.momentum_size()[0] == o.n
+    assert np.array_equal(ctx.assemble_preconditioner_blocks().cpu().numpy(), o.diag())
+    assert np.array_equal(ctx.assemble_rhs().cpu().numpy(), o.rhs())
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        x = rng.standard_normal(o.n)
+        assert np.array_equal(ctx.apply_system_matrix(to_gpu(x)).cpu().numpy(), o.matvec(x))
+    for pc in sorted({1, 2, precond} | ({4} if mono else set())):
+        r = rng.standard_normal(o.n)
+        z = ctx.apply_preconditioner(to_gpu(r), precond=pc).cpu().numpy()
+        zo = o.precond(pc, r)
+        assert np.array_equal(z, zo), f"precond {pc}: max diff {np.abs(z - zo).max()}"
+    tol = 1e-10
+    if mono:
+        p0 = np.zeros((ny + 2, nx + 2))
+        p0[1:-1, 1:-1] = rng.standard_normal((ny, nx))
+        uo, vo, po, it_o, _ = o.mono_step(p0, solver, precond, tol, 2000)
+        p = to_gpu(p0)
+        res = ctx.solve_monolithic_step(cfg, F, p, dt, solver, precond, tol, 2000)
+    else:
+        uo, vo, it_o, _ = o.predictor(solver, precond, tol, 2000)
+        res = ctx.solve_standard_projection_predictor(cfg, F, dt, solver, precond, tol, 2000)
+    print(f"  iters {res.iters} (oracle {it_o})", flush=True)
+    assert res.converged, res
+    assert abs(res.iters - it_o) <= max(2, it_o // 10), (res.iters, it_o)
+    for k, ref in (("u", uo), ("v", vo)):
+        print(f"  {k}: rel diff {_rel(F[k].cpu().numpy(), ref):.3e}", flush=True)
+        assert _rel(F[k].cpu().numpy(), ref) < 1e-8, k
+    if mono:
+        print(f"  p: rel diff {_rel(p.cpu().numpy(), po):.3e}", flush=True)
+        assert _rel(p.cpu().numpy(), po) < 1e-8
+
+
+def test_monolithic_host_entry(cv):
+    """covof_monolithic_step (host arrays) gives the device entry's result
+    bit for bit."""
+    from covof_amd import _lib as L
+    from covof_amd import workloads as wl
+    nx, ny = 48, 40
+    f, dt, _, lx, ly = wl.momentum_case(nx, ny, seed=6, n_circles=3, rmin=4, rmax=10, amplitude=1.0)
+    bc = (1, 1, 1, 1)
+    cfg = cv.Context.momentum_cfg(solver_type=cv.SOLVER_MONOLITHIC)
+    ctx = cv.Context(nx, ny, lx, ly, bc=bc)
+    F = {k: to_gpu(f[k]) for k in NAMES}
+    p = to_gpu(np.zeros((ny + 2, nx + 2)))
+    ctx.solve_monolithic_step(cfg, F, p, dt, 1, 4, 1e-10, 500)
+    ctx2 = cv.Context(nx, ny, lx, ly, bc=bc)
+    H = {k: np.ascontiguousarray(f[k]).copy() for k in NAMES}
+    hp = np.zeros((ny + 2, nx + 2))
+    ptr = {k: H[k].ctypes.data_as(ct.POINTER(ct.c_double)) for k in NAMES}
+    it, rr = ct.c_int(), ct.c_double()
+    st = L.lib().covof_monolithic_step(ctx2._h, ct.byref(cfg), ptr["u"], ptr["v"],
+                                       hp.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                       *[ptr[k] for k in NAMES[2:]], dt, 1, 4, 1e-10, 500, ct.byref(it),
+                                       ct.byref(rr))
+    assert st == L.OK
+    for a, t in ((H["u"], F["u"]), (H["v"], F["v"]), (hp, p)):
+        assert np.array_equal(a, t.cpu().numpy())
 
 
 def test_momentum_solve_stops_and_counts(cv, orc):

@@ -91,24 +91,33 @@ def _momentum(orc, g):
     m = orc.mesh(nx, ny, float(g["lx"]), float(g["ly"]), tuple(int(b) for b in g["bc"]))
     opts = dict(time_scheme=int(g["tscheme"]), has_nm1=int(g["has_nm1"]), enable_convection=int(g["conv"]),
                 conv_scheme=int(g["scheme"]), conv_lin=int(g["lin"]), gx=float(g["gx"]), gy=float(g["gy"]),
-                bc_u=tuple(g["bcu"]), bc_v=tuple(g["bcv"]))
+                bc_u=tuple(g["bcu"]), bc_v=tuple(g["bcv"]), solver_type=int(g["stype"]))
     return orc.Momentum(m, {k: g[k] for k in orc.MOM_FIELDS}, opts, float(g["dt"]))
 
 
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "momentum_*.npz"))),
                          ids=lambda p: os.path.basename(p))
 def test_momentum_block_bitwise(orc, path):
-    """assemble_preconditioner_blocks (diag_u, diag_v), assemble_rhs, two
-    apply_system_matrix products in a row, and the whole SP momentum block
-    (pack, solve_linear_system, unpack, sync, BC): the oracle's serial inner
-    products are the reference's, so even the solve is bit-for-bit."""
+    """assemble_preconditioner_blocks (diag_u, diag_v, and diag_p of the
+    monolithic system), assemble_rhs, two apply_system_matrix products in a
+    row, apply_preconditioner of the fixture's kind (JACOBI, SSOR or
+    BLOCK_TRIANGULAR), and the whole block: the SP predictor (pack,
+    solve_linear_system, unpack, sync, BC) or the monolithic solve (u, v, p).
+    The oracle's serial inner products are the reference's, so even the solve
+    is bit-for-bit."""
     g = np.load(path, allow_pickle=False)
     M = _momentum(orc, g)
     assert np.array_equal(M.diag(), g["diag"])
     assert np.array_equal(M.rhs(), g["rhs"])
     assert np.array_equal(M.matvec(g["x1"]), g["y1"])
     assert np.array_equal(M.matvec(g["x2"]), g["y2"])
-    u, v, it, _ = M.predictor(int(g["solver"]), int(g["precond"]), float(g["tol"]), int(g["maxit"]))
+    assert np.array_equal(M.precond(int(g["precond"]), g["x1"]), g["z1"])
+    args = (int(g["solver"]), int(g["precond"]), float(g["tol"]), int(g["maxit"]))
+    if int(g["stype"]) == 2:
+        u, v, p, it, _ = M.mono_step(g["p"], *args)
+        assert np.array_equal(p, g["p_out"])
+    else:
+        u, v, it, _ = M.predictor(*args)
     assert it == int(g["iters"])
     assert np.array_equal(u, g["u_out"]) and np.array_equal(v, g["v_out"])
 
