@@ -108,7 +108,8 @@ def load_config(path: str) -> dict:
     c["ic_radius"] = ic_real("radius", 0.25)
     c["ic_axis_a"], c["ic_axis_b"] = ic_real("axis_a", 0.0), ic_real("axis_b", 0.0)
     st = g("solver.type", "AUGMENTED_LAGRANGIAN")
-    c["solver_type"] = "STANDARD_PROJECTION" if st in ("STANDARD_PROJECTION", "PROJECTION") else st
+    c["solver_type"] = {"PROJECTION": "STANDARD_PROJECTION", "FULLY_COUPLED": "MONOLITHIC",
+                        "FC": "MONOLITHIC"}.get(st, st)
     ls = g("solver.linear_solver", "BICGSTAB2")
     c["linear_solver"] = {"CG": L.LINEAR_CG, "CONJUGATE_GRADIENT": L.LINEAR_CG}.get(
         ls, L.LINEAR_BICGSTAB2 if ls == "BICGSTAB2" else ls)
@@ -116,7 +117,10 @@ def load_config(path: str) -> dict:
     c["max_iter"] = int(g("solver.max_iter", 300))
     use_precond = bool(g("solver.use_precond", True))
     pc = g("solver.precond_type", "") or g("solver.preconditioner", "JACOBI")
-    pcode = {"JACOBI": L.PRECOND_JACOBI, "DIAGONAL": L.PRECOND_JACOBI, "NONE": L.PRECOND_NONE}.get(pc, pc)
+    pcode = {"JACOBI": L.PRECOND_JACOBI, "DIAGONAL": L.PRECOND_JACOBI, "NONE": L.PRECOND_NONE,
+             "SSOR": L.PRECOND_SSOR, "GAUSS_SEIDEL": L.PRECOND_SSOR}.get(pc, pc)
+    if pc in ("BLOCK_TRIANGULAR", "SCHUR", "BT_SBP", "BT-SBP", "BLOCK", "BLOCK_PRECONDITIONER"):
+        pcode = L.PRECOND_BLOCK_TRIANGULAR
     c["precond"] = pcode if use_precond else L.PRECOND_NONE
     c["advection_method"] = g("vof.advection_method", "WEYMOUTH_YUE")
     rm = g("vof.reconstruction_method", "PLIC")
@@ -132,7 +136,7 @@ def load_config(path: str) -> dict:
 def check_supported(c: dict) -> None:
     """Raises ValueError naming the first setting the GPU run does not cover."""
     rules = [
-        (c["mode"] == "NAVIER_STOKES", "mode"), (c["solver_type"] == "STANDARD_PROJECTION", "solver.type"),
+        (c["mode"] == "NAVIER_STOKES", "mode"), (c["solver_type"] in ("STANDARD_PROJECTION", "MONOLITHIC"), "solver.type"),
         (c["enable_two_phase"], "physics.enable_two_phase"),
         (c["advection_method"] == "WEYMOUTH_YUE", "vof.advection_method"),
         (c["reconstruction_method"] in ("PLIC", "ELVIRA"), "vof.reconstruction_method"),
@@ -143,7 +147,8 @@ def check_supported(c: dict) -> None:
         (not c["enable_energy"], "enable_energy"), (not c["adaptive_dt"], "time.adaptive_dt"),
         (not c["obstacle"], "obstacle"),
         (c["linear_solver"] in (L.LINEAR_CG, L.LINEAR_BICGSTAB2), "solver.linear_solver"),
-        (c["precond"] in (L.PRECOND_NONE, L.PRECOND_JACOBI), "solver.preconditioner"),
+        (c["precond"] in (L.PRECOND_NONE, L.PRECOND_JACOBI, L.PRECOND_SSOR, L.PRECOND_BLOCK_TRIANGULAR),
+         "solver.preconditioner"),
         (c["ic_type"] in ("UNIFORM", "DROPLET", "CYLINDER", "BUBBLE"), "initial_condition.type"),
         (c["bc_top"] != L.BC_NEUMANN, "boundary_conditions.top (NEUMANN)"),
     ]
@@ -244,7 +249,8 @@ def run_case(c: dict, poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECIS
         bc_v=tuple(c[f"bc_{s}_v"] for s in ("left", "right", "bottom", "top")),
         reconstruction_method=c["reconstruction_method"],
         predictor=(c["linear_solver"], c["precond"], c["tol"], c["max_iter"]), poisson=poisson,
-        convection=c["enable_convection"])
+        convection=c["enable_convection"],
+        solver_type=L.SOLVER_MONOLITHIC if c["solver_type"] == "MONOLITHIC" else L.SOLVER_STANDARD_PROJECTION)
     f = run.f
     # the end of set_initial_conditions: CSF of the initial C; then sim_init's velocity BC
     ctx.compute_curvature_and_csf(f["C"], run.sigma, enable=run.sigma > 0.0,

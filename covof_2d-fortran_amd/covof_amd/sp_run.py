@@ -1,4 +1,5 @@
-"""The standard-projection time step of a two-phase run on the GPU.
+"""The time step of a two-phase run on the GPU: standard projection or the
+monolithic (fully coupled) system.
 
 Host-side mirror of the body of the reference's time loop for
 ``SOLVER_STANDARD_PROJECTION`` with two-phase VOF (src/module_simulation.f90:
@@ -13,6 +14,10 @@ library's entry points in the reference's order:
 4. the momentum block (velocity predictor, :2454-2469);
 5. the Poisson block (:2477-2588) and the velocity correction (:2590-2654);
 6. ``sync_u`` / ``sync_v`` + ``apply_boundary_conditions`` (:2657-2660).
+
+With ``SOLVER_MONOLITHIC`` steps 4-6 are the one-pass Uzawa loop of sim_run
+(:550-583): blocks, right-hand side, pack (u, v, p), one Krylov solve of the
+coupled system, unpack into u, v, p.
 
 Supported configuration: the momentum block's (UNIFORM profiles, no
 obstacles, no energy equation, no SVOF / diffusion smoothing, no markers).
@@ -35,7 +40,8 @@ class StandardProjectionRun:
     the Poisson block; the reference hard-codes CG + SSOR, 1e-7, 1000, FP64
     (src/module_simulation.f90:2437-2438).  ``predictor`` = (linear solver,
     preconditioner, tol, max_iter) of the momentum block (config_t's
-    linear_solver, precond_type, tol, max_iter).
+    linear_solver, precond_type, tol, max_iter); with ``solver_type``
+    SOLVER_MONOLITHIC it is the coupled system's and ``poisson`` is unused.
     """
 
     FIELDS = ("C", "u", "v", "p", "rho", "rho_u", "rho_v", "mu", "mu_node", "curvature", "fx_sigma", "fy_sigma",
@@ -45,12 +51,14 @@ class StandardProjectionRun:
                  convective_scheme=L.CONV_UPWIND, convective_linearization=L.CONV_LIN_PICARD, gx=0.0, gy=0.0,
                  bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, reconstruction_method="PLIC",
                  predictor=(L.LINEAR_CG, L.PRECOND_JACOBI, 1e-10, 500),
-                 poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True):
+                 poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True,
+                 solver_type=L.SOLVER_STANDARD_PROJECTION):
         self.ctx = ctx
         self.rho1, self.rho2, self.mu1, self.mu2, self.sigma = (float(a) for a in (rho1, rho2, mu1, mu2, sigma))
         self.opts = dict(time_scheme=time_scheme, enable_convection=bool(convection), convective_scheme=convective_scheme,
                          convective_linearization=convective_linearization, gx=gx, gy=gy, bc_u=tuple(bc_u),
-                         bc_v=tuple(bc_v))
+                         bc_v=tuple(bc_v), solver_type=int(solver_type))
+        self.mono = int(solver_type) == L.SOLVER_MONOLITHIC
         self.recon = reconstruction_method
         self.predictor = predictor
         self.poisson = poisson
@@ -75,7 +83,8 @@ class StandardProjectionRun:
 
     def step(self, dt: float):
         """One time step; returns (predictor iterations, Poisson iterations,
-        predictor status, Poisson status)."""
+        predictor status, Poisson status) -- monolithic: (coupled solve
+        iterations, 0, its status, OK)."""
         ctx, f = self.ctx, self.f
         self.nstep += 1
         has_nm1 = self.nstep > 1
@@ -94,9 +103,13 @@ class StandardProjectionRun:
                                       out={k: f[k] for k in ("curvature", "fx_sigma", "fy_sigma")})
         cfg = self.cfg(has_nm1)
         ctx.apply_velocity_bc(cfg, f["u"], f["v"])
-        # the momentum block
         mom = {k: f[k] for k in ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",
                                  "fx_sigma", "fy_sigma")}
+        if self.mono:
+            # the coupled solve of u, v, p (unpacked with the velocity BC)
+            rm = ctx.solve_monolithic_step(cfg, mom, f["p"], dt, *self.predictor)
+            return rm.iters, 0, rm.status, L.OK
+        # the momentum block
         rp = ctx.solve_standard_projection_predictor(cfg, mom, dt, *self.predictor)
         # the Poisson block, the correction, sync + BC
         solver, precond, tol, maxit, precision = self.poisson

@@ -352,13 +352,14 @@ CASE_JSON = {
 }
 
 
-def case_run():
+def case_run(js=None, out_name="case_bubble_json.npz"):
     """The reference program on a JSON case (sim%init + sim%run): a 32 x 64
     rising bubble to t = 0.0125 (12 steps of 1e-3 and a clamped last step of
     5e-4); the history file and the .vtr files of steps 0, 5, 10, 13."""
     import json
+    js = CASE_JSON if js is None else js
     with tempfile.TemporaryDirectory() as td:
-        cfg = json.loads(json.dumps(CASE_JSON))
+        cfg = json.loads(json.dumps(js))
         cfg["output"]["output_dir"] = os.path.join(td, "out")
         path = os.path.join(td, "case.json")
         with open(path, "w") as f:
@@ -369,14 +370,25 @@ def case_run():
         texts = {"history": open(os.path.join(d, "bubble_history.csv")).read()}
         for step in (0, 5, 10, 13):
             texts[f"vtr_{step}"] = open(os.path.join(d, f"bubble_p0_{step:06d}.vtr")).read()
-    np.savez_compressed(os.path.join(HERE, "case_bubble_json.npz"), json=np.array(json.dumps(CASE_JSON)),
+    np.savez_compressed(os.path.join(HERE, out_name), json=np.array(json.dumps(js)),
                         flags=FLAGS, **{k: np.array(v) for k, v in texts.items()})
+
+
+def case_mono_run():
+    """The same case with the fully coupled solver: MONOLITHIC, BiCGSTAB(2)
+    with the block-triangular Schur-SSOR preconditioner."""
+    import json
+    js = json.loads(json.dumps(CASE_JSON))
+    js["solver"] = {"type": "MONOLITHIC", "linear_solver": "BICGSTAB2", "preconditioner": "BLOCK_TRIANGULAR",
+                    "tol": 1e-10, "max_iter": 500}
+    case_run(js, "case_bubble_mono_json.npz")
 
 
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output", "case"}
+    groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output", "case",
+                                   "casemono"}
     if "surface" in groups:
         surface_cases()
     if "momentum" in groups:
@@ -387,6 +399,8 @@ def main():
         output_cases()
     if "case" in groups:
         case_run()
+    if "casemono" in groups:
+        case_mono_run()
     if not groups & {"geom", "recon", "advect"}:
         print("golden fixtures written to", HERE)
         return

@@ -37,18 +37,33 @@ def test_load_config_keys_and_defaults(tmp_path):
 
 
 @pytest.mark.parametrize("edit,key", [
-    (lambda j: j["solver"].update(type="MONOLITHIC"), "solver.type"),
-    (lambda j: j["solver"].update(preconditioner="SSOR"), "solver.preconditioner"),
+    (lambda j: j["solver"].update(type="AUGMENTED_LAGRANGIAN"), "solver.type"),
+    (lambda j: j["solver"].update(preconditioner="ILU0"), "solver.preconditioner"),
     (lambda j: j["time"].update(adaptive_dt=True), "time.adaptive_dt"),
     (lambda j: j["vof"].update(advection_method="CIAM"), "vof.advection_method"),
     (lambda j: j["boundary_conditions"]["top"].update(type="NEUMANN"), "boundary_conditions.top"),
     (lambda j: j["boundary_conditions"]["left"].update(profile="PARABOLIC"), "boundary_conditions.left.profile"),
-], ids=["monolithic", "ssor", "adaptive-dt", "ciam", "neumann-top", "parabolic"])
+], ids=["augmented-lagrangian", "ilu0", "adaptive-dt", "ciam", "neumann-top", "parabolic"])
 def test_unsupported_settings_are_named(tmp_path, edit, key):
     from covof_amd import case
     c, _ = _case(tmp_path, edit)
     with pytest.raises(ValueError, match=key.split(" ")[0]):
         case.check_supported(c)
+
+
+def test_monolithic_and_sequential_preconditioners_map(tmp_path):
+    """solver.type MONOLITHIC (and its aliases) and the SSOR / block-triangular
+    preconditioner names map as load_config_from_json maps them
+    (src/module_config.f90:569-576, :657-672) and are supported."""
+    from covof_amd import _lib as L
+    from covof_amd import case
+    for st in ("MONOLITHIC", "FULLY_COUPLED", "FC"):
+        for name, code in (("SSOR", L.PRECOND_SSOR), ("GAUSS_SEIDEL", L.PRECOND_SSOR),
+                           ("BLOCK_TRIANGULAR", L.PRECOND_BLOCK_TRIANGULAR), ("SCHUR", L.PRECOND_BLOCK_TRIANGULAR),
+                           ("BT-SBP", L.PRECOND_BLOCK_TRIANGULAR)):
+            c, _ = _case(tmp_path, lambda j: j["solver"].update(type=st, preconditioner=name))
+            assert c["solver_type"] == "MONOLITHIC" and c["precond"] == code
+            case.check_supported(c)
 
 
 def test_initial_volume_fraction_matches_reference_step0(tmp_path):

@@ -21,9 +21,14 @@ def _arrays(text):
     return out
 
 
-def test_json_case_matches_reference_program(cv, tmp_path):
+@pytest.mark.parametrize("fixture", ["case_bubble_json.npz", "case_bubble_mono_json.npz"])
+def test_json_case_matches_reference_program(cv, tmp_path, fixture):
+    """The standard-projection case (CG + Jacobi predictor, the reference's
+    CG + SSOR Poisson block) and the same case with the fully coupled solver
+    (MONOLITHIC, BiCGSTAB(2) + block-triangular Schur-SSOR)."""
     from covof_amd import case
-    g = np.load(os.path.join(GOLDEN, "case_bubble_json.npz"), allow_pickle=False)
+    print(f"\n[json case] {fixture}", flush=True)
+    g = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
     cfg = json.loads(str(g["json"]))
     out = tmp_path / "out"
     cfg["output"]["output_dir"] = str(out)
