@@ -68,7 +68,12 @@ struct Mom {
   // u_a{W,E,S,N,P}, v_a{W,E,S,N,P}, schur_a{W,E,S,N,P}
   double *bk[15] = {nullptr};
   double *aP2[2] = {nullptr};  // u_aP, v_aP on the halo grid (the Schur coefficients' source)
-  double *sx[3] = {nullptr};   // halo work fields of the three SSOR block solves
+  double *sx[3] = {nullptr};   // sx[2]: the block-triangular pressure correction (halo layout)
+  // per block (u, v, Schur), in the sequential sweeps' skewed order: the
+  // coefficient + rhs records and the two state buffers; and the four halo
+  // snapshots (W, E, S, N)
+  double *sk[3][3] = {{nullptr}};
+  double *hal[3][4] = {{nullptr}};
 };
 
 // A linear operator with its preconditioner for the generic Krylov loops

@@ -621,105 +621,251 @@ __global__ void __launch_bounds__(1024) k_exchange2(Mesh m, double *a, double *b
 
 // ----- the sequential preconditioners of apply_preconditioner
 // (src/module_linear_solvers.f90:137-316) with ssor_solve_u / _v / _p
-// (:1278-1403).  Lexicographic Gauss-Seidel is order dependent; it runs here
-// in one workgroup per block along anti-diagonals i + j = const: a cell
-// depends only on its W/S neighbours of the previous diagonal (already
-// updated) and its E/N neighbours of the next one (not yet updated), so the
-// wavefront reproduces the lexicographic sweep exactly, value for value.
-struct Blk5 {
-  const double *aW, *aE, *aS, *aN, *aP;
-};
+// (:1278-1403).  Lexicographic Gauss-Seidel is order dependent: cell (i,j)
+// needs the new W and S values and the old E and N ones.  Here one lane owns
+// one row of a 64-row strip and walks it along i, one cell per step, lane l
+// l steps behind lane l-1: the new W value is the lane's own last result, the
+// new S value the lower lane's last result (a lane shuffle; across strips an
+// LDS slot or, between passes, global memory), and the old E / N / own values
+// come from the half-sweep's input buffer, so every cell sees exactly the
+// operands of the lexicographic order and the result is the sequential
+// recurrence value for value.  A workgroup of W waves runs W strips at once,
+// wave w SEQ_LAG steps behind wave w-1 (one barrier per batch of steps);
+// strips beyond W follow in further passes.  The backward half mirrors it (strips from the
+// top, lane 0 on the strip's top row).
+//
+// The block's arrays are kept in a skewed order -- strip kk, step s = i-1+l,
+// lane l -- so that the 64 lanes of a step read 64 consecutive doubles: all
+// loads of a step are coalesced, and they are issued SEQ_D steps ahead.
+constexpr int SEQ_D = 4;
+constexpr int SEQ_MAXW = 8;
+// waves synchronise once per batch of 2 SEQ_D steps; wave w runs SEQ_LAG
+// steps behind wave w-1, so the value a strip's first row needs from the
+// strip below was made SEQ_LAG - 63 > 2 SEQ_D steps before, in an earlier
+// batch; it passes through a ring of SEQ_RING LDS slots per wave
+constexpr int SEQ_LAG = 64 + 2 * SEQ_D;
+constexpr int SEQ_RING = 32;
 
-// The right-hand side of a block: the residual, or (kind 1: u, 2: v) the
-// residual less the pressure gradient of the block-triangular step.
-__device__ __forceinline__ double blk_rhs(const Mesh &m, const double *src, const double *xp, int kind, int i,
-                                          int j, long dof, int ni, int nj) {
+struct SeqBlk {
+  int ni, nj, K, S;             // block size, strips, steps per strip (ni + 63)
+  double *rec;                  // per cell (skewed): a_P, a_W, a_E, a_S, a_N, rhs (8-double records)
+  double *xa, *xb;              // state buffers (skewed): input / output of a half-sweep
+  double *hW, *hE, *hS, *hN;    // halo snapshots x(0,j), x(ni+1,j), x(i,0), x(i,nj+1)
+};
+// record fields
+enum { SR_P = 0, SR_W = 1, SR_E = 2, SR_S = 3, SR_N = 4, SR_R = 5 };
+
+// A workgroup barrier that orders LDS only: the per-step barrier of the
+// sweeps must not wait for the global loads issued ahead (a full
+// __syncthreads() drains them); the strips exchange values through LDS only.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// lane i <- lane i-1 of a double (DPP wave_shr:1; lane 0 keeps its own value)
+__device__ __forceinline__ double wave_shr1(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
+}
+
+__host__ __device__ __forceinline__ long sk_at(int S, int kk, int s, int l) { return ((long)kk * S + s) * 64 + l; }
+__host__ __device__ __forceinline__ long sk_of(int S, int i, int j) {
+  const int kk = (j - 1) >> 6, l = (j - 1) & 63;
+  return sk_at(S, kk, i - 1 + l, l);
+}
+
+// natural (dof = (j-1) ni + i-1) -> field f of the skewed records
+__global__ void __launch_bounds__(BLK) k_to_skew(int ni, int nj, int S, const double *__restrict__ a,
+                                                 double *__restrict__ rec, int f) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)ni * nj) return;
+  const int j = (int)(k / ni) + 1, i = (int)(k - (long)(j - 1) * ni) + 1;
+  rec[sk_of(S, i, j) * 8 + f] = a[k];
+}
+
+// The right-hand side of a block, skewed: the residual, or (kind 1: u, 2: v)
+// the residual less the pressure gradient of the block-triangular step (xp: a
+// halo field).
+__global__ void __launch_bounds__(BLK) k_seq_rhs(Mesh m, SeqBlk b, int kind, const double *__restrict__ src,
+                                                 const double *__restrict__ xp) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)b.ni * b.nj) return;
+  const int j = (int)(k / b.ni) + 1, i = (int)(k - (long)(j - 1) * b.ni) + 1;
   const bool bl = !m.perx, bb = !m.pery;
+  double v = src[k];
   if (kind == 1) {
-    const bool is_wall = (i == 1 && bl) || (i == ni && bl);
-    return is_wall ? src[dof] : src[dof] - (xp[HX(i, j)] - xp[HX(i - 1, j)]) * m.dy;
+    const bool is_wall = (i == 1 && bl) || (i == b.ni && bl);
+    if (!is_wall) v = src[k] - (xp[HX(i, j)] - xp[HX(i - 1, j)]) * m.dy;
+  } else if (kind == 2) {
+    const bool is_wall = (j == 1 && bb && wallbc(m.bc[2])) || (j == b.nj && bb && wallbc(m.bc[3]));
+    if (!is_wall) v = src[k] - (xp[HX(i, j)] - xp[HX(i, j - 1)]) * m.dx;
   }
-  if (kind == 2) {
-    const bool is_wall = (j == 1 && bb && wallbc(m.bc[2])) || (j == nj && bb && wallbc(m.bc[3]));
-    return is_wall ? src[dof] : src[dof] - (xp[HX(i, j)] - xp[HX(i, j - 1)]) * m.dx;
-  }
-  return src[dof];
+  b.rec[sk_of(b.S, i, j) * 8 + SR_R] = v;
 }
 
-// n_sweeps symmetric sweeps with relaxation omega, from zero, on the (ni, nj)
-// block held in the halo field x; sync: the single-rank exchange of x before
-// every sweep but the first and between the halves of a sweep.
-__device__ void wg_ssor(const Mesh &m, int ni, int nj, Blk5 A, const double *src, const double *xp, int kind,
-                        double *x, int n_sweeps, double omega, bool sync) {
-  const int t = threadIdx.x, T = blockDim.x;
-  const long nh = (long)(m.nx + 2) * (m.ny + 2);
-  for (long k = t; k < nh; k += T) x[k] = 0.0;
-  __syncthreads();
-  auto cell = [&](int i, int j) {
-    const long dof = (long)(j - 1) * ni + (i - 1);
-    double d = A.aP[dof];
-    if (fabs(d) < 1.0e-30) d = 1.0;
-    double val = blk_rhs(m, src, xp, kind, i, j, dof, ni, nj);
-    const double aw = A.aW[dof], ae = A.aE[dof], as = A.aS[dof], an = A.aN[dof];
-    if (aw != 0.0) val = val + aw * x[HX(i - 1, j)];
-    if (ae != 0.0) val = val + ae * x[HX(i + 1, j)];
-    if (as != 0.0) val = val + as * x[HX(i, j - 1)];
-    if (an != 0.0) val = val + an * x[HX(i, j + 1)];
-    x[HX(i, j)] = (1.0 - omega) * x[HX(i, j)] + omega * (val / d);
-  };
-  for (int sweep = 1; sweep <= n_sweeps; ++sweep) {
-    if (sync && sweep > 1) wg_exchange(m, x);
-    for (int dg = 2; dg <= ni + nj; ++dg) {
-      const int lo = dg - nj > 1 ? dg - nj : 1, hi = dg - 1 < ni ? dg - 1 : ni;
-      for (int i = lo + t; i <= hi; i += T) cell(i, dg - i);
-      __syncthreads();
-    }
-    if (sync) wg_exchange(m, x);
-    for (int dg = ni + nj; dg >= 2; --dg) {
-      const int lo = dg - nj > 1 ? dg - nj : 1, hi = dg - 1 < ni ? dg - 1 : ni;
-      for (int i = hi - t; i >= lo; i -= T) cell(i, dg - i);
-      __syncthreads();
-    }
-  }
+// skewed state -> the block's part of a vector (dst) or the interior of a
+// halo field (halo != 0)
+__global__ void __launch_bounds__(BLK) k_from_skew(Mesh m, SeqBlk b, const double *__restrict__ x,
+                                                   double *__restrict__ dst, int halo) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)b.ni * b.nj) return;
+  const int j = (int)(k / b.ni) + 1, i = (int)(k - (long)(j - 1) * b.ni) + 1;
+  dst[halo ? HX(i, j) : k] = x[sk_of(b.S, i, j)];
 }
 
-// interior of the halo field x -> the block's part of dst
-__device__ void wg_store(const Mesh &m, int ni, int nj, const double *x, double *dst) {
-  const long nb = (long)ni * nj;
-  for (long k = threadIdx.x; k < nb; k += blockDim.x) {
-    const int j = (int)(k / ni) + 1, i = (int)(k - (long)(j - 1) * ni) + 1;
-    dst[k] = x[HX(i, j)];
-  }
-}
-
-struct SeqArgs {
-  Blk5 blk[3];     // u, v, Schur
-  int ni[3], nj[3];
-  long off[3];     // offsets of the blocks in the vectors
-  double *x[3];    // halo work fields
+// The operands of one cell, loaded ahead: coefficients, rhs, own old value,
+// the two old neighbours, and the new neighbour across rows where it comes
+// from a halo or an earlier pass.
+struct SeqOps {
+  double P, W, E, S, N, R, XO, O1, O2, B2;
 };
 
-// Blocks 0, 1 (and 2, the pressure) of the SSOR preconditioner, one
-// workgroup each (kind 0, 2 sweeps, omega 1), or the u and v solves of the
-// block-triangular one (kind 1 / 2 with the pressure correction xp, 4 sweeps).
-__global__ void __launch_bounds__(1024) k_pc_blocks(Mesh m, SeqArgs a, const double *__restrict__ src,
-                                                    double *__restrict__ dst, int bt) {
-  const int b = blockIdx.x;
-  const bool sync = m.perx || m.pery;
-  const int kind = bt ? b + 1 : 0;
-  wg_ssor(m, a.ni[b], a.nj[b], a.blk[b], src + a.off[b], a.x[2], kind, a.x[b], bt ? 4 : 2, 1.0, sync);
-  wg_store(m, a.ni[b], a.nj[b], a.x[b], dst + a.off[b]);
+// One half-sweep of the strips (FWD: forward) from xi into xo.  A lane's
+// row is walked as t = 0 .. ni-1 (FWD: i = t+1; backward: i = ni-t), lane l
+// l steps after lane 0, wave w SEQ_LAG w steps after wave 0; everything that
+// stays fixed along the row is set once per pass.
+template <bool FWD>
+__device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double *__restrict__ xo, double omega,
+                         int W, double (*ring)[SEQ_RING]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int l = FWD ? lane : 63 - lane;  // row of the strip
+  const int ni = b.ni, nj = b.nj, S = b.S;
+  const long SS = (long)S * 64;  // one strip
+  const int passes = (b.K + W - 1) / W;
+  const int nsteps = ((S + SEQ_LAG * (W - 1)) + 2 * SEQ_D - 1) / (2 * SEQ_D) * (2 * SEQ_D);
+  const long dstep = FWD ? 64 : -64;  // storage offset of the next cell along the row
+  const long pad = (long)b.K * SS + lane;
+  // across rows: offset of the old neighbour (FWD: N, backward: S) within the
+  // block, and of the new one in the neighbouring strip (lane 0 only)
+  const long offX = FWD ? (l < 63 ? 65 : SS - 63 * 64 - 63) : (l > 0 ? -65 : -SS + 63 * 64 + 63);
+  const long offY = FWD ? -SS + 63 * 64 + 63 : SS - 63 * 64 - 63;
+  const double *hX = FWD ? b.hN : b.hS, *hY = FWD ? b.hS : b.hN;  // halo rows, by i - 1
+  for (int pass = 0; pass < passes; ++pass) {
+    const int kq = pass * W + w;
+    const int kk = FWD ? kq : b.K - 1 - kq;
+    const int j = 64 * kk + l + 1;
+    const bool row_on = w < W && kq < b.K && j <= nj;
+    const long at0 = row_on ? sk_at(S, kk, FWD ? l : ni - 1 + l, l) : 0;  // storage of t = 0
+    const bool cross_in = FWD ? j < nj : j > 1;          // old neighbour across rows inside the block
+    const bool edge_row = FWD ? j == 1 : j == nj;        // new neighbour across rows from the halo
+    const bool prev_pass = lane == 0 && w == 0 && !edge_row;  // ... or from the strip of an earlier pass
+    const double bA = row_on ? (FWD ? b.hW[j - 1] : b.hE[j - 1]) : 0.0;  // new neighbour along the row at t = 0
+    const double eA = row_on ? (FWD ? b.hE[j - 1] : b.hW[j - 1]) : 0.0;  // old neighbour along the row at t = ni-1
+    // branch-free: every step issues the same loads (an inactive lane reads
+    // index 0), so they stay in flight SEQ_D steps
+    auto fetch = [&](int g, SeqOps &q) {
+      const int t = g - SEQ_LAG * w - lane;
+      const bool act = row_on && (unsigned)t < (unsigned)ni;
+      const long at = act ? at0 + t * dstep : 0;
+      const int i = FWD ? t + 1 : ni - t;
+      const double2 *r = (const double2 *)(b.rec + at * 8);
+      const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+      q.P = r0.x, q.W = r0.y, q.E = r1.x, q.S = r1.y, q.N = r2.x, q.R = r2.y;
+      q.XO = xi[at];
+      q.O1 = xi[(act && t < ni - 1) ? at + dstep : 0];
+      q.O2 = *(act ? (cross_in ? xi + at + offX : hX + (i - 1)) : xi);
+      q.B2 = *(act ? (edge_row ? hY + (i - 1) : (prev_pass ? xo + at + offY : xi)) : xi);
+    };
+    double cur = 0.0;
+    auto step = [&](int g, const SeqOps &o) {
+      const int t = g - SEQ_LAG * w - lane;
+      const bool act = row_on && (unsigned)t < (unsigned)ni;
+      const double up = wave_shr1(cur);     // the previous row's last result
+      const double nA = t == 0 ? bA : cur;  // new neighbour along the row
+      double nB;                            // new neighbour across rows
+      if (edge_row || prev_pass) nB = o.B2;
+      else if (lane > 0) nB = up;
+      else nB = ring[w - 1][(g - (SEQ_LAG - 63)) & (SEQ_RING - 1)];
+      const double oA = t == ni - 1 ? eA : o.O1;  // old neighbour along the row
+      const double xW = FWD ? nA : oA, xE = FWD ? oA : nA;
+      const double xS = FWD ? nB : o.O2, xN = FWD ? o.O2 : nB;
+      double dg = o.P;
+      if (fabs(dg) < 1.0e-30) dg = 1.0;
+      double val = o.R;
+      if (o.W != 0.0) val = val + o.W * xW;
+      if (o.E != 0.0) val = val + o.E * xE;
+      if (o.S != 0.0) val = val + o.S * xS;
+      if (o.N != 0.0) val = val + o.N * xN;
+      const double nv = (1.0 - omega) * o.XO + omega * (val / dg);
+      // an inactive lane stores into the padding slot past the strips
+      xo[act ? at0 + t * dstep : pad] = nv;
+      cur = act ? nv : cur;
+      if (W > 1 && lane == 63 && act) ring[w][g & (SEQ_RING - 1)] = nv;
+    };
+    // two batches of SEQ_D steps in flight: batch k+1 loads while batch k computes
+    SeqOps qa[SEQ_D], qb[SEQ_D];
+#pragma unroll
+    for (int d = 0; d < SEQ_D; ++d) fetch(d, qa[d]);
+    for (int g0 = 0; g0 < nsteps; g0 += 2 * SEQ_D) {
+#pragma unroll
+      for (int d = 0; d < SEQ_D; ++d) fetch(g0 + SEQ_D + d, qb[d]);
+#pragma unroll
+      for (int d = 0; d < SEQ_D; ++d) step(g0 + d, qa[d]);
+#pragma unroll
+      for (int d = 0; d < SEQ_D; ++d) fetch(g0 + 2 * SEQ_D + d, qa[d]);
+#pragma unroll
+      for (int d = 0; d < SEQ_D; ++d) step(g0 + SEQ_D + d, qb[d]);
+      if (W > 1) lds_barrier();
+    }
+    __syncthreads();  // the strips of this pass are read by the next one
+  }
 }
 
-// The pressure step of the block-triangular preconditioner: 12 SSOR sweeps
-// with omega 1.25 on the Schur block, the mean removed (serial sum in the
-// reference's order), the halo exchanged.
-__global__ void __launch_bounds__(1024) k_pc_bt_pressure(Mesh m, SeqArgs a, const double *__restrict__ src,
-                                                         double *__restrict__ dst) {
-  __shared__ double mean;
+// halo snapshot of the state x (skewed) as the single-rank exchange sets it:
+// x(0,j) = x(ni,j), x(ni+1,j) = x(1,j) with periodic x, x(i,0) = x(i,nj),
+// x(i,nj+1) = x(i,1) with periodic y; zero on other sides
+__device__ void seq_snapshot(const Mesh &m, const SeqBlk &b, const double *x) {
+  for (int t = threadIdx.x; t < b.nj; t += blockDim.x) {
+    b.hW[t] = m.perx ? x[sk_of(b.S, b.ni, t + 1)] : 0.0;
+    b.hE[t] = m.perx ? x[sk_of(b.S, 1, t + 1)] : 0.0;
+  }
+  for (int t = threadIdx.x; t < b.ni; t += blockDim.x) {
+    b.hS[t] = m.pery ? x[sk_of(b.S, t + 1, b.nj)] : 0.0;
+    b.hN[t] = m.pery ? x[sk_of(b.S, t + 1, 1)] : 0.0;
+  }
+  __syncthreads();
+}
+
+// n_sweeps symmetric sweeps with relaxation omega from zero (ssor_solve_*):
+// with sync (a periodic side on one rank) the halo is exchanged before every
+// sweep but the first and between the halves.  The result ends in b.xa.
+__device__ void seq_ssor(const Mesh &m, const SeqBlk &b, int n_sweeps, double omega) {
+  __shared__ double ring[SEQ_MAXW][SEQ_RING];
   const bool sync = m.perx || m.pery;
-  double *xp = a.x[2];
-  wg_ssor(m, m.nx, m.ny, a.blk[2], src + a.off[2], nullptr, 0, xp, 12, 1.25, sync);
+  const int W = min((int)(blockDim.x >> 6), b.K);
+  const long nsk = (long)b.K * b.S * 64;
+  for (long k = threadIdx.x; k < nsk; k += blockDim.x) b.xa[k] = 0.0;
+  for (int t = threadIdx.x; t < b.nj; t += blockDim.x) b.hW[t] = b.hE[t] = 0.0;
+  for (int t = threadIdx.x; t < b.ni; t += blockDim.x) b.hS[t] = b.hN[t] = 0.0;
+  __syncthreads();
+  for (int sweep = 1; sweep <= n_sweeps; ++sweep) {
+    if (sync && sweep > 1) seq_snapshot(m, b, b.xa);
+    seq_half<true>(b, b.xa, b.xb, omega, W, ring);
+    if (sync) seq_snapshot(m, b, b.xb);
+    seq_half<false>(b, b.xb, b.xa, omega, W, ring);
+  }
+}
+
+struct SeqSet {
+  SeqBlk blk[3];  // u, v, Schur
+};
+
+// One block per workgroup: blocks first .. first + gridDim.x - 1 of the set,
+// n_sweeps sweeps with relaxation omega.
+__global__ void __launch_bounds__(64 * SEQ_MAXW) k_seq_ssor(Mesh m, SeqSet a, int first, int n_sweeps,
+                                                           double omega) {
+  seq_ssor(m, a.blk[first + blockIdx.x], n_sweeps, omega);
+}
+
+// The pressure step of the block-triangular preconditioner after its 12
+// sweeps: xp holds the solution's interior; the mean is removed (serial sum
+// in the reference's order), the halo exchanged, and the pressure part of the
+// result written.
+__global__ void __launch_bounds__(1024) k_bt_mean(Mesh m, double *__restrict__ xp, double *__restrict__ dst) {
+  __shared__ double mean;
   if (threadIdx.x == 0) {
     double sum = 0.0;
     for (int j = 1; j <= m.ny; ++j)
@@ -730,11 +876,12 @@ __global__ void __launch_bounds__(1024) k_pc_bt_pressure(Mesh m, SeqArgs a, cons
   const long np = (long)m.nx * m.ny;
   for (long k = threadIdx.x; k < np; k += blockDim.x) {
     const int j = (int)(k / m.nx) + 1, i = (int)(k - (long)(j - 1) * m.nx) + 1;
-    xp[HX(i, j)] = xp[HX(i, j)] - mean;
+    const double v = xp[HX(i, j)] - mean;
+    xp[HX(i, j)] = v;
+    dst[k] = v;
   }
   __syncthreads();
-  if (sync) wg_exchange(m, xp);
-  wg_store(m, m.nx, m.ny, xp, dst + a.off[2]);
+  if (m.perx || m.pery) wg_exchange(m, xp);
 }
 
 // ns_assemble_rhs, SP branch (src/module_navier_stokes.f90:785-1101)
@@ -866,6 +1013,21 @@ int mom_alloc_blocks(Ctx &c) {
   for (double *&b : M.bk) COVOF_CHECK(hipMalloc(&b, nb * sizeof(double)));
   for (double *&b : M.aP2) COVOF_CHECK(hipMalloc(&b, nh * sizeof(double)));
   for (double *&b : M.sx) COVOF_CHECK(hipMalloc(&b, nh * sizeof(double)));
+  // skewed arrays of the sequential sweeps: u (u_nx, ny), v (nx, v_ny), p (nx, ny)
+  const int u_nx = c.m.nx + (c.m.perx ? 0 : 1), v_ny = c.m.ny + (c.m.pery ? 0 : 1);
+  const int ni[3] = {u_nx, c.m.nx, c.m.nx}, nj[3] = {c.m.ny, v_ny, c.m.ny};
+  for (int b = 0; b < 3; ++b) {
+    // + 64: the padding slot the sweeps' inactive lanes store into
+    const long nsk = (long)((nj[b] + 63) / 64) * (ni[b] + 63) * 64 + 64;
+    for (int q = 0; q < 3; ++q) {
+      const long len = q == 0 ? 8 * nsk : nsk;  // the records, then the two state buffers
+      COVOF_CHECK(hipMalloc(&M.sk[b][q], len * sizeof(double)));
+      hipLaunchKernelGGL(k_fill, grid_of(len), dim3(BLK), 0, c.stream, len, 0.0, M.sk[b][q]);
+    }
+    const long nhal = ni[b] > nj[b] ? ni[b] : nj[b];
+    for (double *&p : M.hal[b]) COVOF_CHECK(hipMalloc(&p, nhal * sizeof(double)));
+  }
+  COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
 
@@ -884,6 +1046,8 @@ struct MomOp : LinOp {
   }
   int precond(const double *r, double *z) override { return mom_precond(c, pc, r, z); }
 };
+
+SeqSet seq_set(Ctx &c);
 
 // assemble_preconditioner_blocks into M.diag and (full) the SSOR / Schur
 // coefficients of M.bk
@@ -916,6 +1080,18 @@ int mom_blocks(Ctx &c, bool full) {
                        (const double *)M.aP2[0], (const double *)M.aP2[1], s[0], s[1], s[2], s[3], s[4],
                        M.diag + M.nuv);
   }
+  if (full) {
+    // the SSOR coefficients in the sweeps' skewed order
+    const SeqSet a = seq_set(c);
+    for (int b = 0; b < (M.cfg.mono ? 3 : 2); ++b) {
+      const SeqBlk &q = a.blk[b];
+      // records a_P, a_W, a_E, a_S, a_N from the blocks' W, E, S, N, P arrays
+      const int field[5] = {SR_W, SR_E, SR_S, SR_N, SR_P};
+      for (int k = 0; k < 5; ++k)
+        hipLaunchKernelGGL(k_to_skew, grid_of((long)q.ni * q.nj), dim3(BLK), 0, c.stream, q.ni, q.nj, q.S,
+                           (const double *)M.bk[5 * b + k], q.rec, field[k]);
+    }
+  }
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -929,6 +1105,10 @@ void mom_free(Ctx &c) {
   for (double *p : M.bk) hipFree(p);
   for (double *p : M.aP2) hipFree(p);
   for (double *p : M.sx) hipFree(p);
+  for (auto &blk : M.sk)
+    for (double *p : blk) hipFree(p);
+  for (auto &blk : M.hal)
+    for (double *p : blk) hipFree(p);
   M = Mom();
 }
 
@@ -1002,16 +1182,24 @@ int mom_product(Ctx &c, const double *x, double *y, bool jacobi) {
 }
 
 // the blocks of the sequential preconditioners, as the kernels take them
-SeqArgs seq_args(Ctx &c) {
+SeqSet seq_set(Ctx &c) {
   Mom &M = c.mom;
-  SeqArgs a{};
-  for (int b = 0; b < 3; ++b)
-    a.blk[b] = Blk5{M.bk[5 * b], M.bk[5 * b + 1], M.bk[5 * b + 2], M.bk[5 * b + 3], M.bk[5 * b + 4]};
-  a.ni[0] = M.u_nx, a.nj[0] = c.m.ny, a.off[0] = 0;
-  a.ni[1] = c.m.nx, a.nj[1] = M.v_ny, a.off[1] = M.nu;
-  a.ni[2] = c.m.nx, a.nj[2] = c.m.ny, a.off[2] = M.nuv;
-  for (int b = 0; b < 3; ++b) a.x[b] = M.sx[b];
+  SeqSet a{};
+  const int ni[3] = {M.u_nx, c.m.nx, c.m.nx}, nj[3] = {c.m.ny, M.v_ny, c.m.ny};
+  for (int b = 0; b < 3; ++b) {
+    SeqBlk &q = a.blk[b];
+    q.ni = ni[b], q.nj = nj[b], q.K = (nj[b] + 63) / 64, q.S = ni[b] + 63;
+    q.rec = M.sk[b][0], q.xa = M.sk[b][1], q.xb = M.sk[b][2];
+    q.hW = M.hal[b][0], q.hE = M.hal[b][1], q.hS = M.hal[b][2], q.hN = M.hal[b][3];
+  }
   return a;
+}
+
+// one wave per strip of the largest of the launched blocks, at most SEQ_MAXW
+inline dim3 seq_threads(const SeqSet &a, int first, int nb) {
+  int K = 1;
+  for (int b = first; b < first + nb; ++b) K = a.blk[b].K > K ? a.blk[b].K : K;
+  return dim3(64 * (K < SEQ_MAXW ? K : SEQ_MAXW));
 }
 }  // namespace
 
@@ -1029,12 +1217,37 @@ int mom_precond(Ctx &c, int precond, const double *r, double *z) {
   if (precond == COVOF_PRECOND_JACOBI) {
     hipLaunchKernelGGL(k_mom_jacobi, grid_of(M.n), dim3(BLK), 0, c.stream, M.n, (const double *)M.diag, r, z);
   } else if (precond == COVOF_PRECOND_SSOR || precond == COVOF_PRECOND_BLOCK_TRIANGULAR) {
-    const SeqArgs a = seq_args(c);
+    const SeqSet a = seq_set(c);
+    const long off[3] = {0, M.nu, M.nuv};
+    auto cells = [&](int b) { return grid_of((long)a.blk[b].ni * a.blk[b].nj); };
     if (precond == COVOF_PRECOND_BLOCK_TRIANGULAR && M.cfg.mono) {
-      hipLaunchKernelGGL(k_pc_bt_pressure, dim3(1), dim3(1024), 0, c.stream, c.m, a, r, z);
-      hipLaunchKernelGGL(k_pc_blocks, dim3(2), dim3(1024), 0, c.stream, c.m, a, r, z, 1);
+      // pressure: 12 sweeps at omega 1.25, mean removed, exchanged
+      double *xp = M.sx[2];
+      const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+      hipLaunchKernelGGL(k_seq_rhs, cells(2), dim3(BLK), 0, c.stream, c.m, a.blk[2], 0, r + off[2],
+                         (const double *)nullptr);
+      hipLaunchKernelGGL(k_seq_ssor, dim3(1), seq_threads(a, 2, 1), 0, c.stream, c.m, a, 2, 12, 1.25);
+      hipLaunchKernelGGL(k_fill, grid_of(nh), dim3(BLK), 0, c.stream, nh, 0.0, xp);
+      hipLaunchKernelGGL(k_from_skew, cells(2), dim3(BLK), 0, c.stream, c.m, a.blk[2], (const double *)a.blk[2].xa,
+                         xp, 1);
+      hipLaunchKernelGGL(k_bt_mean, dim3(1), dim3(1024), 0, c.stream, c.m, xp, z + off[2]);
+      // velocity: 4 sweeps each against rhs - grad p
+      for (int b = 0; b < 2; ++b)
+        hipLaunchKernelGGL(k_seq_rhs, cells(b), dim3(BLK), 0, c.stream, c.m, a.blk[b], b + 1, r + off[b],
+                           (const double *)xp);
+      hipLaunchKernelGGL(k_seq_ssor, dim3(2), seq_threads(a, 0, 2), 0, c.stream, c.m, a, 0, 4, 1.0);
+      for (int b = 0; b < 2; ++b)
+        hipLaunchKernelGGL(k_from_skew, cells(b), dim3(BLK), 0, c.stream, c.m, a.blk[b],
+                           (const double *)a.blk[b].xa, z + off[b], 0);
     } else {
-      hipLaunchKernelGGL(k_pc_blocks, dim3(M.cfg.mono ? 3 : 2), dim3(1024), 0, c.stream, c.m, a, r, z, 0);
+      const int nb = M.cfg.mono ? 3 : 2;
+      for (int b = 0; b < nb; ++b)
+        hipLaunchKernelGGL(k_seq_rhs, cells(b), dim3(BLK), 0, c.stream, c.m, a.blk[b], 0, r + off[b],
+                           (const double *)nullptr);
+      hipLaunchKernelGGL(k_seq_ssor, dim3(nb), seq_threads(a, 0, nb), 0, c.stream, c.m, a, 0, 2, 1.0);
+      for (int b = 0; b < nb; ++b)
+        hipLaunchKernelGGL(k_from_skew, cells(b), dim3(BLK), 0, c.stream, c.m, a.blk[b],
+                           (const double *)a.blk[b].xa, z + off[b], 0);
     }
   } else {
     if (z != r) COVOF_CHECK(hipMemcpyAsync(z, r, M.n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
