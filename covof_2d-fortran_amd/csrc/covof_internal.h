@@ -131,6 +131,11 @@ struct Poisson {
   int nblocks = 0;          // reduction grid size
   PGraph graph[2];          // FP64 cycle, mixed cycle
   hipEvent_t cycle_ev[2] = {nullptr, nullptr};
+  // the exact symmetric Gauss-Seidel (PRECOND_SSOR) in the strip sweeps'
+  // skewed order: coefficient records, two state buffers, four halo rows;
+  // records of the assembly sgs_id
+  double *sgs_rec = nullptr, *sgs_xa = nullptr, *sgs_xb = nullptr, *sgs_h = nullptr;
+  int sgs_id = -1;
 };
 
 // Optional per-launch timing of the hot kernels with HIP events on the

@@ -20,6 +20,7 @@
 // needed inside an iteration.  The host polls the stop flag every few dozen
 // iterations.
 #include "covof_internal.h"
+#include "seq_sweep.h"
 
 #include <algorithm>
 #include <cmath>
@@ -280,47 +281,49 @@ __global__ void k_set_scalars(double *scal, int first, int n, double v) {
   if (t < n) scal[first + t] = v;
 }
 
-// Symmetric Gauss-Seidel of poisson_ssor (src/module_simulation.f90:2709-2758)
-// in one workgroup by anti-diagonals i + j = s: the forward sweep reads only
-// west/south values of earlier diagonals, the backward sweep east/north values
-// of later ones, and the periodic wraps read exactly the states the
-// lexicographic order sees (still 0 forward, the forward value backward), so
-// the result equals the sequential recurrence bit-for-bit.
-__global__ void __launch_bounds__(1024) k_sgs_wavefront(DevOp o, OpT<double> A, const double *__restrict__ r,
-                                                        double *z, const double *flag) {
-  if (flag && *flag != 0.0) return;
+// The reference's symmetric Gauss-Seidel (poisson_ssor, :2709-2758): a
+// forward sweep (D + L) z = r, then a backward sweep (D + U) z = D z, both in
+// lexicographic order, with the periodic wrap terms reading what that order
+// reads (still 0 forward, the forward value backward).  It runs on the strip
+// sweeps of seq_sweep.h in their Poisson mode: per cell a record of a_P and
+// the four face coefficients, zero where the reference adds no term (a wall
+// face), so every cell evaluates the reference's expression on the same
+// operands, bit for bit.  A flag slot (the solve's convergence flag, or null)
+// skips the application once the solve has stopped.
+__global__ void __launch_bounds__(BLK) k_sgs_records(DevOp o, OpT<double> A, int S, double *__restrict__ rec) {
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
   const int nx = o.nx, ny = o.ny;
-  const long n = (long)nx * ny;
-  for (long t = threadIdx.x; t < n; t += blockDim.x) z[t] = 0.0;
-  __syncthreads();
-  for (int s = 2; s <= nx + ny; ++s) {
-    const int ilo = s - ny > 1 ? s - ny : 1, ihi = s - 1 < nx ? s - 1 : nx;
-    for (int i = ilo + threadIdx.x; i <= ihi; i += blockDim.x) {
-      const int j = s - i;
-      const long k = (long)(j - 1) * nx + (i - 1);
-      double val = r[k];
-      if (i > 1) val = val + A.kx[kxi(nx, i, j)] * z[k - 1];
-      else if (o.perW) val = val + A.kx[kxi(nx, 1, j)] * z[k + (nx - 1)];
-      if (j > 1) val = val + A.ky[kyi(nx, i, j)] * z[k - nx];
-      else if (o.perS) val = val + A.ky[kyi(nx, i, 1)] * z[k + (long)(ny - 1) * nx];
-      z[k] = val / A.diag[k];
-    }
-    __syncthreads();
-  }
-  for (int s = nx + ny; s >= 2; --s) {
-    const int ilo = s - ny > 1 ? s - ny : 1, ihi = s - 1 < nx ? s - 1 : nx;
-    for (int i = ihi - threadIdx.x; i >= ilo; i -= blockDim.x) {
-      const int j = s - i;
-      const long k = (long)(j - 1) * nx + (i - 1);
-      double val = A.diag[k] * z[k];
-      if (i < nx) val = val + A.kx[kxi(nx, i + 1, j)] * z[k + 1];
-      else if (o.perE) val = val + A.kx[kxi(nx, nx + 1, j)] * z[k - (nx - 1)];
-      if (j < ny) val = val + A.ky[kyi(nx, i, j + 1)] * z[k + nx];
-      else if (o.perN) val = val + A.ky[kyi(nx, i, ny + 1)] * z[k - (long)(ny - 1) * nx];
-      z[k] = val / A.diag[k];
-    }
-    __syncthreads();
-  }
+  if (k >= (long)nx * ny) return;
+  const int j = (int)(k / nx) + 1, i = (int)(k - (long)(j - 1) * nx) + 1;
+  double *r = rec + sk_of(S, i, j) * 8;
+  r[SR_P] = A.diag[k];
+  r[SR_W] = (i > 1 || o.perW) ? A.kx[kxi(nx, i, j)] : 0.0;
+  r[SR_E] = (i < nx || o.perE) ? A.kx[kxi(nx, i + 1, j)] : 0.0;
+  r[SR_S] = (j > 1 || o.perS) ? A.ky[kyi(nx, i, j)] : 0.0;
+  r[SR_N] = (j < ny || o.perN) ? A.ky[kyi(nx, i, j + 1)] : 0.0;
+}
+
+__global__ void __launch_bounds__(BLK) k_sgs_in(DevOp o, int S, const double *__restrict__ r, double *rec,
+                                                const double *flag) {
+  if (flag && *flag != 0.0) return;
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)o.nx * o.ny) return;
+  const int j = (int)(k / o.nx) + 1, i = (int)(k - (long)(j - 1) * o.nx) + 1;
+  rec[sk_of(S, i, j) * 8 + SR_R] = r[k];
+}
+
+__global__ void __launch_bounds__(64 * SEQ_MAXW) k_sgs_sweep(DevOp o, SeqBlk b, const double *flag) {
+  if (flag && *flag != 0.0) return;
+  seq_sweeps<SEQ_POISSON_SGS>(o.perW != 0, o.perS != 0, b, 1, 1.0);
+}
+
+__global__ void __launch_bounds__(BLK) k_sgs_out(DevOp o, int S, const double *__restrict__ x,
+                                                 double *__restrict__ z, const double *flag) {
+  if (flag && *flag != 0.0) return;
+  const long k = (long)blockIdx.x * BLK + threadIdx.x;
+  if (k >= (long)o.nx * o.ny) return;
+  const int j = (int)(k / o.nx) + 1, i = (int)(k - (long)(j - 1) * o.nx) + 1;
+  z[k] = x[sk_of(S, i, j)];
 }
 
 // ------------------------------------------------------------ PCG (fused)
@@ -1172,7 +1175,7 @@ void poisson_free(Ctx &c) {
   Poisson &P = c.ps;
   void *ptrs[] = {P.kx, P.ky, P.diag, P.kx32, P.ky32, P.diag32, P.b, P.x, P.r, P.p, P.q, P.z,
                   P.t0, P.t1, P.t2, P.t3, P.t4, P.t5, P.r32, P.p32, P.q32, P.x32, P.z32, P.part, P.scal, P.ticket,
-                  P.fx, P.fy, P.cg2};
+                  P.fx, P.fy, P.cg2, P.sgs_rec, P.sgs_xa, P.sgs_xb, P.sgs_h};
   for (void *q : ptrs) hipFree(q);
   if (P.hscal) hipHostFree(P.hscal);
   for (PGraph &G : P.graph)
@@ -1226,15 +1229,42 @@ static int pc_code(int precond) {
 static int mg_kind(int pc) { return pc == 3 ? MG_KIND_SGS_RB : MG_KIND_VCYCLE; }
 
 // z = M^-1 r for the explicit preconditioners of precision T.
+
+// z = SGS(r) on the strip sweeps (records rebuilt after each assembly);
+// flag: a device scalar that skips the application when nonzero, or null.
+static int sgs_apply(Ctx &c, const double *r, double *z, const double *flag) {
+  Poisson &P = c.ps;
+  const DevOp o = dev_op(c.m);
+  const int S = o.nx + 63;
+  const long len = seq_len(o.nx, o.ny), n = (long)o.nx * o.ny;
+  const dim3 g((unsigned)((n + BLK - 1) / BLK));
+  if (!P.sgs_rec) {
+    COVOF_CHECK(hipMalloc(&P.sgs_rec, 8 * len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&P.sgs_xa, len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&P.sgs_xb, len * sizeof(double)));
+    const long nh = o.nx > o.ny ? o.nx : o.ny;
+    COVOF_CHECK(hipMalloc(&P.sgs_h, 4 * nh * sizeof(double)));
+  }
+  if (P.sgs_id != P.assembly_id) {
+    OpT<double> A{P.kx, P.ky, P.diag};
+    hipLaunchKernelGGL(k_sgs_records, g, dim3(BLK), 0, c.stream, o, A, S, P.sgs_rec);
+    P.sgs_id = P.assembly_id;
+  }
+  const long nh = o.nx > o.ny ? o.nx : o.ny;
+  SeqBlk b{o.nx, o.ny, (o.ny + 63) / 64, S, P.sgs_rec, P.sgs_xa, P.sgs_xb,
+           P.sgs_h, P.sgs_h + nh, P.sgs_h + 2 * nh, P.sgs_h + 3 * nh};
+  hipLaunchKernelGGL(k_sgs_in, g, dim3(BLK), 0, c.stream, o, S, r, P.sgs_rec, flag);
+  hipLaunchKernelGGL(k_sgs_sweep, dim3(1), dim3(64 * seq_waves(o.ny)), 0, c.stream, o, b, flag);
+  hipLaunchKernelGGL(k_sgs_out, g, dim3(BLK), 0, c.stream, o, S, (const double *)P.sgs_xa, z, flag);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
 template <typename T>
 static void explicit_precond(Ctx &c, int pc, const T *r, T *z, int fslot);
 template <>
 void explicit_precond<double>(Ctx &c, int pc, const double *r, double *z, int fslot) {
   if (pc == 2) {
-    const DevOp o = dev_op(c.m);
-    OpT<double> A64{c.ps.kx, c.ps.ky, c.ps.diag};
-    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A64, r, z,
-                       (const double *)(c.ps.scal + fslot));
+    sgs_apply(c, r, z, (const double *)(c.ps.scal + fslot));
   } else {
     mg_apply64(c, mg_kind(pc), r, z, S_FLAG, fslot);
   }
@@ -1248,9 +1278,8 @@ int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
   if (!c.ps.assembled) return COVOF_BAD_ARGUMENT;
   const long n = (long)c.m.nx * c.m.ny;
   if (precond == COVOF_PRECOND_SSOR) {
-    const DevOp o = dev_op(c.m);
-    OpT<double> A{c.ps.kx, c.ps.ky, c.ps.diag};
-    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, r, z, (const double *)nullptr);
+    const int st = sgs_apply(c, r, z, nullptr);
+    if (st) return st;
   } else if (precond == COVOF_PRECOND_JACOBI || precond == COVOF_PRECOND_NONE) {
     hipLaunchKernelGGL(k_precond_simple, dim3((n + 255) / 256), dim3(256), 0, c.stream, precond, n,
                        c.ps.diag, r, z);
@@ -1546,8 +1575,7 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
   COVOF_CHECK_LAUNCH();
   if (pc == 2) {
     // z = SGS(r) and r.z into slot 1 of the residual partials, then launch by launch
-    hipLaunchKernelGGL(k_sgs_wavefront, dim3(1), dim3(1024), 0, c.stream, o, A, (const double *)P.r, P.z,
-                       (const double *)nullptr);
+    if ((st = sgs_apply(c, (const double *)P.r, P.z, nullptr))) return st;
     hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
                        (const double *)P.scal, OUTER, pb);
     COVOF_CHECK_LAUNCH();
