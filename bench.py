@@ -341,6 +341,83 @@ def main():
         f"Poisson {sp_step['poisson_iters']} it")
     del run, sctx, zs
 
+    # ------------------------------------------ the plain-method case (128 x 256)
+    # the reference's own CPU-runnable case: 2-D rising bubble on a 1 x 2 box,
+    # rho 1000 / 100, mu 10 / 1, sigma 24.5, g 0.98, circle r 0.25 at (0.5, 0.5)
+    # (set_initial_conditions' BUBBLE), BDF2, TVD superbee, PLIC + WY, dt 1e-3,
+    # 200 steps; predictor CG + Jacobi to 1e-12, the reference's hard-coded
+    # Poisson block (CG + exact symmetric Gauss-Seidel to 1e-7)
+    import tempfile
+    from covof_amd import case as cvcase
+    pj = {"case_name": "plain", "mode": "NAVIER_STOKES", "mesh": {"nx": 128, "ny": 256, "lx": 1.0, "ly": 2.0},
+          "physics": {"enable_two_phase": True, "enable_surface_tension": True, "sigma": 24.5, "gy": -0.98,
+                      "convective_scheme": "TVD_SUPERBEE", "fluid1": {"rho": 1000.0, "mu": 10.0},
+                      "fluid2": {"rho": 100.0, "mu": 1.0}},
+          "time": {"time_scheme": "BDF2", "dt": 1e-3, "t_final": 0.2},
+          "solver": {"type": "STANDARD_PROJECTION", "linear_solver": "CG", "preconditioner": "JACOBI",
+                     "tol": 1e-12, "max_iter": 500},
+          "vof": {"advection_method": "WEYMOUTH_YUE", "reconstruction_method": "PLIC"},
+          "boundary_conditions": {"left": {"type": "FREE_SLIP"}, "right": {"type": "FREE_SLIP"},
+                                  "bottom": {"type": "DIRICHLET"}, "top": {"type": "DIRICHLET"}},
+          "initial_condition": {"type": "BUBBLE", "center_x": 0.5, "center_y": 0.5, "radius": 0.25}}
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as fj:
+        json.dump(pj, fj)
+    pc = cvcase.load_config(fj.name)
+    os.unlink(fj.name)
+    C0 = cvcase.initial_volume_fraction(pc)
+    cvcase._vof_halo(C0, pc)
+    pctx = cv.Context(128, 256, 1.0, 2.0, bc=(cv.BC_FREESLIP, cv.BC_FREESLIP, cv.BC_DIRICHLET, cv.BC_DIRICHLET))
+    z0 = torch.zeros((258, 130), dtype=torch.float64, device=dev)
+    prun = cv.StandardProjectionRun(pctx, torch.tensor(C0, device=dev), z0, z0, 1000.0, 100.0, 10.0, 1.0, 24.5,
+                                    time_scheme=cv.TIME_SCHEME_GEAR, convective_scheme=cv.CONV_TVD_SUPERBEE,
+                                    gy=-0.98, predictor=(cv.LINEAR_CG, cv.PRECOND_JACOBI, 1e-12, 500))
+    f0 = prun.f
+    pctx.compute_curvature_and_csf(f0["C"], 24.5, out={k: f0[k] for k in ("curvature", "fx_sigma", "fy_sigma")})
+    pctx.apply_velocity_bc(prun.cfg(False), f0["u"], f0["v"])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    p_its = [prun.step(1e-3) for _ in range(200)]
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t) / 200
+    plain = {"workload": "plain-method rising bubble 128x256, 200 steps of dt 1e-3 (the reference's configuration; "
+                         "Poisson CG + exact SGS to 1e-7)",
+             "ms_per_step": t_plain * 1e3, "predictor_iters_total": sum(a[0] for a in p_its),
+             "poisson_iters_total": sum(a[1] for a in p_its), "kinetic_energy": prun.kinetic_energy()}
+    log(f"plain case: {t_plain * 1e3:.2f} ms/step, iterations {plain['predictor_iters_total']} + "
+        f"{plain['poisson_iters_total']}")
+    del prun, pctx
+
+    # ---------------------------------------------- the monolithic (coupled) step
+    # one coupled solve of u, v, p (one Uzawa pass) on the momentum case at
+    # 128^2 and 256^2: BiCGSTAB(2) + block-triangular Schur-SSOR to 1e-10
+    mono = {"workload": "monolithic step on the momentum case (rho 1000/100, mu 10/1, BDF2, TVD, no-slip box): "
+                        "BiCGSTAB(2) + block-triangular Schur-SSOR to 1e-10"}
+    for mn in (128, 256):
+        fmn, dtmn, bcmn, lxmn, lymn = wl.momentum_case(mn, mn, seed=0)
+        mctx = cv.Context(mn, mn, lxmn, lymn, bc=bcmn)
+        mcfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98,
+                                       solver_type=cv.SOLVER_MONOLITHIC)
+        mbase = {k: torch.tensor(fmn[k], device=dev) for k in cv.Context.MOMENTUM_FIELDS}
+        mctx.momentum_set_fields(mcfg, mbase, dtmn)
+        rvec = mctx.new_momentum_vector(1.0)
+        mctx.apply_preconditioner(rvec, precond=cv.PRECOND_BLOCK_TRIANGULAR)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            mctx.apply_preconditioner(rvec, precond=cv.PRECOND_BLOCK_TRIANGULAR)
+        torch.cuda.synchronize()
+        t_bt = (time.perf_counter() - t) / 3
+        pm = torch.zeros((mn + 2, mn + 2), dtype=torch.float64, device=dev)
+        t = time.perf_counter()
+        rmn = mctx.solve_monolithic_step(mcfg, mbase, pm, dtmn, cv.LINEAR_BICGSTAB2, cv.PRECOND_BLOCK_TRIANGULAR,
+                                         1e-10, 2000)
+        torch.cuda.synchronize()
+        t_mono = time.perf_counter() - t
+        mono[f"n{mn}"] = {"unknowns": mctx.momentum_size()[0], "iters": rmn.iters, "relres": rmn.relres,
+                          "status": rmn.status, "s": t_mono, "block_triangular_ms": t_bt * 1e3}
+        log(f"monolithic {mn}^2: {rmn.iters} it, {t_mono:.3f} s, block-triangular {t_bt * 1e3:.2f} ms")
+        del mctx, mbase, rvec, pm
+
     # ----------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -390,6 +467,8 @@ def main():
         "vof": vof,
         "sp_predictor": predictor,
         "sp_step": sp_step,
+        "plain_case": plain,
+        "monolithic": mono,
     }
     print(json.dumps(out), flush=True)
 

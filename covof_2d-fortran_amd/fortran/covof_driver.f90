@@ -13,9 +13,12 @@
 !>   pred : the momentum block of sim_solve_standard_projection
 !>          (src/module_simulation.f90:2454-2469) on a lid-driven box holding a
 !>          heavy viscous disc (rho 1000 / 100, mu 10 / 1), BDF2 with an n-1
-!>          state, TVD superbee convection, gravity, BiCGSTAB(2) + Jacobi to 1e-10.
+!>          state, TVD superbee convection, gravity, BiCGSTAB(2) + Jacobi to 1e-10;
+!>   mono : the monolithic branch of sim_run (src/module_simulation.f90:550-583)
+!>          on the same box: u, v, p coupled, BiCGSTAB(2) + block-triangular
+!>          Schur-SSOR to 1e-10.
 !>
-!> usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE
+!> usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE
 !>
 !> OUTFILE (unformatted stream, native endianness):
 !>   int32 mode (1 vof, 2 sp), nx, ny, nsteps, counter (vof: final
@@ -25,6 +28,7 @@
 !>   sp : u*, v*, rho_u, rho_v, p, u, v (each (0:nx+1, 0:ny+1))
 !>   csf: C, C_s, C_tilde, curvature, fx_sigma, fy_sigma
 !>   pred: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, u*, v*
+!>   mono: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, u, v, p
 program covof_driver
   use iso_c_binding
   use covof_hip
@@ -44,7 +48,7 @@ program covof_driver
   real(dp), parameter :: pi = 3.14159265358979323846_dp
 
   if (command_argument_count() < 4) then
-     print '(a)', "usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE"
      stop 2
   end if
   call get_command_argument(1, mode)
@@ -160,7 +164,7 @@ program covof_driver
      write (unit) C, Cs, Ct, curv, fx, fy
      close (unit)
      print '(a,es23.16)', "csf max |curvature| ", maxval(abs(curv))
-  case ("pred")
+  case ("pred", "mono")
      bc = COVOF_BC_DIRICHLET
      st = covof_ctx_create(int(n, c_int), int(n, c_int), 1.0_dp, 1.0_dp, bc, ctx)
      call check(st, "covof_ctx_create")
@@ -207,18 +211,29 @@ program covof_driver
      mcfg%convective_scheme = COVOF_CONV_TVD_SUPERBEE
      mcfg%gy = -0.98_dp
      mcfg%bc_u(4) = 1.0_dp
-     st = covof_sp_predictor(ctx, mcfg, u, v, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt, &
-          COVOF_LINEAR_BICGSTAB2, COVOF_PRECOND_JACOBI, 1.0e-10_dp, 300_c_int, iters, relres)
-     call check(st, "covof_sp_predictor")
+     if (trim(mode) == "pred") then
+        st = covof_sp_predictor(ctx, mcfg, u, v, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt, &
+             COVOF_LINEAR_BICGSTAB2, COVOF_PRECOND_JACOBI, 1.0e-10_dp, 300_c_int, iters, relres)
+        call check(st, "covof_sp_predictor")
+     else
+        allocate(p(0:n + 1, 0:n + 1))
+        p = 0.0_dp
+        mcfg%solver_type = COVOF_SOLVER_MONOLITHIC
+        st = covof_monolithic_step(ctx, mcfg, u, v, p, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt, &
+             COVOF_LINEAR_BICGSTAB2, COVOF_PRECOND_BLOCK_TRIANGULAR, 1.0e-10_dp, 600_c_int, iters, relres)
+        call check(st, "covof_monolithic_step")
+     end if
      counter = iters
      open (newunit=unit, file=trim(outfile), access="stream", form="unformatted", status="replace")
-     write (unit) 4_c_int, int(n, c_int), int(n, c_int), int(nsteps, c_int), int(counter, c_int), int(st, c_int)
+     write (unit) merge(4_c_int, 5_c_int, trim(mode) == "pred"), int(n, c_int), int(n, c_int), &
+          int(nsteps, c_int), int(counter, c_int), int(st, c_int)
      write (unit) dt, relres
      write (unit) u0, v0, um, vm, rho_u, rho_v, mu, mun, u, v
+     if (trim(mode) == "mono") write (unit) p
      close (unit)
-     print '(a,i0,a,es10.3)', "predictor iterations ", counter, " relres ", relres
+     print '(a,a,a,i0,a,es10.3)', "solve (", trim(mode), ") iterations ", counter, " relres ", relres
   case default
-     print '(a)', "usage: covof_driver vof|sp|csf|pred N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE"
      stop 2
   end select
   st = covof_ctx_destroy(ctx)

@@ -18,6 +18,9 @@
 !>     -> covof_sp_predictor(ctx, mcfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u,
 !>            rho_v, mu, mu_node, fx_sigma, fy_sigma, dt, linear_solver,
 !>            precond_type, tol, max_iter, iters, relres)
+!>   the monolithic branch of sim_run          src/module_simulation.f90:550-583
+!>     -> covof_monolithic_step(ctx, mcfg, u, v, p, u_old, ..., fy_sigma, dt,
+!>            linear_solver, precond_type, tol, max_iter, iters, relres)
 !>   the tail of advect_vof                   src/module_interface_vof.f90:810-824
 !>     smooth_svof_helmholtz   -> covof_svof_smooth(ctx, C, width, C_s)
 !>     vof_smooth_diffusion    -> covof_smooth_diffusion(ctx, C, steps, C_tilde)
@@ -38,7 +41,9 @@ module covof_hip
   integer(c_int), parameter :: COVOF_BC_DIRICHLET = 1, COVOF_BC_NEUMANN = 2, COVOF_BC_PERIODIC = 3, &
        COVOF_BC_FREESLIP = 4
   integer(c_int), parameter :: COVOF_LINEAR_BICGSTAB2 = 1, COVOF_LINEAR_CG = 3
-  integer(c_int), parameter :: COVOF_PRECOND_NONE = 0, COVOF_PRECOND_JACOBI = 1, COVOF_PRECOND_SSOR = 2
+  integer(c_int), parameter :: COVOF_PRECOND_NONE = 0, COVOF_PRECOND_JACOBI = 1, COVOF_PRECOND_SSOR = 2, &
+       COVOF_PRECOND_BLOCK_TRIANGULAR = 4
+  integer(c_int), parameter :: COVOF_SOLVER_MONOLITHIC = 2, COVOF_SOLVER_STANDARD_PROJECTION = 3
   !> parallel preconditioners of the GPU library (no reference counterpart)
   integer(c_int), parameter :: COVOF_PRECOND_SGS_RB = 16, COVOF_PRECOND_MG = 17
   integer(c_int), parameter :: COVOF_PRECISION_FP64 = 0, COVOF_PRECISION_MIXED = 1
@@ -130,6 +135,21 @@ module covof_hip
        integer(c_int), intent(out) :: iters
        real(c_double), intent(out) :: relres
      end function covof_sp_predictor
+
+     integer(c_int) function covof_monolithic_step(ctx, mcfg, u, v, p, u_old, v_old, u_nm1, v_nm1, rho_u, &
+          rho_v, mu, mu_node, fx_sigma, fy_sigma, dt, lin_solver, precond, tol, maxit, iters, relres) &
+          bind(C, name="covof_monolithic_step")
+       import :: c_int, c_double, c_ptr, covof_momentum_cfg
+       type(c_ptr), value :: ctx
+       type(covof_momentum_cfg), intent(in) :: mcfg
+       real(c_double), intent(inout) :: u(*), v(*), p(*)
+       real(c_double), intent(in) :: u_old(*), v_old(*), u_nm1(*), v_nm1(*), rho_u(*), rho_v(*), mu(*), &
+            mu_node(*), fx_sigma(*), fy_sigma(*)
+       real(c_double), value :: dt, tol
+       integer(c_int), value :: lin_solver, precond, maxit
+       integer(c_int), intent(out) :: iters
+       real(c_double), intent(out) :: relres
+     end function covof_monolithic_step
 
      integer(c_int) function covof_curvature_csf(ctx, G, C, enable, sigma, curvature, fx_sigma, fy_sigma) &
           bind(C, name="covof_curvature_csf")

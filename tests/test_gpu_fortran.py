@@ -127,3 +127,29 @@ def test_fortran_sp_predictor_matches_python_and_oracle(tmp_path, cv, orc):
     uo, vo, _, _ = o.predictor(1, 1, 1e-10, 300)
     scale = max(np.abs(uo).max(), np.abs(vo).max())
     assert np.abs(us - uo).max() < 1e-8 * scale and np.abs(vs - vo).max() < 1e-8 * scale
+
+
+@pytest.mark.gpu
+def test_fortran_monolithic_step_matches_python(tmp_path, cv):
+    """covof_monolithic_step from Fortran (solver_type MONOLITHIC in the
+    bind(C) options, BiCGSTAB(2) + block-triangular Schur-SSOR) gives the
+    Python device path's u, v, p bit for bit."""
+    n = 40
+    out = tmp_path / "mono.bin"
+    r = subprocess.run([DRIVER, "mono", str(n), "1", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    hdr, dt, relres, (u0, v0, um, vm, rho_u, rho_v, mu, mun, us, vs, ps) = _read(out, 11)
+    assert hdr[0] == 5 and hdr[5] == 0
+    z = np.zeros_like(u0)
+    cfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98, bc_u=(0, 0, 0, 1.0),
+                                  solver_type=cv.SOLVER_MONOLITHIC)
+    ctx = cv.Context(n, n, bc=(1, 1, 1, 1))
+    F = {"u": to_gpu(u0), "v": to_gpu(v0), "u_old": to_gpu(u0), "v_old": to_gpu(v0), "u_nm1": to_gpu(um),
+         "v_nm1": to_gpu(vm), "rho_u": to_gpu(rho_u), "rho_v": to_gpu(rho_v), "mu": to_gpu(mu),
+         "mu_node": to_gpu(mun), "fx_sigma": to_gpu(z), "fy_sigma": to_gpu(z)}
+    p = to_gpu(z)
+    res = ctx.solve_monolithic_step(cfg, F, p, dt, 1, 4, 1e-10, 600)
+    assert res.converged and res.iters == hdr[4]
+    for got, ref in ((F["u"], us), (F["v"], vs), (p, ps)):
+        assert np.array_equal(got.cpu().numpy(), ref)
