@@ -671,23 +671,38 @@ __global__ void __launch_bounds__(64 * SEQ_MAXW) k_seq_ssor(Mesh m, SeqSet a, in
 }
 
 // The pressure step of the block-triangular preconditioner after its 12
-// sweeps: xp holds the solution's interior; the mean is removed (serial sum
-// in the reference's order), the halo exchanged, and the pressure part of the
-// result written.
+// sweeps: xp holds the solution's interior; the mean is removed, the halo
+// exchanged, and the pressure part of the result written.  The mean is the
+// reference's serial sum in its order (bitwise needs that order): one thread
+// adds, while the workgroup stages the next 1024 values into LDS.
 __global__ void __launch_bounds__(1024) k_bt_mean(Mesh m, double *__restrict__ xp, double *__restrict__ dst) {
+  __shared__ double buf[2][1024];
   __shared__ double mean;
-  if (threadIdx.x == 0) {
-    double sum = 0.0;
-    for (int j = 1; j <= m.ny; ++j)
-      for (int i = 1; i <= m.nx; ++i) sum = sum + xp[HX(i, j)];
-    mean = sum / (double)((long)m.nx * m.ny);
-  }
-  __syncthreads();
   const long np = (long)m.nx * m.ny;
-  for (long k = threadIdx.x; k < np; k += blockDim.x) {
+  auto cell = [&](long k) {
     const int j = (int)(k / m.nx) + 1, i = (int)(k - (long)(j - 1) * m.nx) + 1;
-    const double v = xp[HX(i, j)] - mean;
-    xp[HX(i, j)] = v;
+    return HX(i, j);
+  };
+  const int t = threadIdx.x;
+  if (t < np) buf[0][t] = xp[cell(t)];
+  __syncthreads();
+  double sum = 0.0;
+  for (long c0 = 0; c0 < np; c0 += 1024) {
+    const long kn = c0 + 1024 + t;
+    if (kn < np) buf[((c0 >> 10) + 1) & 1][t] = xp[cell(kn)];
+    if (t == 0) {
+      const double *b = buf[(c0 >> 10) & 1];
+      const int cnt = np - c0 < 1024 ? (int)(np - c0) : 1024;
+      for (int q = 0; q < cnt; ++q) sum = sum + b[q];
+    }
+    __syncthreads();
+  }
+  if (t == 0) mean = sum / (double)np;
+  __syncthreads();
+  for (long k = t; k < np; k += blockDim.x) {
+    const long h = cell(k);
+    const double v = xp[h] - mean;
+    xp[h] = v;
     dst[k] = v;
   }
   __syncthreads();

@@ -33,9 +33,11 @@ constexpr int SEQ_D = 4;
 constexpr int SEQ_MAXW = 8;
 // waves synchronise once per batch of 2 SEQ_D steps; wave w runs SEQ_LAG
 // steps behind wave w-1, so the value a strip's first row needs from the
-// strip below was made SEQ_LAG - 63 > 2 SEQ_D steps before, in an earlier
-// batch; it passes through a ring of SEQ_RING LDS slots per wave
-constexpr int SEQ_LAG = 64 + 2 * SEQ_D;
+// strip below was made SEQ_LAG - 63 = 17 steps before: before the barrier
+// that precedes even the earliest load of it (loads run up to 3 SEQ_D - 1
+// steps ahead of their step), so it is read from a ring of SEQ_RING LDS
+// slots per wave together with the step's other operands
+constexpr int SEQ_LAG = 64 + 4 * SEQ_D;
 constexpr int SEQ_RING = 32;
 
 struct SeqBlk {
@@ -74,7 +76,7 @@ __host__ __device__ __forceinline__ long sk_of(int S, int i, int j) {
 // the two old neighbours, and the new neighbour across rows where it comes
 // from a halo or an earlier pass.
 struct SeqOps {
-  double P, W, E, S, N, R, XO, O1, O2, B2;
+  double P, W, E, S, N, R, XO, O1, O2, B2, RG;
 };
 
 // One half-sweep of the strips (FWD: forward) from xi into xo.  A lane's
@@ -126,6 +128,7 @@ __device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double 
         q.O1 = q.O2 = 0.0;
       }
       q.B2 = *(act ? (edge_row ? hY + (i - 1) : (prev_pass ? xo + at + offY : xi)) : xi);
+      q.RG = ring[w > 0 ? w - 1 : 0][(g - (SEQ_LAG - 63)) & (SEQ_RING - 1)];  // the strip below's row
     };
     double cur = 0.0;
     auto step = [&](int g, const SeqOps &o) {
@@ -136,7 +139,7 @@ __device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double 
       double nB;                            // new neighbour across rows
       if (edge_row || prev_pass) nB = o.B2;
       else if (lane > 0) nB = up;
-      else nB = ring[w - 1][(g - (SEQ_LAG - 63)) & (SEQ_RING - 1)];
+      else nB = o.RG;
       double nv;
       if (MODE == SEQ_SSOR) {
         const double oA = t == ni - 1 ? eA : o.O1;  // old neighbour along the row
