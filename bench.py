@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a 256 MiB
 # copy) measured with rocprofv3 --pmc on `bench.py --kernel-sample`; the
 # summary with its commands is profiles/poisson_wy_momentum_2048_hbm_traffic.md.
-TRAFFIC_2048 = {"k_pcg_a<float,1>": 1.620e8, "k_pcg_b<float,1>": 6.733e7}
+TRAFFIC_2048 = {"k_pcg_a<float,1>": 1.615e8, "k_pcg_b<float,1>": 6.733e7}
 
 
 def log(*a):
