@@ -306,40 +306,51 @@ def main():
     # the standard-projection time step of a two-phase run, all on the GPU
     # (transport, properties, CSF, velocity BC, predictor, Poisson,
     # correction): 4096 x 2048 on a 2 x 1 box, 256 seeded non-overlapping
-    # bubbles (radii 8-32 cells), rho 1000 / 1, mu 0.1 / 0.001, sigma 0.07,
-    # g 9.81, BDF2, TVD superbee, PLIC + WY; predictor CG + Jacobi to 1e-10;
-    # Poisson: mixed-precision CG with the multigrid preconditioner to 1e-10
+    # bubbles (radii 8-32 cells, exact volume fractions), rho 1000 / 1,
+    # mu 0.1 / 0.001, sigma 0.07, g 9.81, BDF2, TVD superbee, PLIC + WY;
+    # predictor CG + Jacobi to 1e-10; Poisson: mixed-precision CG with the
+    # multigrid preconditioner to 1e-10.  The CSF curvature from heights of C
+    # (the full coupled step's specification), and the reference's -div(n)
+    # beside it.
     sx, sy = 4096, 2048
     t0 = time.time()
-    Cs, nbub = wl.bubbles_case(sx, sy, 256, 8.0, 32.0, seed=0, lx=2.0, ly=1.0)
+    Cs, nbub = wl.bubbles_case(sx, sy, 256, 8.0, 32.0, seed=0, lx=2.0, ly=1.0, exact=True)
     log(f"bubbles case {sx}x{sy} ({nbub} bubbles) built in {time.time() - t0:.1f}s")
     sctx = cv.Context(sx, sy, 2.0, 1.0, bc=(cv.BC_FREESLIP, cv.BC_FREESLIP, cv.BC_DIRICHLET, cv.BC_DIRICHLET))
     zs = torch.zeros((sy + 2, sx + 2), dtype=torch.float64, device=dev)
-    run = cv.StandardProjectionRun(sctx, torch.tensor(Cs, device=dev), zs, zs, 1000.0, 1.0, 0.1, 1.0e-3, 0.07,
-                                   time_scheme=cv.TIME_SCHEME_GEAR, convective_scheme=cv.CONV_TVD_SUPERBEE,
-                                   gy=-9.81, predictor=(cv.LINEAR_CG, cv.PRECOND_JACOBI, 1e-10, 500),
-                                   poisson=(cv.LINEAR_CG, cv.PRECOND_MG, 1e-10, 20000, cv.PRECISION_MIXED))
+    C_dev = torch.tensor(Cs, device=dev)
     del Cs
-    dts = 1.0e-4
-    for _ in range(2):
-        run.step(dts)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    nst = 5
-    sp_its = [run.step(dts) for _ in range(nst)]
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t) / nst
-    sp_step = {
-        "workload": f"two-phase standard-projection step {sx}x{sy} (2 x 1 box), {nbub} seeded bubbles (radii 8-32 "
-                    "cells), rho 1000/1, mu 0.1/0.001, sigma 0.07, g 9.81, BDF2, TVD superbee, PLIC + WY, dt 1e-4; "
-                    "predictor CG + Jacobi to 1e-10; Poisson mixed-precision MG-PCG to 1e-10",
-        "ms_per_step": t_step * 1e3, "cell_steps_per_s": sx * sy / t_step,
-        "predictor_iters": [a[0] for a in sp_its], "poisson_iters": [a[1] for a in sp_its],
-        "status": [[a[2], a[3]] for a in sp_its], "kinetic_energy": run.kinetic_energy(),
-    }
-    log(f"SP step {sx}x{sy}: {t_step * 1e3:.2f} ms/step, predictor {sp_step['predictor_iters']} it, "
-        f"Poisson {sp_step['poisson_iters']} it")
-    del run, sctx, zs
+    steps_out = {}
+    for curv in ("height_function", "normals"):
+        run = cv.StandardProjectionRun(sctx, C_dev, zs, zs, 1000.0, 1.0, 0.1, 1.0e-3, 0.07,
+                                       time_scheme=cv.TIME_SCHEME_GEAR, convective_scheme=cv.CONV_TVD_SUPERBEE,
+                                       gy=-9.81, predictor=(cv.LINEAR_CG, cv.PRECOND_JACOBI, 1e-10, 500),
+                                       poisson=(cv.LINEAR_CG, cv.PRECOND_MG, 1e-10, 20000, cv.PRECISION_MIXED),
+                                       curvature=curv)
+        dts = 1.0e-4
+        for _ in range(2):
+            run.step(dts)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        nst = 5
+        sp_its = [run.step(dts) for _ in range(nst)]
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t) / nst
+        steps_out[curv] = {
+            "workload": f"two-phase standard-projection step {sx}x{sy} (2 x 1 box), {nbub} seeded bubbles (radii "
+                        f"8-32 cells, exact volume fractions), rho 1000/1, mu 0.1/0.001, sigma 0.07 (CSF, curvature "
+                        f"{curv.replace('_', '-')}), g 9.81, BDF2, TVD superbee, PLIC + WY, dt 1e-4; predictor "
+                        "CG + Jacobi to 1e-10; Poisson mixed-precision MG-PCG to 1e-10",
+            "ms_per_step": t_step * 1e3, "cell_steps_per_s": sx * sy / t_step,
+            "predictor_iters": [a[0] for a in sp_its], "poisson_iters": [a[1] for a in sp_its],
+            "status": [[a[2], a[3]] for a in sp_its], "kinetic_energy": run.kinetic_energy(),
+        }
+        log(f"SP step {sx}x{sy} ({curv}): {t_step * 1e3:.2f} ms/step, predictor "
+            f"{steps_out[curv]['predictor_iters']} it, Poisson {steps_out[curv]['poisson_iters']} it")
+        del run
+    sp_step = steps_out["height_function"]
+    sp_step["reference_curvature"] = steps_out["normals"]
+    del sctx, zs, C_dev
 
     # ------------------------------------------ the plain-method case (128 x 256)
     # the reference's own CPU-runnable case: 2-D rising bubble on a 1 x 2 box,

@@ -22,6 +22,7 @@ BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
 LINEAR_BICGSTAB2, LINEAR_CG = 1, 3
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_SSOR, PRECOND_BLOCK_TRIANGULAR = 0, 1, 2, 4
 SOLVER_MONOLITHIC, SOLVER_STANDARD_PROJECTION = 2, 3
+CURVATURE_NORMALS, CURVATURE_HEIGHT_FUNCTION = 0, 1
 # parallel preconditioners of this library (no reference counterpart)
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
@@ -56,6 +57,7 @@ SIGNATURES = {
     "covof_vof_advect": (_I, [_P, _DP, _DP, _DP, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_DP] * 5),
     "covof_update_properties_d": (_I, [_P, _P, _D, _D, _D, _D] + [_P] * 5),
     "covof_curvature_csf_d": (_I, [_P, _P, _P, _I, _D, _P, _P, _P]),
+    "covof_curvature_csf_method_d": (_I, [_P, _P, _P, _I, _D, _I, _P, _P, _P]),
     "covof_curvature_csf": (_I, [_P, _P, _P, _I, _D, _P, _P, _P]),
     "covof_smooth_diffusion_d": (_I, [_P, _P, _I, _P]),
     "covof_smooth_diffusion": (_I, [_P, _P, _I, _P]),

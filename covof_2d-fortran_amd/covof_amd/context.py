@@ -163,12 +163,18 @@ class Context:
         return props
 
     # ------------------------------------------------ after the transport
-    def compute_curvature_and_csf(self, C, sigma, G=None, enable=True, out=None):
+    def compute_curvature_and_csf(self, C, sigma, G=None, enable=True, out=None, method="normals"):
         """compute_curvature_and_csf (src/module_interface_vof.f90:627-789):
         returns {"curvature", "fx_sigma", "fy_sigma"} (halo fields); normals
         from G (C_tilde / C_s when a smoothing is on; default C).  ``out``
         supplies the three arrays (the force arrays keep their entries outside
-        the face ranges); by default they start at zero."""
+        the face ranges); by default they start at zero.  ``method``:
+        "normals" (the reference's -div(n)) or "height_function" (3 x 7
+        column heights of C where they bracket the interface, the reference's
+        value elsewhere; no reference counterpart)."""
+        codes = {"normals": L.CURVATURE_NORMALS, "height_function": L.CURVATURE_HEIGHT_FUNCTION}
+        if method not in codes:
+            raise ValueError(f"method must be one of {sorted(codes)}")
         self._field(C, "C")
         if G is not None:
             self._field(G, "G")
@@ -176,8 +182,9 @@ class Context:
             out = {k: self.new_field(0.0) for k in ("curvature", "fx_sigma", "fy_sigma")}
         arrs = [self._field(out[k], k) for k in ("curvature", "fx_sigma", "fy_sigma")]
         self._stream()
-        L.check(self._lib.covof_curvature_csf_d(self._h, _ptr(G), _ptr(C), 1 if enable else 0, float(sigma),
-                                                *[_ptr(a) for a in arrs]), "covof_curvature_csf_d")
+        L.check(self._lib.covof_curvature_csf_method_d(self._h, _ptr(G), _ptr(C), 1 if enable else 0, float(sigma),
+                                                       codes[method], *[_ptr(a) for a in arrs]),
+                "covof_curvature_csf_method_d")
         return out
 
     def smooth_diffusion(self, C, steps=3, C_tilde=None):

@@ -42,6 +42,8 @@ class StandardProjectionRun:
     preconditioner, tol, max_iter) of the momentum block (config_t's
     linear_solver, precond_type, tol, max_iter); with ``solver_type``
     SOLVER_MONOLITHIC it is the coupled system's and ``poisson`` is unused.
+    ``curvature``: "normals" (the reference's) or "height_function" (see
+    Context.compute_curvature_and_csf).
     """
 
     FIELDS = ("C", "u", "v", "p", "rho", "rho_u", "rho_v", "mu", "mu_node", "curvature", "fx_sigma", "fy_sigma",
@@ -52,13 +54,14 @@ class StandardProjectionRun:
                  bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, reconstruction_method="PLIC",
                  predictor=(L.LINEAR_CG, L.PRECOND_JACOBI, 1e-10, 500),
                  poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True,
-                 solver_type=L.SOLVER_STANDARD_PROJECTION):
+                 solver_type=L.SOLVER_STANDARD_PROJECTION, curvature="normals"):
         self.ctx = ctx
         self.rho1, self.rho2, self.mu1, self.mu2, self.sigma = (float(a) for a in (rho1, rho2, mu1, mu2, sigma))
         self.opts = dict(time_scheme=time_scheme, enable_convection=bool(convection), convective_scheme=convective_scheme,
                          convective_linearization=convective_linearization, gx=gx, gy=gy, bc_u=tuple(bc_u),
                          bc_v=tuple(bc_v), solver_type=int(solver_type))
         self.mono = int(solver_type) == L.SOLVER_MONOLITHIC
+        self.curvature = curvature
         self.recon = reconstruction_method
         self.predictor = predictor
         self.poisson = poisson
@@ -100,7 +103,8 @@ class StandardProjectionRun:
         ctx.advect_vof(f["C"], f["u"], f["v"], dt, reconstruction_method=self.recon, two_phase=True,
                        rho1=self.rho1, rho2=self.rho2, mu1=self.mu1, mu2=self.mu2, props=self.props)
         ctx.compute_curvature_and_csf(f["C"], self.sigma, enable=self.sigma > 0.0,
-                                      out={k: f[k] for k in ("curvature", "fx_sigma", "fy_sigma")})
+                                      out={k: f[k] for k in ("curvature", "fx_sigma", "fy_sigma")},
+                                      method=self.curvature)
         cfg = self.cfg(has_nm1)
         ctx.apply_velocity_bc(cfg, f["u"], f["v"])
         mom = {k: f[k] for k in ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",

@@ -308,11 +308,33 @@ def momentum_case(nx, ny, seed=0, n_circles=16, rmin=4.0, rmax=24.0, rho_liquid=
     return f, dt, bc, lx, ly
 
 
-def bubbles_case(nx, ny, n_bubbles=256, rmin=8.0, rmax=32.0, seed=0, lx=2.0, ly=1.0, sub=4):
+def _disc_quadrant(x, y, R):
+    """Area of {(t, u): t^2 + u^2 <= R^2, t <= x, u <= y} (disc centred at the
+    origin), closed form from the primitive of sqrt(R^2 - t^2); arrays."""
+    def S(t):
+        t = np.clip(t, -R, R)
+        return 0.5 * (t * np.sqrt(np.maximum(R * R - t * t, 0.0)) + R * R * np.arcsin(t / R))
+    X = np.clip(x, -R, R)
+    T = np.sqrt(np.maximum(R * R - y * y, 0.0))
+    mid = y * (np.clip(X, -T, T) + T) + S(np.clip(X, -T, T)) - S(-T)  # |t| < T: chord part below y
+    sides = 2.0 * (S(np.minimum(X, -T)) - S(-R)) + 2.0 * (S(np.maximum(X, T)) - S(T))  # |t| >= T, y >= 0
+    return np.where(y >= R, 2.0 * (S(X) - S(-R)), np.where(y <= -R, 0.0, np.where(y >= 0.0, sides + mid, mid)))
+
+
+def disc_cell_fractions(x0, x1, y0, y1, cx, cy, R):
+    """Exact area fraction of each cell [x0, x1] x [y0, y1] inside the disc of
+    radius R centred at (cx, cy), by inclusion-exclusion of quadrant areas."""
+    a = (_disc_quadrant(x1 - cx, y1 - cy, R) - _disc_quadrant(x0 - cx, y1 - cy, R)
+         - _disc_quadrant(x1 - cx, y0 - cy, R) + _disc_quadrant(x0 - cx, y0 - cy, R))
+    return np.clip(a / ((x1 - x0) * (y1 - y0)), 0.0, 1.0)
+
+
+def bubbles_case(nx, ny, n_bubbles=256, rmin=8.0, rmax=32.0, seed=0, lx=2.0, ly=1.0, sub=4, exact=False):
     """Liquid (C = 1) holding ``n_bubbles`` seeded non-overlapping circular
     bubbles (C = 0), radii uniform in [rmin, rmax] cells, at least two cells
-    apart and from the walls; volume fractions by sub x sub sampling per cell;
-    zero-gradient halo.  Returns the halo array ``C`` of shape (ny+2, nx+2)."""
+    apart and from the walls; volume fractions by sub x sub sampling per cell,
+    or exact disc-cell areas (``exact``); zero-gradient halo.  Returns the halo
+    array ``C`` of shape (ny+2, nx+2)."""
     rng = np.random.default_rng(seed)
     dx, dy = lx / nx, ly / ny
     h = min(dx, dy)
@@ -330,10 +352,17 @@ def bubbles_case(nx, ny, n_bubbles=256, rmin=8.0, rmax=32.0, seed=0, lx=2.0, ly=
     for cx, cy, r in placed:
         i0, i1 = max(1, int((cx - r) / dx)), min(nx, int((cx + r) / dx) + 2)
         j0, j1 = max(1, int((cy - r) / dy)), min(ny, int((cy + r) / dy) + 2)
-        xs = (np.arange(i0, i1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dx
-        ys = (np.arange(j0, j1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dy
-        inside = ((xs[None, :] - cx) ** 2 + (ys[:, None] - cy) ** 2 <= r * r).astype(np.float64)
-        gas = inside.reshape(j1 - j0 + 1, sub, i1 - i0 + 1, sub).mean(axis=(1, 3))
+        if exact:
+            xi = (np.arange(i0, i1 + 1) - 1) * dx
+            yj = (np.arange(j0, j1 + 1) - 1) * dy
+            X0, Y0 = np.meshgrid(xi, yj)
+            gas = disc_cell_fractions(X0, X0 + dx, Y0, Y0 + dy, cx, cy, r)
+            gas = np.where(gas < 1e-12, 0.0, np.where(gas > 1.0 - 1e-12, 1.0, gas))  # round-off of the closed form
+        else:
+            xs = (np.arange(i0, i1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dx
+            ys = (np.arange(j0, j1 + 1)[:, None] - 1 + off[None, :]).reshape(-1) * dy
+            inside = ((xs[None, :] - cx) ** 2 + (ys[:, None] - cy) ** 2 <= r * r).astype(np.float64)
+            gas = inside.reshape(j1 - j0 + 1, sub, i1 - i0 + 1, sub).mean(axis=(1, 3))
         C[j0:j1 + 1, i0:i1 + 1] -= gas
     fill_halo_zero_gradient(C, nx, ny)
     return C, len(placed)

@@ -246,9 +246,12 @@ extern "C" int covof_update_properties_d(covof_ctx *h, double *C, double rho1, d
 
 static bool same_halo_array(const double *a, const double *b) { return a && b && a == b; }
 
-extern "C" int covof_curvature_csf_d(covof_ctx *h, const double *G, const double *C, int enable, double sigma,
-                                     double *curvature, double *fx_sigma, double *fy_sigma) {
+extern "C" int covof_curvature_csf_method_d(covof_ctx *h, const double *G, const double *C, int enable,
+                                            double sigma, int method, double *curvature, double *fx_sigma,
+                                            double *fy_sigma) {
   if (!h || !C || !curvature || !fx_sigma || !fy_sigma) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (method != COVOF_CURVATURE_NORMALS && method != COVOF_CURVATURE_HEIGHT_FUNCTION)
+    return fail(COVOF_BAD_ARGUMENT, "covof_curvature_csf: unknown curvature method");
   if (!G) G = C;
   const double *in[2] = {G, C};
   const double *out[3] = {curvature, fx_sigma, fy_sigma};
@@ -257,7 +260,13 @@ extern "C" int covof_curvature_csf_d(covof_ctx *h, const double *G, const double
       if (same_halo_array(o, q)) return fail(COVOF_BAD_ARGUMENT, "covof_curvature_csf: an output aliases an input");
   if (curvature == fx_sigma || curvature == fy_sigma || fx_sigma == fy_sigma)
     return fail(COVOF_BAD_ARGUMENT, "covof_curvature_csf: outputs must be distinct");
-  return hip_status(covof::curvature_csf(h->c, G, C, enable, sigma, curvature, fx_sigma, fy_sigma));
+  return hip_status(covof::curvature_csf(h->c, G, C, enable, sigma, curvature, fx_sigma, fy_sigma, method));
+}
+
+extern "C" int covof_curvature_csf_d(covof_ctx *h, const double *G, const double *C, int enable, double sigma,
+                                     double *curvature, double *fx_sigma, double *fy_sigma) {
+  return covof_curvature_csf_method_d(h, G, C, enable, sigma, COVOF_CURVATURE_NORMALS, curvature, fx_sigma,
+                                      fy_sigma);
 }
 
 extern "C" int covof_smooth_diffusion_d(covof_ctx *h, const double *C, int steps, double *C_tilde) {

@@ -207,7 +207,7 @@ void vof_free(Ctx &c);
 // smoothings of advect_vof; device pointers, halo layout, no aliasing of an
 // output with an input
 int curvature_csf(Ctx &c, const double *G, const double *C, int enable, double sigma, double *curv,
-                  double *fx, double *fy);
+                  double *fx, double *fy, int method = COVOF_CURVATURE_NORMALS);
 int smooth_diffusion(Ctx &c, const double *C, int steps, double *Ct);
 int svof_smooth(Ctx &c, const double *C, double width, double *Cs);
 void surface_free(Ctx &c);

@@ -94,16 +94,99 @@ __global__ void __launch_bounds__(SBX *SBY) k_normals(const double *__restrict__
   exchanged_halo(nys, m, i, j, uy);
 }
 
+// Height-function curvature (no reference counterpart; Cummins, Francois &
+// Kothe, Computers & Structures 83 (2005) 425-434): C at (ii, jj) for the
+// 3 x 7 stencils, the periodic image or else the nearest interior cell.
+__device__ __forceinline__ double hf_c(const double *C, const SMesh &m, int ii, int jj) {
+  if (ii < 1) ii = m.perx ? ii + m.nx : 1;
+  else if (ii > m.nx) ii = m.perx ? ii - m.nx : m.nx;
+  if (jj < 1) jj = m.pery ? jj + m.ny : 1;
+  else if (jj > m.ny) jj = m.pery ? jj - m.ny : m.ny;
+  return C[HS(m, ii, jj)];
+}
+
+// Curvature at (i, j) from three 7-cell column sums of C along the dominant
+// direction of the unit normal (nxv, nyv) (towards increasing C):
+// kappa = -h'' / (1 + h'^2)^(3/2), the reference's sign (+1/R for a liquid
+// drop).  False when a column does not run from empty (C < 1e-6) to full
+// (C > 1 - 1e-6) along the normal.
+__device__ bool hf_kappa(const double *C, const SMesh &m, int i, int j, double nxv, double nyv, double &kap) {
+  const bool vert = fabs(nyv) >= fabs(nxv);
+  const double sgn = vert ? nyv : nxv;
+  double h[3];
+#pragma unroll
+  for (int q = -1; q <= 1; ++q) {
+    double sum = 0.0;
+#pragma unroll
+    for (int r = -3; r <= 3; ++r) sum = sum + (vert ? hf_c(C, m, i + q, j + r) : hf_c(C, m, i + r, j + q));
+    const double lo = vert ? hf_c(C, m, i + q, j - 3) : hf_c(C, m, i - 3, j + q);
+    const double hi = vert ? hf_c(C, m, i + q, j + 3) : hf_c(C, m, i + 3, j + q);
+    if (sgn > 0.0 ? (lo > 1.0e-6 || hi < 1.0 - 1.0e-6) : (lo < 1.0 - 1.0e-6 || hi > 1.0e-6)) return false;
+    h[q + 1] = sum;
+  }
+  const double a = vert ? m.dy : m.dx, b = vert ? m.dx : m.dy;  // along the heights / across
+  const double hp = (h[2] - h[0]) * a / (2.0 * b);
+  const double hpp = (h[2] - 2.0 * h[1] + h[0]) * a / (b * b);
+  const double den = 1.0 + hp * hp;
+  kap = -hpp / (den * sqrt(den));
+  return true;
+}
+
 // 2. kappa = -div(n) where n is non-zero (:703-714), written to the curvature
-//    output with its exchanged halo (:716, :728).
+//    output with its exchanged halo (:716, :728).  With hf (C given), the
+//    height-function curvature where the heights bracket the interface.
 __global__ void __launch_bounds__(SBX *SBY) k_curvature(const double *__restrict__ nxs,
-                                                       const double *__restrict__ nys, SMesh m, double *kap) {
+                                                       const double *__restrict__ nys, SMesh m, double *kap,
+                                                       const double *__restrict__ hfC, double *__restrict__ hfok) {
   const int i = blockIdx.x * SBX + threadIdx.x + 1, j = blockIdx.y * SBY + threadIdx.y + 1;
   if (i > m.nx || j > m.ny) return;
-  double k = 0.0;
-  if (nxs[HS(m, i, j)] != 0.0 || nys[HS(m, i, j)] != 0.0)
+  double k = 0.0, ok = 0.0;
+  const double nxv = nxs[HS(m, i, j)], nyv = nys[HS(m, i, j)];
+  if (nxv != 0.0 || nyv != 0.0) {
     k = -((nxs[HS(m, i + 1, j)] - nxs[HS(m, i - 1, j)]) / (2.0 * m.dx) +
           (nys[HS(m, i, j + 1)] - nys[HS(m, i, j - 1)]) / (2.0 * m.dy));
+    double khf;
+    if (hfC && hf_kappa(hfC, m, i, j, nxv, nyv, khf)) {
+      k = khf;
+      ok = 1.0;
+    }
+  }
+  kap[HS(m, i, j)] = k;
+  if (hfok) hfok[HS(m, i, j)] = ok;
+  exchanged_halo(kap, m, i, j, k);
+}
+
+// 2b. Height-function cells whose columns do not bracket the interface take
+//     the mean of their valid neighbours' height-function curvatures (rows
+//     j-1..j+1, i fastest; periodic images, no cells beyond a physical side),
+//     else keep -div(n).  Only such cells are written and only valid cells
+//     are read, so the pass runs in place.
+__global__ void __launch_bounds__(SBX *SBY) k_hf_fill(const double *__restrict__ nxs,
+                                                     const double *__restrict__ nys, SMesh m,
+                                                     const double *__restrict__ hfok, double *kap) {
+  const int i = blockIdx.x * SBX + threadIdx.x + 1, j = blockIdx.y * SBY + threadIdx.y + 1;
+  if (i > m.nx || j > m.ny) return;
+  if (hfok[HS(m, i, j)] != 0.0 || (nxs[HS(m, i, j)] == 0.0 && nys[HS(m, i, j)] == 0.0)) return;
+  double sum = 0.0;
+  int cnt = 0;
+  for (int b = -1; b <= 1; ++b)
+    for (int a = -1; a <= 1; ++a) {
+      int ii = i + a, jj = j + b;
+      if (ii < 1 || ii > m.nx) {
+        if (!m.perx) continue;
+        ii = ii < 1 ? ii + m.nx : ii - m.nx;
+      }
+      if (jj < 1 || jj > m.ny) {
+        if (!m.pery) continue;
+        jj = jj < 1 ? jj + m.ny : jj - m.ny;
+      }
+      if (hfok[HS(m, ii, jj)] != 0.0) {
+        sum = sum + kap[HS(m, ii, jj)];
+        ++cnt;
+      }
+    }
+  if (cnt == 0) return;
+  const double k = sum / (double)cnt;
   kap[HS(m, i, j)] = k;
   exchanged_halo(kap, m, i, j, k);
 }
@@ -282,10 +365,11 @@ __global__ void __launch_bounds__(1024) k_svof_wavefront(const double *__restric
 
 static long halo_size(const Mesh &m) { return (long)(m.nx + 2) * (m.ny + 2); }
 
-// Two halo-size work arrays (the normals; the diffusion ping-pong), made on
-// first use and kept with the context.
+// Three halo-size work arrays (the normals and the height-function flags;
+// the diffusion ping-pong), made on first use and kept with the context.
 static int surface_work(Ctx &c, double **w0, double **w1) {
-  if (!c.surf.work) COVOF_CHECK(hipMalloc(&c.surf.work, 2 * halo_size(c.m) * sizeof(double)));
+  // the third array: the height-function validity flags of curvature_csf
+  if (!c.surf.work) COVOF_CHECK(hipMalloc(&c.surf.work, 3 * halo_size(c.m) * sizeof(double)));
   *w0 = c.surf.work;
   *w1 = c.surf.work + halo_size(c.m);
   return COVOF_OK;
@@ -297,7 +381,7 @@ void surface_free(Ctx &c) {
 }
 
 int curvature_csf(Ctx &c, const double *G, const double *C, int enable, double sigma, double *curv,
-                  double *fx, double *fy) {
+                  double *fx, double *fy, int method) {
   const SMesh m = smesh(c.m);
   if (!enable || !(sigma > 0.0)) {
     hipLaunchKernelGGL(k_zero3, dim3(512), dim3(256), 0, c.stream, halo_size(c.m), curv, fx, fy);
@@ -311,7 +395,13 @@ int curvature_csf(Ctx &c, const double *G, const double *C, int enable, double s
   const dim3 g_in((m.nx + SBX - 1) / SBX, (m.ny + SBY - 1) / SBY);
   const dim3 g_face((m.nx + 1 + SBX - 1) / SBX, (m.ny + 1 + SBY - 1) / SBY);
   hipLaunchKernelGGL(k_normals, g_in, blk, 0, c.stream, G, m, nxs, nys);
-  hipLaunchKernelGGL(k_curvature, g_in, blk, 0, c.stream, (const double *)nxs, (const double *)nys, m, curv);
+  const bool hf = method == COVOF_CURVATURE_HEIGHT_FUNCTION;
+  double *hfok = hf ? c.surf.work + 2 * halo_size(c.m) : nullptr;
+  hipLaunchKernelGGL(k_curvature, g_in, blk, 0, c.stream, (const double *)nxs, (const double *)nys, m, curv,
+                     hf ? C : (const double *)nullptr, hfok);
+  if (hf)
+    hipLaunchKernelGGL(k_hf_fill, g_in, blk, 0, c.stream, (const double *)nxs, (const double *)nys, m,
+                       (const double *)hfok, curv);
   hipLaunchKernelGGL(k_csf, g_face, blk, 0, c.stream, (const double *)curv, C, m, sigma, fx, fy);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;

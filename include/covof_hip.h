@@ -51,6 +51,9 @@ extern "C" {
 #define COVOF_PRECOND_SSOR 2 /* symmetric Gauss-Seidel, lexicographic (reference SP) */
 #define COVOF_PRECOND_BLOCK_TRIANGULAR 4 /* Schur-SSOR then velocity SSOR (monolithic) */
 #define COVOF_SOLVER_MONOLITHIC 2
+/* Curvature of the CSF forces (covof_curvature_csf_method_d). */
+#define COVOF_CURVATURE_NORMALS 0
+#define COVOF_CURVATURE_HEIGHT_FUNCTION 1
 #define COVOF_SOLVER_STANDARD_PROJECTION 3
 /* Parallel preconditioners with no reference counterpart (values clear of
  * the reference's 0-5; they take the place PRECOND_CUSTOM = 5 gives a user
@@ -134,6 +137,16 @@ int covof_update_properties_d(covof_ctx *ctx, double *C, double rho1, double rho
  * (cfg%enable_surface_tension false) or sigma <= 0 zeroes all three arrays. */
 int covof_curvature_csf_d(covof_ctx *ctx, const double *G, const double *C, int enable, double sigma,
                           double *curvature, double *fx_sigma, double *fy_sigma);
+/* The same with a curvature method: COVOF_CURVATURE_NORMALS (the reference's
+ * -div(n), as covof_curvature_csf_d) or COVOF_CURVATURE_HEIGHT_FUNCTION:
+ * curvature from 3 x 7 column heights of C (Cummins, Francois & Kothe,
+ * Computers & Structures 83 (2005) 425-434) wherever the columns run from
+ * empty to full along the normal, the reference's value elsewhere.  The
+ * height-function option has no counterpart in the reference (a new feature
+ * for the full coupled step's CSF forces); the force assembly is the
+ * reference's. */
+int covof_curvature_csf_method_d(covof_ctx *ctx, const double *G, const double *C, int enable, double sigma,
+                                 int method, double *curvature, double *fx_sigma, double *fy_sigma);
 int covof_curvature_csf(covof_ctx *ctx, const double *G, const double *C, int enable, double sigma,
                         double *curvature, double *fx_sigma, double *fy_sigma);
 /* vof_smooth_diffusion (:1416-1461): steps explicit diffusion steps of C

@@ -552,9 +552,49 @@ void orc_smooth_diffusion(const orc_mesh *m, const double *C, int steps, double 
   free(cn);
 }
 
-/* src/module_interface_vof.f90:627-789 (no markers) */
-void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
-                       double *curv, double *fx, double *fy) {
+/* ---- height-function curvature (no reference counterpart: the method of
+ * Cummins, Francois & Kothe, Computers & Structures 83 (2005) 425-434, 3x7
+ * column heights from the volume fractions).  C at (ii, jj) for the stencils:
+ * the periodic image, else the nearest interior cell. */
+static double hf_c(const orc_mesh *m, const double *C, int ii, int jj) {
+  const int nx = m->nx, ny = m->ny;
+  if (ii < 1) ii = per_x(m) ? ii + nx : 1;
+  else if (ii > nx) ii = per_x(m) ? ii - nx : nx;
+  if (jj < 1) jj = per_y(m) ? jj + ny : 1;
+  else if (jj > ny) jj = per_y(m) ? jj - ny : ny;
+  return C[H(m, ii, jj)];
+}
+
+/* Curvature at (i, j) from the heights along the dominant normal direction
+ * (nv: the unit normal, towards increasing C): three 7-cell column sums,
+ * kappa = -h'' / (1 + h'^2)^(3/2) (with the reference's sign: +1/R for a
+ * liquid drop).  Returns 0 when a column does not run from empty (C < 1e-6)
+ * to full (C > 1 - 1e-6) along the normal. */
+int orc_hf_kappa(const orc_mesh *m, const double *C, int i, int j, double nxv, double nyv, double *kap) {
+  const int vert = fabs(nyv) >= fabs(nxv);
+  const double sgn = vert ? nyv : nxv;
+  double h[3];
+  for (int q = -1; q <= 1; ++q) {
+    double sum = 0.0;
+    for (int r = -3; r <= 3; ++r) sum = sum + (vert ? hf_c(m, C, i + q, j + r) : hf_c(m, C, i + r, j + q));
+    const double lo = vert ? hf_c(m, C, i + q, j - 3) : hf_c(m, C, i - 3, j + q);
+    const double hi = vert ? hf_c(m, C, i + q, j + 3) : hf_c(m, C, i + 3, j + q);
+    if (sgn > 0.0 ? (lo > 1.0e-6 || hi < 1.0 - 1.0e-6) : (lo < 1.0 - 1.0e-6 || hi > 1.0e-6)) return 0;
+    h[q + 1] = sum;
+  }
+  const double a = vert ? m->dy : m->dx, b = vert ? m->dx : m->dy;  /* along the heights / across */
+  const double hp = (h[2] - h[0]) * a / (2.0 * b);
+  const double hpp = (h[2] - 2.0 * h[1] + h[0]) * a / (b * b);
+  const double den = 1.0 + hp * hp;
+  *kap = -hpp / (den * sqrt(den));
+  return 1;
+}
+
+/* src/module_interface_vof.f90:627-789 (no markers); method 1: the curvature
+ * from heights of C where they bracket the interface (orc_hf_kappa), the
+ * reference's -div(n) elsewhere. */
+void orc_curvature_csf2(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
+                        int method, double *curv, double *fx, double *fy) {
   const int nx = m->nx, ny = m->ny;
   const long n = (long)(nx + 2) * (ny + 2);
   const double dx = m->dx, dy = m->dy;
@@ -575,11 +615,53 @@ void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int 
     }
   orc_exchange_halos_scalar(m, nxs);
   orc_exchange_halos_scalar(m, nys);
+  unsigned char *ok = calloc(n, 1);
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= nx; ++i)
-      if (nxs[H(m, i, j)] != 0.0 || nys[H(m, i, j)] != 0.0)
-        kap[H(m, i, j)] = -((nxs[H(m, i + 1, j)] - nxs[H(m, i - 1, j)]) / (2.0 * dx) +
-                            (nys[H(m, i, j + 1)] - nys[H(m, i, j - 1)]) / (2.0 * dy));
+      if (nxs[H(m, i, j)] != 0.0 || nys[H(m, i, j)] != 0.0) {
+        double k = -((nxs[H(m, i + 1, j)] - nxs[H(m, i - 1, j)]) / (2.0 * dx) +
+                     (nys[H(m, i, j + 1)] - nys[H(m, i, j - 1)]) / (2.0 * dy));
+        double khf;
+        if (method == 1 && orc_hf_kappa(m, C, i, j, nxs[H(m, i, j)], nys[H(m, i, j)], &khf)) {
+          k = khf;
+          ok[H(m, i, j)] = 1;
+        }
+        kap[H(m, i, j)] = k;
+      }
+  if (method == 1) {
+    /* a cell whose heights do not bracket the interface takes the mean of
+     * its valid neighbours' height-function curvatures (rows j-1..j+1, i
+     * fastest; periodic images, no cells beyond a physical side), else keeps
+     * the -div(n) value */
+    double *kf = malloc(n * sizeof(double));
+    memcpy(kf, kap, n * sizeof(double));
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        if (ok[H(m, i, j)] || (nxs[H(m, i, j)] == 0.0 && nys[H(m, i, j)] == 0.0)) continue;
+        double sum = 0.0;
+        int cnt = 0;
+        for (int b = -1; b <= 1; ++b)
+          for (int a = -1; a <= 1; ++a) {
+            int ii = i + a, jj = j + b;
+            if (ii < 1 || ii > nx) {
+              if (!per_x(m)) continue;
+              ii = ii < 1 ? ii + nx : ii - nx;
+            }
+            if (jj < 1 || jj > ny) {
+              if (!per_y(m)) continue;
+              jj = jj < 1 ? jj + ny : jj - ny;
+            }
+            if (ok[H(m, ii, jj)]) {
+              sum = sum + kap[H(m, ii, jj)];
+              ++cnt;
+            }
+          }
+        if (cnt > 0) kf[H(m, i, j)] = sum / (double)cnt;
+      }
+    memcpy(kap, kf, n * sizeof(double));
+    free(kf);
+  }
+  free(ok);
   orc_exchange_halos_scalar(m, kap);
   memcpy(curv, kap, n * sizeof(double));
   for (int j = 1; j <= ny; ++j)
@@ -597,6 +679,11 @@ void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int 
   free(nxs);
   free(nys);
   free(kap);
+}
+
+void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
+                       double *curv, double *fx, double *fy) {
+  orc_curvature_csf2(m, G, C, enable, sigma, 0, curv, fx, fy);
 }
 
 /* ---------------------------------------------------------------- Poisson */

@@ -56,6 +56,11 @@ void orc_update_properties(const orc_mesh *m, double *C, double rho1, double rho
  * curv, fx, fy.  All arrays are halo fields. */
 void orc_svof_smooth(const orc_mesh *m, const double *C, double width, double *Cs);
 void orc_smooth_diffusion(const orc_mesh *m, const double *C, int steps, double *Ct);
+/* method 0: the reference's curvature; 1: height-function curvature where the
+ * heights bracket the interface (no reference counterpart) */
+void orc_curvature_csf2(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
+                        int method, double *curv, double *fx, double *fy);
+int orc_hf_kappa(const orc_mesh *m, const double *C, int i, int j, double nxv, double nyv, double *kap);
 void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int enable, double sigma,
                        double *curv, double *fx, double *fy);
 

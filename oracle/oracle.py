@@ -74,6 +74,7 @@ def lib():
             "orc_svof_smooth": (None, [M, _D, d, _D]),
             "orc_smooth_diffusion": (None, [M, _D, i, _D]),
             "orc_curvature_csf": (None, [M, _D, _D, i, d, _D, _D, _D]),
+            "orc_curvature_csf2": (None, [M, _D, _D, i, d, i, _D, _D, _D]),
             "orc_poisson_assemble": (None, [M, _D, _D, _D, _D, d] + [_D] * 6),
             "orc_poisson_matvec": (None, [M] + [_D] * 7),
             "orc_poisson_sgs": (None, [M] + [_D] * 7),
@@ -165,16 +166,20 @@ def smooth_diffusion(m: Mesh, C: np.ndarray, steps: int) -> np.ndarray:
     return Ct
 
 
-def curvature_csf(m: Mesh, G: np.ndarray, C: np.ndarray, sigma: float, enable: int = 1, fx0=None, fy0=None):
+def curvature_csf(m: Mesh, G: np.ndarray, C: np.ndarray, sigma: float, enable: int = 1, fx0=None, fy0=None,
+                  method: int = 0):
     """compute_curvature_and_csf: (curvature, fx_sigma, fy_sigma); normals from
     G, force gradient from C.  fx0 / fy0: the force arrays' previous contents
-    (the entries outside the face ranges keep them; zeros by default)."""
+    (the entries outside the face ranges keep them; zeros by default).
+    method 1: height-function curvature where the heights of C bracket the
+    interface (no reference counterpart)."""
     shp = (m.ny + 2, m.nx + 2)
     curv = np.zeros(shp)
     fx = np.zeros(shp) if fx0 is None else np.array(fx0, dtype=np.float64, copy=True)
     fy = np.zeros(shp) if fy0 is None else np.array(fy0, dtype=np.float64, copy=True)
-    lib().orc_curvature_csf(ct.byref(m), _p(np.ascontiguousarray(G, dtype=np.float64)),
-                            _p(np.ascontiguousarray(C, dtype=np.float64)), enable, sigma, _p(curv), _p(fx), _p(fy))
+    lib().orc_curvature_csf2(ct.byref(m), _p(np.ascontiguousarray(G, dtype=np.float64)),
+                             _p(np.ascontiguousarray(C, dtype=np.float64)), enable, ct.c_double(sigma), int(method),
+                             _p(curv), _p(fx), _p(fy))
     return curv, fx, fy
 
 

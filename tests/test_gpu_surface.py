@@ -84,6 +84,32 @@ def test_surface_kernels_vs_oracle(cv, orc, bc):
     assert all(float(t.abs().max()) == 0.0 for t in out.values())
 
 
+@pytest.mark.parametrize("bc", [(4, 4, 4, 4), (3, 3, 3, 3), (3, 3, 1, 1), (2, 2, 3, 3)],
+                         ids=["walls", "periodic", "periodic-x", "periodic-y"])
+def test_height_function_curvature_vs_oracle(cv, orc, bc):
+    """The height-function curvature option (no reference counterpart; its
+    accuracy is pinned on the CPU: zero on lines, 1/R on circles) and the CSF
+    forces from it, bitwise against the oracle for every halo combination,
+    with the normals from C and from the diffusion-smoothed field; most
+    interface cells take the height-function value."""
+    nx, ny = 150, 98
+    m = orc.mesh(nx, ny, 1.3, 0.85, bc)
+    ctx = cv.Context(nx, ny, 1.3, 0.85, bc=bc)
+    C = _field(nx, ny, 5)
+    Cg = to_gpu(C)
+    Ct = orc.smooth_diffusion(m, C, 3)
+    for G in (C, Ct):
+        out = ctx.compute_curvature_and_csf(Cg, 0.0728, G=to_gpu(G), method="height_function")
+        curv, fx, fy = orc.curvature_csf(m, G, C, 0.0728, 1, method=1)
+        for k, ref in (("curvature", curv), ("fx_sigma", fx), ("fy_sigma", fy)):
+            got = out[k].cpu().numpy()
+            assert np.array_equal(got, ref), f"{k}: max diff {np.abs(got - ref).max()}"
+        k0, _, _ = orc.curvature_csf(m, G, C, 0.0728, 1, method=0)
+        band = (C[1:-1, 1:-1] > 1e-6) & (C[1:-1, 1:-1] < 1 - 1e-6)
+        changed = (curv[1:-1, 1:-1] != k0[1:-1, 1:-1]) & band
+        assert changed.sum() > 0.8 * band.sum(), (changed.sum(), band.sum())
+
+
 def test_surface_rejects_aliasing(cv):
     ctx = cv.Context(16, 16)
     C = ctx.new_field(0.5)
