@@ -106,3 +106,17 @@ def test_exchanging_x_and_y_transposes_the_curvature(orc):
     k = _kappa(orc, C, n, 1)
     kt = _kappa(orc, np.ascontiguousarray(C.T), n, 1)
     assert np.allclose(kt, k.T, rtol=0, atol=1e-12 * np.abs(k).max())
+
+
+def test_exact_disc_fractions_of_the_workloads():
+    """workloads.disc_cell_fractions (closed-form quadrant areas): equal to the
+    adaptive quadrature of the clipped chord on every cell, and the fractions
+    of a disc sum to pi R^2."""
+    from covof_amd import workloads as wl
+    n, R, xc, yc = 40, 9.3 / 40, 0.52, 0.47
+    h = 1.0 / n
+    X0, Y0 = np.meshgrid(np.arange(n) * h, np.arange(n) * h)
+    Ce = wl.disc_cell_fractions(X0, X0 + h, Y0, Y0 + h, xc, yc, R)
+    Cq = _disc_fractions(n, xc, yc, R)[1:-1, 1:-1]
+    assert np.abs(Ce - Cq).max() < 1e-12
+    assert abs(Ce.sum() * h * h - np.pi * R * R) < 1e-13
