@@ -34,6 +34,8 @@ struct Vof {
 // halo-size FP64 arrays.
 struct Surface {
   double *work = nullptr;
+  // the SVOF smoothing on the strip sweeps: records, two state buffers, halo rows
+  double *sv_rec = nullptr, *sv_xa = nullptr, *sv_xb = nullptr, *sv_h = nullptr;
 };
 
 // Options of the SP momentum block (momentum.hip), the covof_momentum_cfg of

@@ -28,7 +28,10 @@ namespace {
 // The block's arrays are kept in a skewed order -- strip kk, step s = i-1+l,
 // lane l -- so that the 64 lanes of a step read 64 consecutive doubles: all
 // loads of a step are coalesced, and they are issued SEQ_D steps ahead.
-enum { SEQ_SSOR = 0, SEQ_POISSON_SGS = 1 };  // arithmetic of a cell
+// arithmetic of a cell: the SSOR block solves, the Poisson SGS, the SVOF
+// Helmholtz smoothing (smooth_svof_helmholtz, src/module_interface_vof.f90:
+// 387-476)
+enum { SEQ_SSOR = 0, SEQ_POISSON_SGS = 1, SEQ_SVOF = 2 };
 constexpr int SEQ_D = 4;
 constexpr int SEQ_MAXW = 8;
 // waves synchronise once per batch of 2 SEQ_D steps; wave w runs SEQ_LAG
@@ -121,7 +124,7 @@ __device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double 
       const double2 r0 = r[0], r1 = r[1], r2 = r[2];
       q.P = r0.x, q.W = r0.y, q.E = r1.x, q.S = r1.y, q.N = r2.x, q.R = r2.y;
       q.XO = xi[at];
-      if (MODE == SEQ_SSOR) {
+      if (MODE != SEQ_POISSON_SGS) {
         q.O1 = xi[(act && t < ni - 1) ? at + dstep : 0];
         q.O2 = *(act ? (cross_in ? xi + at + offX : hX + (i - 1)) : xi);
       } else {
@@ -153,6 +156,12 @@ __device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double 
         if (o.S != 0.0) val = val + o.S * xS;
         if (o.N != 0.0) val = val + o.N * xN;
         nv = (1.0 - omega) * o.XO + omega * (val / dg);
+      } else if (MODE == SEQ_SVOF) {  // (C + sum of the neighbours) / a_P, terms E, W, N, S
+        const double oA = t == ni - 1 ? eA : o.O1;
+        const double xW = FWD ? nA : oA, xE = FWD ? oA : nA;
+        const double xS = FWD ? nB : o.O2, xN = FWD ? o.O2 : nB;
+        const double sum = o.E * xE + o.W * xW + o.N * xN + o.S * xS;
+        nv = (o.R + sum) / o.P;
       } else if (FWD) {  // Poisson SGS forward: (D + L) z = r, the wrap terms where present
         double val = o.R;
         if (o.W != 0.0) val = val + o.W * nA;
@@ -191,14 +200,15 @@ __device__ void seq_half(const SeqBlk &b, const double *__restrict__ xi, double 
 // halo snapshot of the state x (skewed) as the single-rank exchange sets it:
 // x(0,j) = x(ni,j), x(ni+1,j) = x(1,j) with periodic x, x(i,0) = x(i,nj),
 // x(i,nj+1) = x(i,1) with periodic y; zero on other sides
-__device__ void seq_snapshot(bool perx, bool pery, const SeqBlk &b, const double *x) {
+// (neumann: a physical side copies the adjacent interior value instead)
+__device__ void seq_snapshot(bool perx, bool pery, const SeqBlk &b, const double *x, bool neumann = false) {
   for (int t = threadIdx.x; t < b.nj; t += blockDim.x) {
-    b.hW[t] = perx ? x[sk_of(b.S, b.ni, t + 1)] : 0.0;
-    b.hE[t] = perx ? x[sk_of(b.S, 1, t + 1)] : 0.0;
+    b.hW[t] = perx ? x[sk_of(b.S, b.ni, t + 1)] : (neumann ? x[sk_of(b.S, 1, t + 1)] : 0.0);
+    b.hE[t] = perx ? x[sk_of(b.S, 1, t + 1)] : (neumann ? x[sk_of(b.S, b.ni, t + 1)] : 0.0);
   }
   for (int t = threadIdx.x; t < b.ni; t += blockDim.x) {
-    b.hS[t] = pery ? x[sk_of(b.S, t + 1, b.nj)] : 0.0;
-    b.hN[t] = pery ? x[sk_of(b.S, t + 1, 1)] : 0.0;
+    b.hS[t] = pery ? x[sk_of(b.S, t + 1, b.nj)] : (neumann ? x[sk_of(b.S, t + 1, 1)] : 0.0);
+    b.hN[t] = pery ? x[sk_of(b.S, t + 1, 1)] : (neumann ? x[sk_of(b.S, t + 1, b.nj)] : 0.0);
   }
   __syncthreads();
 }
@@ -206,21 +216,26 @@ __device__ void seq_snapshot(bool perx, bool pery, const SeqBlk &b, const double
 // n_sweeps symmetric sweeps from zero (SSOR: relaxation omega, ssor_solve_*;
 // the Poisson mode: one symmetric Gauss-Seidel sweep): with a periodic side
 // the halo is exchanged before every sweep but the first and between the
-// halves.  The result ends in b.xa.  One workgroup of up to SEQ_MAXW waves.
+// halves.  The SVOF mode starts from the caller's b.xa and fills the halo
+// (zero-gradient copies, then the exchange) before every sweep only.  The
+// result ends in b.xa.  One workgroup of up to SEQ_MAXW waves.
 template <int MODE>
 __device__ void seq_sweeps(bool perx, bool pery, const SeqBlk &b, int n_sweeps, double omega) {
   __shared__ double ring[SEQ_MAXW][SEQ_RING];
   const bool sync = perx || pery;
   const int W = min((int)(blockDim.x >> 6), b.K);
   const long nsk = (long)b.K * b.S * 64;
-  for (long k = threadIdx.x; k < nsk; k += blockDim.x) b.xa[k] = 0.0;
-  for (int t = threadIdx.x; t < b.nj; t += blockDim.x) b.hW[t] = b.hE[t] = 0.0;
-  for (int t = threadIdx.x; t < b.ni; t += blockDim.x) b.hS[t] = b.hN[t] = 0.0;
+  if (MODE != SEQ_SVOF) {
+    for (long k = threadIdx.x; k < nsk; k += blockDim.x) b.xa[k] = 0.0;
+    for (int t = threadIdx.x; t < b.nj; t += blockDim.x) b.hW[t] = b.hE[t] = 0.0;
+    for (int t = threadIdx.x; t < b.ni; t += blockDim.x) b.hS[t] = b.hN[t] = 0.0;
+  }
   __syncthreads();
   for (int sweep = 1; sweep <= n_sweeps; ++sweep) {
-    if (sync && sweep > 1) seq_snapshot(perx, pery, b, b.xa);
+    if (MODE == SEQ_SVOF) seq_snapshot(perx, pery, b, b.xa, true);
+    else if (sync && sweep > 1) seq_snapshot(perx, pery, b, b.xa);
     seq_half<true, MODE>(b, b.xa, b.xb, omega, W, ring);
-    if (sync) seq_snapshot(perx, pery, b, b.xb);
+    if (MODE != SEQ_SVOF && sync) seq_snapshot(perx, pery, b, b.xb);
     seq_half<false, MODE>(b, b.xb, b.xa, omega, W, ring);
   }
 }

@@ -19,11 +19,12 @@
 //                     in closed form (periodic image or zero-gradient copy),
 //                     the final pass materialises the output's halo.
 //   SVOF              the 25 symmetric Gauss-Seidel iterations are a
-//                     sequential recurrence: one workgroup walks the
-//                     anti-diagonals i + j = s (forward s ascending, backward
-//                     descending), so every cell sees exactly the neighbour
-//                     states the lexicographic sweep sees.
+//                     sequential recurrence: they run on the strip sweeps of
+//                     seq_sweep.h (one wave per 64-row strip, each lane
+//                     walking a row), so every cell sees exactly the
+//                     neighbour states the lexicographic sweep sees.
 #include "covof_internal.h"
+#include "seq_sweep.h"
 
 namespace covof {
 
@@ -268,60 +269,46 @@ __global__ void k_diffusion_halo(double *F, const double *__restrict__ prev, SMe
   F[HS(m, i, j)] = v;
 }
 
-// smooth_svof_helmholtz (:387-476) in one workgroup.  C and Cs are halo
-// fields; Cs = C on entry.
-__global__ void __launch_bounds__(1024) k_svof_wavefront(const double *__restrict__ C, SMesh m, double wx,
-                                                        double wy, double *Cs) {
+// smooth_svof_helmholtz (:387-476): 25 symmetric Gauss-Seidel iterations of
+// (I - D lap) C_s = C from C_s = C, the halo filled (zero-gradient copies on
+// the physical sides, then the exchange) before each iteration.  The
+// recurrence runs on the strip sweeps of seq_sweep.h (their SVOF mode): per
+// cell a record of a_P, a_W, a_E, a_S, a_N and C, the state starting from C.
+__global__ void __launch_bounds__(SBX *SBY) k_svof_records(const double *__restrict__ C, SMesh m, double wx,
+                                                          double wy, int S, double *__restrict__ rec,
+                                                          double *__restrict__ x) {
+  const int i = blockIdx.x * SBX + threadIdx.x + 1, j = blockIdx.y * SBY + threadIdx.y + 1;
+  if (i > m.nx || j > m.ny) return;
+  const bool bl = !m.perx, br = !m.perx, bb = !m.pery, bt = !m.pery;
+  const double aE = (i == m.nx && br) ? 0.0 : wx;
+  const double aW = (i == 1 && bl) ? 0.0 : wx;
+  const double aN = (j == m.ny && bt) ? 0.0 : wy;
+  const double aS = (j == 1 && bb) ? 0.0 : wy;
+  const long q = sk_of(S, i, j);
+  double *r = rec + q * 8;
+  r[SR_P] = 1.0 + aE + aW + aN + aS;
+  r[SR_W] = aW;
+  r[SR_E] = aE;
+  r[SR_S] = aS;
+  r[SR_N] = aN;
+  r[SR_R] = C[HS(m, i, j)];
+  x[q] = C[HS(m, i, j)];
+}
+
+__global__ void __launch_bounds__(64 * SEQ_MAXW) k_svof_sweeps(SMesh m, SeqBlk b) {
+  seq_sweeps<SEQ_SVOF>(m.perx != 0, m.pery != 0, b, 25, 1.0);
+}
+
+// The end of smooth_svof_helmholtz from the sweeps' state x (skewed), one
+// workgroup.
+__global__ void __launch_bounds__(1024) k_svof_finish(const double *__restrict__ x, int S, SMesh m, double *Cs) {
   const int nx = m.nx, ny = m.ny;
   const bool bl = !m.perx, br = !m.perx, bb = !m.pery, bt = !m.pery;
-  auto fill_halo = [&]() {
-    // homogeneous Neumann copies on the physical sides (:408-415)
-    for (int j = 1 + threadIdx.x; j <= ny; j += blockDim.x) {
-      if (bl) Cs[HS(m, 0, j)] = Cs[HS(m, 1, j)];
-      if (br) Cs[HS(m, nx + 1, j)] = Cs[HS(m, nx, j)];
-    }
-    for (int i = 1 + threadIdx.x; i <= nx; i += blockDim.x) {
-      if (bb) Cs[HS(m, i, 0)] = Cs[HS(m, i, 1)];
-      if (bt) Cs[HS(m, i, ny + 1)] = Cs[HS(m, i, ny)];
-    }
-    __syncthreads();
-    // exchange_halos_scalar: x wrap over all rows, then y wrap over all columns
-    if (m.perx)
-      for (int j = threadIdx.x; j <= ny + 1; j += blockDim.x) {
-        Cs[HS(m, 0, j)] = Cs[HS(m, nx, j)];
-        Cs[HS(m, nx + 1, j)] = Cs[HS(m, 1, j)];
-      }
-    __syncthreads();
-    if (m.pery)
-      for (int i = threadIdx.x; i <= nx + 1; i += blockDim.x) {
-        Cs[HS(m, i, 0)] = Cs[HS(m, i, ny)];
-        Cs[HS(m, i, ny + 1)] = Cs[HS(m, i, 1)];
-      }
-    __syncthreads();
-  };
-  auto relax = [&](int i, int j) {
-    const double aE = (i == nx && br) ? 0.0 : wx;
-    const double aW = (i == 1 && bl) ? 0.0 : wx;
-    const double aN = (j == ny && bt) ? 0.0 : wy;
-    const double aS = (j == 1 && bb) ? 0.0 : wy;
-    const double aP = 1.0 + aE + aW + aN + aS;
-    const double sum = aE * Cs[HS(m, i + 1, j)] + aW * Cs[HS(m, i - 1, j)] + aN * Cs[HS(m, i, j + 1)] +
-                       aS * Cs[HS(m, i, j - 1)];
-    Cs[HS(m, i, j)] = (C[HS(m, i, j)] + sum) / aP;
-  };
-  for (int it = 0; it < 25; ++it) {
-    fill_halo();
-    for (int s = 2; s <= nx + ny; ++s) {  // forward: west / south neighbours already updated
-      const int ilo = s - ny > 1 ? s - ny : 1, ihi = s - 1 < nx ? s - 1 : nx;
-      for (int i = ilo + threadIdx.x; i <= ihi; i += blockDim.x) relax(i, s - i);
-      __syncthreads();
-    }
-    for (int s = nx + ny; s >= 2; --s) {  // backward: east / north neighbours already updated
-      const int ilo = s - ny > 1 ? s - ny : 1, ihi = s - 1 < nx ? s - 1 : nx;
-      for (int i = ilo + threadIdx.x; i <= ihi; i += blockDim.x) relax(i, s - i);
-      __syncthreads();
-    }
+  for (long t = threadIdx.x; t < (long)nx * ny; t += blockDim.x) {
+    const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
+    Cs[HS(m, i, j)] = x[sk_of(S, i, j)];
   }
+  __syncthreads();
   // clamp to [0, 1] (:453-458), Neumann copies, the four corners, exchange (:460-474)
   for (long t = threadIdx.x; t < (long)nx * ny; t += blockDim.x) {
     const int i = (int)(t % nx) + 1, j = (int)(t / nx) + 1;
@@ -376,8 +363,8 @@ static int surface_work(Ctx &c, double **w0, double **w1) {
 }
 
 void surface_free(Ctx &c) {
-  hipFree(c.surf.work);
-  c.surf.work = nullptr;
+  for (double *p : {c.surf.work, c.surf.sv_rec, c.surf.sv_xa, c.surf.sv_xb, c.surf.sv_h}) hipFree(p);
+  c.surf = Surface();
 }
 
 int curvature_csf(Ctx &c, const double *G, const double *C, int enable, double sigma, double *curv,
@@ -439,8 +426,22 @@ int svof_smooth(Ctx &c, const double *C, double width, double *Cs) {
   if (k <= 0.0) k = 1.0;
   const double D = k * m.dx * m.dy;
   const double wx = D / (m.dx * m.dx), wy = D / (m.dy * m.dy);
-  COVOF_CHECK(hipMemcpyAsync(Cs, C, halo_size(c.m) * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  hipLaunchKernelGGL(k_svof_wavefront, dim3(1), dim3(1024), 0, c.stream, C, m, wx, wy, Cs);
+  Surface &F = c.surf;
+  const int S = m.nx + 63;
+  const long len = seq_len(m.nx, m.ny);
+  if (!F.sv_rec) {
+    COVOF_CHECK(hipMalloc(&F.sv_rec, 8 * len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_xa, len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_xb, len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_h, 4 * (long)(m.nx > m.ny ? m.nx : m.ny) * sizeof(double)));
+  }
+  const long nh = m.nx > m.ny ? m.nx : m.ny;
+  const SeqBlk b{m.nx, m.ny, (m.ny + 63) / 64, S, F.sv_rec, F.sv_xa, F.sv_xb,
+                 F.sv_h, F.sv_h + nh, F.sv_h + 2 * nh, F.sv_h + 3 * nh};
+  const dim3 blk(SBX, SBY), g_in((m.nx + SBX - 1) / SBX, (m.ny + SBY - 1) / SBY);
+  hipLaunchKernelGGL(k_svof_records, g_in, blk, 0, c.stream, C, m, wx, wy, S, F.sv_rec, F.sv_xa);
+  hipLaunchKernelGGL(k_svof_sweeps, dim3(1), dim3(64 * seq_waves(m.ny)), 0, c.stream, m, b);
+  hipLaunchKernelGGL(k_svof_finish, dim3(1), dim3(1024), 0, c.stream, (const double *)F.sv_xa, S, m, Cs);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
