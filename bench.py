@@ -321,11 +321,13 @@ def main():
     C_dev = torch.tensor(Cs, device=dev)
     del Cs
     steps_out = {}
-    for curv in ("height_function", "normals"):
+    variants = (("height_function", cv.PRECISION_MIXED), ("normals", cv.PRECISION_MIXED),
+                ("height_function", cv.PRECISION_FP64))
+    for curv, prec in variants:
         run = cv.StandardProjectionRun(sctx, C_dev, zs, zs, 1000.0, 1.0, 0.1, 1.0e-3, 0.07,
                                        time_scheme=cv.TIME_SCHEME_GEAR, convective_scheme=cv.CONV_TVD_SUPERBEE,
                                        gy=-9.81, predictor=(cv.LINEAR_CG, cv.PRECOND_JACOBI, 1e-10, 500),
-                                       poisson=(cv.LINEAR_CG, cv.PRECOND_MG, 1e-10, 20000, cv.PRECISION_MIXED),
+                                       poisson=(cv.LINEAR_CG, cv.PRECOND_MG, 1e-10, 20000, prec),
                                        curvature=curv)
         dts = 1.0e-4
         for _ in range(2):
@@ -336,20 +338,23 @@ def main():
         sp_its = [run.step(dts) for _ in range(nst)]
         torch.cuda.synchronize()
         t_step = (time.perf_counter() - t) / nst
-        steps_out[curv] = {
+        key = (curv, prec)
+        steps_out[key] = {
             "workload": f"two-phase standard-projection step {sx}x{sy} (2 x 1 box), {nbub} seeded bubbles (radii "
                         f"8-32 cells, exact volume fractions), rho 1000/1, mu 0.1/0.001, sigma 0.07 (CSF, curvature "
                         f"{curv.replace('_', '-')}), g 9.81, BDF2, TVD superbee, PLIC + WY, dt 1e-4; predictor "
-                        "CG + Jacobi to 1e-10; Poisson mixed-precision MG-PCG to 1e-10",
+                        "CG + Jacobi to 1e-10; Poisson " + ("mixed-precision" if prec == cv.PRECISION_MIXED else
+                                                           "FP64") + " MG-PCG to 1e-10",
             "ms_per_step": t_step * 1e3, "cell_steps_per_s": sx * sy / t_step,
             "predictor_iters": [a[0] for a in sp_its], "poisson_iters": [a[1] for a in sp_its],
             "status": [[a[2], a[3]] for a in sp_its], "kinetic_energy": run.kinetic_energy(),
         }
-        log(f"SP step {sx}x{sy} ({curv}): {t_step * 1e3:.2f} ms/step, predictor "
-            f"{steps_out[curv]['predictor_iters']} it, Poisson {steps_out[curv]['poisson_iters']} it")
+        log(f"SP step {sx}x{sy} ({curv}, precision {prec}): {t_step * 1e3:.2f} ms/step, predictor "
+            f"{steps_out[key]['predictor_iters']} it, Poisson {steps_out[key]['poisson_iters']} it")
         del run
-    sp_step = steps_out["height_function"]
-    sp_step["reference_curvature"] = steps_out["normals"]
+    sp_step = steps_out[("height_function", cv.PRECISION_MIXED)]
+    sp_step["reference_curvature"] = steps_out[("normals", cv.PRECISION_MIXED)]
+    sp_step["fp64_poisson"] = steps_out[("height_function", cv.PRECISION_FP64)]
     del sctx, zs, C_dev
 
     # ------------------------------------------ the plain-method case (128 x 256)
