@@ -46,7 +46,7 @@ enum Slot {
   S_ZERO = 63,  // never written: the stop slot of launches outside a solve
   NSLOT = 64
 };
-enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_MAXIT = 5 };
+enum Flag { F_RUN = 0, F_CONV = 1, F_BREAK = 2, F_NONFINITE = 3, F_MAXIT = 5, F_STALL = 6 };
 
 struct DevOp {
   int nx, ny;
@@ -921,7 +921,7 @@ __global__ void __launch_bounds__(BLK) k_residual(DevOp o, OpT<double> A, const 
 // rounding residue (the projection's b = -div u* telescopes to the boundary
 // flux) the FP64 residual then converges like the FP64 solve's.
 __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int nblk, double *scal, int first,
-                                                        int singular, double inv_n) {
+                                                        int singular, double inv_n, int stall_max) {
   if (scal[S_FLAG] != 0.0) return;
   double tot[4];
   grid_totals<4>(pin, nblk, tot);
@@ -948,13 +948,12 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
   } else if (first || rel < 0.9 * scal[S_BEST]) {
     scal[S_BEST] = rel;
     scal[S_STALL] = 0.0;
-  } else if ((scal[S_STALL] += 1.0) >= 48.0) {
-    // 48 reliable updates without a 10 % gain over the best residual (CG's
-    // residual norm plateaus for about a thousand Jacobi iterations at a time
-    // on the density-jump systems; this is far past that): the FP32
-    // recurrence has hit the attainable accuracy of this system, stop as not
-    // converged
-    scal[S_FLAG] = (double)F_MAXIT;
+  } else if ((scal[S_STALL] += 1.0) >= (double)stall_max) {
+    // stall_max reliable updates without a 10 % gain over the best residual:
+    // the FP32 recurrence has hit the attainable accuracy of this system (the
+    // FP32 operator's rounding times its condition number), and the host
+    // finishes the solve in FP64 from this x
+    scal[S_FLAG] = (double)F_STALL;
   }
 }
 
@@ -1421,8 +1420,12 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
   }
   hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, b, (const double *)x, P.r, P.z, pb,
                      (const double *)P.scal);
+  // stall window: 48 updates of 128 Jacobi iterations (CG's residual norm
+  // plateaus for about a thousand Jacobi iterations at a time on the
+  // density-jump systems; this is far past that), 64 multigrid iterations
+  const int stall_max = pc == 4 ? 8 : 48;
   hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
-                     first, singular, inv_n);
+                     first, singular, inv_n, stall_max);
   const bool cg2 = mixed_cg2(pc);
   // the single-reduction loop restarts from r32 in its parity-0 r buffer
   float *r32 = cg2 ? cg2_bufs(c).r[0] : P.r32;
@@ -1615,7 +1618,9 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
 // is formed with the refreshed residual), and the FP64 ||r||/||b|| <= tol test
 // ends the solve.  Method: reliable updates (Sleijpen & van der Vorst, 1996) as
 // used for mixed-precision CG by Clark et al., Comput. Phys. Commun. 181
-// (2010) 1517.
+// (2010) 1517.  When the FP64 residual stops improving (no 10 % gain over
+// a window of reliable updates) before tol, FP64 PCG continues from the
+// refined x; the iteration count is then the sum of both.
 static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
   const long n = (long)c.m.nx * c.m.ny;
@@ -1646,9 +1651,18 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     if ((st = run_cycles(c, P.graph[1], key, pc, true, cycle, maxit / cycle + 2))) return st;
   }
   if ((st = read_scalars(c, NREAD))) return st;
-  if (iters) *iters = (int)P.hscal[I_ITERS];
+  const int it32 = (int)P.hscal[I_ITERS];
+  if (iters) *iters = it32;
   if (relres) *relres = P.hscal[S_RELRES];
   const double f = P.hscal[S_FLAG];
+  if (f == F_STALL && it32 < maxit) {
+    // the FP32 recurrence stagnated above tol: FP64 PCG from the refined x
+    // (which holds every correction so far) for the remaining iterations
+    int it64 = 0;
+    st = solve_pcg64(c, precond, tol, maxit - it32, &it64, relres);
+    if (iters) *iters = it32 + it64;
+    return st;
+  }
   if (f == F_CONV) return COVOF_OK;
   if (f == F_NONFINITE) return COVOF_NONFINITE;
   return COVOF_NOT_CONVERGED;

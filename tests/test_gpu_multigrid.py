@@ -104,3 +104,26 @@ def test_mg_mixed_2048_reaches_1e10(cv, orc):
         rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x)) / np.linalg.norm(co["b"])
         print(f"precision {prec}: {it} MG-PCG iterations, true relres {rr:.3e}")
         assert rr <= 1.01e-10
+
+
+def test_mg_mixed_stall_continues_in_fp64(cv, orc):
+    """With tol 0 the mixed MG-PCG's FP64 residual reaches its rounding floor
+    and stops improving; the solve then continues as FP64 PCG from the refined
+    x until the iteration cap (NOT_CONVERGED with iters == cap) or until the
+    FP64 recurrence's own breakdown guard (BREAKDOWN, which only the FP64
+    loop reports).  Either way the answer sits at the FP64 floor."""
+    from covof_amd import workloads as wl
+    n, cap = 256, 400
+    case = wl.poisson_case(n, seed=1, n_circles=12, rmin=8, rmax=32)
+    ctx = cv.Context(n, n, bc=case["bc"])
+    b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
+    x = ctx.new_work()
+    r = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=0.0, max_iter=cap, precision=cv.PRECISION_MIXED)
+    print(f"mixed MG-PCG, tol 0: status {r.status}, {r.iters} iterations, relres {r.relres:.3e}")
+    assert r.status in (cv._lib.NOT_CONVERGED, cv._lib.BREAKDOWN), r
+    if r.status == cv._lib.NOT_CONVERGED:
+        assert r.iters == cap, r
+    m = orc.mesh(n, n, bc=case["bc"])
+    co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
+    rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x.cpu().numpy())) / np.linalg.norm(co["b"])
+    assert rr <= 1e-12, rr
