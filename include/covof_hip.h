@@ -183,7 +183,7 @@ int covof_poisson_precond_d(covof_ctx *ctx, int precond, const double *r, double
  *     iterative refinement that recomputes r = b - A x in FP64 and stops on
  *     the FP64 ||r||/||b|| <= tol (CG with NONE, JACOBI, SGS_RB or MG);
  *     when that residual stagnates above tol (no 10 % gain over 48
- *     refinements, 8 with MG) FP64 PCG continues from the refined x.
+ *     refinements, 2 of 8 iterations with MG) FP64 PCG continues from the refined x.
  * *iters: Krylov iterations (inner iterations summed for MIXED, plus the
  * FP64 continuation's); *relres:
  * final relative residual as defined by the stop rule. */
