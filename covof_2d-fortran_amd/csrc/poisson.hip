@@ -1424,7 +1424,7 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
   // plateaus for about a thousand Jacobi iterations at a time on the
   // density-jump systems; this is far past that), 16 multigrid iterations
   // (a converging MG-PCG gains far more than 10 % per 8 iterations)
-  const char *stall_env = getenv("COVOF_MIXED_STALL");  // build-time tunable override
+  const char *stall_env = getenv("COVOF_MIXED_STALL");  // the window in updates, for tuning
   const int stall_max = stall_env && atoi(stall_env) > 0 ? atoi(stall_env) : (pc == 4 ? 2 : 48);
   hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
                      first, singular, inv_n, stall_max);
