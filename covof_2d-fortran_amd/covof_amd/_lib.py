@@ -26,6 +26,7 @@ CURVATURE_NORMALS, CURVATURE_HEIGHT_FUNCTION = 0, 1
 # parallel preconditioners of this library (no reference counterpart)
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+REDUCTION_TREE, REDUCTION_REFERENCE = 0, 1
 TIME_SCHEME_EULER, TIME_SCHEME_GEAR = 1, 2
 CONV_UPWIND, CONV_CENTERED, CONV_TVD_SUPERBEE = 1, 2, 3
 CONV_LIN_PICARD, CONV_LIN_ADAMS_BASHFORTH, CONV_LIN_NEWTON = 1, 2, 3
@@ -50,6 +51,7 @@ SIGNATURES = {
     "covof_ctx_destroy": (_I, [_P]),
     "covof_ctx_set_stream": (_I, [_P, _P]),
     "covof_ctx_synchronize": (_I, [_P]),
+    "covof_ctx_set_reduction_order": (_I, [_P, _I]),
     "covof_last_error": (_S, []),
     "covof_geom_batch_d": (_I, [_P, _I] + [_P] * 10),
     "covof_reconstruct_interface_d": (_I, [_P, _S, _P, _P, _P, _P]),

@@ -96,6 +96,14 @@ class Context:
                 "covof_ctx_profile_read")
         return ms.value, n.value
 
+    def set_reduction_order(self, order: int):
+        """Summation order of the FP64 Krylov inner products and the pressure
+        mean (covof_ctx_set_reduction_order): REDUCTION_TREE (default) or
+        REDUCTION_REFERENCE, the reference's serial column-major sums
+        (src/module_simulation.f90:2760-2774), which reproduce its iterates
+        bit for bit."""
+        L.check(self._lib.covof_ctx_set_reduction_order(self._h, int(order)), "covof_ctx_set_reduction_order")
+
     def synchronize(self):
         L.check(self._lib.covof_ctx_synchronize(self._h), "covof_ctx_synchronize")
 

@@ -43,7 +43,9 @@ class StandardProjectionRun:
     linear_solver, precond_type, tol, max_iter); with ``solver_type``
     SOLVER_MONOLITHIC it is the coupled system's and ``poisson`` is unused.
     ``curvature``: "normals" (the reference's) or "height_function" (see
-    Context.compute_curvature_and_csf).
+    Context.compute_curvature_and_csf).  ``reduction``: the summation order
+    of the FP64 inner products (Context.set_reduction_order); with
+    REDUCTION_REFERENCE every step is the reference's arithmetic bit for bit.
     """
 
     FIELDS = ("C", "u", "v", "p", "rho", "rho_u", "rho_v", "mu", "mu_node", "curvature", "fx_sigma", "fy_sigma",
@@ -54,8 +56,10 @@ class StandardProjectionRun:
                  bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, reconstruction_method="PLIC",
                  predictor=(L.LINEAR_CG, L.PRECOND_JACOBI, 1e-10, 500),
                  poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True,
-                 solver_type=L.SOLVER_STANDARD_PROJECTION, curvature="normals"):
+                 solver_type=L.SOLVER_STANDARD_PROJECTION, curvature="normals", reduction=None):
         self.ctx = ctx
+        if reduction is not None:
+            ctx.set_reduction_order(reduction)
         self.rho1, self.rho2, self.mu1, self.mu2, self.sigma = (float(a) for a in (rho1, rho2, mu1, mu2, sigma))
         self.opts = dict(time_scheme=time_scheme, enable_convection=bool(convection), convective_scheme=convective_scheme,
                          convective_linearization=convective_linearization, gx=gx, gy=gy, bc_u=tuple(bc_u),

@@ -149,6 +149,14 @@ extern "C" int covof_ctx_set_stream(covof_ctx *h, void *stream) {
   return COVOF_OK;
 }
 
+extern "C" int covof_ctx_set_reduction_order(covof_ctx *h, int order) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  if (order != COVOF_REDUCTION_TREE && order != COVOF_REDUCTION_REFERENCE)
+    return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_reduction_order: order must be TREE (0) or REFERENCE (1)");
+  h->c.ref_order = order == COVOF_REDUCTION_REFERENCE;
+  return COVOF_OK;
+}
+
 extern "C" int covof_ctx_synchronize(covof_ctx *h) {
   if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
   if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);

@@ -157,6 +157,9 @@ struct Ctx {
   Mesh m;
   hipStream_t stream = nullptr;
   int own_stream = 0;
+  // COVOF_REDUCTION_REFERENCE: the FP64 Krylov inner products and the
+  // pressure mean are serial sums in the reference's element order
+  int ref_order = 0;
   Vof vof;
   Surface surf;
   Poisson ps;
@@ -195,6 +198,21 @@ inline __host__ __device__ long hidx(const Mesh &m, int i, int j) { return (long
   } while (0)
 
 #define COVOF_CHECK_LAUNCH() COVOF_CHECK(hipGetLastError())
+
+// Serial sums in the reference's element order (sum(x*y) / dot_product over
+// column-major arrays: one accumulator, index order, each product rounded
+// before it is added): sum q of a[q][t] * b[q][t] (b null: a[q][t] * z with
+// z = Jacobi(a[q][t], dg[q][t]) when dg is set, else a[q][t] alone), written
+// to dst[q][0], with dst[q][1 .. nzero-1] zeroed (a partial-sum slot whose
+// grid total is then the serial sum exactly).  Nothing runs while the scalar
+// slot f0 or f1 is nonzero (a stopped solve).
+constexpr int SER_MAX = 5;
+struct SerialSet {
+  const double *a[SER_MAX], *b[SER_MAX], *dg[SER_MAX];
+  double *dst[SER_MAX];
+  int nq, nzero;
+};
+void serial_sums(Ctx &c, long n, const SerialSet &s, int f0, int f1);
 
 int mg_check(const Ctx &c, int kind);
 int mg_prepare(Ctx &c, bool fp32);

@@ -156,7 +156,18 @@ struct Bicg {
       P.ia[q] = ia < ib ? ia : ib;
       P.ib[q] = ia < ib ? ib : ia;
     }
-    if (nv == 1)
+    if (c.ref_order) {
+      // the reference's dot_product: one serial sum per product, in index order
+      SerialSet S{};
+      S.nq = k;
+      S.nzero = 1;
+      for (int q = 0; q < k; ++q) {
+        S.a[q] = a[q];
+        S.b[q] = bb[q];
+        S.dst[q] = c.ps.scal + 40 + q;
+      }
+      serial_sums(c, n, S, 63, 63);
+    } else if (nv == 1)
       hipLaunchKernelGGL(k_dots<1>, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);
     else if (nv == 2)
       hipLaunchKernelGGL(k_dots<2>, dim3(512), dim3(BLK), 0, c.stream, n, P, c.ps.part, c.ps.ticket, c.ps.scal + 40);

@@ -184,6 +184,76 @@ __device__ void grid_totals(const double *part, int nblk, double (&tot)[NV], int
   }
 }
 
+// ------------------------------------ reference-order sums (serial, exact)
+//
+// The reference sums an inner product as sum(x*y) (or dot_product) over a
+// column-major array: one accumulator, elements in index order (i fastest),
+// each product rounded before it is added.  That chain is sequential, so the
+// reference-order mode gives each sum one wave: all 64 lanes form the
+// products of the next 512-element chunk and stage them in LDS while lane 0
+// adds the current chunk's products, in order, into its accumulator.  Several
+// sums run side by side in the waves of one workgroup.
+constexpr int SER_CH = 512;  // products per chunk and sum
+
+__device__ __forceinline__ double ser_term(const SerialSet &s, int q, long t) {
+  const double x = s.a[q][t];
+  if (s.b[q]) return x * s.b[q][t];
+  if (s.dg[q]) return x * jacobi<double>(x, s.dg[q][t]);
+  return x;
+}
+
+__global__ void __launch_bounds__(64 * SER_MAX) k_serial_sums(long n, SerialSet s, const double *scal, int f0,
+                                                              int f1) {
+  if (scal[f0] != 0.0 || scal[f1] != 0.0) return;
+  constexpr int U = SER_CH / 64;
+  __shared__ double buf[SER_MAX][2][SER_CH];
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long nch = (n + SER_CH - 1) / SER_CH;
+  double v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const long t = u * 64 + lane;
+    v[u] = t < n ? ser_term(s, q, t) : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) buf[q][0][u * 64 + lane] = v[u];
+  __syncthreads();
+  double acc = 0.0;
+  for (long c = 0; c < nch; ++c) {
+    const bool more = c + 1 < nch;  // uniform over the workgroup
+    if (more) {
+      const long base = (c + 1) * SER_CH;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long t = base + u * 64 + lane;
+        v[u] = t < n ? ser_term(s, q, t) : 0.0;
+      }
+    }
+    if (lane == 0) {
+      const double *bb = buf[q][c & 1];
+      const long left = n - c * SER_CH;
+      if (left >= SER_CH) {
+#pragma unroll 32
+        for (int m = 0; m < SER_CH; ++m) acc = acc + bb[m];
+      } else {
+        for (int m = 0; m < (int)left; ++m) acc = acc + bb[m];
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) buf[q][(c + 1) & 1][u * 64 + lane] = v[u];
+    }
+    __syncthreads();
+  }
+  if (lane == 0) s.dst[q][0] = acc;
+  for (int k = lane + 1; k < s.nzero; k += 64) s.dst[q][k] = 0.0;
+}
+
+void serial_sums(Ctx &c, long n, const SerialSet &s, int f0, int f1) {
+  hipLaunchKernelGGL(k_serial_sums, dim3(1), dim3(64 * s.nq), 0, c.stream, n, s, (const double *)c.ps.scal, f0,
+                     f1);
+}
+
 // Cell loop over a 1-D grid of 8 * gx * chunks blocks (gx = 64-column strips).
 // The dispatcher deals workgroup ids round-robin over the 8 XCDs, so block id
 // goes to XCD id % 8; that XCD owns a horizontal band of ny/8 rows, and its
@@ -1334,6 +1404,32 @@ static int status_of_flag(double f) {
   return COVOF_NOT_CONVERGED;
 }
 
+// The serial sums that replace a B kernel's partials in reference-order mode:
+// r.r into slot 0 and r.z into slot 1 of pb (z = r without a preconditioner,
+// Jacobi's r / a_P formed on the fly, else the stored z), and b.b into slot 2
+// for the initial residual.
+static SerialSet rz_terms(Ctx &c, int pc, const double *r, const double *z, const double *diag, double *pb,
+                          bool with_b) {
+  SerialSet s{};
+  s.nzero = c.ps.nblocks;
+  s.a[0] = r;
+  s.b[0] = r;
+  s.dst[0] = pb;
+  s.a[1] = r;
+  if (pc == 0) s.b[1] = r;
+  else if (pc == 1) s.dg[1] = diag;
+  else s.b[1] = z;
+  s.dst[1] = pb + c.ps.nblocks;
+  s.nq = 2;
+  if (with_b) {
+    s.a[2] = c.ps.b;
+    s.b[2] = c.ps.b;
+    s.dst[2] = pb + 2L * c.ps.nblocks;
+    s.nq = 3;
+  }
+  return s;
+}
+
 // One PCG iteration k (A_k, B_k) on (x, r, z); A reads r (PC 0/1: z is
 // formed from r) or z (explicit preconditioners); d_prev / d alternate between
 // d0 and d1 with the parity of k (the first iteration, k odd, reads d0, which
@@ -1348,12 +1444,28 @@ static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *A
   T *dprev = (k & 1) ? d0 : d1;
   T *dnew = (k & 1) ? d1 : d0;
   const int tag = timed ? (sizeof(T) == 8 ? COVOF_KERNEL_PCG_A_FP64 : COVOF_KERNEL_PCG_A_FP32) : -1;
+  // reference-order mode (FP64): each grid reduction is replaced by the
+  // serial sum, written as the slot's only nonzero partial
+  const bool ser = sizeof(T) == 8 && c.ref_order;
+  const long n = (long)o.nx * o.ny;
   launch_tagged(c, tag, k_pcg_a<T, PC>, g, dim3(BLK), o, A, zr, (const T *)dprev, dnew, Ad, x, c.ps.scal, sl,
                 (const double *)pb, pa, k);
+  if (ser) {
+    SerialSet s{};
+    s.nq = 1;
+    s.nzero = c.ps.nblocks;
+    s.a[0] = (const double *)dnew;
+    s.b[0] = (const double *)Ad;
+    s.dst[0] = pa;
+    serial_sums(c, n, s, sl.flag, S_FLAG);
+  }
   launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, r, (const T *)Ad,
                 c.ps.scal, sl, (const double *)pa, pb, k);
-  if (PC >= 2) {
-    explicit_precond<T>(c, PC, (const T *)r, z, sl.flag);
+  if (PC >= 2) explicit_precond<T>(c, PC, (const T *)r, z, sl.flag);
+  if (ser) {
+    serial_sums(c, n, rz_terms(c, PC, (const double *)r, (const double *)z, (const double *)A.diag, pb, false),
+                sl.flag, S_FLAG);
+  } else if (PC >= 2) {
     hipLaunchKernelGGL(k_pcg_rz<T>, g, dim3(BLK), 0, c.stream, o, (const T *)r, (const T *)z, c.ps.scal, sl, pb);
   }
 }
@@ -1578,12 +1690,21 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
     hipLaunchKernelGGL((k_residual<0>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
                        P.r, P.z, pb, (const double *)P.scal);
   COVOF_CHECK_LAUNCH();
+  // z = M^-1 r for the explicit preconditioners, then r.z into slot 1 of the
+  // residual partials (reference-order mode: r.r, r.z and b.b as serial sums)
   if (pc == 2) {
-    // z = SGS(r) and r.z into slot 1 of the residual partials, then launch by launch
     if ((st = sgs_apply(c, (const double *)P.r, P.z, nullptr))) return st;
-    hipLaunchKernelGGL(k_pcg_rz, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
+  } else if (pc >= 3) {
+    explicit_precond<double>(c, pc, (const double *)P.r, P.z, OUTER.flag);
+  }
+  if (c.ref_order) {
+    serial_sums(c, n, rz_terms(c, pc, P.r, P.z, P.diag, pb, true), S_FLAG, S_FLAG);
+  } else if (pc >= 2) {
+    hipLaunchKernelGGL(k_pcg_rz<double>, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
                        (const double *)P.scal, OUTER, pb);
-    COVOF_CHECK_LAUNCH();
+  }
+  COVOF_CHECK_LAUNCH();
+  if (pc == 2) {  // the sequential SGS: launch by launch
     for (int k = 1;; ++k) {
       pcg_iteration<double, 2>(c, A, P.x, P.r, P.z, P.p, P.t0, P.q, OUTER, k, prof_sampled(c, k));
       COVOF_CHECK_LAUNCH();
@@ -1593,14 +1714,9 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
       }
     }
   } else {
-    if (pc >= 3) {  // z = M^-1 r and r.z into slot 1 of the residual partials
-      explicit_precond<double>(c, pc, (const double *)P.r, P.z, OUTER.flag);
-      hipLaunchKernelGGL(k_pcg_rz<double>, g, dim3(BLK), 0, c.stream, o, (const double *)P.r, (const double *)P.z,
-                         (const double *)P.scal, OUTER, pb);
-      COVOF_CHECK_LAUNCH();
-    }
     const int cyc = cycle64(pc);
-    if ((st = run_cycles(c, P.graph[0], 1 + pc, pc, false, cyc, maxit / cyc + 2))) return st;
+    if ((st = run_cycles(c, P.graph[0], 1 + pc + (c.ref_order ? 64 : 0), pc, false, cyc, maxit / cyc + 2)))
+      return st;
   }
   flush_x<double>(c, P.x, P.p, P.t0, OUTER, S_ZERO);
   COVOF_CHECK_LAUNCH();
@@ -1712,9 +1828,18 @@ int sp_poisson(Ctx &c, const double *u, const double *v, const double *rho_u, co
   const int solve_st = poisson_solve(c, P.b, P.x, lin_solver, precond, tol, maxit, precision, iters, relres);
   if (solve_st == COVOF_BAD_ARGUMENT || solve_st == COVOF_HIP_ERROR) return solve_st;
   const DevOp o = dev_op(m);
-  hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, P.x, part_buf(c, 2));
-  hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)part_buf(c, 2), P.nblocks,
-                     P.scal, (int)S_DOT0);
+  if (c.ref_order) {  // sum(p_sol) in the reference's order
+    SerialSet s{};
+    s.nq = 1;
+    s.nzero = 1;
+    s.a[0] = P.x;
+    s.dst[0] = P.scal + S_DOT0;
+    serial_sums(c, (long)m.nx * m.ny, s, S_ZERO, S_ZERO);
+  } else {
+    hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, P.x, part_buf(c, 2));
+    hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)part_buf(c, 2), P.nblocks,
+                       P.scal, (int)S_DOT0);
+  }
   hipLaunchKernelGGL(k_write_pressure, grd, blk, 0, c.stream, o, P.x, P.scal, S_DOT0, p);
   hipLaunchKernelGGL(k_wrap_halo, dim3(1), dim3(1024), 0, c.stream, m.nx, m.ny, m.perx, m.pery, p);
   COVOF_CHECK_LAUNCH();

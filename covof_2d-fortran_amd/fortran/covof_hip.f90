@@ -47,6 +47,8 @@ module covof_hip
   !> parallel preconditioners of the GPU library (no reference counterpart)
   integer(c_int), parameter :: COVOF_PRECOND_SGS_RB = 16, COVOF_PRECOND_MG = 17
   integer(c_int), parameter :: COVOF_PRECISION_FP64 = 0, COVOF_PRECISION_MIXED = 1
+  !> summation order of the FP64 inner products (covof_ctx_set_reduction_order)
+  integer(c_int), parameter :: COVOF_REDUCTION_TREE = 0, COVOF_REDUCTION_REFERENCE = 1
   integer(c_int), parameter :: COVOF_TIME_SCHEME_EULER = 1, COVOF_TIME_SCHEME_GEAR = 2
   integer(c_int), parameter :: COVOF_CONV_UPWIND = 1, COVOF_CONV_CENTERED = 2, COVOF_CONV_TVD_SUPERBEE = 3
   integer(c_int), parameter :: COVOF_CONV_LIN_PICARD = 1, COVOF_CONV_LIN_ADAMS_BASHFORTH = 2, &
@@ -82,6 +84,12 @@ module covof_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function covof_ctx_synchronize
+
+     integer(c_int) function covof_ctx_set_reduction_order(ctx, order) bind(C, name="covof_ctx_set_reduction_order")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: order
+     end function covof_ctx_set_reduction_order
 
      type(c_ptr) function covof_last_error_c() bind(C, name="covof_last_error")
        import :: c_ptr

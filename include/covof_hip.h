@@ -78,6 +78,20 @@ int covof_ctx_set_stream(covof_ctx *ctx, void *stream);
 int covof_ctx_synchronize(covof_ctx *ctx);
 /* Message of the last error seen on this thread (static storage). */
 const char *covof_last_error(void);
+/* Summation order of the FP64 Krylov inner products and of the pressure mean.
+ * COVOF_REDUCTION_TREE (the default): deterministic parallel tree sums (the
+ * same inputs give the same bits run to run).  COVOF_REDUCTION_REFERENCE: the
+ * reference's order -- sum(x*y) / dot_product over the column-major arrays,
+ * one accumulator in index order (global_dot_product_2d and the p mean,
+ * src/module_simulation.f90:2577-2585, :2760-2774; global_dot_product,
+ * src/module_mpi_comm.f90:820-833, as the Krylov loops of
+ * src/module_linear_solvers.f90 call it) -- so that FP64 solves reproduce the
+ * reference's iterates bit for bit; each sum is then one sequential chain on
+ * the GPU (about 0.1 ms per 32k terms).  The FP32 recurrence of the mixed
+ * mode always uses tree sums. */
+#define COVOF_REDUCTION_TREE 0
+#define COVOF_REDUCTION_REFERENCE 1
+int covof_ctx_set_reduction_order(covof_ctx *ctx, int order);
 
 /* --- interface geometry (src/module_interface_reconstruction.f90) ----------- */
 

@@ -30,20 +30,25 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+@pytest.mark.parametrize("order", ["tree", "reference"])
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "momentum_*.npz"))),
                          ids=lambda p: os.path.basename(p))
-def test_momentum_golden(cv, path):
+def test_momentum_golden(cv, path, order):
     """diag_u/diag_v (and the Schur diagonal diag_p of the monolithic
     system), assemble_rhs, two products of apply_system_matrix and
     apply_preconditioner (JACOBI, the sequential SSOR or the block-triangular
     Schur-SSOR) bitwise; then the whole block -- the SP predictor (pack,
     solve_linear_system, unpack, sync, BC) or the monolithic solve of u, v,
-    p -- on the same operator state as the reference's run."""
+    p -- on the same operator state as the reference's run.  With the
+    reference's summation order the solve is the reference's bit for bit
+    (iterations and fields)."""
     g = np.load(path, allow_pickle=False)
     nx, ny = int(g["nx"]), int(g["ny"])
     print(f"\n[momentum golden] {os.path.basename(path)} {nx}x{ny} stype {int(g['stype'])} "
-          f"precond {int(g['precond'])}", flush=True)
+          f"precond {int(g['precond'])} order {order}", flush=True)
     ctx = cv.Context(nx, ny, float(g["lx"]), float(g["ly"]), bc=tuple(int(b) for b in g["bc"]))
+    if order == "reference":
+        ctx.set_reduction_order(cv.REDUCTION_REFERENCE)
     F = {k: to_gpu(g[ORC[k]]) for k in NAMES}
     cfg = _cfg(cv, g)
     ctx.momentum_set_fields(cfg, F, float(g["dt"]))
@@ -63,6 +68,15 @@ def test_momentum_golden(cv, path):
     else:
         res = ctx.solve_standard_projection_predictor(cfg, F, *args)
     assert res.converged, res
+    if order == "reference":
+        assert res.iters == int(g["iters"]), (res.iters, int(g["iters"]))
+        outs = [("u", F["u"]), ("v", F["v"])]
+        if int(g["stype"]) == cv.SOLVER_MONOLITHIC:
+            outs.append(("p", p))
+        for k, t in outs:
+            got = t.cpu().numpy()
+            assert np.array_equal(got, g[k + "_out"]), f"{k}: rel diff {_rel(got, g[k + '_out']):.3e}"
+        return
     # the inner products sum in another order than the reference's serial
     # ones: a long, slowly converging run (the Jacobi-preconditioned coupled
     # system takes 130 BiCGSTAB(2) iterations) may stop a few steps apart

@@ -102,9 +102,12 @@ def test_pcg_iteration_cap_vs_oracle(cv, orc, precond):
         assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max(), cap
 
 
-def test_sp_reference_mode_vs_oracle(cv, orc):
+@pytest.mark.parametrize("order", ["tree", "reference"])
+def test_sp_reference_mode_vs_oracle(cv, orc, order):
     """The reference's own SP settings: CG + SGS (SSOR), tol 1e-7, 1000 iterations,
-    x0 = p^n, mean removal; rising-bubble-like density on 64 x 128 walls."""
+    x0 = p^n, mean removal; rising-bubble-like density on 64 x 128 walls.
+    Tree sums: iterations within one, p within 1e-6; the reference's
+    summation order: the same iteration count and p bit for bit."""
     from covof_amd import workloads as wl
     nx, ny = 64, 128
     C = wl.disc(nx, ny, 0.5, 0.5, 0.25, lx=1.0, ly=2.0)
@@ -117,11 +120,16 @@ def test_sp_reference_mode_vs_oracle(cv, orc):
     p0 = rng.normal(size=C.shape) * 1e-3
     dt = 1e-3
     ctx = cv.Context(nx, ny, 1.0, 2.0, bc=(1, 1, 1, 1))
+    if order == "reference":
+        ctx.set_reduction_order(cv.REDUCTION_REFERENCE)
     pg = to_gpu(p0)
     res = ctx.solve_standard_projection_poisson(to_gpu(u), to_gpu(v), to_gpu(rho_u), to_gpu(rho_v), pg, dt)
     co = orc.poisson_assemble(m, u, v, rho_u, rho_v, dt)
     xo, it, rr = orc.sp_pcg(m, 2, co, co["b"], p0[1:ny + 1, 1:nx + 1].copy(), tol=1e-7, maxit=1000)
     po = orc.sp_finish_pressure(m, xo, p0)
+    if order == "reference":
+        assert res.iters == it and res.relres == rr, (res.iters, it, res.relres, rr)
+        assert np.array_equal(pg.cpu().numpy()[1:ny + 1, 1:nx + 1], po[1:ny + 1, 1:nx + 1])
     assert abs(res.iters - it) <= 1, (res.iters, it)
     assert res.relres <= 1e-7
     pgn = pg.cpu().numpy()

@@ -20,7 +20,7 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def _golden_run(cv, g):
+def _golden_run(cv, g, reduction=None):
     nx, ny = int(g["nx"]), int(g["ny"])
     ctx = cv.Context(nx, ny, float(g["lx"]), float(g["ly"]), bc=tuple(int(b) for b in g["bc"]))
     return cv.StandardProjectionRun(
@@ -28,7 +28,7 @@ def _golden_run(cv, g):
         float(g["mu1"]), float(g["mu2"]), float(g["sigma"]), time_scheme=int(g["tscheme"]),
         convective_scheme=int(g["scheme"]), convective_linearization=int(g["lin"]), gx=float(g["gx"]),
         gy=float(g["gy"]), bc_u=tuple(g["bcu"]), bc_v=tuple(g["bcv"]), reconstruction_method=METHOD[int(g["method"])],
-        predictor=(int(g["solver"]), int(g["precond"]), float(g["tol"]), int(g["maxit"])))
+        predictor=(int(g["solver"]), int(g["precond"]), float(g["tol"]), int(g["maxit"])), reduction=reduction)
 
 
 def test_sp_run_golden(cv):
@@ -56,16 +56,32 @@ def test_sp_run_golden(cv):
         assert err < 1e-11, (k, err)
 
 
+def test_sp_run_golden_reference_order(cv):
+    """The same six steps with the inner products summed in the reference's
+    order: every iteration count equal and C, u, v, p bit for bit the
+    compiled reference's."""
+    g = np.load(os.path.join(GOLDEN, "spstep_rising_bubble.npz"), allow_pickle=False)
+    run = _golden_run(cv, g, reduction=cv.REDUCTION_REFERENCE)
+    for s in range(int(g["nsteps"])):
+        iu, ip, su, sp = run.step(float(g["dt"]))
+        assert su == 0 and sp in (0, 1), (s, su, sp)
+        assert iu + ip == int(g["iters"][s]), (s, iu, ip, int(g["iters"][s]))
+        if s == 0:
+            for k, ref in (("C", "C1"), ("u", "u1"), ("v", "v1"), ("p", "p1")):
+                assert np.array_equal(run.f[k].cpu().numpy(), g[ref]), k
+    for k in ("C", "u", "v", "p"):
+        assert np.array_equal(run.f[k].cpu().numpy(), g[k]), k
+
+
 def test_sp_run_plain_case_200_steps(cv):
     """The plain-method case itself: the 128 x 256 rising bubble, 200 steps
-    of dt 1e-3, every step on the GPU against the compiled reference's own
-    200-step run: every step's iteration total equal (measured: all 200),
-    and C, u, v, p within 5e-6 after 200 steps (measured 2-7e-7: the 1e-14
-    summation-order differences of the first steps grow by about 8 % per
-    step through the interface dynamics)."""
+    of dt 1e-3, every step on the GPU with the inner products summed in the
+    reference's order, against the compiled reference's own 200-step run:
+    every step's iteration total equal and C, u, v, p bit for bit after 200
+    steps (the north-star bar is 1e-10 for p, u, v and 1e-12 for C)."""
     import time
     g = np.load(os.path.join(GOLDEN, "spstep_rising_bubble_128x256_200.npz"), allow_pickle=False)
-    run = _golden_run(cv, g)
+    run = _golden_run(cv, g, reduction=cv.REDUCTION_REFERENCE)
     diffs = []
     t0 = time.perf_counter()
     for s in range(int(g["nsteps"])):
@@ -76,9 +92,23 @@ def test_sp_run_plain_case_200_steps(cv):
           f"{sum(d != 0 for d in diffs)} steps differ, max |d| {max(abs(d) for d in diffs)}")
     errs = {k: _rel(run.f[k].cpu().numpy(), g[k]) for k in ("C", "u", "v", "p")}
     print("after 200 steps: " + ", ".join(f"{k} {e:.3e}" for k, e in errs.items()))
-    assert max(abs(d) for d in diffs) <= 1
-    for k, e in errs.items():
-        assert e < 5e-6, (k, e)
+    assert max(abs(d) for d in diffs) == 0
+    for k in ("C", "u", "v", "p"):
+        assert np.array_equal(run.f[k].cpu().numpy(), g[k]), (k, errs[k])
+
+
+def test_sp_run_plain_case_tree_order(cv):
+    """The first 40 steps of the plain-method case with the default (tree)
+    summation order: every iteration total within one of the reference's
+    and the fields within 1e-8 (the 1e-14 summation-order differences of
+    the first steps grow by about 8 % per step through the interface
+    dynamics; measured 2-7e-7 after 200 steps)."""
+    g = np.load(os.path.join(GOLDEN, "spstep_rising_bubble_128x256_200.npz"), allow_pickle=False)
+    run = _golden_run(cv, g)
+    for s in range(40):
+        iu, ip, su, sp = run.step(float(g["dt"]))
+        assert su == 0 and sp in (0, 1), (s, su, sp)
+        assert abs(iu + ip - int(g["iters"][s])) <= 1, s
 
 
 def test_sp_run_vs_oracle(cv, orc):
