@@ -55,6 +55,7 @@ SIGNATURES = {
     "covof_last_error": (_S, []),
     "covof_geom_batch_d": (_I, [_P, _I] + [_P] * 10),
     "covof_reconstruct_interface_d": (_I, [_P, _S, _P, _P, _P, _P]),
+    "covof_reconstruct_interface_noalloc_d": (_I, [_I, _I, _D, _D, _IP, _S, _P, _P, _P, _P, _P]),
     "covof_vof_advect_d": (_I, [_P, _P, _P, _P, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_P] * 5),
     "covof_vof_advect": (_I, [_P, _DP, _DP, _DP, _D, _S, _S, _IP, _I, _I, _D, _D, _D, _D] + [_DP] * 5),
     "covof_update_properties_d": (_I, [_P, _P, _D, _D, _D, _D] + [_P] * 5),

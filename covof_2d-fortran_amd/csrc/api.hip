@@ -20,6 +20,11 @@ int vof_geom_batch(Ctx &c, int n, const double *xmin, const double *xmax, const 
                    const double *ymax, const double *gnx, const double *gny, const double *alpha,
                    const double *cc, double *area, double *aout);
 int vof_corner_fix(Ctx &c, double *C);
+int noalloc_scratch(int **out);
+int noalloc_max_tiles();
+int noalloc_tiles(const Mesh &m);
+int vof_reconstruct_noalloc(const Mesh &mesh, int method, const double *C, double *onx, double *ony, double *oal,
+                            hipStream_t s);
 int poisson_assemble(Ctx &c, const double *u, const double *v, const double *rho_u,
                      const double *rho_v, double dt, double *b);
 int poisson_matvec(Ctx &c, const double *x, double *y);
@@ -60,6 +65,24 @@ static int hip_status(int st) {
 }
 
 extern "C" const char *covof_last_error(void) { return g_err.c_str(); }
+
+// The mesh of a context-free entry point, with covof_ctx_create's checks.
+static int mesh_of(int nx, int ny, double lx, double ly, const int *bc, covof::Mesh *m) {
+  if (!bc || nx < 2 || ny < 2 || !(lx > 0.0) || !(ly > 0.0)) return COVOF_BAD_ARGUMENT;
+  if ((long)(nx + 2) * (ny + 2) > 0x7fffffffL) return COVOF_BAD_ARGUMENT;
+  for (int k = 0; k < 4; ++k)
+    if (bc[k] < 1 || bc[k] > 4) return COVOF_BAD_ARGUMENT;
+  m->nx = nx;
+  m->ny = ny;
+  m->lx = lx;
+  m->ly = ly;
+  m->dx = lx / (double)nx;
+  m->dy = ly / (double)ny;
+  for (int k = 0; k < 4; ++k) m->bc[k] = bc[k];
+  m->perx = bc[0] == covof::BC_PERIODIC || bc[1] == covof::BC_PERIODIC;
+  m->pery = bc[2] == covof::BC_PERIODIC || bc[3] == covof::BC_PERIODIC;
+  return COVOF_OK;
+}
 
 // "WEYMOUTH_YUE" is the only geometric advection scheme of this library; the
 // reconstruction name follows reconstruct_interface's select case
@@ -106,7 +129,11 @@ extern "C" int covof_ctx_create(int nx, int ny, double lx, double ly, const int 
     return hip_status(COVOF_HIP_ERROR);
   }
   c.own_stream = 1;
-  int st = covof::vof_alloc(c);
+  // the device address of the allocation-free entry point's scratch array on
+  // this GPU (looked up once per device)
+  int *scratch_dev = nullptr;
+  int st = covof::noalloc_scratch(&scratch_dev);
+  if (!st) st = covof::vof_alloc(c);
   if (!st) st = covof::poisson_alloc(c);
   if (st) {
     covof_ctx_destroy(h);
@@ -177,6 +204,27 @@ extern "C" int covof_reconstruct_interface_d(covof_ctx *h, const char *method, c
   if (!h || !C || !seg_nx || !seg_ny || !seg_alpha) return fail(COVOF_BAD_ARGUMENT, "null argument");
   if (recon_code(method, &code)) return fail(COVOF_BAD_ARGUMENT, "reconstruction method must be PLIC or ELVIRA");
   return hip_status(covof::vof_reconstruct_interface(h->c, code, C, seg_nx, seg_ny, seg_alpha));
+}
+
+extern "C" int covof_reconstruct_interface_noalloc_d(int nx, int ny, double lx, double ly, const int *bc,
+                                                     const char *method, const double *C, double *seg_nx,
+                                                     double *seg_ny, double *seg_alpha, void *stream) {
+  covof::Mesh m;
+  int code;
+  if (mesh_of(nx, ny, lx, ly, bc, &m))
+    return fail(COVOF_BAD_ARGUMENT, "covof_reconstruct_interface_noalloc_d: need nx, ny >= 2, lx, ly > 0, bc[4]");
+  if (!C || !seg_nx || !seg_ny || !seg_alpha) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (seg_nx == seg_ny || seg_nx == seg_alpha || seg_ny == seg_alpha || C == seg_nx || C == seg_ny ||
+      C == seg_alpha)
+    return fail(COVOF_BAD_ARGUMENT, "covof_reconstruct_interface_noalloc_d: outputs must be distinct from C and "
+                                    "from each other");
+  if (recon_code(method, &code)) return fail(COVOF_BAD_ARGUMENT, "reconstruction method must be PLIC or ELVIRA");
+  if (covof::noalloc_tiles(m) > covof::noalloc_max_tiles())
+    return fail(COVOF_BAD_ARGUMENT, (std::string("covof_reconstruct_interface_noalloc_d: the mesh needs ") +
+                                     std::to_string(covof::noalloc_tiles(m)) + " tiles of 64 x 16 cells, more than the " +
+                                     std::to_string(covof::noalloc_max_tiles()) + " its scratch holds")
+                                        .c_str());
+  return hip_status(covof::vof_reconstruct_noalloc(m, code, C, seg_nx, seg_ny, seg_alpha, (hipStream_t)stream));
 }
 
 extern "C" int covof_vof_advect_d(covof_ctx *h, double *C, const double *u, const double *v, double dt,

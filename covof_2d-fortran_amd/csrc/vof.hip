@@ -29,6 +29,7 @@
 #include "geometry.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace covof {
 
@@ -824,6 +825,69 @@ int vof_reconstruct_interface(Ctx &c, int method, const double *C, double *onx, 
   const dim3 grd((c.m.nx + 2 + 63) / 64, (c.m.ny + 2 + 3) / 4);
   hipLaunchKernelGGL(k_expand_segments, grd, blk, 0, c.stream, C, m, c.vof.snx, c.vof.sny,
                      c.vof.sal, onx, ony, oal);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+// ------------------------------------------------ allocation-free variant
+//
+// covof_reconstruct_interface_noalloc_d needs no context: its only scratch
+// (the per-tile lists of mixed cells, their sizes and offsets) lives in the
+// file-scope array below, one copy per GPU, held until the process ends; the
+// segments are written straight into the caller's output arrays.  Nothing in
+// it must start at zero (k_classify writes every count and list entry the
+// later kernels read, k_tile_scan every offset).  Calls that overlap on one
+// GPU share the array and race.
+constexpr int NOALLOC_MAX_TILES = 1024;  // e.g. 1024 x 1024, 2048 x 512 cells
+constexpr long NOALLOC_INTS = (long)NOALLOC_MAX_TILES * TCELLS + 2L * NOALLOC_MAX_TILES + 1;
+static __device__ __align__(256) int scratch[NOALLOC_INTS];  // 4.2 MB
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static int *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The device address of this GPU's copy of `scratch` (the current device),
+// looked up once per device; the name itself is a host-side stand-in.
+int noalloc_scratch(int **out) {
+  int dev = 0;
+  COVOF_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return COVOF_BAD_ARGUMENT;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int *p = nullptr;
+    COVOF_CHECK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) return COVOF_HIP_ERROR;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return COVOF_OK;
+}
+
+int noalloc_max_tiles() { return NOALLOC_MAX_TILES; }
+int noalloc_tiles(const Mesh &m) { return n_tiles(m); }
+
+// reconstruct_interface on stream s with the tile lists in the scratch array:
+// classify, scan, reconstruct into (onx, ony, oal), then fill the non-mixed
+// cells in place.
+int vof_reconstruct_noalloc(const Mesh &mesh, int method, const double *C, double *onx, double *ony, double *oal,
+                            hipStream_t s) {
+  const int nt = n_tiles(mesh);
+  if (nt > NOALLOC_MAX_TILES) return COVOF_BAD_ARGUMENT;
+  int *base = nullptr;
+  const int st = noalloc_scratch(&base);
+  if (st) return st;
+  int *list = base, *count = base + (long)NOALLOC_MAX_TILES * TCELLS, *offs = count + NOALLOC_MAX_TILES;
+  const DevMesh m = dev_mesh(mesh);
+  hipLaunchKernelGGL(k_classify, dim3(nt), dim3(TBLK), 0, s, C, m, list, count);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, (const int *)count, nt, offs);
+  const int per_block = method == 1 ? TBLK / 8 : TBLK;
+  const int L = 2 * (mesh.nx + 2) + 2 * mesh.ny;
+  const dim3 grid(nt + (L + per_block - 1) / per_block);
+  hipLaunchKernelGGL(method == 1 ? k_recon<1> : k_recon<0>, grid, dim3(TBLK), 0, s, C, m, (const int *)list,
+                     (const int *)offs, onx, ony, oal);
+  const dim3 blk(64, 4), grd((mesh.nx + 2 + 63) / 64, (mesh.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(k_expand_segments, grd, blk, 0, s, C, m, (const double *)onx, (const double *)ony,
+                     (const double *)oal, onx, ony, oal);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }

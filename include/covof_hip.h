@@ -111,6 +111,22 @@ int covof_geom_batch_d(covof_ctx *ctx, int n, const double *xmin, const double *
 int covof_reconstruct_interface_d(covof_ctx *ctx, const char *method, const double *C,
                                   double *seg_nx, double *seg_ny, double *seg_alpha);
 
+/* Allocation-free variant of covof_reconstruct_interface_d, with no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  It needs no
+ * context: the mesh comes as covof_ctx_create's arguments, the work is queued
+ * on `stream` (a hipStream_t, NULL = the null stream) of the current device,
+ * and it returns once queued.  Its only scratch (the per-tile lists of mixed
+ * cells) is a static 4.2 MB device array, one per GPU, held until the process
+ * ends; the segments are written straight into seg_nx, seg_ny, seg_alpha
+ * (distinct from C and from each other).  The mesh may span at most 1024
+ * tiles of 64 x 16 cells (ceil(nx/64) * ceil(ny/16) <= 1024: 1024 x 1024,
+ * 2048 x 512, ...); larger ones return COVOF_BAD_ARGUMENT.  Calls that
+ * overlap on one GPU share the scratch and race: serialise them. */
+int covof_reconstruct_interface_noalloc_d(int nx, int ny, double lx, double ly, const int *bc, const char *method,
+                                          const double *C, double *seg_nx, double *seg_ny, double *seg_alpha,
+                                          void *stream);
+
 /* --- VOF transport (src/module_interface_vof.f90) ---------------------------- */
 
 /* advect_vof(fields, mesh, cfg, dt) (:792-825) with advection_method
