@@ -19,5 +19,5 @@ from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BI
                    PRECOND_BLOCK_TRIANGULAR, SOLVER_MONOLITHIC, SOLVER_STANDARD_PROJECTION, CURVATURE_NORMALS,
                    CURVATURE_HEIGHT_FUNCTION, REDUCTION_TREE, REDUCTION_REFERENCE, CovofError)
 from .context import Context, SolveResult  # noqa: F401
-from .autograd import PressureSolve, pressure_solve  # noqa: F401
+from .autograd import PressureSolve, Projection, pressure_solve, projection  # noqa: F401
 from .sp_run import StandardProjectionRun  # noqa: F401

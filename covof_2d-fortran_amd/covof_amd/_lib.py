@@ -75,6 +75,8 @@ SIGNATURES = {
     "covof_sp_correct_d": (_I, [_P, _P, _P, _P, _D, _P, _P]),
     "covof_sp_correct": (_I, [_P, _DP, _DP, _DP, _D, _DP, _DP]),
     "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
+    "covof_sp_pressure_d": (_I, [_P, _P, _P]),
+    "covof_projection_adjoint_d": (_I, [_P, _P, _P, _P, _D] + [_P] * 7 + [_I, _D, _I, _I, _IP, _DP]),
     "covof_momentum_size": (_I, [_P, _I, _IP, _IP, _IP]),
     "covof_momentum_set_fields_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D]),
     "covof_momentum_diag_d": (_I, [_P, _P]),

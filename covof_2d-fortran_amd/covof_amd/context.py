@@ -295,6 +295,35 @@ class Context:
         L.check(st, "covof_poisson_adjoint_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return g_b, g_lr, SolveResult(it.value, rr.value, st)
 
+    def sp_pressure(self, x, p):
+        """p(1:nx,1:ny) = x - mean(x) with the periodic halo wrap, the end of
+        the SP Poisson block (src/module_simulation.f90:2577-2588)."""
+        self._work(x, "x")
+        self._field(p, "p")
+        self._stream()
+        L.check(self._lib.covof_sp_pressure_d(self._h, _ptr(x), _ptr(p)), "covof_sp_pressure_d")
+
+    def projection_adjoint(self, rho, rho_u, rho_v, dt, x, w_u, w_v, w_p=None, precond=L.PRECOND_JACOBI,
+                           tol=1e-12, max_iter=200000, precision=L.PRECISION_FP64):
+        """dJ/du*, dJ/dv* (halo fields) and dJ/d(log rho) (work array) of the
+        whole projection (divergence, solve, mean removal, correction) for the
+        objective's gradients w_u, w_v, w_p at the forward's solution x; the
+        operator must be the forward's (no reference counterpart)."""
+        for t, n in ((rho, "rho"), (rho_u, "rho_u"), (rho_v, "rho_v"), (w_u, "w_u"), (w_v, "w_v")):
+            self._field(t, n)
+        if w_p is not None:
+            self._field(w_p, "w_p")
+        self._work(x, "x")
+        du, dv, dlr = self.new_field(), self.new_field(), self.new_work()
+        it, rr = ct.c_int(0), ct.c_double(0.0)
+        self._stream()
+        st = self._lib.covof_projection_adjoint_d(self._h, _ptr(rho), _ptr(rho_u), _ptr(rho_v), float(dt), _ptr(x),
+                                                  _ptr(w_u), _ptr(w_v), _ptr(w_p), _ptr(du), _ptr(dv), _ptr(dlr),
+                                                  int(precond), float(tol), int(max_iter), int(precision),
+                                                  ct.byref(it), ct.byref(rr))
+        L.check(st, "covof_projection_adjoint_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
+        return du, dv, dlr, SolveResult(it.value, rr.value, st)
+
     # ------------------------------------------------- SP velocity predictor
     MOMENTUM_FIELDS = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",
                        "fx_sigma", "fy_sigma")

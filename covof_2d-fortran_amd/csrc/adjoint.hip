@@ -106,6 +106,176 @@ __global__ void k_cell_grad(int nx, int ny, int perx, int pery, const double *__
   out[(long)(j - 1) * nx + (i - 1)] = s * rho[(long)j * (nx + 2) + i];
 }
 
+// ---------------------------------------------- adjoint of the projection
+//
+// Forward (src/module_simulation.f90:2477-2654): b = -div(u*) (:2485-2487),
+// A x = b, p = x - mean(x) (:2577-2585), and the correction of every
+// non-wall face, u = u* - (dt/rho_u) (p_e - p_w)/dx (:2590-2654; v alike),
+// whose pressure indices wrap on periodic sides and repeat the edge cell
+// otherwise (a zero gradient).  For J(u, v, p) with w_u = dJ/du, w_v = dJ/dv,
+// w_p = dJ/dp:
+//   g = w_p + sum over corrected faces of the gradient's adjoint (per cell)
+//   A lam = g - mean(g)             (the mean removal is a symmetric projector)
+//   dJ/du* = w_u + dy (lam_E - lam_W) ... the divergence's transpose,
+//   dJ/drho_f = w_f dt grad_f / rho_f^2 (the correction) + the operator's
+//   face sensitivity, each credited half to the two cells of the face mean.
+
+// The correction's reading of face f of row j (k_correct's rule): whether it
+// is corrected, its west / east pressure cells, dt/(rho_f d).
+struct FaceCorr {
+  bool on;
+  int cw, ce;
+  double c;
+};
+__device__ __forceinline__ FaceCorr u_corr(int nx, int perx, int wallL, int wallR, const double *rho_u, int f,
+                                           int j, double dt, double dx) {
+  FaceCorr r{false, 0, 0, 0.0};
+  const int u_nx = perx ? nx : nx + 1;
+  if (f < 1 || f > u_nx) return r;
+  if ((f == 1 && !perx && wallL) || (f == u_nx && !perx && wallR)) return r;
+  double rf = rho_u[(long)j * (nx + 2) + f];
+  if (rf < 1.0e-12) rf = 1.0;
+  r.on = true;
+  r.cw = f > 1 ? f - 1 : (perx ? nx : 1);
+  r.ce = f <= nx ? f : (perx ? 1 : nx);
+  r.c = dt / rf / dx;
+  return r;
+}
+
+// g(i,j) = w_p(i,j) + the correction's pressure adjoint (cells of the work
+// layout), per-block partial sums of g in part[blockIdx].
+__global__ void __launch_bounds__(256) k_proj_adj_g(int nx, int ny, int perx, int pery, int wallL, int wallR,
+                                                    int wallB, int wallT, double dx, double dy, double dt,
+                                                    const double *__restrict__ rho_u,
+                                                    const double *__restrict__ rho_v,
+                                                    const double *__restrict__ w_u, const double *__restrict__ w_v,
+                                                    const double *__restrict__ w_p, double *__restrict__ g,
+                                                    double *__restrict__ part) {
+  __shared__ double sh[4];
+  const long n = (long)nx * ny;
+  double acc = 0.0;
+  const int u_nx = perx ? nx : nx + 1, v_ny = pery ? ny : ny + 1;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int j = (int)(t / nx) + 1, i = (int)(t - (long)(j - 1) * nx) + 1;
+    const long h = (long)j * (nx + 2) + i;
+    double s = 0.0;
+    if (w_p) {
+      // dJ/dp of the cell and of its periodic halo copies (the wrap copies
+      // x first, rows 0..ny+1, then y, columns 0..nx+1)
+      auto Wp = [&](int a, int b) { return w_p[(long)b * (nx + 2) + a]; };
+      auto col = [&](int a) {  // p(a, j) and the y-wrap copies of it
+        double t = Wp(a, j);
+        if (pery && j == ny) t += Wp(a, 0);
+        if (pery && j == 1) t += Wp(a, ny + 1);
+        return t;
+      };
+      s = col(i);
+      if (perx && i == nx) s += col(0);
+      if (perx && i == 1) s += col(nx + 1);
+    }
+    // x faces that can read p(i, j): i, i + 1, the periodic wrap face 1 (cell
+    // nx) and the last corrected face (cell 1); each counted once
+    int fl[4] = {i, i + 1, i == nx ? 1 : -1, i == 1 ? u_nx : -1};
+    for (int q = 0; q < 4; ++q) {
+      const int f = fl[q];
+      bool dup = f < 0;
+      for (int r = 0; r < q; ++r) dup = dup || fl[r] == f;
+      if (dup) continue;
+      const FaceCorr fc = u_corr(nx, perx, wallL, wallR, rho_u, f, j, dt, dx);
+      if (!fc.on || fc.cw == fc.ce) continue;  // an edge face reads one cell twice: no gradient
+      const double wv = w_u[(long)j * (nx + 2) + f];
+      if (fc.cw == i) s += fc.c * wv;
+      if (fc.ce == i) s -= fc.c * wv;
+    }
+    int fr[4] = {j, j + 1, j == ny ? 1 : -1, j == 1 ? v_ny : -1};
+    for (int q = 0; q < 4; ++q) {
+      const int f = fr[q];
+      bool dup = f < 0;
+      for (int r = 0; r < q; ++r) dup = dup || fr[r] == f;
+      if (dup) continue;
+      if (f < 1 || f > v_ny) continue;
+      if ((f == 1 && !pery && wallB) || (f == v_ny && !pery && wallT)) continue;
+      double rf = rho_v[(long)f * (nx + 2) + i];
+      if (rf < 1.0e-12) rf = 1.0;
+      const int cs = f > 1 ? f - 1 : (pery ? ny : 1);
+      const int cn = f <= ny ? f : (pery ? 1 : ny);
+      if (cs == cn) continue;
+      const double c = dt / rf / dy;
+      const double wv = w_v[(long)f * (nx + 2) + i];
+      if (cs == j) s += c * wv;
+      if (cn == j) s -= c * wv;
+    }
+    g[t] = s;
+    acc += s;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// g -= mean(g) (every block sums the nblk partials in the same order)
+__global__ void __launch_bounds__(256) k_proj_adj_center(long n, int nblk, const double *__restrict__ part,
+                                                         double *__restrict__ g) {
+  __shared__ double tot;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[b];
+    tot = s / (double)n;
+  }
+  __syncthreads();
+  const double mean = tot;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) g[t] = g[t] - mean;
+}
+
+// dJ/du*, dJ/dv* (halo layout): the direct term and the divergence's
+// transpose; the correction's face-density terms added to the operator's
+// face sensitivities (kx / ky layout, half per cell of the face mean).
+__global__ void k_proj_adj_vel(int nx, int ny, int perx, int pery, int wallL, int wallR, int wallB, int wallT,
+                               double dx, double dy, double dt, const double *__restrict__ rho_u,
+                               const double *__restrict__ rho_v, const double *__restrict__ x,
+                               const double *__restrict__ lam, const double *__restrict__ w_u,
+                               const double *__restrict__ w_v, double *__restrict__ du, double *__restrict__ dv,
+                               double *__restrict__ gx, double *__restrict__ gy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // 0..nx+1
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..ny+1
+  if (i > nx + 1 || j > ny + 1) return;
+  const long h = (long)j * (nx + 2) + i;
+  auto W = [&](int a, int c) { return (long)(c - 1) * nx + (a - 1); };
+  double su = w_u[h], sv = w_v[h];
+  if (j >= 1 && j <= ny && i >= 1) {
+    if (i <= nx) su += dy * lam[W(i, j)];
+    if (i >= 2) su -= dy * lam[W(i - 1, j)];
+    // the correction's dependence on rho_u of this face
+    const FaceCorr fc = u_corr(nx, perx, wallL, wallR, rho_u, i, j, dt, dx);
+    if (fc.on && i <= nx + 1) {
+      const double rf = rho_u[h];
+      if (rf >= 1.0e-12) {
+        const double grad = (x[W(fc.ce, j)] - x[W(fc.cw, j)]) / dx;
+        gx[kxi(nx, i, j)] += 0.5 * (w_u[h] * (dt * grad) / (rf * rf));
+      }
+    }
+  }
+  if (i >= 1 && i <= nx && j >= 1) {
+    if (j <= ny) sv += dx * lam[W(i, j)];
+    if (j >= 2 && j - 1 <= ny) sv -= dx * lam[W(i, j - 1)];
+    const int v_ny = pery ? ny : ny + 1;
+    const bool on = j <= v_ny && !((j == 1 && !pery && wallB) || (j == v_ny && !pery && wallT));
+    if (on) {
+      const double rf = rho_v[h];
+      if (rf >= 1.0e-12) {
+        const int cs = j > 1 ? j - 1 : (pery ? ny : 1);
+        const int cn = j <= ny ? j : (pery ? 1 : ny);
+        const double grad = (x[W(i, cn)] - x[W(i, cs)]) / dy;
+        gy[kyi(nx, i, j)] += 0.5 * (w_v[h] * (dt * grad) / (rf * rf));
+      }
+    }
+  }
+  du[h] = su;
+  dv[h] = sv;
+}
+
 __global__ void k_zero64(long n, double *x) {
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] = 0.0;
 }
@@ -132,6 +302,45 @@ int poisson_adjoint(Ctx &c, const double *rho, double dt, const double *x, const
   const dim3 grd2((m.nx + 63) / 64, (m.ny + 3) / 4);
   hipLaunchKernelGGL(k_cell_grad, grd2, blk, 0, c.stream, m.nx, m.ny, m.perx, m.pery, c.ps.fx, c.ps.fy,
                      rho, dJ_dlogrho);
+  COVOF_CHECK_LAUNCH();
+  return st;
+}
+
+int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const double *rho_v, double dt,
+                       const double *x, const double *w_u, const double *w_v, const double *w_p, double *dJ_du,
+                       double *dJ_dv, double *dJ_dlogrho, int precond, double tol, int maxit, int precision,
+                       int *iters, double *relres) {
+  if (!c.ps.assembled) return COVOF_BAD_ARGUMENT;
+  const Mesh &m = c.m;
+  const long n = (long)m.nx * m.ny;
+  Poisson &P = c.ps;
+  const int wl = m.bc[0] == BC_DIRICHLET || m.bc[0] == BC_FREESLIP;
+  const int wr = m.bc[1] == BC_DIRICHLET || m.bc[1] == BC_FREESLIP;
+  const int wb = m.bc[2] == BC_DIRICHLET || m.bc[2] == BC_FREESLIP;
+  const int wt = m.bc[3] == BC_DIRICHLET || m.bc[3] == BC_FREESLIP;
+  // g -> t4 (mean-free), lam -> t5 (the CG solves leave both alone); 256
+  // partial sums past the first two partial buffers (the allocation holds
+  // max(12 nblocks, 4608) values)
+  double *g = P.t4, *lam = P.t5, *part = P.part + 2L * P.nblocks * 4;
+  const int nblk = 256;
+  hipLaunchKernelGGL(k_proj_adj_g, dim3(nblk), dim3(256), 0, c.stream, m.nx, m.ny, m.perx, m.pery, wl, wr, wb, wt,
+                     m.dx, m.dy, dt, rho_u, rho_v, w_u, w_v, w_p, g, part);
+  hipLaunchKernelGGL(k_proj_adj_center, dim3(1024), dim3(256), 0, c.stream, n, nblk, (const double *)part, g);
+  hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, c.stream, n, lam);
+  COVOF_CHECK_LAUNCH();
+  const int st = poisson_solve(c, g, lam, COVOF_LINEAR_CG, precond, tol, maxit, precision, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT || st == COVOF_HIP_ERROR) return st;
+  const dim3 blk(64, 4), grd((m.nx + 1 + 63) / 64, (m.ny + 1 + 3) / 4);
+  // the operator's face sensitivities (lam, x), then the correction's terms
+  hipLaunchKernelGGL(k_face_sens, grd, blk, 0, c.stream, m.nx, m.ny, m.bc[0] == BC_PERIODIC,
+                     m.bc[1] == BC_PERIODIC, m.bc[2] == BC_PERIODIC, m.bc[3] == BC_PERIODIC,
+                     P.kx, P.ky, rho, x, (const double *)lam, P.fx, P.fy);
+  const dim3 grdh((m.nx + 2 + 63) / 64, (m.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(k_proj_adj_vel, grdh, blk, 0, c.stream, m.nx, m.ny, m.perx, m.pery, wl, wr, wb, wt, m.dx, m.dy,
+                     dt, rho_u, rho_v, x, (const double *)lam, w_u, w_v, dJ_du, dJ_dv, P.fx, P.fy);
+  const dim3 grd2((m.nx + 63) / 64, (m.ny + 3) / 4);
+  hipLaunchKernelGGL(k_cell_grad, grd2, blk, 0, c.stream, m.nx, m.ny, m.perx, m.pery, P.fx, P.fy, rho,
+                     dJ_dlogrho);
   COVOF_CHECK_LAUNCH();
   return st;
 }

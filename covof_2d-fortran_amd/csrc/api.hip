@@ -39,6 +39,10 @@ int sp_correct(Ctx &c, const double *p, const double *rho_u, const double *rho_v
 int poisson_adjoint(Ctx &c, const double *rho, double dt, const double *x, const double *w,
                     double *dJ_db, double *dJ_dlogrho, int lin_solver, int precond, double tol,
                     int maxit, int precision, int *iters, double *relres);
+int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const double *rho_v, double dt,
+                       const double *x, const double *w_u, const double *w_v, const double *w_p, double *dJ_du,
+                       double *dJ_dv, double *dJ_dlogrho, int precond, double tol, int maxit, int precision,
+                       int *iters, double *relres);
 }  // namespace covof
 
 using covof::Ctx;
@@ -472,6 +476,26 @@ extern "C" int covof_poisson_adjoint_d(covof_ctx *h, const double *rho, double d
   const int st = covof::poisson_adjoint(h->c, rho, dt, x, w, dJ_db, dJ_dlogrho, lin_solver, precond, tol,
                                         maxit, precision, iters, relres);
   if (st == COVOF_BAD_ARGUMENT) return fail(st, "unsupported solver combination");
+  return hip_status(st);
+}
+
+extern "C" int covof_sp_pressure_d(covof_ctx *h, const double *x, double *p) {
+  if (!h || !x || !p) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  return hip_status(covof::sp_pressure(h->c, x, p));
+}
+
+extern "C" int covof_projection_adjoint_d(covof_ctx *h, const double *rho, const double *rho_u,
+                                          const double *rho_v, double dt, const double *x, const double *w_u,
+                                          const double *w_v, const double *w_p, double *dJ_du, double *dJ_dv,
+                                          double *dJ_dlogrho, int precond, double tol, int maxit, int precision,
+                                          int *iters, double *relres) {
+  if (!h || !rho || !rho_u || !rho_v || !x || !w_u || !w_v || !dJ_du || !dJ_dv || !dJ_dlogrho)
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (dJ_du == w_u || dJ_dv == w_v || dJ_du == dJ_dv)
+    return fail(COVOF_BAD_ARGUMENT, "covof_projection_adjoint_d: outputs must not alias the objective's gradients");
+  const int st = covof::projection_adjoint(h->c, rho, rho_u, rho_v, dt, x, w_u, w_v, w_p, dJ_du, dJ_dv, dJ_dlogrho,
+                                           precond, tol, maxit, precision, iters, relres);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "operator not assembled or unsupported preconditioner/precision");
   return hip_status(st);
 }
 

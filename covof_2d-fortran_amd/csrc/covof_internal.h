@@ -234,6 +234,10 @@ void surface_free(Ctx &c);
 int poisson_alloc(Ctx &c);
 void poisson_free(Ctx &c);
 int poisson_matvec(Ctx &c, const double *x, double *y);
+// p(1:nx,1:ny) = x - mean(x) with the periodic halo wrap (end of the SP Poisson block)
+int sp_pressure(Ctx &c, const double *x, double *p);
+int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol, int maxit,
+                  int precision, int *iters, double *relres);
 int poisson_precond(Ctx &c, int precond, const double *r, double *z);
 // krylov.hip: BiCGSTAB(2) and CG of module_linear_solvers on any operator
 int krylov_bicgstab2(Ctx &c, LinOp &op, const double *b, double *x, double *const w[9], double tol,

@@ -1827,23 +1827,34 @@ int sp_poisson(Ctx &c, const double *u, const double *v, const double *rho_u, co
   COVOF_CHECK_LAUNCH();
   const int solve_st = poisson_solve(c, P.b, P.x, lin_solver, precond, tol, maxit, precision, iters, relres);
   if (solve_st == COVOF_BAD_ARGUMENT || solve_st == COVOF_HIP_ERROR) return solve_st;
+  const int pst = sp_pressure(c, P.x, p);
+  if (pst) return pst;
+  return solve_st;
+}
+
+// p(1:nx,1:ny) = x - mean(x) and the periodic halo wrap (the end of the
+// Poisson block, src/module_simulation.f90:2577-2588).  x: (nx, ny).
+int sp_pressure(Ctx &c, const double *x, double *p) {
+  Poisson &P = c.ps;
+  const Mesh &m = c.m;
+  const dim3 blk(64, 4), grd((m.nx + 63) / 64, (m.ny + 3) / 4);
   const DevOp o = dev_op(m);
   if (c.ref_order) {  // sum(p_sol) in the reference's order
     SerialSet s{};
     s.nq = 1;
     s.nzero = 1;
-    s.a[0] = P.x;
+    s.a[0] = x;
     s.dst[0] = P.scal + S_DOT0;
     serial_sums(c, (long)m.nx * m.ny, s, S_ZERO, S_ZERO);
   } else {
-    hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, P.x, part_buf(c, 2));
+    hipLaunchKernelGGL(k_sum, red_grid(m), dim3(BLK), 0, c.stream, o, x, part_buf(c, 2));
     hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)part_buf(c, 2), P.nblocks,
                        P.scal, (int)S_DOT0);
   }
-  hipLaunchKernelGGL(k_write_pressure, grd, blk, 0, c.stream, o, P.x, P.scal, S_DOT0, p);
+  hipLaunchKernelGGL(k_write_pressure, grd, blk, 0, c.stream, o, x, P.scal, S_DOT0, p);
   hipLaunchKernelGGL(k_wrap_halo, dim3(1), dim3(1024), 0, c.stream, m.nx, m.ny, m.perx, m.pery, p);
   COVOF_CHECK_LAUNCH();
-  return solve_st;
+  return COVOF_OK;
 }
 
 int sp_correct(Ctx &c, const double *p, const double *rho_u, const double *rho_v, double dt,

@@ -395,6 +395,29 @@ int covof_poisson_adjoint_d(covof_ctx *ctx, const double *rho, double dt, const 
                             int precond, double tol, int maxit, int precision, int *iters,
                             double *relres);
 
+/* p(1:nx,1:ny) = x - mean(x) with the periodic halo wrap: the end of the SP
+ * Poisson block (src/module_simulation.f90:2577-2588) on a solution x
+ * (nx*ny) of covof_poisson_solve_d; p in halo layout. */
+int covof_sp_pressure_d(covof_ctx *ctx, const double *x, double *p);
+
+/* Adjoint of the whole pressure projection of the standard projection (no
+ * reference counterpart).  Forward (src/module_simulation.f90:2477-2654):
+ * b = -div(u*) with the operator A assembled from rho_u, rho_v, dt; A x = b;
+ * p = x - mean(x); u = u* - (dt/rho_u) dp/dx, v = v* - (dt/rho_v) dp/dy on
+ * the non-wall faces.  The operator must be the forward's (assembled), x its
+ * solution (nx*ny, before the mean removal), rho the cell density (halo
+ * layout) whose arithmetic face means are rho_u, rho_v.  Given a scalar
+ * objective's gradients w_u = dJ/du, w_v = dJ/dv (halo layout) and
+ * w_p = dJ/dp (halo layout, NULL: zero; its periodic halo entries count as
+ * the copies they are, the others are constants), returns dJ/du*, dJ/dv* (halo layout)
+ * and dJ/d(log rho) (nx*ny) through the face means, with halo densities the
+ * copies of interior cells (periodic wrap, zero gradient).  One transpose CG
+ * solve (A is symmetric) with the given preconditioner and precision. */
+int covof_projection_adjoint_d(covof_ctx *ctx, const double *rho, const double *rho_u, const double *rho_v,
+                               double dt, const double *x, const double *w_u, const double *w_v,
+                               const double *w_p, double *dJ_du, double *dJ_dv, double *dJ_dlogrho,
+                               int precond, double tol, int maxit, int precision, int *iters, double *relres);
+
 #ifdef __cplusplus
 }
 #endif
