@@ -139,6 +139,12 @@ double orc_compute_alpha(double xmin, double xmax, double ymin, double ymax,
   return 0.5 * (lo + hi);
 }
 
+/* Threads of the cell loops of the reconstruction and the WY pass (1: serial,
+ * as the tests run them; the loops' iterations are independent, so the
+ * result is the same for any count).  bench.py's CPU baseline raises it. */
+static int g_threads = 1;
+void orc_set_threads(int t) { g_threads = t > 1 ? t : 1; }
+
 static int interior(const orc_mesh *m, int i, int j) {
   return i >= 1 && i <= m->nx && j >= 1 && j <= m->ny;
 }
@@ -227,6 +233,7 @@ static void elvira_cell(const orc_mesh *m, const double *C, int i, int j,
 
 void orc_reconstruct(const orc_mesh *m, int method, const double *C,
                      double *snx, double *sny, double *sal) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads) if (g_threads > 1)
   for (int j = 0; j <= m->ny + 1; ++j)
     for (int i = 0; i <= m->nx + 1; ++i) {
       const long k = H(m, i, j);
@@ -333,6 +340,7 @@ static void wy_pass(const orc_mesh *m, int method, int dir, int clamp, const dou
   orc_reconstruct_interface(m, method, src, snx, sny, sal);
   memset(flux, 0, n * sizeof(double));
   if (dir == 0) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx + 1; ++i) {
         const double uf = u[H(m, i, j)];
@@ -355,6 +363,7 @@ static void wy_pass(const orc_mesh *m, int method, int dir, int clamp, const dou
           flux[H(m, nx + 1, j)] = flux[H(m, 1, j)];
       }
     memcpy(dst, src, n * sizeof(double));
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
         double c = src[H(m, i, j)] - (flux[H(m, i + 1, j)] - flux[H(m, i, j)]) / area +
@@ -363,6 +372,7 @@ static void wy_pass(const orc_mesh *m, int method, int dir, int clamp, const dou
         dst[H(m, i, j)] = c;
       }
   } else {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j <= ny + 1; ++j)
       for (int i = 1; i <= nx; ++i) {
         const double vf = v[H(m, i, j)];
@@ -385,6 +395,7 @@ static void wy_pass(const orc_mesh *m, int method, int dir, int clamp, const dou
           flux[H(m, i, ny + 1)] = flux[H(m, i, 1)];
       }
     memcpy(dst, src, n * sizeof(double));
+#pragma omp parallel for num_threads(g_threads) if (g_threads > 1)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx; ++i) {
         double c = src[H(m, i, j)] - (flux[H(m, i, j + 1)] - flux[H(m, i, j)]) / area +
@@ -1029,6 +1040,69 @@ double orc_bench_jacobi_pcg(const orc_mesh *m, const double *aP, const double *a
     (void)rr;
   }
   const double t1 = omp_get_wtime();
+  free(x); free(r); free(d); free(q);
+  return t1 - t0;
+}
+
+/* CPU baseline: the SP PCG loop with Jacobi (z = r/aP) from x = 0 to
+ * ||r||/||b|| <= tol, OpenMP over `threads` (reductions in OpenMP's order).
+ * Returns wall seconds; *iters and *relres as the loop ends. */
+double orc_bench_jacobi_pcg_solve(const orc_mesh *m, const double *aP, const double *aW,
+                                  const double *aE, const double *aS, const double *aN,
+                                  const double *b, double tol, int maxit, int threads, int *iters,
+                                  double *relres) {
+  const int nx = m->nx, ny = m->ny;
+  const long n = (long)nx * ny;
+  double *x = calloc(n, sizeof(double)), *r = malloc(n * sizeof(double));
+  double *d = malloc(n * sizeof(double)), *q = malloc(n * sizeof(double));
+  omp_set_num_threads(threads);
+  const double t0 = omp_get_wtime();
+  double rz = 0.0, bb = 0.0;
+#pragma omp parallel for reduction(+ : rz, bb)
+  for (long k = 0; k < n; ++k) {
+    r[k] = b[k];
+    d[k] = r[k] / aP[k];
+    rz += r[k] * d[k];
+    bb += b[k] * b[k];
+  }
+  double bn = sqrt(bb);
+  if (bn < 1.0e-30) bn = 1.0;
+  double rel = 1.0;
+  int it = 0;
+  while (it < maxit) {
+    ++it;
+    double dq = 0.0;
+#pragma omp parallel for reduction(+ : dq)
+    for (int j = 1; j <= ny; ++j)
+      for (int i = 1; i <= nx; ++i) {
+        const long k = W(m, i, j);
+        double val = aP[k] * d[k];
+        if (i > 1) val -= aW[k] * d[k - 1];
+        if (i < nx) val -= aE[k] * d[k + 1];
+        if (j > 1) val -= aS[k] * d[k - nx];
+        if (j < ny) val -= aN[k] * d[k + nx];
+        q[k] = val;
+        dq += d[k] * val;
+      }
+    const double alpha = rz / dq;
+    double rz_new = 0.0, rr = 0.0;
+#pragma omp parallel for reduction(+ : rz_new, rr)
+    for (long k = 0; k < n; ++k) {
+      x[k] += alpha * d[k];
+      r[k] -= alpha * q[k];
+      rz_new += r[k] * (r[k] / aP[k]);
+      rr += r[k] * r[k];
+    }
+    rel = sqrt(rr) / bn;
+    if (rel <= tol) break;
+    const double beta = rz_new / rz;
+    rz = rz_new;
+#pragma omp parallel for
+    for (long k = 0; k < n; ++k) d[k] = r[k] / aP[k] + beta * d[k];
+  }
+  const double t1 = omp_get_wtime();
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
   free(x); free(r); free(d); free(q);
   return t1 - t0;
 }

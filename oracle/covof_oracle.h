@@ -169,6 +169,11 @@ double orc_dot(const double *x, const double *y, long n);
 
 /* Timed CPU baseline: `iters` Jacobi-PCG iterations on the face-form
  * operator with OpenMP over `threads` threads (FP64). Returns seconds. */
+void orc_set_threads(int threads);
+double orc_bench_jacobi_pcg_solve(const orc_mesh *m, const double *aP, const double *aW,
+                                  const double *aE, const double *aS, const double *aN,
+                                  const double *b, double tol, int maxit, int threads, int *iters,
+                                  double *relres);
 double orc_bench_jacobi_pcg(const orc_mesh *m, const double *aP, const double *aW,
                             const double *aE, const double *aS, const double *aN,
                             const double *b, int iters, int threads);

@@ -85,6 +85,8 @@ def lib():
             "orc_sp_correct": (None, [M, _D, _D, _D, d, _D, _D]),
             "orc_dot": (d, [_D, _D, ct.c_long]),
             "orc_bench_jacobi_pcg": (d, [M] + [_D] * 6 + [i, i]),
+            "orc_bench_jacobi_pcg_solve": (d, [M] + [_D] * 6 + [d, i, i, ct.POINTER(ct.c_int), _D]),
+            "orc_set_threads": (None, [i]),
             "orc_history": (None, [M] + [_D] * 7),
             "orc_velocity_bc": (None, [M, ct.POINTER(MomCfg), _D, _D, i]),
             "orc_mom_sizes": (None, [M, ip, ip]),
@@ -247,6 +249,19 @@ def sp_correct(m: Mesh, p, rho_u, rho_v, dt, u, v):
     v = np.array(v, dtype=np.float64, copy=True)
     lib().orc_sp_correct(ct.byref(m), _p(p), _p(rho_u), _p(rho_v), dt, _p(u), _p(v))
     return u, v
+
+
+def bench_jacobi_pcg_solve(m: Mesh, co, b, tol: float, maxit: int, threads: int):
+    """(seconds, iterations, relres) of a Jacobi-PCG solve from zero to tol on the host cores."""
+    it, rr = ct.c_int(0), ct.c_double(0.0)
+    sec = lib().orc_bench_jacobi_pcg_solve(ct.byref(m), *_co(co), _p(np.ascontiguousarray(b)), tol, maxit, threads,
+                                           ct.byref(it), ct.byref(rr))
+    return sec, it.value, rr.value
+
+
+def set_threads(threads: int) -> None:
+    """Threads of the reconstruction / WY cell loops (1 = serial)."""
+    lib().orc_set_threads(int(threads))
 
 
 def bench_jacobi_pcg(m: Mesh, co, b, iters: int, threads: int) -> float:

@@ -556,3 +556,38 @@ def test_momentum_solve_vs_dense(orc, solver):
     x, it, _ = M.solve(b, np.zeros(M.n), solver, 1, 1e-13, 4000)
     xd = np.linalg.solve(A, b)
     assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max(), it
+
+
+def test_threaded_wy_step_is_the_serial_one(orc):
+    """The CPU baseline runs the oracle's reconstruction / WY cell loops on
+    several threads: their iterations are independent, so the step is the
+    serial one bit for bit."""
+    from covof_amd import workloads as wl
+    n = 96
+    C = wl.ellipses(n, n, 10, 3.0, 14.0, seed=5)
+    wl.fill_halo_periodic(C, n, n)
+    u, v = wl.stream_velocity(n, n, 6, seed=6, periodic=True)
+    dt = 0.25 / n / max(abs(u).max(), abs(v).max())
+    m = orc.mesh(n, n, bc=(3, 3, 3, 3))
+    out = []
+    for threads in (1, 4):
+        orc.set_threads(threads)
+        Cw, sc = C.copy(), 0
+        for _ in range(3):
+            Cw, sc = orc.wy_step(m, orc.ELVIRA, Cw, u, v, dt, sc)
+        out.append(Cw)
+    orc.set_threads(1)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_cpu_baseline_solve_converges(orc):
+    """The timed CPU baseline solve (OpenMP Jacobi-PCG) reaches the tolerance
+    with the iteration count of the serial SP loop within a few."""
+    from covof_amd import workloads as wl
+    n = 64
+    case = wl.poisson_case(n, seed=0, n_circles=4, rmin=4, rmax=12)
+    m = orc.mesh(n, n, bc=case["bc"])
+    co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
+    sec, it, rr = orc.bench_jacobi_pcg_solve(m, co, co["b"], 1e-10, 100000, 2)
+    _, it_s, _ = orc.sp_pcg(m, 1, co, co["b"], np.zeros((n, n)), tol=1e-10, maxit=100000)
+    assert rr <= 1e-10 and abs(it - it_s) <= max(3, it_s // 50), (it, it_s, rr)
