@@ -39,6 +39,15 @@ class MomentumCfg(ct.Structure):
                 ("gx", ct.c_double), ("gy", ct.c_double), ("bc_u", ct.c_double * 4), ("bc_v", ct.c_double * 4),
                 ("solver_type", ct.c_int)]
 
+class Obstacle(ct.Structure):
+    """covof_obstacle (include/covof_hip.h): config_t's obs_* fields."""
+    _fields_ = [("enabled", ct.c_int), ("type", ct.c_int), ("x0", ct.c_double), ("y0", ct.c_double),
+                ("x1", ct.c_double), ("y1", ct.c_double), ("xc", ct.c_double), ("yc", ct.c_double),
+                ("r", ct.c_double), ("perm", ct.c_double)]
+
+
+OBS_BOX, OBS_CYLINDER = 1, 2
+
 # every symbol the header declares: name -> (restype, argtypes)
 _P = ct.c_void_p
 _I = ct.c_int
@@ -52,6 +61,9 @@ SIGNATURES = {
     "covof_ctx_set_stream": (_I, [_P, _P]),
     "covof_ctx_synchronize": (_I, [_P]),
     "covof_ctx_set_reduction_order": (_I, [_P, _I]),
+    "covof_ctx_set_obstacle": (_I, [_P, ct.c_void_p]),
+    "covof_ctx_set_vof_top_profile": (_I, [_P, _S, _D, _D]),
+    "covof_update_properties_cs_d": (_I, [_P, _P, _P, _D, _D, _D, _D] + [_P] * 5),
     "covof_last_error": (_S, []),
     "covof_geom_batch_d": (_I, [_P, _I] + [_P] * 10),
     "covof_reconstruct_interface_d": (_I, [_P, _S, _P, _P, _P, _P]),

@@ -12,14 +12,16 @@ GPU (``StandardProjectionRun``) and the reference's output files
 (``covof_amd.output``).
 
 Supported: mode NAVIER_STOKES, solver STANDARD_PROJECTION, two-phase VOF with
-WEYMOUTH_YUE + PLIC / ELVIRA, no markers, no SVOF / diffusion smoothing, no
+WEYMOUTH_YUE + PLIC / ELVIRA (with SVOF / diffusion smoothing), no markers, no
 energy equation, incompressible, no obstacles, fixed dt, UNIFORM boundary
 profiles, the predictor's CG / BiCGSTAB(2) with NONE / JACOBI, and the
 initial conditions UNIFORM, DROPLET, CYLINDER and BUBBLE.  Anything else
 raises ``ValueError`` naming the key.  The Poisson block is the reference's
-(CG + SGS, 1e-7, 1000 iterations) unless ``poisson`` says otherwise.
+(CG + SGS, 1e-7, 1000 iterations) unless ``poisson`` says otherwise.  With
+``--reference-order`` the FP64 inner products are summed in the reference's
+order and the run reproduces the reference program's output files exactly.
 
-    python -m covof_amd.case case.json [--poisson-mg-mixed]
+    python -m covof_amd.case case.json [--poisson-mg-mixed] [--reference-order]
 """
 from __future__ import annotations
 
@@ -71,6 +73,11 @@ def load_config(path: str) -> dict:
                                      "NEWTON": L.CONV_LIN_NEWTON}.get(cl, L.CONV_LIN_PICARD)
     c["enable_svof_smoothing"] = bool(g("physics.enable_svof_smoothing", g("enable_svof_smoothing", False)))
     c["enable_diffusion_smoothing"] = bool(g("physics.enable_diffusion_smoothing", False))
+    # svof_filter_width: physics.svof_filter_width, else the top-level key when that left the default
+    # (src/module_config.f90:340-343); diffusion_smoothing_steps (:381)
+    fw = float(g("physics.svof_filter_width", 1.0))
+    c["svof_filter_width"] = float(g("svof_filter_width", 1.0)) if fw == 1.0 else fw
+    c["diffusion_smoothing_steps"] = int(g("physics.diffusion_smoothing_steps", 3))
     c["viscosity_model"] = g("physics.viscosity_model", "CONSTANT")
     c["rho1"] = float(g("physics.fluid1.rho", 1.0))
     c["mu1"] = float(g("physics.fluid1.mu", 1.0e-3))
@@ -140,8 +147,7 @@ def check_supported(c: dict) -> None:
         (c["enable_two_phase"], "physics.enable_two_phase"),
         (c["advection_method"] == "WEYMOUTH_YUE", "vof.advection_method"),
         (c["reconstruction_method"] in ("PLIC", "ELVIRA"), "vof.reconstruction_method"),
-        (not c["use_markers"], "markers"), (not c["enable_svof_smoothing"], "physics.enable_svof_smoothing"),
-        (not c["enable_diffusion_smoothing"], "physics.enable_diffusion_smoothing"),
+        (not c["use_markers"], "markers"),
         (c["viscosity_model"] == "CONSTANT", "physics.viscosity_model"),
         (not c["compressible"], "physics.compressible"), (not c["enable_ssp_rk3_momentum"], "ssp_rk3"),
         (not c["enable_energy"], "enable_energy"), (not c["adaptive_dt"], "time.adaptive_dt"),
@@ -225,8 +231,11 @@ def _vof_halo(C: np.ndarray, c: dict) -> None:
         C[ny + 1, 1:nx + 1] = C[ny, 1:nx + 1]
 
 
-def run_case(c: dict, poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), log=print):
-    """sim_init + sim_run of a supported case on the GPU; returns the run."""
+def run_case(c: dict, poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), log=print,
+             reduction=None):
+    """sim_init + sim_run of a supported case on the GPU; returns the run.
+    ``reduction``: the summation order of the inner products
+    (Context.set_reduction_order; None: the default tree sums)."""
     import torch
 
     from .context import Context
@@ -250,15 +259,18 @@ def run_case(c: dict, poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECIS
         reconstruction_method=c["reconstruction_method"],
         predictor=(c["linear_solver"], c["precond"], c["tol"], c["max_iter"]), poisson=poisson,
         convection=c["enable_convection"],
+        svof=c["svof_filter_width"] if c["enable_svof_smoothing"] else None,
+        diffusion=c["diffusion_smoothing_steps"] if c["enable_diffusion_smoothing"] else None, reduction=reduction,
         solver_type=L.SOLVER_MONOLITHIC if c["solver_type"] == "MONOLITHIC" else L.SOLVER_STANDARD_PROJECTION)
     f = run.f
     # the end of set_initial_conditions: CSF of the initial C; then sim_init's velocity BC
-    ctx.compute_curvature_and_csf(f["C"], run.sigma, enable=run.sigma > 0.0,
+    G = f.get("C_tilde", f.get("C_s"))  # the normals' field of compute_curvature_and_csf (:681-689)
+    ctx.compute_curvature_and_csf(f["C"], run.sigma, G=G, enable=run.sigma > 0.0,
                                   out={k: f[k] for k in ("curvature", "fx_sigma", "fy_sigma")})
     ctx.apply_velocity_bc(run.cfg(False), f["u"], f["v"])
     out_dir, case = c["output_dir"], c["case_name"]
     hist = HistoryWriter(ctx, out_dir, case)
-    fields = lambda: {k: f[k] for k in ("C", "p", "rho", "mu", "curvature", "u", "v")}  # noqa: E731
+    fields = lambda: {k: f[k] for k in ("C", "C_s", "p", "rho", "mu", "curvature", "u", "v") if k in f}  # noqa: E731
     write_step(ctx, out_dir, case, 0, fields())
     hist.write(0, 0.0, c["dt"], f["C"], f["u"], f["v"], f["p"], f["rho"])
     t, step, tf = 0.0, 0, c["t_final"]
@@ -294,7 +306,7 @@ def main(argv=None):
     c = load_config(argv[0])
     poisson = (L.LINEAR_CG, L.PRECOND_MG, 1e-10, 20000, L.PRECISION_MIXED) if "--poisson-mg-mixed" in argv else \
         (L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64)
-    run = run_case(c, poisson=poisson)
+    run = run_case(c, poisson=poisson, reduction=L.REDUCTION_REFERENCE if "--reference-order" in argv else None)
     print(f"{run.steps} steps in {run.wall_s:.2f} s; output in {os.path.abspath(c['output_dir'])}")
     return 0
 

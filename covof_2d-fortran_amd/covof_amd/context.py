@@ -157,18 +157,40 @@ class Context:
             self.step_count = sc.value
         return props if two_phase else None
 
-    def update_physical_properties(self, C, rho1, rho2, mu1, mu2, props=None):
+    def update_physical_properties(self, C, rho1, rho2, mu1, mu2, props=None, C_s=None):
         """update_physical_properties, two-phase incompressible branch
-        (src/module_interface_vof.f90:479-616); fixes C's physical corners in place."""
+        (src/module_interface_vof.f90:479-616); fixes C's physical corners in
+        place.  ``C_s``: the SVOF-smoothed field (cfg%enable_svof_smoothing):
+        rho, mu and mu_node then come from it (:540-545, :596-600)."""
         self._field(C, "C")
+        if C_s is not None:
+            self._field(C_s, "C_s")
         if props is None:
             props = {k: self.new_field(1.0) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")}
         pp = [self._field(props[k], k) for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")]
         self._stream()
-        L.check(self._lib.covof_update_properties_d(self._h, _ptr(C), float(rho1), float(rho2), float(mu1),
-                                                    float(mu2), *[_ptr(p) for p in pp]),
-                "covof_update_properties_d")
+        L.check(self._lib.covof_update_properties_cs_d(self._h, _ptr(C), _ptr(C_s), float(rho1), float(rho2),
+                                                       float(mu1), float(mu2), *[_ptr(p) for p in pp]),
+                "covof_update_properties_cs_d")
         return props
+
+    def set_obstacle(self, obs_type=None, x0=0.0, y0=0.0, x1=0.0, y1=0.0, xc=0.0, yc=0.0, radius=0.0,
+                     perm=0.0):
+        """The solid obstacle of config_t (obs_type 1 box [x0,x1]x[y0,y1], 2
+        cylinder (xc, yc, radius); obs_perm) the Poisson assembly and the
+        velocity correction see (src/module_simulation.f90:2491-2528,
+        :2596-2632); obs_type None removes it."""
+        if obs_type is None:
+            L.check(self._lib.covof_ctx_set_obstacle(self._h, None), "covof_ctx_set_obstacle")
+            return
+        o = L.Obstacle(1, int(obs_type), x0, y0, x1, y1, xc, yc, radius, perm)
+        L.check(self._lib.covof_ctx_set_obstacle(self._h, ct.byref(o)), "covof_ctx_set_obstacle")
+
+    def set_vof_top_profile(self, profile="UNIFORM", jet_center=0.5, jet_width=0.08):
+        """The top side's VOF profile of apply_vof_bc (bc_top%profile,
+        jet_center, jet_width; src/module_mpi_comm.f90:664-676)."""
+        L.check(self._lib.covof_ctx_set_vof_top_profile(self._h, profile.encode(), float(jet_center),
+                                                        float(jet_width)), "covof_ctx_set_vof_top_profile")
 
     # ------------------------------------------------ after the transport
     def compute_curvature_and_csf(self, C, sigma, G=None, enable=True, out=None, method="normals"):

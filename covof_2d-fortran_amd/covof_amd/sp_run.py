@@ -20,7 +20,10 @@ With ``SOLVER_MONOLITHIC`` steps 4-6 are the one-pass Uzawa loop of sim_run
 coupled system, unpack into u, v, p.
 
 Supported configuration: the momentum block's (UNIFORM profiles, no
-obstacles, no energy equation, no SVOF / diffusion smoothing, no markers).
+obstacles, no energy equation, no markers).  SVOF and diffusion smoothing
+run in advect_vof's order: SVOF smoothing of C, the properties from C_s,
+diffusion smoothing, the CSF normals from C_tilde (else C_s, else C)
+(src/module_interface_vof.f90:810-824, :681-689).
 Every field is a float64 CUDA tensor in the reference's halo layout; all the
 arithmetic runs in libcovof_hip.
 """
@@ -46,6 +49,9 @@ class StandardProjectionRun:
     Context.compute_curvature_and_csf).  ``reduction``: the summation order
     of the FP64 inner products (Context.set_reduction_order); with
     REDUCTION_REFERENCE every step is the reference's arithmetic bit for bit.
+    ``svof``: the SVOF filter width (cfg%svof_filter_width) with SVOF
+    smoothing on, else None; ``diffusion``: the diffusion-smoothing step
+    count with diffusion smoothing on, else None.
     """
 
     FIELDS = ("C", "u", "v", "p", "rho", "rho_u", "rho_v", "mu", "mu_node", "curvature", "fx_sigma", "fy_sigma",
@@ -56,7 +62,8 @@ class StandardProjectionRun:
                  bc_u=(0.0,) * 4, bc_v=(0.0,) * 4, reconstruction_method="PLIC",
                  predictor=(L.LINEAR_CG, L.PRECOND_JACOBI, 1e-10, 500),
                  poisson=(L.LINEAR_CG, L.PRECOND_SSOR, 1e-7, 1000, L.PRECISION_FP64), convection=True,
-                 solver_type=L.SOLVER_STANDARD_PROJECTION, curvature="normals", reduction=None):
+                 solver_type=L.SOLVER_STANDARD_PROJECTION, curvature="normals", reduction=None, svof=None,
+                 diffusion=None):
         self.ctx = ctx
         if reduction is not None:
             ctx.set_reduction_order(reduction)
@@ -69,6 +76,8 @@ class StandardProjectionRun:
         self.recon = reconstruction_method
         self.predictor = predictor
         self.poisson = poisson
+        self.svof = None if svof is None else float(svof)
+        self.diffusion = None if diffusion is None else int(diffusion)
         f = {}
         f["C"] = ctx._field(C, "C").clone()
         f["u"] = ctx._field(u, "u").clone()
@@ -79,9 +88,19 @@ class StandardProjectionRun:
             f[k] = ctx.new_field(1.0)
         for k in ("mu", "mu_node"):
             f[k] = ctx.new_field(1.0e-3)
+        if self.svof is not None:
+            f["C_s"] = ctx.new_field(1.0)
+        if self.diffusion is not None:
+            f["C_tilde"] = ctx.new_field(0.0)
         self.f = f
         self.props = {k: f[k] for k in ("rho", "rho_u", "rho_v", "mu", "mu_node")}
-        ctx.update_physical_properties(f["C"], self.rho1, self.rho2, self.mu1, self.mu2, props=self.props)
+        # the end of set_initial_conditions (src/module_interface_vof.f90:368-374)
+        if self.svof is not None:
+            ctx.smooth_svof_helmholtz(f["C"], self.svof, C_s=f["C_s"])
+        if self.diffusion is not None:
+            ctx.smooth_diffusion(f["C"], self.diffusion, C_tilde=f["C_tilde"])
+        ctx.update_physical_properties(f["C"], self.rho1, self.rho2, self.mu1, self.mu2, props=self.props,
+                                       C_s=f.get("C_s"))
         self.nstep = 0
         ctx.step_count = 0
 
@@ -103,10 +122,21 @@ class StandardProjectionRun:
             f["v_nm1"].copy_(f["v"])
         f["u_old"].copy_(f["u"])
         f["v_old"].copy_(f["v"])
-        # advect_vof: transport + properties, then curvature and CSF from C
-        ctx.advect_vof(f["C"], f["u"], f["v"], dt, reconstruction_method=self.recon, two_phase=True,
-                       rho1=self.rho1, rho2=self.rho2, mu1=self.mu1, mu2=self.mu2, props=self.props)
-        ctx.compute_curvature_and_csf(f["C"], self.sigma, enable=self.sigma > 0.0,
+        # advect_vof: transport, [SVOF smoothing], properties, [diffusion
+        # smoothing], then curvature and CSF
+        if self.svof is None:
+            ctx.advect_vof(f["C"], f["u"], f["v"], dt, reconstruction_method=self.recon, two_phase=True,
+                           rho1=self.rho1, rho2=self.rho2, mu1=self.mu1, mu2=self.mu2, props=self.props)
+        else:
+            ctx.advect_vof(f["C"], f["u"], f["v"], dt, reconstruction_method=self.recon)
+            ctx.smooth_svof_helmholtz(f["C"], self.svof, C_s=f["C_s"])
+            ctx.update_physical_properties(f["C"], self.rho1, self.rho2, self.mu1, self.mu2, props=self.props,
+                                           C_s=f["C_s"])
+        G = f.get("C_s")
+        if self.diffusion is not None:
+            ctx.smooth_diffusion(f["C"], self.diffusion, C_tilde=f["C_tilde"])
+            G = f["C_tilde"]
+        ctx.compute_curvature_and_csf(f["C"], self.sigma, G=G, enable=self.sigma > 0.0,
                                       out={k: f[k] for k in ("curvature", "fx_sigma", "fy_sigma")},
                                       method=self.curvature)
         cfg = self.cfg(has_nm1)

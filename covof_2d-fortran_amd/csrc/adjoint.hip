@@ -310,7 +310,7 @@ int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const dou
                        const double *x, const double *w_u, const double *w_v, const double *w_p, double *dJ_du,
                        double *dJ_dv, double *dJ_dlogrho, int precond, double tol, int maxit, int precision,
                        int *iters, double *relres) {
-  if (!c.ps.assembled) return COVOF_BAD_ARGUMENT;
+  if (!c.ps.assembled || c.obs.enabled) return COVOF_BAD_ARGUMENT;  // no obstacle faces in the adjoint
   const Mesh &m = c.m;
   const long n = (long)m.nx * m.ny;
   Poisson &P = c.ps;

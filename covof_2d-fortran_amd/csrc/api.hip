@@ -13,7 +13,7 @@ int vof_advect(Ctx &c, double *C, const double *u, const double *v, double dt, i
                int *step_count, int nsteps, int two_phase, double rho1, double rho2, double mu1,
                double mu2, double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node);
 int vof_properties(Ctx &c, const double *C, double rho1, double rho2, double mu1, double mu2,
-                   double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node);
+                   double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node, const double *Cs);
 int vof_reconstruct_interface(Ctx &c, int method, const double *C, double *onx, double *ony,
                               double *oal);
 int vof_geom_batch(Ctx &c, int n, const double *xmin, const double *xmax, const double *ymin,
@@ -300,8 +300,60 @@ extern "C" int covof_update_properties_d(covof_ctx *h, double *C, double rho1, d
                                          double *mu_node) {
   if (!h || !C || !rho || !rho_u || !rho_v || !mu || !mu_node) return fail(COVOF_BAD_ARGUMENT, "null argument");
   int st = covof::vof_corner_fix(h->c, C);
-  if (!st) st = covof::vof_properties(h->c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node);
+  if (!st) st = covof::vof_properties(h->c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node, nullptr);
   return hip_status(st);
+}
+
+extern "C" int covof_update_properties_cs_d(covof_ctx *h, double *C, const double *C_s, double rho1, double rho2,
+                                            double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+                                            double *mu, double *mu_node) {
+  if (!h || !C || !rho || !rho_u || !rho_v || !mu || !mu_node) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const double *outs[5] = {rho, rho_u, rho_v, mu, mu_node};
+  for (const double *o : outs)
+    if (o == C || (C_s && o == C_s))
+      return fail(COVOF_BAD_ARGUMENT, "covof_update_properties_cs_d: an output aliases C or C_s");
+  int st = covof::vof_corner_fix(h->c, C);
+  if (!st) st = covof::vof_properties(h->c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node, C_s);
+  return hip_status(st);
+}
+
+extern "C" int covof_ctx_set_obstacle(covof_ctx *h, const covof_obstacle *obs) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  covof::Obstacle o;
+  if (obs && obs->enabled) {
+    if (obs->type != 1 && obs->type != 2)
+      return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_obstacle: type must be 1 (box) or 2 (cylinder)");
+    o.enabled = 1;
+    o.type = obs->type;
+    o.x0 = obs->x0;
+    o.y0 = obs->y0;
+    o.x1 = obs->x1;
+    o.y1 = obs->y1;
+    o.xc = obs->xc;
+    o.yc = obs->yc;
+    o.r = obs->r;
+    o.perm = obs->perm;
+  }
+  h->c.obs = o;
+  ++h->c.ps.assembly_id;  // preconditioner records built from an earlier assembly no longer apply
+  return COVOF_OK;
+}
+
+extern "C" int covof_ctx_set_vof_top_profile(covof_ctx *h, const char *profile, double jet_center,
+                                             double jet_width) {
+  if (!h || !profile) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  std::string s(profile);
+  while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+  if (s == "JET_INLET") {
+    h->c.jet.on = 1;
+    h->c.jet.center = jet_center;
+    h->c.jet.width = jet_width;
+  } else if (s == "UNIFORM" || s.empty()) {
+    h->c.jet.on = 0;
+  } else {
+    return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_vof_top_profile: profile must be UNIFORM or JET_INLET");
+  }
+  return COVOF_OK;
 }
 
 static bool same_halo_array(const double *a, const double *b) { return a && b && a == b; }
@@ -543,8 +595,8 @@ extern "C" int covof_momentum_set_fields_d(covof_ctx *h, const covof_momentum_cf
   const covof::MomFields f{u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma};
   const int st = covof::mom_bind(h->c, c, f, dt);
   if (st == COVOF_BAD_ARGUMENT)
-    return fail(st, "momentum block: null field, dt <= 0, missing n-1 fields, or unsupported boundary sides "
-                    "(periodic sides in pairs, no NEUMANN top)");
+    return fail(st, "momentum block: null field, dt <= 0, missing n-1 fields, unsupported boundary sides "
+                    "(periodic sides in pairs, no NEUMANN top), or an obstacle set on the context");
   return hip_status(st);
 }
 

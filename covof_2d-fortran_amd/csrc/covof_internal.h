@@ -153,8 +153,24 @@ struct Prof {
   long count[COVOF_NUM_KERNEL_TAGS] = {0};
 };
 
+// A solid obstacle of config_t (obs_enabled, obs_type, box, disc, obs_perm),
+// for the Poisson assembly and the velocity correction.
+struct Obstacle {
+  int enabled = 0, type = 0;
+  double x0 = 0, y0 = 0, x1 = 0, y1 = 0, xc = 0, yc = 0, r = 0, perm = 1.0e-40;
+};
+
+// The top side's VOF profile of apply_vof_bc (cfg%bc_top%profile): JET_INLET
+// fills the top halo row with C = 0 inside the jet and 1 outside.
+struct TopJet {
+  int on = 0;
+  double center = 0.5, width = 0.08;
+};
+
 struct Ctx {
   Mesh m;
+  Obstacle obs;
+  TopJet jet;
   hipStream_t stream = nullptr;
   int own_stream = 0;
   // COVOF_REDUCTION_REFERENCE: the FP64 Krylov inner products and the

@@ -947,6 +947,7 @@ int mom_bind(Ctx &c, const MomCfg &cfg, const MomFields &f, double dt) {
   if ((bc[0] == BC_PERIODIC) != (bc[1] == BC_PERIODIC) || (bc[2] == BC_PERIODIC) != (bc[3] == BC_PERIODIC) ||
       bc[3] == BC_NEUMANN)
     return COVOF_BAD_ARGUMENT;
+  if (c.obs.enabled) return COVOF_BAD_ARGUMENT;  // Brinkman / solid-obstacle rows are not built
   const int st = mom_alloc(c);
   if (st) return st;
   Mom &M = c.mom;

@@ -276,13 +276,34 @@ void serial_sums(Ctx &c, long n, const SerialSet &s, int f0, int f1) {
 
 // ---------------------------------------------------------------- assembly
 
+// A solid obstacle as the Poisson block sees it: on = obs_enabled and
+// obs_perm <= 1e-12, then a face is a wall when its point lies in the box or
+// disc (is_in_obstacle, src/module_mesh.f90:203-222, at (x_face, y_center)
+// for u faces and (x_center, y_face) for v faces,
+// src/module_navier_stokes.f90:82-131).
+struct DevObs {
+  int on, type;
+  double x0, y0, x1, y1, xc, yc, r;
+};
+__device__ __forceinline__ bool in_obs(const DevObs &o, double x, double y) {
+  if (o.type == 1) return x >= o.x0 - 1.0e-9 && x <= o.x1 + 1.0e-9 && y >= o.y0 - 1.0e-9 && y <= o.y1 + 1.0e-9;
+  if (o.type == 2) return (x - o.xc) * (x - o.xc) + (y - o.yc) * (y - o.yc) <= o.r * o.r;
+  return false;
+}
+__device__ __forceinline__ bool solid_u(const DevObs &o, int i, int j, double dx, double dy) {
+  return o.on && in_obs(o, (double)(i - 1) * dx, ((double)(j - 1) + 0.5) * dy);
+}
+__device__ __forceinline__ bool solid_v(const DevObs &o, int i, int j, double dx, double dy) {
+  return o.on && in_obs(o, ((double)(i - 1) + 0.5) * dx, (double)(j - 1) * dy);
+}
+
 // src/module_simulation.f90:2483-2533 on faces: one thread per cell writes
 // b, kx(i,j) (and kx(nx+1,j) on the last column), ky(i,j) (and ky(i,ny+1)),
 // then diag from the four face values in the reference's summation order.
 __global__ void k_assemble(const double *__restrict__ u, const double *__restrict__ v,
                            const double *__restrict__ rho_u, const double *__restrict__ rho_v,
                            int nx, int ny, double dx, double dy, double dt, int wallW, int wallE,
-                           int wallS, int wallN, double *kx, double *ky, double *diag, double *b,
+                           int wallS, int wallN, DevObs ob, double *kx, double *ky, double *diag, double *b,
                            float *kx32, float *ky32, float *diag32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -298,10 +319,10 @@ __global__ void k_assemble(const double *__restrict__ u, const double *__restric
     if (r < 1.0e-12) r = 1.0;
     return (dt * dx) / (r * dy);
   };
-  const double aW = (i == 1 && wallW) ? 0.0 : cx(i);
-  const double aE = (i == nx && wallE) ? 0.0 : cx(i + 1);
-  const double aS = (j == 1 && wallS) ? 0.0 : cy(j);
-  const double aN = (j == ny && wallN) ? 0.0 : cy(j + 1);
+  const double aW = ((i == 1 && wallW) || solid_u(ob, i, j, dx, dy)) ? 0.0 : cx(i);
+  const double aE = ((i == nx && wallE) || solid_u(ob, i + 1, j, dx, dy)) ? 0.0 : cx(i + 1);
+  const double aS = ((j == 1 && wallS) || solid_v(ob, i, j, dx, dy)) ? 0.0 : cy(j);
+  const double aN = ((j == ny && wallN) || solid_v(ob, i, j + 1, dx, dy)) ? 0.0 : cy(j + 1);
   const long k = (long)(j - 1) * nx + (i - 1);
   kx[kxi(nx, i, j)] = aW;
   kx32[kxi(nx, i, j)] = (float)aW;
@@ -1147,17 +1168,19 @@ __global__ void k_gather_interior(int nx, int ny, const double *__restrict__ p, 
   x[(long)(j - 1) * nx + (i - 1)] = p[(long)j * (nx + 2) + i];
 }
 
-// Velocity correction (src/module_simulation.f90:2590-2654), no obstacles.
+// Velocity correction (src/module_simulation.f90:2590-2654); solid obstacle
+// faces are skipped like walls.
 __global__ void k_correct(int nx, int ny, double dx, double dy, double dt, int bl, int br, int bb,
                           int bt, int wallL, int wallR, int wallB, int wallT, int perL, int perR,
-                          int perB, int perT, const double *__restrict__ p, const double *__restrict__ rho_u,
-                          const double *__restrict__ rho_v, double *u, double *v) {
+                          int perB, int perT, DevObs ob, const double *__restrict__ p,
+                          const double *__restrict__ rho_u, const double *__restrict__ rho_v, double *u,
+                          double *v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
   auto H = [&](int a, int c) { return (long)c * (nx + 2) + a; };
   const int u_nx = br ? nx + 1 : nx, v_ny = bt ? ny + 1 : ny;
   if (i <= u_nx && j <= ny) {
-    const bool wall = (i == 1 && bl && wallL) || (i == u_nx && br && wallR);
+    const bool wall = (i == 1 && bl && wallL) || (i == u_nx && br && wallR) || solid_u(ob, i, j, dx, dy);
     if (!wall) {
       double r = rho_u[H(i, j)];
       if (r < 1.0e-12) r = 1.0;
@@ -1168,7 +1191,7 @@ __global__ void k_correct(int nx, int ny, double dx, double dy, double dt, int b
     }
   }
   if (i <= nx && j <= v_ny) {
-    const bool wall = (j == 1 && bb && wallB) || (j == v_ny && bt && wallT);
+    const bool wall = (j == 1 && bb && wallB) || (j == v_ny && bt && wallT) || solid_v(ob, i, j, dx, dy);
     if (!wall) {
       double r = rho_v[H(i, j)];
       if (r < 1.0e-12) r = 1.0;
@@ -1256,6 +1279,12 @@ void poisson_free(Ctx &c) {
 
 static int is_wall_type(int t) { return t == BC_DIRICHLET || t == BC_FREESLIP; }
 
+static DevObs dev_obs(const Ctx &c) {
+  const Obstacle &o = c.obs;
+  DevObs d{o.enabled && o.perm <= 1.0e-12 ? 1 : 0, o.type, o.x0, o.y0, o.x1, o.y1, o.xc, o.yc, o.r};
+  return d;
+}
+
 int poisson_assemble(Ctx &c, const double *u, const double *v, const double *rho_u,
                      const double *rho_v, double dt, double *b) {
   const Mesh &m = c.m;
@@ -1266,7 +1295,7 @@ int poisson_assemble(Ctx &c, const double *u, const double *v, const double *rho
   const int wS = !m.pery && is_wall_type(m.bc[2]);
   const int wN = !m.pery && is_wall_type(m.bc[3]);
   hipLaunchKernelGGL(k_assemble, grd, blk, 0, c.stream, u, v, rho_u, rho_v, m.nx, m.ny, m.dx, m.dy,
-                     dt, wW, wE, wS, wN, c.ps.kx, c.ps.ky, c.ps.diag, b, c.ps.kx32, c.ps.ky32,
+                     dt, wW, wE, wS, wN, dev_obs(c), c.ps.kx, c.ps.ky, c.ps.diag, b, c.ps.kx32, c.ps.ky32,
                      c.ps.diag32);
   COVOF_CHECK_LAUNCH();
   c.ps.assembled = 1;
@@ -1865,7 +1894,7 @@ int sp_correct(Ctx &c, const double *p, const double *rho_u, const double *rho_v
   hipLaunchKernelGGL(k_correct, grd, blk, 0, c.stream, m.nx, m.ny, m.dx, m.dy, dt, bl, br, bb, bt,
                      is_wall_type(m.bc[0]), is_wall_type(m.bc[1]), is_wall_type(m.bc[2]),
                      is_wall_type(m.bc[3]), m.bc[0] == BC_PERIODIC, m.bc[1] == BC_PERIODIC,
-                     m.bc[2] == BC_PERIODIC, m.bc[3] == BC_PERIODIC, p, rho_u, rho_v, u, v);
+                     m.bc[2] == BC_PERIODIC, m.bc[3] == BC_PERIODIC, dev_obs(c), p, rho_u, rho_v, u, v);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }

@@ -38,6 +38,8 @@ struct DevMesh {
   double dx, dy;
   int bcl, bcb;        // left / bottom BC types (periodic flux fix, apply_vof_bc)
   int perx, pery;
+  int jet;             // JET_INLET top profile (non-periodic y only)
+  double jet_xc, jet_hw;  // its centre and half width
 };
 
 static DevMesh dev_mesh(const Mesh &m) {
@@ -50,6 +52,22 @@ static DevMesh dev_mesh(const Mesh &m) {
   d.bcb = m.bc[2];
   d.perx = m.perx;
   d.pery = m.pery;
+  d.jet = 0;
+  d.jet_xc = 0.0;
+  d.jet_hw = 0.0;
+  return d;
+}
+
+// The mesh with the context's top profile (apply_vof_bc, src/module_mpi_comm.f90:
+// 664-676: xc_top = jet_center if > 0 else lx/2, half width jet_width/2 if
+// jet_width > 0 else 0.04).
+static DevMesh dev_mesh(const Ctx &c) {
+  DevMesh d = dev_mesh(c.m);
+  if (c.jet.on && !c.m.pery) {
+    d.jet = 1;
+    d.jet_xc = c.jet.center > 0.0 ? c.jet.center : 0.5 * c.m.lx;
+    d.jet_hw = c.jet.width > 0.0 ? 0.5 * c.jet.width : 0.04;
+  }
   return d;
 }
 
@@ -421,7 +439,14 @@ __device__ __forceinline__ void write_halo(double *dst, const double *src, const
   if (i == (m.perx ? nx : 1)) dst[H(m, 0, j)] = V;
   if (i == (m.perx ? 1 : nx)) dst[H(m, nx + 1, j)] = V;
   if (j == (m.pery ? ny : 1)) dst[H(m, i, 0)] = V;
-  if (j == (m.pery ? 1 : ny)) dst[H(m, i, ny + 1)] = V;
+  if (j == (m.pery ? 1 : ny)) {
+    if (m.jet) {  // JET_INLET: liquid (C = 0) inside the injector, C = 1 outside
+      const double x_c = ((double)(i - 1) + 0.5) * m.dx;
+      dst[H(m, i, ny + 1)] = (fabs(x_c - m.jet_xc) <= m.jet_hw) ? 0.0 : 1.0;
+    } else {
+      dst[H(m, i, ny + 1)] = V;
+    }
+  }
   if (m.perx && m.pery) {
     if (i == nx && j == ny) dst[H(m, 0, 0)] = V;
     if (i == 1 && j == ny) dst[H(m, nx + 1, 0)] = V;
@@ -637,19 +662,22 @@ __device__ __forceinline__ double clamp01(double c) {
 
 // update_physical_properties, two-phase incompressible (:538-613), C already
 // corner-fixed.  One thread per node / cell of the (0:nx+1, 0:ny+1) grid.
-__global__ void k_properties(const double *__restrict__ C, DevMesh m, double rho1, double rho2,
-                             double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
+// Cs non-null: the SVOF branch (:540-545, :596-600), rho and mu from the
+// clamped C_s, mu_node harmonically from the clamped node mean of C_s.
+__global__ void k_properties(const double *__restrict__ C, const double *__restrict__ Cs, DevMesh m, double rho1,
+                             double rho2, double mu1, double mu2, double *rho, double *rho_u, double *rho_v,
                              double *mu, double *mu_node) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = m.nx, ny = m.ny;
   if (i > nx + 1 || j > ny + 1) return;
+  const double *F = Cs ? Cs : C;
   auto rho_at = [&](int a, int b) {
-    const double c = clamp01(C[H(m, a, b)]);
+    const double c = clamp01(F[H(m, a, b)]);
     return c * rho1 + (1.0 - c) * rho2;
   };
   auto mu_raw = [&](int a, int b) {
-    const double c = clamp01(C[H(m, a, b)]);
+    const double c = clamp01(F[H(m, a, b)]);
     return c * mu1 + (1.0 - c) * mu2;
   };
   // mu corners are copied from the diagonal interior cell (:562-565)
@@ -666,8 +694,15 @@ __global__ void k_properties(const double *__restrict__ C, DevMesh m, double rho
   if (j >= 1 && j <= ny && i >= 1) rho_u[k] = 0.5 * (rho_at(i - 1, j) + rho_at(i, j));
   if (i >= 1 && i <= nx && j >= 1) rho_v[k] = 0.5 * (rho_at(i, j - 1) + rho_at(i, j));
   if (i >= 1 && j >= 1) {
-    const double sw = mu_at(i - 1, j - 1), se = mu_at(i, j - 1), nw = mu_at(i - 1, j), ne = mu_at(i, j);
-    mu_node[k] = 4.0 / (1.0 / sw + 1.0 / se + 1.0 / nw + 1.0 / ne);
+    if (Cs) {
+      const double cn = clamp01(0.25 * (Cs[H(m, i - 1, j - 1)] + Cs[H(m, i, j - 1)] + Cs[H(m, i - 1, j)] +
+                                        Cs[H(m, i, j)]));
+      const double den = mu2 * cn + mu1 * (1.0 - cn);
+      mu_node[k] = (mu1 * mu2) / (1.0e-30 > den ? 1.0e-30 : den);
+    } else {
+      const double sw = mu_at(i - 1, j - 1), se = mu_at(i, j - 1), nw = mu_at(i - 1, j), ne = mu_at(i, j);
+      mu_node[k] = 4.0 / (1.0 / sw + 1.0 / se + 1.0 / nw + 1.0 / ne);
+    }
   }
 }
 
@@ -762,7 +797,7 @@ static int launch_pass(Ctx &c, int method, int dir, bool first, const double *sr
                        const double *u, const double *v, double dt, bool corner_fix) {
   int st = launch_recon(c, method, src);
   if (st) return st;
-  const DevMesh m = dev_mesh(c.m);
+  const DevMesh m = dev_mesh(c);  // the halo written after the pass sees the top profile
   const dim3 grd(n_tiles(c.m)), blk(TBLK);
   int *ln = c.vof.list[0], *cn = c.vof.count;
   const int cf = corner_fix ? 1 : 0;
@@ -781,11 +816,11 @@ int vof_corner_fix(Ctx &c, double *C) {
 }
 
 int vof_properties(Ctx &c, const double *C, double rho1, double rho2, double mu1, double mu2,
-                   double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node) {
+                   double *rho, double *rho_u, double *rho_v, double *mu, double *mu_node, const double *Cs) {
   const DevMesh m = dev_mesh(c.m);
   const dim3 blk(64, 4);
   const dim3 grd((c.m.nx + 2 + 63) / 64, (c.m.ny + 2 + 3) / 4);
-  hipLaunchKernelGGL(k_properties, grd, blk, 0, c.stream, C, m, rho1, rho2, mu1, mu2, rho, rho_u,
+  hipLaunchKernelGGL(k_properties, grd, blk, 0, c.stream, C, Cs, m, rho1, rho2, mu1, mu2, rho, rho_u,
                      rho_v, mu, mu_node);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
@@ -807,7 +842,7 @@ int vof_advect(Ctx &c, double *C, const double *u, const double *v, double dt, i
     if (st) return st;
     *step_count += 1;
     if (two_phase) {
-      st = vof_properties(c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node);
+      st = vof_properties(c, C, rho1, rho2, mu1, mu2, rho, rho_u, rho_v, mu, mu_node, nullptr);
       if (st) return st;
     }
   }

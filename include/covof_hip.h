@@ -93,6 +93,29 @@ const char *covof_last_error(void);
 #define COVOF_REDUCTION_REFERENCE 1
 int covof_ctx_set_reduction_order(covof_ctx *ctx, int order);
 
+/* A solid obstacle: config_t's obs_enabled, obs_type (1 box [x0,x1]x[y0,y1],
+ * 2 disc of centre (xc, yc) and radius r), obs_perm (src/module_config.f90:
+ * 121-129).  With obs_perm <= 1e-12 every velocity face whose point
+ * (x_face, y_center) / (x_center, y_face) lies in the obstacle
+ * (is_in_obstacle, src/module_mesh.f90:203-222; is_u/v_face_in_obstacle,
+ * src/module_navier_stokes.f90:82-131) is a wall of the SP Poisson assembly
+ * and is skipped by the velocity correction (src/module_simulation.f90:
+ * 2491-2528, :2596-2632). */
+typedef struct covof_obstacle {
+  int enabled, type;
+  double x0, y0, x1, y1, xc, yc, r, perm;
+} covof_obstacle;
+/* The context's obstacle (NULL: none, the default).  The momentum block does
+ * not support obstacles (covof_momentum_set_fields_d refuses while one is set). */
+int covof_ctx_set_obstacle(covof_ctx *ctx, const covof_obstacle *obs);
+/* The top side's VOF boundary profile of apply_vof_bc (cfg%bc_top%profile,
+ * src/module_mpi_comm.f90:655-677): "UNIFORM" (default: zero-gradient copy)
+ * or "JET_INLET": the top halo row holds C = 0 where |x_center - xc| <= w
+ * and 1 elsewhere, xc = jet_center (<= 0: lx/2), w = jet_width/2 (jet_width
+ * <= 0: 0.04); it applies when the y direction is not periodic.  The VOF
+ * transport writes this halo after each pass. */
+int covof_ctx_set_vof_top_profile(covof_ctx *ctx, const char *profile, double jet_center, double jet_width);
+
 /* --- interface geometry (src/module_interface_reconstruction.f90) ----------- */
 
 /* Batched cut_box_area (:33-101) and compute_alpha (:105-147) on the device:
@@ -154,6 +177,15 @@ int covof_vof_advect(covof_ctx *ctx, double *C, const double *u, const double *v
 int covof_update_properties_d(covof_ctx *ctx, double *C, double rho1, double rho2, double mu1,
                               double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
                               double *mu_node);
+
+/* update_physical_properties with the SVOF branch (src/module_interface_vof.f90:
+ * 479-616, :540-545, :596-600): C_s non-NULL (cfg%enable_svof_smoothing with
+ * the smoothed field of covof_svof_smooth_d) takes rho and mu from the clamped
+ * C_s and mu_node from the clamped node mean of C_s, harmonically blended;
+ * C_s NULL is covof_update_properties_d. */
+int covof_update_properties_cs_d(covof_ctx *ctx, double *C, const double *C_s, double rho1, double rho2,
+                                 double mu1, double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
+                                 double *mu_node);
 
 /* --- the steps advect_vof runs after the transport (single rank, no markers);
  * halo fields (0:nx+1, 0:ny+1); an output never aliases an input. */
@@ -412,7 +444,8 @@ int covof_sp_pressure_d(covof_ctx *ctx, const double *x, double *p);
  * the copies they are, the others are constants), returns dJ/du*, dJ/dv* (halo layout)
  * and dJ/d(log rho) (nx*ny) through the face means, with halo densities the
  * copies of interior cells (periodic wrap, zero gradient).  One transpose CG
- * solve (A is symmetric) with the given preconditioner and precision. */
+ * solve (A is symmetric) with the given preconditioner and precision.  Not
+ * with an obstacle set on the context (COVOF_BAD_ARGUMENT). */
 int covof_projection_adjoint_d(covof_ctx *ctx, const double *rho, const double *rho_u, const double *rho_v,
                                double dt, const double *x, const double *w_u, const double *w_v,
                                const double *w_p, double *dJ_du, double *dJ_dv, double *dJ_dlogrho,

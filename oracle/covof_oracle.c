@@ -309,7 +309,20 @@ void orc_reconstruct_interface(const orc_mesh *m, int method, const double *C,
   free(gy);
 }
 
-/* src/module_mpi_comm.f90:636-683 (single rank, top profile not JET_INLET) */
+/* The top side's JET_INLET profile (cfg%bc_top%profile, jet_center,
+ * jet_width); off unless orc_set_top_jet turns it on. */
+static struct {
+  int on;
+  double center, width;
+} g_jet = {0, 0.5, 0.08};
+
+void orc_set_top_jet(int on, double center, double width) {
+  g_jet.on = on;
+  g_jet.center = center;
+  g_jet.width = width;
+}
+
+/* src/module_mpi_comm.f90:636-683 (single rank) */
 void orc_apply_vof_bc(const orc_mesh *m, double *C) {
   const int nx = m->nx, ny = m->ny;
   if (m->bc[0] == BC_PERIODIC) {
@@ -325,8 +338,18 @@ void orc_apply_vof_bc(const orc_mesh *m, double *C) {
   } else {
     if (!per_y(m)) for (int i = 1; i <= nx; ++i) C[H(m, i, 0)] = C[H(m, i, 1)];
   }
-  if (!per_y(m))
-    for (int i = 1; i <= nx; ++i) C[H(m, i, ny + 1)] = C[H(m, i, ny)];
+  if (!per_y(m)) {
+    if (g_jet.on) {
+      const double xc_top = g_jet.center > 0.0 ? g_jet.center : 0.5 * m->lx;
+      const double half_w = g_jet.width > 0.0 ? 0.5 * g_jet.width : 0.04;
+      for (int i = 1; i <= nx; ++i) {
+        const double x_c = ((double)(i - 1) + 0.5) * m->dx;
+        C[H(m, i, ny + 1)] = (fabs(x_c - xc_top) <= half_w) ? 0.0 : 1.0;
+      }
+    } else {
+      for (int i = 1; i <= nx; ++i) C[H(m, i, ny + 1)] = C[H(m, i, ny)];
+    }
+  }
 }
 
 /* One directional pass of src/module_interface_vof.f90:1162-1377.
@@ -425,6 +448,48 @@ void orc_wy_step(const orc_mesh *m, int method, double *C, const double *u, cons
   orc_exchange_halos_scalar(m, C);
   orc_apply_vof_bc(m, C);
   free(cd); free(cs); free(fl); free(sx); free(sy); free(sa);
+}
+
+/* src/module_interface_vof.f90:479-616 (incompressible); Cs non-null: the
+ * SVOF branch (rho, mu from C_s, the harmonic node viscosity from the node
+ * mean of C_s, :540-545, :596-600). */
+void orc_update_properties_cs(const orc_mesh *m, double *C, const double *Cs, double rho1, double rho2,
+                              double mu1, double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
+                              double *mu_node) {
+  const int nx = m->nx, ny = m->ny;
+  const int bl = !per_x(m), br = !per_x(m), bb = !per_y(m), bt = !per_y(m);
+  if (bl && bb) C[H(m, 0, 0)] = C[H(m, 1, 1)];
+  if (br && bb) C[H(m, nx + 1, 0)] = C[H(m, nx, 1)];
+  if (bl && bt) C[H(m, 0, ny + 1)] = C[H(m, 1, ny)];
+  if (br && bt) C[H(m, nx + 1, ny + 1)] = C[H(m, nx, ny)];
+  const double *F = Cs ? Cs : C;
+  for (int j = 0; j <= ny + 1; ++j)
+    for (int i = 0; i <= nx + 1; ++i) {
+      const double c = dmax(0.0, dmin(1.0, F[H(m, i, j)]));
+      rho[H(m, i, j)] = c * rho1 + (1.0 - c) * rho2;
+      mu[H(m, i, j)] = c * mu1 + (1.0 - c) * mu2;
+    }
+  mu[H(m, 0, 0)] = mu[H(m, 1, 1)];
+  mu[H(m, nx + 1, 0)] = mu[H(m, nx, 1)];
+  mu[H(m, 0, ny + 1)] = mu[H(m, 1, ny)];
+  mu[H(m, nx + 1, ny + 1)] = mu[H(m, nx, ny)];
+  for (int j = 1; j <= ny; ++j)
+    for (int i = 1; i <= nx + 1; ++i) rho_u[H(m, i, j)] = 0.5 * (rho[H(m, i - 1, j)] + rho[H(m, i, j)]);
+  for (int j = 1; j <= ny + 1; ++j)
+    for (int i = 1; i <= nx; ++i) rho_v[H(m, i, j)] = 0.5 * (rho[H(m, i, j - 1)] + rho[H(m, i, j)]);
+  for (int j = 1; j <= ny + 1; ++j)
+    for (int i = 1; i <= nx + 1; ++i) {
+      if (Cs) {
+        double cn = 0.25 * (Cs[H(m, i - 1, j - 1)] + Cs[H(m, i, j - 1)] + Cs[H(m, i - 1, j)] + Cs[H(m, i, j)]);
+        cn = dmax(0.0, dmin(1.0, cn));
+        const double den = mu2 * cn + mu1 * (1.0 - cn);
+        mu_node[H(m, i, j)] = (mu1 * mu2) / dmax(1.0e-30, den);
+      } else {
+        const double sw = mu[H(m, i - 1, j - 1)], se = mu[H(m, i, j - 1)];
+        const double nw = mu[H(m, i - 1, j)], ne = mu[H(m, i, j)];
+        mu_node[H(m, i, j)] = 4.0 / (1.0 / sw + 1.0 / se + 1.0 / nw + 1.0 / ne);
+      }
+    }
 }
 
 /* src/module_interface_vof.f90:479-616 (incompressible, no SVOF smoothing). */
@@ -701,7 +766,42 @@ void orc_curvature_csf(const orc_mesh *m, const double *G, const double *C, int 
 
 static int wall(int btype) { return btype == BC_DIRICHLET || btype == BC_FREESLIP; }
 
-/* src/module_simulation.f90:2477-2533 (no obstacles). */
+/* The solid obstacle of config_t (obs_enabled, obs_type, box corners, disc
+ * centre and radius, obs_perm); off unless orc_set_obstacle turns it on. */
+static struct {
+  int enabled, type;
+  double x0, y0, x1, y1, xc, yc, r, perm;
+} g_obs = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1.0e-40};
+
+void orc_set_obstacle(int enabled, int type, double x0, double y0, double x1, double y1, double xc, double yc,
+                      double r, double perm) {
+  g_obs.enabled = enabled;
+  g_obs.type = type;
+  g_obs.x0 = x0; g_obs.y0 = y0; g_obs.x1 = x1; g_obs.y1 = y1;
+  g_obs.xc = xc; g_obs.yc = yc; g_obs.r = r; g_obs.perm = perm;
+}
+
+/* is_in_obstacle (src/module_mesh.f90:203-222) */
+static int in_obstacle(double x, double y) {
+  if (g_obs.type == 1)
+    return x >= g_obs.x0 - 1.0e-9 && x <= g_obs.x1 + 1.0e-9 && y >= g_obs.y0 - 1.0e-9 && y <= g_obs.y1 + 1.0e-9;
+  if (g_obs.type == 2)
+    return (x - g_obs.xc) * (x - g_obs.xc) + (y - g_obs.yc) * (y - g_obs.yc) <= g_obs.r * g_obs.r;
+  return 0;
+}
+
+/* is_u_face_in_obstacle / is_v_face_in_obstacle (src/module_navier_stokes.f90:82-131)
+ * and the obs_perm <= 1e-12 test of the Poisson block: a solid face. */
+static int solid_u(const orc_mesh *m, int i, int j) {
+  if (!g_obs.enabled || !(g_obs.perm <= 1.0e-12)) return 0;
+  return in_obstacle(x_face(m, i), ((double)(j - 1) + 0.5) * m->dy);
+}
+static int solid_v(const orc_mesh *m, int i, int j) {
+  if (!g_obs.enabled || !(g_obs.perm <= 1.0e-12)) return 0;
+  return in_obstacle(((double)(i - 1) + 0.5) * m->dx, y_face(m, j));
+}
+
+/* src/module_simulation.f90:2477-2533 (solid obstacle faces: orc_set_obstacle). */
 void orc_poisson_assemble(const orc_mesh *m, const double *u, const double *v,
                           const double *rho_u, const double *rho_v, double dt,
                           double *aW, double *aE, double *aS, double *aN, double *aP,
@@ -715,22 +815,22 @@ void orc_poisson_assemble(const orc_mesh *m, const double *u, const double *v,
       const double div = (u[H(m, i + 1, j)] - u[H(m, i, j)]) * dy + (v[H(m, i, j + 1)] - v[H(m, i, j)]) * dx;
       b[k] = -div;
       aW[k] = aE[k] = aS[k] = aN[k] = 0.0;
-      if (!(i == 1 && bl && wall(m->bc[0]))) {
+      if (!((i == 1 && bl && wall(m->bc[0])) || solid_u(m, i, j))) {
         double r = rho_u[H(m, i, j)];
         if (r < 1.0e-12) r = 1.0;
         aW[k] = (dt * dy) / (r * dx);
       }
-      if (!(i == nx && br && wall(m->bc[1]))) {
+      if (!((i == nx && br && wall(m->bc[1])) || solid_u(m, i + 1, j))) {
         double r = rho_u[H(m, i + 1, j)];
         if (r < 1.0e-12) r = 1.0;
         aE[k] = (dt * dy) / (r * dx);
       }
-      if (!(j == 1 && bb && wall(m->bc[2]))) {
+      if (!((j == 1 && bb && wall(m->bc[2])) || solid_v(m, i, j))) {
         double r = rho_v[H(m, i, j)];
         if (r < 1.0e-12) r = 1.0;
         aS[k] = (dt * dx) / (r * dy);
       }
-      if (!(j == ny && bt && wall(m->bc[3]))) {
+      if (!((j == ny && bt && wall(m->bc[3])) || solid_v(m, i, j + 1))) {
         double r = rho_v[H(m, i, j + 1)];
         if (r < 1.0e-12) r = 1.0;
         aN[k] = (dt * dx) / (r * dy);
@@ -951,7 +1051,7 @@ void orc_sp_finish_pressure(const orc_mesh *m, const double *x, double *p) {
   orc_exchange_halos_scalar(m, p);
 }
 
-/* src/module_simulation.f90:2590-2654 (no obstacles; halo/BC sync not included). */
+/* src/module_simulation.f90:2590-2654 (solid obstacle faces skipped; halo/BC sync not included). */
 void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
                     const double *rho_v, double dt, double *u, double *v) {
   const int nx = m->nx, ny = m->ny;
@@ -960,7 +1060,8 @@ void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
   const int u_nx = br ? nx + 1 : nx, v_ny = bt ? ny + 1 : ny;
   for (int j = 1; j <= ny; ++j)
     for (int i = 1; i <= u_nx; ++i) {
-      const int is_wall = (i == 1 && bl && wall(m->bc[0])) || (i == u_nx && br && wall(m->bc[1]));
+      const int is_wall = (i == 1 && bl && wall(m->bc[0])) || (i == u_nx && br && wall(m->bc[1])) ||
+                          solid_u(m, i, j);
       if (is_wall) continue;
       double r = rho_u[H(m, i, j)];
       if (r < 1.0e-12) r = 1.0;
@@ -976,7 +1077,8 @@ void orc_sp_correct(const orc_mesh *m, const double *p, const double *rho_u,
     }
   for (int j = 1; j <= v_ny; ++j)
     for (int i = 1; i <= nx; ++i) {
-      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3])) ||
+                          solid_v(m, i, j);
       if (is_wall) continue;
       double r = rho_v[H(m, i, j)];
       if (r < 1.0e-12) r = 1.0;
@@ -1581,7 +1683,8 @@ void orc_mom_blocks(const orc_mesh *m, const orc_mom_cfg *c, const orc_mom_field
       const double rho_v = F->rho_v[H(m, i, j)];
       double Dv = (at * rho_v / dt_safe) * dvol;
       if (Dv <= 1.0e-30) Dv = 1.0;
-      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+      const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3])) ||
+                          solid_v(m, i, j);
       if (is_wall) {
         if (B->diag) B->diag[nu + dof] = Dv;
         vaP2[H(m, i, j)] = Dv;
@@ -1751,7 +1854,8 @@ void orc_mom_precond(const orc_mesh *m, const orc_mom_cfg *c, int precond, const
     for (int j = 1; j <= v_ny; ++j)
       for (int i = 1; i <= nx; ++i) {
         const long dof = (long)(j - 1) * nx + (i - 1);
-        const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3]));
+        const int is_wall = (j == 1 && bb && wall(m->bc[2])) || (j == v_ny && bt && wall(m->bc[3])) ||
+                          solid_v(m, i, j);
         rv[dof] = is_wall ? src[nu + dof] : src[nu + dof] - (xp[H(m, i, j)] - xp[H(m, i, j - 1)]) * m->dx;
       }
     orc_ssor_block(m, nx, v_ny, B->vW, B->vE, B->vS, B->vN, B->vP, rv, xv, 4, 1.0, sync);

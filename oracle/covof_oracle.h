@@ -170,6 +170,12 @@ double orc_dot(const double *x, const double *y, long n);
 /* Timed CPU baseline: `iters` Jacobi-PCG iterations on the face-form
  * operator with OpenMP over `threads` threads (FP64). Returns seconds. */
 void orc_set_threads(int threads);
+void orc_set_obstacle(int enabled, int type, double x0, double y0, double x1, double y1, double xc, double yc,
+                      double r, double perm);
+void orc_set_top_jet(int on, double center, double width);
+void orc_update_properties_cs(const orc_mesh *m, double *C, const double *Cs, double rho1, double rho2,
+                              double mu1, double mu2, double *rho, double *rho_u, double *rho_v, double *mu,
+                              double *mu_node);
 double orc_bench_jacobi_pcg_solve(const orc_mesh *m, const double *aP, const double *aW,
                                   const double *aE, const double *aS, const double *aN,
                                   const double *b, double tol, int maxit, int threads, int *iters,

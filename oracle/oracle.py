@@ -71,6 +71,9 @@ def lib():
             "orc_apply_vof_bc": (None, [M, _D]),
             "orc_wy_step": (None, [M, i, _D, _D, _D, d, ip]),
             "orc_update_properties": (None, [M, _D, d, d, d, d, _D, _D, _D, _D, _D]),
+            "orc_update_properties_cs": (None, [M, _D, _D, d, d, d, d, _D, _D, _D, _D, _D]),
+            "orc_set_obstacle": (None, [i, i] + [d] * 8),
+            "orc_set_top_jet": (None, [i, d, d]),
             "orc_svof_smooth": (None, [M, _D, d, _D]),
             "orc_smooth_diffusion": (None, [M, _D, i, _D]),
             "orc_curvature_csf": (None, [M, _D, _D, i, d, _D, _D, _D]),
@@ -190,6 +193,26 @@ def update_properties(m: Mesh, C: np.ndarray, rho1, rho2, mu1, mu2, init=1.0):
     out = [np.full(shp, init) for _ in range(5)]
     lib().orc_update_properties(ct.byref(m), _p(C), rho1, rho2, mu1, mu2, *[_p(a) for a in out])
     return out  # rho, rho_u, rho_v, mu, mu_node
+
+
+def update_properties_cs(m: Mesh, C: np.ndarray, Cs, rho1, rho2, mu1, mu2, init=1.0):
+    """update_physical_properties with the SVOF branch when Cs is given (C's
+    physical corners are fixed in place)."""
+    shp = (m.ny + 2, m.nx + 2)
+    out = [np.full(shp, init) for _ in range(5)]
+    lib().orc_update_properties_cs(ct.byref(m), _p(C), None if Cs is None else _p(np.ascontiguousarray(Cs)),
+                                   rho1, rho2, mu1, mu2, *[_p(a) for a in out])
+    return out  # rho, rho_u, rho_v, mu, mu_node
+
+
+def set_obstacle(enabled=False, type=0, x0=0.0, y0=0.0, x1=0.0, y1=0.0, xc=0.0, yc=0.0, radius=0.0, perm=1e-40):
+    """The solid obstacle the Poisson assembly and correction see (config_t's obs_*)."""
+    lib().orc_set_obstacle(int(bool(enabled)), int(type), x0, y0, x1, y1, xc, yc, radius, perm)
+
+
+def set_top_jet(on=False, center=0.5, width=0.08):
+    """The JET_INLET top profile of apply_vof_bc (bc_top%profile, jet_center, jet_width)."""
+    lib().orc_set_top_jet(int(bool(on)), center, width)
 
 
 def poisson_assemble(m: Mesh, u, v, rho_u, rho_v, dt):

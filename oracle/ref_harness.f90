@@ -41,6 +41,18 @@
 !           advect_vof (WY + properties + CSF), apply_velocity_bc,
 !           apply_vof_bc, solve_standard_projection; C, u, v, p and the
 !           predictor + Poisson iteration total after every step
+!   props   update_physical_properties on given C (and C_s with SVOF
+!           smoothing on): C after the physical-corner fix, rho, rho_u, rho_v,
+!           mu, mu_node (src/module_interface_vof.f90:479-616)
+!   jet     advect_vof with WEYMOUTH_YUE and a JET_INLET top profile, n steps
+!           (src/module_interface_vof.f90:792-825; apply_vof_bc,
+!           src/module_mpi_comm.f90:636-683): C after every step
+!   spobs   one standard-projection step with a solid obstacle (obs_perm 0):
+!           the predicted u*, v* (the predictor calls of
+!           solve_standard_projection replayed on a copy of the fields), then
+!           solve_standard_projection itself; rho_u, rho_v, p, u, v, the
+!           iteration total and the predictor's (src/module_simulation.f90:
+!           2424-2666, obstacle faces :2491-2528, :2596-2632)
 !
 ! Usage: covof_ref_harness <mode> <input.bin> <output.bin>
 program covof_ref_harness
@@ -87,6 +99,12 @@ program covof_ref_harness
      call run_spstep()
   case ('output')
      call run_output()
+  case ('props')
+     call run_props()
+  case ('jet')
+     call run_jet()
+  case ('spobs')
+     call run_spobs()
   case ('sim')
      call run_sim()
   case default
@@ -363,6 +381,126 @@ contains
        write(uout) iters
     end do
   end subroutine run_spstep
+
+  subroutine run_props()
+    integer :: nx, ny, bc(4), svof
+    real(dp) :: lx, ly, rho1, rho2, mu1, mu2
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    read(uin) nx, ny, bc, svof
+    read(uin) lx, ly, rho1, rho2, mu1, mu2
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%enable_two_phase = .true.
+    cfg%enable_svof_smoothing = (svof /= 0)
+    cfg%compressible = .false.
+    cfg%rho1 = rho1
+    cfg%rho2 = rho2
+    cfg%mu1 = mu1
+    cfg%mu2 = mu2
+    call fields%allocate(mesh)
+    read(uin) fields%C, fields%C_s
+    call update_physical_properties(fields, mesh, cfg)
+    write(uout) fields%C, fields%rho, fields%rho_u, fields%rho_v, fields%mu, fields%mu_node
+  end subroutine run_props
+
+  subroutine run_jet()
+    integer :: nx, ny, bc(4), method, nsteps, s
+    real(dp) :: lx, ly, dt, jc, jw
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    read(uin) nx, ny, bc, method, nsteps
+    read(uin) lx, ly, dt, jc, jw
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%advection_method = 'WEYMOUTH_YUE'
+    cfg%reconstruction_method = method_name(method)
+    cfg%enable_two_phase = .false.
+    cfg%use_markers = .false.
+    cfg%enable_svof_smoothing = .false.
+    cfg%enable_diffusion_smoothing = .false.
+    cfg%enable_surface_tension = .false.
+    cfg%compressible = .false.
+    cfg%bc_top%profile = 'JET_INLET'
+    cfg%bc_top%jet_center = jc
+    cfg%bc_top%jet_width = jw
+    call fields%allocate(mesh)
+    read(uin) fields%C, fields%u, fields%v
+    do s = 1, nsteps
+       call advect_vof(fields, mesh, cfg, dt)
+       write(uout) fields%C
+    end do
+  end subroutine run_jet
+
+  subroutine run_spobs()
+    integer :: nx, ny, bc(4), obs_type, iters, iters_u
+    real(dp) :: lx, ly, dt, rho1, rho2, mu1, mu2, gx, gy, ob(7)
+    type(simulation_t) :: sim
+    type(flow_fields_t) :: keep
+    real(dp), allocatable :: rhs(:), state(:)
+    read(uin) nx, ny, bc, obs_type
+    read(uin) lx, ly, dt, rho1, rho2, mu1, mu2, gx, gy, ob
+    call setup(sim%mesh, sim%config, nx, ny, lx, ly, bc)
+    sim%config%solver_type = SOLVER_STANDARD_PROJECTION
+    sim%config%enable_two_phase = .true.
+    sim%config%rho1 = rho1
+    sim%config%rho2 = rho2
+    sim%config%mu1 = mu1
+    sim%config%mu2 = mu2
+    sim%config%enable_surface_tension = .false.
+    sim%config%use_markers = .false.
+    sim%config%enable_svof_smoothing = .false.
+    sim%config%enable_diffusion_smoothing = .false.
+    sim%config%compressible = .false.
+    sim%config%time_scheme = 1
+    sim%config%enable_convection = .false.
+    sim%config%gx = gx
+    sim%config%gy = gy
+    sim%config%linear_solver = 1
+    sim%config%precond_type = 1
+    sim%config%use_precond = .true.
+    sim%config%tol = 1.0e-12_dp
+    sim%config%max_iter = 1000
+    sim%config%enable_energy = .false.
+    sim%config%enable_ssp_rk3_momentum = .false.
+    sim%config%obs_enabled = .true.
+    sim%config%obs_type = obs_type
+    sim%config%obs_x0 = ob(1)
+    sim%config%obs_y0 = ob(2)
+    sim%config%obs_x1 = ob(3)
+    sim%config%obs_y1 = ob(4)
+    sim%config%obs_xc = ob(5)
+    sim%config%obs_yc = ob(6)
+    sim%config%obs_radius = ob(7)
+    sim%config%obs_perm = 0.0_dp
+    call sim%fields%allocate(sim%mesh)
+    call sim%ns_solver%init(sim%mesh, sim%config)
+    read(uin) sim%fields%C, sim%fields%u, sim%fields%v
+    sim%fields%p = 0.0_dp
+    call update_physical_properties(sim%fields, sim%mesh, sim%config)
+    sim%fields%u_nm1 = sim%fields%u
+    sim%fields%v_nm1 = sim%fields%v
+    sim%fields%has_nm1 = .false.
+    sim%fields%u_old = sim%fields%u
+    sim%fields%v_old = sim%fields%v
+    call apply_velocity_bc(sim%fields, sim%mesh, sim%config, .false.)
+    ! the predictor calls of solve_standard_projection on a copy: u*, v*
+    keep = sim%fields
+    call sim%ns_solver%assemble_preconditioner_blocks(sim%mesh, sim%config, sim%fields, dt)
+    allocate(rhs(sim%ns_solver%n_owned_total), state(sim%ns_solver%n_owned_total))
+    call sim%ns_solver%assemble_rhs(sim%mesh, sim%config, sim%fields, dt, rhs)
+    call sim%ns_solver%pack_vector(sim%mesh, sim%config, sim%fields, state)
+    call solve_linear_system(sim%ns_solver, sim%mesh, sim%config, sim%fields, dt, rhs, state, iters_u)
+    call sim%ns_solver%unpack_vector(sim%mesh, sim%config, state, sim%fields, .false.)
+    call sync_u(sim%mesh, sim%fields%u)
+    call sync_v(sim%mesh, sim%fields%v)
+    call apply_boundary_conditions(sim%mesh, sim%config, sim%fields, .false.)
+    write(uout) sim%fields%u, sim%fields%v, sim%fields%rho_u, sim%fields%rho_v
+    sim%fields = keep
+    call sim%solve_standard_projection(dt, iters)
+    write(uout) sim%fields%p, sim%fields%u, sim%fields%v
+    write(uout) iters, iters_u
+  end subroutine run_spobs
 
   subroutine run_sim()
     type(simulation_t) :: sim

@@ -7,7 +7,8 @@ compressed ``.npz`` files (plain arrays, no pickles).  Every fixture records
 the reference build's flags (``-O2 -ffp-contract=off``).
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep, output, case)
+    python tests/golden/make_golden.py surface    # one group (geom, recon, advect, surface, momentum, spstep,
+                                                  # output, case, casemono, props, jet, spobs)
 """
 from __future__ import annotations
 
@@ -384,11 +385,127 @@ def case_mono_run():
     case_run(js, "case_bubble_mono_json.npz")
 
 
+def case_svof_runs():
+    """The case with SVOF smoothing (properties from C_s, CSF normals from C_s),
+    and with SVOF and diffusion smoothing (CSF normals from C_tilde)."""
+    import json
+    js = json.loads(json.dumps(CASE_JSON))
+    js["physics"]["enable_svof_smoothing"] = True
+    js["physics"]["svof_filter_width"] = 1.5
+    case_run(js, "case_bubble_svof_json.npz")
+    js["physics"]["enable_diffusion_smoothing"] = True
+    js["physics"]["diffusion_smoothing_steps"] = 2
+    case_run(js, "case_bubble_svof_diffusion_json.npz")
+
+
+def props(name, C, Cs, nx, ny, bc, svof, rho=(1000.0, 1.0), mu=(10.0, 0.01), lx=1.0, ly=1.0):
+    """update_physical_properties on C (and C_s when svof)."""
+    payload = (i32(nx, ny, *bc, svof) + f64(lx, ly, rho[0], rho[1], mu[0], mu[1]) + C.astype("<f8").tobytes()
+               + Cs.astype("<f8").tobytes())
+    out = np.frombuffer(run("props", payload), dtype="<f8").reshape(6, ny + 2, nx + 2)
+    np.savez_compressed(os.path.join(HERE, f"props_{name}.npz"), C0=C, C_s=Cs, nx=nx, ny=ny, bc=np.array(bc),
+                        svof=svof, rho1=rho[0], rho2=rho[1], mu1=mu[0], mu2=mu[1], lx=lx, ly=ly, C=out[0],
+                        rho=out[1], rho_u=out[2], rho_v=out[3], mu=out[4], mu_node=out[5], flags=FLAGS)
+
+
+def props_cases():
+    # C_s: a smoothed copy of C that overshoots [0, 1] a little (the clamp)
+    # and differs from C everywhere (so the branch is visible in every output)
+    for name, bc, seed in (("svof_walls", (FS, FS, DI, DI), 51), ("svof_periodic", (P, P, P, P), 52),
+                           ("svof_periodic_x", (P, P, NE, DI), 53)):
+        nx, ny = 30, 22
+        rng = np.random.default_rng(seed)
+        C = wl.ellipses(nx, ny, 5, 2.5, 7.0, seed=seed)
+        if bc[0] == P:
+            C[:, 0] = C[:, nx]
+            C[:, nx + 1] = C[:, 1]
+        else:
+            C[:, 0] = C[:, 1]
+            C[:, nx + 1] = C[:, nx]
+        if bc[2] == P:
+            C[0, :] = C[ny, :]
+            C[ny + 1, :] = C[1, :]
+        else:
+            C[0, :] = C[1, :]
+            C[ny + 1, :] = C[ny, :]
+        Cs = np.clip(C + 0.08 * rng.normal(size=C.shape), -0.05, 1.05)
+        props(name, C, Cs, nx, ny, bc, 1)
+    props("plain_walls", C, Cs, nx, ny, (FS, FS, DI, DI), 0)
+
+
+def jet(name, C, u, v, nx, ny, bc, method, dt, nsteps, jc, jw, lx=1.0, ly=1.0):
+    """advect_vof (WY) with a JET_INLET top profile, n steps."""
+    payload = (i32(nx, ny, *bc, method, nsteps) + f64(lx, ly, dt, jc, jw) + C.astype("<f8").tobytes()
+               + u.astype("<f8").tobytes() + v.astype("<f8").tobytes())
+    out = np.frombuffer(run("jet", payload), dtype="<f8").reshape(nsteps, ny + 2, nx + 2)
+    np.savez_compressed(os.path.join(HERE, f"jet_{name}.npz"), C0=C, u=u, v=v, nx=nx, ny=ny, bc=np.array(bc),
+                        method=method, dt=dt, nsteps=nsteps, jet_center=jc, jet_width=jw, lx=lx, ly=ly,
+                        C_steps=out, flags=FLAGS)
+
+
+def jet_cases():
+    # a downward flow through the top edge: the jet's columns inject C = 0,
+    # the rest of the top halo holds C = 1 (apply_vof_bc's JET_INLET branch)
+    for name, method, jc, jw, seed in (("plic", 0, 0.5, 0.25, 61), ("elvira_offcentre", 1, 0.3, 0.18, 62),
+                                       ("default_width", 0, 0.0, 0.0, 63)):
+        nx, ny = 32, 40
+        C = wl.ellipses(nx, ny, 5, 2.5, 8.0, seed=seed)
+        wl.fill_halo_zero_gradient(C, nx, ny)
+        dx = 1.0 / nx
+        xc = (np.arange(1, nx + 1) - 0.5) * dx
+        ctr = jc if jc > 0 else 0.5
+        hw = 0.5 * jw if jw > 0 else 0.04
+        C[ny + 1, 1:nx + 1] = np.where(np.abs(xc - ctr) <= hw, 0.0, 1.0)
+        u, v = wl.stream_velocity(nx, ny, 5, seed=seed + 100, periodic=False, amplitude=0.3)
+        v[1:ny + 2, 1:nx + 1] -= 0.5  # a uniform downward drift (divergence-free)
+        dt = 0.3 * dx / max(np.abs(u).max(), np.abs(v).max())
+        jet(name, C, u, v, nx, ny, (DI, DI, NE, DI), method, dt, 6, jc, jw, ly=40 / 32)
+
+
+def spobs(name, nx, ny, bc, obs_type, ob, C, u, v, dt, rho=(1000.0, 100.0), mu=(10.0, 1.0), g=(0.0, -0.98),
+          lx=1.0, ly=1.0):
+    """One SP step with a solid obstacle: u*, v*, rho_u, rho_v, then p, u, v and the iterations."""
+    payload = (i32(nx, ny, *bc, obs_type) + f64(lx, ly, dt, rho[0], rho[1], mu[0], mu[1], g[0], g[1], *ob)
+               + C.astype("<f8").tobytes() + u.astype("<f8").tobytes() + v.astype("<f8").tobytes())
+    raw = run("spobs", payload)
+    nf = (ny + 2) * (nx + 2)
+    out = np.frombuffer(raw[:7 * nf * 8], dtype="<f8").reshape(7, ny + 2, nx + 2)
+    its = np.frombuffer(raw[7 * nf * 8:], dtype="<i4")
+    np.savez_compressed(os.path.join(HERE, f"spobs_{name}.npz"), C=C, u0=u, v0=v, nx=nx, ny=ny, bc=np.array(bc),
+                        obs_type=obs_type, obs=np.array(ob), dt=dt, rho1=rho[0], rho2=rho[1], mu1=mu[0], mu2=mu[1],
+                        gx=g[0], gy=g[1], lx=lx, ly=ly, u_star=out[0], v_star=out[1], rho_u=out[2],
+                        rho_v=out[3], p=out[4], u=out[5], v=out[6], iters=int(its[0]), iters_u=int(its[1]),
+                        flags=FLAGS)
+
+
+def spobs_cases():
+    # a box (obs_type 1) and a cylinder (2) in a walled box with a heavy drop
+    # above them; obs_perm 0: every face inside the obstacle is a wall
+    nx, ny = 32, 40
+    C = wl.disc(nx, ny, 0.45, 0.9, 0.15, ly=1.25)
+    wl.fill_halo_zero_gradient(C, nx, ny)
+    rng = np.random.default_rng(71)
+    u = 0.05 * rng.normal(size=C.shape)
+    v = 0.05 * rng.normal(size=C.shape)
+    box = (0.3, 0.25, 0.62, 0.47, 0.0, 0.0, 0.0)
+    cyl = (0.0, 0.0, 0.0, 0.0, 0.55, 0.4, 0.17)
+    spobs("box", nx, ny, (DI, DI, DI, DI), 1, box, C, u, v, 1e-3, ly=1.25)
+    spobs("cylinder", nx, ny, (FS, FS, DI, DI), 2, cyl, C, u, v, 1e-3, ly=1.25)
+
+
 def main():
     if not os.path.exists(HARNESS):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     groups = set(sys.argv[1:]) or {"geom", "recon", "advect", "surface", "momentum", "spstep", "output", "case",
-                                   "casemono"}
+                                   "casemono", "props", "jet", "spobs", "casesvof"}
+    if "casesvof" in groups:
+        case_svof_runs()
+    if "props" in groups:
+        props_cases()
+    if "jet" in groups:
+        jet_cases()
+    if "spobs" in groups:
+        spobs_cases()
     if "surface" in groups:
         surface_cases()
     if "momentum" in groups:

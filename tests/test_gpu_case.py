@@ -21,11 +21,41 @@ def _arrays(text):
     return out
 
 
-@pytest.mark.parametrize("fixture", ["case_bubble_json.npz", "case_bubble_mono_json.npz"])
+FIXTURES = ["case_bubble_json.npz", "case_bubble_mono_json.npz", "case_bubble_svof_json.npz",
+            "case_bubble_svof_diffusion_json.npz"]
+
+
+@pytest.mark.parametrize("fixture", FIXTURES)
+def test_json_case_reference_order_identical_files(cv, tmp_path, fixture):
+    """The same cases with the inner products summed in the reference's
+    order: the history file and every .vtr file character for character the
+    reference program's."""
+    from covof_amd import case
+    print(f"\n[json case, reference order] {fixture}", flush=True)
+    g = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
+    cfg = json.loads(str(g["json"]))
+    out = tmp_path / "out"
+    cfg["output"]["output_dir"] = str(out)
+    path = tmp_path / "case.json"
+    path.write_text(json.dumps(cfg))
+    run = case.run_case(case.load_config(str(path)), log=lambda *a: None, reduction=cv.REDUCTION_REFERENCE)
+    assert run.steps == 13
+    assert (out / "bubble_history.csv").read_text() == str(g["history"])
+    for step in (0, 5, 10, 13):
+        t = (out / f"bubble_p0_{step:06d}.vtr").read_text()
+        if t != str(g[f"vtr_{step}"]):
+            A, B = _arrays(t), _arrays(str(g[f"vtr_{step}"]))
+            bad = [k for k in B if k not in A or not np.array_equal(A[k], B[k])]
+            raise AssertionError(f"step {step} file differs in {bad}")
+
+
+@pytest.mark.parametrize("fixture", FIXTURES)
 def test_json_case_matches_reference_program(cv, tmp_path, fixture):
     """The standard-projection case (CG + Jacobi predictor, the reference's
-    CG + SSOR Poisson block) and the same case with the fully coupled solver
-    (MONOLITHIC, BiCGSTAB(2) + block-triangular Schur-SSOR)."""
+    CG + SSOR Poisson block), the same case with the fully coupled solver
+    (MONOLITHIC, BiCGSTAB(2) + block-triangular Schur-SSOR), with SVOF
+    smoothing (properties and CSF normals from C_s) and with SVOF + diffusion
+    smoothing (CSF normals from C_tilde)."""
     from covof_amd import case
     print(f"\n[json case] {fixture}", flush=True)
     g = np.load(os.path.join(GOLDEN, fixture), allow_pickle=False)
@@ -37,7 +67,11 @@ def test_json_case_matches_reference_program(cv, tmp_path, fixture):
     run = case.run_case(case.load_config(str(path)), log=lambda *a: None)
     assert run.steps == 13
     # step 0: initial condition, properties, curvature: the same file
-    assert (out / "bubble_p0_000000.vtr").read_text() == str(g["vtr_0"])
+    t0 = (out / "bubble_p0_000000.vtr").read_text()
+    if t0 != str(g["vtr_0"]):
+        A, B = _arrays(t0), _arrays(str(g["vtr_0"]))
+        bad = [k for k in B if k not in A or not np.array_equal(A[k], B[k])]
+        raise AssertionError(f"step-0 file differs in {bad}")
     # history: same rows; step, time and dt columns identical, the others
     # within the rounding the solves' summation order leaves
     got = (out / "bubble_history.csv").read_text().splitlines()
@@ -62,6 +96,6 @@ def test_json_case_matches_reference_program(cv, tmp_path, fixture):
         assert A.keys() == B.keys()
         for k in B:
             scale = max(np.abs(B[k]).max(), 1e-300)
-            tol = 1e-6 if k == "Divergence" else 1e-9
+            tol = 1e-5 if k == "Divergence" else 1e-9  # the divergence is a solve residual (1e-7 stop)
             err = np.abs(A[k] - B[k]).max() / scale
             assert err <= tol, (step, k, err)
