@@ -21,3 +21,4 @@ from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BI
 from .context import Context, SolveResult  # noqa: F401
 from .autograd import PressureSolve, Projection, pressure_solve, projection  # noqa: F401
 from .sp_run import StandardProjectionRun  # noqa: F401
+from .coupled_adjoint import KineticEnergyAdjoint, kinetic_energy_after  # noqa: F401

@@ -89,6 +89,12 @@ SIGNATURES = {
     "covof_poisson_adjoint_d": (_I, [_P, _P, _D, _P, _P, _P, _P, _I, _I, _D, _I, _I, _IP, _DP]),
     "covof_sp_pressure_d": (_I, [_P, _P, _P]),
     "covof_projection_adjoint_d": (_I, [_P, _P, _P, _P, _D] + [_P] * 7 + [_I, _D, _I, _I, _IP, _DP]),
+    "covof_vof_advect_adjoint_d": (_I, [_P, _P, _P, _P, _D, ct.c_char_p, _I, _I, _I, _P, _P, _P, _P]),
+    "covof_vof_fill_halo_d": (_I, [_P, _P]),
+    "covof_kinetic_energy_d": (_I, [_P, _P, _P, _P, _D, _D, _DP, _P, _P, _P]),
+    "covof_properties_adjoint_d": (_I, [_P, _P, _D, _D, _P, _P]),
+    "covof_inertial_predictor_d": (_I, [_P, _I, _D, _D, _D, _P, _P, _P, _P, _P, _P]),
+    "covof_inertial_predictor_adjoint_d": (_I, [_P, _I, _P, _P, _P, _P, _P, _P]),
     "covof_momentum_size": (_I, [_P, _I, _IP, _IP, _IP]),
     "covof_momentum_set_fields_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D]),
     "covof_momentum_diag_d": (_I, [_P, _P]),

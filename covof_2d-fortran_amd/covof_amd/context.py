@@ -346,6 +346,88 @@ class Context:
         L.check(st, "covof_projection_adjoint_d", allow=(L.OK, L.NOT_CONVERGED, L.BREAKDOWN, L.NONFINITE))
         return du, dv, dlr, SolveResult(it.value, rr.value, st)
 
+    # ------------------------------- pieces of the multi-step kinetic-energy adjoint
+    def vof_advect_adjoint(self, C, u, v, dt, dJ_dCnew, step_count, reconstruction_method="PLIC", two_phase=True,
+                           fold_input_halo=False, dJ_dC=None, dJ_du=None, dJ_dv=None):
+        """Reverse mode of one Weymouth-Yue step of advect_vof from C (the
+        step's input; pass order from ``step_count``): returns (dJ/dC^n,
+        dJ/du, dJ/dv) for dJ_dCnew = dJ/dC^{n+1}; dJ/du and dJ/dv are added
+        to ``dJ_du``, ``dJ_dv`` when given (new zero fields otherwise).
+        ``fold_input_halo``: C's halo is the copy of its interior, and its
+        entries are folded onto the cells they copy (no reference
+        counterpart)."""
+        for t, n in ((C, "C"), (u, "u"), (v, "v"), (dJ_dCnew, "dJ_dCnew")):
+            self._field(t, n)
+        dJ_dC = self.new_field() if dJ_dC is None else self._field(dJ_dC, "dJ_dC")
+        dJ_du = self.new_field() if dJ_du is None else self._field(dJ_du, "dJ_du")
+        dJ_dv = self.new_field() if dJ_dv is None else self._field(dJ_dv, "dJ_dv")
+        self._stream()
+        L.check(self._lib.covof_vof_advect_adjoint_d(self._h, _ptr(C), _ptr(u), _ptr(v), float(dt),
+                                                     reconstruction_method.encode(), int(step_count),
+                                                     int(bool(two_phase)), int(bool(fold_input_halo)),
+                                                     _ptr(dJ_dCnew), _ptr(dJ_dC), _ptr(dJ_du), _ptr(dJ_dv)),
+                "covof_vof_advect_adjoint_d")
+        return dJ_dC, dJ_du, dJ_dv
+
+    def fill_vof_halo(self, C):
+        """C's halo from its interior as two-phase transport leaves it
+        (zero-gradient copies or wrap, corners from the diagonal cell), in place."""
+        self._field(C, "C")
+        self._stream()
+        L.check(self._lib.covof_vof_fill_halo_d(self._h, _ptr(C)), "covof_vof_fill_halo_d")
+        return C
+
+    def kinetic_energy(self, C, u, v, rho1, rho2, grad=False):
+        """The history row's kinetic energy (src/module_vtk_writer.f90:399-531)
+        with rho from C; with ``grad`` also (dKE/dC, dKE/du, dKE/dv)."""
+        for t, n in ((C, "C"), (u, "u"), (v, "v")):
+            self._field(t, n)
+        ke = ct.c_double(0.0)
+        g = [self.new_field() for _ in range(3)] if grad else [None] * 3
+        self._stream()
+        L.check(self._lib.covof_kinetic_energy_d(self._h, _ptr(C), _ptr(u), _ptr(v), float(rho1), float(rho2),
+                                                 ct.byref(ke), *[_ptr(a) for a in g]), "covof_kinetic_energy_d")
+        return (ke.value, *g) if grad else ke.value
+
+    def properties_adjoint(self, C, rho1, rho2, dJ_dlogrho, dJ_dC):
+        """dJ_dC += dJ/d(log rho) * drho/dC of update_physical_properties's
+        density (no reference counterpart)."""
+        self._field(C, "C")
+        self._work(dJ_dlogrho, "dJ_dlogrho")
+        self._field(dJ_dC, "dJ_dC")
+        self._stream()
+        L.check(self._lib.covof_properties_adjoint_d(self._h, _ptr(C), float(rho1), float(rho2), _ptr(dJ_dlogrho),
+                                                     _ptr(dJ_dC)), "covof_properties_adjoint_d")
+        return dJ_dC
+
+    def inertial_predictor(self, first, dt, gx, gy, u, v, u_nm1=None, v_nm1=None, u_star=None, v_star=None):
+        """u*, v* of the momentum block with convection and viscosity off:
+        BDF2 inertia and gravity, explicit Euler when ``first``."""
+        for t, n in ((u, "u"), (v, "v")):
+            self._field(t, n)
+        if not first:
+            self._field(u_nm1, "u_nm1")
+            self._field(v_nm1, "v_nm1")
+        u_star = self.new_field() if u_star is None else self._field(u_star, "u_star")
+        v_star = self.new_field() if v_star is None else self._field(v_star, "v_star")
+        self._stream()
+        L.check(self._lib.covof_inertial_predictor_d(self._h, int(bool(first)), float(dt), float(gx), float(gy),
+                                                     _ptr(u), _ptr(v), _ptr(u_nm1), _ptr(v_nm1), _ptr(u_star),
+                                                     _ptr(v_star)), "covof_inertial_predictor_d")
+        return u_star, v_star
+
+    def inertial_predictor_adjoint(self, first, w_u, w_v, dJ_du, dJ_dv, dJ_du_nm1=None, dJ_dv_nm1=None):
+        """Transpose of inertial_predictor, added to the gradients."""
+        for t, n in ((w_u, "w_u"), (w_v, "w_v"), (dJ_du, "dJ_du"), (dJ_dv, "dJ_dv")):
+            self._field(t, n)
+        if not first:
+            self._field(dJ_du_nm1, "dJ_du_nm1")
+            self._field(dJ_dv_nm1, "dJ_dv_nm1")
+        self._stream()
+        L.check(self._lib.covof_inertial_predictor_adjoint_d(self._h, int(bool(first)), _ptr(w_u), _ptr(w_v),
+                                                             _ptr(dJ_du), _ptr(dJ_dv), _ptr(dJ_du_nm1),
+                                                             _ptr(dJ_dv_nm1)), "covof_inertial_predictor_adjoint_d")
+
     # ------------------------------------------------- SP velocity predictor
     MOMENTUM_FIELDS = ("u", "v", "u_old", "v_old", "u_nm1", "v_nm1", "rho_u", "rho_v", "mu", "mu_node",
                        "fx_sigma", "fy_sigma")

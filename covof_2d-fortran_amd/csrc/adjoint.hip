@@ -276,6 +276,44 @@ __global__ void k_proj_adj_vel(int nx, int ny, int perx, int pery, int wallL, in
   dv[h] = sv;
 }
 
+// Scale of the transpose solve's right-hand side: the Krylov loop's
+// breakdown guards are absolute (|r.z|, |d.Ad| >= 1e-35), so a right-hand
+// side near rounding level (an objective whose velocity gradient the
+// correction nearly annihilates, as for the kinetic energy) is solved at
+// unit norm and the solution scaled back.  Per-block partials of g^2:
+__global__ void __launch_bounds__(256) k_sumsq_part(long n, const double *__restrict__ g, double *__restrict__ part) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) acc += g[t] * g[t];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// g /= |g| (every block sums the nblk partials in the same order); the norm
+// (1 when it is zero or not finite, g then unchanged) to *s by block 0.
+__global__ void __launch_bounds__(256) k_scale_unit(long n, int nblk, const double *__restrict__ part,
+                                                    double *__restrict__ g, double *__restrict__ s) {
+  __shared__ double nrm;
+  if (threadIdx.x == 0) {
+    double q = 0.0;
+    for (int b = 0; b < nblk; ++b) q += part[b];
+    const double r = sqrt(q);
+    nrm = (r > 0.0 && r < 1.0e300) ? r : 1.0;
+    if (blockIdx.x == 0) *s = nrm;
+  }
+  __syncthreads();
+  const double inv = 1.0 / nrm;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) g[t] = g[t] * inv;
+}
+
+__global__ void k_scale_by(long n, const double *__restrict__ s, double *__restrict__ x) {
+  const double f = *s;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] = x[t] * f;
+}
+
 __global__ void k_zero64(long n, double *x) {
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] = 0.0;
 }
@@ -326,10 +364,15 @@ int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const dou
   hipLaunchKernelGGL(k_proj_adj_g, dim3(nblk), dim3(256), 0, c.stream, m.nx, m.ny, m.perx, m.pery, wl, wr, wb, wt,
                      m.dx, m.dy, dt, rho_u, rho_v, w_u, w_v, w_p, g, part);
   hipLaunchKernelGGL(k_proj_adj_center, dim3(1024), dim3(256), 0, c.stream, n, nblk, (const double *)part, g);
+  // the solve at unit right-hand side norm (slot 57 keeps the norm)
+  hipLaunchKernelGGL(k_sumsq_part, dim3(nblk), dim3(256), 0, c.stream, n, (const double *)g, part);
+  hipLaunchKernelGGL(k_scale_unit, dim3(nblk), dim3(256), 0, c.stream, n, nblk, (const double *)part, g,
+                     P.scal + 57);
   hipLaunchKernelGGL(k_zero64, dim3(1024), dim3(256), 0, c.stream, n, lam);
   COVOF_CHECK_LAUNCH();
   const int st = poisson_solve(c, g, lam, COVOF_LINEAR_CG, precond, tol, maxit, precision, iters, relres);
   if (st == COVOF_BAD_ARGUMENT || st == COVOF_HIP_ERROR) return st;
+  hipLaunchKernelGGL(k_scale_by, dim3(1024), dim3(256), 0, c.stream, n, (const double *)(P.scal + 57), lam);
   const dim3 blk(64, 4), grd((m.nx + 1 + 63) / 64, (m.ny + 1 + 3) / 4);
   // the operator's face sensitivities (lam, x), then the correction's terms
   hipLaunchKernelGGL(k_face_sens, grd, blk, 0, c.stream, m.nx, m.ny, m.bc[0] == BC_PERIODIC,
@@ -343,6 +386,220 @@ int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const dou
                      dJ_dlogrho);
   COVOF_CHECK_LAUNCH();
   return st;
+}
+
+// ------------------------------------------- pieces of the coupled step adjoint
+//
+// The multi-step adjoint of the kinetic energy chains, per step, the
+// adjoint of the transport (vof.hip), of the projection (above), of the
+// density update and of the inertial predictor; these are the last two and
+// the objective.
+
+namespace {
+
+// rho of update_physical_properties (src/module_interface_vof.f90:552-553)
+// from the clamped C, and whether the clamp passed C through (derivative 1).
+__device__ __forceinline__ double rho_of(double c, double rho1, double rho2) {
+  const double t = 1.0 < c ? 1.0 : c;
+  const double cc = 0.0 > t ? 0.0 : t;
+  return cc * rho1 + (1.0 - cc) * rho2;
+}
+__device__ __forceinline__ bool clamp_passes(double c) { return !(c > 1.0) && !(c < 0.0); }
+
+// Per-block partials of the kinetic energy of the history row
+// (src/module_vtk_writer.f90:399-531): 0.5 rho (0.5 (uw^2 + ue^2) +
+// 0.5 (vs^2 + vn^2)) dx dy summed over the cells.
+__global__ void __launch_bounds__(256) k_ke_part(int nx, int ny, double dvol, const double *__restrict__ C,
+                                                 const double *__restrict__ u, const double *__restrict__ v,
+                                                 double rho1, double rho2, double *__restrict__ part) {
+  __shared__ double sh[4];
+  const long n = (long)nx * ny;
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int j = (int)(t / nx) + 1, i = (int)(t - (long)(j - 1) * nx) + 1;
+    const long h = (long)j * (nx + 2) + i;
+    const double uw = u[h], ue = u[h + 1], vs = v[h], vn = v[h + nx + 2];
+    const double u2p = 0.5 * (uw * uw + ue * ue), v2p = 0.5 * (vs * vs + vn * vn);
+    acc += 0.5 * rho_of(C[h], rho1, rho2) * (u2p + v2p) * dvol;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ void k_ke_total(int nblk, const double *__restrict__ part, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[b];
+  *out = s;
+}
+
+// dKE/dC (cells), dKE/du, dKE/dv (faces), halo layout; null outputs skipped.
+__global__ void k_ke_grad(int nx, int ny, double dvol, const double *__restrict__ C, const double *__restrict__ u,
+                          const double *__restrict__ v, double rho1, double rho2, double *__restrict__ dC,
+                          double *__restrict__ du, double *__restrict__ dv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > nx + 1 || j > ny + 1) return;
+  const long h = (long)j * (nx + 2) + i;
+  const bool cell = i >= 1 && i <= nx && j >= 1 && j <= ny;
+  if (dC) {
+    double g = 0.0;
+    if (cell && clamp_passes(C[h])) {
+      const double uw = u[h], ue = u[h + 1], vs = v[h], vn = v[h + nx + 2];
+      g = 0.5 * (0.5 * (uw * uw + ue * ue) + 0.5 * (vs * vs + vn * vn)) * dvol * (rho1 - rho2);
+    }
+    dC[h] = g;
+  }
+  if (du) {
+    double g = 0.0;
+    if (j >= 1 && j <= ny && i >= 1) {
+      double r = 0.0;
+      if (i >= 2) r += rho_of(C[h - 1], rho1, rho2);
+      if (i <= nx) r += rho_of(C[h], rho1, rho2);
+      g = 0.5 * u[h] * dvol * r;
+    }
+    du[h] = g;
+  }
+  if (dv) {
+    double g = 0.0;
+    if (i >= 1 && i <= nx && j >= 1) {
+      double r = 0.0;
+      if (j >= 2) r += rho_of(C[h - (nx + 2)], rho1, rho2);
+      if (j <= ny) r += rho_of(C[h], rho1, rho2);
+      g = 0.5 * v[h] * dvol * r;
+    }
+    dv[h] = g;
+  }
+}
+
+// dJ/dC += dJ/d(log rho) (rho1 - rho2) / rho on the interior cells.
+__global__ void k_props_adj(int nx, int ny, const double *__restrict__ C, double rho1, double rho2,
+                            const double *__restrict__ g, double *__restrict__ dC) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  if (i > nx || j > ny) return;
+  const long h = (long)j * (nx + 2) + i;
+  const double c = C[h];
+  if (!clamp_passes(c)) return;
+  dC[h] += g[(long)(j - 1) * nx + (i - 1)] * (rho1 - rho2) / rho_of(c, rho1, rho2);
+}
+
+// The face sets of the predictor: x faces (1..nx+1, 1..ny), y faces
+// (1..nx, 1..ny+1), without the wall faces (zero normal velocity).
+struct PredFaces {
+  int nx, ny, wl, wr, wb, wt;
+  __device__ bool xface(int i, int j) const {
+    return j >= 1 && j <= ny && i >= 1 && i <= nx + 1 && !(i == 1 && wl) && !(i == nx + 1 && wr);
+  }
+  __device__ bool yface(int i, int j) const {
+    return i >= 1 && i <= nx && j >= 1 && j <= ny + 1 && !(j == 1 && wb) && !(j == ny + 1 && wt);
+  }
+};
+
+// u* of the momentum block with convection and viscosity off: the BDF2
+// inertia (4 u^n - u^{n-1} + 2 dt g) / 3, or explicit Euler u^n + dt g on
+// the first step; zero on wall faces and outside the face sets.
+__global__ void k_pred(PredFaces F, int first, double dt, double gx, double gy, const double *__restrict__ u,
+                       const double *__restrict__ v, const double *__restrict__ un1, const double *__restrict__ vn1,
+                       double *__restrict__ us, double *__restrict__ vs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > F.nx + 1 || j > F.ny + 1) return;
+  const long h = (long)j * (F.nx + 2) + i;
+  double a = 0.0, b = 0.0;
+  if (F.xface(i, j)) a = first ? u[h] + dt * gx : (4.0 * u[h] - un1[h] + 2.0 * dt * gx) / 3.0;
+  if (F.yface(i, j)) b = first ? v[h] + dt * gy : (4.0 * v[h] - vn1[h] + 2.0 * dt * gy) / 3.0;
+  us[h] = a;
+  vs[h] = b;
+}
+
+// The predictor's transpose: du += a w_u, dun1 += b w_u on its faces.
+__global__ void k_pred_adj(PredFaces F, int first, const double *__restrict__ wu, const double *__restrict__ wv,
+                           double *__restrict__ du, double *__restrict__ dv, double *__restrict__ dun1,
+                           double *__restrict__ dvn1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > F.nx + 1 || j > F.ny + 1) return;
+  const long h = (long)j * (F.nx + 2) + i;
+  const double a = first ? 1.0 : 4.0 / 3.0, b = -1.0 / 3.0;
+  if (F.xface(i, j)) {
+    du[h] += a * wu[h];
+    if (!first) dun1[h] += b * wu[h];
+  }
+  if (F.yface(i, j)) {
+    dv[h] += a * wv[h];
+    if (!first) dvn1[h] += b * wv[h];
+  }
+}
+
+PredFaces pred_faces(const Mesh &m) {
+  auto wall = [](int b) { return b == BC_DIRICHLET || b == BC_FREESLIP ? 1 : 0; };
+  return PredFaces{m.nx, m.ny, wall(m.bc[0]), wall(m.bc[1]), wall(m.bc[2]), wall(m.bc[3])};
+}
+
+}  // namespace
+
+// Sides must be walls (DIRICHLET / FREESLIP) or periodic pairs.
+static bool closed_or_periodic(const Mesh &m) {
+  for (int s = 0; s < 4; ++s)
+    if (m.bc[s] != BC_DIRICHLET && m.bc[s] != BC_FREESLIP && m.bc[s] != BC_PERIODIC) return false;
+  return true;
+}
+
+int kinetic_energy(Ctx &c, const double *C, const double *u, const double *v, double rho1, double rho2, double *ke,
+                   double *dC, double *du, double *dv) {
+  const Mesh &m = c.m;
+  Poisson &P = c.ps;
+  const double dvol = m.dx * m.dy;
+  const int nblk = 256;
+  double *part = P.part + 2L * P.nblocks * 4;
+  if (ke) {
+    hipLaunchKernelGGL(k_ke_part, dim3(nblk), dim3(256), 0, c.stream, m.nx, m.ny, dvol, C, u, v, rho1, rho2, part);
+    hipLaunchKernelGGL(k_ke_total, dim3(1), dim3(64), 0, c.stream, nblk, (const double *)part, P.scal + 56);
+    COVOF_CHECK_LAUNCH();
+  }
+  if (dC || du || dv) {
+    const dim3 blk(64, 4), grd((m.nx + 2 + 63) / 64, (m.ny + 2 + 3) / 4);
+    hipLaunchKernelGGL(k_ke_grad, grd, blk, 0, c.stream, m.nx, m.ny, dvol, C, u, v, rho1, rho2, dC, du, dv);
+    COVOF_CHECK_LAUNCH();
+  }
+  if (ke) {
+    COVOF_CHECK(hipMemcpyAsync(P.hscal + 56, P.scal + 56, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    COVOF_CHECK(hipStreamSynchronize(c.stream));
+    *ke = P.hscal[56];
+  }
+  return COVOF_OK;
+}
+
+int properties_adjoint(Ctx &c, const double *C, double rho1, double rho2, const double *g, double *dC) {
+  const Mesh &m = c.m;
+  const dim3 blk(64, 4), grd((m.nx + 63) / 64, (m.ny + 3) / 4);
+  hipLaunchKernelGGL(k_props_adj, grd, blk, 0, c.stream, m.nx, m.ny, C, rho1, rho2, g, dC);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+int inertial_predictor(Ctx &c, int first, double dt, double gx, double gy, const double *u, const double *v,
+                       const double *un1, const double *vn1, double *us, double *vs) {
+  if (!closed_or_periodic(c.m)) return COVOF_BAD_ARGUMENT;
+  const Mesh &m = c.m;
+  const dim3 blk(64, 4), grd((m.nx + 2 + 63) / 64, (m.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(k_pred, grd, blk, 0, c.stream, pred_faces(m), first, dt, gx, gy, u, v, un1, vn1, us, vs);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+int inertial_predictor_adjoint(Ctx &c, int first, const double *wu, const double *wv, double *du, double *dv,
+                               double *dun1, double *dvn1) {
+  if (!closed_or_periodic(c.m)) return COVOF_BAD_ARGUMENT;
+  const Mesh &m = c.m;
+  const dim3 blk(64, 4), grd((m.nx + 2 + 63) / 64, (m.ny + 2 + 3) / 4);
+  hipLaunchKernelGGL(k_pred_adj, grd, blk, 0, c.stream, pred_faces(m), first, wu, wv, du, dv, dun1, dvn1);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
 }
 
 }  // namespace covof

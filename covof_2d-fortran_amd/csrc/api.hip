@@ -43,6 +43,17 @@ int projection_adjoint(Ctx &c, const double *rho, const double *rho_u, const dou
                        const double *x, const double *w_u, const double *w_v, const double *w_p, double *dJ_du,
                        double *dJ_dv, double *dJ_dlogrho, int precond, double tol, int maxit, int precision,
                        int *iters, double *relres);
+int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, const double *v, double dt,
+                       int step_count, int two_phase, int fold_in, const double *cbar_new, double *cbar,
+                       double *ubar, double *vbar);
+int vof_fill_halo(Ctx &c, double *C);
+int kinetic_energy(Ctx &c, const double *C, const double *u, const double *v, double rho1, double rho2, double *ke,
+                   double *dC, double *du, double *dv);
+int properties_adjoint(Ctx &c, const double *C, double rho1, double rho2, const double *g, double *dC);
+int inertial_predictor(Ctx &c, int first, double dt, double gx, double gy, const double *u, const double *v,
+                       const double *un1, const double *vn1, double *us, double *vs);
+int inertial_predictor_adjoint(Ctx &c, int first, const double *wu, const double *wv, double *du, double *dv,
+                               double *dun1, double *dvn1);
 }  // namespace covof
 
 using covof::Ctx;
@@ -548,6 +559,77 @@ extern "C" int covof_projection_adjoint_d(covof_ctx *h, const double *rho, const
   const int st = covof::projection_adjoint(h->c, rho, rho_u, rho_v, dt, x, w_u, w_v, w_p, dJ_du, dJ_dv, dJ_dlogrho,
                                            precond, tol, maxit, precision, iters, relres);
   if (st == COVOF_BAD_ARGUMENT) return fail(st, "operator not assembled or unsupported preconditioner/precision");
+  return hip_status(st);
+}
+
+// ------------------------------------------- multi-step adjoint of the kinetic energy
+extern "C" int covof_vof_advect_adjoint_d(covof_ctx *h, const double *C, const double *u, const double *v,
+                                          double dt, const char *reconstruction_method, int step_count,
+                                          int two_phase, int fold_input_halo, const double *dJ_dCnew,
+                                          double *dJ_dC, double *dJ_du, double *dJ_dv) {
+  int code;
+  if (!h || !C || !u || !v || !dJ_dCnew || !dJ_dC || !dJ_du || !dJ_dv)
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (recon_code(reconstruction_method, &code))
+    return fail(COVOF_BAD_ARGUMENT, "reconstruction method must be PLIC or ELVIRA");
+  if (!(dt >= 0.0) || step_count < 0) return fail(COVOF_BAD_ARGUMENT, "dt and step_count must be >= 0");
+  if (dJ_du == dJ_dv || dJ_dC == dJ_du || dJ_dC == dJ_dv || dJ_dCnew == dJ_du || dJ_dCnew == dJ_dv ||
+      C == dJ_dC || u == dJ_du || v == dJ_dv)
+    return fail(COVOF_BAD_ARGUMENT, "covof_vof_advect_adjoint_d: outputs must not alias other arguments");
+  const int st = covof::vof_advect_adjoint(h->c, code, C, u, v, dt, step_count, two_phase, fold_input_halo,
+                                           dJ_dCnew, dJ_dC, dJ_du, dJ_dv);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "covof_vof_advect_adjoint_d: the mesh must be periodic in both directions or in neither, "
+                    "with the UNIFORM top profile");
+  return hip_status(st);
+}
+
+extern "C" int covof_vof_fill_halo_d(covof_ctx *h, double *C) {
+  if (!h || !C) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const int st = covof::vof_fill_halo(h->c, C);
+  if (st == COVOF_BAD_ARGUMENT)
+    return fail(st, "covof_vof_fill_halo_d: the mesh must be periodic in both directions or in neither");
+  return hip_status(st);
+}
+
+extern "C" int covof_kinetic_energy_d(covof_ctx *h, const double *C, const double *u, const double *v, double rho1,
+                                      double rho2, double *ke, double *dKE_dC, double *dKE_du, double *dKE_dv) {
+  if (!h || !C || !u || !v) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if ((dKE_dC && (dKE_dC == C || dKE_dC == u || dKE_dC == v)) || (dKE_du && (dKE_du == u || dKE_du == C)) ||
+      (dKE_dv && (dKE_dv == v || dKE_dv == C)) || (dKE_du && dKE_du == dKE_dv))
+    return fail(COVOF_BAD_ARGUMENT, "covof_kinetic_energy_d: outputs must not alias the inputs or each other");
+  return hip_status(covof::kinetic_energy(h->c, C, u, v, rho1, rho2, ke, dKE_dC, dKE_du, dKE_dv));
+}
+
+extern "C" int covof_properties_adjoint_d(covof_ctx *h, const double *C, double rho1, double rho2,
+                                          const double *dJ_dlogrho, double *dJ_dC) {
+  if (!h || !C || !dJ_dlogrho || !dJ_dC) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (dJ_dC == C) return fail(COVOF_BAD_ARGUMENT, "covof_properties_adjoint_d: dJ_dC must not alias C");
+  return hip_status(covof::properties_adjoint(h->c, C, rho1, rho2, dJ_dlogrho, dJ_dC));
+}
+
+extern "C" int covof_inertial_predictor_d(covof_ctx *h, int first, double dt, double gx, double gy, const double *u,
+                                          const double *v, const double *u_nm1, const double *v_nm1,
+                                          double *u_star, double *v_star) {
+  if (!h || !u || !v || !u_star || !v_star || (!first && (!u_nm1 || !v_nm1)))
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (u_star == v_star || u_star == u || v_star == v || u_star == u_nm1 || v_star == v_nm1)
+    return fail(COVOF_BAD_ARGUMENT, "covof_inertial_predictor_d: outputs must not alias the inputs");
+  const int st = covof::inertial_predictor(h->c, first, dt, gx, gy, u, v, u_nm1, v_nm1, u_star, v_star);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "every side must be a wall (DIRICHLET, FREESLIP) or periodic");
+  return hip_status(st);
+}
+
+extern "C" int covof_inertial_predictor_adjoint_d(covof_ctx *h, int first, const double *w_u, const double *w_v,
+                                                  double *dJ_du, double *dJ_dv, double *dJ_du_nm1,
+                                                  double *dJ_dv_nm1) {
+  if (!h || !w_u || !w_v || !dJ_du || !dJ_dv || (!first && (!dJ_du_nm1 || !dJ_dv_nm1)))
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  if (dJ_du == dJ_dv || dJ_du == w_u || dJ_dv == w_v || (!first && (dJ_du_nm1 == dJ_du || dJ_dv_nm1 == dJ_dv ||
+                                                                     dJ_du_nm1 == dJ_dv_nm1)))
+    return fail(COVOF_BAD_ARGUMENT, "covof_inertial_predictor_adjoint_d: outputs must be distinct arrays");
+  const int st = covof::inertial_predictor_adjoint(h->c, first, w_u, w_v, dJ_du, dJ_dv, dJ_du_nm1, dJ_dv_nm1);
+  if (st == COVOF_BAD_ARGUMENT) return fail(st, "every side must be a wall (DIRICHLET, FREESLIP) or periodic");
   return hip_status(st);
 }
 

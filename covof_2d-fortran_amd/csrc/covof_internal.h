@@ -28,6 +28,7 @@ struct Vof {
   int *list[1] = {nullptr};             // per-tile lists of mixed cells (linear halo index)
   int *count = nullptr;                 // per-tile list sizes
   int *offs = nullptr;                  // exclusive prefix of count (ntiles + 1 entries)
+  double *adj = nullptr;                // work arrays of the step adjoint (made on its first call)
 };
 
 // Work arrays of the surface steps (surface.hip), made on first use: two

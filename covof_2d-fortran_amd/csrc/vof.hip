@@ -766,6 +766,7 @@ void vof_free(Ctx &c) {
   hipFree(c.vof.list[0]);
   hipFree(c.vof.count);
   hipFree(c.vof.offs);
+  hipFree(c.vof.adj);
 }
 
 // Reconstruction of src: tile lists (left by the previous update or by
@@ -924,6 +925,543 @@ int vof_reconstruct_noalloc(const Mesh &mesh, int method, const double *C, doubl
   hipLaunchKernelGGL(k_expand_segments, grd, blk, 0, s, C, m, (const double *)onx, (const double *)ony,
                      (const double *)oal, onx, ony, oal);
   COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+// ======================================================= adjoint of one step
+//
+// Reverse mode of one advect_vof(WEYMOUTH_YUE) step C^n -> C^{n+1} on a mesh
+// with walls on all four sides or periodic in both directions (no property
+// update, no top profile).  The discrete map is differentiated with its
+// branch pattern frozen: the dilation indicator c_d, the donor of every face,
+// the empty / mixed / full class of every cell, ELVIRA's chosen candidate
+// and its sign factors, and the clamp of the second pass (derivative 1 where
+// the clamp left the value alone).  compute_alpha's 45-step bisection is
+// differentiated as the exact root it approximates (implicit function:
+// area(alpha, n) = c |cell|).  For a unit normal n and a box, with L the
+// length of the chord n.x + alpha = 0 inside the box and M its midpoint:
+//   d area / d alpha = -L,   d area / d n = -L M,
+//   d area / d xmin = -h(xmin), d area / d xmax = +h(xmax)
+// (h: the covered length of that box edge), so d alpha / d c = -|cell| / L and
+// d alpha / d n = -M in a cell.  The normal is m / |m| with m linear in the
+// 3x3 stencil (Youngs, or the chosen ELVIRA candidate), a ghost cell's normal
+// the renormalised normal of its source cell.
+
+// Chord of n.x + alpha = 0 in the box: length and midpoint (0 if it misses).
+__device__ void chord(double xmin, double xmax, double ymin, double ymax, double nx, double ny, double alpha,
+                      double &L, double &mx, double &my) {
+  const double px[4] = {xmin, xmax, xmax, xmin}, py[4] = {ymin, ymin, ymax, ymax};
+  double d[4];
+  for (int k = 0; k < 4; ++k) d[k] = nx * px[k] + ny * py[k] + alpha;
+  double qx[2] = {0.0, 0.0}, qy[2] = {0.0, 0.0};
+  int n = 0;
+  for (int k = 0; k < 4 && n < 2; ++k) {
+    const int k1 = (k + 1) & 3;
+    if (d[k] == 0.0) {
+      if (n == 0 || qx[0] != px[k] || qy[0] != py[k]) {
+        qx[n] = px[k];
+        qy[n] = py[k];
+        ++n;
+      }
+    } else if ((d[k] < 0.0) != (d[k1] < 0.0) && d[k1] != 0.0) {
+      const double t = d[k] / (d[k] - d[k1]);
+      qx[n] = px[k] + t * (px[k1] - px[k]);
+      qy[n] = py[k] + t * (py[k1] - py[k]);
+      ++n;
+    }
+  }
+  if (n < 2) {
+    L = 0.0;
+    mx = my = 0.0;
+    return;
+  }
+  L = sqrt((qx[1] - qx[0]) * (qx[1] - qx[0]) + (qy[1] - qy[0]) * (qy[1] - qy[0]));
+  mx = 0.5 * (qx[0] + qx[1]);
+  my = 0.5 * (qy[0] + qy[1]);
+}
+
+// Covered length of the segment [a0, a1] of the line x = X (vert) or y = Y
+// by {nx x + ny y + alpha <= 0}: t ranges over the free coordinate.
+__device__ double covered(double fixed, double a0, double a1, double nfix, double nfree, double alpha) {
+  const double r = -(nfix * fixed + alpha);  // nfree * t <= r
+  if (nfree == 0.0) return r >= 0.0 ? a1 - a0 : 0.0;
+  const double t = r / nfree;
+  double lo = a0, hi = a1;
+  if (nfree > 0.0) hi = t < hi ? t : hi;
+  else lo = t > lo ? t : lo;
+  return hi > lo ? hi - lo : 0.0;
+}
+
+// Flux of face (i, j) of pass DIR as k_flux_update computes it there (the
+// same closed forms for empty and full donors).
+template <int DIR>
+__device__ double adj_face_flux(const double *src, const double *vel, const double *sx, const double *sy,
+                                const double *sa, const DevMesh &m, int i, int j, double dt) {
+  const FaceBox b = face_box<DIR>(vel[H(m, i, j)], m, i, j, dt);
+  if (b.zero) return 0.0;
+  const double cdn = src[b.kd];
+  double a;
+  if (is_mixed(cdn)) a = cut_box_area(b.xmin, b.xmax, b.ymin, b.ymax, sx[b.kd], sy[b.kd], sa[b.kd]);
+  else if (cdn <= 1.0e-6) return b.pos ? 0.0 : -0.0;
+  else a = full_box_area(b.xmin, b.xmax, b.ymin, b.ymax);
+  return b.pos ? a : -a;
+}
+
+// g = dJ/dC_out * (clamp derivative) per interior cell, written to G and as
+// the direct part of dJ/dC_in.  The halo of dJ/dC_in starts at zero, except
+// with corners_through its four corners: dJ/dC_out's (the pass copied src's
+// corners into dst).
+template <int DIR>
+__global__ void k_adj_g(DevMesh m, const double *__restrict__ src, const double *__restrict__ Cn,
+                        const double *__restrict__ vel, const double *__restrict__ sx, const double *__restrict__ sy,
+                        const double *__restrict__ sa, const double *__restrict__ cbar_out, int clamp,
+                        int corners_through, double dt, double *__restrict__ G, double *__restrict__ cbar_in) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > m.nx + 1 || j > m.ny + 1) return;
+  const long k = H(m, i, j);
+  if (i < 1 || j < 1 || i > m.nx || j > m.ny) {
+    const bool corner = (i == 0 || i == m.nx + 1) && (j == 0 || j == m.ny + 1);
+    G[k] = 0.0;
+    cbar_in[k] = (corner && corners_through) ? cbar_out[k] : 0.0;
+    return;
+  }
+  double g = cbar_out[k];
+  if (clamp) {
+    int il = i, jl = j, ih = DIR == 0 ? i + 1 : i, jh = DIR == 0 ? j : j + 1;
+    pass_face<DIR>(vel, m, il, jl);
+    pass_face<DIR>(vel, m, ih, jh);
+    const double flo = adj_face_flux<DIR>(src, vel, sx, sy, sa, m, il, jl, dt);
+    const double fhi = adj_face_flux<DIR>(src, vel, sx, sy, sa, m, ih, jh, dt);
+    const double area = m.dx * m.dy;
+    const double dth = dt / (DIR == 0 ? m.dx : m.dy);
+    const double du = (DIR == 0 ? vel[H(m, i + 1, j)] : vel[H(m, i, j + 1)]) - vel[k];
+    const double cdv = Cn[k] >= 0.5 ? 1.0 : 0.0;
+    const double cpre = src[k] - (fhi - flo) / area + cdv * dth * du;  // k_flux_update's expression
+    // a clamp that removes no more than rounding (|overshoot| <= 1e-12, as
+    // full and empty cells carry) passes the derivative through: the
+    // one-sided derivative from inside [0, 1], the same whatever the
+    // rounding of the forward
+    if (cpre < -1.0e-12 || cpre > 1.0 + 1.0e-12) g = 0.0;
+  }
+  G[k] = g;
+  cbar_in[k] = g;
+}
+
+// Per face of the pass: the flux adjoint (update and periodic copy), the
+// velocity adjoint (dilation term and swept-box width, accumulated into vbar)
+// and the donor segment's adjoint (fs: d/dnx, d/dny, d/dalpha per face).
+template <int DIR>
+__global__ void k_adj_face(DevMesh m, const double *__restrict__ src, const double *__restrict__ Cn,
+                           const double *__restrict__ vel, const double *__restrict__ sx,
+                           const double *__restrict__ sy, const double *__restrict__ sa,
+                           const double *__restrict__ G, double dt, double *__restrict__ vbar,
+                           double *__restrict__ fs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = m.nx, ny = m.ny;
+  const long k = H(m, i, j);
+  const long nh = (long)(nx + 2) * (ny + 2);
+  const bool face = DIR == 0 ? (i >= 1 && i <= nx + 1 && j >= 1 && j <= ny)
+                             : (i >= 1 && i <= nx && j >= 1 && j <= ny + 1);
+  if (i > nx + 1 || j > ny + 1) return;
+  fs[k] = fs[nh + k] = fs[2 * nh + k] = 0.0;
+  if (!face) return;
+  const double area = m.dx * m.dy, dh = dt / (DIR == 0 ? m.dx : m.dy);
+  // g and c_d of the cells on the high (this face is their low face) and low side
+  auto gcell = [&](int a, int b) { return (a >= 1 && a <= nx && b >= 1 && b <= ny) ? G[H(m, a, b)] : 0.0; };
+  auto cdcell = [&](int a, int b) { return (a >= 1 && a <= nx && b >= 1 && b <= ny && Cn[H(m, a, b)] >= 0.5) ? 1.0 : 0.0; };
+  const int ia = DIR == 0 ? i - 1 : i, ja = DIR == 0 ? j : j - 1;  // the cell below / left
+  // dJ/dF as the update used it: + g(high) / |cell|, - g(low) / |cell|
+  auto fbar_used = [&](int fi, int fj) {
+    const int la = DIR == 0 ? fi - 1 : fi, lb = DIR == 0 ? fj : fj - 1;
+    return (gcell(fi, fj) - gcell(la, lb)) / area;
+  };
+  double fb = fbar_used(i, j);
+  // periodic copy: faces 1 and n+1 both use the flux computed at the face whose donor is real
+  const bool per = DIR == 0 ? (m.bcl == BC_PERIODIC) : (m.bcb == BC_PERIODIC);
+  const int last = DIR == 0 ? nx + 1 : ny + 1;
+  const int pos = DIR == 0 ? i : j;
+  if (per && (pos == 1 || pos == last)) {
+    const double v1 = DIR == 0 ? vel[H(m, 1, j)] : vel[H(m, i, 1)];
+    const int keep = v1 >= 0.0 ? last : 1;
+    const double both = DIR == 0 ? fbar_used(1, j) + fbar_used(last, j) : fbar_used(i, 1) + fbar_used(i, last);
+    fb = (pos == keep) ? both : 0.0;
+  }
+  // dilation term of the two cells: + cd(high) dh u_f ... - : d/du_f
+  double ub = dh * (cdcell(ia, ja) * gcell(ia, ja) - cdcell(i, j) * gcell(i, j));
+  const FaceBox b = face_box<DIR>(vel[k], m, i, j, dt);
+  if (!b.zero && fb != 0.0) {
+    const double sgn = b.pos ? 1.0 : -1.0;
+    const double c = src[b.kd];
+    const double nxd = sx[b.kd], nyd = sy[b.kd], ald = sa[b.kd];
+    // the swept box's far edge moves by -dt * sign(u) per unit u: dF/du = dt * h(far edge)
+    double hfar;
+    if (DIR == 0) {
+      const double X = b.pos ? b.xmin : b.xmax;
+      hfar = covered(X, b.ymin, b.ymax, nxd, nyd, ald);
+    } else {
+      const double Y = b.pos ? b.ymin : b.ymax;
+      hfar = covered(Y, b.xmin, b.xmax, nyd, nxd, ald);
+    }
+    if (c >= 1.0 - 1.0e-6) hfar = DIR == 0 ? b.ymax - b.ymin : b.xmax - b.xmin;
+    if (c <= 1.0e-6) hfar = 0.0;
+    ub += fb * dt * hfar;
+    if (is_mixed(c)) {
+      double L, mx, my;
+      chord(b.xmin, b.xmax, b.ymin, b.ymax, nxd, nyd, ald, L, mx, my);
+      const double w = fb * sgn;
+      fs[k] = -w * L * mx;
+      fs[nh + k] = -w * L * my;
+      fs[2 * nh + k] = -w * L;
+    }
+  }
+  vbar[k] += ub;
+}
+
+// Segment adjoint of every cell: the sum over the (at most two) faces of the
+// pass it is the donor of.
+template <int DIR>
+__global__ void k_adj_segbar(DevMesh m, const double *__restrict__ vel, const double *__restrict__ fs, double dt,
+                             double *__restrict__ sb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int nx = m.nx, ny = m.ny;
+  if (i > nx + 1 || j > ny + 1) return;
+  const long nh = (long)(nx + 2) * (ny + 2);
+  const long k = H(m, i, j);
+  double a = 0.0, bq = 0.0, c = 0.0;
+  auto take = [&](int fi, int fj, bool want_pos) {
+    const bool face = DIR == 0 ? (fi >= 1 && fi <= nx + 1 && fj >= 1 && fj <= ny)
+                               : (fi >= 1 && fi <= nx && fj >= 1 && fj <= ny + 1);
+    if (!face) return;
+    const long f = H(m, fi, fj);
+    const double uf = vel[f];
+    if (fabs(uf) * dt <= 1.0e-14 || (uf > 0.0) != want_pos) return;
+    a += fs[f];
+    bq += fs[nh + f];
+    c += fs[2 * nh + f];
+  };
+  if (DIR == 0) {
+    take(i + 1, j, true);   // the face above-right carries this cell's volume when u > 0
+    take(i, j, false);      // the low face when u < 0
+  } else {
+    take(i, j + 1, true);
+    take(i, j, false);
+  }
+  sb[k] = a;
+  sb[nh + k] = bq;
+  sb[2 * nh + k] = c;
+}
+
+// Source cell of a ghost (ring) cell's normal (reconstruct_interface's
+// exchange + outward copies): wrap on periodic axes, clamp on physical ones.
+__device__ __forceinline__ long ghost_src(const DevMesh &m, int i, int j) {
+  return H(m, wrap_or_clamp(i, m.nx, m.perx), wrap_or_clamp(j, m.ny, m.pery));
+}
+
+// Ring cells: the ghost segment's alpha and renormalised normal -> dJ/dC of
+// the ring cell, and per ring entry the source normal's adjoint (gathered in
+// ring order by k_adj_ghost_acc).
+__global__ void k_adj_ghost(DevMesh m, const double *__restrict__ src, const double *__restrict__ sx,
+                            const double *__restrict__ sy, const double *__restrict__ sa,
+                            const double *__restrict__ sb, double *__restrict__ cbar, double *__restrict__ ring) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ring_len(m)) return;
+  int i, j;
+  ring_cell(m, r, i, j);
+  const long nh = (long)(m.nx + 2) * (m.ny + 2);
+  const long k = H(m, i, j);
+  const long ks = ghost_src(m, i, j);
+  ring[3 * r] = ring[3 * r + 1] = 0.0;
+  ring[3 * r + 2] = (double)ks;
+  const double c = src[k];
+  double nbx = sb[k], nby = sb[nh + k], ab = sb[2 * nh + k];
+  if (!is_mixed(c) || (nbx == 0.0 && nby == 0.0 && ab == 0.0)) return;
+  const double xmin = face_coord(i, m.dx), xmax = face_coord(i + 1, m.dx);
+  const double ymin = face_coord(j, m.dy), ymax = face_coord(j + 1, m.dy);
+  const double nx = sx[k], ny = sy[k];
+  double L, mx, my;
+  chord(xmin, xmax, ymin, ymax, nx, ny, sa[k], L, mx, my);
+  if (L > 0.0) {
+    cbar[k] += ab * (-(m.dx * m.dy) / L);
+    nbx += ab * (-mx);
+    nby += ab * (-my);
+  }
+  // n = s / |s| with s the source cell's unit normal (a non-mixed source
+  // gives the constant (1, 0))
+  if (!is_mixed(src[ks])) return;
+  const double s = hypot_ref(sx[ks], sy[ks]);
+  if (!(s > 1.0e-12)) return;
+  const double dot = nbx * nx + nby * ny;
+  ring[3 * r] = (nbx - dot * nx) / s;
+  ring[3 * r + 1] = (nby - dot * ny) / s;
+}
+
+__global__ void k_adj_ghost_acc(DevMesh m, const double *__restrict__ ring, double *__restrict__ sb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const long nh = (long)(m.nx + 2) * (m.ny + 2);
+  const int L = ring_len(m);
+  for (int r = 0; r < L; ++r) {
+    const long ks = (long)ring[3 * r + 2];
+    sb[ks] += ring[3 * r];
+    sb[nh + ks] += ring[3 * r + 1];
+  }
+}
+
+// d(cnx, cny)/dC of ELVIRA candidate k (0..6, 6 = Youngs) over the 3x3
+// stencil, index (di + 1) + 3 (dj + 1), with the sign factors frozen.
+__device__ void cand_weights(int k, double sgn_x, double sgn_y, double dx, double dy, double *wx, double *wy) {
+  for (int q = 0; q < 9; ++q) wx[q] = wy[q] = 0.0;
+  auto at = [](int di, int dj) { return (di + 1) + 3 * (dj + 1); };
+  const double w3[3] = {1.0, 2.0, 1.0};
+  for (int d = -1; d <= 1; ++d) {
+    if (k == 0) { wx[at(1, d)] += -sgn_y * dy / (2.0 * dx); wx[at(-1, d)] += sgn_y * dy / (2.0 * dx); }
+    if (k == 1) { wx[at(1, d)] += -sgn_y * dy / dx; wx[at(0, d)] += sgn_y * dy / dx; }
+    if (k == 2) { wx[at(0, d)] += -sgn_y * dy / dx; wx[at(-1, d)] += sgn_y * dy / dx; }
+    if (k == 3) { wy[at(d, 1)] += -sgn_x * dx / (2.0 * dy); wy[at(d, -1)] += sgn_x * dx / (2.0 * dy); }
+    if (k == 4) { wy[at(d, 1)] += -sgn_x * dx / dy; wy[at(d, 0)] += sgn_x * dx / dy; }
+    if (k == 5) { wy[at(d, 0)] += -sgn_x * dx / dy; wy[at(d, -1)] += sgn_x * dx / dy; }
+    if (k == 6) {
+      wx[at(1, d)] += -w3[d + 1] / (8.0 * dx);
+      wx[at(-1, d)] += w3[d + 1] / (8.0 * dx);
+      wy[at(d, 1)] += -w3[d + 1] / (8.0 * dy);
+      wy[at(d, -1)] += w3[d + 1] / (8.0 * dy);
+    }
+  }
+}
+
+// Interior mixed cells: alpha's dependence on the cell's own C and on the
+// normal, then the normal's on the 3x3 stencil (W: 9 weights per cell).
+template <int METHOD>
+__global__ void k_adj_interior(DevMesh m, const double *__restrict__ src, const double *__restrict__ sx,
+                               const double *__restrict__ sy, const double *__restrict__ sa,
+                               const double *__restrict__ sb, double *__restrict__ cbar, double *__restrict__ W) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  if (i > m.nx || j > m.ny) return;
+  const long nh = (long)(m.nx + 2) * (m.ny + 2);
+  const long k = H(m, i, j);
+  for (int q = 0; q < 9; ++q) W[q * nh + k] = 0.0;
+  const double c = src[k];
+  double nbx = sb[k], nby = sb[nh + k], ab = sb[2 * nh + k];
+  if (!is_mixed(c) || (nbx == 0.0 && nby == 0.0 && ab == 0.0)) return;
+  const double dx = m.dx, dy = m.dy;
+  // the candidate the reconstruction chose (PLIC: Youngs).  ELVIRA: the
+  // derivative is taken along the lowest-index candidate that ties with the
+  // winner to rounding -- error within 1e-10 relative and the same unit
+  // normal to 1e-9 (Youngs' normal and the central height-function one
+  // coincide wherever the outer rows or columns of the block are uniform,
+  // and rounding alone then decides the winner)
+  int kc = 6;
+  if (METHOD == 1) {
+    double cux[7], cuy[7], cer[7];
+    bool cok[7];
+    double best = 1.0e30;
+    int win = -1;
+    for (int q = 0; q < 7; ++q) {
+      double al;
+      elvira_candidate(src, m, i, j, c, q, cux[q], cuy[q], al, cer[q], cok[q]);
+      if (cok[q] && cer[q] < best) {
+        best = cer[q];
+        win = q;
+      }
+    }
+    if (win < 0) return;  // no candidate: the constant segment (1, 0, 0)
+    const double tol = 1.0e-10 * best + 1.0e-20 * (dx * dy) * (dx * dy);
+    kc = win;
+    for (int q = 0; q < win; ++q)
+      if (cok[q] && cer[q] <= best + tol && fabs(cux[q] - cux[win]) + fabs(cuy[q] - cuy[win]) <= 1.0e-9) {
+        kc = q;
+        break;
+      }
+  }
+  const double nx = sx[k], ny = sy[k];
+  double L, mx, my;
+  chord(face_coord(i, dx), face_coord(i + 1, dx), face_coord(j, dy), face_coord(j + 1, dy), nx, ny, sa[k], L, mx,
+        my);
+  if (L > 0.0) {
+    cbar[k] += ab * (-(dx * dy) / L);
+    nbx += ab * (-mx);
+    nby += ab * (-my);
+  }
+#define CC(a, b) src[H(m, i + (a), j + (b))]
+  const double sgn_y = (CC(0, 1) < CC(0, -1)) ? 1.0 : -1.0;
+  const double sgn_x = (CC(1, 0) < CC(-1, 0)) ? 1.0 : -1.0;
+  double cx, cy;  // the unnormalised candidate
+  if (kc == 6) {
+    youngs(src, m, i, j, cx, cy);
+  } else if (kc < 3) {
+    const double H_m1 = (CC(-1, -1) + CC(-1, 0) + CC(-1, 1)) * dy;
+    const double H_0 = (CC(0, -1) + CC(0, 0) + CC(0, 1)) * dy;
+    const double H_p1 = (CC(1, -1) + CC(1, 0) + CC(1, 1)) * dy;
+    cx = kc == 0 ? -(sgn_y * (H_p1 - H_m1) / (2.0 * dx)) : kc == 1 ? -(sgn_y * (H_p1 - H_0) / dx)
+                                                                   : -(sgn_y * (H_0 - H_m1) / dx);
+    cy = sgn_y;
+  } else {
+    const double W_m1 = (CC(-1, -1) + CC(0, -1) + CC(1, -1)) * dx;
+    const double W_0 = (CC(-1, 0) + CC(0, 0) + CC(1, 0)) * dx;
+    const double W_p1 = (CC(-1, 1) + CC(0, 1) + CC(1, 1)) * dx;
+    cx = sgn_x;
+    cy = kc == 3 ? -(sgn_x * (W_p1 - W_m1) / (2.0 * dy)) : kc == 4 ? -(sgn_x * (W_p1 - W_0) / dy)
+                                                                   : -(sgn_x * (W_0 - W_m1) / dy);
+  }
+#undef CC
+  const double cn = hypot_ref(cx, cy);
+  if (!(cn > 1.0e-12)) return;
+  const double dot = nbx * nx + nby * ny;
+  const double mbx = (nbx - dot * nx) / cn, mby = (nby - dot * ny) / cn;
+  double wx[9], wy[9];
+  cand_weights(kc, sgn_x, sgn_y, dx, dy, wx, wy);
+  for (int q = 0; q < 9; ++q) W[q * nh + k] = mbx * wx[q] + mby * wy[q];
+}
+
+// dJ/dC of every cell from the stencil weights of the interior cells around it.
+__global__ void k_adj_gather(DevMesh m, const double *__restrict__ W, double *__restrict__ cbar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > m.nx + 1 || j > m.ny + 1) return;
+  const long nh = (long)(m.nx + 2) * (m.ny + 2);
+  double s = 0.0;
+  for (int dj = -1; dj <= 1; ++dj)
+    for (int di = -1; di <= 1; ++di) {
+      const int a = i - di, b = j - dj;  // the cell whose stencil holds (i, j) at offset (di, dj)
+      if (a < 1 || a > m.nx || b < 1 || b > m.ny) continue;
+      s += W[(long)((di + 1) + 3 * (dj + 1)) * nh + H(m, a, b)];
+    }
+  cbar[H(m, i, j)] += s;
+}
+
+// Halo of a field copied from its interior (zero-gradient copy or wrap, the
+// corners from the diagonal cell): moves the halo entries of cbar onto their
+// interior sources in ring order (one thread, a fixed summation order).
+// corners == 0 leaves the four corner entries in place (their values came
+// from another field).
+__global__ void k_adj_fold(DevMesh m, double *__restrict__ cbar, int corners) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nx = m.nx, ny = m.ny;
+  const int L = ring_len(m);
+  for (int r = 0; r < L; ++r) {
+    int i, j;
+    ring_cell(m, r, i, j);
+    const bool corner = (i == 0 || i == nx + 1) && (j == 0 || j == ny + 1);
+    if (corner && !corners) continue;
+    const long k = H(m, i, j);
+    const double v = cbar[k];
+    cbar[k] = 0.0;
+    cbar[H(m, wrap_or_clamp(i, nx, m.perx), wrap_or_clamp(j, ny, m.pery))] += v;
+  }
+}
+
+// Halo of C as the transport leaves it with the two-phase corner fix, from
+// the interior alone: zero-gradient copies on physical sides, the wrap on
+// periodic ones, the corners from the diagonal cell (one thread per ring cell).
+__global__ void k_fill_halo(DevMesh m, double *C) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ring_len(m)) return;
+  int i, j;
+  ring_cell(m, r, i, j);
+  C[H(m, i, j)] = C[H(m, wrap_or_clamp(i, m.nx, m.perx), wrap_or_clamp(j, m.ny, m.pery))];
+}
+
+int vof_fill_halo(Ctx &c, double *C) {
+  if (c.m.perx != c.m.pery) return COVOF_BAD_ARGUMENT;
+  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
+  hipLaunchKernelGGL(k_fill_halo, dim3((L + 255) / 256), dim3(256), 0, c.stream, dev_mesh(c.m), C);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+// Work arrays of the step adjoint (made on the first call): the first
+// pass's segments (3), g, the per-face segment adjoints (3), the per-cell
+// segment adjoints (3), the stencil weights (9), dJ/dC* and a copy of
+// dJ/dC^{n+1} (halo-size each), and the ring records (3 per ring cell).
+static int adj_alloc(Ctx &c) {
+  if (c.vof.adj) return COVOF_OK;
+  const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+  const long L = 2L * (c.m.nx + 2) + 2L * c.m.ny;
+  COVOF_CHECK(hipMalloc(&c.vof.adj, (21 * nh + 3 * L) * sizeof(double)));
+  return COVOF_OK;
+}
+
+// Reverse mode of one pass (src -> dst with the segments sx, sy, sa of src):
+// dJ/dC_out -> dJ/dC_in, and dJ/d(face velocity) added to velbar.
+template <int DIR>
+static int adj_pass(Ctx &c, int method, const double *src, const double *Cn, const double *vel, const double *sx,
+                    const double *sy, const double *sa, const double *cbar_out, int clamp, int corners_through,
+                    double dt, double *cbar_in, double *velbar) {
+  const DevMesh m = dev_mesh(c.m);
+  const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+  double *G = c.vof.adj + 3 * nh, *fs = G + nh, *sb = fs + 3 * nh, *W = sb + 3 * nh, *ring = W + 9 * nh + 2 * nh;
+  const dim3 blk(64, 4), grd((c.m.nx + 2 + 63) / 64, (c.m.ny + 2 + 3) / 4);
+  const dim3 grd_in((c.m.nx + 63) / 64, (c.m.ny + 3) / 4);
+  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
+  hipLaunchKernelGGL(k_adj_g<DIR>, grd, blk, 0, c.stream, m, src, Cn, vel, sx, sy, sa, cbar_out, clamp,
+                     corners_through, dt, G, cbar_in);
+  hipLaunchKernelGGL(k_adj_face<DIR>, grd, blk, 0, c.stream, m, src, Cn, vel, sx, sy, sa, (const double *)G, dt,
+                     velbar, fs);
+  hipLaunchKernelGGL(k_adj_segbar<DIR>, grd, blk, 0, c.stream, m, vel, (const double *)fs, dt, sb);
+  hipLaunchKernelGGL(k_adj_ghost, dim3((L + 255) / 256), dim3(256), 0, c.stream, m, src, sx, sy, sa,
+                     (const double *)sb, cbar_in, ring);
+  hipLaunchKernelGGL(k_adj_ghost_acc, dim3(1), dim3(64), 0, c.stream, m, (const double *)ring, sb);
+  hipLaunchKernelGGL(method == 1 ? k_adj_interior<1> : k_adj_interior<0>, grd_in, blk, 0, c.stream, m, src, sx, sy,
+                     sa, (const double *)sb, cbar_in, W);
+  hipLaunchKernelGGL(k_adj_gather, grd, blk, 0, c.stream, m, (const double *)W, cbar_in);
+  COVOF_CHECK_LAUNCH();
+  return COVOF_OK;
+}
+
+// Reverse mode of one vof_advect step (WEYMOUTH_YUE, pass order from
+// step_count, the two-phase corner fix with two_phase): recomputes the
+// step's intermediate field C* and both passes' segments, then runs the
+// passes backwards.  cbar_new = dJ/dC^{n+1} (halo layout, halo entries
+// included); cbar = dJ/dC^n (halo layout; with fold_in the halo entries are
+// moved onto the interior cells they copy, else they are the derivatives
+// with respect to C^n's halo values as given); ubar, vbar += dJ/du, dJ/dv.
+int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, const double *v, double dt,
+                       int step_count, int two_phase, int fold_in, const double *cbar_new, double *cbar,
+                       double *ubar, double *vbar) {
+  if (c.m.perx != c.m.pery || (c.jet.on && !c.m.pery)) return COVOF_BAD_ARGUMENT;
+  int st = adj_alloc(c);
+  if (st) return st;
+  const long nh = (long)(c.m.nx + 2) * (c.m.ny + 2);
+  const DevMesh m = dev_mesh(c.m);
+  const int per = c.m.perx;
+  const bool x_first = (step_count % 2) == 0;
+  double *seg1 = c.vof.adj, *cmid = c.vof.adj + 19 * nh, *cout = cmid + nh;
+  // the forward step's first pass (C -> C*) and the segments of C and of C*
+  st = launch_classify(c, C);
+  if (!st) st = launch_pass(c, method, x_first ? 0 : 1, true, C, c.vof.Cb, u, v, dt, false);
+  if (st) return st;
+  COVOF_CHECK(hipMemcpyAsync(seg1, c.vof.snx, nh * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  COVOF_CHECK(hipMemcpyAsync(seg1 + nh, c.vof.sny, nh * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  COVOF_CHECK(hipMemcpyAsync(seg1 + 2 * nh, c.vof.sal, nh * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  st = launch_recon(c, method, c.vof.Cb);
+  if (st) return st;
+  // second pass backwards: dJ/dC^{n+1} (its halo onto the interior it copies)
+  COVOF_CHECK(hipMemcpyAsync(cout, cbar_new, nh * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+  hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cout, (per || two_phase) ? 1 : 0);
+  COVOF_CHECK_LAUNCH();
+  const double *Cs = c.vof.Cb;
+  if (x_first)
+    st = adj_pass<1>(c, method, Cs, C, v, c.vof.snx, c.vof.sny, c.vof.sal, cout, 1, (!per && !two_phase) ? 1 : 0, dt,
+                     cmid, vbar);
+  else
+    st = adj_pass<0>(c, method, Cs, C, u, c.vof.snx, c.vof.sny, c.vof.sal, cout, 1, (!per && !two_phase) ? 1 : 0, dt,
+                     cmid, ubar);
+  if (st) return st;
+  // C*'s halo: edges copied from its interior, corners from C's (walls)
+  hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cmid, per);
+  COVOF_CHECK_LAUNCH();
+  if (x_first)
+    st = adj_pass<0>(c, method, C, C, u, seg1, seg1 + nh, seg1 + 2 * nh, cmid, 0, per ? 0 : 1, dt, cbar, ubar);
+  else
+    st = adj_pass<1>(c, method, C, C, v, seg1, seg1 + nh, seg1 + 2 * nh, cmid, 0, per ? 0 : 1, dt, cbar, vbar);
+  if (st) return st;
+  if (fold_in) {
+    hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cbar, 1);
+    COVOF_CHECK_LAUNCH();
+  }
   return COVOF_OK;
 }
 

@@ -451,6 +451,71 @@ int covof_projection_adjoint_d(covof_ctx *ctx, const double *rho, const double *
                                const double *w_p, double *dJ_du, double *dJ_dv, double *dJ_dlogrho,
                                int precond, double tol, int maxit, int precision, int *iters, double *relres);
 
+/* --- multi-step adjoint of the kinetic energy (no reference counterpart) -- */
+/* The reverse-mode pieces of a two-phase step reduced to transport, density
+ * update, inertial predictor and projection; a host loop over stored
+ * per-step states chains them with covof_projection_adjoint_d into the
+ * gradient of the final kinetic energy with respect to the initial C. */
+
+/* Reverse mode of one advect_vof step with WEYMOUTH_YUE (covof_vof_advect_d
+ * with nsteps = 1 and the given step_count, which fixes the pass order;
+ * two_phase: the physical-corner fix of the halo).  C, u, v: the step's
+ * inputs (halo layout).  dJ_dCnew = dJ/dC^{n+1} (halo layout, halo entries
+ * included); dJ_dC = dJ/dC^n (halo layout; may be dJ_dCnew itself): with
+ * fold_input_halo the halo entries are moved onto the interior cells that
+ * C^n's halo copies (zero-gradient copy or wrap, corners from the diagonal
+ * cell), else they stay the derivatives with respect to the halo values as
+ * given.  dJ_du, dJ_dv: dJ/du, dJ/dv are ADDED to them (face layout).  The
+ * map is differentiated with its branches frozen (dilation indicator, donor
+ * and empty / mixed / full class of every face and cell, ELVIRA's choice
+ * and sign factors, the second pass's clamp), the line constant as the
+ * exact root of the volume equation.  Where rounding alone decides a branch
+ * the derivative follows a fixed rule: a clamp that removes at most 1e-12
+ * passes it, and an ELVIRA winner whose error ties within 1e-10 (relative)
+ * with a lower-index candidate of the same unit normal (to 1e-9) is
+ * differentiated along that candidate.  Meshes periodic in both directions
+ * or in neither, UNIFORM top profile. */
+int covof_vof_advect_adjoint_d(covof_ctx *ctx, const double *C, const double *u, const double *v, double dt,
+                               const char *reconstruction_method, int step_count, int two_phase,
+                               int fold_input_halo, const double *dJ_dCnew, double *dJ_dC, double *dJ_du,
+                               double *dJ_dv);
+
+/* C's halo from its interior as the transport leaves it with two_phase:
+ * zero-gradient copies on physical sides (apply_vof_bc,
+ * src/module_mpi_comm.f90:636-683), the wrap on periodic ones, the corners
+ * from the diagonal cell (src/module_interface_vof.f90:532-535).  Meshes
+ * periodic in both directions or in neither. */
+int covof_vof_fill_halo_d(covof_ctx *ctx, double *C);
+
+/* Kinetic energy of the history row (src/module_vtk_writer.f90:399-531),
+ * KE = sum 0.5 rho (0.5 (uw^2 + ue^2) + 0.5 (vs^2 + vn^2)) dx dy with rho
+ * from the clamped C as update_physical_properties forms it; into *ke (host
+ * memory, synchronises; NULL: skipped), and its gradients dKE/dC, dKE/du,
+ * dKE/dv (halo layout, overwritten; each may be NULL). */
+int covof_kinetic_energy_d(covof_ctx *ctx, const double *C, const double *u, const double *v, double rho1,
+                           double rho2, double *ke, double *dKE_dC, double *dKE_du, double *dKE_dv);
+
+/* Reverse mode of the density of update_physical_properties
+ * (src/module_interface_vof.f90:552-553): dJ_dC(interior) += dJ/d(log rho)
+ * (nx*ny) * (rho1 - rho2) / rho where the clamp passes C through. */
+int covof_properties_adjoint_d(covof_ctx *ctx, const double *C, double rho1, double rho2,
+                               const double *dJ_dlogrho, double *dJ_dC);
+
+/* u* of the momentum block with convection and viscosity off: the
+ * diagonal alpha rho / dt (alpha = 1.5 with BDF2) and the right-hand side
+ * rho (2 u^n - 0.5 u^{n-1}) / dt + rho g (src/module_navier_stokes.f90:
+ * 841-847, :895-904) give u* = (4 u^n - u^{n-1} + 2 dt g) / 3, or
+ * u^n + dt g when first (explicit Euler inertia); zero on wall
+ * faces and outside the face ranges.  Every side a wall (DIRICHLET,
+ * FREESLIP) or periodic.  u_nm1, v_nm1 may be NULL when first. */
+int covof_inertial_predictor_d(covof_ctx *ctx, int first, double dt, double gx, double gy, const double *u,
+                               const double *v, const double *u_nm1, const double *v_nm1, double *u_star,
+                               double *v_star);
+/* Its transpose: dJ_du += a w_u, dJ_du_nm1 += b w_u (and v) on the
+ * predictor's faces, a = 4/3, b = -1/3 (first: a = 1, no n-1 term). */
+int covof_inertial_predictor_adjoint_d(covof_ctx *ctx, int first, const double *w_u, const double *w_v,
+                                       double *dJ_du, double *dJ_dv, double *dJ_du_nm1, double *dJ_dv_nm1);
+
 #ifdef __cplusplus
 }
 #endif
