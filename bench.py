@@ -8,8 +8,9 @@ solve of the same system (FP32 Krylov recurrence, FP64 reliable updates of
 the true residual) and the multigrid-preconditioned solves, the ELVIRA +
 Weymouth-Yue VOF step at 2048^2 and 4096^2 (128 seeded ellipses,
 divergence-free Fourier velocity, CFL 0.25), the SP predictor, the whole
-two-phase time step at 4096 x 2048 (500 BDF2 steps), the adjoint, the
-plain-method case and the monolithic step, and, on the host cores, the
+two-phase time step at 4096 x 2048 (500 BDF2 steps), the 16-step
+checkpointed kinetic-energy adjoint at 4096 x 2048, the projection adjoint,
+the plain-method case and the monolithic step, and, on the host cores, the
 oracle's FP64 Jacobi-PCG solve of the same system and its ELVIRA + WY step as
 the CPU baselines.  Prints one JSON line.
 
@@ -463,6 +464,7 @@ def main():
                "max_cfl_sampled": umax * dts / (2.0 / sx)}
     log(f"SP step {sx}x{sy} x{n500}: {t_step * 1e3:.2f} ms/step, Poisson {sp_step['poisson_iters_mean']:.0f} it "
         f"(max {max(pit)}), all converged {sp_step['all_converged']}, KE {sp_step['kinetic_energy']:.3e}")
+    Cw, uw, vw = (run.f[k].clone() for k in ("C", "u", "v"))
     del run
     # the reference's curvature and the FP64 Poisson solve beside it, 5 steps each
     for curv, prec in (("normals", cv.PRECISION_MIXED), ("height_function", cv.PRECISION_FP64)):
@@ -484,6 +486,53 @@ def main():
         del run
     sp_step["reference_curvature"] = steps_out[("normals", cv.PRECISION_MIXED)]
     sp_step["fp64_poisson"] = steps_out[("height_function", cv.PRECISION_FP64)]
+
+    # ------------------------- the 16-step checkpointed kinetic-energy adjoint
+    # dJ/dC0 of J = KE after 16 steps, from the state the 500-step run ended
+    # in, with the momentum block reduced to BDF2 inertia and gravity
+    # (KineticEnergyAdjoint): forward with per-step checkpoints, then the
+    # reverse sweep; forward and transpose solves FP64 MG-PCG to 1e-12; one
+    # central difference (h = 1e-5) along a seeded direction on the mixed
+    # cells as a check (entries uniform in [0, 1]: the gradient there is
+    # mostly positive, so the directional derivative stands well above the
+    # ~1e-13 relative rounding noise of J, which a zero-mean direction's
+    # near-cancelling derivative would not)
+    nwin = 16
+    kadj = cv.KineticEnergyAdjoint(sctx, 1000.0, 1.0, dts, gy=-9.81, reconstruction="PLIC", tol=1e-12,
+                                   precond=cv.PRECOND_MG)
+    kadj.forward(Cw, uw, vw, 1)  # warm-up: graph captures, work arrays
+    kadj.backward()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    Jk = kadj.forward(Cw, uw, vw, nwin)
+    torch.cuda.synchronize()
+    t_kf = time.perf_counter() - t
+    t = time.perf_counter()
+    gC, _, _ = kadj.backward()
+    torch.cuda.synchronize()
+    t_kb = time.perf_counter() - t
+    fwd_iters = list(kadj.iters)
+    Cin = Cw[1:sy + 1, 1:sx + 1]
+    mixed_cells = (Cin >= 0.02) & (Cin <= 0.98)
+    gen = torch.Generator(device=dev).manual_seed(11)
+    dk = torch.zeros_like(Cw)
+    dk[1:sy + 1, 1:sx + 1] = torch.rand(Cin.shape, generator=gen, device=dev, dtype=torch.float64) * mixed_cells
+    adk = float((gC * dk).sum())
+    gscale = float(gC[1:sy + 1, 1:sx + 1][mixed_cells].norm())
+    hk = 1e-5
+    fdk = (kadj.forward(Cw + hk * dk, uw, vw, nwin) - kadj.forward(Cw - hk * dk, uw, vw, nwin)) / (2.0 * hk)
+    nh_w = (sx + 2) * (sy + 2)
+    ke_adjoint = {"workload": f"16-step checkpointed adjoint of the kinetic energy w.r.t. C0, {sx}x{sy} walls, "
+                              f"from the state after the {n500}-step run ({nbub} bubbles), rho 1000/1, g 9.81, "
+                              "dt 1e-4, PLIC + WY, momentum block reduced to BDF2 inertia + gravity; forward and "
+                              "transpose FP64 MG-PCG to 1e-12",
+                  "steps": nwin, "kinetic_energy": Jk, "forward_s": t_kf, "backward_s": t_kb,
+                  "forward_poisson_iters": fwd_iters, "checkpoint_bytes": 4 * 8 * nh_w * (nwin + 1),
+                  "fd_direction": {"h": hk, "fd": fdk, "ad": adk, "grad_norm_mixed": gscale,
+                                   "rel_err": abs(fdk - adk) / abs(adk)}}
+    log(f"KE adjoint {sx}x{sy} x{nwin}: forward {t_kf:.2f} s, backward {t_kb:.2f} s, FD rel "
+        f"{ke_adjoint['fd_direction']['rel_err']:.2e}")
+    del kadj, gC, dk, Cw, uw, vw
     del sctx, zs, C_dev
 
     # --------------------------------------------- the adjoint of the projection
@@ -675,10 +724,25 @@ def main():
         "sp_predictor": predictor,
         "sp_step": sp_step,
         "adjoint": adjoint,
+        "ke_adjoint_16step": ke_adjoint,
         "plain_case": plain,
         "monolithic": mono,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(_plain(out)), flush=True)
+
+
+def _plain(o, path="out"):
+    """The result with one-element tensors as numbers (and a log line naming
+    each, so the producing expression can be fixed)."""
+    import torch
+    if isinstance(o, dict):
+        return {k: _plain(v, f"{path}.{k}") for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [_plain(v, f"{path}[{i}]") for i, v in enumerate(o)]
+    if isinstance(o, torch.Tensor):
+        log(f"note: tensor value at {path}")
+        return o.item() if o.numel() == 1 else o.tolist()
+    return o
 
 
 if __name__ == "__main__":
