@@ -26,6 +26,9 @@ CURVATURE_NORMALS, CURVATURE_HEIGHT_FUNCTION = 0, 1
 # parallel preconditioners of this library (no reference counterpart)
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+OPT_MIXED_STALL_WINDOW, OPT_MIXED_CYCLE, OPT_MIXED_LOOP, OPT_MG_COARSE, OPT_MG_SWEEPS, OPT_MG_OMEGA = 1, 2, 3, 4, 5, 6
+(RESERVE_MOMENTUM, RESERVE_MULTIGRID, RESERVE_SGS, RESERVE_MIXED_LOOP, RESERVE_SURFACE, RESERVE_ADJOINT,
+ RESERVE_HOST_MIRRORS, RESERVE_ALL) = 1, 2, 4, 8, 16, 32, 64, 127
 REDUCTION_TREE, REDUCTION_REFERENCE = 0, 1
 TIME_SCHEME_EULER, TIME_SCHEME_GEAR = 1, 2
 CONV_UPWIND, CONV_CENTERED, CONV_TVD_SUPERBEE = 1, 2, 3
@@ -91,6 +94,9 @@ SIGNATURES = {
     "covof_projection_adjoint_d": (_I, [_P, _P, _P, _P, _D] + [_P] * 7 + [_I, _D, _I, _I, _IP, _DP]),
     "covof_vof_advect_adjoint_d": (_I, [_P, _P, _P, _P, _D, ct.c_char_p, _I, _I, _I, _P, _P, _P, _P]),
     "covof_vof_fill_halo_d": (_I, [_P, _P]),
+    "covof_ctx_set_solver_option": (_I, [_P, _I, _D]),
+    "covof_ctx_mixed_solve_info": (_I, [_P, _IP, _IP]),
+    "covof_ctx_reserve": (_I, [_P, _I]),
     "covof_kinetic_energy_d": (_I, [_P, _P, _P, _P, _D, _D, _DP, _P, _P, _P]),
     "covof_properties_adjoint_d": (_I, [_P, _P, _D, _D, _P, _P]),
     "covof_inertial_predictor_d": (_I, [_P, _I, _D, _D, _D, _P, _P, _P, _P, _P, _P]),

@@ -174,6 +174,24 @@ class Context:
                 "covof_update_properties_cs_d")
         return props
 
+    def set_solver_option(self, option: int, value: float):
+        """covof_ctx_set_solver_option: one of the OPT_* tunings of the
+        Poisson solves (drops the captured cycles and coarse operators)."""
+        L.check(self._lib.covof_ctx_set_solver_option(self._h, int(option), float(value)),
+                "covof_ctx_set_solver_option")
+
+    def reserve(self, parts: int = L.RESERVE_ALL):
+        """covof_ctx_reserve: make the lazily made work arrays now (an OR of
+        RESERVE_* parts), so the calls of a time step allocate nothing."""
+        L.check(self._lib.covof_ctx_reserve(self._h, int(parts)), "covof_ctx_reserve")
+
+    def mixed_solve_info(self):
+        """(FP32 iterations, FP64 continuation iterations) of the last
+        mixed-precision CG solve."""
+        a, b = ct.c_int(0), ct.c_int(0)
+        L.check(self._lib.covof_ctx_mixed_solve_info(self._h, ct.byref(a), ct.byref(b)), "covof_ctx_mixed_solve_info")
+        return a.value, b.value
+
     def set_obstacle(self, obs_type=None, x0=0.0, y0=0.0, x1=0.0, y1=0.0, xc=0.0, yc=0.0, radius=0.0,
                      perm=0.0):
         """The solid obstacle of config_t (obs_type 1 box [x0,x1]x[y0,y1], 2

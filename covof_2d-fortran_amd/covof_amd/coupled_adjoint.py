@@ -55,6 +55,8 @@ class KineticEnergyAdjoint:
         self.tol, self.max_iter, self.precond = float(tol), int(max_iter), int(precond)
         self.C, self.u, self.v, self.x = [], [], [], []
         self.iters = []
+        ctx.reserve(L.RESERVE_ADJOINT | (L.RESERVE_MULTIGRID if self.precond in (L.PRECOND_MG, L.PRECOND_SGS_RB)
+                                         else 0))
 
     def _props(self, C):
         ctx = self.ctx

@@ -78,6 +78,15 @@ class StandardProjectionRun:
         self.poisson = poisson
         self.svof = None if svof is None else float(svof)
         self.diffusion = None if diffusion is None else int(diffusion)
+        # the work arrays of every block the step uses, made now: the steps
+        # themselves allocate nothing on the device
+        parts = L.RESERVE_MOMENTUM | L.RESERVE_SURFACE
+        pcs = {int(predictor[1]), int(poisson[1])}
+        if pcs & {L.PRECOND_MG, L.PRECOND_SGS_RB}:
+            parts |= L.RESERVE_MULTIGRID
+        if L.PRECOND_SSOR in pcs:
+            parts |= L.RESERVE_SGS
+        ctx.reserve(parts)
         f = {}
         f["C"] = ctx._field(C, "C").clone()
         f["u"] = ctx._field(u, "u").clone()

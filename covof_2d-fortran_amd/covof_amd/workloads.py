@@ -28,12 +28,12 @@ BC_DIRICHLET, BC_NEUMANN, BC_PERIODIC, BC_FREESLIP = 1, 2, 3, 4
 
 
 def x_face(i, dx):
-    """``x_face(i) = real(i-1) * dx`` (``src/module_mesh.f90:337-341``)."""
+    """``x_face(i) = real(i-1) * dx`` (``src/module_mesh.f90:162-166``)."""
     return (np.asarray(i, dtype=np.float64) - 1.0) * dx
 
 
 def x_center(i, dx):
-    """``x_center(i) = (real(i-1) + 0.5) * dx`` (``src/module_mesh.f90:357-361``)."""
+    """``x_center(i) = (real(i-1) + 0.5) * dx`` (``src/module_mesh.f90:182-186``)."""
     return ((np.asarray(i, dtype=np.float64) - 1.0) + 0.5) * dx
 
 

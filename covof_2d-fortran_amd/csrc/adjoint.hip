@@ -157,7 +157,6 @@ __global__ void __launch_bounds__(256) k_proj_adj_g(int nx, int ny, int perx, in
   const int u_nx = perx ? nx : nx + 1, v_ny = pery ? ny : ny + 1;
   for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
     const int j = (int)(t / nx) + 1, i = (int)(t - (long)(j - 1) * nx) + 1;
-    const long h = (long)j * (nx + 2) + i;
     double s = 0.0;
     if (w_p) {
       // dJ/dp of the cell and of its periodic halo copies (the wrap copies

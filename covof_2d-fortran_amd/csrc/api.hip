@@ -199,6 +199,73 @@ extern "C" int covof_ctx_set_reduction_order(covof_ctx *h, int order) {
   return COVOF_OK;
 }
 
+extern "C" int covof_ctx_set_solver_option(covof_ctx *h, int option, double value) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  covof::SolverOpts &o = h->c.opts;
+  const bool whole = value == (double)(int)value;
+  switch (option) {
+    case COVOF_OPT_MIXED_STALL_WINDOW:
+      if (!whole || value < 0.0) return fail(COVOF_BAD_ARGUMENT, "MIXED_STALL_WINDOW must be a whole number >= 0");
+      o.mixed_stall = (int)value;
+      break;
+    case COVOF_OPT_MIXED_CYCLE:
+      if (!whole || value < 0.0 || value == 1.0 || ((int)value & 1))
+        return fail(COVOF_BAD_ARGUMENT, "MIXED_CYCLE must be 0 or an even whole number >= 2");
+      o.mixed_cycle = (int)value;
+      break;
+    case COVOF_OPT_MIXED_LOOP:
+      if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MIXED_LOOP must be 0 or 1");
+      o.mixed_loop = (int)value;
+      break;
+    case COVOF_OPT_MG_COARSE:
+      if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MG_COARSE must be 0 or 1");
+      o.mg_coarse = (int)value;
+      break;
+    case COVOF_OPT_MG_SWEEPS:
+      if (!whole || value < 1.0 || value > 4.0) return fail(COVOF_BAD_ARGUMENT, "MG_SWEEPS must be 1, 2, 3 or 4");
+      o.mg_sweeps = (int)value;
+      break;
+    case COVOF_OPT_MG_OMEGA:
+      if (!(value > 0.0 && value <= 2.0)) return fail(COVOF_BAD_ARGUMENT, "MG_OMEGA must be in (0, 2]");
+      o.mg_omega = value;
+      break;
+    default:
+      return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_solver_option: unknown option");
+  }
+  if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+  covof::solver_invalidate(h->c);
+  return COVOF_OK;
+}
+
+static double *mirror(covof_ctx *h, int k);
+
+extern "C" int covof_ctx_reserve(covof_ctx *h, int parts) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  if (parts & ~COVOF_RESERVE_ALL) return fail(COVOF_BAD_ARGUMENT, "covof_ctx_reserve: unknown part bits");
+  Ctx &c = h->c;
+  int st = COVOF_OK;
+  if (!st && (parts & COVOF_RESERVE_MOMENTUM)) {
+    st = covof::mom_reserve(c);
+  }
+  if (!st && (parts & COVOF_RESERVE_MULTIGRID)) st = covof::mg_reserve(c);
+  if (!st && (parts & COVOF_RESERVE_SGS)) st = covof::sgs_reserve(c);
+  if (!st && (parts & COVOF_RESERVE_MIXED_LOOP)) st = covof::cg2_reserve(c);
+  if (!st && (parts & COVOF_RESERVE_SURFACE)) st = covof::surface_reserve(c);
+  if (!st && (parts & COVOF_RESERVE_ADJOINT)) st = covof::vof_adjoint_reserve(c);
+  if (!st && (parts & COVOF_RESERVE_HOST_MIRRORS))
+    for (int k = 0; k < 13 && !st; ++k)
+      if (!mirror(h, k)) st = COVOF_HIP_ERROR;
+  if (!st && hipStreamSynchronize(c.stream) != hipSuccess) st = COVOF_HIP_ERROR;
+  return hip_status(st);
+}
+
+extern "C" int covof_ctx_mixed_solve_info(covof_ctx *h, int *fp32_iters, int *fp64_iters) {
+  if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
+  if (fp32_iters) *fp32_iters = h->c.ps.last_it32;
+  if (fp64_iters) *fp64_iters = h->c.ps.last_it64;
+  return COVOF_OK;
+}
+
 extern "C" int covof_ctx_synchronize(covof_ctx *h) {
   if (!h) return fail(COVOF_BAD_ARGUMENT, "null context");
   if (hipStreamSynchronize(h->c.stream) != hipSuccess) return hip_status(COVOF_HIP_ERROR);

@@ -130,6 +130,9 @@ struct Poisson {
   double *hscal = nullptr;  // pinned host mirror of scal
   int assembled = 0;
   int assembly_id = 0;      // incremented by every assembly
+  // the last mixed-precision solve: iterations of the FP32 recurrence and of
+  // the FP64 continuation (0 when it did not run)
+  int last_it32 = 0, last_it64 = 0;
   Mg mg64, mg32;
   int nblocks = 0;          // reduction grid size
   PGraph graph[2];          // FP64 cycle, mixed cycle
@@ -168,8 +171,20 @@ struct TopJet {
   double center = 0.5, width = 0.08;
 };
 
+// Solver tuning of a context (covof_ctx_set_solver_option); zeros in the
+// mixed-mode fields mean "the default of the preconditioner".
+struct SolverOpts {
+  int mixed_stall = 0;   // reliable updates without a 10 % gain before the FP64 continuation
+  int mixed_cycle = 0;   // FP32 iterations per reliable update
+  int mixed_loop = 0;    // 1: single-reduction Chronopoulos-Gear loop (Jacobi)
+  int mg_coarse = 0;     // 1: series coarse faces (else averaged)
+  int mg_sweeps = 1;     // red-black sweeps before and after the coarse correction
+  double mg_omega = 1.8; // weight of the coarse-grid correction
+};
+
 struct Ctx {
   Mesh m;
+  SolverOpts opts;
   Obstacle obs;
   TopJet jet;
   hipStream_t stream = nullptr;
@@ -233,6 +248,16 @@ void serial_sums(Ctx &c, long n, const SerialSet &s, int f0, int f1);
 
 int mg_check(const Ctx &c, int kind);
 int mg_prepare(Ctx &c, bool fp32);
+// Drops the captured solver cycles and the multigrid coarse operators (the
+// next solve captures and coarsens again): after a solver option changes.
+void solver_invalidate(Ctx &c);
+// Work arrays made on a path's first use, made ahead (covof_ctx_reserve).
+int mom_reserve(Ctx &c);
+int mg_reserve(Ctx &c);
+int sgs_reserve(Ctx &c);
+int cg2_reserve(Ctx &c);
+int surface_reserve(Ctx &c);
+int vof_adjoint_reserve(Ctx &c);
 // z = M^-1 r; f0, f1: the scalar slots of the solve's stop flags
 void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1);
 void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1);

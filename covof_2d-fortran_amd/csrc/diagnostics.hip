@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(BLK) k_history(Mesh m, const double *__restric
   double mx[NM] = {0, 0};
   for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < ncell; t += (long)gridDim.x * BLK) {
     const int j = (int)(t / m.nx) + 1, i = (int)(t - (long)(j - 1) * m.nx) + 1;
-    const double y_pos = ((double)(j - 1) + 0.5) * dy;  // y_center (src/module_mesh.f90:366-371)
+    const double y_pos = ((double)(j - 1) + 0.5) * dy;  // y_center (src/module_mesh.f90:192-196)
     const double c_val = C[HX(i, j)];
     const double ind = 1.0 - c_val;
     const double pc = p[HX(i, j)];

@@ -11,7 +11,7 @@ namespace covof {
 
 #define CV_HD __host__ __device__ __forceinline__
 
-// x_face / y_face of src/module_mesh.f90:337-351 for one rank.
+// x_face / y_face of src/module_mesh.f90:162-176 for one rank.
 CV_HD double face_coord(int i, double d) { return (double)(i - 1) * d; }
 
 // Euclidean norm sqrt(x^2 + y^2) as the reference's HYPOT evaluates it.  The

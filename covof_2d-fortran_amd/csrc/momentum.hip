@@ -922,6 +922,11 @@ int mom_blocks(Ctx &c, bool full) {
 
 }  // namespace
 
+int mom_reserve(Ctx &c) {
+  const int st = mom_alloc(c);
+  return st ? st : mom_alloc_blocks(c);
+}
+
 void mom_free(Ctx &c) {
   Mom &M = c.mom;
   for (double *p : {M.tu, M.tv, M.tp, M.diag, M.rhs, M.x}) hipFree(p);

@@ -448,35 +448,23 @@ Lev<T> level(Ctx &c, int l, const T *b, T *x) {
   return L;
 }
 
-// Coarse-face scheme (see k_mg_coarsen_faces): COVOF_MG_COARSE=average or
-// series; average by default (fewer iterations on the density-jump
-// workloads measured).
-inline int coarse_scheme() {
-  const char *e = getenv("COVOF_MG_COARSE");
-  return (e && e[0] == 's') ? 1 : 0;
-}
+// Coarse-face scheme (see k_mg_coarsen_faces): averaged faces by default
+// (fewer iterations on the density-jump workloads measured), series faces
+// with the context's mg_coarse option.
+inline int coarse_scheme(const Ctx &c) { return c.opts.mg_coarse ? 1 : 0; }
 
-// Red-black sweeps before and after each coarse correction: COVOF_MG_NU
-// (1..4), 1 by default.
-inline int smoothing_sweeps() {
-  const char *e = getenv("COVOF_MG_NU");
-  const int v = e ? atoi(e) : 1;
-  return v < 1 ? 1 : (v > 4 ? 4 : v);
-}
+// Red-black sweeps before and after each coarse correction (1..4, 1 by
+// default).
+inline int smoothing_sweeps(const Ctx &c) { return c.opts.mg_sweeps; }
 
-// Weight of the coarse-grid correction (scales R): COVOF_MG_OMEGA in (0, 2],
-// 1.8 by default.  With the piecewise-constant transfer the Galerkin
-// correction (weight 1) is too weak -- the coarse operators of aggregation
-// are known to over-estimate the energy of smooth modes -- and an
-// over-correction in (1, 2) restores the convergence (D. Braess, Computing 55
-// (1995) 379-393); a weight <= 2 keeps the cycle symmetric positive
-// definite.  At 2048^2 with density ratio 1000 it takes the CG from 231 to
-// 34 iterations.
-inline double correction_weight() {
-  const char *e = getenv("COVOF_MG_OMEGA");
-  const double v = e ? atof(e) : 1.8;
-  return (v > 0.0 && v <= 2.0) ? v : 1.8;
-}
+// Weight of the coarse-grid correction (scales R), in (0, 2], 1.8 by
+// default.  With the piecewise-constant transfer the Galerkin correction
+// (weight 1) is too weak -- the coarse operators of aggregation are known to
+// over-estimate the energy of smooth modes -- and an over-correction in
+// (1, 2) restores the convergence (D. Braess, Computing 55 (1995) 379-393);
+// a weight <= 2 keeps the cycle symmetric positive definite.  At 2048^2 with
+// density ratio 1000 it takes the CG from 231 to 34 iterations.
+inline double correction_weight(const Ctx &c) { return c.opts.mg_omega; }
 
 // Coarsen while the level has more than 16 cells and both sides of the next
 // level keep at least 2 cells (a side of 1 in a walled box would leave a cell
@@ -486,8 +474,9 @@ inline bool coarsen_more(int nx, int ny) { return (long)nx * ny > 16 && nx >= 4 
 inline dim3 pair_grid(int nx, int ny) { return dim3(((nx + 1) / 2 + 63) / 64, (ny + 3) / 4); }
 inline dim3 cell_grid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
 
+// The level buffers of the hierarchy (mesh-dependent only).
 template <typename T>
-int prepare(Ctx &c) {
+int reserve(Ctx &c) {
   Mg &G = Hier<T>::get(c);
   const Mesh &m = c.m;
   if (!G.nlev) {
@@ -516,6 +505,15 @@ int prepare(Ctx &c) {
     }
     G.nlev = nl;
   }
+  return COVOF_OK;
+}
+
+template <typename T>
+int prepare(Ctx &c) {
+  Mg &G = Hier<T>::get(c);
+  const Mesh &m = c.m;
+  const int st = reserve<T>(c);
+  if (st) return st;
   if (G.built != c.ps.assembly_id) {
     for (int l = 1; l < G.nlev; ++l) {
       const Lev<T> F = level<T>(c, l - 1, nullptr, nullptr);
@@ -523,7 +521,7 @@ int prepare(Ctx &c) {
       const int nxc = B.nx, nyc = B.ny;
       hipLaunchKernelGGL(k_mg_coarsen_faces<T>, cell_grid(nxc + 1, nyc + 1), dim3(64, 4), 0, c.stream, F.nx, F.ny,
                          F.kxp, F.kx, F.ky, nxc, nyc, pitch_of(nxc), (T *)B.kx, (T *)B.ky, m.perx, m.pery,
-                         coarse_scheme());
+                         coarse_scheme(c));
       hipLaunchKernelGGL(k_mg_coarsen_diag<T>, cell_grid(nxc, nyc), dim3(64, 4), 0, c.stream, m.nx, m.ny, l,
                          1.0e-12 * (m.dx * m.dy), nxc, nyc, pitch_of(nxc), (const T *)B.kx, (const T *)B.ky,
                          (T *)B.diag);
@@ -547,7 +545,7 @@ void smooth(Ctx &c, const Lev<T> &L, int color, bool init, int f0, int f1) {
 template <typename T>
 void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
   Mg &G = Hier<T>::get(c);
-  const int NU = smoothing_sweeps();  // red-black sweeps before and after the coarse correction
+  const int NU = smoothing_sweeps(c);  // red-black sweeps before and after the coarse correction
   if (kind == MG_KIND_SGS_RB) {
     const Lev<T> L = level<T>(c, 0, r, z);
     smooth(c, L, 0, true, f0, f1);
@@ -566,7 +564,7 @@ void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
     held += cells;
     --lb;
   }
-  const T scale = (T)(0.5 * correction_weight());
+  const T scale = (T)(0.5 * correction_weight(c));
   for (int l = 0; l < lb; ++l) {
     const Lev<T> L = level<T>(c, l, r, z);
     for (int s = 0; s < NU; ++s) {
@@ -598,7 +596,7 @@ void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
     const Lev<T> Lc = level<T>(c, nl - 1, r, z);
     Bt.K = std::min(64, 2 * std::max(Lc.nx, Lc.ny));
     Bt.NU = NU;
-    Bt.scale = 0.5 * correction_weight();
+    Bt.scale = 0.5 * correction_weight(c);
     const Lev<T> Lb = level<T>(c, lb, r, z);
     hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(1024), 0, c.stream, Bt, Lb.b, Lb.x, (const double *)c.ps.scal,
                        f0, f1);
@@ -639,6 +637,10 @@ int mg_check(const Ctx &c, int kind) {
 }
 
 int mg_prepare(Ctx &c, bool fp32) { return fp32 ? prepare<float>(c) : prepare<double>(c); }
+int mg_reserve(Ctx &c) {
+  const int st = reserve<double>(c);
+  return st ? st : reserve<float>(c);
+}
 
 void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1) {
   apply<double>(c, kind, r, z, f0, f1);

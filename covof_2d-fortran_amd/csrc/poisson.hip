@@ -1019,15 +1019,12 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
   if (threadIdx.x != 0) return;
   scal[S_RMEAN] = singular ? tot[3] * inv_n : 0.0;
   scal[I_PEND] = 0.0;  // folded into x by the flush before the correction
-  if (scal[I_FLAG] == (double)F_MAXIT || scal[I_ITERS] >= scal[I_MAXIT]) {
-    scal[S_FLAG] = (double)F_MAXIT;
-    return;
-  }
   if (first) {
     double bn = sqrt(tot[2]);
     if (bn < 1.0e-30) bn = 1.0;
     scal[S_BNORM] = bn;
   }
+  // the residual of the x the solve returns, whatever ends it
   const double rn = sqrt(tot[0]);
   const double rel = rn / scal[S_BNORM];
   scal[S_RR] = tot[0];
@@ -1036,6 +1033,8 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
     scal[S_FLAG] = (double)F_NONFINITE;
   } else if (rel <= scal[S_TOL]) {
     scal[S_FLAG] = (double)F_CONV;
+  } else if (scal[I_FLAG] == (double)F_MAXIT || scal[I_ITERS] >= scal[I_MAXIT]) {
+    scal[S_FLAG] = (double)F_MAXIT;
   } else if (first || rel < 0.9 * scal[S_BEST]) {
     scal[S_BEST] = rel;
     scal[S_STALL] = 0.0;
@@ -1046,6 +1045,30 @@ __global__ void __launch_bounds__(BLK) k_residual_check(const double *pin, int n
     // finishes the solve in FP64 from this x
     scal[S_FLAG] = (double)F_STALL;
   }
+}
+
+// ||b - A x|| and ||b|| partials with no stop flag (the true residual of a
+// finished solve), then ||b - A x|| / ||b|| (||b|| < 1e-30 counts as 1) into
+// *out.
+__global__ void __launch_bounds__(BLK) k_true_res(DevOp o, OpT<double> A, const double *__restrict__ b,
+                                                  const double *__restrict__ x, double *pout) {
+  double acc[2] = {0.0, 0.0};
+  CELL_LOOP(o) {
+    const long q = (long)(j - 1) * o.nx + (i - 1);
+    const double rk = b[q] - apply_op<double>(o, A, [&](long m) { return x[m]; }, i, j, q);
+    acc[0] += rk * rk;
+    acc[1] += b[q] * b[q];
+  }
+  block_partials<2, double>(acc, pout);
+}
+
+__global__ void __launch_bounds__(BLK) k_true_res_finish(const double *pin, int nblk, double *out) {
+  double tot[2];
+  grid_totals<2>(pin, nblk, tot);
+  if (threadIdx.x != 0) return;
+  double bn = sqrt(tot[1]);
+  if (bn < 1.0e-30) bn = 1.0;
+  *out = sqrt(tot[0]) / bn;
 }
 
 // sum(dx32) into slot 0 of this kernel's partials (singular operator only).
@@ -1242,10 +1265,9 @@ int poisson_alloc(Ctx &c) {
   for (double **pp : d64) COVOF_CHECK(hipMalloc(pp, n * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.fx, nkx * sizeof(double)));
   COVOF_CHECK(hipMalloc(&P.fy, nky * sizeof(double)));
-  float **d32[] = {&P.r32, &P.p32, &P.q32, &P.x32, &P.z32};
+  float **d32[] = {&P.r32, &P.q32, &P.x32, &P.z32};
   for (float **pp : d32) COVOF_CHECK(hipMalloc(pp, n * sizeof(float)));
-  // z32 doubles as the second ping-pong buffer of p32: allocate 2n for p32
-  hipFree(P.p32);
+  // p32 holds the two ping-pong direction buffers (2n)
   COVOF_CHECK(hipMalloc(&P.p32, 2 * n * sizeof(float)));
   const dim3 g = red_grid(c.m);
   P.nblocks = g.x * g.y;
@@ -1330,19 +1352,26 @@ static int mg_kind(int pc) { return pc == 3 ? MG_KIND_SGS_RB : MG_KIND_VCYCLE; }
 
 // z = SGS(r) on the strip sweeps (records rebuilt after each assembly);
 // flag: a device scalar that skips the application when nonzero, or null.
+int sgs_reserve(Ctx &c) {
+  Poisson &P = c.ps;
+  if (P.sgs_rec) return COVOF_OK;
+  const long len = seq_len(c.m.nx, c.m.ny);
+  COVOF_CHECK(hipMalloc(&P.sgs_rec, 8 * len * sizeof(double)));
+  COVOF_CHECK(hipMalloc(&P.sgs_xa, len * sizeof(double)));
+  COVOF_CHECK(hipMalloc(&P.sgs_xb, len * sizeof(double)));
+  const long nh = c.m.nx > c.m.ny ? c.m.nx : c.m.ny;
+  COVOF_CHECK(hipMalloc(&P.sgs_h, 4 * nh * sizeof(double)));
+  return COVOF_OK;
+}
+
 static int sgs_apply(Ctx &c, const double *r, double *z, const double *flag) {
   Poisson &P = c.ps;
   const DevOp o = dev_op(c.m);
   const int S = o.nx + 63;
-  const long len = seq_len(o.nx, o.ny), n = (long)o.nx * o.ny;
+  const long n = (long)o.nx * o.ny;
   const dim3 g((unsigned)((n + BLK - 1) / BLK));
-  if (!P.sgs_rec) {
-    COVOF_CHECK(hipMalloc(&P.sgs_rec, 8 * len * sizeof(double)));
-    COVOF_CHECK(hipMalloc(&P.sgs_xa, len * sizeof(double)));
-    COVOF_CHECK(hipMalloc(&P.sgs_xb, len * sizeof(double)));
-    const long nh = o.nx > o.ny ? o.nx : o.ny;
-    COVOF_CHECK(hipMalloc(&P.sgs_h, 4 * nh * sizeof(double)));
-  }
+  const int rs = sgs_reserve(c);
+  if (rs) return rs;
   if (P.sgs_id != P.assembly_id) {
     OpT<double> A{P.kx, P.ky, P.diag};
     hipLaunchKernelGGL(k_sgs_records, g, dim3(BLK), 0, c.stream, o, A, S, P.sgs_rec);
@@ -1528,14 +1557,10 @@ static void cg2_iteration(Ctx &c, int k, bool timed) {
                 part_buf(c, wo), k);
 }
 
-// Whether the mixed Jacobi loop runs as the single-reduction kernel
-// (COVOF_MIXED_LOOP=cg2) or as the two-kernel PCG (the default: at 2048^2
-// it is as fast per iteration and needs 4 % fewer iterations).
-static bool mixed_cg2(int pc) {
-  if (pc != 1) return false;
-  const char *e = getenv("COVOF_MIXED_LOOP");
-  return e && e[0] == 'c' && e[1] == 'g' && e[2] == '2' && e[3] == 0;
-}
+// Whether the mixed Jacobi loop runs as the single-reduction kernel (the
+// context's mixed_loop option) or as the two-kernel PCG (the default: at
+// 2048^2 it is as fast per iteration and needs 4 % fewer iterations).
+static bool mixed_cg2(const Ctx &c, int pc) { return pc == 1 && c.opts.mixed_loop == 1; }
 
 // The reliable update of the mixed mode: x += x32, x32 = 0, r = b - A x in
 // FP64 (+ its FP32 copy), the convergence test, and the FP32 recurrence's
@@ -1564,12 +1589,13 @@ static void mixed_refresh(Ctx &c, int pc, const double *b, double *x, int first)
   // stall window: 48 updates of 128 Jacobi iterations (CG's residual norm
   // plateaus for about a thousand Jacobi iterations at a time on the
   // density-jump systems; this is far past that), 16 multigrid iterations
-  // (a converging MG-PCG gains far more than 10 % per 8 iterations)
-  const char *stall_env = getenv("COVOF_MIXED_STALL");  // the window in updates, for tuning
-  const int stall_max = stall_env && atoi(stall_env) > 0 ? atoi(stall_env) : (pc == 4 ? 2 : 48);
+  // (a converging MG-PCG gains far more than 10 % per 8 iterations; fitted
+  // on the 4096 x 2048 bubble step); the context's mixed_stall option
+  // overrides it
+  const int stall_max = c.opts.mixed_stall > 0 ? c.opts.mixed_stall : (pc == 4 ? 2 : 48);
   hipLaunchKernelGGL(k_residual_check, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks, P.scal,
                      first, singular, inv_n, stall_max);
-  const bool cg2 = mixed_cg2(pc);
+  const bool cg2 = mixed_cg2(c, pc);
   // the single-reduction loop restarts from r32 in its parity-0 r buffer
   float *r32 = cg2 ? cg2_bufs(c).r[0] : P.r32;
   if (pc == 1 && !cg2)
@@ -1609,7 +1635,7 @@ static void enqueue_cycle(Ctx &c, int pc, bool mixed, int iters) {
     const bool timed = prof_sampled(c, k);
     if (mixed) {
       float *d0 = P.p32, *d1 = P.p32 + n;
-      if (mixed_cg2(pc)) cg2_iteration(c, k, timed);
+      if (mixed_cg2(c, pc)) cg2_iteration(c, k, timed);
       else if (pc == 1) pcg_iteration<float, 1>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
       else if (pc == 3) pcg_iteration<float, 3>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
       else if (pc == 4) pcg_iteration<float, 4>(c, A32, P.x32, P.r32, P.z32, d0, d1, P.q32, INNER, k, timed);
@@ -1768,12 +1794,13 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
 // (2010) 1517.  When the FP64 residual stops improving (no 10 % gain over
 // a window of reliable updates) before tol, FP64 PCG continues from the
 // refined x; the iteration count is then the sum of both.
+int cg2_reserve(Ctx &c);
+
 static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
   const long n = (long)c.m.nx * c.m.ny;
   const int pc = pc_code(precond);
-  const char *cyc_env = getenv("COVOF_MIXED_CYCLE");
-  int cycle = cyc_env ? atoi(cyc_env) : (pc == 4 ? 8 : (pc == 3 ? 32 : 128));
+  int cycle = c.opts.mixed_cycle > 0 ? c.opts.mixed_cycle : (pc == 4 ? 8 : (pc == 3 ? 32 : 128));
   if (cycle < 2) cycle = 2;
   cycle &= ~1;  // even: the parity pattern of k repeats from one replay to the next
   int st;
@@ -1784,21 +1811,24 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
   if ((st = fill<float>(c, P.x32, n, 0.0f))) return st;
   if ((st = fill<float>(c, P.p32, n, 0.0f))) return st;
   if ((st = fill<float>(c, P.q32, n, 0.0f))) return st;
-  if (mixed_cg2(pc)) {
-    if (!P.cg2) COVOF_CHECK(hipMalloc(&P.cg2, 8 * n * sizeof(float)));
+  if (mixed_cg2(c, pc)) {
+    if ((st = cg2_reserve(c))) return st;
     const Cg2Bufs B = cg2_bufs(c);
     if ((st = fill<float>(c, B.p[0], n, 0.0f))) return st;
     if ((st = fill<float>(c, B.s[0], n, 0.0f))) return st;
   }
+  P.last_it32 = 0;
+  P.last_it64 = 0;
   mixed_refresh(c, pc, P.b, P.x, 1);
   COVOF_CHECK_LAUNCH();
   if ((st = read_scalars(c, NREAD))) return st;
   if (P.hscal[S_FLAG] == 0.0) {
-    const int key = 1000 + cycle * 16 + pc * 2 + (mixed_cg2(pc) ? 1 : 0);
+    const int key = 1000 + cycle * 16 + pc * 2 + (mixed_cg2(c, pc) ? 1 : 0);
     if ((st = run_cycles(c, P.graph[1], key, pc, true, cycle, maxit / cycle + 2))) return st;
   }
   if ((st = read_scalars(c, NREAD))) return st;
   const int it32 = (int)P.hscal[I_ITERS];
+  P.last_it32 = it32;
   if (iters) *iters = it32;
   if (relres) *relres = P.hscal[S_RELRES];
   const double f = P.hscal[S_FLAG];
@@ -1806,8 +1836,25 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     // the FP32 recurrence stagnated above tol: FP64 PCG from the refined x
     // (which holds every correction so far) for the remaining iterations
     int it64 = 0;
-    st = solve_pcg64(c, precond, tol, maxit - it32, &it64, relres);
+    double rec = 0.0;
+    st = solve_pcg64(c, precond, tol, maxit - it32, &it64, &rec);
+    P.last_it64 = it64;
     if (iters) *iters = it32 + it64;
+    if (st == COVOF_HIP_ERROR || st == COVOF_BAD_ARGUMENT) return st;
+    // the mixed mode reports the true FP64 residual of the x it returns (the
+    // continuation stopped on its recurrence residual)
+    OpT<double> A{P.kx, P.ky, P.diag};
+    double *pb = part_buf(c, 1);
+    hipLaunchKernelGGL(k_true_res, red_grid(c.m), dim3(BLK), 0, c.stream, dev_op(c.m), A, (const double *)P.b,
+                       (const double *)P.x, pb);
+    hipLaunchKernelGGL(k_true_res_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks,
+                       P.scal + 58);
+    COVOF_CHECK_LAUNCH();
+    COVOF_CHECK(hipMemcpyAsync(P.hscal + 58, P.scal + 58, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    COVOF_CHECK(hipStreamSynchronize(c.stream));
+    const double tr = P.hscal[58];
+    if (relres) *relres = tr;
+    if (st == COVOF_OK && !(tr <= tol)) return COVOF_NOT_CONVERGED;
     return st;
   }
   if (f == F_CONV) return COVOF_OK;
@@ -1842,6 +1889,21 @@ int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precon
     return solve_bicgstab2(c, b, x, precond, tol, maxit, iters, relres);
   }
   return COVOF_BAD_ARGUMENT;
+}
+
+int cg2_reserve(Ctx &c) {
+  if (!c.ps.cg2) COVOF_CHECK(hipMalloc(&c.ps.cg2, 8L * c.m.nx * c.m.ny * sizeof(float)));
+  return COVOF_OK;
+}
+
+void solver_invalidate(Ctx &c) {
+  for (PGraph &G : c.ps.graph) {
+    if (G.exec) hipGraphExecDestroy(G.exec);
+    G.exec = nullptr;
+    G.key = -1;
+  }
+  c.ps.mg64.built = -1;
+  c.ps.mg32.built = -1;
 }
 
 int sp_poisson(Ctx &c, const double *u, const double *v, const double *rho_u, const double *rho_v,

@@ -354,6 +354,19 @@ static long halo_size(const Mesh &m) { return (long)(m.nx + 2) * (m.ny + 2); }
 
 // Three halo-size work arrays (the normals and the height-function flags;
 // the diffusion ping-pong), made on first use and kept with the context.
+int surface_reserve(Ctx &c) {
+  if (!c.surf.work) COVOF_CHECK(hipMalloc(&c.surf.work, 3 * halo_size(c.m) * sizeof(double)));
+  Surface &F = c.surf;
+  if (!F.sv_rec) {
+    const long len = seq_len(c.m.nx, c.m.ny);
+    COVOF_CHECK(hipMalloc(&F.sv_rec, 8 * len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_xa, len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_xb, len * sizeof(double)));
+    COVOF_CHECK(hipMalloc(&F.sv_h, 4 * (long)(c.m.nx > c.m.ny ? c.m.nx : c.m.ny) * sizeof(double)));
+  }
+  return COVOF_OK;
+}
+
 static int surface_work(Ctx &c, double **w0, double **w1) {
   // the third array: the height-function validity flags of curvature_csf
   if (!c.surf.work) COVOF_CHECK(hipMalloc(&c.surf.work, 3 * halo_size(c.m) * sizeof(double)));
@@ -428,12 +441,9 @@ int svof_smooth(Ctx &c, const double *C, double width, double *Cs) {
   const double wx = D / (m.dx * m.dx), wy = D / (m.dy * m.dy);
   Surface &F = c.surf;
   const int S = m.nx + 63;
-  const long len = seq_len(m.nx, m.ny);
   if (!F.sv_rec) {
-    COVOF_CHECK(hipMalloc(&F.sv_rec, 8 * len * sizeof(double)));
-    COVOF_CHECK(hipMalloc(&F.sv_xa, len * sizeof(double)));
-    COVOF_CHECK(hipMalloc(&F.sv_xb, len * sizeof(double)));
-    COVOF_CHECK(hipMalloc(&F.sv_h, 4 * (long)(m.nx > m.ny ? m.nx : m.ny) * sizeof(double)));
+    const int rs = surface_reserve(c);
+    if (rs) return rs;
   }
   const long nh = m.nx > m.ny ? m.nx : m.ny;
   const SeqBlk b{m.nx, m.ny, (m.ny + 63) / 64, S, F.sv_rec, F.sv_xa, F.sv_xb,

@@ -1418,6 +1418,8 @@ static int adj_pass(Ctx &c, int method, const double *src, const double *Cn, con
 // included); cbar = dJ/dC^n (halo layout; with fold_in the halo entries are
 // moved onto the interior cells they copy, else they are the derivatives
 // with respect to C^n's halo values as given); ubar, vbar += dJ/du, dJ/dv.
+int vof_adjoint_reserve(Ctx &c) { return adj_alloc(c); }
+
 int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, const double *v, double dt,
                        int step_count, int two_phase, int fold_in, const double *cbar_new, double *cbar,
                        double *ubar, double *vbar) {

@@ -93,6 +93,38 @@ const char *covof_last_error(void);
 #define COVOF_REDUCTION_REFERENCE 1
 int covof_ctx_set_reduction_order(covof_ctx *ctx, int order);
 
+/* Tuning of the context's Poisson solves (no reference counterpart).
+ * Setting one drops the context's captured solver cycles and multigrid
+ * coarse operators (made again by the next solve).  value: */
+#define COVOF_OPT_MIXED_STALL_WINDOW 1 /* reliable updates without a 10 % gain before the mixed mode continues
+                                          in FP64; 0 (default): 48 (Jacobi / none), 2 (MG) */
+#define COVOF_OPT_MIXED_CYCLE 2        /* FP32 iterations per reliable update, even, >= 2; 0 (default): 128
+                                          (Jacobi / none), 32 (SGS_RB), 8 (MG) */
+#define COVOF_OPT_MIXED_LOOP 3         /* 0 (default): two-kernel PCG; 1: the single-reduction
+                                          Chronopoulos-Gear loop (mixed Jacobi only) */
+#define COVOF_OPT_MG_COARSE 4          /* coarse faces of PRECOND_MG: 0 (default) averaged, 1 in series */
+#define COVOF_OPT_MG_SWEEPS 5          /* red-black sweeps before and after the coarse correction, 1..4
+                                          (default 1) */
+#define COVOF_OPT_MG_OMEGA 6           /* weight of the coarse-grid correction, (0, 2] (default 1.8) */
+int covof_ctx_set_solver_option(covof_ctx *ctx, int option, double value);
+/* Makes ahead the work arrays a path otherwise makes on its first use, so
+ * that the calls of a time step allocate nothing (no reference counterpart;
+ * the reference allocates its temporaries every call).  parts: an OR of */
+#define COVOF_RESERVE_MOMENTUM 1      /* the momentum / monolithic block (vectors, SSOR and Schur blocks) */
+#define COVOF_RESERVE_MULTIGRID 2     /* the PRECOND_MG / SGS_RB hierarchy buffers, FP64 and FP32 */
+#define COVOF_RESERVE_SGS 4           /* the exact SGS (PRECOND_SSOR) strip records */
+#define COVOF_RESERVE_MIXED_LOOP 8    /* the single-reduction mixed loop's vectors */
+#define COVOF_RESERVE_SURFACE 16      /* curvature / smoothing work arrays and the SVOF strip records */
+#define COVOF_RESERVE_ADJOINT 32      /* the Weymouth-Yue step adjoint's work arrays */
+#define COVOF_RESERVE_HOST_MIRRORS 64 /* device copies behind the host-array entry points */
+#define COVOF_RESERVE_ALL 127
+int covof_ctx_reserve(covof_ctx *ctx, int parts);
+/* The split of the context's last mixed-precision CG solve: iterations of
+ * the FP32 recurrence and of the FP64 continuation that follows a stall (0
+ * when it did not run).  After a continuation the solve's relres is the
+ * recomputed true FP64 residual. */
+int covof_ctx_mixed_solve_info(covof_ctx *ctx, int *fp32_iters, int *fp64_iters);
+
 /* A solid obstacle: config_t's obs_enabled, obs_type (1 box [x0,x1]x[y0,y1],
  * 2 disc of centre (xc, yc) and radius r), obs_perm (src/module_config.f90:
  * 121-129).  With obs_perm <= 1e-12 every velocity face whose point

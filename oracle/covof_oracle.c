@@ -63,7 +63,7 @@ void orc_mesh_init(orc_mesh *m, int nx, int ny, double lx, double ly, const int 
 static int per_x(const orc_mesh *m) { return m->bc[0] == BC_PERIODIC || m->bc[1] == BC_PERIODIC; }
 static int per_y(const orc_mesh *m) { return m->bc[2] == BC_PERIODIC || m->bc[3] == BC_PERIODIC; }
 
-/* src/module_mesh.f90:337-351 */
+/* src/module_mesh.f90:162-176 */
 static double x_face(const orc_mesh *m, int i) { return (double)(i - 1) * m->dx; }
 static double y_face(const orc_mesh *m, int j) { return (double)(j - 1) * m->dy; }
 

@@ -119,11 +119,66 @@ def test_mg_mixed_stall_continues_in_fp64(cv, orc):
     b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
     x = ctx.new_work()
     r = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=0.0, max_iter=cap, precision=cv.PRECISION_MIXED)
-    print(f"mixed MG-PCG, tol 0: status {r.status}, {r.iters} iterations, relres {r.relres:.3e}")
+    it32, it64 = ctx.mixed_solve_info()
+    print(f"mixed MG-PCG, tol 0: status {r.status}, {r.iters} iterations ({it32} FP32 + {it64} FP64), "
+          f"relres {r.relres:.3e}")
     assert r.status in (cv._lib.NOT_CONVERGED, cv._lib.BREAKDOWN), r
+    assert it64 > 0 and it32 + it64 == r.iters  # the FP64 continuation ran
     if r.status == cv._lib.NOT_CONVERGED:
         assert r.iters == cap, r
     m = orc.mesh(n, n, bc=case["bc"])
     co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
     rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x.cpu().numpy())) / np.linalg.norm(co["b"])
     assert rr <= 1e-12, rr
+
+
+def test_mixed_fp64_continuation_reaches_tolerance(cv, orc):
+    """A stall forced early (two FP32 iterations per reliable update, a window
+    of one update) hands the solve to the FP64 continuation, which ends it at
+    tol 1e-10: status OK, both stages counted, and the reported relres is the
+    true FP64 residual of the returned x (the oracle's matvec agrees)."""
+    from covof_amd import workloads as wl
+    n = 128
+    case = wl.poisson_case(n, seed=2, n_circles=8, rmin=6, rmax=24)
+    ctx = cv.Context(n, n, bc=case["bc"])
+    ctx.set_solver_option(cv.OPT_MIXED_CYCLE, 2)
+    ctx.set_solver_option(cv.OPT_MIXED_STALL_WINDOW, 1)
+    b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
+    x = ctx.new_work()
+    r = ctx.poisson_solve(b, x, precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=100000, precision=cv.PRECISION_MIXED)
+    it32, it64 = ctx.mixed_solve_info()
+    print(f"forced stall: status {r.status}, {it32} FP32 + {it64} FP64 iterations, relres {r.relres:.3e}")
+    assert r.status == cv._lib.OK, r
+    assert it32 > 0 and it64 > 0 and it32 + it64 == r.iters
+    m = orc.mesh(n, n, bc=case["bc"])
+    co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
+    rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x.cpu().numpy())) / np.linalg.norm(co["b"])
+    assert rr <= 1e-10 and abs(rr - r.relres) <= 1e-3 * rr, (rr, r.relres)
+    # the default window on a fresh context: no stall on this system
+    ctx2 = cv.Context(n, n, bc=case["bc"])
+    b2 = ctx2.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
+    r2 = ctx2.poisson_solve(b2, ctx2.new_work(), precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=100000,
+                            precision=cv.PRECISION_MIXED)
+    assert r2.status == cv._lib.OK and ctx2.mixed_solve_info()[1] == 0
+
+
+def test_solver_options_validated(cv):
+    """Out-of-range tunings are refused; a changed multigrid option takes
+    effect on the next solve (the captured cycles are dropped)."""
+    from covof_amd import workloads as wl
+    ctx = cv.Context(64, 64, bc=(1, 1, 1, 1))
+    for opt, bad in ((cv.OPT_MIXED_CYCLE, 3), (cv.OPT_MIXED_CYCLE, 1), (cv.OPT_MIXED_STALL_WINDOW, -1),
+                     (cv.OPT_MIXED_LOOP, 2), (cv.OPT_MG_SWEEPS, 5), (cv.OPT_MG_OMEGA, 2.5), (cv.OPT_MG_COARSE, 0.5),
+                     (99, 1)):
+        with pytest.raises(cv.CovofError):
+            ctx.set_solver_option(opt, bad)
+    case = wl.poisson_case(64, seed=4, n_circles=4, rmin=4, rmax=12)
+    ctx = cv.Context(64, 64, bc=case["bc"])
+    b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
+    its = []
+    for sweeps in (1, 2, 1):
+        ctx.set_solver_option(cv.OPT_MG_SWEEPS, sweeps)
+        r = ctx.poisson_solve(b, ctx.new_work(), precond=cv.PRECOND_MG, tol=1e-10, max_iter=1000)
+        assert r.converged, r
+        its.append(r.iters)
+    assert its[0] == its[2] and its[1] < its[0], its

@@ -210,3 +210,49 @@ def test_projection_pressure_velocity_parity(cv, orc, precision):
     print(f"precision {precision}: {res.iters} iterations (oracle {it}); rel err p {ep:.2e} u {eu:.2e} v {ev:.2e}")
     bound = 1e-10 if precision == 0 else 1e-9
     assert ep < bound and eu < bound and ev < bound
+
+
+def test_error_paths(cv, orc):
+    """The failures a caller can force: NaN in the right-hand side or in a
+    face density (COVOF_NONFINITE, FP64 and mixed), b = 0 (the reference's
+    ||b|| < 1e-30 -> 1 rule: converged at once, x untouched), and a
+    diagonal below the Jacobi guard |a_P| <= 1e-15 (z = r; the solve
+    still lands on the oracle's answer)."""
+    n = 32
+    c = _case(n, circles=2)
+    ctx = cv.Context(n, n, bc=c["bc"])
+    U = {k: to_gpu(c[k]) for k in ("u", "v", "rho_u", "rho_v")}
+    for prec in (cv.PRECISION_FP64, cv.PRECISION_MIXED):
+        b = ctx.poisson_assemble(U["u"], U["v"], U["rho_u"], U["rho_v"], c["dt"])
+        b[7, 9] = float("nan")
+        r = ctx.poisson_solve(b, ctx.new_work(), precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=1000, precision=prec)
+        assert r.status == cv._lib.NONFINITE, (prec, r)
+        ru = U["rho_u"].clone()
+        ru[5, 6] = float("nan")
+        b = ctx.poisson_assemble(U["u"], U["v"], ru, U["rho_v"], c["dt"])
+        r = ctx.poisson_solve(b, ctx.new_work(), precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=1000, precision=prec)
+        assert r.status == cv._lib.NONFINITE, (prec, r)
+        # b = 0
+        ctx.poisson_assemble(U["u"], U["v"], U["rho_u"], U["rho_v"], c["dt"])
+        x = ctx.new_work()
+        r = ctx.poisson_solve(ctx.new_work(), x, precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=1000, precision=prec)
+        assert r.status == cv._lib.OK and r.iters == 0 and r.relres == 0.0, (prec, r)
+        assert not x.abs().max().item()
+    # the SP block reports a NaN velocity
+    p = ctx.new_field()
+    un = U["u"].clone()
+    un[10, 10] = float("nan")
+    r = ctx.solve_standard_projection_poisson(un, U["v"], U["rho_u"], U["rho_v"], p, c["dt"])
+    assert r.status == cv._lib.NONFINITE, r
+    # densities of 1e300: every face coefficient ~1e-302, a_P = 1e-12 dvol < 1e-15
+    big = np.full_like(c["rho_u"], 1.0e300)
+    b = ctx.poisson_assemble(U["u"], U["v"], to_gpu(big), to_gpu(big), c["dt"])
+    x = ctx.new_work()
+    r = ctx.poisson_solve(b, x, precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=1000)
+    m = orc.mesh(n, n, bc=c["bc"])
+    co = orc.poisson_assemble(m, c["u"], c["v"], big, big, c["dt"])
+    assert co["aP"].max() < 1e-15
+    xo, ito, _ = orc.sp_pcg(m, 1, co, co["b"], np.zeros((n, n)), tol=1e-10, maxit=1000)
+    assert r.status == cv._lib.OK and r.iters == ito, (r, ito)
+    xg = x.cpu().numpy()
+    assert np.abs(xg - xo).max() <= 1e-12 * np.abs(xo).max()
