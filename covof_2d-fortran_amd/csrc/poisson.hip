@@ -1841,8 +1841,12 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     P.last_it64 = it64;
     if (iters) *iters = it32 + it64;
     if (st == COVOF_HIP_ERROR || st == COVOF_BAD_ARGUMENT) return st;
-    // the mixed mode reports the true FP64 residual of the x it returns (the
-    // continuation stopped on its recurrence residual)
+    // the mixed mode reports the true FP64 residual of the x it returns; the
+    // status is the continuation's, which stops on its recurrence residual
+    // like every FP64 PCG solve (where the system's attainable accuracy is
+    // above tol -- 4096 x 2048 bubbles at density ratio 1000: FP64 MG-PCG
+    // alone ends at a true 1.7e-9 when its recurrence reads 9e-11 -- the
+    // true residual can exceed tol)
     OpT<double> A{P.kx, P.ky, P.diag};
     double *pb = part_buf(c, 1);
     hipLaunchKernelGGL(k_true_res, red_grid(c.m), dim3(BLK), 0, c.stream, dev_op(c.m), A, (const double *)P.b,
@@ -1852,9 +1856,7 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     COVOF_CHECK_LAUNCH();
     COVOF_CHECK(hipMemcpyAsync(P.hscal + 58, P.scal + 58, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     COVOF_CHECK(hipStreamSynchronize(c.stream));
-    const double tr = P.hscal[58];
-    if (relres) *relres = tr;
-    if (st == COVOF_OK && !(tr <= tol)) return COVOF_NOT_CONVERGED;
+    if (relres) *relres = P.hscal[58];
     return st;
   }
   if (f == F_CONV) return COVOF_OK;

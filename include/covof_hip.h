@@ -122,7 +122,9 @@ int covof_ctx_reserve(covof_ctx *ctx, int parts);
 /* The split of the context's last mixed-precision CG solve: iterations of
  * the FP32 recurrence and of the FP64 continuation that follows a stall (0
  * when it did not run).  After a continuation the solve's relres is the
- * recomputed true FP64 residual. */
+ * recomputed true FP64 residual and its status the FP64 PCG's (stopped on
+ * its recurrence residual), so where the system's attainable accuracy is
+ * above tol the relres can exceed tol with status COVOF_OK. */
 int covof_ctx_mixed_solve_info(covof_ctx *ctx, int *fp32_iters, int *fp64_iters);
 
 /* A solid obstacle: config_t's obs_enabled, obs_type (1 box [x0,x1]x[y0,y1],
