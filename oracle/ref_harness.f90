@@ -107,6 +107,10 @@ program covof_ref_harness
      call run_spobs()
   case ('sim')
      call run_sim()
+  case ('tadvect')
+     call run_tadvect()
+  case ('tsp')
+     call run_tsp()
   case default
      write(*, '(A,A)') 'unknown mode ', trim(mode)
      call MPI_Abort(MPI_COMM_WORLD, 2, ierr)
@@ -210,6 +214,107 @@ contains
     end do
     write(uout) fields%rho, fields%rho_u, fields%rho_v, fields%mu, fields%mu_node
   end subroutine run_advect
+
+  ! The advect mode's inputs; nsteps of advect_vof timed by the wall clock
+  ! (MPI_Wtime); writes the seconds and the final liquid volume.
+  subroutine run_tadvect()
+    integer :: nx, ny, bc(4), method, two_phase, nsteps, s
+    real(dp) :: lx, ly, dt, rho1, rho2, mu1, mu2, t0, t1
+    type(mesh_2d_t) :: mesh
+    type(config_t) :: cfg
+    type(flow_fields_t) :: fields
+    read(uin) nx, ny, bc, method, two_phase, nsteps
+    read(uin) lx, ly, dt, rho1, rho2, mu1, mu2
+    call setup(mesh, cfg, nx, ny, lx, ly, bc)
+    cfg%advection_method = 'WEYMOUTH_YUE'
+    cfg%reconstruction_method = method_name(method)
+    cfg%enable_two_phase = (two_phase /= 0)
+    cfg%rho1 = rho1
+    cfg%rho2 = rho2
+    cfg%mu1 = mu1
+    cfg%mu2 = mu2
+    cfg%use_markers = .false.
+    cfg%enable_svof_smoothing = .false.
+    cfg%enable_diffusion_smoothing = .false.
+    cfg%enable_surface_tension = .false.
+    cfg%compressible = .false.
+    call fields%allocate(mesh)
+    fields%rho = 1.0_dp
+    fields%rho_u = 1.0_dp
+    fields%rho_v = 1.0_dp
+    fields%mu = 1.0_dp
+    fields%mu_node = 1.0_dp
+    read(uin) fields%C, fields%u, fields%v
+    t0 = MPI_Wtime()
+    do s = 1, nsteps
+       call advect_vof(fields, mesh, cfg, dt)
+    end do
+    t1 = MPI_Wtime()
+    write(uout) t1 - t0, sum(fields%C(1:nx, 1:ny))
+  end subroutine run_tadvect
+
+  ! One standard-projection step without convection, timed by the wall
+  ! clock: the predictor calls alone (on a copy, as spobs replays them), then
+  ! solve_standard_projection (predictor + the Poisson block with its
+  ! hard-coded CG + SGS to 1e-7 + the correction).  Writes both times and
+  ! the iteration counts.
+  subroutine run_tsp()
+    integer :: nx, ny, bc(4), iters, iters_u
+    real(dp) :: lx, ly, dt, rho1, rho2, mu1, mu2, gx, gy, t0, t1, t2
+    type(simulation_t) :: sim
+    type(flow_fields_t) :: keep
+    real(dp), allocatable :: rhs(:), state(:)
+    read(uin) nx, ny, bc
+    read(uin) lx, ly, dt, rho1, rho2, mu1, mu2, gx, gy
+    call setup(sim%mesh, sim%config, nx, ny, lx, ly, bc)
+    sim%config%solver_type = SOLVER_STANDARD_PROJECTION
+    sim%config%enable_two_phase = .true.
+    sim%config%rho1 = rho1
+    sim%config%rho2 = rho2
+    sim%config%mu1 = mu1
+    sim%config%mu2 = mu2
+    sim%config%enable_surface_tension = .false.
+    sim%config%use_markers = .false.
+    sim%config%enable_svof_smoothing = .false.
+    sim%config%enable_diffusion_smoothing = .false.
+    sim%config%compressible = .false.
+    sim%config%time_scheme = 1
+    sim%config%enable_convection = .false.
+    sim%config%gx = gx
+    sim%config%gy = gy
+    sim%config%linear_solver = 1
+    sim%config%precond_type = 1
+    sim%config%use_precond = .true.
+    sim%config%tol = 1.0e-10_dp
+    sim%config%max_iter = 1000
+    sim%config%enable_energy = .false.
+    sim%config%enable_ssp_rk3_momentum = .false.
+    sim%config%obs_enabled = .false.
+    call sim%fields%allocate(sim%mesh)
+    call sim%ns_solver%init(sim%mesh, sim%config)
+    read(uin) sim%fields%C, sim%fields%u, sim%fields%v
+    sim%fields%p = 0.0_dp
+    call update_physical_properties(sim%fields, sim%mesh, sim%config)
+    sim%fields%u_nm1 = sim%fields%u
+    sim%fields%v_nm1 = sim%fields%v
+    sim%fields%has_nm1 = .false.
+    sim%fields%u_old = sim%fields%u
+    sim%fields%v_old = sim%fields%v
+    call apply_velocity_bc(sim%fields, sim%mesh, sim%config, .false.)
+    keep = sim%fields
+    t0 = MPI_Wtime()
+    call sim%ns_solver%assemble_preconditioner_blocks(sim%mesh, sim%config, sim%fields, dt)
+    allocate(rhs(sim%ns_solver%n_owned_total), state(sim%ns_solver%n_owned_total))
+    call sim%ns_solver%assemble_rhs(sim%mesh, sim%config, sim%fields, dt, rhs)
+    call sim%ns_solver%pack_vector(sim%mesh, sim%config, sim%fields, state)
+    call solve_linear_system(sim%ns_solver, sim%mesh, sim%config, sim%fields, dt, rhs, state, iters_u)
+    call sim%ns_solver%unpack_vector(sim%mesh, sim%config, state, sim%fields, .false.)
+    t1 = MPI_Wtime()
+    sim%fields = keep
+    call sim%solve_standard_projection(dt, iters)
+    t2 = MPI_Wtime()
+    write(uout) t1 - t0, t2 - t1, iters, iters_u
+  end subroutine run_tsp
 
   subroutine run_surface()
     integer :: nx, ny, bc(4), method, svof, diff, dsteps, tension, nsteps, s

@@ -30,7 +30,7 @@ P, DI, NE, FS = 3, 1, 2, 4
 
 
 def run(mode: str, payload: bytes) -> bytes:
-    with tempfile.TemporaryDirectory() as td:
+    with tempfile.TemporaryDirectory(dir=os.path.dirname(HARNESS)) as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fi, "wb") as f:
             f.write(payload)
@@ -318,7 +318,7 @@ def output_cases():
     payload = i32(nx, ny, FS, FS, DI, DI, len(snaps)) + f64(lx, ly)
     for step, t, dt, f in snaps:
         payload += i32(step) + f64(t, dt) + b"".join(f[k].astype("<f8").tobytes() for k in keys)
-    with tempfile.TemporaryDirectory() as td:
+    with tempfile.TemporaryDirectory(dir=os.path.dirname(HARNESS)) as td:
         fi, fo = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fi, "wb") as fh:
             fh.write(payload)
@@ -359,7 +359,7 @@ def case_run(js=None, out_name="case_bubble_json.npz"):
     5e-4); the history file and the .vtr files of steps 0, 5, 10, 13."""
     import json
     js = CASE_JSON if js is None else js
-    with tempfile.TemporaryDirectory() as td:
+    with tempfile.TemporaryDirectory(dir=os.path.dirname(HARNESS)) as td:
         cfg = json.loads(json.dumps(js))
         cfg["output"]["output_dir"] = os.path.join(td, "out")
         path = os.path.join(td, "case.json")
