@@ -470,31 +470,39 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
                                                const T *__restrict__ dprev, T *__restrict__ d,
                                                T *__restrict__ Ad, T *__restrict__ x, double *scal,
                                                PcgSlots sl, const double *pin, double *pout, int k) {
-  // Each wave walks a contiguous run of rows of its 64-column strip keeping
-  // d, a_P and the face coefficients of the rows below / at / above in
-  // registers: every value is loaded once, x-neighbours come from the
-  // adjacent lanes, and only the strip edges and the periodic wraps load
-  // extra values.  The row above is loaded one step ahead (its z needs its
-  // own a_P).  Rows go in batches of G whose loads are all issued before the
-  // batch is evaluated, so each wave keeps several rows of loads in flight.
-  // The matvec evaluates poisson_matvec's expression in its order on the
-  // same operands.
+  // The block's four waves walk the same run of rows over four adjacent
+  // 64-column strips, each keeping d, a_P and the face coefficients of the
+  // rows below / at / above in registers: every value is loaded once,
+  // x-neighbours come from the adjacent lanes, and at the strip edges inside
+  // the block from the adjacent wave through LDS (one exchange of the west
+  // faces and one of the edge lanes' d per batch of rows); only the block's
+  // outer edges and the periodic wraps load extra values.  The row above is
+  // loaded one step ahead (its z needs its own a_P).  Rows go in batches of G
+  // whose loads are all issued before the batch is evaluated, so each wave
+  // keeps several rows of loads in flight.  The matvec evaluates
+  // poisson_matvec's expression in its order on the same operands.
   constexpr int G = sizeof(T) == 8 ? PCG_ROWS_F64 : PCG_ROWS_F32;
+  __shared__ T s_kw[4][G], s_c0[4][G], s_c63[4][G], s_ke0[4];
   const int nx = o.nx, ny = o.ny;
   const int gx_ = (nx + 63) / 64;
+  const int gq_ = (gx_ + 3) / 4;  // groups of four strips
   const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;
-  const int chunks_ = (gridDim.x >> 3) / gx_;
+  const int chunks_ = (gridDim.x >> 3) / gq_;
   const int band_ = (((ny + 7) / 8) + 3) & ~3;
-  const int rows_ = (((band_ + chunks_ - 1) / chunks_) + 3) & ~3;
+  const int rows_ = (band_ + chunks_ - 1) / chunks_;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = (local_ % gx_) * 64 + lane + 1;
-  const int jb = xcd_ * band_ + (local_ / gx_) * rows_;
+  const int i = ((local_ % gq_) * 4 + w) * 64 + lane + 1;
+  const int jb = xcd_ * band_ + (local_ / gq_) * rows_;
   const int je = min(min(jb + rows_, xcd_ * band_ + band_), ny);
-  const int rw = (je - jb + 3) / 4;
-  const int jw0 = jb + 1 + w * rw, jw1 = min(jw0 + rw - 1, je);
+  const int jw0 = jb + 1, jw1 = je;
   const bool vi = i <= nx;
-  const bool act = local_ < gx_ * chunks_ && jw0 <= jw1;  // uniform over the wave
-  const bool edge_r = i == nx || lane == 63;              // right neighbour not in the wave
+  const bool act = local_ < gq_ * chunks_ && jw0 <= jw1;  // uniform over the block
+  // the right neighbour: the next lane, the next wave's lane 0 (through LDS),
+  // or a load (the block's last strip, the mesh's last column)
+  const bool right_lds = lane == 63 && w < 3 && i < nx;
+  const bool right_load = i == nx || (lane == 63 && w == 3);
+  const bool left_lds = lane == 0 && w > 0;
+  const bool left_load = lane == 0 && w == 0;
   auto at = [&](int a, int b) { return (long)(b - 1) * nx + (a - 1); };
   auto zval = [&](T rv, T ap) -> T { return PC == 1 ? jacobi<T>(rv, ap) : rv; };
   // The first row's operands and the row below do not depend on beta: they
@@ -514,7 +522,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
     kw = A.kx[kxi(nx, i, jw0)];
     kn = A.ky[kyi(nx, i, jw0 + 1)];
     if (i == nx) ke = A.kx[kxi(nx, nx + 1, jw0)];
-    else if (lane == 63) ke = A.kx[kxi(nx, i + 1, jw0)];
+    else if (right_load) ke = A.kx[kxi(nx, i + 1, jw0)];
   }
   const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
   const bool first = scal[sl.first] != 0.0;
@@ -561,7 +569,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
       scal[sl.pend] = 0.0;  // this kernel applies the pending update
     }
   }
-  if (stop != F_RUN) return;  // the pending update stays recorded
+  if (stop != F_RUN) return;  // the pending update stays recorded (uniform: every block returns)
   // the update of iteration k-1 is pending unless this is the first
   // iteration (after a reliable update it was folded into the correction)
   const bool pending = !first && !(sl.inner && k == 1);
@@ -575,8 +583,12 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   else if (!first) bd = tot[1] / rz_prev;
   const T beta = (T)bd;
   T acc[1] = {T(0)};
+  // the first row's east face at the strip edges inside the block
+  if (lane == 0) s_ke0[w] = kw;
+  lds_barrier();
+  if (right_lds) ke = s_ke0[w + 1];
   T aE = __shfl_down(kw, 1, 64);
-  if (edge_r) aE = ke;
+  if (right_lds || right_load) aE = ke;
   T aP = kw + aE + aS + kn + (T)o.eps;
   T pcur = p0;
   T cur = zval(r0, aP) + beta * p0;
@@ -584,8 +596,8 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   if (act) {
     for (int j0 = jw0; j0 <= jw1; j0 += G) {
       // loads of the batch: the row above (r or z, d_prev, kx(i), ky(i, +1),
-      // and kx(i+1) at the strip edge), x, and at the strip edges the
-      // x-neighbours (r or z, d_prev, diagonal)
+      // and kx(i+1) at the block's right edge), x, and at the block's edges
+      // the x-neighbours (r or z, d_prev, diagonal)
       T ru[G], pu[G], kwu[G], knu[G], keu[G], ksu[G], xo[G], rl[G], pl[G], dl[G], rr[G], pr[G], dr[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -601,10 +613,10 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
             knu[g] = A.ky[kyi(nx, i, ju + 1)];
             if (ju != j + 1) ksu[g] = A.ky[kyi(nx, i, ju)];  // periodic wrap: the row's own south face
             if (i == nx) keu[g] = A.kx[kxi(nx, nx + 1, ju)];
-            else if (lane == 63) keu[g] = A.kx[kxi(nx, i + 1, ju)];
+            else if (right_load) keu[g] = A.kx[kxi(nx, i + 1, ju)];
           }
           if (pending) xo[g] = x[at(i, j)];
-          if (lane == 0) {
+          if (left_load) {
             const int il = i > 1 ? i - 1 : (o.perW ? nx : 0);
             if (il > 0) {
               rl[g] = zr[at(il, j)];
@@ -618,34 +630,68 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
               pr[g] = dprev[at(1, j)];
               if (PC == 1) dr[g] = A.diag[at(1, j)];
             }
-          } else if (lane == 63) {
+          } else if (right_load) {
             rr[g] = zr[at(i + 1, j)];
             pr[g] = dprev[at(i + 1, j)];
             if (PC == 1) dr[g] = A.diag[at(i + 1, j)];
           }
         }
       }
+      // the next wave's west faces of the rows above (this wave's east faces at lane 63)
+      if (lane == 0) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) s_kw[w][g] = kwu[g];
+      }
+      lds_barrier();
+      if (right_lds) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) keu[g] = s_kw[w + 1][g];
+      }
+      // the rows above: a_P from their four faces (the south face is the
+      // row's north face unless the row wraps), z, d; curs[g] = d of row j0 + g
+      T ups[G], aEus[G], aPus[G], curs[G];
+      {
+        T knr = kn;  // north face of the row below the one being formed
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int j = j0 + g;
+          T aEu = __shfl_down(kwu[g], 1, 64);
+          if (right_lds || right_load) aEu = keu[g];
+          const T aSu = (j < ny) ? knr : ksu[g];
+          const T aPu = kwu[g] + aEu + aSu + knu[g] + (T)o.eps;
+          ups[g] = zval(ru[g], aPu) + beta * pu[g];
+          aEus[g] = aEu;
+          aPus[g] = aPu;
+          curs[g] = g == 0 ? cur : ups[g - 1];
+          knr = knu[g];
+        }
+      }
+      // the edge lanes' d of the batch's rows for the adjacent waves
+      if (lane == 0) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) s_c0[w][g] = curs[g];
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) s_c63[w][g] = curs[g];
+      }
+      lds_barrier();
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int j = j0 + g;
-        if (j > jw1) break;  // uniform over the wave
-        // the row above: a_P from its four faces (its south face is this
-        // row's north face unless the row wraps), z, d
-        T aEu = __shfl_down(kwu[g], 1, 64);
-        if (edge_r) aEu = keu[g];
-        const T aSu = (j < ny) ? kn : ksu[g];
-        const T aPu = kwu[g] + aEu + aSu + knu[g] + (T)o.eps;
-        const T up = zval(ru[g], aPu) + beta * pu[g];
+        if (j > jw1) break;  // uniform over the block
         T left = __shfl_up(cur, 1, 64);
         T right = __shfl_down(cur, 1, 64);
-        if (lane == 0) left = zval(rl[g], dl[g]) + beta * pl[g];
-        if (edge_r) right = zval(rr[g], dr[g]) + beta * pr[g];
+        if (left_lds) left = s_c63[w - 1][g];
+        if (left_load) left = zval(rl[g], dl[g]) + beta * pl[g];
+        if (right_lds) right = s_c0[w + 1][g];
+        if (right_load) right = zval(rr[g], dr[g]) + beta * pr[g];
         if (vi) {
           T val = aP * cur;
           if (i > 1 || o.perW) val = val - kw * left;
           if (i < nx || o.perE) val = val - aE * right;
           if (j > 1 || o.perS) val = val - aS * down;
-          if (j < ny || o.perN) val = val - kn * up;
+          if (j < ny || o.perN) val = val - kn * ups[g];
           const long q = at(i, j);
           d[q] = cur;
           Ad[q] = val;
@@ -653,14 +699,17 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
           acc[0] += cur * val;
         }
         down = cur;
-        cur = up;
+        cur = ups[g];
         pcur = pu[g];
         aS = kn;
         kw = kwu[g];
         kn = knu[g];
-        aE = aEu;
-        aP = aPu;
+        aE = aEus[g];
+        aP = aPus[g];
       }
+      // no barrier here: the next batch's writes of s_kw follow this batch's
+      // second barrier (every read of s_kw precedes it) and its writes of
+      // s_c0 / s_c63 its own first barrier (every read of them precedes it)
     }
   }
   block_partials<1, T>(acc, pout);
