@@ -161,14 +161,12 @@ CV_HD double compute_alpha(double xmin, double xmax, double ymin, double ymax, d
 }
 
 // Area of {a X + b Y <= g} inside [0, W] x [0, H] for a, b >= 0 in closed
-// form: with the roles of x and y arranged so that a W <= b H, a triangle
-// (g <= a W), a trapezoid across the width (g <= b H) or the box less a
-// triangle.  The constants depend on the box and the normal only; evaluate()
-// is branch-free (all three forms, then selects), so the lanes of a wave that
-// sit in different cases do not serialise.  Not bit-exact: used with an
-// error margin only.
+// form, and its inverse: with the roles of x and y arranged so that
+// a W <= b H, a triangle (g <= a W, area g^2 / 2ab), a trapezoid across the
+// width (g <= b H, area W (g - a W / 2) / b) or the box less a triangle.  Not
+// bit-exact: used with an error margin only.
 struct HalfPlaneArea {
-  double W, H, aW, bH, k2, ib, WH;
+  double W, aW, bH, WH, ab2, bW, T1;
   CV_HD void init(double W_, double H_, double a, double b) {
     if (a * W_ > b * H_) {
       const double t = a;
@@ -179,27 +177,101 @@ struct HalfPlaneArea {
       H_ = u;
     }
     W = W_;
-    H = H_;
-    aW = a * W;
-    bH = b * H;
-    k2 = 0.5 / (a * b);  // inf when a == 0: its form is then never selected
-    ib = 1.0 / b;
-    WH = W * H;
+    aW = a * W_;
+    bH = b * H_;
+    WH = W_ * H_;
+    ab2 = 2.0 * a * b;
+    bW = b / W_;
+    T1 = 0.5 * aW * W_ / b;  // area at g = a W (0 when a == 0)
   }
-  CV_HD double evaluate(double g) const {
-    const double tri = g * g * k2;
-    const double trap = W * (g - 0.5 * aW) * ib;
-    const double r = aW + bH - g;
-    const double top = WH - r * r * k2;
-    double v = g <= bH ? trap : top;
-    v = g <= aW ? tri : v;
-    v = g >= aW + bH ? WH : v;
-    return g <= 0.0 ? 0.0 : v;
+  // g with area(g) = T, for 0 < T < W H
+  CV_HD double inverse(double T) const {
+    if (T <= T1) return sqrt(ab2 * T);
+    if (T <= WH - T1) return T * bW + 0.5 * aW;
+    return (aW + bH) - sqrt(ab2 * (WH - T));
   }
 };
 
-// compute_alpha with the same result bit for bit and fewer clips.  The
-// bisection's step k only needs the sign of a(mid) - target, where a is
+// cut_area_w for lines whose inside corners (d_k <= 0) are a fixed set: one to
+// three consecutive corners, entered across edge ea (outside -> inside) and
+// left across edge eb.  The clip then always emits the same vertex sequence --
+// a rotation of the cycle X_a, C_f .. C_l, X_b (crossings X, inside corners C
+// from the first to the last in counter-clockwise order) that starts where
+// edge 0's emission starts -- so its shoelace sum is that of the cycle's terms
+//   V1 = t(X_a, C_f), K1, K2 (corner-to-corner, constant), V2 = t(C_l, X_b),
+//   V3 = t(X_b, X_a),      t(p, q) = p.x q.y - q.x p.y,
+// added in the rotated order from a zero start.  Only the two crossings depend
+// on alpha; each is the clip's own t = -d1 / (d2 - d1), x1 + t (x2 - x1),
+// y1 + t (y2 - y1).  A missing K is a zero added after a nonzero partial sum
+// or to a zero one, which changes at most the sign of a zero, and the area is
+// |s| / 2: area() equals cut_area_w bit for bit for every alpha that keeps the
+// inside set.  The rotation (cls) is fixed by which of corners 0, 1, 2 are
+// inside: corner 0 outside -> V1 first; 0 in, 1 out -> V3 first; 0, 1 in,
+// 2 out -> V2 first; 0, 1, 2 in -> K2 first.
+struct FixedCut {
+  double wa1, wa2, xa, ya, dxa, dya;  // entry edge: corner values, start corner, x2 - x1, y2 - y1
+  double wb1, wb2, xb, yb, dxb, dyb;  // exit edge
+  double cfx, cfy, clx, cly;          // first and last inside corners
+  double K1, K2;
+  int cls;
+
+  CV_HD static double pick(int k, double a0, double a1, double a2, double a3) {
+    return k == 0 ? a0 : k == 1 ? a1 : k == 2 ? a2 : a3;
+  }
+  // false when the inside set is not one to three consecutive corners
+  CV_HD bool init(double xmin, double xmax, double ymin, double ymax, double w0, double w1, double w2,
+                  double w3, bool i0, bool i1, bool i2, bool i3) {
+    const int m = (int)i0 + (int)i1 + (int)i2 + (int)i3;
+    const int nin = (int)(!i0 && i1) + (int)(!i1 && i2) + (int)(!i2 && i3) + (int)(!i3 && i0);
+    if (m < 1 || m > 3 || nin != 1) return false;
+    const int ea = (!i0 && i1) ? 0 : (!i1 && i2) ? 1 : (!i2 && i3) ? 2 : 3;
+    const int eb = (i0 && !i1) ? 0 : (i1 && !i2) ? 1 : (i2 && !i3) ? 2 : 3;
+    const int f = (ea + 1) & 3, f1 = (ea + 2) & 3, f2 = (ea + 3) & 3;
+    const int ea1 = f, eb1 = (eb + 1) & 3;
+    wa1 = pick(ea, w0, w1, w2, w3);
+    wa2 = pick(ea1, w0, w1, w2, w3);
+    wb1 = pick(eb, w0, w1, w2, w3);
+    wb2 = pick(eb1, w0, w1, w2, w3);
+    xa = pick(ea, xmin, xmax, xmax, xmin);
+    ya = pick(ea, ymin, ymin, ymax, ymax);
+    dxa = pick(ea1, xmin, xmax, xmax, xmin) - xa;
+    dya = pick(ea1, ymin, ymin, ymax, ymax) - ya;
+    xb = pick(eb, xmin, xmax, xmax, xmin);
+    yb = pick(eb, ymin, ymin, ymax, ymax);
+    dxb = pick(eb1, xmin, xmax, xmax, xmin) - xb;
+    dyb = pick(eb1, ymin, ymin, ymax, ymax) - yb;
+    cfx = pick(f, xmin, xmax, xmax, xmin);
+    cfy = pick(f, ymin, ymin, ymax, ymax);
+    clx = pick(eb, xmin, xmax, xmax, xmin);
+    cly = pick(eb, ymin, ymin, ymax, ymax);
+    const double c1x = pick(f1, xmin, xmax, xmax, xmin), c1y = pick(f1, ymin, ymin, ymax, ymax);
+    const double c2x = pick(f2, xmin, xmax, xmax, xmin), c2y = pick(f2, ymin, ymin, ymax, ymax);
+    K1 = m >= 2 ? cfx * c1y - c1x * cfy : 0.0;
+    K2 = m >= 3 ? c1x * c2y - c2x * c1y : 0.0;
+    cls = !i0 ? 0 : !i1 ? 1 : !i2 ? 2 : 3;
+    return true;
+  }
+  CV_HD double area(double alpha) const {
+    const double a1 = wa1 + alpha, a2 = wa2 + alpha;
+    const double ta = -a1 / (a2 - a1);
+    const double pax = xa + ta * dxa, pay = ya + ta * dya;
+    const double b1 = wb1 + alpha, b2 = wb2 + alpha;
+    const double tb = -b1 / (b2 - b1);
+    const double pbx = xb + tb * dxb, pby = yb + tb * dyb;
+    const double V1 = pax * cfy - cfx * pay;
+    const double V2 = clx * pby - pbx * cly;
+    const double V3 = pbx * pay - pax * pby;
+    double s;
+    if (cls == 0) s = (((V1 + K1) + K2) + V2) + V3;
+    else if (cls == 1) s = (((V3 + V1) + K1) + K2) + V2;
+    else if (cls == 2) s = (((V2 + V3) + V1) + K1) + K2;
+    else s = (((K2 + V2) + V3) + V1) + K1;
+    return 0.5 * fabs(s);
+  }
+};
+
+// compute_alpha with the same result bit for bit and far fewer operations.
+// The bisection's step k only needs the sign of a(mid) - target, where a is
 // cut_area_w's floating-point area.  That area differs from the exact area A
 // of the line's cut by at most Ef: the shoelace terms are cross products of
 // vertices in absolute coordinates (|x| <= X, |y| <= Y, |p| <= Rp), each
@@ -207,13 +279,17 @@ struct HalfPlaneArea {
 // bounded by Rp * diag; the corner values n.p + mid carry at most ~4u S
 // (S = |nx| X + |ny| Y) and every computed cut vertex lies within ~3 of
 // those of the exact line, so the cut polygon differs from the exact one
-// within a band of that width along a chord of at most diag.  A is evaluated
-// in closed form from the box corner of least n.p (HalfPlaneArea; its
-// offset vmin carries ~2u S, moving the area by at most 2u S diag / |n|).
-// Steps whose closed-form area is farther than E (four times the sum of
-// those bounds) from the target have the same outcome as the clip and skip
-// it; the steps near the root -- about the last 20 of 45 at cell sizes of
-// 1e-3 -- evaluate cut_area_w exactly as compute_alpha does.
+// within a band of that width along a chord of at most diag.  A is known in
+// closed form from the box corner of least n.p (HalfPlaneArea; its offset
+// vmin carries ~2u S, moving the area by at most 2u S diag / |n|); E is four
+// times the sum of those bounds.  Inverting the closed form at target +- 2E
+// gives a window [m_lo, m_hi] of line constants: below it A > target + E
+// (a(mid) > target, the step moves lo), above it A < target - E (the step
+// moves hi); the window's own rounding (a few u of S and diag in mid) moves
+// A by far less than E.  Only the steps whose mid falls in the window -- about
+// the last 20 of 45 at cell sizes of 1e-3 -- evaluate the clip, and where the
+// window keeps every corner on one side (the corner values are monotone in
+// mid) they evaluate it through FixedCut.
 CV_HD double compute_alpha_fast(double xmin, double xmax, double ymin, double ymax, double nx,
                                 double ny, double c) {
   const double total = (xmax - xmin) * (ymax - ymin);
@@ -241,20 +317,40 @@ CV_HD double compute_alpha_fast(double xmin, double xmax, double ymin, double ym
   const double amin = a < b ? a : b;
   const bool cert = !degenerate && nn > 0.5 && nn < 2.0 && X < 1.0e100 && Y < 1.0e100 &&
                     (amin == 0.0 || amin > 1.0e-150);
-  const double E = 4.0 * 0x1p-53 * (8.0 * X * Y + 20.0 * Rp * diag + 30.0 * S * diag / nn + 8.0 * W * Hh);
-  HalfPlaneArea hp;
-  hp.init(W, Hh, a, b);
+  // the window of steps evaluated exactly, within the bracket (every mid lies strictly inside it)
+  double m_lo = lo, m_hi = hi;
+  if (cert) {
+    const double E = 4.0 * 0x1p-53 * (8.0 * X * Y + 20.0 * Rp * diag + 30.0 * S * diag / nn + 8.0 * W * Hh);
+    HalfPlaneArea hp;
+    hp.init(W, Hh, a, b);
+    const double Tp = target + 2.0 * E, Tm = target - 2.0 * E;
+    if (Tp < hp.WH) {
+      const double m = -hp.inverse(Tp) - vmin;
+      m_lo = m > m_lo ? m : m_lo;
+    }
+    if (Tm > 0.0) {
+      const double m = -hp.inverse(Tm) - vmin;
+      m_hi = m < m_hi ? m : m_hi;
+    }
+  }
+  FixedCut fc;
+  const bool fixed = !degenerate &&
+                     ((v1 + m_lo <= 0.0) == (v1 + m_hi <= 0.0)) && ((v2 + m_lo <= 0.0) == (v2 + m_hi <= 0.0)) &&
+                     ((v3 + m_lo <= 0.0) == (v3 + m_hi <= 0.0)) && ((v4 + m_lo <= 0.0) == (v4 + m_hi <= 0.0)) &&
+                     fc.init(xmin, xmax, ymin, ymax, v1, v2, v3, v4, v1 + m_lo <= 0.0, v2 + m_lo <= 0.0,
+                             v3 + m_lo <= 0.0, v4 + m_lo <= 0.0);
   for (int it = 0; it < 45; ++it) {
     const double mid = 0.5 * (lo + hi);
-    int below = -1;  // 1: a(mid) < target, 0: a(mid) >= target
-    if (cert) {
-      const double ah = hp.evaluate(-(mid + vmin));
-      if (ah < target - E) below = 1;
-      else if (ah > target + E) below = 0;
-    }
-    if (below < 0) {
-      const double ar = degenerate ? 0.0 : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
-      below = ar < target ? 1 : 0;
+    bool below;  // a(mid) < target
+    if (mid < m_lo) {
+      below = false;
+    } else if (mid > m_hi) {
+      below = true;
+    } else {
+      const double ar = degenerate ? 0.0
+                        : fixed    ? fc.area(mid)
+                                   : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
+      below = ar < target;
     }
     if (below)
       hi = mid;

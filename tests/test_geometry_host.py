@@ -21,8 +21,9 @@ def host(tmp_path_factory):
     subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-I", INC, SRC,
                     "-o", out], check=True)
     L = ct.CDLL(out)
-    L.host_alpha_fast_mismatches.restype = ct.c_long
-    L.host_alpha_fast_mismatches.argtypes = [ct.c_long, ct.c_ulonglong]
+    for n in ("host_alpha_fast_mismatches", "host_fixed_cut_mismatches"):
+        getattr(L, n).restype = ct.c_long
+        getattr(L, n).argtypes = [ct.c_long, ct.c_ulonglong]
     for n, k in (("host_hypot_ref", 2), ("host_cut_box_area", 7), ("host_compute_alpha", 7),
                  ("host_full_box_area", 4), ("host_compute_alpha_fast", 7)):
         f = getattr(L, n)
@@ -80,16 +81,24 @@ def test_full_box_area_is_the_clip_of_a_full_cell(host):
 
 
 def test_compute_alpha_fast_is_compute_alpha(host):
-    """The kernels' compute_alpha skips the bisection steps whose outcome its
-    closed-form area certifies; the result must equal the 45-step clip
-    bisection bit for bit: on the golden vectors and on 1.2 million random
-    ELVIRA-like calls (cells of 16..8192-cell grids on domains of 1e-3..250,
-    unit, axis-aligned, nearly axis-aligned and unnormalised normals,
-    fractions at and around the thresholds).  With the certification margin
-    cut 13-fold the same sample shows hundreds of mismatches, so it does
-    exercise the margin."""
+    """The kernels' compute_alpha settles the bisection steps outside the
+    window its closed-form area certifies without a clip and evaluates the
+    others through the fixed-topology clip; the result must equal the 45-step
+    clip bisection bit for bit: on the golden vectors and on 1.2 million
+    random ELVIRA-like calls (cells of 16..8192-cell grids on domains of
+    1e-3..250, unit, diagonal, axis-aligned, nearly axis-aligned and
+    unnormalised normals, fractions at and around the thresholds and 0.5).
+    With the certification margin cut 26-fold the same sample shows about
+    140 mismatches per 400000 calls, so it does exercise the margin."""
     g = np.load(os.path.join(ROOT, "tests", "golden", "geom.npz"), allow_pickle=False)
     for k in range(len(g["xmin"])):
         a = (g["xmin"][k], g["xmax"][k], g["ymin"][k], g["ymax"][k], g["nx"][k], g["ny"][k], g["c"][k])
         assert host.host_compute_alpha_fast(*a) == g["alpha_out"][k]
     assert sum(host.host_alpha_fast_mismatches(400000, s) for s in (101, 202, 303)) == 0
+
+
+def test_fixed_cut_is_the_clip(host):
+    """The fixed-topology clip (the shoelace terms of the clip's vertex cycle
+    in its emission order) equals cut_area_w bit for bit on 600000 random
+    boxes and lines cutting them, a third through a corner exactly."""
+    assert sum(host.host_fixed_cut_mismatches(200000, s) for s in (7, 8, 9)) == 0
