@@ -261,11 +261,12 @@ struct FixedCut {
     const double V1 = pax * cfy - cfx * pay;
     const double V2 = clx * pby - pbx * cly;
     const double V3 = pbx * pay - pax * pby;
-    double s;
-    if (cls == 0) s = (((V1 + K1) + K2) + V2) + V3;
-    else if (cls == 1) s = (((V3 + V1) + K1) + K2) + V2;
-    else if (cls == 2) s = (((V2 + V3) + V1) + K1) + K2;
-    else s = (((K2 + V2) + V3) + V1) + K1;
+    // the four rotations, then a select (no branches in a wave whose lanes differ)
+    const double s0 = (((V1 + K1) + K2) + V2) + V3;
+    const double s1 = (((V3 + V1) + K1) + K2) + V2;
+    const double s2 = (((V2 + V3) + V1) + K1) + K2;
+    const double s3 = (((K2 + V2) + V3) + V1) + K1;
+    const double s = cls == 0 ? s0 : cls == 1 ? s1 : cls == 2 ? s2 : s3;
     return 0.5 * fabs(s);
   }
 };
@@ -339,23 +340,34 @@ CV_HD double compute_alpha_fast(double xmin, double xmax, double ymin, double ym
                      ((v3 + m_lo <= 0.0) == (v3 + m_hi <= 0.0)) && ((v4 + m_lo <= 0.0) == (v4 + m_hi <= 0.0)) &&
                      fc.init(xmin, xmax, ymin, ymax, v1, v2, v3, v4, v1 + m_lo <= 0.0, v2 + m_lo <= 0.0,
                              v3 + m_lo <= 0.0, v4 + m_lo <= 0.0);
-  for (int it = 0; it < 45; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    bool below;  // a(mid) < target
-    if (mid < m_lo) {
-      below = false;
-    } else if (mid > m_hi) {
-      below = true;
-    } else {
-      const double ar = degenerate ? 0.0
-                        : fixed    ? fc.area(mid)
-                                   : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
-      below = ar < target;
+  // below: a(mid) < target.  Two loops, so that the general clip's operands
+  // are not held through the common one.
+  if (fixed) {
+    for (int it = 0; it < 45; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      const bool below = mid < m_lo ? false : mid > m_hi ? true : fc.area(mid) < target;
+      if (below)
+        hi = mid;
+      else
+        lo = mid;
     }
-    if (below)
-      hi = mid;
-    else
-      lo = mid;
+  } else {
+    for (int it = 0; it < 45; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      bool below;
+      if (mid < m_lo) {
+        below = false;
+      } else if (mid > m_hi) {
+        below = true;
+      } else {
+        const double ar = degenerate ? 0.0 : cut_area_w(xmin, xmax, ymin, ymax, v1, v2, v3, v4, mid);
+        below = ar < target;
+      }
+      if (below)
+        hi = mid;
+      else
+        lo = mid;
+    }
   }
   return 0.5 * (lo + hi);
 }

@@ -102,6 +102,10 @@ __device__ void plic_mixed(const double *C, const DevMesh &m, int i, int j, doub
                       face_coord(j + 1, m.dy), pnx, pny, c);
 }
 
+// Neighbour q = 0..7 of a 3x3 block in row-major order without the centre.
+__device__ __forceinline__ int nb_di(int q) { return q < 3 ? q - 1 : q == 3 ? -1 : q == 4 ? 1 : q - 6; }
+__device__ __forceinline__ int nb_dj(int q) { return q < 3 ? -1 : q < 5 ? 0 : 1; }
+
 // One ELVIRA candidate k (0..6) of a mixed interior cell (:365-440): unit
 // normal, alpha and the squared 3x3 volume error.  valid = false when the
 // candidate's norm is below 1e-12 (skipped by the reference).
@@ -144,16 +148,39 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
   const double xmin = face_coord(i, dx), xmax = face_coord(i + 1, dx);
   const double ymin = face_coord(j, dy), ymax = face_coord(j + 1, dy);
   al = compute_alpha_fast(xmin, xmax, ymin, ymax, ux, uy, c);
+  // cut areas of the 8 neighbours (row-major, centre skipped): the line
+  // leaves every corner of most of them on one side -- area 0, or the box's
+  // own shoelace sum, which is the clip's with all four corners kept -- and
+  // each lane clips only the neighbours it cuts, so a wave runs as many clips
+  // as its busiest lane rather than eight.
+  double vr[8];
+  unsigned cut = 0u;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int di = nb_di(q), dj = nb_dj(q);
+    const double x0 = face_coord(i + di, dx), x1 = face_coord(i + di + 1, dx);
+    const double y0 = face_coord(j + dj, dy), y1 = face_coord(j + dj + 1, dy);
+    const int kin = (int)(ux * x0 + uy * y0 + al <= 0.0) + (int)(ux * x1 + uy * y0 + al <= 0.0) +
+                    (int)(ux * x1 + uy * y1 + al <= 0.0) + (int)(ux * x0 + uy * y1 + al <= 0.0);
+    vr[q] = kin == 4 ? full_box_area(x0, x1, y0, y1) : 0.0;
+    if (kin > 0 && kin < 4) cut |= 1u << q;
+  }
+  while (cut) {
+    const int q = __builtin_ctz(cut);
+    cut &= cut - 1u;
+    const int di = nb_di(q), dj = nb_dj(q);
+    const double a = cut_box_area(face_coord(i + di, dx), face_coord(i + di + 1, dx), face_coord(j + dj, dy),
+                                  face_coord(j + dj + 1, dy), ux, uy, al);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) vr[r] = (q == r) ? a : vr[r];
+  }
   double e = 0.0;
-  for (int dj = -1; dj <= 1; ++dj)
-    for (int di = -1; di <= 1; ++di) {
-      if (di == 0 && dj == 0) continue;
-      const double vt = CC(di, dj) * dx * dy;
-      const double vr = cut_box_area(face_coord(i + di, dx), face_coord(i + di + 1, dx),
-                                     face_coord(j + dj, dy), face_coord(j + dj + 1, dy), ux, uy, al);
-      const double df = vr - vt;
-      e = e + df * df;
-    }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double vt = CC(nb_di(q), nb_dj(q)) * dx * dy;
+    const double df = vr[q] - vt;
+    e = e + df * df;
+  }
   err = e;
 #undef CC
 }
@@ -165,6 +192,8 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
 // of its mixed cells (block-local compaction through LDS: no global atomics)
 // for the next reconstruction, which runs one block per tile.
 constexpr int TX = 64, TY = 16, TCELLS = TX * TY, TBLK = 256;
+// ELVIRA reconstruction: 9 cells of 7 candidate lanes per wave
+constexpr int ELV_PER_WAVE = 9;
 
 __device__ __forceinline__ int tiles_x(const DevMesh &m) { return (m.nx + TX - 1) / TX; }
 __device__ __forceinline__ int tiles_y(const DevMesh &m) { return (m.ny + TY - 1) / TY; }
@@ -250,37 +279,57 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ coun
   if (threadIdx.x == 1023) offs[ntiles] = part[1023];
 }
 
-// METHOD 0 = PLIC, 1 = ELVIRA.  The mixed interior cells of all tile lists
-// followed by the halo ring form one index space, dealt in equal slices to a
-// fixed grid, so every wave is full whatever the interface's layout: an
-// interior index maps to its tile by binary search in offs.  For every mixed
-// ring cell the normal of its source cell is reconstructed again (the same
-// arithmetic), renormalised and alpha re-inverted in the ghost box.  ELVIRA
-// spreads the 7 candidates of a cell over 8 lanes; the winner is the first
-// candidate reaching the smallest error below 1e30 (the reference's strict '<'
-// scan from best_err = 1e30).
+// METHOD 0 = PLIC, 1 = ELVIRA.  The work is a list of units, each one wave's
+// batch of 64 cells (PLIC, a lane per cell) or 9 (ELVIRA, the 7 candidates of
+// a cell on 7 lanes, lane 63 idle): first the halo ring, then the mixed
+// interior cells of all tile lists (an interior index maps to its tile by
+// binary search in offs).  Wave w of the grid takes units w, w + (waves in
+// the grid), ...; the grid has a wave per unit of a typical interface, so the
+// hardware dispatcher balances the workgroups over the CUs, and the ring
+// comes first because most of its cells are not mixed: its waves end at once
+// and free their slots for interior waves instead of queueing behind them.
+// (A counter handing out units to a resident grid was slower: its atomics
+// are serialised at one address.)  For every mixed ring cell the normal of
+// its source cell is reconstructed again (the same arithmetic),
+// renormalised and alpha re-inverted in the ghost box.  The kernel is held to
+// 128 VGPRs, four waves to a SIMD.  ELVIRA's winner is the first candidate
+// reaching the smallest error below 1e30 (the reference's strict '<' scan
+// from best_err = 1e30).
 template <int METHOD>
-__global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
-                                                const int *__restrict__ offs, double *snx, double *sny,
-                                                double *sal) {
+__global__ void __launch_bounds__(TBLK, 4) k_recon(const double *__restrict__ C, DevMesh m, const int *__restrict__ list,
+                                                   const int *__restrict__ offs, double *snx, double *sny, double *sal) {
   const int ntiles = tiles_x(m) * tiles_y(m);
+  const int nring = ring_len(m);
   const int nmix = offs[ntiles];
-  const int n = nmix + ring_len(m);
-  const int per_pass = (METHOD == 1) ? TBLK / 8 : TBLK;
-  const int stride = per_pass * gridDim.x;
-  const int first = per_pass * blockIdx.x;
-  const int lane8 = threadIdx.x & 7;
-  const int slot0 = (METHOD == 1) ? (threadIdx.x >> 3) : threadIdx.x;
-  // wave-uniform trip count: every lane of a wave stays while any has work
-  const int wave_first = (METHOD == 1) ? ((threadIdx.x & ~63) >> 3) : (threadIdx.x & ~63);
-  for (int e = first + slot0, ew = first + wave_first; ew < n; e += stride, ew += stride) {
-    const bool active = e < n;
+  constexpr int PER_UNIT = (METHOD == 1) ? ELV_PER_WAVE : 64;
+  const int nru = (nring + PER_UNIT - 1) / PER_UNIT;
+  const int nunits = nru + (nmix + PER_UNIT - 1) / PER_UNIT;
+  const int lane = threadIdx.x & 63;
+  const int cand = lane % 7, gbase = lane - cand;  // ELVIRA: candidate and its cell's first lane
+  const int slot = (METHOD == 1) ? lane / 7 : lane;
+  const bool lane_used = METHOD == 0 || lane < 63;
+  const int nwaves = gridDim.x * (TBLK / 64);
+  int u = blockIdx.x * (TBLK / 64) + (threadIdx.x >> 6);
+  while (u < nunits) {
+    const bool ring = u < nru;
     int i = 0, j = 0, si = 0, sj = 0;
     bool work = false, zero = false;
     double c = 0.0;
-    const bool ring = e >= nmix;
-    if (active) {
-      if (!ring) {
+    if (ring) {
+      const int r = PER_UNIT * u + slot;
+      if (lane_used && r < nring) {
+        ring_cell(m, r, i, j);
+        c = C[H(m, i, j)];
+        if (is_mixed(c)) {
+          work = true;
+          zero = ghost_zero(m, i, j);
+          si = wrap_or_clamp(i, m.nx, m.perx);
+          sj = wrap_or_clamp(j, m.ny, m.pery);
+        }
+      }
+    } else {
+      const int e = PER_UNIT * (u - nru) + slot;
+      if (lane_used && e < nmix) {
         int lo = 0, hi = ntiles;  // offs[lo] <= e < offs[hi]
         while (hi - lo > 1) {
           const int mid = (lo + hi) >> 1;
@@ -294,15 +343,6 @@ __global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, De
         sj = j;
         c = C[k];
         work = true;
-      } else {
-        ring_cell(m, e - nmix, i, j);
-        c = C[H(m, i, j)];
-        if (is_mixed(c)) {
-          work = true;
-          zero = ghost_zero(m, i, j);
-          si = wrap_or_clamp(i, m.nx, m.perx);
-          sj = wrap_or_clamp(j, m.ny, m.pery);
-        }
       }
     }
     // normal (and alpha for interior cells) of the source cell (si, sj)
@@ -314,45 +354,48 @@ __global__ void __launch_bounds__(TBLK) k_recon(const double *__restrict__ C, De
     } else {
       double ux = 1.0, uy = 0.0, al = 0.0, err = 0.0;
       bool valid = false;
-      if (src_mixed && lane8 < 7) elvira_candidate(C, m, si, sj, cs, lane8, ux, uy, al, err, valid);
-      double key = (valid && err < 1.0e30) ? err : __builtin_inf();
-      int idx = lane8;
+      if (src_mixed) elvira_candidate(C, m, si, sj, cs, cand, ux, uy, al, err, valid);
+      const double key = (valid && err < 1.0e30) ? err : __builtin_inf();
+      // strict '<' over the candidates in order: the first smallest wins
+      double best = __builtin_inf();
+      int win = gbase;
 #pragma unroll
-      for (int off = 1; off < 8; off <<= 1) {
-        const double ok = __shfl_xor(key, off, 8);
-        const int oi = __shfl_xor(idx, off, 8);
-        if (ok < key || (ok == key && oi < idx)) {
-          key = ok;
-          idx = oi;
+      for (int q = 0; q < 7; ++q) {
+        const double kq = __shfl(key, gbase + q);
+        if (kq < best) {
+          best = kq;
+          win = gbase + q;
         }
       }
-      bx = __shfl(ux, idx, 8);
-      by = __shfl(uy, idx, 8);
-      ba = __shfl(al, idx, 8);
-      if (key == __builtin_inf()) {
+      bx = __shfl(ux, win);
+      by = __shfl(uy, win);
+      ba = __shfl(al, win);
+      if (best == __builtin_inf()) {
         bx = 1.0;
         by = 0.0;
         ba = 0.0;
       }
-      if (lane8 != 0) work = false;
+      if (cand != 0) work = false;
     }
-    if (!work) continue;
-    if (!ring) {
-      const long k = H(m, i, j);
-      snx[k] = bx;
-      sny[k] = by;
-      sal[k] = ba;
-    } else {
-      double gx = bx, gy = by;
-      if (zero) {
-        gx = 0.0;
-        gy = 0.0;
-      } else if (!src_mixed) {
-        gx = 1.0;
-        gy = 0.0;
+    if (work) {
+      if (!ring) {
+        const long k = H(m, i, j);
+        snx[k] = bx;
+        sny[k] = by;
+        sal[k] = ba;
+      } else {
+        double gx = bx, gy = by;
+        if (zero) {
+          gx = 0.0;
+          gy = 0.0;
+        } else if (!src_mixed) {
+          gx = 1.0;
+          gy = 0.0;
+        }
+        ghost_segment(m, i, j, c, gx, gy, snx, sny, sal);
       }
-      ghost_segment(m, i, j, c, gx, gy, snx, sny, sal);
     }
+    u += nwaves;
   }
 }
 
@@ -769,19 +812,25 @@ void vof_free(Ctx &c) {
   hipFree(c.vof.adj);
 }
 
+// Workgroups of k_recon: a wave per unit of an interface with up to one
+// mixed cell in 28 (ELVIRA) or 4 (PLIC) plus the ring's units; waves past the
+// units return at once, and larger interfaces take further units in stride.
+static dim3 recon_grid(const Mesh &mesh, int method) {
+  const int per_unit = method == 1 ? ELV_PER_WAVE : 64;
+  const int L = 2 * (mesh.nx + 2) + 2 * mesh.ny;
+  const int waves = n_tiles(mesh) * (TBLK / 64) + (L + per_unit - 1) / per_unit;
+  return dim3((waves + TBLK / 64 - 1) / (TBLK / 64));
+}
+
 // Reconstruction of src: tile lists (left by the previous update or by
 // k_classify) plus the halo ring.
 static int launch_recon(Ctx &c, int method, const double *src) {
   const DevMesh m = dev_mesh(c.m);
   const int nt = n_tiles(c.m);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, c.stream, (const int *)c.vof.count, nt, c.vof.offs);
-  // one workgroup per tile plus the ring's share: a typical interface fills
-  // the grid's first slice; the blocks past the index space return at once
-  const int per_block = method == 1 ? TBLK / 8 : TBLK;
-  const int L = 2 * (c.m.nx + 2) + 2 * c.m.ny;
-  const dim3 grid(nt + (L + per_block - 1) / per_block);
-  launch_tagged(c, COVOF_KERNEL_VOF_RECON, method == 1 ? k_recon<1> : k_recon<0>, grid, dim3(TBLK), src, m,
-                (const int *)c.vof.list[0], (const int *)c.vof.offs, c.vof.snx, c.vof.sny, c.vof.sal);
+  launch_tagged(c, COVOF_KERNEL_VOF_RECON, method == 1 ? k_recon<1> : k_recon<0>, recon_grid(c.m, method),
+                dim3(TBLK), src, m, (const int *)c.vof.list[0], (const int *)c.vof.offs, c.vof.snx, c.vof.sny,
+                c.vof.sal);
   COVOF_CHECK_LAUNCH();
   return COVOF_OK;
 }
@@ -916,11 +965,8 @@ int vof_reconstruct_noalloc(const Mesh &mesh, int method, const double *C, doubl
   const DevMesh m = dev_mesh(mesh);
   hipLaunchKernelGGL(k_classify, dim3(nt), dim3(TBLK), 0, s, C, m, list, count);
   hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, (const int *)count, nt, offs);
-  const int per_block = method == 1 ? TBLK / 8 : TBLK;
-  const int L = 2 * (mesh.nx + 2) + 2 * mesh.ny;
-  const dim3 grid(nt + (L + per_block - 1) / per_block);
-  hipLaunchKernelGGL(method == 1 ? k_recon<1> : k_recon<0>, grid, dim3(TBLK), 0, s, C, m, (const int *)list,
-                     (const int *)offs, onx, ony, oal);
+  hipLaunchKernelGGL(method == 1 ? k_recon<1> : k_recon<0>, recon_grid(mesh, method), dim3(TBLK), 0, s, C, m,
+                     (const int *)list, (const int *)offs, onx, ony, oal);
   const dim3 blk(64, 4), grd((mesh.nx + 2 + 63) / 64, (mesh.ny + 2 + 3) / 4);
   hipLaunchKernelGGL(k_expand_segments, grd, blk, 0, s, C, m, (const double *)onx, (const double *)ony,
                      (const double *)oal, onx, ony, oal);
