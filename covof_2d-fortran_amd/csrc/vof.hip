@@ -192,6 +192,8 @@ __device__ void elvira_candidate(const double *C, const DevMesh &m, int i, int j
 // of its mixed cells (block-local compaction through LDS: no global atomics)
 // for the next reconstruction, which runs one block per tile.
 constexpr int TX = 64, TY = 16, TCELLS = TX * TY, TBLK = 256;
+constexpr int TRG = TBLK / TX, TROWS = TY / TRG;  // a thread's rows: its row group g, g + TRG, ...
+static_assert(TBLK % TX == 0 && TY % TRG == 0, "tile shape");
 // ELVIRA reconstruction: 9 cells of 7 candidate lanes per wave
 constexpr int ELV_PER_WAVE = 9;
 
@@ -406,9 +408,9 @@ __global__ void __launch_bounds__(TBLK) k_classify(const double *__restrict__ C,
   if (threadIdx.x == 0) s_count = 0;
   __syncthreads();
   const int tx = blockIdx.x % tiles_x(m), ty = blockIdx.x / tiles_x(m);
-  const int i = tx * TX + (threadIdx.x & 63) + 1;
-  for (int r = 0; r < TY / 4; ++r) {
-    const int j = ty * TY + (threadIdx.x >> 6) + 4 * r + 1;
+  const int i = tx * TX + (threadIdx.x % TX) + 1;
+  for (int r = 0; r < TROWS; ++r) {
+    const int j = ty * TY + (threadIdx.x / TX) + TRG * r + 1;
     const bool inside = i <= m.nx && j <= m.ny;
     const long k = inside ? H(m, i, j) : 0;
     lds_append(inside && is_mixed(C[k]), (int)k, s_list, &s_count);
@@ -628,16 +630,16 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
     s_flux[f] = b.pos ? a : -a;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int i = i0 + lane;
+  const int col = threadIdx.x % TX;
+  const int i = i0 + col;
   const double area = m.dx * m.dy;
   const double dth = dt / (DIR == 0 ? m.dx : m.dy);
-  constexpr int NRC = TY / 4;  // rows per thread
+  constexpr int NRC = TROWS;  // rows per thread
   double c0[NRC], vlo[NRC], vhi[NRC];
   bool cdo[NRC];
 #pragma unroll
   for (int r = 0; r < NRC; ++r) {  // loads of all rows first
-    const int j = j0 + (threadIdx.x >> 6) + 4 * r;
+    const int j = j0 + (threadIdx.x / TX) + TRG * r;
     c0[r] = vlo[r] = vhi[r] = 0.0;
     cdo[r] = false;
     if (i <= m.nx && j <= m.ny) {
@@ -650,14 +652,14 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   }
 #pragma unroll
   for (int r = 0; r < NRC; ++r) {
-    const int row = (threadIdx.x >> 6) + 4 * r;
+    const int row = (threadIdx.x / TX) + TRG * r;
     const int j = j0 + row;
     const bool inside = i <= m.nx && j <= m.ny;
     long k = 0;
     double cn = 0.0;
     if (inside) {
       k = H(m, i, j);
-      const int lo = row * FW + lane, hi = DIR == 0 ? lo + 1 : lo + FW;
+      const int lo = row * FW + col, hi = DIR == 0 ? lo + 1 : lo + FW;
       const double flo = s_flux[lo], fhi = s_flux[hi];
       const double du = vhi[r] - vlo[r];
       double cdv;

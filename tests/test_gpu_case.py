@@ -89,13 +89,20 @@ def test_json_case_matches_reference_program(cv, tmp_path, fixture):
                 worst = max(worst, abs(x - y) / max(abs(y), 1e-300))
     print(f"history: worst relative difference {worst:.2e}")
     assert worst <= 2e-6
-    # the last step's file: every array within 1e-9 of the reference's scale
+    # the files of steps 5, 10, 13: every array within 1e-9 of the reference's
+    # scale, except the solves' own outputs: the divergence is a solve
+    # residual (1e-7 stop), and the pressure is the Poisson block's iterate
+    # at that stop, where the two summation orders leave differences of a
+    # few 1e-9 (the reference-order run above is identical to the byte)
+    worst = {}
     for step in (5, 10, 13):
         A = _arrays((out / f"bubble_p0_{step:06d}.vtr").read_text())
         B = _arrays(str(g[f"vtr_{step}"]))
         assert A.keys() == B.keys()
         for k in B:
             scale = max(np.abs(B[k]).max(), 1e-300)
-            tol = 1e-5 if k == "Divergence" else 1e-9  # the divergence is a solve residual (1e-7 stop)
-            err = np.abs(A[k] - B[k]).max() / scale
-            assert err <= tol, (step, k, err)
+            worst[k] = max(worst.get(k, 0.0), np.abs(A[k] - B[k]).max() / scale)
+    print("fields: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+    for k, err in worst.items():
+        tol = {"Divergence": 1e-5, "Pressure": 1e-8}.get(k, 1e-9)
+        assert err <= tol, (k, err)
