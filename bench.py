@@ -254,9 +254,13 @@ def main():
 
     # ------------------------------------------- multigrid-preconditioned CG
     # the same system with the library's V-cycle preconditioner (no reference
-    # counterpart), FP64 and mixed, timed the same way
+    # counterpart), timed the same way: FP64; mixed (the default form: FP64 CG
+    # whose V-cycle runs in FP32); and the FP32 Krylov recurrence with FP64
+    # reliable updates (the form of the Jacobi mixed solve)
     mg = {}
-    for name, prec in (("fp64", cv.PRECISION_FP64), ("mixed", cv.PRECISION_MIXED)):
+    for name, prec, form in (("fp64", cv.PRECISION_FP64, 0), ("mixed", cv.PRECISION_MIXED, 0),
+                             ("mixed_fp32_krylov", cv.PRECISION_MIXED, 1)):
+        ctx.set_solver_option(cv.OPT_MIXED_MG_FORM, form)
         x = ctx.new_work()
         ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=tol, max_iter=20000, precision=prec)  # warm-up
         torch.cuda.synchronize()
@@ -270,6 +274,8 @@ def main():
         assert rmg.converged and rmg.relres <= tol, rmg
         mg[name] = {"s": dt_mg, "iters": rmg.iters, "relres": rmg.relres}
         log(f"MG-PCG {name}: {rmg.iters} it, {dt_mg * 1e3:.1f} ms")
+    ctx.set_solver_option(cv.OPT_MIXED_MG_FORM, 0)
+    mg["speedup_mixed_mg_vs_fp64_mg"] = mg["fp64"]["s"] / mg["mixed"]["s"]
     mg["speedup_mixed_mg_vs_fp64_jacobi"] = t64 / mg["mixed"]["s"]
     mg["speedup_fp64_mg_vs_fp64_jacobi"] = t64 / mg["fp64"]["s"]
 
@@ -429,7 +435,8 @@ def main():
         return (f"two-phase standard-projection step {sx}x{sy} (2 x 1 box), {nbub} seeded bubbles (radii 8-32 "
                 f"cells, exact volume fractions), rho 1000/1, mu 0.1/0.001, sigma 0.07 (CSF, curvature "
                 f"{curv.replace('_', '-')}), g 9.81, BDF2, TVD superbee, PLIC + WY, dt 1e-4, {nst} steps; "
-                "predictor BiCGSTAB(2) + Jacobi to 1e-10; Poisson " + ("mixed-precision" if prec == cv.PRECISION_MIXED
+                "predictor BiCGSTAB(2) + Jacobi to 1e-10; Poisson " + ("mixed-precision (FP64 CG, FP32 V-cycle)"
+                                                             if prec == cv.PRECISION_MIXED
                                                              else "FP64") + " MG-PCG to 1e-10")
 
     dts = 1.0e-4

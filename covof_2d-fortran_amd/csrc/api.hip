@@ -229,6 +229,14 @@ extern "C" int covof_ctx_set_solver_option(covof_ctx *h, int option, double valu
       if (!(value > 0.0 && value <= 2.0)) return fail(COVOF_BAD_ARGUMENT, "MG_OMEGA must be in (0, 2]");
       o.mg_omega = value;
       break;
+    case COVOF_OPT_MG_FUSED:
+      if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MG_FUSED must be 0 or 1");
+      o.mg_fused = (int)value;
+      break;
+    case COVOF_OPT_MIXED_MG_FORM:
+      if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MIXED_MG_FORM must be 0 or 1");
+      o.mixed_mg = (int)value;
+      break;
     default:
       return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_solver_option: unknown option");
   }

@@ -133,6 +133,9 @@ struct Poisson {
   // the last mixed-precision solve: iterations of the FP32 recurrence and of
   // the FP64 continuation (0 when it did not run)
   int last_it32 = 0, last_it64 = 0;
+  // set while an FP64 CG runs with the multigrid cycle in FP32 (the MIXED
+  // multigrid solve's default form)
+  int mg_fp32_cycle = 0;
   Mg mg64, mg32;
   int nblocks = 0;          // reduction grid size
   PGraph graph[2];          // FP64 cycle, mixed cycle
@@ -180,6 +183,8 @@ struct SolverOpts {
   int mg_coarse = 0;     // 1: series coarse faces (else averaged)
   int mg_sweeps = 1;     // red-black sweeps before and after the coarse correction
   double mg_omega = 1.8; // weight of the coarse-grid correction
+  int mg_fused = 1;      // one sweep: fused level passes (k_mg_pre / k_mg_post)
+  int mixed_mg = 0;      // MIXED with MG: 0 FP64 CG + FP32 V-cycle, 1 FP32 Krylov + reliable updates
 };
 
 struct Ctx {
@@ -261,6 +266,8 @@ int vof_adjoint_reserve(Ctx &c);
 // z = M^-1 r; f0, f1: the scalar slots of the solve's stop flags
 void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1);
 void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1);
+// z = M^-1 r of an FP64 solve with the V-cycle in FP32 (r rounded, z widened)
+void mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1);
 void mg_free(Ctx &c);
 
 int vof_alloc(Ctx &c);

@@ -21,6 +21,7 @@
 // launches of a captured cycle after convergence cost only their dispatch.
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "covof_internal.h"
 
@@ -244,30 +245,306 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth(Lev<T> F, int nxc, co
   F.x[k] = s / F.diag[k];
 }
 
+// ---------------------------------------------------------------- fused level
+//
+// With one red-black sweep before and after the coarse correction (the
+// default), a level's five launches above fold into two, each one pass over
+// the level's arrays:
+//  * k_mg_pre: the pre-smoothing from x = 0 leaves red = b / diag and black
+//    = the Gauss-Seidel value from those reds; the kernel forms both on a
+//    tile with a two-cell halo in LDS, then the residual of the tile's cells
+//    and its restriction.  It writes only the coarse right-hand side: the
+//    post-smoothing needs no pre-smoothed value but the reds, which it forms
+//    again from b / diag.
+//  * k_mg_post: red + P x_c from b / diag, the black sweep from those, the
+//    red sweep from the new blacks; writes the level's x.
+// Every value is the expression of the kernels above on the same operands
+// (same terms, same order), so the cycle is bitwise the five-launch form.
+// Tile: 64 x 16 fine cells (32 x 8 coarse), 256 threads; region = tile + 2
+// (the blacks next to the tile need the reds two cells out); a periodic side
+// wraps the region's cells (even sizes: the colours agree across the wrap).
+constexpr int FT_X = 64, FT_Y = 16, FT_H = 2;
+constexpr int FR_X = FT_X + 2 * FT_H, FR_Y = FT_Y + 2 * FT_H;
+constexpr int FR_N = FR_X * FR_Y;
+constexpr int F1_X = FR_X - 2, F1_N = (FR_X - 2) * (FR_Y - 2);  // region minus its outer ring
+constexpr int FT_BLK = 256;
+static_assert((FT_X / 2) * (FT_Y / 2) == FT_BLK, "one thread per coarse cell of the tile");
+
+__device__ __forceinline__ int wrap1(int g, int n) {
+  int q = (g - 1) % n;
+  return (q < 0 ? q + n : q) + 1;
+}
+
+// Tile of block b: consecutive blocks go to the eight XCDs in turn, so XCD x
+// takes the contiguous run of tiles [start_x, start_x + count_x): neighbour
+// tiles, whose cells the halos re-read, mostly share that XCD's L2.
+__device__ __forceinline__ int mg_xcd_tile(int b, int nt) {
+  const int q = nt >> 3, rem = nt & 7, x = b & 7;
+  return x * q + min(x, rem) + (b >> 3);
+}
+
+// Global cell of region cell (a, c) of the tile whose first region cell is
+// (i0, j0); false: outside a non-periodic side.
+__device__ __forceinline__ bool region_cell(int nx, int ny, int perx, int pery, int i0, int j0, int a, int c,
+                                            int &gi, int &gj) {
+  gi = i0 + a;
+  gj = j0 + c;
+  bool ok = true;
+  if (gi < 1 || gi > nx) {
+    if (perx) gi = wrap1(gi, nx);
+    else ok = false;
+  }
+  if (gj < 1 || gj > ny) {
+    if (pery) gj = wrap1(gj, ny);
+    else ok = false;
+  }
+  return ok;
+}
+
+// The four face coefficients of cell (gi, gj): west, east, south, north.
+template <typename T>
+struct Faces {
+  T w, e, s, n;
+};
+template <typename T>
+__device__ __forceinline__ Faces<T> load_faces(const Lev<T> &L, int gi, int gj) {
+  const long k = (long)(gj - 1) * L.nx + (gi - 1);
+  const T *kxr = L.kx + (long)(gj - 1) * L.kxp;
+  return Faces<T>{kxr[gi - 1], kxr[gi], L.ky[k], L.ky[k + L.nx]};
+}
+
+// Gauss-Seidel value of the region's cell (a, c) = global (gi, gj) from the
+// LDS x of its neighbours (gs_value's terms and order).
+template <typename T>
+__device__ __forceinline__ T gs_lds(const Lev<T> &L, int gi, int gj, const Faces<T> &f, T (*xs)[FR_X], T bv, T dv,
+                                    int a, int c) {
+  T s = bv;
+  if (gi > 1 || L.perx) s = s + f.w * xs[c][a - 1];
+  if (gi < L.nx || L.perx) s = s + f.e * xs[c][a + 1];
+  if (gj > 1 || L.pery) s = s + f.s * xs[c - 1][a];
+  if (gj < L.ny || L.pery) s = s + f.n * xs[c + 1][a];
+  return s / dv;
+}
+
+// Per thread: NR_1 region cells (b, diag), NR_2 cells of the region minus its
+// outer ring (faces of its blacks), NR_3 tile cells (faces).  Every global
+// load of a block is issued before its first barrier.
+constexpr int NR_1 = (FR_N + FT_BLK - 1) / FT_BLK;
+constexpr int NR_2 = (F1_N + FT_BLK - 1) / FT_BLK;
+constexpr int NR_3 = FT_X * FT_Y / FT_BLK;
+
+// The loads of a fused level pass.  POST: the region's reds also load their
+// coarse cell's x_c.
+template <typename T, bool POST>
+struct TileLoads {
+  T b[NR_1], d[NR_1], xc[NR_1];
+  Faces<T> f2[NR_2], f3[NR_3];
+  // bin: the level's right-hand side (of type TB: FP64 for the FP32 cycle
+  // that preconditions an FP64 solve, rounded to T on load)
+  template <typename TB>
+  __device__ __forceinline__ void issue(const Lev<T> &F, const TB *__restrict__ bin, int i0, int j0, int nxc,
+                                        const T *__restrict__ xcv) {
+    const int nx = F.nx, ny = F.ny;
+#pragma unroll
+    for (int q = 0; q < NR_1; ++q) {
+      const int e = threadIdx.x + q * FT_BLK;
+      int gi, gj;
+      b[q] = T(0);
+      d[q] = T(1);
+      xc[q] = T(0);
+      if (e < FR_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % FR_X, e / FR_X, gi, gj)) {
+        const long k = (long)(gj - 1) * nx + (gi - 1);
+        b[q] = (T)bin[k];
+        d[q] = F.diag[k];
+        if (POST && ((gi + gj) & 1) == 0) xc[q] = xcv[(long)((gj + 1) / 2 - 1) * nxc + ((gi + 1) / 2 - 1)];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NR_2; ++q) {
+      const int e = threadIdx.x + q * FT_BLK;
+      int gi, gj;
+      f2[q] = Faces<T>{T(0), T(0), T(0), T(0)};
+      if (e < F1_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % F1_X + 1, e / F1_X + 1, gi, gj) &&
+          ((gi + gj) & 1) == 1)
+        f2[q] = load_faces(F, gi, gj);
+    }
+#pragma unroll
+    for (int q = 0; q < NR_3; ++q) {
+      const int e = threadIdx.x + q * FT_BLK;
+      const int gi = i0 + e % FT_X + FT_H, gj = j0 + e / FT_X + FT_H;
+      f3[q] = Faces<T>{T(0), T(0), T(0), T(0)};
+      if (gi <= nx && gj <= ny) f3[q] = load_faces(F, gi, gj);
+    }
+  }
+};
+
+// The blacks of the region minus its outer ring from the reds in LDS.
+template <typename T, bool POST>
+__device__ __forceinline__ void black_sweep(const Lev<T> &F, int i0, int j0, const TileLoads<T, POST> &ld,
+                                            T (*xs)[FR_X], T (*bs)[FR_X], T (*ds)[FR_X]) {
+#pragma unroll
+  for (int q = 0; q < NR_2; ++q) {
+    const int e = threadIdx.x + q * FT_BLK;
+    const int a = e % F1_X + 1, c = e / F1_X + 1;
+    int gi, gj;
+    if (e < F1_N && region_cell(F.nx, F.ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 1)
+      xs[c][a] = gs_lds(F, gi, gj, ld.f2[q], xs, bs[c][a], ds[c][a], a, c);
+  }
+}
+
+template <typename T, typename TB>
+__global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restrict__ bin, int nxc, int nyc,
+                                                   T *__restrict__ bc, T scale, const double *scal, int f0, int f1) {
+  if (stopped(scal, f0, f1)) return;
+  __shared__ T xs[FR_Y][FR_X];
+  __shared__ T bs[FR_Y][FR_X];
+  __shared__ T ds[FR_Y][FR_X];
+  __shared__ T rs[FT_Y][FT_X];
+  const int nx = F.nx, ny = F.ny;
+  const int ntx = (nx + FT_X - 1) / FT_X;
+  const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = t / ntx;
+  const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;  // global cell of region cell (0, 0)
+  TileLoads<T, false> ld;
+  ld.issue(F, bin, i0, j0, nxc, nullptr);
+  // the region's b, diag; reds = b / diag (the first half-sweep from x = 0)
+#pragma unroll
+  for (int q = 0; q < NR_1; ++q) {
+    const int e = threadIdx.x + q * FT_BLK;
+    if (e >= FR_N) break;
+    const int a = e % FR_X, c = e / FR_X;
+    int gi, gj;
+    T xv = T(0);
+    if (region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 0) xv = ld.b[q] / ld.d[q];
+    bs[c][a] = ld.b[q];
+    ds[c][a] = ld.d[q];
+    xs[c][a] = xv;
+  }
+  __syncthreads();
+  black_sweep(F, i0, j0, ld, xs, bs, ds);
+  __syncthreads();
+  // residual b - A x of the tile's cells (k_mg_restrict's terms and order)
+#pragma unroll
+  for (int q = 0; q < NR_3; ++q) {
+    const int e = threadIdx.x + q * FT_BLK;
+    const int a = e % FT_X + FT_H, c = e / FT_X + FT_H;
+    const int gi = i0 + a, gj = j0 + c;
+    if (gi > nx || gj > ny) continue;
+    const Faces<T> &f = ld.f3[q];
+    T ax = ds[c][a] * xs[c][a];
+    if (gi > 1 || F.perx) ax = ax - f.w * xs[c][a - 1];
+    if (gi < nx || F.perx) ax = ax - f.e * xs[c][a + 1];
+    if (gj > 1 || F.pery) ax = ax - f.s * xs[c - 1][a];
+    if (gj < ny || F.pery) ax = ax - f.n * xs[c + 1][a];
+    rs[c - FT_H][a - FT_H] = bs[c][a] - ax;
+  }
+  __syncthreads();
+  // restriction: b_c(I, J) = scale * (sum of the children's residuals)
+  const int ci = threadIdx.x % (FT_X / 2), cj = threadIdx.x / (FT_X / 2);
+  const int I = tx * (FT_X / 2) + ci + 1, J = ty * (FT_Y / 2) + cj + 1;
+  if (I > nxc || J > nyc) return;
+  T s = T(0);
+  for (int dj = 0; dj < 2; ++dj)
+    for (int di = 0; di < 2; ++di) {
+      if (2 * I - 1 + di > nx || 2 * J - 1 + dj > ny) continue;
+      s = s + rs[2 * cj + dj][2 * ci + di];
+    }
+  bc[(long)(J - 1) * nxc + (I - 1)] = scale * s;
+}
+
+template <typename T, typename TB, typename TX>
+__global__ void __launch_bounds__(FT_BLK) k_mg_post(Lev<T> F, const TB *__restrict__ bin, TX *__restrict__ xout,
+                                                    int nxc, const T *__restrict__ xc, const double *scal, int f0,
+                                                    int f1) {
+  if (stopped(scal, f0, f1)) return;
+  __shared__ T xs[FR_Y][FR_X];
+  __shared__ T bs[FR_Y][FR_X];
+  __shared__ T ds[FR_Y][FR_X];
+  const int nx = F.nx, ny = F.ny;
+  const int ntx = (nx + FT_X - 1) / FT_X;
+  const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = t / ntx;
+  const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;
+  TileLoads<T, true> ld;
+  ld.issue(F, bin, i0, j0, nxc, xc);
+  // reds corrected by the coarse level: the pre-smoothed red b / diag + x_c
+#pragma unroll
+  for (int q = 0; q < NR_1; ++q) {
+    const int e = threadIdx.x + q * FT_BLK;
+    if (e >= FR_N) break;
+    const int a = e % FR_X, c = e / FR_X;
+    int gi, gj;
+    T xv = T(0);
+    if (region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 0)
+      xv = ld.b[q] / ld.d[q] + ld.xc[q];
+    bs[c][a] = ld.b[q];
+    ds[c][a] = ld.d[q];
+    xs[c][a] = xv;
+  }
+  __syncthreads();
+  // black half-sweep from the corrected reds (k_mg_prolong_smooth)
+  black_sweep(F, i0, j0, ld, xs, bs, ds);
+  __syncthreads();
+  // red half-sweep from the new blacks; the tile's x
+#pragma unroll
+  for (int q = 0; q < NR_3; ++q) {
+    const int e = threadIdx.x + q * FT_BLK;
+    const int a = e % FT_X + FT_H, c = e / FT_X + FT_H;
+    const int gi = i0 + a, gj = j0 + c;
+    if (gi > nx || gj > ny) continue;
+    const T v = (((gi + gj) & 1) == 0) ? gs_lds(F, gi, gj, ld.f3[q], xs, bs[c][a], ds[c][a], a, c) : xs[c][a];
+    xout[(long)(gj - 1) * nx + (gi - 1)] = (TX)v;
+  }
+}
+
+inline unsigned fused_tiles(int nx, int ny) {
+  return (unsigned)(((nx + FT_X - 1) / FT_X) * ((ny + FT_Y - 1) / FT_Y));
+}
+
 // The bottom of the V-cycle in one workgroup: every level from the first one
 // with at most MG_BOTTOM_CELLS cells down to the coarsest, with x and the
-// right-hand sides in LDS (the face coefficients are read from global memory,
-// where they stay cached).  Same cycle as the multi-block levels: NU
+// right-hand sides in LDS.  Same cycle as the multi-block levels: NU
 // red-black sweeps from x = 0, restriction, ..., K red-black sweeps and K in
 // the adjoint order on the coarsest level, then prolongation and NU sweeps in
 // the adjoint order.  Within a half-sweep every update reads the state before
 // the half-sweep (values are staged in registers across a barrier), so odd
 // periodic sizes (same-colour wrap neighbours) stay deterministic.
+// The levels from the first one of at most MG_WAVE_CELLS cells down are
+// cycled by the first wave alone, with their face coefficients in LDS (loaded
+// by the whole workgroup at the start) and wave-level ordering in place of
+// the workgroup barriers: the coarsest level's 4K half-sweeps then cost no
+// barrier.  Same operations in the same order either way.
 constexpr int MG_BOTTOM_CELLS = 4096;
 constexpr int MG_BOTTOM_LDS = 6144;  // cells of all bottom levels together
+// largest level above the coarsest that the bottom takes (build-time tunable)
+#ifndef MG_BOTTOM_TAKE
+#define MG_BOTTOM_TAKE 4096
+#endif
+constexpr int MG_WAVE_CELLS = 256;   // a wave level: at most 4 cells per lane
+constexpr int MG_WAVE_LDS = 512;     // cells of all wave levels together
 
 struct BotLev {
   int nx, ny;
   long kxp;
   const void *kx, *ky, *diag;
-  int off;  // offset of the level's cells in the LDS arrays
+  int off;   // offset of the level's cells in the LDS arrays
+  int woff;  // wave levels: offset of its coefficients in the LDS coefficient arrays
 };
 struct Bottom {
   int nlev;  // levels handled here
+  int lw;    // first wave level (nlev: none)
   BotLev lev[MG_MAX_LEVELS];
   int perx, pery, K, NU;
   double scale;  // restriction weight (R = scale * P^T)
 };
+
+// Ordering of the first wave's LDS accesses across its lanes (LDS operations
+// of one wave complete in issue order; the fences keep the compiler's order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename T>
 __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restrict__ b_in, T *__restrict__ x_out,
@@ -275,10 +552,27 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   if (stopped(scal, f0, f1)) return;
   __shared__ T xs[MG_BOTTOM_LDS];
   __shared__ T bs[MG_BOTTOM_LDS];
+  __shared__ T wc[5][MG_WAVE_LDS];  // west, east, south, north, diagonal of the wave levels
+  __shared__ T rsd[MG_BOTTOM_CELLS];  // residuals of a workgroup level, for its restriction
   const int tid = threadIdx.x;
+  const int last = B.nlev - 1, lw = B.lw;
   {
     const int n0 = B.lev[0].nx * B.lev[0].ny;
     for (int t = tid; t < n0; t += 1024) bs[t] = b_in[t];
+    for (int l = lw; l <= last; ++l) {
+      const BotLev &L = B.lev[l];
+      const int nx = L.nx, n = nx * L.ny;
+      const T *kx = (const T *)L.kx, *ky = (const T *)L.ky, *dg = (const T *)L.diag;
+      for (int t = tid; t < n; t += 1024) {
+        const int i = t % nx + 1, j = t / nx + 1;
+        const T *kxr = kx + (long)(j - 1) * L.kxp;
+        wc[0][L.woff + t] = kxr[i - 1];
+        wc[1][L.woff + t] = kxr[i];
+        wc[2][L.woff + t] = ky[t];
+        wc[3][L.woff + t] = ky[t + nx];
+        wc[4][L.woff + t] = dg[t];
+      }
+    }
   }
   // coefficients of this thread's cells (at most 4 per level) in registers,
   // loaded once per visit of a level: a half-sweep then reads only LDS
@@ -302,9 +596,55 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       cd[q] = dg[t];
     }
   };
+  // Gauss-Seidel value of cell t = (i, j) of a level from x (LDS)
+  auto gs = [&](const BotLev &L, const T *x, T bv, T w, T e, T so, T no, T d, int t, int i, int j) {
+    const int nx = L.nx, ny = L.ny;
+    T s = bv;
+    if (i > 1) s = s + w * x[t - 1];
+    else if (B.perx) s = s + w * x[t + (nx - 1)];
+    if (i < nx) s = s + e * x[t + 1];
+    else if (B.perx) s = s + e * x[t - (nx - 1)];
+    if (j > 1) s = s + so * x[t - nx];
+    else if (B.pery) s = s + so * x[t + (ny - 1) * nx];
+    if (j < ny) s = s + no * x[t + nx];
+    else if (B.pery) s = s + no * x[t - (ny - 1) * nx];
+    return s / d;
+  };
+  // coarse right-hand side of cell t of level l + 1: scale * sum of the
+  // children's residuals b - A x; coef(k, q) gives face q of child k
+  auto restrict_cell = [&](int l, int t, auto coef) {
+    const BotLev &F = B.lev[l], &C = B.lev[l + 1];
+    const T *x = xs + F.off, *b = bs + F.off;
+    const int nx = F.nx, ny = F.ny;
+    const int I = t % C.nx + 1, J = t / C.nx + 1;
+    T sum = T(0);
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) {
+        const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
+        if (i > nx || j > ny) continue;
+        const int k = (j - 1) * nx + (i - 1);
+        T ax = coef(k, i, j, 4) * x[k];
+        if (i > 1) ax = ax - coef(k, i, j, 0) * x[k - 1];
+        else if (B.perx) ax = ax - coef(k, i, j, 0) * x[k + (nx - 1)];
+        if (i < nx) ax = ax - coef(k, i, j, 1) * x[k + 1];
+        else if (B.perx) ax = ax - coef(k, i, j, 1) * x[k - (nx - 1)];
+        if (j > 1) ax = ax - coef(k, i, j, 2) * x[k - nx];
+        else if (B.pery) ax = ax - coef(k, i, j, 2) * x[k + (ny - 1) * nx];
+        if (j < ny) ax = ax - coef(k, i, j, 3) * x[k + nx];
+        else if (B.pery) ax = ax - coef(k, i, j, 3) * x[k - (ny - 1) * nx];
+        sum = sum + (b[k] - ax);
+      }
+    bs[C.off + t] = (T)B.scale * sum;
+  };
+  auto prolong_cell = [&](int l, int t) {
+    const BotLev &F = B.lev[l], &C = B.lev[l + 1];
+    const int i = t % F.nx + 1, j = t / F.nx + 1;
+    xs[F.off + t] += xs[C.off + ((j + 1) / 2 - 1) * C.nx + ((i + 1) / 2 - 1)];
+  };
+  // ---- the workgroup's levels
   auto half = [&](int l, int color) {
     const BotLev &L = B.lev[l];
-    const int nx = L.nx, ny = L.ny, n = nx * ny;
+    const int nx = L.nx, n = nx * L.ny;
     T *x = xs + L.off;
     const T *b = bs + L.off;
     T v[4];
@@ -315,16 +655,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       if (t >= n) continue;
       const int i = t % nx + 1, j = t / nx + 1;
       if (((i + j) & 1) != color) continue;
-      T s = b[t];
-      if (i > 1) s = s + cw[q] * x[t - 1];
-      else if (B.perx) s = s + cw[q] * x[t + (nx - 1)];
-      if (i < nx) s = s + ce[q] * x[t + 1];
-      else if (B.perx) s = s + ce[q] * x[t - (nx - 1)];
-      if (j > 1) s = s + cs[q] * x[t - nx];
-      else if (B.pery) s = s + cs[q] * x[t + (ny - 1) * nx];
-      if (j < ny) s = s + cn[q] * x[t + nx];
-      else if (B.pery) s = s + cn[q] * x[t - (ny - 1) * nx];
-      v[q] = s / cd[q];
+      v[q] = gs(L, x, b[t], cw[q], ce[q], cs[q], cn[q], cd[q], t, i, j);
     }
     __syncthreads();
 #pragma unroll
@@ -341,60 +672,183 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
     for (int t = tid; t < n; t += 1024) xs[B.lev[l].off + t] = T(0);
     __syncthreads();
   };
-  const int last = B.nlev - 1;
-  for (int l = 0; l < last; ++l) {
+  const int lblock = lw < last ? lw : last;  // levels [0, lblock) go down and up with the workgroup
+  for (int l = 0; l < lblock; ++l) {
     zero(l);
     load_coef(l);
     for (int s = 0; s < B.NU; ++s) {
       half(l, 0);
       half(l, 1);
     }
-    // restriction of the residual into the next level's right-hand side
+    // restriction: the residuals of the level's cells from the coefficients
+    // in registers (restrict_cell's terms and order), then the children's sums
     const BotLev &F = B.lev[l], &C = B.lev[l + 1];
-    const T *kx = (const T *)F.kx, *ky = (const T *)F.ky, *dg = (const T *)F.diag;
-    const T *x = xs + F.off, *b = bs + F.off;
-    const int nx = F.nx, ny = F.ny, nc = C.nx * C.ny;
-    for (int t = tid; t < nc; t += 1024) {
-      const int I = t % C.nx + 1, J = t / C.nx + 1;
-      T sum = T(0);
-      for (int dj = 0; dj < 2; ++dj)
-        for (int di = 0; di < 2; ++di) {
-          const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
-          if (i > nx || j > ny) continue;
-          const int k = (j - 1) * nx + (i - 1);
-          const T *kxr = kx + (long)(j - 1) * F.kxp;
-          T ax = dg[k] * x[k];
-          if (i > 1) ax = ax - kxr[i - 1] * x[k - 1];
-          else if (B.perx) ax = ax - kxr[0] * x[k + (nx - 1)];
-          if (i < nx) ax = ax - kxr[i] * x[k + 1];
-          else if (B.perx) ax = ax - kxr[nx] * x[k - (nx - 1)];
-          if (j > 1) ax = ax - ky[k] * x[k - nx];
-          else if (B.pery) ax = ax - ky[k] * x[k + (ny - 1) * nx];
-          if (j < ny) ax = ax - ky[k + nx] * x[k + nx];
-          else if (B.pery) ax = ax - ky[k + nx] * x[k - (ny - 1) * nx];
-          sum = sum + (b[k] - ax);
+    {
+      const int nx = F.nx, ny = F.ny, n = nx * ny;
+      const T *x = xs + F.off, *b = bs + F.off;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = tid + q * 1024;
+        if (t >= n) continue;
+        const int i = t % nx + 1, j = t / nx + 1;
+        T ax = cd[q] * x[t];
+        if (i > 1) ax = ax - cw[q] * x[t - 1];
+        else if (B.perx) ax = ax - cw[q] * x[t + (nx - 1)];
+        if (i < nx) ax = ax - ce[q] * x[t + 1];
+        else if (B.perx) ax = ax - ce[q] * x[t - (nx - 1)];
+        if (j > 1) ax = ax - cs[q] * x[t - nx];
+        else if (B.pery) ax = ax - cs[q] * x[t + (ny - 1) * nx];
+        if (j < ny) ax = ax - cn[q] * x[t + nx];
+        else if (B.pery) ax = ax - cn[q] * x[t - (ny - 1) * nx];
+        rsd[t] = b[t] - ax;
+      }
+      __syncthreads();
+      const int nc = C.nx * C.ny;
+      for (int t = tid; t < nc; t += 1024) {
+        const int I = t % C.nx + 1, J = t / C.nx + 1;
+        T sum = T(0);
+        for (int dj = 0; dj < 2; ++dj)
+          for (int di = 0; di < 2; ++di) {
+            const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
+            if (i > nx || j > ny) continue;
+            sum = sum + rsd[(j - 1) * nx + (i - 1)];
+          }
+        bs[C.off + t] = (T)B.scale * sum;
+      }
+      __syncthreads();
+    }
+  }
+  if (lw > last) {
+    // no wave level: the coarsest level with the workgroup
+    zero(last);
+    load_coef(last);
+    for (int s = 0; s < B.K; ++s) {
+      half(last, 0);
+      half(last, 1);
+    }
+    for (int s = 0; s < B.K; ++s) {
+      half(last, 1);
+      half(last, 0);
+    }
+  } else if (tid < 64) {
+    // ---- the wave levels, first wave only
+    const int lane = tid;
+    auto half_w = [&](int l, int color) {
+      const BotLev &L = B.lev[l];
+      const int nx = L.nx, n = nx * L.ny, o = L.woff;
+      T *x = xs + L.off;
+      const T *b = bs + L.off;
+      T v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = lane + q * 64;
+        v[q] = T(0);
+        if (t >= n) continue;
+        const int i = t % nx + 1, j = t / nx + 1;
+        if (((i + j) & 1) != color) continue;
+        v[q] = gs(L, x, b[t], wc[0][o + t], wc[1][o + t], wc[2][o + t], wc[3][o + t], wc[4][o + t], t, i, j);
+      }
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t = lane + q * 64;
+        if (t >= n) continue;
+        const int i = t % nx + 1, j = t / nx + 1;
+        if (((i + j) & 1) == color) x[t] = v[q];
+      }
+      wave_sync();
+    };
+    auto zero_w = [&](int l) {
+      const int n = B.lev[l].nx * B.lev[l].ny;
+      for (int t = lane; t < n; t += 64) xs[B.lev[l].off + t] = T(0);
+      wave_sync();
+    };
+    for (int l = lw; l < last; ++l) {
+      zero_w(l);
+      for (int s = 0; s < B.NU; ++s) {
+        half_w(l, 0);
+        half_w(l, 1);
+      }
+      const int o = B.lev[l].woff;
+      auto lcoef = [&](int k, int, int, int q) -> T { return wc[q][o + k]; };
+      const int nc = B.lev[l + 1].nx * B.lev[l + 1].ny;
+      for (int t = lane; t < nc; t += 64) restrict_cell(l, t, lcoef);
+      wave_sync();
+    }
+    if (B.lev[last].nx * B.lev[last].ny <= 64) {
+      // the coarsest level, one cell per lane: its operands and neighbour
+      // indices (-1: no term) in registers for the 4K half-sweeps
+      const BotLev &L = B.lev[last];
+      const int nx = L.nx, ny = L.ny, n = nx * ny, o = L.woff;
+      T *x = xs + L.off;
+      const int t = lane;
+      const bool live = t < n;
+      const int i = t % nx + 1, j = t / nx + 1;
+      const int col = (i + j) & 1;
+      int iw = -1, ie = -1, is = -1, in = -1;
+      T bv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), d = T(1);
+      if (live) {
+        iw = i > 1 ? t - 1 : (B.perx ? t + (nx - 1) : -1);
+        ie = i < nx ? t + 1 : (B.perx ? t - (nx - 1) : -1);
+        is = j > 1 ? t - nx : (B.pery ? t + (ny - 1) * nx : -1);
+        in = j < ny ? t + nx : (B.pery ? t - (ny - 1) * nx : -1);
+        bv = bs[L.off + t];
+        w = wc[0][o + t];
+        e = wc[1][o + t];
+        so = wc[2][o + t];
+        no = wc[3][o + t];
+        d = wc[4][o + t];
+        x[t] = T(0);
+      }
+      wave_sync();
+      auto half_c = [&](int color) {
+        T v = T(0);
+        const bool upd = live && col == color;
+        if (upd) {
+          T s = bv;
+          if (iw >= 0) s = s + w * x[iw];
+          if (ie >= 0) s = s + e * x[ie];
+          if (is >= 0) s = s + so * x[is];
+          if (in >= 0) s = s + no * x[in];
+          v = s / d;
         }
-      bs[C.off + t] = (T)B.scale * sum;
+        wave_sync();
+        if (upd) x[t] = v;
+        wave_sync();
+      };
+      for (int s = 0; s < B.K; ++s) {
+        half_c(0);
+        half_c(1);
+      }
+      for (int s = 0; s < B.K; ++s) {
+        half_c(1);
+        half_c(0);
+      }
+    } else {
+      zero_w(last);
+      for (int s = 0; s < B.K; ++s) {
+        half_w(last, 0);
+        half_w(last, 1);
+      }
+      for (int s = 0; s < B.K; ++s) {
+        half_w(last, 1);
+        half_w(last, 0);
+      }
     }
-    __syncthreads();
-  }
-  zero(last);
-  load_coef(last);
-  for (int s = 0; s < B.K; ++s) {
-    half(last, 0);
-    half(last, 1);
-  }
-  for (int s = 0; s < B.K; ++s) {
-    half(last, 1);
-    half(last, 0);
-  }
-  for (int l = last - 1; l >= 0; --l) {
-    const BotLev &F = B.lev[l], &C = B.lev[l + 1];
-    const int n = F.nx * F.ny;
-    for (int t = tid; t < n; t += 1024) {
-      const int i = t % F.nx + 1, j = t / F.nx + 1;
-      xs[F.off + t] += xs[C.off + ((j + 1) / 2 - 1) * C.nx + ((i + 1) / 2 - 1)];
+    for (int l = last - 1; l >= lw; --l) {
+      const int n = B.lev[l].nx * B.lev[l].ny;
+      for (int t = lane; t < n; t += 64) prolong_cell(l, t);
+      wave_sync();
+      for (int s = 0; s < B.NU; ++s) {
+        half_w(l, 1);
+        half_w(l, 0);
+      }
     }
+  }
+  __syncthreads();
+  for (int l = lblock - 1; l >= 0; --l) {
+    const int n = B.lev[l].nx * B.lev[l].ny;
+    for (int t = tid; t < n; t += 1024) prolong_cell(l, t);
     __syncthreads();
     load_coef(l);
     for (int s = 0; s < B.NU; ++s) {
@@ -542,36 +996,66 @@ void smooth(Ctx &c, const Lev<T> &L, int color, bool init, int f0, int f1) {
                        (const double *)c.ps.scal, f0, f1);
 }
 
+// First level of the single-workgroup bottom cycle: as many of the coarsest
+// levels as fit its LDS arrays (each above the coarsest at most
+// MG_BOTTOM_TAKE cells).
+inline int bottom_level(const Mg &G) {
+  const int nl = G.nlev;
+  int lb = nl - 1;
+  long held = (long)G.lev[lb].nx * G.lev[lb].ny;
+  while (lb > 0) {
+    const long cells = (long)G.lev[lb - 1].nx * G.lev[lb - 1].ny;
+    if (cells > MG_BOTTOM_TAKE || held + cells > MG_BOTTOM_LDS) break;
+    held += cells;
+    --lb;
+  }
+  return lb;
+}
+
+// Whether the V-cycle runs its multi-workgroup levels as the fused passes.
 template <typename T>
-void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
+inline bool fused_cycle(Ctx &c) {
+  return smoothing_sweeps(c) == 1 && c.opts.mg_fused && bottom_level(Hier<T>::get(c)) >= 1;
+}
+
+// z = M^-1 r with the cycle in precision T; r and z may be of another type
+// (TB, TZ: FP64 around an FP32 cycle) only on the fused path, where the first
+// level's passes round r on load and widen z on store.
+template <typename T, typename TB = T, typename TZ = T>
+void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
+  constexpr bool same = std::is_same<TB, T>::value && std::is_same<TZ, T>::value;
+  const T *r0 = same ? (const T *)r : nullptr;
+  T *z0 = same ? (T *)z : nullptr;
   Mg &G = Hier<T>::get(c);
   const int NU = smoothing_sweeps(c);  // red-black sweeps before and after the coarse correction
   if (kind == MG_KIND_SGS_RB) {
-    const Lev<T> L = level<T>(c, 0, r, z);
+    const Lev<T> L = level<T>(c, 0, r0, z0);
     smooth(c, L, 0, true, f0, f1);
     smooth(c, L, 1, false, f0, f1);
     smooth(c, L, 0, false, f0, f1);
     return;
   }
   const int nl = G.nlev;
-  // first level handled by the single-workgroup bottom cycle: as many of the
-  // coarsest levels as fit its LDS arrays (each at most MG_BOTTOM_CELLS)
-  int lb = nl - 1;
-  long held = (long)G.lev[lb].nx * G.lev[lb].ny;
-  while (lb > 0) {
-    const long cells = (long)G.lev[lb - 1].nx * G.lev[lb - 1].ny;
-    if (cells > MG_BOTTOM_CELLS || held + cells > MG_BOTTOM_LDS) break;
-    held += cells;
-    --lb;
-  }
+  const int lb = bottom_level(G);
   const T scale = (T)(0.5 * correction_weight(c));
+  const bool fused = fused_cycle<T>(c);
   for (int l = 0; l < lb; ++l) {
-    const Lev<T> L = level<T>(c, l, r, z);
+    const Lev<T> L = level<T>(c, l, r0, z0);
+    const MgLevelBuf &B = G.lev[l + 1];
+    if (fused) {
+      const dim3 grid(fused_tiles(L.nx, L.ny));
+      if (l == 0)
+        hipLaunchKernelGGL((k_mg_pre<T, TB>), grid, dim3(FT_BLK), 0, c.stream, L, r, B.nx, B.ny, (T *)B.b, scale,
+                           (const double *)c.ps.scal, f0, f1);
+      else
+        hipLaunchKernelGGL((k_mg_pre<T, T>), grid, dim3(FT_BLK), 0, c.stream, L, L.b, B.nx, B.ny, (T *)B.b, scale,
+                           (const double *)c.ps.scal, f0, f1);
+      continue;
+    }
     for (int s = 0; s < NU; ++s) {
       smooth(c, L, 0, s == 0, f0, f1);
       smooth(c, L, 1, false, f0, f1);
     }
-    const MgLevelBuf &B = G.lev[l + 1];
     hipLaunchKernelGGL(k_mg_restrict<T>, cell_grid(B.nx, B.ny), dim3(64, 4), 0, c.stream, L, B.nx, B.ny, (T *)B.b,
                        scale, (const double *)c.ps.scal, f0, f1);
   }
@@ -580,7 +1064,7 @@ void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
     Bt.nlev = nl - lb;
     int off = 0;
     for (int l = lb; l < nl; ++l) {
-      const Lev<T> L = level<T>(c, l, r, z);
+      const Lev<T> L = level<T>(c, l, r0, z0);
       BotLev &D = Bt.lev[l - lb];
       D.nx = L.nx;
       D.ny = L.ny;
@@ -591,19 +1075,44 @@ void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
       D.off = off;
       off += L.nx * L.ny;
     }
+    // wave levels: the coarsest ones of at most MG_WAVE_CELLS cells each whose
+    // coefficients fit MG_WAVE_LDS together
+    Bt.lw = Bt.nlev;
+    int woff = 0;
+    while (Bt.lw > 0) {
+      const BotLev &D = Bt.lev[Bt.lw - 1];
+      const int cells = D.nx * D.ny;
+      if (cells > MG_WAVE_CELLS || woff + cells > MG_WAVE_LDS) break;
+      woff += cells;
+      --Bt.lw;
+    }
+    for (int l = Bt.lw, o = 0; l < Bt.nlev; ++l) {
+      Bt.lev[l].woff = o;
+      o += Bt.lev[l].nx * Bt.lev[l].ny;
+    }
     Bt.perx = c.m.perx;
     Bt.pery = c.m.pery;
-    const Lev<T> Lc = level<T>(c, nl - 1, r, z);
+    const Lev<T> Lc = level<T>(c, nl - 1, r0, z0);
     Bt.K = std::min(64, 2 * std::max(Lc.nx, Lc.ny));
     Bt.NU = NU;
     Bt.scale = 0.5 * correction_weight(c);
-    const Lev<T> Lb = level<T>(c, lb, r, z);
+    const Lev<T> Lb = level<T>(c, lb, r0, z0);
     hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(1024), 0, c.stream, Bt, Lb.b, Lb.x, (const double *)c.ps.scal,
                        f0, f1);
   }
   for (int l = lb - 1; l >= 0; --l) {
-    const Lev<T> L = level<T>(c, l, r, z);
+    const Lev<T> L = level<T>(c, l, r0, z0);
     const MgLevelBuf &B = G.lev[l + 1];
+    if (fused) {
+      const dim3 grid(fused_tiles(L.nx, L.ny));
+      if (l == 0)
+        hipLaunchKernelGGL((k_mg_post<T, TB, TZ>), grid, dim3(FT_BLK), 0, c.stream, L, r, z, B.nx, (const T *)B.x,
+                           (const double *)c.ps.scal, f0, f1);
+      else
+        hipLaunchKernelGGL((k_mg_post<T, T, T>), grid, dim3(FT_BLK), 0, c.stream, L, L.b, L.x, B.nx,
+                           (const T *)B.x, (const double *)c.ps.scal, f0, f1);
+      continue;
+    }
     hipLaunchKernelGGL(k_mg_prolong_smooth<T>, pair_grid(L.nx, L.ny), dim3(64, 4), 0, c.stream, L, B.nx,
                        (const T *)B.x, (const double *)c.ps.scal, f0, f1);
     smooth(c, L, 0, false, f0, f1);
@@ -612,6 +1121,14 @@ void apply(Ctx &c, int kind, const T *r, T *z, int f0, int f1) {
       smooth(c, L, 0, false, f0, f1);
     }
   }
+}
+
+// Rounding / widening copy (the FP32 cycle of an FP64 solve off the fused path).
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256) k_mg_convert(long n, const TI *__restrict__ in, TO *__restrict__ out,
+                                                    const double *scal, int f0, int f1) {
+  if (stopped(scal, f0, f1)) return;
+  for (long k = blockIdx.x * 256L + threadIdx.x; k < n; k += (long)gridDim.x * 256) out[k] = (TO)in[k];
 }
 
 }  // namespace
@@ -647,6 +1164,18 @@ void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1) {
 }
 void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1) {
   apply<float>(c, kind, r, z, f0, f1);
+}
+void mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1) {
+  if (fused_cycle<float>(c)) {
+    apply<float, double, double>(c, MG_KIND_VCYCLE, r, z, f0, f1);
+    return;
+  }
+  const long n = (long)c.m.nx * c.m.ny;
+  const double *scal = (const double *)c.ps.scal;
+  hipLaunchKernelGGL((k_mg_convert<double, float>), dim3(1024), dim3(256), 0, c.stream, n, r, c.ps.r32, scal, f0, f1);
+  apply<float>(c, MG_KIND_VCYCLE, (const float *)c.ps.r32, c.ps.z32, f0, f1);
+  hipLaunchKernelGGL((k_mg_convert<float, double>), dim3(1024), dim3(256), 0, c.stream, n, (const float *)c.ps.z32,
+                     z, scal, f0, f1);
 }
 
 void mg_free(Ctx &c) {

@@ -1441,6 +1441,8 @@ template <>
 void explicit_precond<double>(Ctx &c, int pc, const double *r, double *z, int fslot) {
   if (pc == 2) {
     sgs_apply(c, r, z, (const double *)(c.ps.scal + fslot));
+  } else if (pc == 4 && c.ps.mg_fp32_cycle) {
+    mg_apply_fp32_cycle(c, r, z, S_FLAG, fslot);
   } else {
     mg_apply64(c, mg_kind(pc), r, z, S_FLAG, fslot);
   }
@@ -1773,8 +1775,16 @@ static int cycle64(int pc) { return pc == 4 ? 8 : (pc == 3 ? 32 : 64); }
 // FP64 PCG of the SP loop on the context vectors P.b -> P.x.  NONE / JACOBI
 // run as captured cycles of fused kernels; SSOR (the sequential SGS
 // wavefront) runs launch by launch with the same kernels.
-static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
+static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres,
+                       bool fp32_cycle = false) {
   Poisson &P = c.ps;
+  // the multigrid cycle in FP32 (its launches read the flag while this solve
+  // enqueues or captures them)
+  struct CycleFlag {
+    int &f;
+    CycleFlag(int &f_, bool on) : f(f_) { f = on ? 1 : 0; }
+    ~CycleFlag() { f = 0; }
+  } cycle_flag(P.mg_fp32_cycle, fp32_cycle && pc_code(precond) == 4);
   const Mesh &m = c.m;
   const long n = (long)m.nx * m.ny;
   const DevOp o = dev_op(m);
@@ -1785,7 +1795,7 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
   hipLaunchKernelGGL(k_set_scalars, dim3(1), dim3(64), 0, c.stream, P.scal, 0, NREAD, 0.0);
   if ((st = set_scalars(c, {{S_TOL, tol}, {S_FIRST, 1.0}, {S_MAXIT, (double)maxit}}))) return st;
   const int pc = pc_code(precond);
-  if (pc >= 3 && (st = mg_prepare(c, false))) return st;
+  if (pc >= 3 && (st = mg_prepare(c, P.mg_fp32_cycle != 0))) return st;
   double *pb = part_buf(c, 1);
   if (pc == 1)
     hipLaunchKernelGGL((k_residual<1>), g, dim3(BLK), 0, c.stream, o, A, (const double *)P.b, (const double *)P.x,
@@ -1819,8 +1829,8 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
     }
   } else {
     const int cyc = cycle64(pc);
-    if ((st = run_cycles(c, P.graph[0], 1 + pc + (c.ref_order ? 64 : 0), pc, false, cyc, maxit / cyc + 2)))
-      return st;
+    const int key = 1 + pc + (c.ref_order ? 64 : 0) + (P.mg_fp32_cycle ? 128 : 0);
+    if ((st = run_cycles(c, P.graph[0], key, pc, false, cyc, maxit / cyc + 2))) return st;
   }
   flush_x<double>(c, P.x, P.p, P.t0, OUTER, S_ZERO);
   COVOF_CHECK_LAUNCH();
@@ -1845,10 +1855,26 @@ static int solve_pcg64(Ctx &c, int precond, double tol, int maxit, int *iters, d
 // refined x; the iteration count is then the sum of both.
 int cg2_reserve(Ctx &c);
 
+static int true_relres(Ctx &c, double *relres);
+
 static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, double *relres) {
   Poisson &P = c.ps;
   const long n = (long)c.m.nx * c.m.ny;
   const int pc = pc_code(precond);
+  if (pc == 4 && c.opts.mixed_mg == 0) {
+    // FP64 CG whose preconditioner, the V-cycle, runs in FP32: the FP64
+    // recurrence keeps FP64 MG-PCG's iteration count and attainable accuracy,
+    // the cycle's passes move half the bytes
+    P.last_it32 = 0;
+    P.last_it64 = 0;
+    int it = 0;
+    const int st = solve_pcg64(c, precond, tol, maxit, &it, relres, true);
+    P.last_it64 = it;
+    if (iters) *iters = it;
+    if (st == COVOF_HIP_ERROR || st == COVOF_BAD_ARGUMENT) return st;
+    const int st2 = true_relres(c, relres);
+    return st2 ? st2 : st;
+  }
   int cycle = c.opts.mixed_cycle > 0 ? c.opts.mixed_cycle : (pc == 4 ? 8 : (pc == 3 ? 32 : 128));
   if (cycle < 2) cycle = 2;
   cycle &= ~1;  // even: the parity pattern of k repeats from one replay to the next
@@ -1896,21 +1922,28 @@ static int solve_mixed(Ctx &c, int precond, double tol, int maxit, int *iters, d
     // above tol -- 4096 x 2048 bubbles at density ratio 1000: FP64 MG-PCG
     // alone ends at a true 1.7e-9 when its recurrence reads 9e-11 -- the
     // true residual can exceed tol)
-    OpT<double> A{P.kx, P.ky, P.diag};
-    double *pb = part_buf(c, 1);
-    hipLaunchKernelGGL(k_true_res, red_grid(c.m), dim3(BLK), 0, c.stream, dev_op(c.m), A, (const double *)P.b,
-                       (const double *)P.x, pb);
-    hipLaunchKernelGGL(k_true_res_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks,
-                       P.scal + 58);
-    COVOF_CHECK_LAUNCH();
-    COVOF_CHECK(hipMemcpyAsync(P.hscal + 58, P.scal + 58, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    COVOF_CHECK(hipStreamSynchronize(c.stream));
-    if (relres) *relres = P.hscal[58];
-    return st;
+    const int st2 = true_relres(c, relres);
+    return st2 ? st2 : st;
   }
   if (f == F_CONV) return COVOF_OK;
   if (f == F_NONFINITE) return COVOF_NONFINITE;
   return COVOF_NOT_CONVERGED;
+}
+
+// ||b - A x|| / ||b|| of the context vectors P.b, P.x in FP64, recomputed.
+static int true_relres(Ctx &c, double *relres) {
+  Poisson &P = c.ps;
+  OpT<double> A{P.kx, P.ky, P.diag};
+  double *pb = part_buf(c, 1);
+  hipLaunchKernelGGL(k_true_res, red_grid(c.m), dim3(BLK), 0, c.stream, dev_op(c.m), A, (const double *)P.b,
+                     (const double *)P.x, pb);
+  hipLaunchKernelGGL(k_true_res_finish, dim3(1), dim3(BLK), 0, c.stream, (const double *)pb, P.nblocks,
+                     P.scal + 58);
+  COVOF_CHECK_LAUNCH();
+  COVOF_CHECK(hipMemcpyAsync(P.hscal + 58, P.scal + 58, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  COVOF_CHECK(hipStreamSynchronize(c.stream));
+  if (relres) *relres = P.hscal[58];
+  return COVOF_OK;
 }
 
 int poisson_solve(Ctx &c, const double *b, double *x, int lin_solver, int precond, double tol,

@@ -106,6 +106,12 @@ int covof_ctx_set_reduction_order(covof_ctx *ctx, int order);
 #define COVOF_OPT_MG_SWEEPS 5          /* red-black sweeps before and after the coarse correction, 1..4
                                           (default 1) */
 #define COVOF_OPT_MG_OMEGA 6           /* weight of the coarse-grid correction, (0, 2] (default 1.8) */
+#define COVOF_OPT_MG_FUSED 7           /* 1 (default): with one sweep, each multi-workgroup level of the V-cycle
+                                          in two passes (smoothing + restriction, prolongation + smoothing);
+                                          0: five launches per level.  Same values either way. */
+#define COVOF_OPT_MIXED_MG_FORM 8      /* PRECISION_MIXED with PRECOND_MG: 0 (default) FP64 CG whose V-cycle
+                                          runs in FP32; 1 the FP32 Krylov recurrence with FP64 reliable
+                                          updates (the form of the other preconditioners) */
 int covof_ctx_set_solver_option(covof_ctx *ctx, int option, double value);
 /* Makes ahead the work arrays a path otherwise makes on its first use, so
  * that the calls of a time step allocate nothing (no reference counterpart;
@@ -121,7 +127,8 @@ int covof_ctx_set_solver_option(covof_ctx *ctx, int option, double value);
 int covof_ctx_reserve(covof_ctx *ctx, int parts);
 /* The split of the context's last mixed-precision CG solve: iterations of
  * the FP32 recurrence and of the FP64 continuation that follows a stall (0
- * when it did not run).  After a continuation the solve's relres is the
+ * when it did not run); for the FP64 CG with an FP32 V-cycle (MG's default
+ * mixed form) 0 and its iterations.  After a continuation the solve's relres is the
  * recomputed true FP64 residual and its status the FP64 PCG's (stopped on
  * its recurrence residual), so where the system's attainable accuracy is
  * above tol the relres can exceed tol with status COVOF_OK. */
@@ -280,6 +287,9 @@ int covof_poisson_precond_d(covof_ctx *ctx, int precond, const double *r, double
  *     the FP64 ||r||/||b|| <= tol (CG with NONE, JACOBI, SGS_RB or MG);
  *     when that residual stagnates above tol (no 10 % gain over 48
  *     refinements, 2 of 8 iterations with MG) FP64 PCG continues from the refined x.
+ *     With MG the default form is instead FP64 CG whose preconditioner, the
+ *     V-cycle, runs in FP32 (COVOF_OPT_MIXED_MG_FORM); its relres is the
+ *     true FP64 residual of the returned x.
  * *iters: Krylov iterations (inner iterations summed for MIXED, plus the
  * FP64 continuation's); *relres:
  * final relative residual as defined by the stop rule. */

@@ -14,7 +14,7 @@ CASES = [((96, 80), (4, 4, 4, 4)), ((128, 64), (3, 3, 3, 3)), ((100, 60), (1, 2,
 IDS = ["walls", "periodic", "wall-neumann-periodic", "odd-coarsening", "one-workgroup"]
 
 
-def _ctx(cv, shape, bc, seed=0):
+def _ctx(cv, shape, bc, seed=0, with_rho=False):
     from covof_amd import workloads as wl
     nx, ny = shape
     n = max(nx, ny)
@@ -32,7 +32,7 @@ def _ctx(cv, shape, bc, seed=0):
     ctx = cv.Context(nx, ny, bc=bc)
     z = ctx.new_field()
     ctx.poisson_assemble(z, z, to_gpu(rho_u), to_gpu(rho_v), 1.0)
-    return ctx
+    return (ctx, (rho_u, rho_v)) if with_rho else ctx
 
 
 @pytest.mark.parametrize("shape,bc", CASES, ids=IDS)
@@ -85,25 +85,36 @@ def test_mg_pcg_matches_jacobi_pcg(cv, orc, shape, bc):
 
 
 def test_mg_mixed_2048_reaches_1e10(cv, orc):
-    """The 2048^2 Poisson workload: mixed-precision MG-PCG to 1e-10 (FP64 true
-    residual checked on the host with the oracle's matvec)."""
+    """The 2048^2 Poisson workload: MG-PCG to 1e-10 in FP64 and in both mixed
+    forms (FP64 CG with the V-cycle in FP32, the default; the FP32 Krylov
+    recurrence with FP64 reliable updates), each checked by its FP64 true
+    residual on the host with the oracle's matvec.  The FP32 cycle keeps the
+    FP64 solve's iteration count to within two, and the solve reports the
+    true residual."""
     from covof_amd import workloads as wl
     n = 2048
     case = wl.poisson_case(n, seed=0)
     ctx = cv.Context(n, n, bc=case["bc"])
     b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
     out = {}
-    for prec in (cv.PRECISION_FP64, cv.PRECISION_MIXED):
+    for name, prec, form in (("fp64", cv.PRECISION_FP64, 0), ("mixed", cv.PRECISION_MIXED, 0),
+                             ("mixed-fp32-krylov", cv.PRECISION_MIXED, 1)):
+        ctx.set_solver_option(cv.OPT_MIXED_MG_FORM, form)
         x = ctx.new_work()
         r = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=1e-10, max_iter=10000, precision=prec)
-        assert r.converged, (prec, r)
-        out[prec] = (x.cpu().numpy(), r.iters)
+        assert r.converged, (name, r)
+        out[name] = (x.cpu().numpy(), r)
     m = orc.mesh(n, n, bc=case["bc"])
     co = orc.poisson_assemble(m, case["u"], case["v"], case["rho_u"], case["rho_v"], case["dt"])
-    for prec, (x, it) in out.items():
+    for name, (x, r) in out.items():
         rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x)) / np.linalg.norm(co["b"])
-        print(f"precision {prec}: {it} MG-PCG iterations, true relres {rr:.3e}")
+        print(f"{name}: {r.iters} MG-PCG iterations, true relres {rr:.3e} (reported {r.relres:.3e})")
         assert rr <= 1.01e-10
+    rr_mixed = out["mixed"][1].relres
+    x = out["mixed"][0]
+    rr = np.linalg.norm(co["b"] - orc.poisson_matvec(m, co, x)) / np.linalg.norm(co["b"])
+    assert abs(rr - rr_mixed) <= 1e-3 * rr
+    assert out["mixed"][1].iters <= out["fp64"][1].iters + 2
 
 
 def test_mg_mixed_stall_continues_in_fp64(cv, orc):
@@ -116,6 +127,7 @@ def test_mg_mixed_stall_continues_in_fp64(cv, orc):
     n, cap = 256, 400
     case = wl.poisson_case(n, seed=1, n_circles=12, rmin=8, rmax=32)
     ctx = cv.Context(n, n, bc=case["bc"])
+    ctx.set_solver_option(cv.OPT_MIXED_MG_FORM, 1)  # the FP32 Krylov recurrence
     b = ctx.poisson_assemble(*(to_gpu(case[k]) for k in ("u", "v", "rho_u", "rho_v")), case["dt"])
     x = ctx.new_work()
     r = ctx.poisson_solve(b, x, precond=cv.PRECOND_MG, tol=0.0, max_iter=cap, precision=cv.PRECISION_MIXED)
@@ -168,7 +180,7 @@ def test_solver_options_validated(cv):
     from covof_amd import workloads as wl
     ctx = cv.Context(64, 64, bc=(1, 1, 1, 1))
     for opt, bad in ((cv.OPT_MIXED_CYCLE, 3), (cv.OPT_MIXED_CYCLE, 1), (cv.OPT_MIXED_STALL_WINDOW, -1),
-                     (cv.OPT_MIXED_LOOP, 2), (cv.OPT_MG_SWEEPS, 5), (cv.OPT_MG_OMEGA, 2.5), (cv.OPT_MG_COARSE, 0.5),
+                     (cv.OPT_MIXED_LOOP, 2), (cv.OPT_MG_SWEEPS, 5), (cv.OPT_MG_OMEGA, 2.5), (cv.OPT_MG_COARSE, 0.5), (cv.OPT_MG_FUSED, 2), (cv.OPT_MIXED_MG_FORM, 3),
                      (99, 1)):
         with pytest.raises(cv.CovofError):
             ctx.set_solver_option(opt, bad)
@@ -182,3 +194,79 @@ def test_solver_options_validated(cv):
         assert r.converged, r
         its.append(r.iters)
     assert its[0] == its[2] and its[1] < its[0], its
+
+
+FUSED_CASES = [((300, 260), (4, 4, 4, 4)), ((256, 192), (3, 3, 3, 3)), ((200, 136), (1, 2, 3, 3)),
+               ((130, 514), (2, 2, 4, 4)), ((8, 2048), (3, 3, 4, 4)), ((1030, 512), (4, 4, 3, 3))]
+FUSED_IDS = ["walls-odd", "periodic", "wall-neumann-periodic", "neumann-tall", "narrow-periodic-x",
+             "wide-periodic-y"]
+
+
+@pytest.mark.parametrize("shape,bc", FUSED_CASES, ids=FUSED_IDS)
+def test_mg_fused_levels_bitwise(cv, shape, bc):
+    """The fused level passes (smoothing + restriction in one kernel,
+    prolongation + both smoothing half-sweeps in another) evaluate the
+    five-launch V-cycle's expressions on the same operands: M r is bitwise
+    the same, and so are whole FP64 and mixed-precision MG-PCG solves (both
+    mixed forms: FP64 CG around FP32 V-cycles, where the first level's passes
+    round r and widen z, and the FP32 Krylov recurrence): status, iteration
+    count and x.  (The narrow
+    periodic strip converges slowly -- its coarsest level is a 2 x 512 line
+    that the bottom's sweeps do not solve -- which does not matter here.)"""
+    ctx = _ctx(cv, shape, bc)
+    nx, ny = shape
+    rng = np.random.default_rng(5)
+    r = to_gpu(rng.normal(size=(ny, nx)))
+    z = {}
+    for fused in (1, 0):
+        ctx.set_solver_option(cv.OPT_MG_FUSED, fused)
+        z[fused] = ctx.poisson_precond(cv.PRECOND_MG, r).cpu().numpy()
+    assert np.array_equal(z[0], z[1])
+    b = rng.normal(size=(ny, nx))
+    if bc[0] != 2 and bc[1] != 2:
+        b -= b.mean()
+    bg = to_gpu(b)
+    out = {}
+    runs = ((cv.PRECISION_FP64, 0), (cv.PRECISION_MIXED, 0), (cv.PRECISION_MIXED, 1))
+    for fused in (1, 0):
+        ctx.set_solver_option(cv.OPT_MG_FUSED, fused)
+        for prec, form in runs:
+            ctx.set_solver_option(cv.OPT_MIXED_MG_FORM, form)
+            x = ctx.new_work()
+            res = ctx.poisson_solve(bg, x, precond=cv.PRECOND_MG, tol=1e-10, max_iter=400, precision=prec)
+            out[(fused, prec, form)] = (res.status, res.iters, x.cpu().numpy())
+    for prec, form in runs:
+        assert out[(1, prec, form)][:2] == out[(0, prec, form)][:2]
+        assert np.array_equal(out[(1, prec, form)][2], out[(0, prec, form)][2])
+
+
+PIN_CASES = [((96, 80), (4, 4, 4, 4)), ((300, 260), (4, 4, 4, 4)), ((256, 192), (3, 3, 3, 3)),
+             ((200, 136), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4)), ((8, 2048), (3, 3, 4, 4)), ((40, 30), (4, 4, 3, 3))]
+PIN_IDS = ["walls-small", "walls-odd", "periodic", "wall-neumann-periodic", "odd-coarsening", "narrow-periodic-x",
+           "one-workgroup"]
+
+
+@pytest.mark.parametrize("shape,bc", PIN_CASES, ids=PIN_IDS)
+def test_mg_vcycle_bitwise_vs_numpy(cv, orc, shape, bc):
+    """M r of PRECOND_MG (FP64) bit for bit against the numpy restatement of
+    the cycle (tests/mg_numpy.py) on the oracle's assembled operator: the
+    hierarchy, the fused multi-workgroup levels, the single-workgroup bottom
+    and its wave-only coarse levels."""
+    import mg_numpy as mgn
+    nx, ny = shape
+    ctx, (rho_u, rho_v) = _ctx(cv, shape, bc, with_rho=True)
+    m = orc.mesh(nx, ny, bc=bc)
+    zf = np.zeros((ny + 2, nx + 2))
+    co = orc.poisson_assemble(m, zf, zf, rho_u, rho_v, 1.0)
+    kx, ky = mgn.faces_from_assembly(co)
+    perx, pery = int(bc[0] == 3), int(bc[2] == 3)
+    levels = mgn.hierarchy(kx, ky, co["aP"], perx, pery, 1.0e-12 * (1.0 / nx) * (1.0 / ny))
+    rng = np.random.default_rng(6)
+    for trial in range(2):
+        r = rng.normal(size=(ny, nx))
+        zn = mgn.vcycle(levels, r, perx, pery)
+        for fused in (1, 0):
+            ctx.set_solver_option(cv.OPT_MG_FUSED, fused)
+            zg = ctx.poisson_precond(cv.PRECOND_MG, to_gpu(r)).cpu().numpy()
+            bad = np.argwhere(zg != zn)
+            assert bad.size == 0, (fused, len(bad), bad[:4], np.abs(zg - zn).max())
