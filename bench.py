@@ -23,9 +23,12 @@ the CPU baselines.  Prints one JSON line.
 
 The kernel-sample run executes 256 iterations of the mixed solve, 64 of the
 FP64 solve and 2 VOF steps on the same workloads, 4 momentum-operator
-products and one BiCGSTAB(2) sweep of the 2048^2 predictor, then a plain copy of
-256 MiB (torch copy_ of float64, reads and writes 256 MiB) whose FETCH_SIZE /
-WRITE_SIZE readings calibrate the counters into bytes.
+products and one BiCGSTAB(2) sweep of the 2048^2 predictor, the FP64 and
+mixed multigrid-preconditioned solves, then a plain copy of 256 MiB (torch
+copy_ of float64, reads and writes 256 MiB) whose FETCH_SIZE / WRITE_SIZE
+readings calibrate the counters into bytes.  With --n 4096 the same sample
+runs at 4096^2, where the FP32 Jacobi working set (8 arrays, 537 MB) no
+longer fits the 256 MB Infinity Cache.
 """
 from __future__ import annotations
 
@@ -44,9 +47,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a 256 MiB
 # copy) measured with rocprofv3 --pmc on `bench.py --kernel-sample` and
 # summarised by `bench.py --summarize`; the summary with its commands is
-# profiles/pcg_wy_2048_hbm_traffic.md.
-TRAFFIC_2048 = {"k_pcg_a<double,1>": 2.083e8 + 1.007e8, "k_pcg_b<double,1>": 1.008e8 + 3.365e7,
-                "k_pcg_a<float,1>": 1.109e8 + 5.036e7, "k_pcg_b<float,1>": 5.045e7 + 1.688e7}
+# profiles/poisson_wy_momentum_2048_hbm_traffic.md.
+TRAFFIC_2048 = {"k_pcg_a<double,1>": 1.904e8 + 1.007e8, "k_pcg_b<double,1>": 1.008e8 + 3.365e7,
+                "k_pcg_a<float,1>": 9.668e7 + 5.036e7, "k_pcg_b<float,1>": 5.045e7 + 1.688e7}
 
 
 def log(*a):
@@ -75,6 +78,10 @@ def kernel_sample(cv, wl, ctx, b, n, dev):
     mctx.solve_linear_system(xm, mctx.new_momentum_vector(), cv.LINEAR_BICGSTAB2, cv.PRECOND_JACOBI, 1e-30, 2)
     torch.cuda.synchronize()
     log("momentum sample done")
+    # the multigrid-preconditioned solves (FP64, and FP64 CG with the FP32 V-cycle)
+    for precision in (cv.PRECISION_FP64, cv.PRECISION_MIXED):
+        r = ctx.poisson_solve(b, ctx.new_work(), precond=cv.PRECOND_MG, tol=1e-10, max_iter=100, precision=precision)
+        log(f"sample MG solve precision={precision}: {r.iters} iterations")
     a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
     c = torch.empty_like(a)
     for _ in range(3):
@@ -98,10 +105,19 @@ def summarize(path, min_us=5.0):
     sample (a dispatch whose name holds "copyBuffer")."""
     import sqlite3
     db = sqlite3.connect(path)
-    rows = db.execute("select name, end - start from kernels").fetchall()
+    rows = db.execute("select name, end - start, grid_x from kernels").fetchall()
+    # a kernel launched on grids of several sizes is listed once per grid
+    grids = {}
+    for name, _, gx in rows:
+        grids.setdefault(_short(name), set()).add(gx)
+
+    def label(name, gx):
+        k = _short(name)
+        return f"{k} [grid {gx}]" if len(grids.get(k, ())) > 1 else k
+
     stats = {}
-    for name, dur in rows:
-        st = stats.setdefault(_short(name), [0, 0.0, 0, 0.0])
+    for name, dur, gx in rows:
+        st = stats.setdefault(label(name, gx), [0, 0.0, 0, 0.0])
         st[0] += 1
         st[1] += dur
         if dur >= min_us * 1e3:
@@ -117,25 +133,35 @@ def summarize(path, min_us=5.0):
         lines.append(f"| {k} | {c} | {tot / 1e6:.3f} | {100 * tot / total:.1f} | {tot / c / 1e3:.2f} | {cf} | "
                      + (f"{tf / cf / 1e3:.2f} |" if cf else "- |"))
     try:
-        pmc = db.execute("select kernel_name, counter_name, value, end - start from counters_collection").fetchall()
+        cols = [r[1] for r in db.execute("pragma table_info(counters_collection)")]
+        gcol = "grid_x" if "grid_x" in cols else ("grid_size" if "grid_size" in cols else "0")
+        pmc = db.execute(f"select kernel_name, counter_name, value, end - start, {gcol} "
+                         "from counters_collection").fetchall()
     except sqlite3.OperationalError:
         pmc = []
     if pmc:
         # per kernel and counter: the median over the launches that did full
         # work (at least half the kernel's largest reading; counter collection
         # serialises dispatches, so durations cannot tell early returns apart)
+        pgrids = {}
+        for name, _, _, _, gx in pmc:
+            pgrids.setdefault(_short(name), set()).add(gx)
         vals = {}
-        for name, ctr, val, dur in pmc:
-            vals.setdefault((_short(name), ctr), []).append(val)
+        for name, ctr, val, dur, gx in pmc:
+            k = _short(name)
+            k = f"{k} [grid {gx}]" if len(pgrids[k]) > 1 else k
+            vals.setdefault((k, ctr), []).append(val)
         med, nfull = {}, {}
         for k, v in vals.items():
             full = sorted(x for x in v if x >= 0.5 * max(v))
             med[k] = full[len(full) // 2] if full else 0.0
             nfull[k] = len(full)
         cal = {}
-        for (k, ctr), v in med.items():
-            if "copyBuffer" in k and ctr in ("FETCH_SIZE", "WRITE_SIZE") and v > 0:
-                cal[ctr] = 256.0 * 1024 * 1024 / (v * 1024.0)  # the 256 MiB copies are the largest
+        for (k, ctr), v in vals.items():
+            # the 256 MiB copies are the largest copy dispatches of the sample
+            if "copyBuffer" in k and ctr in ("FETCH_SIZE", "WRITE_SIZE") and max(v) > 0:
+                f = 256.0 * 1024 * 1024 / (max(v) * 1024.0)
+                cal[ctr] = min(cal.get(ctr, f), f)
         lines += ["", "Counters: median per full-work launch (FETCH_SIZE / WRITE_SIZE in KiB); bytes = KiB x 1024 "
                   "x the 256 MiB copy's calibration " + ", ".join(f"{c} x{f:.3f}" for c, f in sorted(cal.items())),
                   "", "| kernel | counter | launches | value/launch | bytes/launch |", "|---|---|---|---|---|"]
@@ -152,7 +178,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--vof-steps", type=int, default=20)
+    ap.add_argument("--vof-steps", type=int, default=100)
     ap.add_argument("--sp-steps", type=int, default=500)
     ap.add_argument("--cpu-vof-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
@@ -252,6 +278,32 @@ def main():
                                  "achieved_GBps": domm[1] / (domm[2] * 1e-3) / 1e9,
                                  "traffic": TRAFFIC_2048.get(domm[0]) if n == 2048 else None}}
 
+    # ------------------------------------------- the PCG kernels at 4096^2
+    # 2048 iterations of the FP64 and of the FP32 (mixed) Jacobi loop on the
+    # same workload at 4096^2, where the FP32 working set (537 MB) no longer
+    # fits the 256 MB Infinity Cache: the kernels' HBM-resident rates
+    n4 = 2 * n
+    case4 = wl.poisson_case(n4, seed=0)
+    ctx4 = cv.Context(n4, n4, bc=case4["bc"])
+    T4 = {k: torch.tensor(case4[k], device=dev) for k in ("u", "v", "rho_u", "rho_v")}
+    b4 = ctx4.poisson_assemble(T4["u"], T4["v"], T4["rho_u"], T4["rho_v"], case4["dt"])
+    del T4, case4
+    p4 = {"workload": f"the same Poisson workload at {n4}x{n4}, 2048 Jacobi-PCG iterations (FP64, then the "
+                      "mixed solve's FP32 recurrence); kernel times from HIP events on every 16th iteration"}
+    for name, prec, tags, word in (("fp64", cv.PRECISION_FP64, (L.KERNEL_PCG_A_FP64, L.KERNEL_PCG_B_FP64), 8),
+                                   ("fp32", cv.PRECISION_MIXED, (L.KERNEL_PCG_A_FP32, L.KERNEL_PCG_B_FP32), 4)):
+        ctx4.poisson_solve(b4, ctx4.new_work(), precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=256, precision=prec)
+        ctx4.profile(16)
+        ctx4.poisson_solve(b4, ctx4.new_work(), precond=cv.PRECOND_JACOBI, tol=1e-10, max_iter=2048, precision=prec)
+        (ms_a, n_a), (ms_b, n_b) = ctx4.profile_read(tags[0]), ctx4.profile_read(tags[1])
+        ctx4.profile(0)
+        ua, ub = ms_a / max(n_a, 1) * 1e3, ms_b / max(n_b, 1) * 1e3
+        ba, bb = 8 * word * n4 * n4, 4 * word * n4 * n4
+        p4[name] = {"pcg_a_us": ua, "pcg_a_GBps": ba / (ua * 1e-6) / 1e9, "pcg_a_frac": ba / (ua * 1e-6) / 1e9 /
+                    HBM_PEAK_GBS, "pcg_b_us": ub, "pcg_b_GBps": bb / (ub * 1e-6) / 1e9, "launches": n_a}
+        log(f"PCG kernels {n4}^2 {name}: A {ua:.1f} us ({p4[name]['pcg_a_GBps']:.0f} GB/s), B {ub:.1f} us")
+    del ctx4, b4
+
     # ------------------------------------------- multigrid-preconditioned CG
     # the same system with the library's V-cycle preconditioner (no reference
     # counterpart), timed the same way: FP64; mixed (the default form: FP64 CG
@@ -302,7 +354,8 @@ def main():
     upd = vctx.profile_read(L.KERNEL_VOF_UPDATE)
     vctx.profile(0)
     vof = {
-        "workload": f"ELVIRA + Weymouth-Yue, {vn}x{vn} periodic, 128 seeded ellipses, CFL 0.25",
+        "workload": f"ELVIRA + Weymouth-Yue, {vn}x{vn} periodic, 128 seeded ellipses, CFL 0.25, "
+                    f"{args.vof_steps} steps",
         "cell_updates_per_s": vn * vn / t_vof,
         "ms_per_step": t_vof * 1e3,
         "algorithmic_GBps": 50.0 * vn * vn / t_vof / 1e9,
@@ -426,6 +479,13 @@ def main():
     steps_out = {}
 
     def make_run(curv, prec):
+        # The mean-free coarsest solve of the V-cycle (the default for a
+        # singular operator) takes this system's FP64 MG-PCG from about 250
+        # to 150 iterations with the reference's curvature, but from 255 to
+        # 285 with height-function curvature, whose solves end below the
+        # FP64 floor of the true residual (|b| is 8x smaller); the
+        # height-function runs keep the plain coarsest sweeps.
+        sctx.set_solver_option(cv.OPT_MG_PROJECT, 0 if curv == "height_function" else 1)
         return cv.StandardProjectionRun(sctx, C_dev, zs, zs, 1000.0, 1.0, 0.1, 1.0e-3, 0.07,
                                         time_scheme=cv.TIME_SCHEME_GEAR, convective_scheme=cv.CONV_TVD_SUPERBEE,
                                         gy=-9.81, predictor=(cv.LINEAR_BICGSTAB2, cv.PRECOND_JACOBI, 1e-10, 500),
@@ -505,6 +565,7 @@ def main():
     # ~1e-13 relative rounding noise of J, which a zero-mean direction's
     # near-cancelling derivative would not)
     nwin = 16
+    sctx.set_solver_option(cv.OPT_MG_PROJECT, 0)  # the height-function state of the 500-step run (see make_run)
     kadj = cv.KineticEnergyAdjoint(sctx, 1000.0, 1.0, dts, gy=-9.81, reconstruction="PLIC", tol=1e-12,
                                    precond=cv.PRECOND_MG)
     kadj.forward(Cw, uw, vw, 1)  # warm-up: graph captures, work arrays
@@ -726,6 +787,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_vof": cpu_vof,
         "mixed_precision": mixed,
+        "pcg_kernels_4096": p4,
         "multigrid_pcg": mg,
         "vof": vof,
         "sp_predictor": predictor,

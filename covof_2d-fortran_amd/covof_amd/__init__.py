@@ -19,7 +19,7 @@ from ._lib import (BC_DIRICHLET, BC_FREESLIP, BC_NEUMANN, BC_PERIODIC, LINEAR_BI
                    PRECOND_BLOCK_TRIANGULAR, SOLVER_MONOLITHIC, SOLVER_STANDARD_PROJECTION, CURVATURE_NORMALS,
                    CURVATURE_HEIGHT_FUNCTION, REDUCTION_TREE, REDUCTION_REFERENCE, OPT_MIXED_STALL_WINDOW,
                    OPT_MIXED_CYCLE, OPT_MIXED_LOOP, OPT_MG_COARSE, OPT_MG_SWEEPS, OPT_MG_OMEGA, OPT_MG_FUSED,
-                   OPT_MIXED_MG_FORM,                   CovofError)
+                   OPT_MIXED_MG_FORM, OPT_MG_PROJECT, CovofError)
 from .context import Context, SolveResult  # noqa: F401
 from .autograd import PressureSolve, Projection, pressure_solve, projection  # noqa: F401
 from .sp_run import StandardProjectionRun  # noqa: F401

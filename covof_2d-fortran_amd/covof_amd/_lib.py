@@ -27,7 +27,7 @@ CURVATURE_NORMALS, CURVATURE_HEIGHT_FUNCTION = 0, 1
 PRECOND_SGS_RB, PRECOND_MG = 16, 17
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 OPT_MIXED_STALL_WINDOW, OPT_MIXED_CYCLE, OPT_MIXED_LOOP, OPT_MG_COARSE, OPT_MG_SWEEPS, OPT_MG_OMEGA = 1, 2, 3, 4, 5, 6
-OPT_MG_FUSED, OPT_MIXED_MG_FORM = 7, 8
+OPT_MG_FUSED, OPT_MIXED_MG_FORM, OPT_MG_PROJECT = 7, 8, 9
 (RESERVE_MOMENTUM, RESERVE_MULTIGRID, RESERVE_SGS, RESERVE_MIXED_LOOP, RESERVE_SURFACE, RESERVE_ADJOINT,
  RESERVE_HOST_MIRRORS, RESERVE_ALL) = 1, 2, 4, 8, 16, 32, 64, 127
 REDUCTION_TREE, REDUCTION_REFERENCE = 0, 1

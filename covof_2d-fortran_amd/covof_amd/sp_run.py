@@ -162,6 +162,7 @@ class StandardProjectionRun:
         solver, precond, tol, maxit, precision = self.poisson
         rq = ctx.solve_standard_projection_poisson(f["u"], f["v"], f["rho_u"], f["rho_v"], f["p"], dt, solver,
                                                    precond, tol, maxit, precision)
+        self.last_poisson = rq  # iterations, relres (the true FP64 residual for MIXED), status
         ctx.correct_velocity(f["p"], f["rho_u"], f["rho_v"], dt, f["u"], f["v"])
         ctx.apply_velocity_bc(cfg, f["u"], f["v"])
         return rp.iters, rq.iters, rp.status, rq.status

@@ -237,6 +237,10 @@ extern "C" int covof_ctx_set_solver_option(covof_ctx *h, int option, double valu
       if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MIXED_MG_FORM must be 0 or 1");
       o.mixed_mg = (int)value;
       break;
+    case COVOF_OPT_MG_PROJECT:
+      if (value != 0.0 && value != 1.0) return fail(COVOF_BAD_ARGUMENT, "MG_PROJECT must be 0 or 1");
+      o.mg_project = (int)value;
+      break;
     default:
       return fail(COVOF_BAD_ARGUMENT, "covof_ctx_set_solver_option: unknown option");
   }

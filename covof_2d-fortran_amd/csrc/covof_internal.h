@@ -185,6 +185,7 @@ struct SolverOpts {
   double mg_omega = 1.8; // weight of the coarse-grid correction
   int mg_fused = 1;      // one sweep: fused level passes (k_mg_pre / k_mg_post)
   int mixed_mg = 0;      // MIXED with MG: 0 FP64 CG + FP32 V-cycle, 1 FP32 Krylov + reliable updates
+  int mg_project = 1;    // singular operator: mean-free coarsest solve (P S P)
 };
 
 struct Ctx {

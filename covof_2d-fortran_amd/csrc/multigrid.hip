@@ -535,6 +535,7 @@ struct Bottom {
   int lw;    // first wave level (nlev: none)
   BotLev lev[MG_MAX_LEVELS];
   int perx, pery, K, NU;
+  int project;   // singular operator: the coarsest solve is P S P (see k_mg_bottom)
   double scale;  // restriction weight (R = scale * P^T)
 };
 
@@ -554,6 +555,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   __shared__ T bs[MG_BOTTOM_LDS];
   __shared__ T wc[5][MG_WAVE_LDS];  // west, east, south, north, diagonal of the wave levels
   __shared__ T rsd[MG_BOTTOM_CELLS];  // residuals of a workgroup level, for its restriction
+  __shared__ T s_mean;
   const int tid = threadIdx.x;
   const int last = B.nlev - 1, lw = B.lw;
   {
@@ -718,8 +720,30 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       __syncthreads();
     }
   }
+  // With a singular operator (no Dirichlet pressure side: the constant is in
+  // its null space up to the assembly's 1e-12 dvol shift) the coarsest solve
+  // is P S P, S its sweeps and P the removal of the mean (a serial sum in
+  // cell order): the sweeps would otherwise grow the constant mode of the
+  // coarsest right-hand side's mean without bound, and the prolonged
+  // constant, invisible to A, would cost the fine levels' stencils their
+  // digits by cancellation.  P S P keeps the cycle symmetric.
+  auto mean_free = [&](T *a, int n, bool wave) {
+    if ((wave ? threadIdx.x : tid) == 0) {
+      T sum = T(0);
+      for (int t = 0; t < n; ++t) sum = sum + a[t];
+      s_mean = sum / (T)n;
+    }
+    if (wave) wave_sync();
+    else __syncthreads();
+    const T mu = s_mean;
+    for (int t = wave ? threadIdx.x : tid; t < n; t += wave ? 64 : 1024) a[t] = a[t] - mu;
+    if (wave) wave_sync();
+    else __syncthreads();
+  };
   if (lw > last) {
     // no wave level: the coarsest level with the workgroup
+    const BotLev &L = B.lev[last];
+    if (B.project) mean_free(bs + L.off, L.nx * L.ny, false);
     zero(last);
     load_coef(last);
     for (int s = 0; s < B.K; ++s) {
@@ -730,6 +754,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       half(last, 1);
       half(last, 0);
     }
+    if (B.project) mean_free(xs + L.off, L.nx * L.ny, false);
   } else if (tid < 64) {
     // ---- the wave levels, first wave only
     const int lane = tid;
@@ -781,6 +806,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       const BotLev &L = B.lev[last];
       const int nx = L.nx, ny = L.ny, n = nx * ny, o = L.woff;
       T *x = xs + L.off;
+      if (B.project) mean_free(bs + L.off, n, true);
       const int t = lane;
       const bool live = t < n;
       const int i = t % nx + 1, j = t / nx + 1;
@@ -824,7 +850,10 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
         half_c(1);
         half_c(0);
       }
+      if (B.project) mean_free(x, n, true);
     } else {
+      const BotLev &L = B.lev[last];
+      if (B.project) mean_free(bs + L.off, L.nx * L.ny, true);
       zero_w(last);
       for (int s = 0; s < B.K; ++s) {
         half_w(last, 0);
@@ -834,6 +863,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
         half_w(last, 1);
         half_w(last, 0);
       }
+      if (B.project) mean_free(xs + L.off, L.nx * L.ny, true);
     }
     for (int l = last - 1; l >= lw; --l) {
       const int n = B.lev[l].nx * B.lev[l].ny;
@@ -906,6 +936,12 @@ Lev<T> level(Ctx &c, int l, const T *b, T *x) {
 // (fewer iterations on the density-jump workloads measured), series faces
 // with the context's mg_coarse option.
 inline int coarse_scheme(const Ctx &c) { return c.opts.mg_coarse ? 1 : 0; }
+
+// No side with a Dirichlet pressure (BC_NEUMANN): the operator is singular up
+// to the assembly's 1e-12 dvol shift, its null space the constant.
+inline bool singular_operator(const Mesh &m) {
+  return m.bc[0] != BC_NEUMANN && m.bc[1] != BC_NEUMANN && m.bc[2] != BC_NEUMANN && m.bc[3] != BC_NEUMANN;
+}
 
 // Red-black sweeps before and after each coarse correction (1..4, 1 by
 // default).
@@ -1095,6 +1131,7 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
     const Lev<T> Lc = level<T>(c, nl - 1, r0, z0);
     Bt.K = std::min(64, 2 * std::max(Lc.nx, Lc.ny));
     Bt.NU = NU;
+    Bt.project = c.opts.mg_project && singular_operator(c.m);
     Bt.scale = 0.5 * correction_weight(c);
     const Lev<T> Lb = level<T>(c, lb, r0, z0);
     hipLaunchKernelGGL(k_mg_bottom<T>, dim3(1), dim3(1024), 0, c.stream, Bt, Lb.b, Lb.x, (const double *)c.ps.scal,

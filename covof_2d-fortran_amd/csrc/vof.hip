@@ -258,27 +258,59 @@ __device__ void ghost_segment(const DevMesh &m, int i, int j, double c, double g
 
 // Exclusive prefix sum of the per-tile counts (one workgroup of 1024
 // threads, each summing a contiguous run of tiles): offs[t] = first global
-// index of tile t's mixed cells, offs[ntiles] = their total.
+// index of tile t's mixed cells, offs[ntiles] = their total.  A thread's
+// loads are issued eight at a time; the run sums are scanned within each wave
+// by lane shifts, then across the 16 waves through LDS (two barriers).
 __global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ count, int ntiles, int *offs) {
-  __shared__ int part[1024];
+  __shared__ int wsum[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int per = (ntiles + 1023) / 1024;
-  const int t0 = threadIdx.x * per, t1 = min(t0 + per, ntiles);
+  const int t0 = threadIdx.x * per;
   int s = 0;
-  for (int t = t0; t < t1; ++t) s += count[t];
-  part[threadIdx.x] = s;
+  for (int base = 0; base < per; base += 8) {
+    int v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int t = t0 + base + q;
+      v[q] = (base + q < per && t < ntiles) ? count[t] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += v[q];
+  }
+  int inc = s;  // inclusive scan of the run sums within the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) wsum[w] = inc;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  if (w == 0) {
+    int x = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int u = __shfl_up(x, off, 64);
+      if (lane >= off) x += u;
+    }
+    if (lane < 16) wsum[lane] = x;  // inclusive totals of the waves
   }
-  int run = part[threadIdx.x] - s;  // exclusive
-  for (int t = t0; t < t1; ++t) {
-    offs[t] = run;
-    run += count[t];
+  __syncthreads();
+  int run = inc - s + (w > 0 ? wsum[w - 1] : 0);  // exclusive
+  for (int base = 0; base < per; base += 8) {
+    int v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int t = t0 + base + q;
+      v[q] = (base + q < per && t < ntiles) ? count[t] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int t = t0 + base + q;
+      if (base + q < per && t < ntiles) offs[t] = run;
+      run += v[q];
+    }
   }
-  if (threadIdx.x == 1023) offs[ntiles] = part[1023];
+  if (threadIdx.x == 1023) offs[ntiles] = wsum[15];
 }
 
 // METHOD 0 = PLIC, 1 = ELVIRA.  The work is a list of units, each one wave's

@@ -112,6 +112,8 @@ int covof_ctx_set_reduction_order(covof_ctx *ctx, int order);
 #define COVOF_OPT_MIXED_MG_FORM 8      /* PRECISION_MIXED with PRECOND_MG: 0 (default) FP64 CG whose V-cycle
                                           runs in FP32; 1 the FP32 Krylov recurrence with FP64 reliable
                                           updates (the form of the other preconditioners) */
+#define COVOF_OPT_MG_PROJECT 9         /* 1 (default): with a singular operator (no BC_NEUMANN side) the V-cycle's
+                                          coarsest solve is mean-free on both sides; 0: plain sweeps */
 int covof_ctx_set_solver_option(covof_ctx *ctx, int option, double value);
 /* Makes ahead the work arrays a path otherwise makes on its first use, so
  * that the calls of a time step allocate nothing (no reference counterpart;

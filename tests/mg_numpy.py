@@ -11,7 +11,9 @@ faces; a coarse diagonal is its four faces plus eps0 * (level-0 cells
 covered) / 2^l.  Levels with more than 4096 cells (or beyond 6144 cells in
 all) start from x = 0 with the reds set to b / diag; the others start from
 x = 0 with a full Gauss-Seidel red half-sweep.  The coarsest level takes K
-red-black sweeps and K in the adjoint order, K = min(64, 2 max(nx, ny)).
+red-black sweeps and K in the adjoint order, K = min(64, 2 max(nx, ny)); with
+a singular operator its right-hand side and its result are made mean-free
+(a serial sum in cell order).
 """
 import numpy as np
 
@@ -129,7 +131,17 @@ def _prolong(xc, shape):
     return np.repeat(np.repeat(xc, 2, axis=0), 2, axis=1)[:ny, :nx]
 
 
-def vcycle(levels, r, perx, pery, omega=1.8):
+def _serial_mean(a):
+    s = 0.0
+    for v in a.ravel():  # cell order, one accumulator
+        s = s + float(v)
+    return s / a.size
+
+
+def vcycle(levels, r, perx, pery, omega=1.8, project=False):
+    """M r; project: the operator is singular (no Dirichlet pressure side) and
+    the coarsest solve removes the mean of its right-hand side and of its
+    result (P S P)."""
     nl = len(levels)
     cells = [lev[2].size for lev in levels]
     lb = nl - 1
@@ -147,6 +159,8 @@ def vcycle(levels, r, perx, pery, omega=1.8):
         lev = levels[l]
         _, _, dg = lev
         col = _has(dg.shape, perx, pery)[4]
+        if l == nl - 1 and project:
+            b = b - _serial_mean(b)
         if l < lb:
             x = np.where(col == 0, b / dg, 0.0)
         else:
@@ -157,7 +171,7 @@ def vcycle(levels, r, perx, pery, omega=1.8):
                 x = _gs(lev, _gs(lev, x, b, 0, perx, pery), b, 1, perx, pery)
             for s in range(K):
                 x = _gs(lev, _gs(lev, x, b, 1, perx, pery), b, 0, perx, pery)
-            return x
+            return x - _serial_mean(x) if project else x
         x = _gs(lev, x, b, 1, perx, pery)
         bc = _restrict(lev, x, b, scale, perx, pery, levels[l + 1][2].shape)
         x = x + _prolong(cycle(l + 1, bc), dg.shape)

@@ -180,7 +180,7 @@ def test_solver_options_validated(cv):
     from covof_amd import workloads as wl
     ctx = cv.Context(64, 64, bc=(1, 1, 1, 1))
     for opt, bad in ((cv.OPT_MIXED_CYCLE, 3), (cv.OPT_MIXED_CYCLE, 1), (cv.OPT_MIXED_STALL_WINDOW, -1),
-                     (cv.OPT_MIXED_LOOP, 2), (cv.OPT_MG_SWEEPS, 5), (cv.OPT_MG_OMEGA, 2.5), (cv.OPT_MG_COARSE, 0.5), (cv.OPT_MG_FUSED, 2), (cv.OPT_MIXED_MG_FORM, 3),
+                     (cv.OPT_MIXED_LOOP, 2), (cv.OPT_MG_SWEEPS, 5), (cv.OPT_MG_OMEGA, 2.5), (cv.OPT_MG_COARSE, 0.5), (cv.OPT_MG_FUSED, 2), (cv.OPT_MIXED_MG_FORM, 3), (cv.OPT_MG_PROJECT, 2),
                      (99, 1)):
         with pytest.raises(cv.CovofError):
             ctx.set_solver_option(opt, bad)
@@ -251,7 +251,8 @@ def test_mg_vcycle_bitwise_vs_numpy(cv, orc, shape, bc):
     """M r of PRECOND_MG (FP64) bit for bit against the numpy restatement of
     the cycle (tests/mg_numpy.py) on the oracle's assembled operator: the
     hierarchy, the fused multi-workgroup levels, the single-workgroup bottom
-    and its wave-only coarse levels."""
+    and its wave-only coarse levels, and the mean-free coarsest solve of a
+    singular operator (on and off)."""
     import mg_numpy as mgn
     nx, ny = shape
     ctx, (rho_u, rho_v) = _ctx(cv, shape, bc, with_rho=True)
@@ -261,12 +262,15 @@ def test_mg_vcycle_bitwise_vs_numpy(cv, orc, shape, bc):
     kx, ky = mgn.faces_from_assembly(co)
     perx, pery = int(bc[0] == 3), int(bc[2] == 3)
     levels = mgn.hierarchy(kx, ky, co["aP"], perx, pery, 1.0e-12 * (1.0 / nx) * (1.0 / ny))
+    singular = 2 not in bc  # no Dirichlet pressure side
     rng = np.random.default_rng(6)
     for trial in range(2):
         r = rng.normal(size=(ny, nx))
-        zn = mgn.vcycle(levels, r, perx, pery)
-        for fused in (1, 0):
-            ctx.set_solver_option(cv.OPT_MG_FUSED, fused)
-            zg = ctx.poisson_precond(cv.PRECOND_MG, to_gpu(r)).cpu().numpy()
-            bad = np.argwhere(zg != zn)
-            assert bad.size == 0, (fused, len(bad), bad[:4], np.abs(zg - zn).max())
+        for project in ((1, 0) if singular else (1,)):
+            ctx.set_solver_option(cv.OPT_MG_PROJECT, project)
+            zn = mgn.vcycle(levels, r, perx, pery, project=bool(project and singular))
+            for fused in (1, 0):
+                ctx.set_solver_option(cv.OPT_MG_FUSED, fused)
+                zg = ctx.poisson_precond(cv.PRECOND_MG, to_gpu(r)).cpu().numpy()
+                bad = np.argwhere(zg != zn)
+                assert bad.size == 0, (project, fused, len(bad), bad[:4], np.abs(zg - zn).max())
