@@ -335,52 +335,57 @@ constexpr int NR_3 = FT_X * FT_Y / FT_BLK;
 
 // The loads of a fused level pass.  POST: the region's reds also load their
 // coarse cell's x_c.
-template <typename T, bool POST>
+template <typename T, typename TB, bool POST>
 struct TileLoads {
-  T b[NR_1], d[NR_1], xc[NR_1];
+  TB b[NR_1];  // the right-hand side as loaded (rounded to T where it is used,
+               // so no load waits on a conversion)
+  T d[NR_1], xc[NR_1];
   Faces<T> f2[NR_2], f3[NR_3];
   // bin: the level's right-hand side (of type TB: FP64 for the FP32 cycle
-  // that preconditions an FP64 solve, rounded to T on load)
-  template <typename TB>
+  // that preconditions an FP64 solve)
   __device__ __forceinline__ void issue(const Lev<T> &F, const TB *__restrict__ bin, int i0, int j0, int nxc,
                                         const T *__restrict__ xcv) {
+    // Every load is unconditional: a cell that has no value to load (outside
+    // a non-periodic side, or of the other colour) reads cell (1, 1) instead,
+    // and its value is never used (the users test the same conditions).  No
+    // load then sits in a branch whose join would wait for it.
     const int nx = F.nx, ny = F.ny;
 #pragma unroll
     for (int q = 0; q < NR_1; ++q) {
       const int e = threadIdx.x + q * FT_BLK;
       int gi, gj;
-      b[q] = T(0);
-      d[q] = T(1);
-      xc[q] = T(0);
-      if (e < FR_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % FR_X, e / FR_X, gi, gj)) {
-        const long k = (long)(gj - 1) * nx + (gi - 1);
-        b[q] = (T)bin[k];
-        d[q] = F.diag[k];
-        if (POST && ((gi + gj) & 1) == 0) xc[q] = xcv[(long)((gj + 1) / 2 - 1) * nxc + ((gi + 1) / 2 - 1)];
+      const bool ok = e < FR_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % FR_X, e / FR_X, gi, gj);
+      if (!ok) gi = gj = 1;
+      const long k = (long)(gj - 1) * nx + (gi - 1);
+      b[q] = bin[k];
+      d[q] = F.diag[k];
+      if (POST) {
+        const bool red = ok && ((gi + gj) & 1) == 0;
+        xc[q] = xcv[red ? (long)((gj + 1) / 2 - 1) * nxc + ((gi + 1) / 2 - 1) : 0L];
       }
     }
 #pragma unroll
     for (int q = 0; q < NR_2; ++q) {
       const int e = threadIdx.x + q * FT_BLK;
       int gi, gj;
-      f2[q] = Faces<T>{T(0), T(0), T(0), T(0)};
-      if (e < F1_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % F1_X + 1, e / F1_X + 1, gi, gj) &&
-          ((gi + gj) & 1) == 1)
-        f2[q] = load_faces(F, gi, gj);
+      const bool ok = e < F1_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % F1_X + 1, e / F1_X + 1, gi, gj) &&
+                      ((gi + gj) & 1) == 1;
+      if (!ok) gi = gj = 1;
+      f2[q] = load_faces(F, gi, gj);
     }
 #pragma unroll
     for (int q = 0; q < NR_3; ++q) {
       const int e = threadIdx.x + q * FT_BLK;
-      const int gi = i0 + e % FT_X + FT_H, gj = j0 + e / FT_X + FT_H;
-      f3[q] = Faces<T>{T(0), T(0), T(0), T(0)};
-      if (gi <= nx && gj <= ny) f3[q] = load_faces(F, gi, gj);
+      int gi = i0 + e % FT_X + FT_H, gj = j0 + e / FT_X + FT_H;
+      if (gi > nx || gj > ny) gi = gj = 1;
+      f3[q] = load_faces(F, gi, gj);
     }
   }
 };
 
 // The blacks of the region minus its outer ring from the reds in LDS.
-template <typename T, bool POST>
-__device__ __forceinline__ void black_sweep(const Lev<T> &F, int i0, int j0, const TileLoads<T, POST> &ld,
+template <typename T, typename TB, bool POST>
+__device__ __forceinline__ void black_sweep(const Lev<T> &F, int i0, int j0, const TileLoads<T, TB, POST> &ld,
                                             T (*xs)[FR_X], T (*bs)[FR_X], T (*ds)[FR_X]) {
 #pragma unroll
   for (int q = 0; q < NR_2; ++q) {
@@ -405,7 +410,7 @@ __global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restric
   const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = t / ntx;
   const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;  // global cell of region cell (0, 0)
-  TileLoads<T, false> ld;
+  TileLoads<T, TB, false> ld;
   ld.issue(F, bin, i0, j0, nxc, nullptr);
   // the region's b, diag; reds = b / diag (the first half-sweep from x = 0)
 #pragma unroll
@@ -415,9 +420,11 @@ __global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restric
     const int a = e % FR_X, c = e / FR_X;
     int gi, gj;
     T xv = T(0);
-    if (region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 0) xv = ld.b[q] / ld.d[q];
-    bs[c][a] = ld.b[q];
-    ds[c][a] = ld.d[q];
+    const bool ok = region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj);
+    const T bq = ok ? (T)ld.b[q] : T(0), dq = ok ? ld.d[q] : T(1);
+    if (ok && ((gi + gj) & 1) == 0) xv = bq / dq;
+    bs[c][a] = bq;
+    ds[c][a] = dq;
     xs[c][a] = xv;
   }
   __syncthreads();
@@ -465,7 +472,7 @@ __global__ void __launch_bounds__(FT_BLK) k_mg_post(Lev<T> F, const TB *__restri
   const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = t / ntx;
   const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;
-  TileLoads<T, true> ld;
+  TileLoads<T, TB, true> ld;
   ld.issue(F, bin, i0, j0, nxc, xc);
   // reds corrected by the coarse level: the pre-smoothed red b / diag + x_c
 #pragma unroll
@@ -475,10 +482,11 @@ __global__ void __launch_bounds__(FT_BLK) k_mg_post(Lev<T> F, const TB *__restri
     const int a = e % FR_X, c = e / FR_X;
     int gi, gj;
     T xv = T(0);
-    if (region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 0)
-      xv = ld.b[q] / ld.d[q] + ld.xc[q];
-    bs[c][a] = ld.b[q];
-    ds[c][a] = ld.d[q];
+    const bool ok = region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj);
+    const T bq = ok ? (T)ld.b[q] : T(0), dq = ok ? ld.d[q] : T(1);
+    if (ok && ((gi + gj) & 1) == 0) xv = bq / dq + ld.xc[q];
+    bs[c][a] = bq;
+    ds[c][a] = dq;
     xs[c][a] = xv;
   }
   __syncthreads();

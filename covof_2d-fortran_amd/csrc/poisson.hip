@@ -261,6 +261,19 @@ void serial_sums(Ctx &c, long n, const SerialSet &s, int f0, int f1) {
 // a time.  Stencil neighbours left/right and above/below then come from the
 // same XCD's L2 (only the band edges cross), and the rows a block reads as
 // "above" it touched a moment before.
+// CELL_LOOP's partition without the loop: i, the row range [jb_ + 1, je_] and
+// active_ (the block has a piece of the grid).
+#define CELL_BOUNDS(o)                                                                         \
+  const int gx_ = ((o).nx + 63) / 64;                                                          \
+  const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;                                   \
+  const int chunks_ = (gridDim.x >> 3) / gx_;                                                  \
+  const int band_ = ((((o).ny + 7) / 8) + 3) & ~3;                                             \
+  const int rows_ = (((band_ + chunks_ - 1) / chunks_) + 3) & ~3;                              \
+  const int i = (local_ % gx_) * 64 + (threadIdx.x & 63) + 1;                                  \
+  const int jb_ = xcd_ * band_ + (local_ / gx_) * rows_;                                        \
+  const int je_ = min(min(jb_ + rows_, xcd_ * band_ + band_), (o).ny);                          \
+  const bool active_ = local_ < gx_ * chunks_
+
 #define CELL_LOOP(o)                                                                           \
   const int gx_ = ((o).nx + 63) / 64;                                                          \
   const int xcd_ = blockIdx.x & 7, local_ = blockIdx.x >> 3;                                   \
@@ -736,13 +749,39 @@ __global__ void __launch_bounds__(BLK) k_pcg_b(DevOp o, const T *__restrict__ di
     scal[sl.pend] = (k & 1) ? 2.0 : 1.0;  // d of iteration k: d1 when k is odd
   }
   T acc[3] = {T(0), T(0), T(0)};
-  CELL_LOOP(o) {
-    const long q = (long)(j - 1) * o.nx + (i - 1);
-    const T rk = r[q] - alpha * Ad[q];
-    r[q] = rk;
-    acc[0] += rk * rk;
-    if (PC == 0) acc[1] += rk * rk;
-    if (PC == 1) acc[1] += rk * jacobi<T>(rk, diag[q]);
+  // CELL_LOOP's rows, U of a wave's rows per batch with the batch's loads
+  // issued first (the same rows in the same order: the partials are those of
+  // the one-row loop); FP32 loads move half the bytes per instruction, so
+  // FP32 batches four rows
+  constexpr int U = sizeof(T) == 4 ? 4 : 1;
+  CELL_BOUNDS(o);
+  if (active_ && i <= o.nx) {
+    for (int j0 = jb_ + (threadIdx.x >> 6) + 1; j0 <= je_; j0 += 4 * U) {
+      T rv[U], av[U], dv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + 4 * u;
+        rv[u] = av[u] = T(0);
+        dv[u] = T(1);
+        if (j <= je_) {
+          const long q = (long)(j - 1) * o.nx + (i - 1);
+          rv[u] = r[q];
+          av[u] = Ad[q];
+          if (PC == 1) dv[u] = diag[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + 4 * u;
+        if (j > je_) break;
+        const long q = (long)(j - 1) * o.nx + (i - 1);
+        const T rk = rv[u] - alpha * av[u];
+        r[q] = rk;
+        acc[0] += rk * rk;
+        if (PC == 0) acc[1] += rk * rk;
+        if (PC == 1) acc[1] += rk * jacobi<T>(rk, dv[u]);
+      }
+    }
   }
   block_partials<3, T>(acc, pout);
 }

@@ -257,33 +257,31 @@ __device__ void ghost_segment(const DevMesh &m, int i, int j, double c, double g
 // --------------------------------------------------------- reconstruction
 
 // Exclusive prefix sum of the per-tile counts (one workgroup of 1024
-// threads, each summing a contiguous run of tiles): offs[t] = first global
-// index of tile t's mixed cells, offs[ntiles] = their total.  A thread's
-// loads are issued eight at a time; the run sums are scanned within each wave
-// by lane shifts, then across the 16 waves through LDS (two barriers).
+// threads): offs[t] = first global index of tile t's mixed cells,
+// offs[ntiles] = their total.  Each of the 16 waves owns a contiguous run of
+// tiles (a multiple of 64) and reads and writes it 64 consecutive tiles at a
+// time: first the run's total, then -- after the 16 totals are scanned
+// through LDS -- the run's offsets, each chunk scanned by lane shifts and
+// carried to the next.  Integer sums, so any order gives the same offsets.
 __global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ count, int ntiles, int *offs) {
   __shared__ int wsum[16];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int per = (ntiles + 1023) / 1024;
-  const int t0 = threadIdx.x * per;
+  const int per = ((ntiles + 15) / 16 + 63) & ~63;  // tiles per wave
+  const int t0 = w * per, t1 = min(t0 + per, ntiles);
   int s = 0;
-  for (int base = 0; base < per; base += 8) {
-    int v[8];
+  for (int base = t0; base < t1; base += 256) {
+    int v[4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int t = t0 + base + q;
-      v[q] = (base + q < per && t < ntiles) ? count[t] : 0;
+    for (int q = 0; q < 4; ++q) {
+      const int t = base + q * 64 + lane;
+      v[q] = (t < t1) ? count[t] : 0;
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s += v[q];
+    for (int q = 0; q < 4; ++q) s += v[q];
   }
-  int inc = s;  // inclusive scan of the run sums within the wave
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int u = __shfl_up(inc, off, 64);
-    if (lane >= off) inc += u;
-  }
-  if (lane == 63) wsum[w] = inc;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) wsum[w] = s;
   __syncthreads();
   if (w == 0) {
     int x = lane < 16 ? wsum[lane] : 0;
@@ -295,19 +293,25 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int *__restrict__ coun
     if (lane < 16) wsum[lane] = x;  // inclusive totals of the waves
   }
   __syncthreads();
-  int run = inc - s + (w > 0 ? wsum[w - 1] : 0);  // exclusive
-  for (int base = 0; base < per; base += 8) {
-    int v[8];
+  int carry = w > 0 ? wsum[w - 1] : 0;
+  for (int base = t0; base < t1; base += 256) {
+    int v[4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int t = t0 + base + q;
-      v[q] = (base + q < per && t < ntiles) ? count[t] : 0;
+    for (int q = 0; q < 4; ++q) {  // four chunks' loads first
+      const int t = base + q * 64 + lane;
+      v[q] = (t < t1) ? count[t] : 0;
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int t = t0 + base + q;
-      if (base + q < per && t < ntiles) offs[t] = run;
-      run += v[q];
+    for (int q = 0; q < 4; ++q) {
+      const int t = base + q * 64 + lane;
+      int inc = v[q];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+      }
+      if (t < t1) offs[t] = carry + inc - v[q];
+      carry += __shfl(inc, 63, 64);
     }
   }
   if (threadIdx.x == 1023) offs[ntiles] = wsum[15];
@@ -612,21 +616,36 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   double uf[NR], cpos[NR], cneg[NR];
   int fis[NR], fjs[NR];
   bool ok[NR];
+  // (loads are unconditional, at the face's own index -- a face outside the
+  // tile's range reads face (1, 1) and is never used -- so no load waits on
+  // another; a periodic edge face, whose flux comes from the real donor's
+  // face, is loaded again at that face below)
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const int f = r * TBLK + threadIdx.x;
     const int fr = f / FW;
-    int fi = i0 + (f - fr * FW), fj = j0 + fr;
+    const int fi = i0 + (f - fr * FW), fj = j0 + fr;
     ok[r] = f < NF && fi <= imax && fj <= jmax;
-    uf[r] = cpos[r] = cneg[r] = 0.0;
-    if (ok[r]) {
+    const int li = ok[r] ? fi : 1, lj = ok[r] ? fj : 1;
+    uf[r] = vel[H(m, li, lj)];
+    cpos[r] = DIR == 0 ? src[H(m, li - 1, lj)] : src[H(m, li, lj - 1)];
+    cneg[r] = src[H(m, li, lj)];
+    fis[r] = fi;
+    fjs[r] = fj;
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    int fi = fis[r], fj = fjs[r];
+    const bool edge = ok[r] && (DIR == 0 ? (m.bcl == BC_PERIODIC && (fi == 1 || fi == m.nx + 1))
+                                         : (m.bcb == BC_PERIODIC && (fj == 1 || fj == m.ny + 1)));
+    if (edge) {
       pass_face<DIR>(vel, m, fi, fj);
       uf[r] = vel[H(m, fi, fj)];
       cpos[r] = DIR == 0 ? src[H(m, fi - 1, fj)] : src[H(m, fi, fj - 1)];
       cneg[r] = src[H(m, fi, fj)];
+      fis[r] = fi;
+      fjs[r] = fj;
     }
-    fis[r] = fi;
-    fjs[r] = fj;
   }
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
