@@ -230,6 +230,7 @@ __device__ __forceinline__ bool ghost_zero(const DevMesh &m, int i, int j) {
 }
 
 __device__ __forceinline__ int ring_len(const DevMesh &m) { return 2 * (m.nx + 2) + 2 * m.ny; }
+static inline int ring_cells(const Mesh &m) { return 2 * (m.nx + 2) + 2 * m.ny; }
 __device__ __forceinline__ void ring_cell(const DevMesh &m, int r, int &i, int &j) {
   const int W = m.nx + 2;
   if (r < W) { i = r; j = 0; }
@@ -1297,14 +1298,44 @@ __global__ void k_adj_ghost(DevMesh m, const double *__restrict__ src, const dou
   ring[3 * r + 1] = (nby - dot * ny) / s;
 }
 
+// Whether interior cell (i, j) of a ring entry's wrap / clamp target is one
+// of the four corner cells, the only targets more than one ring cell maps to
+// (on meshes of at least 2 x 2 cells): the others take one contribution each,
+// in any order; the corner targets take theirs in ring order, as one serial
+// loop would.
+__device__ __forceinline__ bool corner_target(const DevMesh &m, int ti, int tj) {
+  return (ti == 1 || ti == m.nx) && (tj == 1 || tj == m.ny);
+}
+__device__ __forceinline__ bool ring_serial_only(const DevMesh &m) { return m.nx < 2 || m.ny < 2; }
+
 __global__ void k_adj_ghost_acc(DevMesh m, const double *__restrict__ ring, double *__restrict__ sb) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const long nh = (long)(m.nx + 2) * (m.ny + 2);
   const int L = ring_len(m);
-  for (int r = 0; r < L; ++r) {
-    const long ks = (long)ring[3 * r + 2];
-    sb[ks] += ring[3 * r];
-    sb[nh + ks] += ring[3 * r + 1];
+  const bool serial = ring_serial_only(m);
+  auto target_of = [&](int r, int &ti, int &tj) {
+    int i, j;
+    ring_cell(m, r, i, j);
+    ti = wrap_or_clamp(i, m.nx, m.perx);
+    tj = wrap_or_clamp(j, m.ny, m.pery);
+  };
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!serial && r < L) {
+    int ti, tj;
+    target_of(r, ti, tj);
+    if (!corner_target(m, ti, tj)) {
+      const long ks = (long)ring[3 * r + 2];
+      sb[ks] += ring[3 * r];
+      sb[nh + ks] += ring[3 * r + 1];
+    }
+  }
+  if (r != 0) return;
+  for (int q = 0; q < L; ++q) {  // the corner targets (every target on a degenerate mesh), in ring order
+    int ti, tj;
+    target_of(q, ti, tj);
+    if (!serial && !corner_target(m, ti, tj)) continue;
+    const long ks = (long)ring[3 * q + 2];
+    sb[ks] += ring[3 * q];
+    sb[nh + ks] += ring[3 * q + 1];
   }
 }
 
@@ -1437,19 +1468,27 @@ __global__ void k_adj_gather(DevMesh m, const double *__restrict__ W, double *__
 // corners == 0 leaves the four corner entries in place (their values came
 // from another field).
 __global__ void k_adj_fold(DevMesh m, double *__restrict__ cbar, int corners) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nx = m.nx, ny = m.ny;
   const int L = ring_len(m);
-  for (int r = 0; r < L; ++r) {
+  const bool serial = ring_serial_only(m);
+  // ring cell q moved onto its target; false: a skipped corner, or (with
+  // `corner_pass`) a target another pass handles
+  auto fold = [&](int q, bool corner_pass) {
     int i, j;
-    ring_cell(m, r, i, j);
+    ring_cell(m, q, i, j);
     const bool corner = (i == 0 || i == nx + 1) && (j == 0 || j == ny + 1);
-    if (corner && !corners) continue;
+    if (corner && !corners) return;
+    const int ti = wrap_or_clamp(i, nx, m.perx), tj = wrap_or_clamp(j, ny, m.pery);
+    if (!serial && corner_target(m, ti, tj) != corner_pass) return;
     const long k = H(m, i, j);
     const double v = cbar[k];
     cbar[k] = 0.0;
-    cbar[H(m, wrap_or_clamp(i, nx, m.perx), wrap_or_clamp(j, ny, m.pery))] += v;
-  }
+    cbar[H(m, ti, tj)] += v;
+  };
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!serial && r < L) fold(r, false);
+  if (r != 0) return;
+  for (int q = 0; q < L; ++q) fold(q, true);  // the corner targets, in ring order
 }
 
 // Halo of C as the transport leaves it with the two-phase corner fix, from
@@ -1502,7 +1541,8 @@ static int adj_pass(Ctx &c, int method, const double *src, const double *Cn, con
   hipLaunchKernelGGL(k_adj_segbar<DIR>, grd, blk, 0, c.stream, m, vel, (const double *)fs, dt, sb);
   hipLaunchKernelGGL(k_adj_ghost, dim3((L + 255) / 256), dim3(256), 0, c.stream, m, src, sx, sy, sa,
                      (const double *)sb, cbar_in, ring);
-  hipLaunchKernelGGL(k_adj_ghost_acc, dim3(1), dim3(64), 0, c.stream, m, (const double *)ring, sb);
+  hipLaunchKernelGGL(k_adj_ghost_acc, dim3((ring_cells(c.m) + 255) / 256), dim3(256), 0, c.stream, m,
+                     (const double *)ring, sb);
   hipLaunchKernelGGL(method == 1 ? k_adj_interior<1> : k_adj_interior<0>, grd_in, blk, 0, c.stream, m, src, sx, sy,
                      sa, (const double *)sb, cbar_in, W);
   hipLaunchKernelGGL(k_adj_gather, grd, blk, 0, c.stream, m, (const double *)W, cbar_in);
@@ -1541,7 +1581,8 @@ int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, con
   if (st) return st;
   // second pass backwards: dJ/dC^{n+1} (its halo onto the interior it copies)
   COVOF_CHECK(hipMemcpyAsync(cout, cbar_new, nh * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
-  hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cout, (per || two_phase) ? 1 : 0);
+  hipLaunchKernelGGL(k_adj_fold, dim3((ring_cells(c.m) + 255) / 256), dim3(256), 0, c.stream, m, cout,
+                     (per || two_phase) ? 1 : 0);
   COVOF_CHECK_LAUNCH();
   const double *Cs = c.vof.Cb;
   if (x_first)
@@ -1552,7 +1593,7 @@ int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, con
                      cmid, ubar);
   if (st) return st;
   // C*'s halo: edges copied from its interior, corners from C's (walls)
-  hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cmid, per);
+  hipLaunchKernelGGL(k_adj_fold, dim3((ring_cells(c.m) + 255) / 256), dim3(256), 0, c.stream, m, cmid, per);
   COVOF_CHECK_LAUNCH();
   if (x_first)
     st = adj_pass<0>(c, method, C, C, u, seg1, seg1 + nh, seg1 + 2 * nh, cmid, 0, per ? 0 : 1, dt, cbar, ubar);
@@ -1560,7 +1601,7 @@ int vof_advect_adjoint(Ctx &c, int method, const double *C, const double *u, con
     st = adj_pass<1>(c, method, C, C, v, seg1, seg1 + nh, seg1 + 2 * nh, cmid, 0, per ? 0 : 1, dt, cbar, vbar);
   if (st) return st;
   if (fold_in) {
-    hipLaunchKernelGGL(k_adj_fold, dim3(1), dim3(64), 0, c.stream, m, cbar, 1);
+    hipLaunchKernelGGL(k_adj_fold, dim3((ring_cells(c.m) + 255) / 256), dim3(256), 0, c.stream, m, cbar, 1);
     COVOF_CHECK_LAUNCH();
   }
   return COVOF_OK;
