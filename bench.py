@@ -354,8 +354,8 @@ def main():
     upd = vctx.profile_read(L.KERNEL_VOF_UPDATE)
     vctx.profile(0)
     vof = {
-        "workload": f"ELVIRA + Weymouth-Yue, {vn}x{vn} periodic, 128 seeded ellipses, CFL 0.25, "
-                    f"{args.vof_steps} steps",
+        "workload": f"ELVIRA + Weymouth-Yue, {vn}x{vn} periodic, 128 seeded ellipses (exact cell fractions), "
+                    f"CFL 0.25, {args.vof_steps} steps",
         "cell_updates_per_s": vn * vn / t_vof,
         "ms_per_step": t_vof * 1e3,
         "algorithmic_GBps": 50.0 * vn * vn / t_vof / 1e9,
@@ -385,8 +385,8 @@ def main():
         torch.cuda.synchronize()
         t_big = (time.perf_counter() - t) / 100
         w1 = Cg[1:vbig + 1, 1:vbig + 1].sum().item()
-        vof["n4096"] = {"workload": "ELVIRA + Weymouth-Yue, 4096x4096 periodic, 128 seeded ellipses, CFL 0.25, "
-                                    "100 steps",
+        vof["n4096"] = {"workload": "ELVIRA + Weymouth-Yue, 4096x4096 periodic, 128 seeded ellipses (exact cell "
+                                    "fractions), CFL 0.25, 100 steps",
                         "ms_per_step": t_big * 1e3, "cell_updates_per_s": vbig * vbig / t_big,
                         "algorithmic_GBps": 50.0 * vbig * vbig / t_big / 1e9, "volume_drift_rel": abs(w1 - w0) / w0}
         log(f"VOF 4096^2: {t_big * 1e3:.3f} ms/step, {vbig * vbig / t_big:.3e} cell-updates/s")

@@ -70,18 +70,81 @@ def _ellipse_fraction(C, nx, ny, dx, dy, xc, yc, a, b, th, sub=32):
     C[j0:j1 + 1, i0:i1 + 1] += frac
 
 
-def ellipses(nx, ny, n, semi_min_cells, semi_max_cells, seed, lx=1.0, ly=1.0):
-    """Clamped union of ``n`` seeded random ellipses; interior only (halo zero)."""
+def _disc_polygon_area(px, py):
+    """Area of the unit disc inside the polygon with vertices (px[k], py[k])
+    (arrays of shape (K, ...), counter-clockwise): the sum over the edges P->Q
+    of the signed area of the disc inside the triangle (0, P, Q) -- where the
+    segment runs inside the disc, the triangle's area 0.5 P x Q; outside it,
+    the circular sector 0.5 angle(P, Q)."""
+    def ang(ux, uy, vx, vy):
+        return np.arctan2(ux * vy - uy * vx, ux * vx + uy * vy)
+
+    total = np.zeros(px.shape[1:])
+    K = px.shape[0]
+    for k in range(K):
+        Px, Py, Qx, Qy = px[k], py[k], px[(k + 1) % K], py[(k + 1) % K]
+        dx, dy = Qx - Px, Qy - Py
+        A = dx * dx + dy * dy
+        B = 2.0 * (Px * dx + Py * dy)
+        Cq = Px * Px + Py * Py - 1.0
+        disc = B * B - 4.0 * A * Cq
+        cross = disc > 0.0
+        sq = np.sqrt(np.where(cross, disc, 0.0))
+        A2 = np.where(A > 0.0, 2.0 * A, 1.0)
+        t1 = np.where(cross, np.clip((-B - sq) / A2, 0.0, 1.0), 1.0)
+        t2 = np.where(cross, np.clip((-B + sq) / A2, 0.0, 1.0), 1.0)
+        m1x, m1y = Px + t1 * dx, Py + t1 * dy
+        m2x, m2y = Px + t2 * dx, Py + t2 * dy
+        total += 0.5 * (ang(Px, Py, m1x, m1y) + (m1x * m2y - m1y * m2x) + ang(m2x, m2y, Qx, Qy))
+    return total
+
+
+def ellipse_cell_fractions(x0, x1, y0, y1, xc, yc, a, b, th):
+    """Exact area fraction of each cell [x0, x1] x [y0, y1] inside the ellipse
+    of semi-axes a, b rotated by th about (xc, yc): the map
+    q = diag(1/a, 1/b) R(th)^T (p - c) takes the ellipse to the unit disc and
+    the cell to a parallelogram, whose area inside the disc, times a b, is the
+    cell's covered area."""
+    ct, st = math.cos(th), math.sin(th)
+    xs = np.stack([x0, x1, x1, x0]) - xc
+    ys = np.stack([y0, y0, y1, y1]) - yc
+    qx = (ct * xs + st * ys) / a
+    qy = (-st * xs + ct * ys) / b
+    area = _disc_polygon_area(qx, qy) * (a * b)
+    f = area / ((x1 - x0) * (y1 - y0))
+    return np.where(f < 1e-12, 0.0, np.where(f > 1.0 - 1e-12, 1.0, f))  # round-off of the closed form
+
+
+def _ellipse_fraction_exact(C, nx, ny, dx, dy, xc, yc, a, b, th):
+    """Add the exact area fraction of one rotated ellipse to the interior of ``C``."""
+    ct_, st_ = math.cos(th), math.sin(th)
+    ext_x = math.sqrt((a * ct_) ** 2 + (b * st_) ** 2)
+    ext_y = math.sqrt((a * st_) ** 2 + (b * ct_) ** 2)
+    i0 = max(1, int(math.floor((xc - ext_x) / dx)))
+    i1 = min(nx, int(math.floor((xc + ext_x) / dx)) + 2)
+    j0 = max(1, int(math.floor((yc - ext_y) / dy)))
+    j1 = min(ny, int(math.floor((yc + ext_y) / dy)) + 2)
+    if i0 > i1 or j0 > j1:
+        return
+    X0, Y0 = np.meshgrid((np.arange(i0, i1 + 1) - 1.0) * dx, (np.arange(j0, j1 + 1) - 1.0) * dy)
+    C[j0:j1 + 1, i0:i1 + 1] += ellipse_cell_fractions(X0, X0 + dx, Y0, Y0 + dy, xc, yc, a, b, th)
+
+
+def ellipses(nx, ny, n, semi_min_cells, semi_max_cells, seed, lx=1.0, ly=1.0, exact=False):
+    """Clamped union of ``n`` seeded random ellipses; interior only (halo zero).
+    Each ellipse's cell fractions by 32-point sub-column quadrature, or its
+    exact cell areas (``exact``)."""
     rng = np.random.default_rng(seed)
     dx, dy = lx / nx, ly / ny
     C = np.zeros((ny + 2, nx + 2))
+    add = _ellipse_fraction_exact if exact else _ellipse_fraction
     for _ in range(n):
         a = rng.uniform(semi_min_cells, semi_max_cells) * dx
         b = rng.uniform(semi_min_cells, semi_max_cells) * dx
         th = rng.uniform(0.0, math.pi)
         xc = rng.uniform(0.0, lx)
         yc = rng.uniform(0.0, ly)
-        _ellipse_fraction(C, nx, ny, dx, dy, xc, yc, a, b, th)
+        add(C, nx, ny, dx, dy, xc, yc, a, b, th)
     np.clip(C, 0.0, 1.0, out=C)
     return C
 
@@ -151,9 +214,10 @@ def stream_velocity(nx, ny, nmodes, seed, lx=1.0, ly=1.0, kmax=4, periodic=True,
     return u, v
 
 
-def vof_case(n, seed=0, n_ellipses=128, semi=(20.0, 200.0), nmodes=8, cfl=0.25):
-    """Periodic ``n x n`` VOF advection case; returns ``(C, u, v, dt)``."""
-    C = ellipses(n, n, n_ellipses, semi[0], semi[1], seed)
+def vof_case(n, seed=0, n_ellipses=128, semi=(20.0, 200.0), nmodes=8, cfl=0.25, exact=True):
+    """Periodic ``n x n`` VOF advection case (the ellipses' cell fractions
+    exact); returns ``(C, u, v, dt)``."""
+    C = ellipses(n, n, n_ellipses, semi[0], semi[1], seed, exact=exact)
     fill_halo_periodic(C, n, n)
     u, v = stream_velocity(n, n, nmodes, seed + 1, periodic=True)
     vmax = max(np.abs(u).max(), np.abs(v).max())
