@@ -326,33 +326,40 @@ __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const 
     if (i == u_nx && bl) {
       return m.bc[1] == BC_NEUMANN ? diag_bc * (TU(u_nx, j) - TU(u_nx - 1, j)) : diag_bc * TU(u_nx, j);
     }
-    double val = (at * rho_u / dt) * TU(i, j) * dvol;
+    // every operand is loaded before the row is formed, without a condition
+    // (a stencil cell the row does not have reads the row's own cell and is
+    // not used), so the loads are in flight together
+    const double tuP = TU(i, j), tuE = TU(i + 1, j), tuW = TU(i - 1, j), tuN = TU(i, j + 1), tuS = TU(i, j - 1);
+    const double tvSW = TV(i - 1, j), tvSE = TV(i, j), tvNW = TV(i - 1, j + 1), tvNE = TV(i, j + 1);
+    const double mu_E = F.mu[HX(i, j)], mu_W = F.mu[HX(i - 1, j)];
+    const double mnN = F.mn[HX(i, j + 1)], mnS = F.mn[HX(i, j)];
+    const double uW = u[HX(i - 1, j)], uP = u[HX(i, j)], uE = u[HX(i + 1, j)];
+    const double uS = u[HX(i, j - 1)], uN = u[HX(i, j + 1)];
+    const double r_ee = u[HX(has_ee ? i + 2 : i, j)], r_ww = u[HX(has_ww ? i - 2 : i, j)];
+    const double r_nn = u[HX(i, has_nn ? j + 2 : j)], r_ss = u[HX(i, has_ss ? j - 2 : j)];
+    double val = (at * rho_u / dt) * tuP * dvol;
     if (c.conv) {
       double cE, cW, cN, cS;
       u_fluxes(m, c, F, i, j, cE, cW, cN, cS);
-      const double q_ee = has_ee ? u[HX(i + 2, j)] : 0.0, q_ww = has_ww ? u[HX(i - 2, j)] : 0.0;
-      const double q_nn = has_nn ? u[HX(i, j + 2)] : 0.0, q_ss = has_ss ? u[HX(i, j - 2)] : 0.0;
-      const double uW = u[HX(i - 1, j)], uP = u[HX(i, j)], uE = u[HX(i + 1, j)];
-      const double uS = u[HX(i, j - 1)], uN = u[HX(i, j + 1)];
-      const double pE = conv_face_flux(sch, cE, TU(i, j), TU(i + 1, j), uW, uP, uE, q_ee, true, has_ee);
-      const double pW = conv_face_flux(sch, cW, TU(i - 1, j), TU(i, j), q_ww, uW, uP, uE, has_ww, true);
-      const double pN = conv_face_flux(sch, cN, TU(i, j), TU(i, j + 1), uS, uP, uN, q_nn, true, has_nn);
-      const double pS = conv_face_flux(sch, cS, TU(i, j - 1), TU(i, j), q_ss, uS, uP, uN, has_ss, true);
+      const double q_ee = has_ee ? r_ee : 0.0, q_ww = has_ww ? r_ww : 0.0;
+      const double q_nn = has_nn ? r_nn : 0.0, q_ss = has_ss ? r_ss : 0.0;
+      const double pE = conv_face_flux(sch, cE, tuP, tuE, uW, uP, uE, q_ee, true, has_ee);
+      const double pW = conv_face_flux(sch, cW, tuW, tuP, q_ww, uW, uP, uE, has_ww, true);
+      const double pN = conv_face_flux(sch, cN, tuP, tuN, uS, uP, uN, q_nn, true, has_nn);
+      const double pS = conv_face_flux(sch, cS, tuS, tuP, q_ss, uS, uP, uN, has_ss, true);
       val = val + rho_u * (pE - pW + pN - pS);
       if (c.lin == 3) {
         const double du_dx = (uE - uW) / (2.0 * dx);
         const double du_dy = (uN - uS) / (2.0 * dy);
-        const double tvf = 0.25 * (TV(i - 1, j) + TV(i, j) + TV(i - 1, j + 1) + TV(i, j + 1));
-        val = val + rho_u * (TU(i, j) * du_dx + tvf * du_dy) * dvol;
+        const double tvf = 0.25 * (tvSW + tvSE + tvNW + tvNE);
+        val = val + rho_u * (tuP * du_dx + tvf * du_dy) * dvol;
       }
     }
-    const double mu_E = F.mu[HX(i, j)], mu_W = F.mu[HX(i - 1, j)];
-    const double deE = 2.0 * mu_E * (TU(i + 1, j) - TU(i, j)) / dx * dy;
-    const double deW = 2.0 * mu_W * (TU(i, j) - TU(i - 1, j)) / dx * dy;
+    const double deE = 2.0 * mu_E * (tuE - tuP) / dx * dy;
+    const double deW = 2.0 * mu_W * (tuP - tuW) / dx * dy;
     val = val - (deE - deW);
-    const double mnN = F.mn[HX(i, j + 1)], mnS = F.mn[HX(i, j)];
-    const double shN = mnN * ((TU(i, j + 1) - TU(i, j)) / dy + (TV(i, j + 1) - TV(i - 1, j + 1)) / dx) * dx;
-    const double shS = mnS * ((TU(i, j) - TU(i, j - 1)) / dy + (TV(i, j) - TV(i - 1, j)) / dx) * dx;
+    const double shN = mnN * ((tuN - tuP) / dy + (tvNE - tvNW) / dx) * dx;
+    const double shS = mnS * ((tuP - tuS) / dy + (tvSE - tvSW) / dx) * dx;
     val = val - (shN - shS);
     if (c.mono) val = val + (tp[HX(i, j)] - tp[HX(i - 1, j)]) * dy;  // B_u^T p
     return val;
@@ -363,33 +370,37 @@ __device__ __forceinline__ double mom_row(const Mesh &m, const MomCfg &c, const 
     if ((j == 1 && bb && wallbc(m.bc[2])) || (j == v_ny && bb && wallbc(m.bc[3]))) {
       return diag_bc * TV(i, j);
     }
-    double val = (at * rho_v / dt) * TV(i, j) * dvol;
+    const double tvP = TV(i, j), tvE = TV(i + 1, j), tvW = TV(i - 1, j), tvN = TV(i, j + 1), tvS = TV(i, j - 1);
+    const double tuSW = TU(i, j - 1), tuSE = TU(i + 1, j - 1), tuNW = TU(i, j), tuNE = TU(i + 1, j);
+    const double mu_N = F.mu[HX(i, j)], mu_S = F.mu[HX(i, j - 1)];
+    const double mnE = F.mn[HX(i + 1, j)], mnW = F.mn[HX(i, j)];
+    const double vW = v[HX(i - 1, j)], vP = v[HX(i, j)], vE = v[HX(i + 1, j)];
+    const double vS = v[HX(i, j - 1)], vN = v[HX(i, j + 1)];
+    const double r_ee = v[HX(has_ee ? i + 2 : i, j)], r_ww = v[HX(has_ww ? i - 2 : i, j)];
+    const double r_nn = v[HX(i, has_nn ? j + 2 : j)], r_ss = v[HX(i, has_ss ? j - 2 : j)];
+    double val = (at * rho_v / dt) * tvP * dvol;
     if (c.conv) {
       double cE, cW, cN, cS;
       v_fluxes(m, c, F, i, j, cE, cW, cN, cS);
-      const double q_ee = has_ee ? v[HX(i + 2, j)] : 0.0, q_ww = has_ww ? v[HX(i - 2, j)] : 0.0;
-      const double q_nn = has_nn ? v[HX(i, j + 2)] : 0.0, q_ss = has_ss ? v[HX(i, j - 2)] : 0.0;
-      const double vW = v[HX(i - 1, j)], vP = v[HX(i, j)], vE = v[HX(i + 1, j)];
-      const double vS = v[HX(i, j - 1)], vN = v[HX(i, j + 1)];
-      const double pE = conv_face_flux(sch, cE, TV(i, j), TV(i + 1, j), vW, vP, vE, q_ee, true, has_ee);
-      const double pW = conv_face_flux(sch, cW, TV(i - 1, j), TV(i, j), q_ww, vW, vP, vE, has_ww, true);
-      const double pN = conv_face_flux(sch, cN, TV(i, j), TV(i, j + 1), vS, vP, vN, q_nn, true, has_nn);
-      const double pS = conv_face_flux(sch, cS, TV(i, j - 1), TV(i, j), q_ss, vS, vP, vN, has_ss, true);
+      const double q_ee = has_ee ? r_ee : 0.0, q_ww = has_ww ? r_ww : 0.0;
+      const double q_nn = has_nn ? r_nn : 0.0, q_ss = has_ss ? r_ss : 0.0;
+      const double pE = conv_face_flux(sch, cE, tvP, tvE, vW, vP, vE, q_ee, true, has_ee);
+      const double pW = conv_face_flux(sch, cW, tvW, tvP, q_ww, vW, vP, vE, has_ww, true);
+      const double pN = conv_face_flux(sch, cN, tvP, tvN, vS, vP, vN, q_nn, true, has_nn);
+      const double pS = conv_face_flux(sch, cS, tvS, tvP, q_ss, vS, vP, vN, has_ss, true);
       val = val + rho_v * (pE - pW + pN - pS);
       if (c.lin == 3) {
         const double dv_dx = (vE - vW) / (2.0 * dx);
         const double dv_dy = (vN - vS) / (2.0 * dy);
-        const double tuf = 0.25 * (TU(i, j - 1) + TU(i + 1, j - 1) + TU(i, j) + TU(i + 1, j));
-        val = val + rho_v * (tuf * dv_dx + TV(i, j) * dv_dy) * dvol;
+        const double tuf = 0.25 * (tuSW + tuSE + tuNW + tuNE);
+        val = val + rho_v * (tuf * dv_dx + tvP * dv_dy) * dvol;
       }
     }
-    const double mu_N = F.mu[HX(i, j)], mu_S = F.mu[HX(i, j - 1)];
-    const double deN = 2.0 * mu_N * (TV(i, j + 1) - TV(i, j)) / dy * dx;
-    const double deS = 2.0 * mu_S * (TV(i, j) - TV(i, j - 1)) / dy * dx;
+    const double deN = 2.0 * mu_N * (tvN - tvP) / dy * dx;
+    const double deS = 2.0 * mu_S * (tvP - tvS) / dy * dx;
     val = val - (deN - deS);
-    const double mnE = F.mn[HX(i + 1, j)], mnW = F.mn[HX(i, j)];
-    const double shE = mnE * ((TV(i + 1, j) - TV(i, j)) / dx + (TU(i + 1, j) - TU(i + 1, j - 1)) / dy) * dy;
-    const double shW = mnW * ((TV(i, j) - TV(i - 1, j)) / dx + (TU(i, j) - TU(i, j - 1)) / dy) * dy;
+    const double shE = mnE * ((tvE - tvP) / dx + (tuNE - tuSE) / dy) * dy;
+    const double shW = mnW * ((tvP - tvW) / dx + (tuNW - tuSW) / dy) * dy;
     val = val - (shE - shW);
     if (c.mono) val = val + (tp[HX(i, j)] - tp[HX(i, j - 1)]) * dx;  // B_v^T p
     return val;
