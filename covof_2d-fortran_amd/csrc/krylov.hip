@@ -45,15 +45,27 @@ __global__ void __launch_bounds__(BLK) k_dots(long n, Pairs P, double *part, uns
   double acc[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) acc[p] = 0.0;
-  for (long t = (long)blockIdx.x * BLK + threadIdx.x; t < n; t += (long)gridDim.x * BLK) {
-    double x[NV];
+  // the grid-stride loop four strides at a time, the four elements' loads
+  // issued together (the same elements in the same order as one stride at a
+  // time)
+  const long stride = (long)gridDim.x * BLK;
+  for (long t0 = (long)blockIdx.x * BLK + threadIdx.x; t0 < n; t0 += 4 * stride) {
+    double x[4][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) x[k] = P.v[k][t];
-    int p = 0;
+    for (int u = 0; u < 4; ++u) {
+      const long t = t0 + u * stride;
 #pragma unroll
-    for (int a = 0; a < NV; ++a)
+      for (int k = 0; k < NV; ++k) x[u][k] = t < n ? P.v[k][t] : 0.0;
+    }
 #pragma unroll
-      for (int b = a; b < NV; ++b) acc[p++] += x[a] * x[b];
+    for (int u = 0; u < 4; ++u) {
+      if (t0 + u * stride >= n) break;
+      int p = 0;
+#pragma unroll
+      for (int a = 0; a < NV; ++a)
+#pragma unroll
+        for (int b = a; b < NV; ++b) acc[p++] += x[u][a] * x[u][b];
+    }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
