@@ -161,18 +161,37 @@ template <int NV>
 __device__ void grid_totals(const double *part, int nblk, double (&tot)[NV], int q0 = 0, int q1 = NV) {
   __shared__ double sh[NV][BLK / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  // every slot's loads of a round are issued before any is added (a thread's
+  // pairs of a slot are still added in index order)
+  constexpr int U = 2;  // pairs per thread and slot per round: 1024 partials in one round
+  double t[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) t[q] = 0.0;
+  for (int b0 = threadIdx.x; 2 * b0 < nblk; b0 += U * BLK) {
+    double2 v[NV][U];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double2 *p2 = reinterpret_cast<const double2 *>(part + (long)q * nblk);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int b = b0 + u * BLK;
+        v[q][u] = (q >= q0 && q < q1 && 2 * b < nblk) ? p2[b] : make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (2 * (b0 + u * BLK) < nblk) {
+          t[q] += v[q][u].x;
+          t[q] += v[q][u].y;
+        }
+  }
 #pragma unroll
   for (int q = 0; q < NV; ++q) {
     if (q < q0 || q >= q1) continue;
-    const double2 *p2 = reinterpret_cast<const double2 *>(part + (long)q * nblk);
-    double t = 0.0;
-    for (int b = threadIdx.x; 2 * b < nblk; b += BLK) {
-      const double2 v = p2[b];
-      t += v.x;
-      t += v.y;
-    }
-    t = wave_sum<double>(t);
-    if (lane == 0) sh[q][wid] = t;
+    const double w = wave_sum<double>(t[q]);
+    if (lane == 0) sh[q][wid] = w;
   }
   __syncthreads();
 #pragma unroll
