@@ -44,7 +44,15 @@ __host__ __device__ __forceinline__ long pitch_of(int nx) { return ((long)(nx + 
 // The solve's stop flags (two scalar slots; a launch outside a solve names a
 // slot that is never set).
 __device__ __forceinline__ bool stopped(const double *scal, int f0, int f1) {
-  return scal[f0] != 0.0 || scal[f1] != 0.0;
+  const double a = scal[f0], b = scal[f1];  // both loads issued together
+  return (a != 0.0) | (b != 0.0);
+}
+
+// Keeps the loads issued above it in place (the compiler would otherwise sink
+// them past a following early return, behind the stop flags' load).
+// (p, q: the restrict-qualified arrays among them, which the barrier must see)
+__device__ __forceinline__ void loads_stay_above(const void *p, const void *q) {
+  asm volatile("" ::"s"(p), "s"(q) : "memory");
 }
 
 // Gauss-Seidel value of cell (i, j) from the current x of its neighbours.
@@ -266,7 +274,6 @@ __global__ void __launch_bounds__(256) k_mg_prolong_smooth(Lev<T> F, int nxc, co
 constexpr int FT_X = 64, FT_Y = 16, FT_H = 2;
 constexpr int FR_X = FT_X + 2 * FT_H, FR_Y = FT_Y + 2 * FT_H;
 constexpr int FR_N = FR_X * FR_Y;
-constexpr int F1_X = FR_X - 2, F1_N = (FR_X - 2) * (FR_Y - 2);  // region minus its outer ring
 constexpr int FT_BLK = 256;
 static_assert((FT_X / 2) * (FT_Y / 2) == FT_BLK, "one thread per coarse cell of the tile");
 
@@ -283,226 +290,320 @@ __device__ __forceinline__ int mg_xcd_tile(int b, int nt) {
   return x * q + min(x, rem) + (b >> 3);
 }
 
-// Global cell of region cell (a, c) of the tile whose first region cell is
-// (i0, j0); false: outside a non-periodic side.
-__device__ __forceinline__ bool region_cell(int nx, int ny, int perx, int pery, int i0, int j0, int a, int c,
-                                            int &gi, int &gj) {
-  gi = i0 + a;
-  gj = j0 + c;
-  bool ok = true;
-  if (gi < 1 || gi > nx) {
-    if (perx) gi = wrap1(gi, nx);
-    else ok = false;
-  }
-  if (gj < 1 || gj > ny) {
-    if (pery) gj = wrap1(gj, ny);
-    else ok = false;
-  }
-  return ok;
-}
+// The region's cells are taken in horizontal pairs, region columns (2p, 2p+1)
+// of a row c.  Region column 0 is global column i0 = 64 tx - 1 (odd) and a
+// periodic side is even, so a pair is an (odd, even) global column pair: the
+// two fine columns of one coarse column, with one red cell, the first iff c is
+// even (j0 = 16 ty - 1 is odd).  A thread keeps its pairs' b, diag and face
+// coefficients in registers from the loads to the last phase; only x goes
+// through LDS (a cell needs its neighbours' x and nothing else of theirs).
+constexpr int FP_X = FR_X / 2;                      // pairs per region row
+constexpr int FP_N = FP_X * FR_Y;                   // pairs per region
+constexpr int FP_S = (FP_N + FT_BLK - 1) / FT_BLK;  // pair slots per thread
 
-// The four face coefficients of cell (gi, gj): west, east, south, north.
 template <typename T>
-struct Faces {
-  T w, e, s, n;
+struct Two;
+template <>
+struct Two<float> {
+  using type = float2;
 };
-template <typename T>
-__device__ __forceinline__ Faces<T> load_faces(const Lev<T> &L, int gi, int gj) {
-  const long k = (long)(gj - 1) * L.nx + (gi - 1);
-  const T *kxr = L.kx + (long)(gj - 1) * L.kxp;
-  return Faces<T>{kxr[gi - 1], kxr[gi], L.ky[k], L.ky[k + L.nx]};
-}
+template <>
+struct Two<double> {
+  using type = double2;
+};
 
-// Gauss-Seidel value of the region's cell (a, c) = global (gi, gj) from the
-// LDS x of its neighbours (gs_value's terms and order).
-template <typename T>
-__device__ __forceinline__ T gs_lds(const Lev<T> &L, int gi, int gj, const Faces<T> &f, T (*xs)[FR_X], T bv, T dv,
-                                    int a, int c) {
-  T s = bv;
-  if (gi > 1 || L.perx) s = s + f.w * xs[c][a - 1];
-  if (gi < L.nx || L.perx) s = s + f.e * xs[c][a + 1];
-  if (gj > 1 || L.pery) s = s + f.s * xs[c - 1][a];
-  if (gj < L.ny || L.pery) s = s + f.n * xs[c + 1][a];
-  return s / dv;
-}
+// One pair: cell 0 = region column a0, cell 1 = a0 + 1, region row c.
+// w, e, s, n: the cell's west, east, south and north face coefficients.
+template <typename T, typename TB>
+struct Pair {
+  TB b0, b1;
+  T d0, d1, w0, e0, w1, e1, s0, s1, n0, n1;
+  T xc;          // the coarse cell's x (post pass)
+  int c, a0;     // region row and first column
+  int gi0, gj;   // global column of cell 0 and global row (wrapped)
+  bool ok0, ok1; // inside the domain
+};
 
-// Per thread: NR_1 region cells (b, diag), NR_2 cells of the region minus its
-// outer ring (faces of its blacks), NR_3 tile cells (faces).  Every global
-// load of a block is issued before its first barrier.
-constexpr int NR_1 = (FR_N + FT_BLK - 1) / FT_BLK;
-constexpr int NR_2 = (F1_N + FT_BLK - 1) / FT_BLK;
-constexpr int NR_3 = FT_X * FT_Y / FT_BLK;
-
-// The loads of a fused level pass.  POST: the region's reds also load their
-// coarse cell's x_c.
-template <typename T, typename TB, bool POST>
-struct TileLoads {
-  TB b[NR_1];  // the right-hand side as loaded (rounded to T where it is used,
-               // so no load waits on a conversion)
-  T d[NR_1], xc[NR_1];
-  Faces<T> f2[NR_2], f3[NR_3];
-  // bin: the level's right-hand side (of type TB: FP64 for the FP32 cycle
-  // that preconditions an FP64 solve)
-  __device__ __forceinline__ void issue(const Lev<T> &F, const TB *__restrict__ bin, int i0, int j0, int nxc,
-                                        const T *__restrict__ xcv) {
-    // Every load is unconditional: a cell that has no value to load (outside
-    // a non-periodic side, or of the other colour) reads cell (1, 1) instead,
-    // and its value is never used (the users test the same conditions).  No
-    // load then sits in a branch whose join would wait for it.
-    const int nx = F.nx, ny = F.ny;
+// The loads of a thread's pairs.  INNER: the tile's whole region lies inside
+// the domain and every array is aligned for two-value loads (even nx), so a
+// pair's values come in two-value loads with no wrap or validity to settle.
+// Otherwise every load is a single value and unconditional: a cell outside
+// the domain reads cell (1, 1), whose value its users never take.
+template <bool INNER, bool POST, typename T, typename TB>
+__device__ __forceinline__ void pair_loads(const Lev<T> &F, const TB *__restrict__ bin, int i0, int j0, int nxc,
+                                           const T *__restrict__ xcv, Pair<T, TB> (&p)[FP_S]) {
+  const int nx = F.nx, ny = F.ny, kxp = (int)F.kxp;
 #pragma unroll
-    for (int q = 0; q < NR_1; ++q) {
-      const int e = threadIdx.x + q * FT_BLK;
-      int gi, gj;
-      const bool ok = e < FR_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % FR_X, e / FR_X, gi, gj);
-      if (!ok) gi = gj = 1;
-      const long k = (long)(gj - 1) * nx + (gi - 1);
-      b[q] = bin[k];
-      d[q] = F.diag[k];
+  for (int s = 0; s < FP_S; ++s) {
+    int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) P = FP_N - 1;  // a slot past the region reads the last pair again (never used)
+    const int c = P / FP_X, a0 = 2 * (P - c * FP_X);
+    Pair<T, TB> &q = p[s];
+    q.c = c;
+    q.a0 = a0;
+    int gi0 = i0 + a0, gj = j0 + c;
+    if (INNER) {
+      using T2 = typename Two<T>::type;
+      using TB2 = typename Two<TB>::type;
+      const int k = (gj - 1) * nx + (gi0 - 1), kk = (gj - 1) * kxp + (gi0 - 1);  // both even
+      const TB2 bv = *reinterpret_cast<const TB2 *>(bin + k);
+      const T2 dv = *reinterpret_cast<const T2 *>(F.diag + k);
+      const T2 sv = *reinterpret_cast<const T2 *>(F.ky + k);
+      const T2 nv = *reinterpret_cast<const T2 *>(F.ky + k + nx);
+      const T2 xa = *reinterpret_cast<const T2 *>(F.kx + kk);
+      const T2 xb = *reinterpret_cast<const T2 *>(F.kx + kk + 2);  // second value: row padding at most
+      q.b0 = bv.x;
+      q.b1 = bv.y;
+      q.d0 = dv.x;
+      q.d1 = dv.y;
+      q.s0 = sv.x;
+      q.s1 = sv.y;
+      q.n0 = nv.x;
+      q.n1 = nv.y;
+      q.w0 = xa.x;
+      q.e0 = xa.y;
+      q.w1 = xa.y;
+      q.e1 = xb.x;
+      q.ok0 = q.ok1 = true;
+      if (POST) q.xc = xcv[((gj + 1) / 2 - 1) * nxc + ((gi0 + 1) / 2 - 1)];
+    } else {
+      bool okx = true, oky = true;
+      if (gi0 < 1 || gi0 > nx) {
+        if (F.perx) gi0 = wrap1(gi0, nx);
+        else okx = false;
+      }
+      if (gj < 1 || gj > ny) {
+        if (F.pery) gj = wrap1(gj, ny);
+        else oky = false;
+      }
+      // gi0 is odd: outside on the west it is -1 (so is cell 1, column 0);
+      // wrapped it is at most nx - 1 (nx even), so cell 1 needs no wrap
+      q.ok0 = okx && oky;
+      q.ok1 = q.ok0 && gi0 + 1 <= nx;
+      const int k0 = q.ok0 ? (gj - 1) * nx + (gi0 - 1) : 0, k1 = q.ok1 ? (gj - 1) * nx + gi0 : 0;
+      const int kk0 = q.ok0 ? (gj - 1) * kxp + (gi0 - 1) : 0, kk1 = q.ok1 ? (gj - 1) * kxp + gi0 : 0;
+      q.b0 = bin[k0];
+      q.b1 = bin[k1];
+      q.d0 = F.diag[k0];
+      q.d1 = F.diag[k1];
+      q.s0 = F.ky[k0];
+      q.s1 = F.ky[k1];
+      q.n0 = F.ky[k0 + nx];
+      q.n1 = F.ky[k1 + nx];
+      q.w0 = F.kx[kk0];
+      q.e0 = F.kx[kk0 + 1];
+      q.w1 = F.kx[kk1];
+      q.e1 = F.kx[kk1 + 1];
       if (POST) {
-        const bool red = ok && ((gi + gj) & 1) == 0;
-        xc[q] = xcv[red ? (long)((gj + 1) / 2 - 1) * nxc + ((gi + 1) / 2 - 1) : 0L];
+        // the red cell's coarse cell (both cells of a pair share it)
+        const bool okr = (c & 1) ? q.ok1 : q.ok0;
+        q.xc = xcv[okr ? ((gj + 1) / 2 - 1) * nxc + ((gi0 + 1) / 2 - 1) : 0];
       }
     }
-#pragma unroll
-    for (int q = 0; q < NR_2; ++q) {
-      const int e = threadIdx.x + q * FT_BLK;
-      int gi, gj;
-      const bool ok = e < F1_N && region_cell(nx, ny, F.perx, F.pery, i0, j0, e % F1_X + 1, e / F1_X + 1, gi, gj) &&
-                      ((gi + gj) & 1) == 1;
-      if (!ok) gi = gj = 1;
-      f2[q] = load_faces(F, gi, gj);
-    }
-#pragma unroll
-    for (int q = 0; q < NR_3; ++q) {
-      const int e = threadIdx.x + q * FT_BLK;
-      int gi = i0 + e % FT_X + FT_H, gj = j0 + e / FT_X + FT_H;
-      if (gi > nx || gj > ny) gi = gj = 1;
-      f3[q] = load_faces(F, gi, gj);
-    }
-  }
-};
-
-// The blacks of the region minus its outer ring from the reds in LDS.
-template <typename T, typename TB, bool POST>
-__device__ __forceinline__ void black_sweep(const Lev<T> &F, int i0, int j0, const TileLoads<T, TB, POST> &ld,
-                                            T (*xs)[FR_X], T (*bs)[FR_X], T (*ds)[FR_X]) {
-#pragma unroll
-  for (int q = 0; q < NR_2; ++q) {
-    const int e = threadIdx.x + q * FT_BLK;
-    const int a = e % F1_X + 1, c = e / F1_X + 1;
-    int gi, gj;
-    if (e < F1_N && region_cell(F.nx, F.ny, F.perx, F.pery, i0, j0, a, c, gi, gj) && ((gi + gj) & 1) == 1)
-      xs[c][a] = gs_lds(F, gi, gj, ld.f2[q], xs, bs[c][a], ds[c][a], a, c);
+    q.gi0 = gi0;
+    q.gj = gj;
   }
 }
 
-template <typename T, typename TB>
-__global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restrict__ bin, int nxc, int nyc,
-                                                   T *__restrict__ bc, T scale, const double *scal, int f0, int f1) {
-  if (stopped(scal, f0, f1)) return;
-  __shared__ T xs[FR_Y][FR_X];
-  __shared__ T bs[FR_Y][FR_X];
-  __shared__ T ds[FR_Y][FR_X];
-  __shared__ T rs[FT_Y][FT_X];
-  const int nx = F.nx, ny = F.ny;
-  const int ntx = (nx + FT_X - 1) / FT_X;
-  const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
-  const int tx = t % ntx, ty = t / ntx;
-  const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;  // global cell of region cell (0, 0)
-  TileLoads<T, TB, false> ld;
-  ld.issue(F, bin, i0, j0, nxc, nullptr);
-  // the region's b, diag; reds = b / diag (the first half-sweep from x = 0)
+// x of the pairs' reds from x = 0 (b / diag; POST: + the coarse correction),
+// 0 at the blacks.
+template <bool POST, typename T, typename TB>
+__device__ __forceinline__ void pair_reds(const Pair<T, TB> (&p)[FP_S], T *xs) {
 #pragma unroll
-  for (int q = 0; q < NR_1; ++q) {
-    const int e = threadIdx.x + q * FT_BLK;
-    if (e >= FR_N) break;
-    const int a = e % FR_X, c = e / FR_X;
-    int gi, gj;
-    T xv = T(0);
-    const bool ok = region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj);
-    const T bq = ok ? (T)ld.b[q] : T(0), dq = ok ? ld.d[q] : T(1);
-    if (ok && ((gi + gj) & 1) == 0) xv = bq / dq;
-    bs[c][a] = bq;
-    ds[c][a] = dq;
-    xs[c][a] = xv;
+  for (int s = 0; s < FP_S; ++s) {
+    const int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) break;
+    const Pair<T, TB> &q = p[s];
+    const int rc = q.c & 1;  // which cell is red
+    const T br = rc ? (T)q.b1 : (T)q.b0, dr = rc ? q.d1 : q.d0;
+    const bool okr = rc ? q.ok1 : q.ok0;
+    T v = br / dr;
+    if (POST) v = v + q.xc;
+    xs[2 * P + rc] = okr ? v : T(0);
+    xs[2 * P + 1 - rc] = T(0);
   }
+}
+
+// Gauss-Seidel value of region cell e (global column gi, row gj) from the x
+// of its neighbours in LDS (gs_value's terms and order).
+template <bool INNER, typename T>
+__device__ __forceinline__ T pair_gs(const Lev<T> &F, int gi, int gj, T bv, T dv, T w, T e, T s, T n,
+                                     const T *xs, int k) {
+  T a = bv;
+  if (INNER || gi > 1 || F.perx) a = a + w * xs[k - 1];
+  if (INNER || gi < F.nx || F.perx) a = a + e * xs[k + 1];
+  if (INNER || gj > 1 || F.pery) a = a + s * xs[k - FR_X];
+  if (INNER || gj < F.ny || F.pery) a = a + n * xs[k + FR_X];
+  return a / dv;
+}
+
+// The black half-sweep over the region minus its outer ring, from the reds.
+template <bool INNER, typename T, typename TB>
+__device__ __forceinline__ void pair_blacks(const Lev<T> &F, const Pair<T, TB> (&p)[FP_S], T *xs) {
+#pragma unroll
+  for (int s = 0; s < FP_S; ++s) {
+    const int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) break;
+    const Pair<T, TB> &q = p[s];
+    const int bk = 1 - (q.c & 1);  // which cell is black
+    const int ab = q.a0 + bk;
+    const bool okb = bk ? q.ok1 : q.ok0;
+    if (q.c < 1 || q.c > FR_Y - 2 || ab < 1 || ab > FR_X - 2 || !okb) continue;
+    const T v = bk ? pair_gs<INNER>(F, q.gi0 + 1, q.gj, (T)q.b1, q.d1, q.w1, q.e1, q.s1, q.n1, xs, 2 * P + 1)
+                   : pair_gs<INNER>(F, q.gi0, q.gj, (T)q.b0, q.d0, q.w0, q.e0, q.s0, q.n0, xs, 2 * P);
+    xs[2 * P + bk] = v;
+  }
+}
+
+// Whether pair slot q holds tile cells (region columns 2..65, rows 2..17),
+// and for each cell whether it is inside the domain (a tile cell is never
+// west or south of it, and is not wrapped).
+template <typename T, typename TB>
+__device__ __forceinline__ bool pair_in_tile(const Pair<T, TB> &q, int i0, int j0, int nx, int ny, bool &in0,
+                                             bool &in1) {
+  if (q.c < FT_H || q.c >= FT_H + FT_Y || q.a0 < FT_H || q.a0 >= FT_H + FT_X) return false;
+  const bool row = j0 + q.c <= ny;
+  in0 = row && i0 + q.a0 <= nx;
+  in1 = row && i0 + q.a0 + 1 <= nx;
+  return true;
+}
+
+template <bool INNER, typename T, typename TB>
+__device__ __forceinline__ void pre_pass(const Lev<T> &F, const TB *__restrict__ bin, int tx, int ty, int i0, int j0,
+                                         int nxc, int nyc, T *__restrict__ bc, T scale, const double *scal, int f0,
+                                         int f1, T *xs, T *rs) {
+  const int nx = F.nx, ny = F.ny;
+  Pair<T, TB> p[FP_S];
+  pair_loads<INNER, false>(F, bin, i0, j0, nxc, (const T *)nullptr, p);
+  loads_stay_above(bin, bin);
+  if (stopped(scal, f0, f1)) return;
+  pair_reds<false>(p, xs);
   __syncthreads();
-  black_sweep(F, i0, j0, ld, xs, bs, ds);
+  pair_blacks<INNER>(F, p, xs);
   __syncthreads();
   // residual b - A x of the tile's cells (k_mg_restrict's terms and order)
 #pragma unroll
-  for (int q = 0; q < NR_3; ++q) {
-    const int e = threadIdx.x + q * FT_BLK;
-    const int a = e % FT_X + FT_H, c = e / FT_X + FT_H;
-    const int gi = i0 + a, gj = j0 + c;
-    if (gi > nx || gj > ny) continue;
-    const Faces<T> &f = ld.f3[q];
-    T ax = ds[c][a] * xs[c][a];
-    if (gi > 1 || F.perx) ax = ax - f.w * xs[c][a - 1];
-    if (gi < nx || F.perx) ax = ax - f.e * xs[c][a + 1];
-    if (gj > 1 || F.pery) ax = ax - f.s * xs[c - 1][a];
-    if (gj < ny || F.pery) ax = ax - f.n * xs[c + 1][a];
-    rs[c - FT_H][a - FT_H] = bs[c][a] - ax;
+  for (int s = 0; s < FP_S; ++s) {
+    const int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) break;
+    const Pair<T, TB> &q = p[s];
+    bool in0 = true, in1 = true;
+    if (!pair_in_tile(q, i0, j0, nx, ny, in0, in1)) continue;
+    const int gi = i0 + q.a0, gj = j0 + q.c;
+    const int r = (q.c - FT_H) * FT_X + (q.a0 - FT_H);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (!(h ? in1 : in0)) continue;
+      const int k = 2 * P + h;
+      const T w = h ? q.w1 : q.w0, e = h ? q.e1 : q.e0, so = h ? q.s1 : q.s0, no = h ? q.n1 : q.n0;
+      T ax = (h ? q.d1 : q.d0) * xs[k];
+      if (INNER || gi + h > 1 || F.perx) ax = ax - w * xs[k - 1];
+      if (INNER || gi + h < nx || F.perx) ax = ax - e * xs[k + 1];
+      if (INNER || gj > 1 || F.pery) ax = ax - so * xs[k - FR_X];
+      if (INNER || gj < ny || F.pery) ax = ax - no * xs[k + FR_X];
+      rs[r + h] = (T)(h ? q.b1 : q.b0) - ax;
+    }
   }
   __syncthreads();
   // restriction: b_c(I, J) = scale * (sum of the children's residuals)
   const int ci = threadIdx.x % (FT_X / 2), cj = threadIdx.x / (FT_X / 2);
   const int I = tx * (FT_X / 2) + ci + 1, J = ty * (FT_Y / 2) + cj + 1;
   if (I > nxc || J > nyc) return;
-  T s = T(0);
+  T sum = T(0);
   for (int dj = 0; dj < 2; ++dj)
     for (int di = 0; di < 2; ++di) {
-      if (2 * I - 1 + di > nx || 2 * J - 1 + dj > ny) continue;
-      s = s + rs[2 * cj + dj][2 * ci + di];
+      if (!INNER && (2 * I - 1 + di > nx || 2 * J - 1 + dj > ny)) continue;
+      sum = sum + rs[(2 * cj + dj) * FT_X + 2 * ci + di];
     }
-  bc[(long)(J - 1) * nxc + (I - 1)] = scale * s;
+  bc[(long)(J - 1) * nxc + (I - 1)] = scale * sum;
+}
+
+template <bool INNER, typename T, typename TB, typename TX>
+__device__ __forceinline__ void post_pass(const Lev<T> &F, const TB *__restrict__ bin, TX *__restrict__ xout,
+                                          int i0, int j0, int nxc, const T *__restrict__ xc, const double *scal,
+                                          int f0, int f1, T *xs) {
+  const int nx = F.nx, ny = F.ny;
+  Pair<T, TB> p[FP_S];
+  pair_loads<INNER, true>(F, bin, i0, j0, nxc, xc, p);
+  loads_stay_above(bin, xc);
+  if (stopped(scal, f0, f1)) return;
+  // reds corrected by the coarse level: the pre-smoothed red b / diag + x_c
+  pair_reds<true>(p, xs);
+  __syncthreads();
+  // black half-sweep from the corrected reds (k_mg_prolong_smooth)
+  pair_blacks<INNER>(F, p, xs);
+  __syncthreads();
+  // red half-sweep from the new blacks; the tile's x
+#pragma unroll
+  for (int s = 0; s < FP_S; ++s) {
+    const int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) break;
+    const Pair<T, TB> &q = p[s];
+    bool in0 = true, in1 = true;
+    if (!pair_in_tile(q, i0, j0, nx, ny, in0, in1)) continue;
+    const int gi = i0 + q.a0, gj = j0 + q.c;
+    const int rc = q.c & 1;
+    const T v0 = rc ? xs[2 * P] : pair_gs<INNER>(F, gi, gj, (T)q.b0, q.d0, q.w0, q.e0, q.s0, q.n0, xs, 2 * P);
+    const T v1 = rc ? pair_gs<INNER>(F, gi + 1, gj, (T)q.b1, q.d1, q.w1, q.e1, q.s1, q.n1, xs, 2 * P + 1)
+                    : xs[2 * P + 1];
+    const int k = (gj - 1) * nx + (gi - 1);
+    if (INNER) {
+      using TX2 = typename Two<TX>::type;
+      TX2 o;
+      o.x = (TX)v0;
+      o.y = (TX)v1;
+      *reinterpret_cast<TX2 *>(xout + k) = o;
+    } else {
+      if (in0) xout[k] = (TX)v0;
+      if (in1) xout[k + 1] = (TX)v1;
+    }
+  }
+}
+
+// The region of tile (tx, ty) lies inside the domain and the arrays allow
+// two-value loads (vec).
+__device__ __forceinline__ bool inner_tile(int vec, int nx, int ny, int tx, int ty, int i0, int j0) {
+  return vec && tx >= 1 && ty >= 1 && i0 + FR_X - 1 <= nx && j0 + FR_Y - 1 <= ny;
+}
+
+template <typename T, typename TB>
+__global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restrict__ bin, int nxc, int nyc,
+                                                   T *__restrict__ bc, T scale, int vec, const double *scal, int f0,
+                                                   int f1) {
+  __shared__ T xs[FR_N];
+  __shared__ T rs[FT_X * FT_Y];
+  const int ntx = (F.nx + FT_X - 1) / FT_X;
+  const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = t / ntx;
+  const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;  // global cell of region cell (0, 0)
+  if (inner_tile(vec, F.nx, F.ny, tx, ty, i0, j0))
+    pre_pass<true>(F, bin, tx, ty, i0, j0, nxc, nyc, bc, scale, scal, f0, f1, xs, rs);
+  else
+    pre_pass<false>(F, bin, tx, ty, i0, j0, nxc, nyc, bc, scale, scal, f0, f1, xs, rs);
 }
 
 template <typename T, typename TB, typename TX>
 __global__ void __launch_bounds__(FT_BLK) k_mg_post(Lev<T> F, const TB *__restrict__ bin, TX *__restrict__ xout,
-                                                    int nxc, const T *__restrict__ xc, const double *scal, int f0,
-                                                    int f1) {
-  if (stopped(scal, f0, f1)) return;
-  __shared__ T xs[FR_Y][FR_X];
-  __shared__ T bs[FR_Y][FR_X];
-  __shared__ T ds[FR_Y][FR_X];
-  const int nx = F.nx, ny = F.ny;
-  const int ntx = (nx + FT_X - 1) / FT_X;
+                                                    int nxc, const T *__restrict__ xc, int vec, const double *scal,
+                                                    int f0, int f1) {
+  __shared__ T xs[FR_N];
+  const int ntx = (F.nx + FT_X - 1) / FT_X;
   const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = t / ntx;
   const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;
-  TileLoads<T, TB, true> ld;
-  ld.issue(F, bin, i0, j0, nxc, xc);
-  // reds corrected by the coarse level: the pre-smoothed red b / diag + x_c
-#pragma unroll
-  for (int q = 0; q < NR_1; ++q) {
-    const int e = threadIdx.x + q * FT_BLK;
-    if (e >= FR_N) break;
-    const int a = e % FR_X, c = e / FR_X;
-    int gi, gj;
-    T xv = T(0);
-    const bool ok = region_cell(nx, ny, F.perx, F.pery, i0, j0, a, c, gi, gj);
-    const T bq = ok ? (T)ld.b[q] : T(0), dq = ok ? ld.d[q] : T(1);
-    if (ok && ((gi + gj) & 1) == 0) xv = bq / dq + ld.xc[q];
-    bs[c][a] = bq;
-    ds[c][a] = dq;
-    xs[c][a] = xv;
-  }
-  __syncthreads();
-  // black half-sweep from the corrected reds (k_mg_prolong_smooth)
-  black_sweep(F, i0, j0, ld, xs, bs, ds);
-  __syncthreads();
-  // red half-sweep from the new blacks; the tile's x
-#pragma unroll
-  for (int q = 0; q < NR_3; ++q) {
-    const int e = threadIdx.x + q * FT_BLK;
-    const int a = e % FT_X + FT_H, c = e / FT_X + FT_H;
-    const int gi = i0 + a, gj = j0 + c;
-    if (gi > nx || gj > ny) continue;
-    const T v = (((gi + gj) & 1) == 0) ? gs_lds(F, gi, gj, ld.f3[q], xs, bs[c][a], ds[c][a], a, c) : xs[c][a];
-    xout[(long)(gj - 1) * nx + (gi - 1)] = (TX)v;
-  }
+  if (inner_tile(vec, F.nx, F.ny, tx, ty, i0, j0))
+    post_pass<true>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs);
+  else
+    post_pass<false>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs);
+}
+
+// The fused passes' two-value loads and stores: an even level width and
+// 16-byte aligned arrays (the face pitch is a multiple of 64).
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+template <typename T>
+inline int pair_vec(const Lev<T> &L, const void *b, const void *x) {
+  return L.nx % 2 == 0 && L.kxp % 2 == 0 && aligned16(L.kx) && aligned16(L.ky) && aligned16(L.diag) &&
+         aligned16(b) && (x == nullptr || aligned16(x));
 }
 
 inline unsigned fused_tiles(int nx, int ny) {
@@ -567,37 +668,68 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   const int tid = threadIdx.x;
   const int last = B.nlev - 1, lw = B.lw;
   {
+    // the first level's right-hand side (at most 4096 cells) and the wave
+    // levels' coefficients (one cell of them per thread): all loads issued
+    // before any is stored
     const int n0 = B.lev[0].nx * B.lev[0].ny;
-    for (int t = tid; t < n0; t += 1024) bs[t] = b_in[t];
+    T bl[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = tid + q * 1024;
+      bl[q] = b_in[t < n0 ? t : 0];
+    }
+    // the thread's wave level (a uniform walk over the levels, so each level's
+    // record is read with scalar loads) and its cell t there
+    bool wave = false;
+    int t = 0, nx = B.lev[last].nx;
+    long kxp = B.lev[last].kxp;
+    const T *pkx = (const T *)B.lev[last].kx, *pky = (const T *)B.lev[last].ky, *pdg = (const T *)B.lev[last].diag;
     for (int l = lw; l <= last; ++l) {
       const BotLev &L = B.lev[l];
-      const int nx = L.nx, n = nx * L.ny;
-      const T *kx = (const T *)L.kx, *ky = (const T *)L.ky, *dg = (const T *)L.diag;
-      for (int t = tid; t < n; t += 1024) {
-        const int i = t % nx + 1, j = t / nx + 1;
-        const T *kxr = kx + (long)(j - 1) * L.kxp;
-        wc[0][L.woff + t] = kxr[i - 1];
-        wc[1][L.woff + t] = kxr[i];
-        wc[2][L.woff + t] = ky[t];
-        wc[3][L.woff + t] = ky[t + nx];
-        wc[4][L.woff + t] = dg[t];
+      if (tid >= L.woff && tid < L.woff + L.nx * L.ny) {
+        wave = true;
+        t = tid - L.woff;
+        nx = L.nx;
+        kxp = L.kxp;
+        pkx = (const T *)L.kx;
+        pky = (const T *)L.ky;
+        pdg = (const T *)L.diag;
       }
     }
+    T cf[5];
+    {
+      const int i = t % nx + 1, j = t / nx + 1;
+      const T *kxr = pkx + (long)(j - 1) * kxp;
+      cf[0] = kxr[i - 1];
+      cf[1] = kxr[i];
+      cf[2] = pky[t];
+      cf[3] = pky[t + nx];
+      cf[4] = pdg[t];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int t = tid + q * 1024;
+      if (t < n0) bs[t] = bl[q];
+    }
+    if (wave)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) wc[q][tid] = cf[q];
   }
-  // coefficients of this thread's cells (at most 4 per level) in registers,
-  // loaded once per visit of a level: a half-sweep then reads only LDS
+  // coefficients of this thread's cells (at most 4 per level) and their
+  // (i, j) in registers, loaded once per visit of a level: a half-sweep then
+  // reads only LDS
   T cw[4], ce[4], cs[4], cn[4], cd[4];
+  int ci[4], cj[4];
   auto load_coef = [&](int l) {
     const BotLev &L = B.lev[l];
     const int nx = L.nx, n = nx * L.ny;
     const T *kx = (const T *)L.kx, *ky = (const T *)L.ky, *dg = (const T *)L.diag;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int t = tid + q * 1024;
-      cw[q] = ce[q] = cs[q] = cn[q] = T(0);
-      cd[q] = T(1);
-      if (t >= n) continue;
+      const int t0 = tid + q * 1024, t = t0 < n ? t0 : 0;  // past the level: cell (1, 1), never stored
       const int i = t % nx + 1, j = t / nx + 1;
+      ci[q] = i;
+      cj[q] = j;
       const T *kxr = kx + (long)(j - 1) * L.kxp;
       cw[q] = kxr[i - 1];
       ce[q] = kxr[i];
@@ -606,74 +738,94 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       cd[q] = dg[t];
     }
   };
+  // The x of the four neighbours of cell t = (i, j) of a level: every read is
+  // issued (a neighbour across a non-periodic side reads the wrapped cell,
+  // which is in range, and its term is then left out by a select, not a
+  // branch, so no LDS read waits inside a branch).
+  struct Nb {
+    T w, e, s, n;
+    bool hw, he, hs, hn;
+  };
+  auto nbrs = [&](const BotLev &L, const T *x, int t, int i, int j) {
+    const int nx = L.nx, ny = L.ny;
+    Nb r;
+    r.w = x[i > 1 ? t - 1 : t + (nx - 1)];
+    r.e = x[i < nx ? t + 1 : t - (nx - 1)];
+    r.s = x[j > 1 ? t - nx : t + (ny - 1) * nx];
+    r.n = x[j < ny ? t + nx : t - (ny - 1) * nx];
+    r.hw = i > 1 || B.perx;
+    r.he = i < nx || B.perx;
+    r.hs = j > 1 || B.pery;
+    r.hn = j < ny || B.pery;
+    return r;
+  };
   // Gauss-Seidel value of cell t = (i, j) of a level from x (LDS)
   auto gs = [&](const BotLev &L, const T *x, T bv, T w, T e, T so, T no, T d, int t, int i, int j) {
-    const int nx = L.nx, ny = L.ny;
+    const Nb v = nbrs(L, x, t, i, j);
     T s = bv;
-    if (i > 1) s = s + w * x[t - 1];
-    else if (B.perx) s = s + w * x[t + (nx - 1)];
-    if (i < nx) s = s + e * x[t + 1];
-    else if (B.perx) s = s + e * x[t - (nx - 1)];
-    if (j > 1) s = s + so * x[t - nx];
-    else if (B.pery) s = s + so * x[t + (ny - 1) * nx];
-    if (j < ny) s = s + no * x[t + nx];
-    else if (B.pery) s = s + no * x[t - (ny - 1) * nx];
+    const T s1 = s + w * v.w;
+    s = v.hw ? s1 : s;
+    const T s2 = s + e * v.e;
+    s = v.he ? s2 : s;
+    const T s3 = s + so * v.s;
+    s = v.hs ? s3 : s;
+    const T s4 = s + no * v.n;
+    s = v.hn ? s4 : s;
     return s / d;
   };
+  // residual b - A x of cell t = (i, j) (k_mg_restrict's terms and order)
+  auto resid = [&](const BotLev &L, const T *x, T bv, T w, T e, T so, T no, T d, int t, int i, int j) {
+    const Nb v = nbrs(L, x, t, i, j);
+    T ax = d * x[t];
+    const T a1 = ax - w * v.w;
+    ax = v.hw ? a1 : ax;
+    const T a2 = ax - e * v.e;
+    ax = v.he ? a2 : ax;
+    const T a3 = ax - so * v.s;
+    ax = v.hs ? a3 : ax;
+    const T a4 = ax - no * v.n;
+    ax = v.hn ? a4 : ax;
+    return bv - ax;
+  };
   // coarse right-hand side of cell t of level l + 1: scale * sum of the
-  // children's residuals b - A x; coef(k, q) gives face q of child k
+  // children's residuals; coef(k, q) gives face q of child k
   auto restrict_cell = [&](int l, int t, auto coef) {
     const BotLev &F = B.lev[l], &C = B.lev[l + 1];
     const T *x = xs + F.off, *b = bs + F.off;
     const int nx = F.nx, ny = F.ny;
     const int I = t % C.nx + 1, J = t / C.nx + 1;
     T sum = T(0);
-    for (int dj = 0; dj < 2; ++dj)
-      for (int di = 0; di < 2; ++di) {
-        const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
-        if (i > nx || j > ny) continue;
-        const int k = (j - 1) * nx + (i - 1);
-        T ax = coef(k, i, j, 4) * x[k];
-        if (i > 1) ax = ax - coef(k, i, j, 0) * x[k - 1];
-        else if (B.perx) ax = ax - coef(k, i, j, 0) * x[k + (nx - 1)];
-        if (i < nx) ax = ax - coef(k, i, j, 1) * x[k + 1];
-        else if (B.perx) ax = ax - coef(k, i, j, 1) * x[k - (nx - 1)];
-        if (j > 1) ax = ax - coef(k, i, j, 2) * x[k - nx];
-        else if (B.pery) ax = ax - coef(k, i, j, 2) * x[k + (ny - 1) * nx];
-        if (j < ny) ax = ax - coef(k, i, j, 3) * x[k + nx];
-        else if (B.pery) ax = ax - coef(k, i, j, 3) * x[k - (ny - 1) * nx];
-        sum = sum + (b[k] - ax);
-      }
+#pragma unroll 1
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const int i = 2 * I - 1 + (c4 & 1), j = 2 * J - 1 + (c4 >> 1);
+      const bool has = i <= nx && j <= ny;
+      const int ii = has ? i : 2 * I - 1, jj = has ? j : 2 * J - 1;  // a missing child reads its sibling
+      const int k = (jj - 1) * nx + (ii - 1);
+      const T r = resid(F, x, b[k], coef(k, 0), coef(k, 1), coef(k, 2), coef(k, 3), coef(k, 4), k, ii, jj);
+      const T s1 = sum + r;
+      sum = has ? s1 : sum;
+    }
     bs[C.off + t] = (T)B.scale * sum;
-  };
-  auto prolong_cell = [&](int l, int t) {
-    const BotLev &F = B.lev[l], &C = B.lev[l + 1];
-    const int i = t % F.nx + 1, j = t / F.nx + 1;
-    xs[F.off + t] += xs[C.off + ((j + 1) / 2 - 1) * C.nx + ((i + 1) / 2 - 1)];
   };
   // ---- the workgroup's levels
   auto half = [&](int l, int color) {
     const BotLev &L = B.lev[l];
-    const int nx = L.nx, n = nx * L.ny;
+    const int n = L.nx * L.ny;
     T *x = xs + L.off;
     const T *b = bs + L.off;
     T v[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int t = tid + q * 1024;
-      v[q] = T(0);
-      if (t >= n) continue;
-      const int i = t % nx + 1, j = t / nx + 1;
-      if (((i + j) & 1) != color) continue;
-      v[q] = gs(L, x, b[t], cw[q], ce[q], cs[q], cn[q], cd[q], t, i, j);
+      if (q * 1024 >= n) break;
+      const int t0 = tid + q * 1024, t = t0 < n ? t0 : 0;
+      v[q] = gs(L, x, b[t], cw[q], ce[q], cs[q], cn[q], cd[q], t, ci[q], cj[q]);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+      if (q * 1024 >= n) break;
       const int t = tid + q * 1024;
-      if (t >= n) continue;
-      const int i = t % nx + 1, j = t / nx + 1;
-      if (((i + j) & 1) == color) x[t] = v[q];
+      if (t < n && ((ci[q] + cj[q]) & 1) == color) x[t] = v[q];
     }
     __syncthreads();
   };
@@ -684,8 +836,8 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   };
   const int lblock = lw < last ? lw : last;  // levels [0, lblock) go down and up with the workgroup
   for (int l = 0; l < lblock; ++l) {
-    zero(l);
     load_coef(l);
+    zero(l);
     for (int s = 0; s < B.NU; ++s) {
       half(l, 0);
       half(l, 1);
@@ -696,33 +848,37 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
     {
       const int nx = F.nx, ny = F.ny, n = nx * ny;
       const T *x = xs + F.off, *b = bs + F.off;
+      T r[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
+        if (q * 1024 >= n) break;
+        const int t0 = tid + q * 1024, t = t0 < n ? t0 : 0;
+        r[q] = resid(F, x, b[t], cw[q], ce[q], cs[q], cn[q], cd[q], t, ci[q], cj[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (q * 1024 >= n) break;
         const int t = tid + q * 1024;
-        if (t >= n) continue;
-        const int i = t % nx + 1, j = t / nx + 1;
-        T ax = cd[q] * x[t];
-        if (i > 1) ax = ax - cw[q] * x[t - 1];
-        else if (B.perx) ax = ax - cw[q] * x[t + (nx - 1)];
-        if (i < nx) ax = ax - ce[q] * x[t + 1];
-        else if (B.perx) ax = ax - ce[q] * x[t - (nx - 1)];
-        if (j > 1) ax = ax - cs[q] * x[t - nx];
-        else if (B.pery) ax = ax - cs[q] * x[t + (ny - 1) * nx];
-        if (j < ny) ax = ax - cn[q] * x[t + nx];
-        else if (B.pery) ax = ax - cn[q] * x[t - (ny - 1) * nx];
-        rsd[t] = b[t] - ax;
+        if (t < n) rsd[t] = r[q];
       }
       __syncthreads();
       const int nc = C.nx * C.ny;
       for (int t = tid; t < nc; t += 1024) {
         const int I = t % C.nx + 1, J = t / C.nx + 1;
+        T c[4];
+        bool has[4];
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+          const int i = 2 * I - 1 + (c4 & 1), j = 2 * J - 1 + (c4 >> 1);
+          has[c4] = i <= nx && j <= ny;
+          c[c4] = rsd[has[c4] ? (j - 1) * nx + (i - 1) : 0];
+        }
         T sum = T(0);
-        for (int dj = 0; dj < 2; ++dj)
-          for (int di = 0; di < 2; ++di) {
-            const int i = 2 * I - 1 + di, j = 2 * J - 1 + dj;
-            if (i > nx || j > ny) continue;
-            sum = sum + rsd[(j - 1) * nx + (i - 1)];
-          }
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+          const T s1 = sum + c[c4];
+          sum = has[c4] ? s1 : sum;
+        }
         bs[C.off + t] = (T)B.scale * sum;
       }
       __syncthreads();
@@ -766,28 +922,36 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   } else if (tid < 64) {
     // ---- the wave levels, first wave only
     const int lane = tid;
+    // (i, j) of the lane's cells of the wave level being visited
+    int wi[4], wj[4];
+    auto wave_cells = [&](int l) {
+      const int nx = B.lev[l].nx, n = nx * B.lev[l].ny;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int t0 = lane + q * 64, t = t0 < n ? t0 : 0;
+        wi[q] = t % nx + 1;
+        wj[q] = t / nx + 1;
+      }
+    };
     auto half_w = [&](int l, int color) {
       const BotLev &L = B.lev[l];
-      const int nx = L.nx, n = nx * L.ny, o = L.woff;
+      const int n = L.nx * L.ny, o = L.woff;
       T *x = xs + L.off;
       const T *b = bs + L.off;
       T v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int t = lane + q * 64;
-        v[q] = T(0);
-        if (t >= n) continue;
-        const int i = t % nx + 1, j = t / nx + 1;
-        if (((i + j) & 1) != color) continue;
-        v[q] = gs(L, x, b[t], wc[0][o + t], wc[1][o + t], wc[2][o + t], wc[3][o + t], wc[4][o + t], t, i, j);
+        if (q * 64 >= n) break;
+        const int t0 = lane + q * 64, t = t0 < n ? t0 : 0;
+        v[q] = gs(L, x, b[t], wc[0][o + t], wc[1][o + t], wc[2][o + t], wc[3][o + t], wc[4][o + t], t, wi[q],
+                  wj[q]);
       }
       wave_sync();
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
+        if (q * 64 >= n) break;
         const int t = lane + q * 64;
-        if (t >= n) continue;
-        const int i = t % nx + 1, j = t / nx + 1;
-        if (((i + j) & 1) == color) x[t] = v[q];
+        if (t < n && ((wi[q] + wj[q]) & 1) == color) x[t] = v[q];
       }
       wave_sync();
     };
@@ -797,57 +961,51 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       wave_sync();
     };
     for (int l = lw; l < last; ++l) {
+      wave_cells(l);
       zero_w(l);
       for (int s = 0; s < B.NU; ++s) {
         half_w(l, 0);
         half_w(l, 1);
       }
       const int o = B.lev[l].woff;
-      auto lcoef = [&](int k, int, int, int q) -> T { return wc[q][o + k]; };
+      auto lcoef = [&](int k, int q) -> T { return wc[q][o + k]; };
       const int nc = B.lev[l + 1].nx * B.lev[l + 1].ny;
       for (int t = lane; t < nc; t += 64) restrict_cell(l, t, lcoef);
       wave_sync();
     }
     if (B.lev[last].nx * B.lev[last].ny <= 64) {
-      // the coarsest level, one cell per lane: its operands and neighbour
-      // indices (-1: no term) in registers for the 4K half-sweeps
+      // the coarsest level, one cell per lane: its operands, neighbour
+      // indices (wrapped, in range) and which terms it has in registers for
+      // the 4K half-sweeps
       const BotLev &L = B.lev[last];
       const int nx = L.nx, ny = L.ny, n = nx * ny, o = L.woff;
       T *x = xs + L.off;
       if (B.project) mean_free(bs + L.off, n, true);
-      const int t = lane;
-      const bool live = t < n;
+      const bool live = lane < n;
+      const int t = live ? lane : 0;
       const int i = t % nx + 1, j = t / nx + 1;
       const int col = (i + j) & 1;
-      int iw = -1, ie = -1, is = -1, in = -1;
-      T bv = T(0), w = T(0), e = T(0), so = T(0), no = T(0), d = T(1);
-      if (live) {
-        iw = i > 1 ? t - 1 : (B.perx ? t + (nx - 1) : -1);
-        ie = i < nx ? t + 1 : (B.perx ? t - (nx - 1) : -1);
-        is = j > 1 ? t - nx : (B.pery ? t + (ny - 1) * nx : -1);
-        in = j < ny ? t + nx : (B.pery ? t - (ny - 1) * nx : -1);
-        bv = bs[L.off + t];
-        w = wc[0][o + t];
-        e = wc[1][o + t];
-        so = wc[2][o + t];
-        no = wc[3][o + t];
-        d = wc[4][o + t];
-        x[t] = T(0);
-      }
+      const int iw = i > 1 ? t - 1 : t + (nx - 1), ie = i < nx ? t + 1 : t - (nx - 1);
+      const int is = j > 1 ? t - nx : t + (ny - 1) * nx, in = j < ny ? t + nx : t - (ny - 1) * nx;
+      const bool hw = i > 1 || B.perx, he = i < nx || B.perx, hs = j > 1 || B.pery, hn = j < ny || B.pery;
+      const T bv = bs[L.off + t], w = wc[0][o + t], e = wc[1][o + t], so = wc[2][o + t], no = wc[3][o + t],
+              d = wc[4][o + t];
+      if (live) x[t] = T(0);
       wave_sync();
       auto half_c = [&](int color) {
-        T v = T(0);
-        const bool upd = live && col == color;
-        if (upd) {
-          T s = bv;
-          if (iw >= 0) s = s + w * x[iw];
-          if (ie >= 0) s = s + e * x[ie];
-          if (is >= 0) s = s + so * x[is];
-          if (in >= 0) s = s + no * x[in];
-          v = s / d;
-        }
+        const T xw = x[iw], xe = x[ie], xs_ = x[is], xn = x[in];
+        T s = bv;
+        const T s1 = s + w * xw;
+        s = hw ? s1 : s;
+        const T s2 = s + e * xe;
+        s = he ? s2 : s;
+        const T s3 = s + so * xs_;
+        s = hs ? s3 : s;
+        const T s4 = s + no * xn;
+        s = hn ? s4 : s;
+        const T v = s / d;
         wave_sync();
-        if (upd) x[t] = v;
+        if (live && col == color) x[t] = v;
         wave_sync();
       };
       for (int s = 0; s < B.K; ++s) {
@@ -862,6 +1020,7 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
     } else {
       const BotLev &L = B.lev[last];
       if (B.project) mean_free(bs + L.off, L.nx * L.ny, true);
+      wave_cells(last);
       zero_w(last);
       for (int s = 0; s < B.K; ++s) {
         half_w(last, 0);
@@ -874,8 +1033,14 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
       if (B.project) mean_free(xs + L.off, L.nx * L.ny, true);
     }
     for (int l = last - 1; l >= lw; --l) {
-      const int n = B.lev[l].nx * B.lev[l].ny;
-      for (int t = lane; t < n; t += 64) prolong_cell(l, t);
+      const BotLev &F = B.lev[l], &C = B.lev[l + 1];
+      const int n = F.nx * F.ny;
+      wave_cells(l);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {  // prolongation: x += x of the coarse cell
+        const int t = lane + q * 64;
+        if (t < n) xs[F.off + t] += xs[C.off + ((wj[q] + 1) / 2 - 1) * C.nx + ((wi[q] + 1) / 2 - 1)];
+      }
       wave_sync();
       for (int s = 0; s < B.NU; ++s) {
         half_w(l, 1);
@@ -885,10 +1050,18 @@ __global__ void __launch_bounds__(1024) k_mg_bottom(Bottom B, const T *__restric
   }
   __syncthreads();
   for (int l = lblock - 1; l >= 0; --l) {
-    const int n = B.lev[l].nx * B.lev[l].ny;
-    for (int t = tid; t < n; t += 1024) prolong_cell(l, t);
+    // (FP32: the last level the workgroup went down through still has its
+    // coefficients in registers when the wave levels took the coarsest; FP64
+    // loads them again rather than keep them across the wave levels)
+    if (sizeof(T) == 8 || l < lblock - 1 || lw > last) load_coef(l);
+    const BotLev &F = B.lev[l], &C = B.lev[l + 1];
+    const int n = F.nx * F.ny;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // prolongation: x += x of the coarse cell
+      const int t = tid + q * 1024;
+      if (t < n) xs[F.off + t] += xs[C.off + ((cj[q] + 1) / 2 - 1) * C.nx + ((ci[q] + 1) / 2 - 1)];
+    }
     __syncthreads();
-    load_coef(l);
     for (int s = 0; s < B.NU; ++s) {
       half(l, 1);
       half(l, 0);
@@ -1090,10 +1263,10 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
       const dim3 grid(fused_tiles(L.nx, L.ny));
       if (l == 0)
         hipLaunchKernelGGL((k_mg_pre<T, TB>), grid, dim3(FT_BLK), 0, c.stream, L, r, B.nx, B.ny, (T *)B.b, scale,
-                           (const double *)c.ps.scal, f0, f1);
+                           pair_vec(L, r, nullptr), (const double *)c.ps.scal, f0, f1);
       else
         hipLaunchKernelGGL((k_mg_pre<T, T>), grid, dim3(FT_BLK), 0, c.stream, L, L.b, B.nx, B.ny, (T *)B.b, scale,
-                           (const double *)c.ps.scal, f0, f1);
+                           pair_vec(L, L.b, nullptr), (const double *)c.ps.scal, f0, f1);
       continue;
     }
     for (int s = 0; s < NU; ++s) {
@@ -1152,10 +1325,10 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
       const dim3 grid(fused_tiles(L.nx, L.ny));
       if (l == 0)
         hipLaunchKernelGGL((k_mg_post<T, TB, TZ>), grid, dim3(FT_BLK), 0, c.stream, L, r, z, B.nx, (const T *)B.x,
-                           (const double *)c.ps.scal, f0, f1);
+                           pair_vec(L, r, z), (const double *)c.ps.scal, f0, f1);
       else
         hipLaunchKernelGGL((k_mg_post<T, T, T>), grid, dim3(FT_BLK), 0, c.stream, L, L.b, L.x, B.nx,
-                           (const T *)B.x, (const double *)c.ps.scal, f0, f1);
+                           (const T *)B.x, pair_vec(L, L.b, L.x), (const double *)c.ps.scal, f0, f1);
       continue;
     }
     hipLaunchKernelGGL(k_mg_prolong_smooth<T>, pair_grid(L.nx, L.ny), dim3(64, 4), 0, c.stream, L, B.nx,
