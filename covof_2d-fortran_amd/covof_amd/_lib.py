@@ -109,6 +109,8 @@ SIGNATURES = {
     "covof_momentum_matvec_d": (_I, [_P, _P, _P]),
     "covof_momentum_solve_d": (_I, [_P, _P, _P, _I, _I, _D, _I, _IP, _DP]),
     "covof_momentum_precond_d": (_I, [_P, _I, _P, _P]),
+    "covof_momentum_set_fields": (_I, [_P, ct.c_void_p] + [_DP] * 12 + [_D]),
+    "covof_momentum_precond": (_I, [_P, _I, _I, _DP, _DP]),
     "covof_velocity_bc_d": (_I, [_P, ct.c_void_p, _P, _P, _I]),
     "covof_sp_predictor_d": (_I, [_P, ct.c_void_p] + [_P] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),
     "covof_sp_predictor": (_I, [_P, ct.c_void_p] + [_DP] * 12 + [_D, _I, _I, _D, _I, _IP, _DP]),

@@ -62,6 +62,8 @@ struct covof_ctx {
   Ctx c;
   // device mirrors for the host-array entry points (made on first use)
   double *mirror[13] = {nullptr};
+  // r and z of covof_momentum_precond (made on first use, monolithic size)
+  double *hvec[2] = {nullptr};
 };
 
 static thread_local std::string g_err;
@@ -169,6 +171,7 @@ extern "C" int covof_ctx_destroy(covof_ctx *h) {
   if (c.prof.created)
     for (int k = 0; k < 2 * 1024; ++k) hipEventDestroy(c.prof.ev[k]);
   for (double *p : h->mirror) hipFree(p);
+  for (double *p : h->hvec) hipFree(p);
   if (c.own_stream && c.stream) hipStreamDestroy(c.stream);
   delete h;
   return COVOF_OK;
@@ -798,6 +801,49 @@ extern "C" int covof_momentum_precond_d(covof_ctx *h, int precond, const double 
   if (st == COVOF_BAD_ARGUMENT)
     return fail(st, "momentum fields not set, or preconditioner not NONE/JACOBI/SSOR/BLOCK_TRIANGULAR");
   return hip_status(st);
+}
+
+// Host-array forms for a caller whose fields live on the host (the
+// PRECOND_CUSTOM hook of a CPU solve): the fields go to the context's device
+// mirrors 0-11, which stay bound until the next host entry point reuses them.
+extern "C" int covof_momentum_set_fields(covof_ctx *h, const covof_momentum_cfg *cfg, const double *u,
+                                         const double *v, const double *u_old, const double *v_old,
+                                         const double *u_nm1, const double *v_nm1, const double *rho_u,
+                                         const double *rho_v, const double *mu, const double *mu_node,
+                                         const double *fx_sigma, const double *fy_sigma, double dt) {
+  if (!h || !u || !v || !u_old || !v_old || !rho_u || !rho_v || !mu || !mu_node || !fx_sigma || !fy_sigma)
+    return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const double *hs[12] = {u, v, u_old, v_old, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, fx_sigma, fy_sigma};
+  double *d[12] = {nullptr};
+  int st = 0;
+  for (int k = 0; k < 12 && !st; ++k) {
+    if (!hs[k]) continue;
+    if (!(d[k] = mirror(h, k))) return hip_status(COVOF_HIP_ERROR);
+    st = h2d(h, d[k], hs[k]);
+  }
+  if (st) return hip_status(st);
+  return covof_momentum_set_fields_d(h, cfg, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9], d[10],
+                                     d[11], dt);
+}
+
+extern "C" int covof_momentum_precond(covof_ctx *h, int precond, int n, const double *r, double *z) {
+  if (!h || !r || !z) return fail(COVOF_BAD_ARGUMENT, "null argument");
+  const Ctx &c = h->c;
+  if (!c.mom.bound) return fail(COVOF_BAD_ARGUMENT, "momentum fields not set");
+  if (n != c.mom.n) return fail(COVOF_BAD_ARGUMENT, "covof_momentum_precond: n is not the bound system's size");
+  int nmax = 0;
+  covof_momentum_size(h, COVOF_SOLVER_MONOLITHIC, &nmax, nullptr, nullptr);
+  for (double *&p : h->hvec)
+    if (!p && hipMalloc(&p, (size_t)nmax * sizeof(double)) != hipSuccess) return hip_status(COVOF_HIP_ERROR);
+  const size_t bytes = (size_t)n * sizeof(double);
+  if (hipMemcpyAsync(h->hvec[0], r, bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+    return hip_status(COVOF_HIP_ERROR);
+  const int st = covof_momentum_precond_d(h, precond, h->hvec[0], h->hvec[1]);
+  if (st) return st;
+  if (hipMemcpyAsync(z, h->hvec[1], bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess ||
+      hipStreamSynchronize(c.stream) != hipSuccess)
+    return hip_status(COVOF_HIP_ERROR);
+  return COVOF_OK;
 }
 
 extern "C" int covof_velocity_bc_d(covof_ctx *h, const covof_momentum_cfg *cfg, double *u, double *v,

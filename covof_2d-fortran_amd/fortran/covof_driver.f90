@@ -16,9 +16,12 @@
 !>          state, TVD superbee convection, gravity, BiCGSTAB(2) + Jacobi to 1e-10;
 !>   mono : the monolithic branch of sim_run (src/module_simulation.f90:550-583)
 !>          on the same box: u, v, p coupled, BiCGSTAB(2) + block-triangular
-!>          Schur-SSOR to 1e-10.
+!>          Schur-SSOR to 1e-10;
+!>   hook : the body of the PRECOND_CUSTOM hook (covof_precond_hook.f90) on
+!>          the pred box's fields, bound from host arrays: dst = M^-1 src with
+!>          the GPU's SSOR, src(k) = sin(0.37 k) + 0.1.
 !>
-!> usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE
+!> usage: covof_driver vof|sp|csf|pred|mono|hook N NSTEPS OUTFILE
 !>
 !> OUTFILE (unformatted stream, native endianness):
 !>   int32 mode (1 vof, 2 sp), nx, ny, nsteps, counter (vof: final
@@ -29,6 +32,8 @@
 !>   csf: C, C_s, C_tilde, curvature, fx_sigma, fy_sigma
 !>   pred: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, u*, v*
 !>   mono: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, u, v, p
+!>   hook: u0, v0, u_nm1, v_nm1, rho_u, rho_v, mu, mu_node, then src and dst
+!>         (counter entries each; counter = the momentum system's size)
 program covof_driver
   use iso_c_binding
   use covof_hip
@@ -44,11 +49,13 @@ program covof_driver
   real(dp), allocatable :: rho(:, :), rho_u(:, :), rho_v(:, :), p(:, :), us(:, :), vs(:, :)
   real(dp), allocatable :: Cs(:, :), Ct(:, :), curv(:, :), fx(:, :), fy(:, :)
   real(dp), allocatable :: mu(:, :), mun(:, :), u0(:, :), v0(:, :), um(:, :), vm(:, :)
+  real(dp), allocatable :: src(:), dst(:)
+  integer(c_int) :: nm, unx, vny
   type(covof_momentum_cfg) :: mcfg
   real(dp), parameter :: pi = 3.14159265358979323846_dp
 
   if (command_argument_count() < 4) then
-     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono|hook N NSTEPS OUTFILE"
      stop 2
   end if
   call get_command_argument(1, mode)
@@ -164,7 +171,7 @@ program covof_driver
      write (unit) C, Cs, Ct, curv, fx, fy
      close (unit)
      print '(a,es23.16)', "csf max |curvature| ", maxval(abs(curv))
-  case ("pred", "mono")
+  case ("pred", "mono", "hook")
      bc = COVOF_BC_DIRICHLET
      st = covof_ctx_create(int(n, c_int), int(n, c_int), 1.0_dp, 1.0_dp, bc, ctx)
      call check(st, "covof_ctx_create")
@@ -211,7 +218,28 @@ program covof_driver
      mcfg%convective_scheme = COVOF_CONV_TVD_SUPERBEE
      mcfg%gy = -0.98_dp
      mcfg%bc_u(4) = 1.0_dp
-     if (trim(mode) == "pred") then
+     if (trim(mode) == "hook") then
+        st = covof_momentum_set_fields(ctx, mcfg, u0, v0, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt)
+        call check(st, "covof_momentum_set_fields")
+        st = covof_momentum_size(ctx, COVOF_SOLVER_STANDARD_PROJECTION, nm, unx, vny)
+        call check(st, "covof_momentum_size")
+        allocate(src(nm), dst(nm))
+        do i = 1, nm
+           src(i) = sin(0.37_dp * i) + 0.1_dp
+        end do
+        call covof_precond_select(ctx, COVOF_PRECOND_SSOR)
+        call covof_precond_apply(src, dst)
+        st = covof_precond_status()
+        call check(st, "covof_precond_apply")
+        open (newunit=unit, file=trim(outfile), access="stream", form="unformatted", status="replace")
+        write (unit) 6_c_int, int(n, c_int), int(n, c_int), int(nsteps, c_int), nm, int(st, c_int)
+        write (unit) dt, 0.0_dp
+        write (unit) u0, v0, um, vm, rho_u, rho_v, mu, mun, src, dst
+        close (unit)
+        print '(a,i0,a,es23.16)', "hook: ", nm, " unknowns, max |dst| ", maxval(abs(dst))
+        st = covof_ctx_destroy(ctx)
+        stop
+     else if (trim(mode) == "pred") then
         st = covof_sp_predictor(ctx, mcfg, u, v, u0, v0, um, vm, rho_u, rho_v, mu, mun, fx, fy, dt, &
              COVOF_LINEAR_BICGSTAB2, COVOF_PRECOND_JACOBI, 1.0e-10_dp, 300_c_int, iters, relres)
         call check(st, "covof_sp_predictor")
@@ -233,7 +261,7 @@ program covof_driver
      close (unit)
      print '(a,a,a,i0,a,es10.3)', "solve (", trim(mode), ") iterations ", counter, " relres ", relres
   case default
-     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono N NSTEPS OUTFILE"
+     print '(a)', "usage: covof_driver vof|sp|csf|pred|mono|hook N NSTEPS OUTFILE"
      stop 2
   end select
   st = covof_ctx_destroy(ctx)

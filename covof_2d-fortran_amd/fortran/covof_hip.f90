@@ -34,7 +34,7 @@
 module covof_hip
   use iso_c_binding
   implicit none
-  private :: c_str
+  private :: c_str, hook_ctx, hook_precond, hook_status
 
   integer(c_int), parameter :: COVOF_OK = 0, COVOF_NOT_CONVERGED = 1, COVOF_BREAKDOWN = 2, &
        COVOF_NONFINITE = 3, COVOF_BAD_ARGUMENT = 4, COVOF_HIP_ERROR = 5
@@ -144,6 +144,31 @@ module covof_hip
        real(c_double), intent(out) :: relres
      end function covof_sp_predictor
 
+     integer(c_int) function covof_momentum_set_fields(ctx, mcfg, u, v, u_old, v_old, u_nm1, v_nm1, rho_u, &
+          rho_v, mu, mu_node, fx_sigma, fy_sigma, dt) bind(C, name="covof_momentum_set_fields")
+       import :: c_int, c_double, c_ptr, covof_momentum_cfg
+       type(c_ptr), value :: ctx
+       type(covof_momentum_cfg), intent(in) :: mcfg
+       real(c_double), intent(in) :: u(*), v(*), u_old(*), v_old(*), u_nm1(*), v_nm1(*), rho_u(*), rho_v(*), &
+            mu(*), mu_node(*), fx_sigma(*), fy_sigma(*)
+       real(c_double), value :: dt
+     end function covof_momentum_set_fields
+
+     integer(c_int) function covof_momentum_size(ctx, solver_type, n, u_nx, v_ny) bind(C, name="covof_momentum_size")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: solver_type
+       integer(c_int), intent(out) :: n, u_nx, v_ny
+     end function covof_momentum_size
+
+     integer(c_int) function covof_momentum_precond(ctx, precond, n, r, z) bind(C, name="covof_momentum_precond")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: precond, n
+       real(c_double), intent(in) :: r(*)
+       real(c_double), intent(out) :: z(*)
+     end function covof_momentum_precond
+
      integer(c_int) function covof_monolithic_step(ctx, mcfg, u, v, p, u_old, v_old, u_nm1, v_nm1, rho_u, &
           rho_v, mu, mu_node, fx_sigma, fy_sigma, dt, lin_solver, precond, tol, maxit, iters, relres) &
           bind(C, name="covof_monolithic_step")
@@ -240,7 +265,46 @@ module covof_hip
      end function covof_sp_poisson_d
   end interface
 
+  ! The GPU preconditioner behind the PRECOND_CUSTOM hook (covof_precond_hook.f90):
+  ! the context holding the bound fields, the preconditioner, and the status
+  ! of the last application that failed (0: none).
+  type(c_ptr), save :: hook_ctx = c_null_ptr
+  integer(c_int), save :: hook_precond = COVOF_PRECOND_SSOR
+  integer(c_int), save :: hook_status = COVOF_OK
+
 contains
+
+  !> Selects the context and the preconditioner (NONE, JACOBI, SSOR or
+  !> BLOCK_TRIANGULAR of apply_preconditioner, src/module_linear_solvers.f90:
+  !> 48-316) that covof_precond_apply uses; clears the failure status.
+  subroutine covof_precond_select(ctx, precond)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: precond
+    hook_ctx = ctx
+    hook_precond = precond
+    hook_status = COVOF_OK
+  end subroutine covof_precond_select
+
+  !> dst = M^-1 src on the GPU with the selected context's bound fields (the
+  !> body of the custom_precond_proc hook).  If the call fails, dst = src,
+  !> what apply_preconditioner does for PRECOND_CUSTOM with no hook
+  !> associated (src/module_linear_solvers.f90:310-311), and the status is
+  !> kept for covof_precond_status.
+  subroutine covof_precond_apply(src, dst)
+    real(c_double), intent(in) :: src(:)
+    real(c_double), intent(out) :: dst(:)
+    integer(c_int) :: st
+    st = covof_momentum_precond(hook_ctx, hook_precond, int(size(src), c_int), src, dst)
+    if (st /= COVOF_OK) then
+       hook_status = st
+       dst = src
+    end if
+  end subroutine covof_precond_apply
+
+  !> Status of the last failed covof_precond_apply since covof_precond_select.
+  integer(c_int) function covof_precond_status()
+    covof_precond_status = hook_status
+  end function covof_precond_status
 
   !> trim(s) // NUL for the C string arguments.
   function c_str(s) result(cs)

@@ -375,6 +375,23 @@ int covof_momentum_solve_d(covof_ctx *ctx, const double *b, double *x, int lin_s
  * block, mean removed, then 4 sweeps on u and v against its gradient; SSOR
  * otherwise); n entries. */
 int covof_momentum_precond_d(covof_ctx *ctx, int precond, const double *r, double *z);
+/* Host-array forms, for a CPU solve that keeps its own Krylov loop and hands
+ * the preconditioner to the GPU through the reference's only extension point,
+ * custom_precond_hook (src/module_linear_solvers.f90:22-41), which
+ * apply_preconditioner calls for PRECOND_CUSTOM (:306-312) with
+ * (ns, mesh, cfg, src, dst): covof_momentum_set_fields copies the
+ * fields_old arrays (host, halo layout) into the context's device mirrors and
+ * binds them (they stay bound until another host-array entry point reuses the
+ * mirrors); covof_momentum_precond is dst = M^-1 src of
+ * covof_momentum_precond_d on host arrays of n entries, n the bound system's
+ * size (covof_momentum_size), else COVOF_BAD_ARGUMENT.  The Fortran module
+ * procedure covof_gpu_precond (fortran/covof_precond_hook.f90) has the hook's
+ * interface and calls it. */
+int covof_momentum_set_fields(covof_ctx *ctx, const covof_momentum_cfg *cfg, const double *u, const double *v,
+                              const double *u_old, const double *v_old, const double *u_nm1, const double *v_nm1,
+                              const double *rho_u, const double *rho_v, const double *mu, const double *mu_node,
+                              const double *fx_sigma, const double *fy_sigma, double dt);
+int covof_momentum_precond(covof_ctx *ctx, int precond, int n, const double *r, double *z);
 /* The momentum block of sim_solve_standard_projection (src/module_simulation.f90:
  * 2454-2469): diagonal, right-hand side, pack (the current u, v are the
  * initial guess), solve, unpack_vector(.false.), sync_u / sync_v and

@@ -153,3 +153,46 @@ def test_fortran_monolithic_step_matches_python(tmp_path, cv):
     assert res.converged and res.iters == hdr[4]
     for got, ref in ((F["u"], us), (F["v"], vs), (p, ps)):
         assert np.array_equal(got.cpu().numpy(), ref)
+
+
+@pytest.mark.gpu
+def test_fortran_precond_hook_body_matches_python(tmp_path, cv):
+    """The PRECOND_CUSTOM hook's body (covof_precond_apply: the fields bound
+    from host arrays with covof_momentum_set_fields, dst = M^-1 src through
+    covof_momentum_precond) gives the device path's SSOR application bit for
+    bit; a size that is not the bound system's is refused and leaves the
+    hook's fallback dst = src."""
+    import ctypes as ct
+    from covof_amd import _lib as L
+    n = 40
+    out = tmp_path / "hook.bin"
+    r = subprocess.run([DRIVER, "hook", str(n), "1", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    with open(out, "rb") as f:
+        hdr = np.frombuffer(f.read(24), dtype=np.int32)
+        dt = float(np.frombuffer(f.read(16), dtype=np.float64)[0])
+        rest = np.frombuffer(f.read(), dtype=np.float64)
+    assert hdr[0] == 6 and hdr[5] == 0
+    nm, m = int(hdr[4]), (n + 2) * (n + 2)
+    assert rest.size == 8 * m + 2 * nm
+    u0, v0, um, vm, rho_u, rho_v, mu, mun = [a.reshape(n + 2, n + 2).copy() for a in np.split(rest[:8 * m], 8)]
+    src, dst = rest[8 * m:8 * m + nm].copy(), rest[8 * m + nm:].copy()
+    z = np.zeros_like(u0)
+    cfg = cv.Context.momentum_cfg(time_scheme=2, has_nm1=True, convective_scheme=3, gy=-0.98, bc_u=(0, 0, 0, 1.0))
+    ctx = cv.Context(n, n, bc=(1, 1, 1, 1))
+    F = {"u": to_gpu(u0), "v": to_gpu(v0), "u_old": to_gpu(u0), "v_old": to_gpu(v0), "u_nm1": to_gpu(um),
+         "v_nm1": to_gpu(vm), "rho_u": to_gpu(rho_u), "rho_v": to_gpu(rho_v), "mu": to_gpu(mu),
+         "mu_node": to_gpu(mun), "fx_sigma": to_gpu(z), "fy_sigma": to_gpu(z)}
+    ctx.momentum_set_fields(cfg, F, dt)
+    assert ctx.momentum_size()[0] == nm
+    want = ctx.apply_preconditioner(to_gpu(src), precond=cv.PRECOND_SSOR).cpu().numpy()
+    assert np.array_equal(dst, want)
+    assert not np.array_equal(dst, src)
+    # the host entry refuses a vector that is not the bound system's size
+    lib = L.load()
+    a = np.ascontiguousarray(src)
+    b = np.zeros_like(a)
+    dp = ct.POINTER(ct.c_double)
+    st = lib.covof_momentum_precond(ctx._h, cv.PRECOND_SSOR, nm - 1, a.ctypes.data_as(dp), b.ctypes.data_as(dp))
+    assert st == L.BAD_ARGUMENT
