@@ -112,6 +112,7 @@ struct Mg {
   int nlev = 0;
   int built = 0;  // assembly_id the coarse operators were made from
   MgLevelBuf lev[MG_MAX_LEVELS];
+  double *rzt = nullptr;  // r.z partials of the finest level's post-pass tiles
 };
 
 struct Poisson {
@@ -265,10 +266,13 @@ int cg2_reserve(Ctx &c);
 int surface_reserve(Ctx &c);
 int vof_adjoint_reserve(Ctx &c);
 // z = M^-1 r; f0, f1: the scalar slots of the solve's stop flags
-void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1);
+// rz: when not null and the cycle's finest post pass is fused, that pass also
+// forms the partials of r.z, folded into the nblocks entries at rz (the
+// reduction grid's r.z slot); the return value says whether it did.
+bool mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1, double *rz = nullptr);
 void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1);
 // z = M^-1 r of an FP64 solve with the V-cycle in FP32 (r rounded, z widened)
-void mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1);
+bool mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1, double *rz = nullptr);
 void mg_free(Ctx &c);
 
 int vof_alloc(Ctx &c);

@@ -518,10 +518,19 @@ __device__ __forceinline__ void pre_pass(const Lev<T> &F, const TB *__restrict__
   bc[(long)(J - 1) * nxc + (I - 1)] = scale * sum;
 }
 
+// Lane 0's value: the sum of the wave's 64 values (a fixed tree).
+__device__ __forceinline__ double wave_total(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// rzt: when not null, the tile's sum of r z (r = the right-hand side as
+// loaded, z = the value stored) goes to rzt[tile] (wsum: four doubles of LDS).
 template <bool INNER, typename T, typename TB, typename TX>
 __device__ __forceinline__ void post_pass(const Lev<T> &F, const TB *__restrict__ bin, TX *__restrict__ xout,
                                           int i0, int j0, int nxc, const T *__restrict__ xc, const double *scal,
-                                          int f0, int f1, T *xs) {
+                                          int f0, int f1, T *xs, double *rzt, int tile, double *wsum) {
   const int nx = F.nx, ny = F.ny;
   Pair<T, TB> p[FP_S];
   pair_loads<INNER, true>(F, bin, i0, j0, nxc, xc, p);
@@ -534,6 +543,7 @@ __device__ __forceinline__ void post_pass(const Lev<T> &F, const TB *__restrict_
   pair_blacks<INNER>(F, p, xs);
   __syncthreads();
   // red half-sweep from the new blacks; the tile's x
+  double rz = 0.0;
 #pragma unroll
   for (int s = 0; s < FP_S; ++s) {
     const int P = threadIdx.x + s * FT_BLK;
@@ -547,17 +557,79 @@ __device__ __forceinline__ void post_pass(const Lev<T> &F, const TB *__restrict_
     const T v1 = rc ? pair_gs<INNER>(F, gi + 1, gj, (T)q.b1, q.d1, q.w1, q.e1, q.s1, q.n1, xs, 2 * P + 1)
                     : xs[2 * P + 1];
     const int k = (gj - 1) * nx + (gi - 1);
+    const TX z0 = (TX)v0, z1 = (TX)v1;
     if (INNER) {
       using TX2 = typename Two<TX>::type;
       TX2 o;
-      o.x = (TX)v0;
-      o.y = (TX)v1;
+      o.x = z0;
+      o.y = z1;
       *reinterpret_cast<TX2 *>(xout + k) = o;
     } else {
-      if (in0) xout[k] = (TX)v0;
-      if (in1) xout[k + 1] = (TX)v1;
+      if (in0) xout[k] = z0;
+      if (in1) xout[k + 1] = z1;
+    }
+    if (rzt) {
+      if (in0) rz += (double)q.b0 * (double)z0;
+      if (in1) rz += (double)q.b1 * (double)z1;
     }
   }
+  if (rzt) {
+    rz = wave_total(rz);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = rz;
+    __syncthreads();
+    if (threadIdx.x == 0) rzt[tile] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+  }
+}
+
+// The tile partials of r.z as the finest fused post pass forms them (same
+// tiles, same pairs per thread, same order), for a cycle that ran without the
+// fused passes: the solve then takes the same r.z either way.
+__global__ void __launch_bounds__(FT_BLK) k_rz_tiles(int nx, int ny, const double *__restrict__ r,
+                                                     const double *__restrict__ z, double *__restrict__ rzt,
+                                                     const double *scal, int f0, int f1) {
+  if (stopped(scal, f0, f1)) return;
+  __shared__ double wsum[FT_BLK / 64];
+  const int ntx = (nx + FT_X - 1) / FT_X;
+  const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % ntx, ty = t / ntx;
+  const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;
+  double rz = 0.0;
+#pragma unroll
+  for (int s = 0; s < FP_S; ++s) {
+    const int P = threadIdx.x + s * FT_BLK;
+    if (P >= FP_N) break;
+    const int c = P / FP_X, a0 = 2 * (P - c * FP_X);
+    if (c < FT_H || c >= FT_H + FT_Y || a0 < FT_H || a0 >= FT_H + FT_X) continue;
+    const bool row = j0 + c <= ny, in0 = row && i0 + a0 <= nx, in1 = row && i0 + a0 + 1 <= nx;
+    const int k = (j0 + c - 1) * nx + (i0 + a0 - 1);
+    if (in0) rz += r[k] * z[k];
+    if (in1) rz += r[k + 1] * z[k + 1];
+  }
+  rz = wave_total(rz);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = rz;
+  __syncthreads();
+  if (threadIdx.x == 0) rzt[t] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+// The reduction grid's r.z partials from the finest post pass's tile
+// partials: entry b = the sum of tiles [b per, (b + 1) per) in tile order.
+__global__ void __launch_bounds__(256) k_rz_fold(const double *__restrict__ tp, int nt, int per,
+                                                 double *__restrict__ pout, int nblk, const double *scal, int f0,
+                                                 int f1) {
+  if (stopped(scal, f0, f1)) return;
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nblk) return;
+  const int t0 = b * per, t1 = min(nt, t0 + per);
+  double s = 0.0;
+  for (int t = t0; t < t1; t += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = tp[t + u < t1 ? t + u : t0];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t + u < t1) s += v[u];
+  }
+  pout[b] = s;
 }
 
 // The region of tile (tx, ty) lies inside the domain and the arrays allow
@@ -585,16 +657,17 @@ __global__ void __launch_bounds__(FT_BLK) k_mg_pre(Lev<T> F, const TB *__restric
 template <typename T, typename TB, typename TX>
 __global__ void __launch_bounds__(FT_BLK) k_mg_post(Lev<T> F, const TB *__restrict__ bin, TX *__restrict__ xout,
                                                     int nxc, const T *__restrict__ xc, int vec, const double *scal,
-                                                    int f0, int f1) {
+                                                    int f0, int f1, double *rzt) {
   __shared__ T xs[FR_N];
+  __shared__ double wsum[FT_BLK / 64];
   const int ntx = (F.nx + FT_X - 1) / FT_X;
   const int t = mg_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = t % ntx, ty = t / ntx;
   const int i0 = tx * FT_X + 1 - FT_H, j0 = ty * FT_Y + 1 - FT_H;
   if (inner_tile(vec, F.nx, F.ny, tx, ty, i0, j0))
-    post_pass<true>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs);
+    post_pass<true>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs, rzt, t, wsum);
   else
-    post_pass<false>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs);
+    post_pass<false>(F, bin, xout, i0, j0, nxc, xc, scal, f0, f1, xs, rzt, t, wsum);
 }
 
 // The fused passes' two-value loads and stores: an even level width and
@@ -608,6 +681,19 @@ inline int pair_vec(const Lev<T> &L, const void *b, const void *x) {
 
 inline unsigned fused_tiles(int nx, int ny) {
   return (unsigned)(((nx + FT_X - 1) / FT_X) * ((ny + FT_Y - 1) / FT_Y));
+}
+
+// rz (the reduction grid's r.z slot) from the tile partials rzt.
+inline void rz_fold(Ctx &c, const double *rzt, double *rz, int f0, int f1) {
+  const int nt = (int)fused_tiles(c.m.nx, c.m.ny), nblk = c.ps.nblocks, per = (nt + nblk - 1) / nblk;
+  hipLaunchKernelGGL(k_rz_fold, dim3((nblk + 255) / 256), dim3(256), 0, c.stream, rzt, nt, per, rz, nblk,
+                     (const double *)c.ps.scal, f0, f1);
+}
+// rz from r and z after a cycle without the fused finest pass.
+inline void rz_unfused(Ctx &c, double *rzt, const double *r, const double *z, double *rz, int f0, int f1) {
+  hipLaunchKernelGGL(k_rz_tiles, dim3(fused_tiles(c.m.nx, c.m.ny)), dim3(FT_BLK), 0, c.stream, c.m.nx, c.m.ny, r,
+                     z, rzt, (const double *)c.ps.scal, f0, f1);
+  rz_fold(c, rzt, rz, f0, f1);
 }
 
 // The bottom of the V-cycle in one workgroup: every level from the first one
@@ -1175,6 +1261,8 @@ int reserve(Ctx &c) {
       ++nl;
     }
     G.nlev = nl;
+    // r.z partials of the finest level's fused post pass, one per tile
+    COVOF_CHECK(hipMalloc(&G.rzt, (size_t)fused_tiles(m.nx, m.ny) * sizeof(double)));
   }
   return COVOF_OK;
 }
@@ -1239,7 +1327,7 @@ inline bool fused_cycle(Ctx &c) {
 // (TB, TZ: FP64 around an FP32 cycle) only on the fused path, where the first
 // level's passes round r on load and widen z on store.
 template <typename T, typename TB = T, typename TZ = T>
-void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
+bool apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1, double *rz = nullptr) {
   constexpr bool same = std::is_same<TB, T>::value && std::is_same<TZ, T>::value;
   const T *r0 = same ? (const T *)r : nullptr;
   T *z0 = same ? (T *)z : nullptr;
@@ -1250,12 +1338,13 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
     smooth(c, L, 0, true, f0, f1);
     smooth(c, L, 1, false, f0, f1);
     smooth(c, L, 0, false, f0, f1);
-    return;
+    return false;
   }
   const int nl = G.nlev;
   const int lb = bottom_level(G);
   const T scale = (T)(0.5 * correction_weight(c));
   const bool fused = fused_cycle<T>(c);
+  bool done_rz = false;
   for (int l = 0; l < lb; ++l) {
     const Lev<T> L = level<T>(c, l, r0, z0);
     const MgLevelBuf &B = G.lev[l + 1];
@@ -1323,12 +1412,19 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
     const MgLevelBuf &B = G.lev[l + 1];
     if (fused) {
       const dim3 grid(fused_tiles(L.nx, L.ny));
+      // the finest pass also forms r.z when asked (an FP64 z)
+      double *rzt = (l == 0 && rz && std::is_same<TZ, double>::value && G.rzt) ? G.rzt : nullptr;
       if (l == 0)
         hipLaunchKernelGGL((k_mg_post<T, TB, TZ>), grid, dim3(FT_BLK), 0, c.stream, L, r, z, B.nx, (const T *)B.x,
-                           pair_vec(L, r, z), (const double *)c.ps.scal, f0, f1);
+                           pair_vec(L, r, z), (const double *)c.ps.scal, f0, f1, rzt);
       else
         hipLaunchKernelGGL((k_mg_post<T, T, T>), grid, dim3(FT_BLK), 0, c.stream, L, L.b, L.x, B.nx,
-                           (const T *)B.x, pair_vec(L, L.b, L.x), (const double *)c.ps.scal, f0, f1);
+                           (const T *)B.x, pair_vec(L, L.b, L.x), (const double *)c.ps.scal, f0, f1,
+                           (double *)nullptr);
+      if (rzt) {
+        rz_fold(c, rzt, rz, f0, f1);
+        done_rz = true;
+      }
       continue;
     }
     hipLaunchKernelGGL(k_mg_prolong_smooth<T>, pair_grid(L.nx, L.ny), dim3(64, 4), 0, c.stream, L, B.nx,
@@ -1339,6 +1435,11 @@ void apply(Ctx &c, int kind, const TB *r, TZ *z, int f0, int f1) {
       smooth(c, L, 0, false, f0, f1);
     }
   }
+  if (!fused && rz && std::is_same<TZ, double>::value && std::is_same<TB, double>::value && G.rzt) {
+    rz_unfused(c, G.rzt, (const double *)r, (const double *)z, rz, f0, f1);
+    done_rz = true;
+  }
+  return done_rz;
 }
 
 // Rounding / widening copy (the FP32 cycle of an FP64 solve off the fused path).
@@ -1377,23 +1478,23 @@ int mg_reserve(Ctx &c) {
   return st ? st : reserve<float>(c);
 }
 
-void mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1) {
-  apply<double>(c, kind, r, z, f0, f1);
+bool mg_apply64(Ctx &c, int kind, const double *r, double *z, int f0, int f1, double *rz) {
+  return apply<double>(c, kind, r, z, f0, f1, rz);
 }
 void mg_apply32(Ctx &c, int kind, const float *r, float *z, int f0, int f1) {
   apply<float>(c, kind, r, z, f0, f1);
 }
-void mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1) {
-  if (fused_cycle<float>(c)) {
-    apply<float, double, double>(c, MG_KIND_VCYCLE, r, z, f0, f1);
-    return;
-  }
+bool mg_apply_fp32_cycle(Ctx &c, const double *r, double *z, int f0, int f1, double *rz) {
+  if (fused_cycle<float>(c)) return apply<float, double, double>(c, MG_KIND_VCYCLE, r, z, f0, f1, rz);
   const long n = (long)c.m.nx * c.m.ny;
   const double *scal = (const double *)c.ps.scal;
   hipLaunchKernelGGL((k_mg_convert<double, float>), dim3(1024), dim3(256), 0, c.stream, n, r, c.ps.r32, scal, f0, f1);
   apply<float>(c, MG_KIND_VCYCLE, (const float *)c.ps.r32, c.ps.z32, f0, f1);
   hipLaunchKernelGGL((k_mg_convert<float, double>), dim3(1024), dim3(256), 0, c.stream, n, (const float *)c.ps.z32,
                      z, scal, f0, f1);
+  if (!rz || !c.ps.mg32.rzt) return false;
+  rz_unfused(c, c.ps.mg32.rzt, r, z, rz, f0, f1);
+  return true;
 }
 
 void mg_free(Ctx &c) {
@@ -1406,6 +1507,8 @@ void mg_free(Ctx &c) {
     }
     G->nlev = 0;
     G->built = 0;
+    if (G->rzt) (void)hipFree(G->rzt);
+    G->rzt = nullptr;
   }
 }
 

@@ -1494,20 +1494,22 @@ static int sgs_apply(Ctx &c, const double *r, double *z, const double *flag) {
   return COVOF_OK;
 }
 template <typename T>
-static void explicit_precond(Ctx &c, int pc, const T *r, T *z, int fslot);
+static bool explicit_precond(Ctx &c, int pc, const T *r, T *z, int fslot, double *rz = nullptr);
+// z = M^-1 r; rz: the reduction grid's r.z slot, filled by the multigrid
+// cycle's finest pass when it can (the return value says whether it did)
 template <>
-void explicit_precond<double>(Ctx &c, int pc, const double *r, double *z, int fslot) {
+bool explicit_precond<double>(Ctx &c, int pc, const double *r, double *z, int fslot, double *rz) {
   if (pc == 2) {
     sgs_apply(c, r, z, (const double *)(c.ps.scal + fslot));
-  } else if (pc == 4 && c.ps.mg_fp32_cycle) {
-    mg_apply_fp32_cycle(c, r, z, S_FLAG, fslot);
-  } else {
-    mg_apply64(c, mg_kind(pc), r, z, S_FLAG, fslot);
+    return false;
   }
+  if (pc == 4 && c.ps.mg_fp32_cycle) return mg_apply_fp32_cycle(c, r, z, S_FLAG, fslot, rz);
+  return mg_apply64(c, mg_kind(pc), r, z, S_FLAG, fslot, rz);
 }
 template <>
-void explicit_precond<float>(Ctx &c, int pc, const float *r, float *z, int fslot) {
+bool explicit_precond<float>(Ctx &c, int pc, const float *r, float *z, int fslot, double *) {
   mg_apply32(c, mg_kind(pc), r, z, S_FLAG, fslot);
+  return false;
 }
 
 int poisson_precond(Ctx &c, int precond, const double *r, double *z) {
@@ -1628,11 +1630,13 @@ static void pcg_iteration(Ctx &c, OpT<T> A, T *x, T *r, T *z, T *d0, T *d1, T *A
   }
   launch_tagged(c, timed ? tag + 1 : -1, k_pcg_b<T, PC>, g, dim3(BLK), o, (const T *)A.diag, r, (const T *)Ad,
                 c.ps.scal, sl, (const double *)pa, pb, k);
-  if (PC >= 2) explicit_precond<T>(c, PC, (const T *)r, z, sl.flag);
+  // (FP64 with multigrid: the cycle's finest pass forms the r.z partials)
+  bool rz_done = false;
+  if (PC >= 2) rz_done = explicit_precond<T>(c, PC, (const T *)r, z, sl.flag, ser ? nullptr : pb + c.ps.nblocks);
   if (ser) {
     serial_sums(c, n, rz_terms(c, PC, (const double *)r, (const double *)z, (const double *)A.diag, pb, false),
                 sl.flag, S_FLAG);
-  } else if (PC >= 2) {
+  } else if (PC >= 2 && !rz_done) {
     hipLaunchKernelGGL(k_pcg_rz<T>, g, dim3(BLK), 0, c.stream, o, (const T *)r, (const T *)z, c.ps.scal, sl, pb);
   }
 }
