@@ -540,21 +540,30 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   // The first row's operands and the row below do not depend on beta: they
   // are loaded before the stop flags and the grid totals are read, so the
   // latencies overlap.
+  // Row chunks alternate their walking direction: an odd chunk walks down
+  // from its last row.  Two chunks that share a boundary then read the rows
+  // on either side of it at the same moment (both at their start, or both at
+  // their end), so the second reader finds them in L2; walking the same way,
+  // one of them would read the row at its start and the other at its end.
+  // "prev" is the row before the first one in walking order (the row below,
+  // or the row above when walking down).
+  const bool rev = ((local_ / gq_) & 1) != 0;
+  const int jf = rev ? jw1 : jw0;  // first row of the walk
   T r0 = T(0), p0 = T(0), rd = T(0), pd = T(0), dgd = T(0), aS = T(0), kw = T(0), kn = T(0), ke = T(0);
   if (act && vi) {
-    r0 = zr[at(i, jw0)];
-    p0 = dprev[at(i, jw0)];
-    const int jd = jw0 > 1 ? jw0 - 1 : (o.perS ? ny : 0);
+    r0 = zr[at(i, jf)];
+    p0 = dprev[at(i, jf)];
+    const int jd = rev ? (jf < ny ? jf + 1 : (o.perN ? 1 : 0)) : (jf > 1 ? jf - 1 : (o.perS ? ny : 0));
     if (jd > 0) {
       rd = zr[at(i, jd)];
       pd = dprev[at(i, jd)];
       if (PC == 1) dgd = A.diag[at(i, jd)];
     }
-    aS = A.ky[kyi(nx, i, jw0)];
-    kw = A.kx[kxi(nx, i, jw0)];
-    kn = A.ky[kyi(nx, i, jw0 + 1)];
-    if (i == nx) ke = A.kx[kxi(nx, nx + 1, jw0)];
-    else if (right_load) ke = A.kx[kxi(nx, i + 1, jw0)];
+    aS = A.ky[kyi(nx, i, jf)];
+    kw = A.kx[kxi(nx, i, jf)];
+    kn = A.ky[kyi(nx, i, jf + 1)];
+    if (i == nx) ke = A.kx[kxi(nx, nx + 1, jf)];
+    else if (right_load) ke = A.kx[kxi(nx, i + 1, jf)];
   }
   const double flag_own = scal[sl.flag], flag_outer = scal[S_FLAG];
   const bool first = scal[sl.first] != 0.0;
@@ -624,26 +633,30 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
   T aP = kw + aE + aS + kn + (T)o.eps;
   T pcur = p0;
   T cur = zval(r0, aP) + beta * p0;
-  T down = zval(rd, dgd) + beta * pd;
+  T down = zval(rd, dgd) + beta * pd;  // d of the walk's previous row
   if (act) {
-    for (int j0 = jw0; j0 <= jw1; j0 += G) {
-      // loads of the batch: the row above (r or z, d_prev, kx(i), ky(i, +1),
-      // and kx(i+1) at the block's right edge), x, and at the block's edges
-      // the x-neighbours (r or z, d_prev, diagonal)
-      T ru[G], pu[G], kwu[G], knu[G], keu[G], ksu[G], xo[G], rl[G], pl[G], dl[G], rr[G], pr[G], dr[G];
+    const int nrow = jw1 - jw0 + 1;
+    for (int t0 = 0; t0 < nrow; t0 += G) {
+      const int j0 = rev ? jw1 - t0 : jw0 + t0;  // batch row g: j0 + g (j0 - g walking down)
+      // loads of the batch: the next row in walking order (r or z, d_prev,
+      // kx(i), the face away from the row being formed -- ky(i, +1) walking
+      // up, ky(i) walking down -- and kx(i+1) at the block's right edge), x,
+      // and at the block's edges the x-neighbours (r or z, d_prev, diagonal)
+      T ru[G], pu[G], kwu[G], fnu[G], fwu[G], keu[G], xo[G], rl[G], pl[G], dl[G], rr[G], pr[G], dr[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int j = j0 + g;
-        ru[g] = pu[g] = kwu[g] = knu[g] = keu[g] = ksu[g] = xo[g] = T(0);
+        const int j = rev ? j0 - g : j0 + g;
+        ru[g] = pu[g] = kwu[g] = fnu[g] = fwu[g] = keu[g] = xo[g] = T(0);
         rl[g] = pl[g] = dl[g] = rr[g] = pr[g] = dr[g] = T(0);
-        if (j <= jw1 && vi) {
-          const int ju = j < ny ? j + 1 : (o.perN ? 1 : 0);
+        if (t0 + g < nrow && vi) {
+          const int ju = rev ? (j > 1 ? j - 1 : (o.perS ? ny : 0)) : (j < ny ? j + 1 : (o.perN ? 1 : 0));
           if (ju > 0) {
             ru[g] = zr[at(i, ju)];
             pu[g] = dprev[at(i, ju)];
             kwu[g] = A.kx[kxi(nx, i, ju)];
-            knu[g] = A.ky[kyi(nx, i, ju + 1)];
-            if (ju != j + 1) ksu[g] = A.ky[kyi(nx, i, ju)];  // periodic wrap: the row's own south face
+            fnu[g] = A.ky[kyi(nx, i, rev ? ju : ju + 1)];
+            // periodic wrap: the row's own face toward the row being formed
+            if (ju != (rev ? j - 1 : j + 1)) fwu[g] = A.ky[kyi(nx, i, rev ? ju + 1 : ju)];
             if (i == nx) keu[g] = A.kx[kxi(nx, nx + 1, ju)];
             else if (right_load) keu[g] = A.kx[kxi(nx, i + 1, ju)];
           }
@@ -669,7 +682,7 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
           }
         }
       }
-      // the next wave's west faces of the rows above (this wave's east faces at lane 63)
+      // the next wave's west faces of the next rows (this wave's east faces at lane 63)
       if (lane == 0) {
 #pragma unroll
         for (int g = 0; g < G; ++g) s_kw[w][g] = kwu[g];
@@ -679,23 +692,28 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
 #pragma unroll
         for (int g = 0; g < G; ++g) keu[g] = s_kw[w + 1][g];
       }
-      // the rows above: a_P from their four faces (the south face is the
-      // row's north face unless the row wraps), z, d; curs[g] = d of row j0 + g
-      T ups[G], aEus[G], aPus[G], curs[G];
+      // the next rows: a_P from their four faces in west, east, south, north
+      // order (the face toward the row being formed is that row's face away
+      // from them unless the row wraps), z, d; curs[g] = d of batch row g
+      T ups[G], aEus[G], aPus[G], aSus[G], aNus[G], curs[G];
       {
-        T knr = kn;  // north face of the row below the one being formed
+        T fsh = rev ? aS : kn;  // face of the row being formed toward the next row
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const int j = j0 + g;
+          const int j = rev ? j0 - g : j0 + g;
           T aEu = __shfl_down(kwu[g], 1, 64);
           if (right_lds || right_load) aEu = keu[g];
-          const T aSu = (j < ny) ? knr : ksu[g];
-          const T aPu = kwu[g] + aEu + aSu + knu[g] + (T)o.eps;
+          const bool wraps = rev ? j == 1 : j == ny;
+          const T ftow = wraps ? fwu[g] : fsh;
+          const T aSu = rev ? fnu[g] : ftow, aNu = rev ? ftow : fnu[g];
+          const T aPu = kwu[g] + aEu + aSu + aNu + (T)o.eps;
           ups[g] = zval(ru[g], aPu) + beta * pu[g];
           aEus[g] = aEu;
           aPus[g] = aPu;
+          aSus[g] = aSu;
+          aNus[g] = aNu;
           curs[g] = g == 0 ? cur : ups[g - 1];
-          knr = knu[g];
+          fsh = fnu[g];
         }
       }
       // the edge lanes' d of the batch's rows for the adjacent waves
@@ -710,8 +728,8 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
       lds_barrier();
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int j = j0 + g;
-        if (j > jw1) break;  // uniform over the block
+        if (t0 + g >= nrow) break;  // uniform over the block
+        const int j = rev ? j0 - g : j0 + g;
         T left = __shfl_up(cur, 1, 64);
         T right = __shfl_down(cur, 1, 64);
         if (left_lds) left = s_c63[w - 1][g];
@@ -719,11 +737,12 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
         if (right_lds) right = s_c0[w + 1][g];
         if (right_load) right = zval(rr[g], dr[g]) + beta * pr[g];
         if (vi) {
+          const T south = rev ? ups[g] : down, north = rev ? down : ups[g];
           T val = aP * cur;
           if (i > 1 || o.perW) val = val - kw * left;
           if (i < nx || o.perE) val = val - aE * right;
-          if (j > 1 || o.perS) val = val - aS * down;
-          if (j < ny || o.perN) val = val - kn * ups[g];
+          if (j > 1 || o.perS) val = val - aS * south;
+          if (j < ny || o.perN) val = val - kn * north;
           const long q = at(i, j);
           d[q] = cur;
           Ad[q] = val;
@@ -733,9 +752,9 @@ __global__ void __launch_bounds__(BLK) k_pcg_a(DevOp o, OpT<T> A, const T *__res
         down = cur;
         cur = ups[g];
         pcur = pu[g];
-        aS = kn;
+        aS = aSus[g];
+        kn = aNus[g];
         kw = kwu[g];
-        kn = knu[g];
         aE = aEus[g];
         aP = aPus[g];
       }
