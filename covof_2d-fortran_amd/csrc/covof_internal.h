@@ -2,6 +2,7 @@
 // Not part of the public C-ABI (that is include/covof_hip.h).
 #pragma once
 
+#include <cmath>
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 #include <cstdint>
@@ -177,6 +178,13 @@ struct TopJet {
 
 // Solver tuning of a context (covof_ctx_set_solver_option); zeros in the
 // mixed-mode fields mean "the default of the preconditioner".
+// Whether h is a power of two (its reciprocal is then exact, and a quotient
+// by h equals the product with that reciprocal bit for bit).
+inline bool is_pow2(double h) {
+  int e;
+  return h > 0.0 && std::frexp(h, &e) == 0.5;
+}
+
 struct SolverOpts {
   int mixed_stall = 0;   // reliable updates without a 10 % gain before the FP64 continuation
   int mixed_cycle = 0;   // FP32 iterations per reliable update

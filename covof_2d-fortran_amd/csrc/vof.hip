@@ -584,7 +584,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // shoelace area), a face whose donor is mixed joins the tile's list, and
 // phase 2 clips those with every lane busy; phase 3 updates the cells from
 // the two face values each.
-template <int DIR, bool FIRST>
+template <int DIR, bool FIRST, bool P2>
 __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
                                                       const double *__restrict__ vel, DevMesh m, double dt,
                                                       const double *__restrict__ snx, const double *__restrict__ sny,
@@ -684,8 +684,11 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   __syncthreads();
   const int col = threadIdx.x % TX;
   const int i = i0 + col;
+  // P2: power-of-two spacings, whose reciprocals are exact: a quotient by
+  // them is the product with the reciprocal, the same value
   const double area = m.dx * m.dy;
-  const double dth = dt / (DIR == 0 ? m.dx : m.dy);
+  const double rarea = P2 ? 1.0 / area : 0.0;
+  const double dth = P2 ? dt * (1.0 / (DIR == 0 ? m.dx : m.dy)) : dt / (DIR == 0 ? m.dx : m.dy);
   constexpr int NRC = TROWS;  // rows per thread
   double c0[NRC], vlo[NRC], vhi[NRC];
   bool cdo[NRC];
@@ -722,7 +725,7 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
       } else {
         cdv = cdo[r] ? 1.0 : 0.0;
       }
-      cn = c0[r] - (fhi - flo) / area + cdv * dth * du;
+      cn = c0[r] - (P2 ? (fhi - flo) * rarea : (fhi - flo) / area) + cdv * dth * du;
       if (!FIRST) {
         const double tc = 1.0 < cn ? 1.0 : cn;  // min(1, c)
         cn = 0.0 > tc ? 0.0 : tc;               // max(0, .)
@@ -905,8 +908,11 @@ static int launch_pass(Ctx &c, int method, int dir, bool first, const double *sr
   const dim3 grd(n_tiles(c.m)), blk(TBLK);
   int *ln = c.vof.list[0], *cn = c.vof.count;
   const int cf = corner_fix ? 1 : 0;
-  auto kern = dir == 0 ? (first ? k_flux_update<0, true> : k_flux_update<0, false>)
-                      : (first ? k_flux_update<1, true> : k_flux_update<1, false>);
+  const bool p2 = is_pow2(c.m.dx) && is_pow2(c.m.dy);
+  auto kern = dir == 0 ? (first ? (p2 ? k_flux_update<0, true, true> : k_flux_update<0, true, false>)
+                                : (p2 ? k_flux_update<0, false, true> : k_flux_update<0, false, false>))
+                       : (first ? (p2 ? k_flux_update<1, true, true> : k_flux_update<1, true, false>)
+                                : (p2 ? k_flux_update<1, false, true> : k_flux_update<1, false, false>));
   launch_tagged(c, COVOF_KERNEL_VOF_UPDATE, kern, grd, blk, src, dst, dir == 0 ? u : v, m, dt,
                 (const double *)c.vof.snx, (const double *)c.vof.sny, (const double *)c.vof.sal, c.vof.cd, ln, cn, cf);
   COVOF_CHECK_LAUNCH();
