@@ -48,8 +48,8 @@ HBM_PEAK_GBS = 8000.0
 # copy) measured with rocprofv3 --pmc on `bench.py --kernel-sample` and
 # summarised by `bench.py --summarize`; the summary with its commands is
 # profiles/poisson_wy_momentum_2048_hbm_traffic.md.
-TRAFFIC_2048 = {"k_pcg_a<double,1>": 1.904e8 + 1.007e8, "k_pcg_b<double,1>": 1.008e8 + 3.365e7,
-                "k_pcg_a<float,1>": 9.668e7 + 5.036e7, "k_pcg_b<float,1>": 5.045e7 + 1.688e7}
+TRAFFIC_2048 = {"k_pcg_a<double,1>": 1.790e8 + 1.007e8, "k_pcg_b<double,1>": 1.008e8 + 3.365e7,
+                "k_pcg_a<float,1>": 8.947e7 + 5.036e7, "k_pcg_b<float,1>": 5.047e7 + 1.688e7}
 
 
 def log(*a):
