@@ -367,14 +367,20 @@ __global__ void __launch_bounds__(TBLK, 4) k_recon(const double *__restrict__ C,
         }
       }
     } else {
-      const int e = PER_UNIT * (u - nru) + slot;
+      // the tile of the unit's first entry e0: the last tile t with
+      // offs[t] <= e0, found by the wave in rounds of 64 probes (two rounds
+      // for 4096 tiles) instead of a per-lane bisection's chain of dependent
+      // loads; each lane then steps on to its own entry's tile
+      const int e0 = PER_UNIT * (u - nru), e = e0 + slot;
+      int lo = 0;
+      for (int stride = (ntiles + 63) / 64;; stride = (stride + 63) / 64) {
+        const int t = min(lo + lane * stride, ntiles);
+        const unsigned long long le = __ballot(offs[t] <= e0 && lo + lane * stride <= ntiles);
+        lo += (63 - __builtin_clzll(le)) * stride;  // lane 0 always qualifies (offs[lo] <= e0)
+        if (stride == 1) break;
+      }
       if (lane_used && e < nmix) {
-        int lo = 0, hi = ntiles;  // offs[lo] <= e < offs[hi]
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (offs[mid] <= e) lo = mid;
-          else hi = mid;
-        }
+        while (offs[lo + 1] <= e) ++lo;  // offs[lo] <= e < offs[lo + 1]
         const int k = list[(long)lo * TCELLS + (e - offs[lo])];
         j = k / (m.nx + 2);
         i = k - j * (m.nx + 2);
