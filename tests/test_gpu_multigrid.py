@@ -241,9 +241,10 @@ def test_mg_fused_levels_bitwise(cv, shape, bc):
 
 
 PIN_CASES = [((96, 80), (4, 4, 4, 4)), ((300, 260), (4, 4, 4, 4)), ((256, 192), (3, 3, 3, 3)),
-             ((200, 136), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4)), ((8, 2048), (3, 3, 4, 4)), ((40, 30), (4, 4, 3, 3))]
+             ((200, 136), (1, 2, 3, 3)), ((70, 90), (2, 2, 4, 4)), ((8, 2048), (3, 3, 4, 4)), ((40, 30), (4, 4, 3, 3)),
+             ((206, 128), (1, 1, 3, 3))]
 PIN_IDS = ["walls-small", "walls-odd", "periodic", "wall-neumann-periodic", "odd-coarsening", "narrow-periodic-x",
-           "one-workgroup"]
+           "one-workgroup", "odd-periodic-y"]
 
 
 @pytest.mark.parametrize("shape,bc", PIN_CASES, ids=PIN_IDS)
