@@ -589,7 +589,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // full has the closed-form value of the clip (0, or the swept box's
 // shoelace area), a face whose donor is mixed joins the tile's list, and
 // phase 2 clips those with every lane busy; phase 3 updates the cells from
-// the two face values each.
+// the two face values each, with the cell's C and face velocities taken from
+// the copies phase 1 left in LDS (face (i, j) is cell (i, j)), so the tile is
+// read from memory once.
 template <int DIR, bool FIRST, bool P2>
 __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__ src, double *__restrict__ dst,
                                                       const double *__restrict__ vel, DevMesh m, double dt,
@@ -600,10 +602,11 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   constexpr int FW = DIR == 0 ? TX + 1 : TX;  // faces per LDS row
   constexpr int NF = DIR == 0 ? (TX + 1) * TY : TX * (TY + 1);
   // the list of mixed-donor faces is dead once phase 2 has run: the list of
-  // mixed cells of dst reuses its memory (12.8 KiB of LDS per block, so
-  // VGPRs rather than LDS bound the blocks per CU)
+  // mixed cells of dst reuses its memory (29 KiB of LDS per block: five
+  // blocks per CU)
   static_assert(NF >= TCELLS, "the face list's memory holds the cell list");
   __shared__ double s_flux[NF];
+  __shared__ double s_c[NF], s_u[NF];  // src and vel at each face's own index
   __shared__ int s_mix[NF];
   int *s_list = s_mix;
   __shared__ int s_count, s_nmix;
@@ -616,6 +619,15 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   const int tx = t % tiles_x(m), ty = t / tiles_x(m);
   const int i0 = tx * TX + 1, j0 = ty * TY + 1;
   const int imax = DIR == 0 ? m.nx + 1 : m.nx, jmax = DIR == 0 ? m.ny : m.ny + 1;
+  const int col = threadIdx.x % TX;
+  const int i = i0 + col;
+  constexpr int NRC = TROWS;  // rows per thread
+  bool cdo[NRC];              // second pass: the dilation indicator of the step
+#pragma unroll
+  for (int r = 0; r < NRC; ++r) {
+    const int j = j0 + (threadIdx.x / TX) + TRG * r;
+    cdo[r] = !FIRST && i <= m.nx && j <= m.ny && cd[H(m, i, j)] != 0;
+  }
   // all loads of a thread's faces first -- the face velocity and both
   // candidate donors (the velocity's sign picks one) -- so they are in flight
   // together; then the closed forms and the list of mixed-donor faces
@@ -643,6 +655,11 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     int fi = fis[r], fj = fjs[r];
+    if (ok[r]) {  // the cell update reads them here: face (i, j) is cell (i, j)
+      const int f = r * TBLK + threadIdx.x;
+      s_c[f] = cneg[r];
+      s_u[f] = uf[r];
+    }
     const bool edge = ok[r] && (DIR == 0 ? (m.bcl == BC_PERIODIC && (fi == 1 || fi == m.nx + 1))
                                          : (m.bcb == BC_PERIODIC && (fj == 1 || fj == m.ny + 1)));
     if (edge) {
@@ -688,29 +705,11 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
     s_flux[f] = b.pos ? a : -a;
   }
   __syncthreads();
-  const int col = threadIdx.x % TX;
-  const int i = i0 + col;
   // P2: power-of-two spacings, whose reciprocals are exact: a quotient by
   // them is the product with the reciprocal, the same value
   const double area = m.dx * m.dy;
   const double rarea = P2 ? 1.0 / area : 0.0;
   const double dth = P2 ? dt * (1.0 / (DIR == 0 ? m.dx : m.dy)) : dt / (DIR == 0 ? m.dx : m.dy);
-  constexpr int NRC = TROWS;  // rows per thread
-  double c0[NRC], vlo[NRC], vhi[NRC];
-  bool cdo[NRC];
-#pragma unroll
-  for (int r = 0; r < NRC; ++r) {  // loads of all rows first
-    const int j = j0 + (threadIdx.x / TX) + TRG * r;
-    c0[r] = vlo[r] = vhi[r] = 0.0;
-    cdo[r] = false;
-    if (i <= m.nx && j <= m.ny) {
-      const long k = H(m, i, j);
-      c0[r] = src[k];
-      vlo[r] = vel[k];
-      vhi[r] = vel[DIR == 0 ? H(m, i + 1, j) : H(m, i, j + 1)];
-      if (!FIRST) cdo[r] = cd[k] != 0;
-    }
-  }
 #pragma unroll
   for (int r = 0; r < NRC; ++r) {
     const int row = (threadIdx.x / TX) + TRG * r;
@@ -722,16 +721,17 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
       k = H(m, i, j);
       const int lo = row * FW + col, hi = DIR == 0 ? lo + 1 : lo + FW;
       const double flo = s_flux[lo], fhi = s_flux[hi];
-      const double du = vhi[r] - vlo[r];
+      const double c0 = s_c[lo];
+      const double du = s_u[hi] - s_u[lo];
       double cdv;
       if (FIRST) {
-        const bool bb = c0[r] >= 0.5;
+        const bool bb = c0 >= 0.5;
         cd[k] = bb ? 1 : 0;
         cdv = bb ? 1.0 : 0.0;
       } else {
         cdv = cdo[r] ? 1.0 : 0.0;
       }
-      cn = c0[r] - (P2 ? (fhi - flo) * rarea : (fhi - flo) / area) + cdv * dth * du;
+      cn = c0 - (P2 ? (fhi - flo) * rarea : (fhi - flo) / area) + cdv * dth * du;
       if (!FIRST) {
         const double tc = 1.0 < cn ? 1.0 : cn;  // min(1, c)
         cn = 0.0 > tc ? 0.0 : tc;               // max(0, .)
