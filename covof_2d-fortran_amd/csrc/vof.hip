@@ -698,9 +698,17 @@ __global__ void __launch_bounds__(TBLK) k_flux_update(const double *__restrict__
   for (int e = threadIdx.x; e < nmix; e += TBLK) {
     const int f = s_mix[e];
     const int fr = f / FW;
-    int fi = i0 + (f - fr * FW), fj = j0 + fr;
+    const int fi0 = i0 + (f - fr * FW), fj0 = j0 + fr;
+    int fi = fi0, fj = fj0;
     pass_face<DIR>(vel, m, fi, fj);
-    const FaceBox b = face_box<DIR>(vel[H(m, fi, fj)], m, fi, fj, dt);
+    // the face's own velocity is in LDS; only a periodic edge face moved to
+    // the real donor's side loads it again
+    double u;
+    if (fi == fi0 && fj == fj0)
+      u = s_u[f];
+    else
+      u = vel[H(m, fi, fj)];
+    const FaceBox b = face_box<DIR>(u, m, fi, fj, dt);
     const double a = cut_box_area(b.xmin, b.xmax, b.ymin, b.ymax, snx[b.kd], sny[b.kd], sal[b.kd]);
     s_flux[f] = b.pos ? a : -a;
   }
